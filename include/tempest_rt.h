@@ -1,0 +1,292 @@
+/*
+ * tempest_rt.h - C ABI of the MI355X path tracer.
+ *
+ * The entry points mirror the ray-trace scene API of the reference renderer,
+ * the duck-typed twins RayTracerScene
+ * (include/tempest/graphics/ray-tracing/ray-tracing.hh:241-395) and
+ * RayTracerCuda (include/tempest/compute/ray-tracing-cuda.hh:80-172).  Each
+ * declaration below names the member it replaces.  Plain C types only: no
+ * HIP, no torch.  Every call returns a tr_status and never aborts;
+ * tr_last_error_message() describes the last failure of the calling thread.
+ *
+ * Ownership differs from the CPU reference on purpose: every add* call
+ * deep-copies vertices, indices, materials and lights, so the caller keeps
+ * its buffers (the reference retains index and material pointers,
+ * ray-tracing.cc:1532-1535, and takes ownership of LightSource*).
+ */
+#ifndef TEMPEST_RT_H
+#define TEMPEST_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tr_status {
+    TR_OK = 0,
+    TR_ERROR_INVALID_ARG = 1,
+    TR_ERROR_OUT_OF_MEMORY = 2,
+    TR_ERROR_DEVICE = 3,
+    TR_ERROR_NOT_COMMITTED = 4,
+    TR_ERROR_UNSUPPORTED = 5
+} tr_status;
+
+/* IlluminationModel, numeric values kept (ray-tracing-common.hh:68-105). */
+enum {
+    TR_MODEL_EMISSIVE = 0,
+    TR_MODEL_BLINN_PHONG = 1,
+    TR_MODEL_KAJIYA_KAY = 2,
+    TR_MODEL_ASHIKHMIN_SHIRLEY = 3,
+    TR_MODEL_MICRO_FLAKE = 4,
+    TR_MODEL_PATCHWORK = 8,
+    TR_MODEL_MIRROR = 9,
+    TR_MODEL_GGX_MICROFACET = 11,
+    TR_MODEL_GGX_MICROFACET_CONDUCTOR = 12,
+    TR_MODEL_GGX_MICROFACET_DIELECTRIC = 13,
+    TR_MODEL_MIX = 14,
+    TR_MODEL_GGX_MICROFACET_ANISOTROPIC = 15,
+    TR_MODEL_GGX_MICROFACET_CONDUCTOR_ANISOTROPIC = 16,
+    TR_MODEL_GGX_MICROFACET_DIELECTRIC_ANISOTROPIC = 17,
+    TR_MODEL_STOCHASTIC_ROTATOR = 20,
+    TR_MODEL_SGGX_MICRO_FLAKE = 22,
+    TR_MODEL_SPATIALLY_VARYING_EMISSIVE = 23,
+    TR_MODEL_SGGX_SURFACE = 24,
+    TR_MODEL_SGGX_PSEUDO_VOLUME = 25,
+    TR_MODEL_BTF = 27,
+    TR_MODEL_BECKMANN_MICROFACET = 28,
+    TR_MODEL_BECKMANN_MICROFACET_CONDUCTOR = 29,
+    TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC = 30,
+    TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC = 31,
+    TR_MODEL_BECKMANN_MICROFACET_CONDUCTOR_ANISOTROPIC = 32,
+    TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC_ANISOTROPIC = 33,
+    TR_MODEL_COUNT = 34
+};
+
+/* LightSourceType (ray-tracing-common.hh:478-485). */
+enum {
+    TR_LIGHT_DIRECTIONAL = 0,
+    TR_LIGHT_SPHERE_AREA = 1,
+    TR_LIGHT_MESH_AREA = 2,
+    TR_LIGHT_POINT = 3
+};
+
+/* PicturePostProcess (ray-tracing-common.hh:59-66). */
+enum {
+    TR_POST_LINEAR = 0,
+    TR_POST_SRGB = 1,
+    TR_POST_AUTO_EXPOSURE_SRGB = 2,
+    TR_POST_AUTO_EXPOSURE_HDR = 3,
+    TR_POST_ACES = 4
+};
+
+/* Estimator of the path loop.  REFERENCE reproduces the CPU reference's
+ * balance-heuristic weighting (ray-tracing.cc:1791-1930, 2182-2332),
+ * including its quirks; it is the mode of the per-path parity contract.
+ * UNBIASED weights next-event estimation of every area light (mesh lights
+ * included) with the balance heuristic and evaluates mesh-light PDFs only
+ * along directions that hit the light. */
+enum {
+    TR_ESTIMATOR_REFERENCE = 0,
+    TR_ESTIMATOR_UNBIASED = 1
+};
+
+/* One flat record for the material structs of ray-tracing-common.hh.
+ * RTMicrofacetMaterial (:159-174): specular, specular_power (the union
+ * member StandardDeviation for SGGX surfaces), diffuse (radiance for
+ * Emissive), fresnel {eta, k}, normalization.  RTSGGXSurface (:397-406):
+ * sggx_basis quaternion (x, y, z, w), sggx_depth, sggx_sample_count.
+ * RTVolumeMaterial / RTSGGXMicroFlakeMaterial (:379-395): albedo,
+ * sggx_stddev.  tr_material_init fills the reference's defaults. */
+typedef struct tr_material {
+    uint32_t model;
+    float specular[3];
+    float specular_power[2];
+    float diffuse[3];
+    float fresnel[2];
+    float normalization;
+    float sggx_basis[4];
+    uint32_t sggx_depth;
+    uint32_t sggx_sample_count;
+    float albedo[3];
+    float sggx_stddev[3];
+} tr_material;
+
+/* LightSource and its subclasses (ray-tracing-common.hh:486-544). */
+typedef struct tr_light {
+    uint32_t type;
+    float direction[3];      /* DirectionalLight::Direction (towards the light) */
+    float position[3];       /* PointLight::Position */
+    float radiance[3];       /* Radiance of every type */
+    float sphere_center[3];  /* SphereAreaLight::SphereShape */
+    float sphere_radius;
+    uint32_t geometry_id;    /* set by tr_scene_add_sphere_light_source */
+} tr_light;
+
+/* RTSubmesh (ray-tracing-common.hh:240-247): vertex_count is the number of
+ * indices of the submesh, vertex_offset a byte offset into the vertex
+ * buffer, base_index an offset into the index array, stride the vertex
+ * size in bytes.  Positions sit at byte 0 of a vertex, the normal at
+ * stride-12 (ray-tracing.cc:377); strides 44 (PTTNFormat) and 48
+ * (PTTcNFormat) carry a tangent at byte 20. */
+typedef struct tr_submesh {
+    const tr_material* material;
+    uint32_t vertex_count;
+    uint32_t vertex_offset;
+    uint32_t base_index;
+    uint32_t stride;
+} tr_submesh;
+
+/* MeshOptions (ray-tracing-common.hh:408-415) without the host callbacks,
+ * which a GPU backend cannot call. */
+typedef struct tr_mesh_options {
+    int32_t two_sided;
+} tr_mesh_options;
+
+/* RTSettings (ray-tracing-common.hh:141-144) plus the device path pool. */
+typedef struct tr_settings {
+    uint32_t data_pool_size;   /* accepted for compatibility, unused */
+    uint32_t path_pool_size;   /* paths resident per batch; 0 = automatic */
+} tr_settings;
+
+/* A ray of a fixed ray set and its closest hit. */
+typedef struct tr_ray {
+    float org[3];
+    float tnear;
+    float dir[3];
+    float tfar;
+} tr_ray;
+
+typedef struct tr_hit {
+    float t;
+    float u;        /* barycentric weight of vertex 1 */
+    float v;        /* barycentric weight of vertex 2 */
+    uint32_t geom;  /* scene geometry slot, TR_INVALID_GEOMETRY on a miss */
+    uint32_t prim;  /* triangle index within the submesh, 0 for spheres */
+} tr_hit;
+
+#define TR_INVALID_GEOMETRY 0xffffffffu
+
+typedef struct tr_scene tr_scene;
+
+/* Counters and timings of the last drawOnce / render call. */
+typedef struct tr_stats {
+    uint64_t paths;              /* camera samples traced */
+    uint64_t segments;           /* extension rays traced */
+    uint64_t shadow_rays;        /* next-event shadow rays traced */
+    double frame_ms;             /* host wall time of the call */
+    double trace_ms;             /* summed HIP-event time of closest-hit kernels */
+    double shade_ms;             /* summed HIP-event time of shading kernels */
+    double shadow_ms;            /* summed HIP-event time of shadow kernels */
+    double other_ms;             /* ray generation and accumulation */
+    uint64_t trace_launches;     /* closest-hit launches behind trace_ms */
+    uint64_t node_visits;        /* BVH8 nodes fetched (only if counting on) */
+    uint64_t prim_tests;         /* primitives tested (only if counting on) */
+    uint64_t counted_rays;       /* rays behind node_visits / prim_tests */
+} tr_stats;
+
+/* Scene-construction information. */
+typedef struct tr_scene_info {
+    uint64_t primitives;
+    uint64_t bvh_nodes;
+    uint64_t bvh_bytes;
+    uint64_t prim_bytes;
+    uint32_t bvh_depth;
+    uint32_t geometries;
+    uint32_t lights;
+    uint32_t materials;
+    double build_ms;
+} tr_scene_info;
+
+const char* tr_last_error_message(void);
+
+/* RTMaterial::setup (ray-tracing.cc:80-87): applies MaterialSetupLookup,
+ * i.e. DefaultMicrofacetMaterialSetup (illumination-models.hh:599-614). */
+tr_status tr_material_init(tr_material* material, uint32_t model);
+tr_status tr_material_setup(tr_material* material);
+
+/* RayTracerScene(width, height, view_proj_inv, settings) (ray-tracing.hh:302). */
+tr_status tr_scene_create(uint32_t width, uint32_t height, const float view_proj_inv[16],
+                          const tr_settings* settings, tr_scene** out_scene);
+/* ~RayTracerScene. */
+void tr_scene_destroy(tr_scene* scene);
+
+/* Setters (ray-tracing.hh:305-321). */
+tr_status tr_scene_set_chunk_size(tr_scene* scene, uint32_t chunk_size);
+tr_status tr_scene_set_samples_local_area_light(tr_scene* scene, uint32_t rays);
+tr_status tr_scene_set_samples_global_illumination(tr_scene* scene, uint32_t rays);
+tr_status tr_scene_set_samples_camera(tr_scene* scene, uint32_t rays);
+tr_status tr_scene_set_max_ray_depth(tr_scene* scene, uint32_t depth);
+tr_status tr_scene_set_russian_roulette(tr_scene* scene, float rval);
+tr_status tr_scene_set_background_spectrum(tr_scene* scene, const float rgb[3]);
+tr_status tr_scene_set_picture_post_process(tr_scene* scene, uint32_t mode);
+tr_status tr_scene_set_transparent_background(tr_scene* scene, int32_t enabled);
+tr_status tr_scene_set_render_mode(tr_scene* scene, uint32_t mode);
+/* repaint (ray-tracing.hh:323): restart progressive accumulation. */
+tr_status tr_scene_repaint(tr_scene* scene);
+
+/* addLightSource / updateLightSource / clearLightSources (ray-tracing.hh:326-331). */
+tr_status tr_scene_add_light_source(tr_scene* scene, const tr_light* light, uint32_t* out_index);
+tr_status tr_scene_update_light_source(tr_scene* scene, uint32_t index, const tr_light* light);
+tr_status tr_scene_clear_light_sources(tr_scene* scene);
+/* addSphereLightSource (ray-tracing.cc:1285-1301): light + emissive sphere. */
+tr_status tr_scene_add_sphere_light_source(tr_scene* scene, const tr_light* light,
+                                           uint32_t* out_geom_id);
+
+/* addTriangleMesh (ray-tracing.hh:356-362, ray-tracing.cc:1506-1582).
+ * geom_ids[i] = (instance << 32) | submesh.  Emissive submeshes register a
+ * MeshLight, as the reference does (ray-tracing.cc:1554-1564). */
+tr_status tr_scene_add_triangle_mesh(tr_scene* scene, const float world[16],
+                                     size_t submesh_count, const tr_submesh* submeshes,
+                                     size_t index_count, const int32_t* tris,
+                                     size_t vert_size, const void* verts,
+                                     const tr_mesh_options* mesh_opts, uint64_t* geom_ids);
+/* addSphere (ray-tracing.cc:1341-1355). */
+tr_status tr_scene_add_sphere(tr_scene* scene, const float center[3], float radius,
+                              const tr_material* material, uint64_t* out_geom_id);
+
+/* commitScene (ray-tracing.cc:1633-1636): packs the scene into HBM and
+ * builds the 8-wide BVH. */
+tr_status tr_scene_commit(tr_scene* scene);
+tr_status tr_scene_get_info(const tr_scene* scene, tr_scene_info* info);
+
+/* drawOnce (ray-tracing.cc:1698-1703): one frame of SamplesCamera camera
+ * samples per pixel, synchronous, accumulated progressively. */
+tr_status tr_scene_draw_once(tr_scene* scene);
+/* draw(width, height, view_proj_inv) (ray-tracing.cc:1705-1773): starts the
+ * next frame; a changed camera or size resets accumulation. */
+tr_status tr_scene_draw(tr_scene* scene, uint32_t width, uint32_t height,
+                        const float view_proj_inv[16]);
+/* Completes the frame started by tr_scene_draw. */
+tr_status tr_scene_frame_wait(tr_scene* scene);
+/* The RGBA8 backbuffer of the last frame (FrameData::Backbuffer). */
+tr_status tr_scene_read_backbuffer(tr_scene* scene, uint32_t* rgba8, size_t pixel_count);
+/* The FP64 accumulation: per pixel the RGB sum over all camera samples of
+ * all frames since the last reset, and the number of samples in it. */
+tr_status tr_scene_read_accum(tr_scene* scene, double* rgb_sum, size_t pixel_count,
+                              uint64_t* samples_per_pixel);
+
+/* Philox key of the counter-based RNG (new; the reference has no seed). */
+tr_status tr_scene_set_seed(tr_scene* scene, uint64_t seed);
+tr_status tr_scene_set_estimator(tr_scene* scene, uint32_t mode);
+/* Frame index used in the RNG counter of the next draw. */
+tr_status tr_scene_set_frame_index(tr_scene* scene, uint32_t frame);
+
+/* Closest hit of a fixed ray set, as rtcIntersect / IntersectBVH
+ * (ray-tracing.cc:2056; ray-tracing.cu:821-846) answer it. */
+tr_status tr_scene_trace_rays(tr_scene* scene, size_t ray_count, const tr_ray* rays,
+                              tr_hit* hits);
+/* Renders every camera sample of the listed pixels and returns, per
+ * camera sample, its radiance (3 floats) and a hash of its hit sequence. */
+tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint32_t* pixels,
+                                float* out_rgb, uint32_t* out_hit_hash);
+tr_status tr_scene_get_stats(const tr_scene* scene, tr_stats* stats);
+/* Turns on per-launch node/primitive counting (slower; for measurement). */
+tr_status tr_scene_set_counting(tr_scene* scene, int32_t enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEMPEST_RT_H */

@@ -1,0 +1,886 @@
+/*
+ * oracle.cc - CPU oracle of the path tracer.
+ *
+ * ORACLE: test infrastructure.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, as the checker or as the timed
+ * CPU baseline; the product never calls it.
+ *
+ * It restates the reference CPU integrator RayTracerScene::subdraw
+ * (src/graphics/ray-tracing/ray-tracing.cc:1981-2379), its next-event
+ * estimation EvaluateLightSources (:1809-1930) and balance heuristic
+ * EvaluateTotalBalanceHeuristic (:1791-1806), the light samplers of
+ * include/tempest/graphics/ray-tracing/ray-tracing-common.hh:567-687 and
+ * the geometry samplers DefaultMeshSample (:375-435), SphereGeometry::sample
+ * (:1166-1180) and SphereAreaLightGeometry::sample (:1775-1783).  Embree's
+ * traversal is replaced by a binary BVH over the same primitives with the
+ * closest-hit rule of orc_geom.h; orc_trace_rays can also brute-force.
+ *
+ * Philox counter layout (shared with the GPU kernels):
+ *   ctr = (pixel, camera sample index, split<<16 | vertex<<8 | stream, sub)
+ *   key = (seed low, seed high)
+ * Streams: 0 camera jitter, 1 BSDF sample (x, y, lobe), 2 light sample
+ * (sub = light<<16 | sample; triangle pick, ra, rb), 3 Russian roulette.
+ */
+#include "oracle.h"
+#include "orc_philox.h"
+#include "orc_math.h"
+#include "orc_bsdf.h"
+#include "orc_geom.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+using namespace orc;
+
+namespace {
+
+enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2 };
+enum { STREAM_CAMERA = 0, STREAM_BSDF = 1, STREAM_LIGHT = 2, STREAM_RR = 3 };
+
+struct Geom {
+    uint32_t type;
+    uint32_t material;
+    uint32_t two_sided;
+    uint32_t vbuf;          /* index into Scene::vbufs */
+    uint32_t vertex_offset; /* bytes */
+    uint32_t stride;
+    uint32_t index_base;    /* into Scene::indices (triplets of this submesh) */
+    uint32_t tri_count;
+    float sphere[4];
+    uint32_t light;         /* sphere light: light index */
+};
+
+struct Prim {
+    float v0[3], v1[3], v2[3]; /* triangle; spheres: v0 = centre, v1[0] = radius */
+    uint32_t geom, prim;
+    uint32_t is_sphere, two_sided;
+};
+
+struct Light {
+    tr_light src;
+    uint32_t geom;          /* mesh lights and sphere lights: geometry slot */
+};
+
+struct BNode {
+    float lo[3], hi[3];
+    uint32_t left_or_first; /* internal: left child; leaf: first prim index */
+    uint32_t count;         /* 0 = internal (right = left + 1 is not assumed: see right) */
+    uint32_t right;
+};
+
+}  // namespace
+
+struct orc_scene {
+    uint32_t width, height;
+    float vpi[16];
+    uint32_t spp = 1, n_gi = 64, n_light = 4, max_depth = 1;
+    float rr = 1.0f;
+    float background[3] = {0, 0, 0};
+    uint32_t key[2] = {1, 0};
+    uint32_t estimator = TR_ESTIMATOR_REFERENCE;
+    std::vector<tr_material> materials;
+    std::vector<const tr_material*> material_keys;
+    std::vector<std::vector<uint8_t>> vbufs;
+    std::vector<int32_t> indices;
+    std::vector<Geom> geoms;
+    std::vector<Prim> prims;
+    std::vector<uint32_t> prim_order;
+    std::vector<BNode> nodes;
+    std::vector<Light> lights;
+    uint32_t instances = 0;
+    bool committed = false;
+};
+
+namespace {
+
+uint32_t intern_material(orc_scene* s, const tr_material* m)
+{
+    for (size_t i = 0; i < s->material_keys.size(); ++i)
+        if (s->material_keys[i] == m && memcmp(&s->materials[i], m, sizeof(*m)) == 0) return (uint32_t)i;
+    s->material_keys.push_back(m);
+    s->materials.push_back(*m);
+    return (uint32_t)(s->materials.size() - 1);
+}
+
+inline const float* vertex_ptr(const orc_scene* s, const Geom& g, int32_t idx)
+{
+    return reinterpret_cast<const float*>(s->vbufs[g.vbuf].data() + g.vertex_offset + (size_t)idx * g.stride);
+}
+
+/* ---------------- BVH (binary, median split) ---------------- */
+
+void prim_bounds(const Prim& p, float lo[3], float hi[3])
+{
+    if (p.is_sphere) {
+        for (int a = 0; a < 3; ++a) { lo[a] = p.v0[a] - p.v1[0]; hi[a] = p.v0[a] + p.v1[0]; }
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
+            hi[a] = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));
+        }
+    }
+}
+
+uint32_t build_node(orc_scene* s, uint32_t begin, uint32_t end, std::vector<float>& cen,
+                    std::vector<float>& blo, std::vector<float>& bhi)
+{
+    BNode n;
+    for (int a = 0; a < 3; ++a) { n.lo[a] = INFINITY; n.hi[a] = -INFINITY; }
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = begin; i < end; ++i) {
+        uint32_t p = s->prim_order[i];
+        for (int a = 0; a < 3; ++a) {
+            n.lo[a] = std::min(n.lo[a], blo[p * 3 + a]);
+            n.hi[a] = std::max(n.hi[a], bhi[p * 3 + a]);
+            clo[a] = std::min(clo[a], cen[p * 3 + a]);
+            chi[a] = std::max(chi[a], cen[p * 3 + a]);
+        }
+    }
+    /* pad the box outward so rounding in the slab test cannot cull a hit */
+    for (int a = 0; a < 3; ++a) {
+        float pad = (n.hi[a] - n.lo[a]) * 1e-5f + 1e-5f * (fabsf(n.lo[a]) + fabsf(n.hi[a])) + 1e-30f;
+        n.lo[a] -= pad;
+        n.hi[a] += pad;
+    }
+    uint32_t idx = (uint32_t)s->nodes.size();
+    s->nodes.push_back(n);
+    if (end - begin <= 4) {
+        s->nodes[idx].left_or_first = begin;
+        s->nodes[idx].count = end - begin;
+        s->nodes[idx].right = 0;
+        return idx;
+    }
+    int axis = 0;
+    float ext = chi[0] - clo[0];
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > ext) { ext = chi[a] - clo[a]; axis = a; }
+    uint32_t mid = (begin + end) / 2;
+    std::nth_element(s->prim_order.begin() + begin, s->prim_order.begin() + mid, s->prim_order.begin() + end,
+                     [&](uint32_t x, uint32_t y) {
+                         float cx = cen[x * 3 + axis], cy = cen[y * 3 + axis];
+                         return cx < cy || (cx == cy && x < y);
+                     });
+    uint32_t l = build_node(s, begin, mid, cen, blo, bhi);
+    uint32_t r = build_node(s, mid, end, cen, blo, bhi);
+    s->nodes[idx].left_or_first = l;
+    s->nodes[idx].count = 0;
+    s->nodes[idx].right = r;
+    return idx;
+}
+
+/* Slab test in float64 on the padded box. */
+inline bool box_hit(const BNode& n, const double o[3], const double inv[3], double tnear, double tfar, double* tmin_out)
+{
+    double t0 = tnear, t1 = tfar;
+    for (int a = 0; a < 3; ++a) {
+        double ta = (n.lo[a] - o[a]) * inv[a];
+        double tb = (n.hi[a] - o[a]) * inv[a];
+        if (ta > tb) std::swap(ta, tb);
+        if (ta != ta) ta = -INFINITY;
+        if (tb != tb) tb = INFINITY;
+        t0 = std::max(t0, ta);
+        t1 = std::min(t1, tb);
+    }
+    *tmin_out = t0;
+    return t0 <= t1;
+}
+
+inline void test_prim(const orc_scene* s, uint32_t pi, const WatertightRay& wr, const float org[3],
+                      const float dir[3], float tnear, float tfar, tr_hit* best)
+{
+    const Prim& p = s->prims[pi];
+    float t, u = 0.0f, v = 0.0f;
+    int hit;
+    if (p.is_sphere) hit = sphere_hit(org, dir, p.v0, p.v1[0], tnear, tfar, &t);
+    else hit = watertight_triangle(wr, org, p.v0, p.v1, p.v2, (int)p.two_sided, tnear, tfar, &t, &u, &v);
+    if (!hit) return;
+    if (best->geom != TR_INVALID_GEOMETRY && !hit_better(t, p.geom, p.prim, best->t, best->geom, best->prim)) return;
+    best->t = t; best->u = u; best->v = v; best->geom = p.geom; best->prim = p.prim;
+}
+
+tr_hit trace_closest(const orc_scene* s, const float org[3], const float dir[3], float tnear, float tfar, bool brute)
+{
+    tr_hit best;
+    best.t = tfar; best.u = best.v = 0.0f; best.geom = TR_INVALID_GEOMETRY; best.prim = TR_INVALID_GEOMETRY;
+    if (s->prims.empty()) return best;
+    WatertightRay wr = watertight_setup(dir);
+    if (brute) {
+        for (uint32_t i = 0; i < s->prims.size(); ++i) test_prim(s, i, wr, org, dir, tnear, tfar, &best);
+    } else {
+        double o[3] = {org[0], org[1], org[2]};
+        double inv[3];
+        for (int a = 0; a < 3; ++a) inv[a] = dir[a] != 0.0f ? 1.0 / (double)dir[a] : copysign(1e300, (double)dir[a]);
+        uint32_t stack[128];
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const BNode& n = s->nodes[stack[--sp]];
+            double tmin;
+            double tmax = best.geom == TR_INVALID_GEOMETRY ? (double)tfar : (double)best.t;
+            if (!box_hit(n, o, inv, (double)tnear, tmax * (1.0 + 1e-6) + 1e-30, &tmin)) continue;
+            if (n.count) {
+                for (uint32_t i = 0; i < n.count; ++i)
+                    test_prim(s, s->prim_order[n.left_or_first + i], wr, org, dir, tnear, tfar, &best);
+            } else {
+                double tl, tr;
+                const BNode& L = s->nodes[n.left_or_first];
+                const BNode& R = s->nodes[n.right];
+                bool hl = box_hit(L, o, inv, (double)tnear, tmax * (1.0 + 1e-6) + 1e-30, &tl);
+                bool hr = box_hit(R, o, inv, (double)tnear, tmax * (1.0 + 1e-6) + 1e-30, &tr);
+                if (hl && hr) {
+                    if (tl <= tr) { stack[sp++] = n.right; stack[sp++] = n.left_or_first; }
+                    else { stack[sp++] = n.left_or_first; stack[sp++] = n.right; }
+                } else if (hl) {
+                    stack[sp++] = n.left_or_first;
+                } else if (hr) {
+                    stack[sp++] = n.right;
+                }
+            }
+        }
+    }
+    if (best.geom == TR_INVALID_GEOMETRY) best.t = tfar;
+    return best;
+}
+
+/* ---------------- RNG ---------------- */
+
+struct Rng {
+    uint32_t key[2];
+    uint32_t pixel, sample, split;
+    void draw(uint32_t vertex, uint32_t stream, uint32_t sub, uint32_t out[4]) const
+    {
+        uint32_t ctr[4] = {pixel, sample, (split << 16) | (vertex << 8) | stream, sub};
+        orc_philox4x32_10(ctr, key, out);
+    }
+};
+
+/* ---------------- geometry sampling ---------------- */
+
+template <class T>
+void sample_geometry(const orc_scene* s, const tr_hit& h, V3<T> org, V3<T> dir, Sample<T>* sd)
+{
+    const Geom& g = s->geoms[h.geom];
+    sd->mat = &s->materials[g.material];
+    if (g.type == GEOM_MESH) {
+        /* RTMesh::sample + DefaultMeshSample (ray-tracing.cc:375-461) */
+        const int32_t* ib = &s->indices[g.index_base + (size_t)h.prim * 3];
+        const uint8_t* base = s->vbufs[g.vbuf].data() + g.vertex_offset;
+        const uint8_t* p0 = base + (size_t)ib[0] * g.stride;
+        const uint8_t* p1 = base + (size_t)ib[1] * g.stride;
+        const uint8_t* p2 = base + (size_t)ib[2] * g.stride;
+        uint32_t no = g.stride - 12;
+        const float* n0 = reinterpret_cast<const float*>(p0 + no);
+        const float* n1 = reinterpret_cast<const float*>(p1 + no);
+        const float* n2 = reinterpret_cast<const float*>(p2 + no);
+        T u = (T)h.u, v = (T)h.v;
+        T w = ((T)1 - u) - v;
+        V3<T> N0 = v3((T)n0[0], (T)n0[1], (T)n0[2]);
+        V3<T> N1 = v3((T)n1[0], (T)n1[1], (T)n1[2]);
+        V3<T> N2 = v3((T)n2[0], (T)n2[1], (T)n2[2]);
+        V3<T> norm = normalize(N0 * w + N1 * u + N2 * v);
+        sd->tangent = v3((T)0, (T)0, (T)0);
+        sd->binormal = v3((T)0, (T)0, (T)0);
+        if (g.stride >= 44) {
+            const float* t0 = reinterpret_cast<const float*>(p0 + 20);
+            const float* t1 = reinterpret_cast<const float*>(p1 + 20);
+            const float* t2 = reinterpret_cast<const float*>(p2 + 20);
+            V3<T> tan = v3((T)t0[0], (T)t0[1], (T)t0[2]) * w + v3((T)t1[0], (T)t1[1], (T)t1[2]) * u +
+                        v3((T)t2[0], (T)t2[1], (T)t2[2]) * v;
+            M3<T> b = make_basis_orthogonalize(tan, norm);
+            sd->tangent = b.c0;
+            sd->binormal = b.c1;
+        }
+        sd->normal = norm;
+    } else {
+        float of[3] = {(float)org.x, (float)org.y, (float)org.z};
+        float df[3] = {(float)dir.x, (float)dir.y, (float)dir.z};
+        if (g.type == GEOM_SPHERE) {
+            /* SphereGeometry::sample (ray-tracing.cc:1166-1180); the normal
+             * is the one IntersectSphere wrote into ray.Ng */
+            float n[3];
+            sphere_normal(of, df, g.sphere, h.t, n);
+            V3<T> N = v3((T)n[0], (T)n[1], (T)n[2]);
+            sd->normal = N;
+            sd->binormal = normalize(cross(v3((T)0, (T)0, (T)1), N));
+            sd->tangent = cross(sd->binormal, sd->normal);
+        } else {
+            /* SphereAreaLightGeometry::sample (ray-tracing.cc:1775-1783) */
+            V3<T> c = v3((T)g.sphere[0], (T)g.sphere[1], (T)g.sphere[2]);
+            sd->normal = normalize(org + dir * (T)h.t - c);
+            sd->tangent = v3((T)0, (T)0, (T)0);
+            sd->binormal = v3((T)0, (T)0, (T)0);
+        }
+    }
+}
+
+/* ---------------- lights ---------------- */
+
+bool is_area_light(uint32_t type) { return type == TR_LIGHT_SPHERE_AREA; }
+
+template <class T> struct LightSample {
+    uint32_t geom;
+    V3<T> dir;
+    T dist;
+    V3<T> radiance;
+    T pdf;
+};
+
+/* TriangleArea of the projected vertices, sampling3.hh:206-212 */
+template <class T> T projected_triangle_area(V3<T> pos, V3<T> p0, V3<T> p1, V3<T> p2)
+{
+    V3<T> a = normalize(p0 - pos), b = normalize(p1 - pos), c = normalize(p2 - pos);
+    return (T)0.5 * length(cross(b - a, c - a));
+}
+
+template <class T> V3<T> vpos(const orc_scene* s, const Geom& g, uint32_t tri, int k)
+{
+    const int32_t* ib = &s->indices[g.index_base + (size_t)tri * 3];
+    const float* f = vertex_ptr(s, g, ib[k]);
+    return v3((T)f[0], (T)f[1], (T)f[2]);
+}
+
+/* Light sampling (ray-tracing-common.hh:567-647).  xi: triangle pick, ra, rb. */
+template <class T>
+LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, const uint32_t xi_raw[4])
+{
+    LightSample<T> r;
+    const tr_light& l = L.src;
+    T ra = (T)orc_u01(xi_raw[1]), rb = (T)orc_u01(xi_raw[2]);
+    if (l.type == TR_LIGHT_DIRECTIONAL) {
+        r.dist = (T)INFINITY;
+        r.dir = spec3<T>(l.direction);
+        r.radiance = spec3<T>(l.radiance);
+        r.pdf = (T)1;
+        r.geom = TR_INVALID_GEOMETRY;
+    } else if (l.type == TR_LIGHT_POINT) {
+        V3<T> dv = spec3<T>(l.position) - pos;
+        T dist = length(dv);
+        r.dist = (T)INFINITY;
+        r.dir = dv / dist;
+        r.radiance = spec3<T>(l.radiance) / (C<T>::tau * dist * dist);
+        r.pdf = (T)1;
+        r.geom = TR_INVALID_GEOMETRY;
+    } else if (l.type == TR_LIGHT_SPHERE_AREA) {
+        /* SampleSphereArea, sampling3.hh:157-203 */
+        V3<T> c = spec3<T>(l.sphere_center);
+        T radius = (T)l.sphere_radius;
+        V3<T> dc = c - pos;
+        T dist2 = dot(dc, dc);
+        T dist = xsqrt(dist2);
+        r.radiance = spec3<T>(l.radiance);
+        r.geom = L.geom;
+        if (dist <= radius) {
+            /* UniformSampleSphereArea, sampling3.hh:142-154 */
+            T sin_phi, cos_phi;
+            sincos_turns(rb, &sin_phi, &cos_phi);
+            T r2 = (T)2 * radius;
+            T q = xsqrt(ra * ((T)1 - ra));
+            V3<T> res = c - pos + v3(r2 * cos_phi * q, r2 * sin_phi * q, radius * ((T)1 - (T)2 * ra));
+            T t = length(res);
+            r.dir = res / t;
+            r.dist = t;
+            r.pdf = (T)1 / ((T)4 * C<T>::pi);
+            return r;
+        }
+        V3<T> to_c = dc / dist;
+        T sin_rc = radius / dist;
+        T cos_rc = xsqrt((T)1 - sin_rc * sin_rc);
+        T sin_phi, cos_phi;
+        sincos_turns(ra, &sin_phi, &cos_phi);
+        T cos_theta = (T)1 - rb + rb * cos_rc;
+        T sin_theta = xsqrt((T)1 - cos_theta * cos_theta);
+        M3<T> basis = make_basis(to_c);
+        V3<T> d = normalize(transform(basis, v3(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta)));
+        T b = dist / cos_theta;
+        T cc = dist2 - radius * radius;
+        T D = b * b - cc;
+        T t = b;
+        if (D >= (T)0) t -= xsqrt(D);
+        r.dir = d;
+        r.dist = t;
+        r.pdf = (T)1 / ((T)2 * C<T>::pi * ((T)1 - cos_rc));
+    } else {
+        /* MeshAreaSampleLightSource + UniformSampleTriangleArea
+         * (ray-tracing-common.hh:622-647, sampling3.hh:214-227) */
+        const Geom& g = s->geoms[L.geom];
+        uint32_t tri = (uint32_t)(((uint64_t)xi_raw[0] * g.tri_count) >> 32);
+        V3<T> p0 = vpos<T>(s, g, tri, 0), p1 = vpos<T>(s, g, tri, 1), p2 = vpos<T>(s, g, tri, 2);
+        T sqrt_r = xsqrt(ra);
+        V3<T> e0 = p1 - p0, e1 = p2 - p0;
+        V3<T> point = p0 + (sqrt_r * rb) * e0 + ((T)1 - sqrt_r) * e1;
+        V3<T> du = point - pos;
+        T len = length(du);
+        r.dir = du / len;
+        r.dist = len;
+        T pdf = dot(cross(e0, e1), r.dir) < -(T)1e-6f ? (T)1 / projected_triangle_area(pos, p0, p1, p2) : (T)0;
+        r.pdf = pdf / (T)g.tri_count;
+        r.radiance = spec3<T>(s->materials[g.material].diffuse);
+        r.geom = L.geom;
+    }
+    return r;
+}
+
+/* LightSourceComputePDFLookup (ray-tracing-common.hh:582-687) */
+template <class T> T light_pdf(const orc_scene* s, const Light& L, V3<T> pos, V3<T> dir)
+{
+    const tr_light& l = L.src;
+    if (l.type == TR_LIGHT_DIRECTIONAL || l.type == TR_LIGHT_POINT) return (T)1;
+    if (l.type == TR_LIGHT_SPHERE_AREA) {
+        /* ComputeSphereAreaPDF, sampling3.hh:229-262 */
+        V3<T> c = spec3<T>(l.sphere_center);
+        T radius = (T)l.sphere_radius;
+        V3<T> dc = c - pos;
+        T dist = xsqrt(dot(dc, dc));
+        if (dist <= radius) return (T)1 / ((T)4 * C<T>::pi);
+        V3<T> to_c = dc / dist;
+        T sin_rc = radius / dist;
+        T cos_rc = xsqrt((T)1 - sin_rc * sin_rc);
+        T cos_theta = dot(dir, to_c);
+        if (cos_theta < cos_rc) return (T)0;
+        return (T)1 / ((T)2 * C<T>::pi * ((T)1 - cos_rc));
+    }
+    /* MeshAreaLightComputePDF, ray-tracing-common.hh:650-687 */
+    const Geom& g = s->geoms[L.geom];
+    T total = (T)0;
+    for (uint32_t tri = 0; tri < g.tri_count; ++tri) {
+        V3<T> p0 = vpos<T>(s, g, tri, 0), p1 = vpos<T>(s, g, tri, 1), p2 = vpos<T>(s, g, tri, 2);
+        V3<T> d0 = p0 - p1, d1 = p2 - p1;
+        V3<T> un = cross(d1, d0);
+        if (dot(dir, un) >= (T)0) continue;
+        total += projected_triangle_area(pos, p0, p1, p2);
+    }
+    return total != (T)0 ? (T)1 / total : (T)0;
+}
+
+/* EvaluateTotalBalanceHeuristic (ray-tracing.cc:1791-1806) */
+template <class T> T total_balance(const orc_scene* s, T light_samples, T surface_samples, const Sample<T>& sd, V3<T> pos)
+{
+    T total = (T)0;
+    for (const Light& L : s->lights) {
+        T n = is_area_light(L.src.type) ? light_samples : (T)1;
+        total += light_pdf(s, L, pos, sd.wi) * n;
+    }
+    total += bsdf_pdf(sd) * surface_samples;
+    return total;
+}
+
+/* Hash of a camera sample whose GI splits all share one hit sequence. */
+inline uint32_t repeat_hash(uint32_t h, uint32_t n)
+{
+    uint32_t c = h;
+    for (uint32_t j = 1; j < n; ++j) c = (c * 0x9E3779B1u) ^ h;
+    return c;
+}
+
+inline void hash_hit(uint32_t* h, const tr_hit& hit)
+{
+    *h = (*h ^ hit.geom) * 16777619u;
+    *h = (*h ^ hit.prim) * 16777619u;
+}
+
+/* EvaluateLightSources (ray-tracing.cc:1809-1930), surfaces only. */
+template <class T>
+V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const Sample<T>& sd,
+                      uint32_t max_light_samples, uint32_t max_surface_samples, V3<T> pos, V3<T> throughput,
+                      bool apply_balance)
+{
+    V3<T> radiance = v3((T)0, (T)0, (T)0);
+    for (uint32_t li = 0; li < s->lights.size(); ++li) {
+        const Light& L = s->lights[li];
+        bool area = is_area_light(L.src.type);
+        uint32_t samples = area ? max_light_samples : 1;
+        V3<T> color = v3((T)0, (T)0, (T)0);
+        Sample<T> lsd = sd;
+        for (uint32_t k = 0; k < samples; ++k) {
+            uint32_t xi[4];
+            rng.draw(vertex, STREAM_LIGHT, (li << 16) | k, xi);
+            LightSample<T> smp = sample_light(s, L, pos, xi);
+            if (smp.pdf == (T)0) continue;
+            lsd.wi = smp.dir;
+            V3<T> cur = throughput;
+            float of[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
+            float df[3] = {(float)smp.dir.x, (float)smp.dir.y, (float)smp.dir.z};
+            tr_hit h = trace_closest(s, of, df, 0.0f, (float)smp.dist, false);
+            if (h.geom == TR_INVALID_GEOMETRY || h.geom == smp.geom) {
+                if (apply_balance && area) {
+                    T total = total_balance(s, (T)max_light_samples, (T)max_surface_samples, lsd, pos);
+                    T balance = (light_pdf(s, L, pos, lsd.wi) * (T)samples) / total;
+                    cur *= balance;
+                }
+                color += smp.radiance * cur * bsdf_eval(lsd) / smp.pdf;
+            }
+        }
+        radiance += color / (T)samples;
+    }
+    return radiance;
+}
+
+template <class T> inline V3<T> to_v3(const float* f) { return v3((T)f[0], (T)f[1], (T)f[2]); }
+
+/* One camera sample of one pixel: the body of the camera loop of subdraw
+ * (ray-tracing.cc:2011-2345) for surfaces. */
+template <class T>
+V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint32_t* hash_out)
+{
+    uint32_t x = pixel % s->width, y = pixel / s->width;
+    Rng rng;
+    rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
+    rng.pixel = pixel; rng.sample = cs; rng.split = 0;
+    uint32_t hash = 2166136261u;
+    float sx, sy;
+    if (s->spp > 1) {
+        uint32_t xi[4];
+        rng.draw(0, STREAM_CAMERA, 0, xi);
+        sx = 2.0f * ((float)x + orc_u01(xi[0])) / (float)s->width - 1.0f;
+        sy = 2.0f * ((float)y + orc_u01(xi[1])) / (float)s->height - 1.0f;
+    } else {
+        sx = 2.0f * (float)x / (float)(s->width - 1) - 1.0f;
+        sy = 2.0f * (float)y / (float)(s->height - 1) - 1.0f;
+    }
+    /* Matrix4 * Vector4 (matrix4.hh:858-880) and ToVector3 (vector4.hh:125-130) */
+    const float* m = s->vpi;
+    float in0[4] = {sx, sy, -1.0f, 1.0f}, in1[4] = {sx, sy, 1.0f, 1.0f};
+    float p0[4], p1[4];
+    for (int r = 0; r < 4; ++r) {
+        p0[r] = m[r] * in0[0]; p1[r] = m[r] * in1[0];
+        p0[r] += m[4 + r] * in0[1]; p1[r] += m[4 + r] * in1[1];
+        p0[r] += m[8 + r] * in0[2]; p1[r] += m[8 + r] * in1[2];
+        p0[r] += m[12 + r] * in0[3]; p1[r] += m[12 + r] * in1[3];
+    }
+    float w0 = 1.0f / p0[3], w1 = 1.0f / p1[3];
+    float start_f[3] = {p0[0] * w0, p0[1] * w0, p0[2] * w0};
+    float end_f[3] = {p1[0] * w1, p1[1] * w1, p1[2] * w1};
+    float dir_f[3] = {end_f[0] - start_f[0], end_f[1] - start_f[1], end_f[2] - start_f[2]};
+    {
+        float l = sqrtf(dir_f[0] * dir_f[0] + dir_f[1] * dir_f[1] + dir_f[2] * dir_f[2]);
+        if (l != 0.0f) { float r = 1.0f / l; dir_f[0] *= r; dir_f[1] *= r; dir_f[2] *= r; }
+    }
+    V3<T> pos = to_v3<T>(start_f);
+    Sample<T> sd;
+    sd.wi = to_v3<T>(dir_f);
+    sd.dir_density = (T)0;
+    sd.total_density = (T)0;
+    float of[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
+    float df[3] = {(float)sd.wi.x, (float)sd.wi.y, (float)sd.wi.z};
+    tr_hit h = trace_closest(s, of, df, 0.0f, INFINITY, false);
+    if (h.geom == TR_INVALID_GEOMETRY) {
+        hash = (hash ^ 0xffffffffu) * 16777619u;
+        *hash_out = repeat_hash(hash, s->n_gi);
+        return to_v3<T>(s->background);
+    }
+    hash_hit(&hash, h);
+    sample_geometry(s, h, pos, sd.wi, &sd);
+    pos = pos + (T)h.t * sd.wi;
+    sd.position = pos;
+    material_cache(&sd);
+    V3<T> start_T = v3((T)1, (T)1, (T)1);
+    sd.wo = -sd.wi;
+    if (is_emissive(sd.mat->model)) {
+        *hash_out = repeat_hash(hash, s->n_gi);
+        return start_T * spec3<T>(sd.mat->diffuse);
+    }
+    V3<T> local = evaluate_lights(s, rng, 0, sd, s->n_light, s->n_gi, pos, start_T, s->max_depth != 0);
+    V3<T> global = v3((T)0, (T)0, (T)0);
+    uint32_t camera_hash = hash;
+    uint32_t combined = 0;
+    uint32_t rays_sq = (uint32_t)sqrtf((float)s->n_gi);
+    for (uint32_t split = 0; split < s->n_gi; ++split) {
+        /* each split keeps its own radiance sum and hit hash; the GPU kernels
+         * combine them in the same order */
+        V3<T> gsplit = v3((T)0, (T)0, (T)0);
+        hash = camera_hash;
+        Rng srng = rng;
+        srng.split = split;
+        Sample<T> cur = sd;
+        uint32_t ss = s->n_gi, ls = s->n_light;
+        V3<T> cpos = pos;
+        V3<T> ct = start_T;
+        Strata<T> st;
+        st.x = (T)(float)(split % rays_sq);
+        st.y = (T)(float)(split / rays_sq);
+        st.tx = st.ty = (T)(float)rays_sq;
+        for (uint32_t d = 0; d < s->max_depth; ++d) {
+            uint32_t xr[4];
+            srng.draw(d, STREAM_BSDF, 0, xr);
+            T xi[3] = {(T)orc_u01(xr[0]), (T)orc_u01(xr[1]), (T)orc_u01(xr[2])};
+            bsdf_sample(st, &cur, xi);
+            if (cur.pdf == (T)0) break;
+            T balance = (T)1;
+            {
+                T total = total_balance(s, (T)ls, (T)ss, cur, cpos);
+                balance = total > (T)0 ? ((T)ss * cur.pdf) / total : (T)0;
+                ct *= bsdf_eval(cur) / cur.pdf;
+            }
+            float o2[3] = {(float)cpos.x, (float)cpos.y, (float)cpos.z};
+            float d2[3] = {(float)cur.wi.x, (float)cur.wi.y, (float)cur.wi.z};
+            tr_hit hh = trace_closest(s, o2, d2, 0.0f, INFINITY, false);
+            if (hh.geom == TR_INVALID_GEOMETRY) {
+                hash = (hash ^ 0xffffffffu) * 16777619u;
+                gsplit += to_v3<T>(s->background) * ct;
+                break;
+            }
+            hash_hit(&hash, hh);
+            sample_geometry(s, hh, cpos, cur.wi, &cur);
+            cpos = cpos + (T)hh.t * cur.wi;
+            cur.position = cpos;
+            material_cache(&cur);
+            if (is_emissive(cur.mat->model)) {
+                gsplit += ct * balance * spec3<T>(cur.mat->diffuse);
+                break;
+            }
+            cur.wo = -cur.wi;
+            ss = 1;
+            ls = 1;
+            gsplit += evaluate_lights(s, srng, d + 1, cur, ls, ss, cpos, ct, d + 1 != s->max_depth);
+            ct *= balance / (T)s->rr;
+            st.x = st.y = (T)0;
+            st.tx = st.ty = (T)1;
+            if (s->rr < 1.0f) {
+                uint32_t xq[4];
+                srng.draw(d + 1, STREAM_RR, 0, xq);
+                if ((T)orc_u01(xq[0]) > (T)s->rr) break;
+            }
+        }
+        if (split == 0) { global = gsplit; combined = hash; }
+        else { global += gsplit; combined = (combined * 0x9E3779B1u) ^ hash; }
+    }
+    hash = combined;
+    global = global / (T)s->n_gi;
+    *hash_out = hash;
+    return local + global;
+}
+
+template <class F> void parallel_for(uint32_t n, int nthreads, uint32_t chunk, F fn)
+{
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            uint32_t b = next.fetch_add(chunk);
+            if (b >= n) break;
+            uint32_t e = std::min(n, b + chunk);
+            for (uint32_t i = b; i < e; ++i) fn(i);
+        }
+    };
+    if (nthreads == 1) { worker(); return; }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_scene* orc_scene_create(uint32_t width, uint32_t height, const float vpi[16])
+{
+    orc_scene* s = new orc_scene();
+    s->width = width;
+    s->height = height;
+    memcpy(s->vpi, vpi, sizeof(s->vpi));
+    s->materials.reserve(1024);
+    return s;
+}
+
+void orc_scene_destroy(orc_scene* s) { delete s; }
+void orc_set_samples_camera(orc_scene* s, uint32_t n) { s->spp = n; }
+void orc_set_samples_global_illumination(orc_scene* s, uint32_t n) { s->n_gi = n; }
+void orc_set_samples_local_area_light(orc_scene* s, uint32_t n) { s->n_light = n; }
+void orc_set_max_ray_depth(orc_scene* s, uint32_t d) { s->max_depth = d; }
+void orc_set_russian_roulette(orc_scene* s, float rr) { s->rr = rr; }
+void orc_set_background_spectrum(orc_scene* s, const float rgb[3]) { memcpy(s->background, rgb, 12); }
+void orc_set_seed(orc_scene* s, uint64_t seed) { s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32); }
+void orc_set_estimator(orc_scene* s, uint32_t mode) { s->estimator = mode; }
+
+int orc_add_triangle_mesh(orc_scene* s, const float world[16], size_t submesh_count, const tr_submesh* subs,
+                          size_t index_count, const int32_t* tris, size_t vert_size, const void* verts,
+                          const tr_mesh_options* opts, uint64_t* geom_ids)
+{
+    if (s->committed) return -1;
+    if (s->materials.size() + submesh_count > s->materials.capacity()) return -1; /* keep pointers stable */
+    uint32_t vb = (uint32_t)s->vbufs.size();
+    s->vbufs.emplace_back((const uint8_t*)verts, (const uint8_t*)verts + vert_size);
+    uint32_t inst = s->instances++;
+    for (size_t si = 0; si < submesh_count; ++si) {
+        const tr_submesh& sm = subs[si];
+        Geom g;
+        memset(&g, 0, sizeof(g));
+        g.type = GEOM_MESH;
+        g.material = intern_material(s, sm.material);
+        g.two_sided = opts && opts->two_sided ? 1 : 0;
+        g.vbuf = vb;
+        g.vertex_offset = sm.vertex_offset;
+        g.stride = sm.stride;
+        g.index_base = (uint32_t)s->indices.size();
+        g.tri_count = sm.vertex_count / 3;
+        s->indices.insert(s->indices.end(), tris + sm.base_index, tris + sm.base_index + (size_t)g.tri_count * 3);
+        /* world transform applied to positions (column-major 4x4) */
+        const uint8_t* base = s->vbufs[vb].data() + sm.vertex_offset;
+        uint32_t slot = (uint32_t)s->geoms.size();
+        for (uint32_t t = 0; t < g.tri_count; ++t) {
+            Prim p;
+            memset(&p, 0, sizeof(p));
+            float* dst[3] = {p.v0, p.v1, p.v2};
+            for (int k = 0; k < 3; ++k) {
+                const float* f = reinterpret_cast<const float*>(base + (size_t)s->indices[g.index_base + t * 3 + k] * sm.stride);
+                for (int r = 0; r < 3; ++r)
+                    dst[k][r] = ((world[r] * f[0] + world[4 + r] * f[1]) + world[8 + r] * f[2]) + world[12 + r];
+            }
+            p.geom = slot;
+            p.prim = t;
+            p.two_sided = g.two_sided;
+            s->prims.push_back(p);
+        }
+        s->geoms.push_back(g);
+        if (s->materials[g.material].model == TR_MODEL_EMISSIVE) {
+            Light L;
+            memset(&L, 0, sizeof(L));
+            L.src.type = TR_LIGHT_MESH_AREA;
+            L.geom = slot;
+            s->lights.push_back(L);
+        }
+        if (geom_ids) geom_ids[si] = ((uint64_t)inst << 32) | (uint64_t)si;
+    }
+    (void)index_count;
+    return 0;
+}
+
+static uint32_t add_sphere_geom(orc_scene* s, const float c[3], float r, uint32_t type, uint32_t material, uint32_t light)
+{
+    Geom g;
+    memset(&g, 0, sizeof(g));
+    g.type = type;
+    g.material = material;
+    g.sphere[0] = c[0]; g.sphere[1] = c[1]; g.sphere[2] = c[2]; g.sphere[3] = r;
+    g.light = light;
+    uint32_t slot = (uint32_t)s->geoms.size();
+    s->geoms.push_back(g);
+    Prim p;
+    memset(&p, 0, sizeof(p));
+    p.v0[0] = c[0]; p.v0[1] = c[1]; p.v0[2] = c[2];
+    p.v1[0] = r;
+    p.geom = slot;
+    p.prim = 0;
+    p.is_sphere = 1;
+    s->prims.push_back(p);
+    return slot;
+}
+
+int orc_add_sphere(orc_scene* s, const float center[3], float radius, const tr_material* m, uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    uint32_t slot = add_sphere_geom(s, center, radius, GEOM_SPHERE, intern_material(s, m), 0);
+    if (out) *out = slot;
+    return 0;
+}
+
+int orc_add_light_source(orc_scene* s, const tr_light* light, uint32_t* out)
+{
+    Light L;
+    memset(&L, 0, sizeof(L));
+    L.src = *light;
+    L.geom = TR_INVALID_GEOMETRY;
+    if (out) *out = (uint32_t)s->lights.size();
+    s->lights.push_back(L);
+    return 0;
+}
+
+int orc_add_sphere_light_source(orc_scene* s, const tr_light* light, uint32_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    tr_material m;
+    memset(&m, 0, sizeof(m));
+    m.model = TR_MODEL_EMISSIVE;
+    memcpy(m.diffuse, light->radiance, 12);
+    m.fresnel[0] = 1.0f;
+    s->material_keys.push_back(nullptr);
+    s->materials.push_back(m);
+    uint32_t mat = (uint32_t)s->materials.size() - 1;
+    Light L;
+    memset(&L, 0, sizeof(L));
+    L.src = *light;
+    L.src.type = TR_LIGHT_SPHERE_AREA;
+    uint32_t slot = add_sphere_geom(s, light->sphere_center, light->sphere_radius, GEOM_SPHERE_LIGHT, mat,
+                                    (uint32_t)s->lights.size());
+    L.geom = slot;
+    L.src.geometry_id = slot;
+    s->lights.push_back(L);
+    if (out) *out = slot;
+    return 0;
+}
+
+int orc_commit(orc_scene* s)
+{
+    uint32_t n = (uint32_t)s->prims.size();
+    s->prim_order.resize(n);
+    s->nodes.clear();
+    if (n == 0) { s->committed = true; return 0; }
+    std::vector<float> cen(n * 3), blo(n * 3), bhi(n * 3);
+    for (uint32_t i = 0; i < n; ++i) {
+        s->prim_order[i] = i;
+        prim_bounds(s->prims[i], &blo[i * 3], &bhi[i * 3]);
+        for (int a = 0; a < 3; ++a) cen[i * 3 + a] = 0.5f * (blo[i * 3 + a] + bhi[i * 3 + a]);
+    }
+    s->nodes.reserve(2 * n);
+    build_node(s, 0, n, cen, blo, bhi);
+    s->committed = true;
+    return 0;
+}
+
+void orc_trace_rays(const orc_scene* s, size_t n, const tr_ray* rays, tr_hit* hits, int brute, int nthreads)
+{
+    parallel_for((uint32_t)n, nthreads, 256, [&](uint32_t i) {
+        hits[i] = trace_closest(s, rays[i].org, rays[i].dir, rays[i].tnear, rays[i].tfar, brute != 0);
+    });
+}
+
+void orc_render_paths(const orc_scene* s, size_t pixel_count, const uint32_t* pixels, uint32_t frame,
+                      float* out_rgb, uint32_t* out_hash, int nthreads)
+{
+    uint32_t total = (uint32_t)(pixel_count * s->spp);
+    parallel_for(total, nthreads, 16, [&](uint32_t i) {
+        uint32_t pi = i / s->spp, k = i % s->spp;
+        uint32_t hash;
+        V3<float> c = render_camera_sample<float>(s, pixels[pi], frame * s->spp + k, &hash);
+        out_rgb[i * 3 + 0] = c.x;
+        out_rgb[i * 3 + 1] = c.y;
+        out_rgb[i * 3 + 2] = c.z;
+        if (out_hash) out_hash[i] = hash;
+    });
+}
+
+double orc_render_rows(const orc_scene* s, uint32_t row_begin, uint32_t row_end, uint32_t frame, int use_fp64,
+                       double* out_sum, double* out_sumsq, int nthreads)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t first = row_begin * s->width;
+    uint32_t count = (row_end - row_begin) * s->width;
+    parallel_for(count, nthreads, 16, [&](uint32_t i) {
+        uint32_t p = first + i;
+        double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < s->spp; ++k) {
+            uint32_t hash;
+            double c[3];
+            if (use_fp64) {
+                V3<double> v = render_camera_sample<double>(s, p, frame * s->spp + k, &hash);
+                c[0] = v.x; c[1] = v.y; c[2] = v.z;
+            } else {
+                V3<float> v = render_camera_sample<float>(s, p, frame * s->spp + k, &hash);
+                c[0] = v.x; c[1] = v.y; c[2] = v.z;
+            }
+            for (int a = 0; a < 3; ++a) { sum[a] += c[a]; sq[a] += c[a] * c[a]; }
+        }
+        for (int a = 0; a < 3; ++a) {
+            out_sum[(size_t)p * 3 + a] = sum[a];
+            if (out_sumsq) out_sumsq[(size_t)p * 3 + a] = sq[a];
+        }
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

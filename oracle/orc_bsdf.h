@@ -1,0 +1,359 @@
+/*
+ * orc_bsdf.h - BSDF evaluation, PDFs and sampling of the oracle.
+ *
+ * ORACLE: test infrastructure only.  Each function restates the reference
+ * function named in its comment, in the reference's operation order.
+ * Deviations, applied identically in the GPU kernels:
+ *  - random numbers come from Philox (three per surface sample: the two
+ *    stratified coordinates and the lobe selector, in the reference's draw
+ *    order) instead of FastFloatRand;
+ *  - sin/cos of 2*pi*u use tr_sincos_turns instead of FastSinCos
+ *    (functions.hh:322-353), logf uses tr_logf, expf uses tr_expf,
+ *    pow(x, 5) is x^2*x^2*x.
+ */
+#ifndef ORC_BSDF_H
+#define ORC_BSDF_H
+
+#include "orc_math.h"
+#include "../include/tempest_rt.h"
+
+namespace orc {
+
+/* SampleData (ray-tracing-common.hh:184-198). */
+template <class T> struct Sample {
+    const tr_material* mat;
+    V3<T> wo;       /* OutgoingLight */
+    V3<T> wi;       /* IncidentLight */
+    V3<T> tangent;
+    V3<T> binormal;
+    V3<T> normal;
+    V3<T> position;
+    T total_density;
+    T dir_density;  /* DirectionalDensity */
+    T pdf;
+};
+
+template <class T> struct Strata {
+    T x, y, tx, ty;
+};
+
+template <class T> inline V3<T> spec3(const float* f) { return v3((T)f[0], (T)f[1], (T)f[2]); }
+
+/* SpecularPowerToGeometryFactorSq, microfacet-common.hh:109-112 */
+template <class T> inline T geom_factor_sq(T power) { return (T)2 / (power + (T)2); }
+
+/* FresnelSchlick, functions.hh:262-265 */
+template <class T> inline T fresnel_schlick(T refl, T cos_theta) { return refl + ((T)1 - refl) * pow5((T)1 - cos_theta); }
+
+/* CosTransmittance, functions.hh:268-271 */
+template <class T> inline T cos_transmittance(T eta, T cos_inc)
+{
+    return xsqrt((T)1 - ((T)1 - cos_inc * cos_inc) / (eta * eta));
+}
+
+/* FresnelConductor, functions.hh:275-294 */
+template <class T> inline T fresnel_conductor(T eta, T k, T cos_inc)
+{
+    T cos_inc2 = cos_inc * cos_inc;
+    T sin_inc2 = (T)1 - cos_inc2;
+    T eta2 = eta * eta;
+    T etak2 = k * k;
+    T t0 = eta2 - etak2 - sin_inc2;
+    T a2plusb2 = xsqrt(t0 * t0 + (T)4 * eta2 * etak2);
+    T t1 = a2plusb2 + cos_inc2;
+    T a = xsqrt((T)0.5 * (a2plusb2 + t0));
+    T t2 = (T)2 * a * cos_inc;
+    T r_perp = (t1 - t2) / (t1 + t2);
+    T t3 = cos_inc2 * a2plusb2 + sin_inc2 * sin_inc2;
+    T t4 = t2 * sin_inc2;
+    T r_parallel = r_perp * (t3 - t4) / (t3 + t4);
+    return (T)0.5 * (r_perp + r_parallel);
+}
+
+/* FresnelDielectric, functions.hh:296-301 */
+template <class T> inline T fresnel_dielectric(T eta, T cos_inc, T cos_trans)
+{
+    T r_parallel = (eta * cos_inc - cos_trans) / (eta * cos_inc + cos_trans);
+    T r_perp = (cos_inc - eta * cos_trans) / (cos_inc + eta * cos_trans);
+    return (T)0.5 * (r_parallel * r_parallel + r_perp * r_perp);
+}
+
+template <class T> struct Angles {
+    T cos_in, cos_on, cos_im, cos_mn;
+};
+
+/* ComputeMicrofacetAngles, microfacet-common.hh:74-84 */
+template <class T> inline Angles<T> microfacet_angles(const Sample<T>& s)
+{
+    V3<T> h = normalize(s.wi + s.wo);
+    Angles<T> a;
+    a.cos_in = dot(s.wi, s.normal);
+    a.cos_on = dot(s.wo, s.normal);
+    a.cos_im = dot(s.wi, h);
+    a.cos_mn = dot(s.normal, h);
+    return a;
+}
+
+/* GGXMicrofacetDistribution, ggx-models-impl.hh:126-139 */
+template <class T> inline T ggx_d(T g2, T cos_nh)
+{
+    if (cos_nh <= (T)0) return (T)0;
+    T c2 = cos_nh * cos_nh;
+    T c4 = c2 * c2;
+    T tan2 = ((T)1 - c2) / c2;
+    T den = g2 + tan2;
+    return g2 / (C<T>::pi * c4 * den * den);
+}
+
+/* G1_GGX, ggx-models-impl.hh:141-151 */
+template <class T> inline T ggx_g1(T g2, T cos_vm, T cos_vn)
+{
+    T ratio = cos_vm / cos_vn;
+    if (ratio <= (T)0) return (T)0;
+    T c2 = cos_vn * cos_vn;
+    T tan2 = ((T)1 - c2) / c2;
+    return (T)2 / ((T)1 + xsqrt((T)1 + g2 * tan2));
+}
+
+/* BeckmannMicrofacetDistribution, beckmann-models-impl.hh:57-70 */
+template <class T> inline T beckmann_d(T g2, T cos_nh)
+{
+    if (cos_nh <= (T)0) return (T)0;
+    T c2 = cos_nh * cos_nh;
+    T c4 = c2 * c2;
+    T tan2 = ((T)1 - c2) / c2;
+    return xexp(-tan2 / g2) / (C<T>::pi * g2 * c4);
+}
+
+/* G1_Beckmann (rational approximation), beckmann-models-impl.hh:34-55 */
+template <class T> inline T beckmann_g1(T g2, T cos_vm, T cos_vn)
+{
+    T ratio = cos_vm / cos_vn;
+    if (ratio <= (T)0) return (T)0;
+    T c2 = cos_vn * cos_vn;
+    T tan2 = ((T)1 - c2) / c2;
+    T a2 = (T)1 / (g2 * tan2);
+    T a = xsqrt(a2);
+    return a < (T)1.6f ? ((T)3.535f * a + (T)2.181f * a2) / ((T)1 + (T)2.276f * a + (T)2.577f * a2) : (T)1;
+}
+
+inline bool is_beckmann(uint32_t m) { return m >= TR_MODEL_BECKMANN_MICROFACET && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC; }
+inline bool is_ggx(uint32_t m) { return m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC; }
+inline bool is_microfacet(uint32_t m) { return is_ggx(m) || is_beckmann(m); }
+
+/* GGXSpecularBRDF / BeckmannSpecularBRDF (ggx-models-impl.hh:156-166,
+ * beckmann-models-impl.hh:121-131) */
+template <class T> inline T specular_brdf(const Sample<T>& s, const Angles<T>& a, bool beckmann)
+{
+    T g2 = geom_factor_sq((T)s.mat->specular_power[0]);
+    T shadow, d;
+    if (beckmann) {
+        shadow = beckmann_g1(g2, a.cos_im, a.cos_in) * beckmann_g1(g2, a.cos_im, a.cos_on);
+        d = beckmann_d(g2, a.cos_mn);
+    } else {
+        shadow = ggx_g1(g2, a.cos_im, a.cos_in) * ggx_g1(g2, a.cos_im, a.cos_on);
+        d = ggx_d(g2, a.cos_mn);
+    }
+    return shadow * d / ((T)4 * xabs(a.cos_in) * xabs(a.cos_on));
+}
+
+/* GGXMicrofacet{,Conductor,Dielectric}BRDF (ggx-models-impl.hh:168-209) and
+ * the Beckmann twins (beckmann-models-impl.hh:133-172): BRDF times cos. */
+template <class T> inline V3<T> microfacet_brdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    uint32_t model = m->model;
+    Angles<T> a = microfacet_angles(s);
+    T fresnel;
+    uint32_t variant = model - (is_beckmann(model) ? TR_MODEL_BECKMANN_MICROFACET : TR_MODEL_GGX_MICROFACET);
+    if (variant == 0) {
+        fresnel = fresnel_schlick((T)m->fresnel[0], a.cos_im);
+    } else if (variant == 1) {
+        fresnel = fresnel_conductor((T)m->fresnel[0], (T)m->fresnel[1], a.cos_im);
+    } else {
+        T cos_trans = cos_transmittance((T)m->fresnel[0], a.cos_im);
+        fresnel = fresnel_dielectric((T)m->fresnel[0], a.cos_im, cos_trans);
+    }
+    T brdf = fresnel * specular_brdf(s, a, is_beckmann(model));
+    return (spec3<T>(m->diffuse) * ((T)1 / C<T>::pi) + spec3<T>(m->specular) * brdf) * a.cos_in;
+}
+
+/* GGXMicrofacetPDF / BeckmannMicrofacetPDF (ggx-models-impl.hh:211-237,
+ * beckmann-models-impl.hh:240-266) */
+template <class T> inline T microfacet_pdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    T cos_ln = dot(s.wi, s.normal);
+    if (cos_ln < (T)0) return (T)0;
+    V3<T> h = normalize(s.wi + s.wo);
+    T cos_hn = dot(h, s.normal);
+    T g2 = geom_factor_sq((T)m->specular_power[0]);
+    T norm_pdf = (is_beckmann(m->model) ? beckmann_d(g2, cos_hn) : ggx_d(g2, cos_hn)) * cos_hn;
+    if (cos_hn <= (T)0 || norm_pdf <= (T)0) return (T)0;
+    T jacobian = (T)1 / ((T)4 * dot(h, s.wo));
+    T pdf0 = norm_pdf * jacobian;
+    T mv = max_value(spec3<T>(m->diffuse) + spec3<T>(m->specular));
+    T ratio = mv != (T)0 ? max_value(spec3<T>(m->specular)) / mv : (T)0;
+    T pdf1 = cos_ln * ((T)1 / C<T>::pi);
+    return (pdf0 - pdf1) * ratio + pdf1;
+}
+
+/* GGXMicrofacetSampleIncidentLight / BeckmannMicrofacetSampleIncidentLight
+ * (ggx-models-impl.hh:239-282, beckmann-models-impl.hh:268-311).
+ * xi[0], xi[1]: the stratified coordinates; xi[2]: the lobe selector. */
+template <class T> inline void microfacet_sample(const Strata<T>& st, Sample<T>* s, const T xi[3])
+{
+    const tr_material* m = s->mat;
+    T x_ray = (st.x + xi[0]) / st.tx;
+    T y_ray = (st.y + xi[1]) / st.ty;
+    T mv = max_value(spec3<T>(m->diffuse) + spec3<T>(m->specular));
+    T ratio = mv != (T)0 ? max_value(spec3<T>(m->specular)) / mv : (T)0;
+    M3<T> basis = make_basis(s->normal);
+    if (xi[2] <= ratio) {
+        T g2 = geom_factor_sq((T)m->specular_power[0]);
+        T g = xsqrt(g2);
+        T sin_phi, cos_phi;
+        sincos_turns(y_ray, &sin_phi, &cos_phi);
+        T r = is_beckmann(m->model) ? -xlog(x_ray) : x_ray / ((T)1 - x_ray);
+        T sqrt_ratio = xsqrt(r);
+        T rcp_len = (T)1 / xsqrt(r * g2 + (T)1);
+        T common = sqrt_ratio * g * rcp_len;
+        V3<T> mn = rcp_len > (T)1e-6f ? v3(-common * cos_phi, -common * sin_phi, rcp_len) : v3((T)0, (T)0, (T)1);
+        mn = transform(basis, mn);
+        s->wi = reflect(s->wo, mn);
+    } else {
+        /* CosineSampleHemisphere, sampling3.hh:69-76 */
+        T sin_phi, cos_phi;
+        sincos_turns(y_ray, &sin_phi, &cos_phi);
+        T r = xsqrt(x_ray);
+        V3<T> d = v3(cos_phi * r, sin_phi * r, xsqrt(xmax((T)0, (T)1 - x_ray)));
+        s->wi = normalize(transform(basis, d));
+    }
+    s->pdf = microfacet_pdf(*s);
+}
+
+/* UniformHemispherePDF, illumination-models.hh:723-727 */
+template <class T> inline T uniform_hemisphere_pdf(const Sample<T>& s)
+{
+    return dot(s.normal, s.wi) >= (T)0 ? (T)1 / ((T)2 * C<T>::pi) : (T)0;
+}
+
+/* UniformSampleIncidentLight, illumination-models.hh:729-738, with
+ * UniformSampleHemisphere, sampling3.hh:55-62 */
+template <class T> inline void uniform_hemisphere_sample(const Strata<T>& st, Sample<T>* s, const T xi[3])
+{
+    M3<T> basis = make_basis(s->normal);
+    T ra = (st.x + xi[0]) / st.tx;
+    T rb = (st.y + xi[1]) / st.ty;
+    T r = xsqrt((T)1 - ra * ra);
+    T sin_phi, cos_phi;
+    sincos_turns(rb, &sin_phi, &cos_phi);
+    V3<T> d = v3(cos_phi * r, sin_phi * r, ra);
+    s->pdf = (T)1 / ((T)2 * C<T>::pi);
+    s->wi = normalize(transform(basis, d));
+}
+
+/* SGGXProjectedArea, ggx-models-impl.hh:99-102 */
+template <class T> inline T sggx_projected_area(V3<T> stddev, const M3<T>& basis, V3<T> dir)
+{
+    return length(transform_rotation_inverse(basis, dir) * stddev);
+}
+
+/* SGGXMicroFlakeNDF(stddev, basis, normal), ggx-models-impl.hh:48-57 */
+template <class T> inline T sggx_ndf(V3<T> stddev, const M3<T>& basis, V3<T> n)
+{
+    V3<T> sq = stddev * stddev;
+    M3<T> inv_cov = transform_covariance(basis, v3((T)1 / sq.x, (T)1 / sq.y, (T)1 / sq.z));
+    T det = determinant(inv_cov);
+    if (det < (T)1e-6f) return (T)0;
+    T den = dot(n, transform(inv_cov, n));
+    return (T)0.5 * xsqrt(det) / (C<T>::pi * den * den);
+}
+
+template <class T> inline V3<T> sggx_surface_stddev(const tr_material* m)
+{
+    return v3((T)m->specular_power[0], (T)m->specular_power[1], (T)1);
+}
+
+/* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619 */
+template <class T> inline T sggx_surface_density(const Sample<T>& s)
+{
+    T q[4] = {(T)s.mat->sggx_basis[0], (T)s.mat->sggx_basis[1], (T)s.mat->sggx_basis[2], (T)s.mat->sggx_basis[3]};
+    M3<T> orient = m3(s.tangent, s.binormal, s.normal);
+    M3<T> basis = mul(orient, quat_to_m3(q));
+    return sggx_projected_area(sggx_surface_stddev<T>(s.mat), basis, s.wi);
+}
+
+/* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551 (no texture maps). */
+template <class T> inline V3<T> sggx_surface_brdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    T din = dot(s.normal, s.wi);
+    T don = dot(s.normal, s.wo);
+    if (din < (T)0 || don < (T)0) return v3((T)0, (T)0, (T)0);
+    V3<T> result = spec3<T>(m->diffuse) * din * ((T)1 / C<T>::pi);
+    V3<T> stddev = sggx_surface_stddev<T>(m);
+    if (stddev.x >= (T)1e-6f && stddev.y >= (T)1e-6f && stddev.z >= (T)1e-6f && s.dir_density >= (T)1e-6f) {
+        V3<T> mn = normalize(s.wi + s.wo);
+        M3<T> orient = m3(s.tangent, s.binormal, s.normal);
+        T q[4] = {(T)m->sggx_basis[0], (T)m->sggx_basis[1], (T)m->sggx_basis[2], (T)m->sggx_basis[3]};
+        T qn[4];
+        quat_normalize(q, qn);
+        M3<T> basis = mul(orient, quat_to_m3(qn));
+        T cos_micro = dot(mn, s.wi);
+        T fresnel;
+        T eta = (T)m->fresnel[0];
+        if (eta == (T)1) {
+            fresnel = (T)1;
+        } else {
+            T cos_trans = cos_transmittance(eta, cos_micro);
+            fresnel = fresnel_dielectric(eta, cos_micro, cos_trans);
+            fresnel = fresnel < (T)0 ? (T)0 : (fresnel > (T)1 ? (T)1 : fresnel);
+        }
+        T out_density = s.dir_density;
+        T in_density = sggx_projected_area(stddev, basis, s.wi);
+        T contrib = (T)1 / (out_density + in_density * don / din);
+        result += contrib * spec3<T>(m->specular) * sggx_ndf(stddev, basis, mn) * fresnel * (T)2 * (T)0.25;
+    }
+    return result;
+}
+
+inline bool is_emissive(uint32_t m) { return m == TR_MODEL_EMISSIVE; }
+
+/* TransmittanceLookup (illumination-models.hh:279-324) for the models the
+ * tracer supports. */
+template <class T> inline V3<T> bsdf_eval(const Sample<T>& s)
+{
+    uint32_t m = s.mat->model;
+    if (is_microfacet(m)) return microfacet_brdf(s);
+    if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
+    return v3((T)0, (T)0, (T)0);
+}
+
+/* PDFLookup (illumination-models.hh:233-277) */
+template <class T> inline T bsdf_pdf(const Sample<T>& s)
+{
+    uint32_t m = s.mat->model;
+    if (is_microfacet(m)) return microfacet_pdf(s);
+    if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
+    return (T)0;
+}
+
+/* SampleIncidentLightLookup (illumination-models.hh:325-370) */
+template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, const T xi[3])
+{
+    uint32_t m = s->mat->model;
+    if (is_microfacet(m)) microfacet_sample(st, s, xi);
+    else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi);
+    else s->pdf = (T)0;
+}
+
+/* MaterialCacheLookup (illumination-models.hh:455-492) */
+template <class T> inline void material_cache(Sample<T>* s)
+{
+    if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s);
+}
+
+}  // namespace orc
+
+#endif

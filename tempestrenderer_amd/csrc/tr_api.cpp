@@ -1,0 +1,1004 @@
+/*
+ * tr_api.cpp - the C ABI of include/tempest_rt.h: scene construction on
+ * the host, packing into HBM, and orchestration of the wavefront kernels.
+ *
+ * Mirrors RayTracerScene (include/tempest/graphics/ray-tracing/
+ * ray-tracing.hh:241-395, src/graphics/ray-tracing/ray-tracing.cc) and
+ * RayTracerCuda (include/tempest/compute/ray-tracing-cuda.hh:80-172):
+ * the same setters and defaults (SamplesGlobalIllumination 64,
+ * SamplesLocalAreaLight 4, SamplesCamera 1, MaxRayDepth 1, RussianRoulette
+ * 1, ray-tracing.hh:255-261), the same geometry-id scheme for meshes and
+ * the same emissive-submesh -> MeshLight registration.
+ */
+#include "../../include/tempest_rt.h"
+#include "../../include/tempest_rt_debug.h"
+#include "tr_bvh8.h"
+#include "tr_kernels.h"
+#include "tr_traverse.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace trk;
+
+namespace {
+
+thread_local std::string g_error;
+
+tr_status fail(tr_status st, const std::string& msg)
+{
+    g_error = msg;
+    return st;
+}
+
+#define TR_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(TR_ERROR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    hipError_t reserve(size_t n)
+    {
+        if (n <= count && ptr) return hipSuccess;
+        release();
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc((void**)&ptr, bytes);
+        if (e == hipSuccess) count = n;
+        return e;
+    }
+};
+
+enum { EV_TRACE = 0, EV_SHADE = 1, EV_SHADOW = 2, EV_OTHER = 3 };
+
+}  // namespace
+
+struct tr_scene {
+    uint32_t width = 0, height = 0;
+    float vpi[16];
+    tr_settings settings;
+    uint32_t spp = 1, n_gi = 64, n_light = 4, max_depth = 1, chunk = 16;
+    float rr = 1.0f;
+    float background[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t post = TR_POST_SRGB;
+    int32_t transparent = 0;
+    uint32_t render_mode = 0;
+    uint64_t seed = 1;
+    uint32_t estimator = TR_ESTIMATOR_REFERENCE;
+    uint32_t frame_index = 0;
+    bool counting = false;
+
+    /* host copy of the scene */
+    std::vector<tr_material> mats;
+    std::vector<const tr_material*> mat_keys;
+    std::vector<uint8_t> vpool;
+    std::vector<int32_t> ipool;
+    std::vector<GeomRec> geoms;
+    std::vector<PrimRec> prims_in;
+    std::vector<LightRec> lights;
+    uint32_t instances = 0;
+    bool committed = false;
+    bool lights_dirty = false;
+
+    /* BVH (host copy kept for the debug traversal) */
+    std::vector<Node8> nodes;
+    std::vector<PrimRec> prims;
+    Bvh8Stats bvh_stats;
+    double build_ms = 0.0;
+
+    /* device */
+    bool device_ready = false;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    DevBuf<Node8> d_nodes;
+    DevBuf<PrimRec> d_prims;
+    DevBuf<GeomRec> d_geoms;
+    DevBuf<tr_material> d_mats;
+    DevBuf<LightRec> d_lights;
+    DevBuf<uint8_t> d_vpool;
+    DevBuf<int32_t> d_ipool;
+
+    /* path pool */
+    uint32_t capacity = 0;
+    uint32_t max_shadow = 0;
+    DevBuf<float4> ray_o, ray_d, hit, thr, local, global, sh_o, sh_d, sh_c;
+    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, sh_n, counters;
+    DevBuf<uint2> spill;
+    DevBuf<uint64_t> counts;
+
+    /* frame accumulation */
+    DevBuf<double> accum;
+    DevBuf<uint32_t> rgba;
+    std::vector<uint32_t> rgba_host;
+    uint64_t accum_samples = 0;
+    bool frame_pending = false;
+
+    /* events */
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    tr_stats stats;
+
+    ~tr_scene();
+};
+
+namespace {
+
+void release_events(tr_scene* s)
+{
+    for (auto& e : s->ev_pool) (void)hipEventDestroy(e);
+    s->ev_pool.clear();
+    s->ev_used.clear();
+}
+
+hipEvent_t get_event(tr_scene* s)
+{
+    if (!s->ev_pool.empty()) {
+        hipEvent_t e = s->ev_pool.back();
+        s->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+}  // namespace
+
+tr_scene::~tr_scene()
+{
+    if (frame_pending && stream) (void)hipStreamSynchronize(stream);
+    for (auto& u : ev_used) { ev_pool.push_back(u.second.first); ev_pool.push_back(u.second.second); }
+    ev_used.clear();
+    release_events(this);
+    d_nodes.release(); d_prims.release(); d_geoms.release(); d_mats.release(); d_lights.release();
+    d_vpool.release(); d_ipool.release();
+    ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
+    sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
+    shq.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+uint32_t intern_material(tr_scene* s, const tr_material* m)
+{
+    for (size_t i = 0; i < s->mat_keys.size(); ++i)
+        if (s->mat_keys[i] == m && memcmp(&s->mats[i], m, sizeof(*m)) == 0) return (uint32_t)i;
+    s->mat_keys.push_back(m);
+    s->mats.push_back(*m);
+    return (uint32_t)(s->mats.size() - 1);
+}
+
+bool model_supported(uint32_t m)
+{
+    return m == TR_MODEL_EMISSIVE || (m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC) ||
+           (m >= TR_MODEL_BECKMANN_MICROFACET && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC) ||
+           m == TR_MODEL_SGGX_SURFACE;
+}
+
+tr_status ensure_device(tr_scene* s)
+{
+    if (s->device_ready) return TR_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(TR_ERROR_DEVICE, "no HIP device available");
+    int dev = 0;
+    TR_HIP(hipGetDevice(&dev));
+    TR_HIP(hipDeviceGetAttribute(&s->cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+    TR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->device_ready = true;
+    return TR_OK;
+}
+
+template <class T> tr_status upload(tr_scene* s, DevBuf<T>& buf, const std::vector<T>& v)
+{
+    TR_HIP(buf.reserve(v.size()));
+    if (!v.empty()) TR_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s->stream));
+    return TR_OK;
+}
+
+SceneView scene_view(const tr_scene* s)
+{
+    SceneView v;
+    v.nodes = s->d_nodes.ptr;
+    v.prims = s->d_prims.ptr;
+    v.geoms = s->d_geoms.ptr;
+    v.mats = s->d_mats.ptr;
+    v.lights = s->d_lights.ptr;
+    v.vpool = s->d_vpool.ptr;
+    v.ipool = s->d_ipool.ptr;
+    v.n_lights = (uint32_t)s->lights.size();
+    v.n_prims = (uint32_t)s->prims.size();
+    return v;
+}
+
+uint32_t persistent_grid(const tr_scene* s) { return (uint32_t)std::max(1, s->cu_count) * 16u; }
+
+tr_status sync_lights(tr_scene* s)
+{
+    if (!s->lights_dirty) return TR_OK;
+    tr_status st = upload(s, s->d_lights, s->lights);
+    if (st != TR_OK) return st;
+    s->lights_dirty = false;
+    return TR_OK;
+}
+
+uint32_t shadow_entries(const tr_scene* s)
+{
+    uint32_t n = 0;
+    for (const LightRec& L : s->lights) n += L.area ? std::max(1u, s->n_light) : 1u;
+    return n;
+}
+
+tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
+{
+    uint32_t want = s->settings.path_pool_size ? s->settings.path_pool_size : (1u << 22);
+    uint64_t per_pixel = (uint64_t)s->spp * s->n_gi;
+    if (per_pixel > want) want = (uint32_t)per_pixel;
+    if (slots_needed < want) want = (uint32_t)std::max<uint64_t>(slots_needed, per_pixel);
+    want = (uint32_t)(want / per_pixel * per_pixel);
+    uint32_t ms = std::max(1u, shadow_entries(s));
+    if (want <= s->capacity && ms <= s->max_shadow) return TR_OK;
+    uint32_t cap = std::max(want, s->capacity);
+    uint32_t msh = std::max(ms, s->max_shadow);
+    TR_HIP(s->ray_o.reserve(cap));
+    TR_HIP(s->ray_d.reserve(cap));
+    TR_HIP(s->hit.reserve(cap));
+    TR_HIP(s->thr.reserve(cap));
+    TR_HIP(s->local.reserve(cap));
+    TR_HIP(s->global.reserve(cap));
+    TR_HIP(s->hit_prim.reserve(cap));
+    TR_HIP(s->hash.reserve(cap));
+    TR_HIP(s->q0.reserve(cap));
+    TR_HIP(s->q1.reserve(cap));
+    TR_HIP(s->shq.reserve(cap));
+    TR_HIP(s->sh_n.reserve(cap));
+    TR_HIP(s->sh_o.reserve((size_t)cap * msh));
+    TR_HIP(s->sh_d.reserve((size_t)cap * msh));
+    TR_HIP(s->sh_c.reserve((size_t)cap * msh));
+    TR_HIP(s->counters.reserve(8));
+    TR_HIP(s->spill.reserve((size_t)persistent_grid(s) * kTraceBlock * kStackSpill));
+    TR_HIP(s->counts.reserve(2));
+    s->capacity = cap;
+    s->max_shadow = msh;
+    return TR_OK;
+}
+
+PathPool path_pool(const tr_scene* s)
+{
+    PathPool p;
+    p.ray_o = s->ray_o.ptr; p.ray_d = s->ray_d.ptr; p.hit = s->hit.ptr; p.hit_prim = s->hit_prim.ptr;
+    p.thr = s->thr.ptr; p.local = s->local.ptr; p.global = s->global.ptr; p.hash = s->hash.ptr;
+    p.queue[0] = s->q0.ptr; p.queue[1] = s->q1.ptr; p.shadow_queue = s->shq.ptr;
+    p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr;
+    p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity;
+    return p;
+}
+
+FrameParams frame_params(const tr_scene* s)
+{
+    FrameParams p;
+    p.sc = scene_view(s);
+    p.width = s->width;
+    p.height = s->height;
+    p.spp = s->spp;
+    p.n_gi = s->n_gi;
+    p.n_light = s->n_light;
+    p.max_depth = s->max_depth;
+    p.rays_sq = (uint32_t)sqrtf((float)s->n_gi);
+    if (p.rays_sq == 0) p.rays_sq = 1;
+    p.max_shadow = s->max_shadow;
+    p.rr = s->rr;
+    memcpy(p.background, s->background, sizeof(p.background));
+    p.key0 = (uint32_t)s->seed;
+    p.key1 = (uint32_t)(s->seed >> 32);
+    p.frame = s->frame_index;
+    memcpy(p.vpi, s->vpi, sizeof(p.vpi));
+    return p;
+}
+
+void record(tr_scene* s, int cat, hipEvent_t a, hipEvent_t b) { s->ev_used.push_back({cat, {a, b}}); }
+
+/* Enqueues every batch of a frame (or of a pixel list) on the scene stream. */
+tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
+{
+    if (!s->committed) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    if (s->spp == 0 || s->n_gi == 0) return fail(TR_ERROR_INVALID_ARG, "SamplesCamera and SamplesGlobalIllumination must be positive");
+    uint32_t rsq = (uint32_t)sqrtf((float)s->n_gi);
+    if (rsq * rsq != s->n_gi) return fail(TR_ERROR_INVALID_ARG, "SamplesGlobalIllumination must be a perfect square");
+    if (s->bvh_stats.depth + 2 > (uint32_t)(kStackLds + kStackSpill))
+        return fail(TR_ERROR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    tr_status st = sync_lights(s);
+    if (st != TR_OK) return st;
+    uint64_t per_pixel = (uint64_t)s->spp * s->n_gi;
+    st = ensure_pool(s, n_pixels * per_pixel);
+    if (st != TR_OK) return st;
+    FrameParams p = frame_params(s);
+    PathPool pool = path_pool(s);
+    uint32_t ppb = (uint32_t)std::max<uint64_t>(1, s->capacity / per_pixel);
+    uint32_t grid = persistent_grid(s);
+    uint64_t* counts = nullptr;
+    if (s->counting) {
+        counts = s->counts.ptr;
+        TR_HIP(hipMemsetAsync(counts, 0, 2 * sizeof(uint64_t), s->stream));
+    }
+    for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += ppb) {
+        uint32_t npx = (uint32_t)std::min<uint64_t>(ppb, n_pixels - pix0);
+        uint32_t nslots = (uint32_t)(npx * per_pixel);
+        uint64_t slot_base = pix0 * per_pixel;
+        hipEvent_t a = get_event(s), b = get_event(s);
+        TR_HIP(hipEventRecord(a, s->stream));
+        launch_raygen(s->stream, p, pool, out, slot_base, nslots);
+        TR_HIP(hipGetLastError());
+        TR_HIP(hipEventRecord(b, s->stream));
+        record(s, EV_OTHER, a, b);
+        int cur = 0, nxt = 1;
+        for (uint32_t v = 0; v <= s->max_depth; ++v) {
+            hipEvent_t e0 = get_event(s), e1 = get_event(s), e2 = get_event(s), e3 = get_event(s);
+            TR_HIP(hipEventRecord(e0, s->stream));
+            launch_trace(s->stream, p.sc, pool, cur, nxt, grid, counts);
+            TR_HIP(hipGetLastError());
+            TR_HIP(hipEventRecord(e1, s->stream));
+            launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots);
+            TR_HIP(hipGetLastError());
+            TR_HIP(hipEventRecord(e2, s->stream));
+            launch_shadow(s->stream, p, pool, v, grid, counts);
+            TR_HIP(hipGetLastError());
+            TR_HIP(hipEventRecord(e3, s->stream));
+            record(s, EV_TRACE, e0, e1);
+            record(s, EV_SHADE, e1, e2);
+            record(s, EV_SHADOW, e2, e3);
+            std::swap(cur, nxt);
+        }
+        hipEvent_t c = get_event(s), d = get_event(s);
+        TR_HIP(hipEventRecord(c, s->stream));
+        launch_accumulate(s->stream, p, pool, out, slot_base, npx);
+        TR_HIP(hipGetLastError());
+        TR_HIP(hipEventRecord(d, s->stream));
+        record(s, EV_OTHER, c, d);
+    }
+    return TR_OK;
+}
+
+/* Collects event timings after the stream has been synchronised. */
+void collect_stats(tr_scene* s, uint64_t paths)
+{
+    double ms[4] = {0, 0, 0, 0};
+    uint64_t traces = 0;
+    for (auto& u : s->ev_used) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, u.second.first, u.second.second) == hipSuccess) ms[u.first] += t;
+        if (u.first == EV_TRACE) ++traces;
+        s->ev_pool.push_back(u.second.first);
+        s->ev_pool.push_back(u.second.second);
+    }
+    s->ev_used.clear();
+    s->stats.paths = paths;
+    s->stats.trace_ms = ms[EV_TRACE];
+    s->stats.shade_ms = ms[EV_SHADE];
+    s->stats.shadow_ms = ms[EV_SHADOW];
+    s->stats.other_ms = ms[EV_OTHER];
+    s->stats.trace_launches = traces;
+    s->stats.node_visits = s->stats.prim_tests = 0;
+    if (s->counting) {
+        uint64_t c[2] = {0, 0};
+        if (hipMemcpy(c, s->counts.ptr, sizeof(c), hipMemcpyDeviceToHost) == hipSuccess) {
+            s->stats.node_visits = c[0];
+            s->stats.prim_tests = c[1];
+        }
+    }
+}
+
+tr_status finish_frame(tr_scene* s, std::chrono::steady_clock::time_point t0, uint64_t paths)
+{
+    TR_HIP(hipStreamSynchronize(s->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    collect_stats(s, paths);
+    s->stats.frame_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    s->frame_pending = false;
+    return TR_OK;
+}
+
+tr_status start_frame(tr_scene* s)
+{
+    if (!s->committed) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    uint64_t npx = (uint64_t)s->width * s->height;
+    TR_HIP(s->accum.reserve(npx * 3));
+    TR_HIP(s->rgba.reserve(npx));
+    if (s->accum_samples == 0) TR_HIP(hipMemsetAsync(s->accum.ptr, 0, npx * 3 * sizeof(double), s->stream));
+    BatchOut out;
+    memset(&out, 0, sizeof(out));
+    out.accum = s->accum.ptr;
+    tr_status st = enqueue_frame(s, npx, out);
+    if (st != TR_OK) return st;
+    s->accum_samples += s->spp;
+    s->frame_index += 1;
+    launch_resolve(s->stream, s->accum.ptr, 1.0 / (double)s->accum_samples, s->rgba.ptr, (uint32_t)npx, s->post);
+    TR_HIP(hipGetLastError());
+    s->frame_pending = true;
+    return TR_OK;
+}
+
+bool valid(const tr_scene* s) { return s != nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+const char* tr_last_error_message(void) { return g_error.c_str(); }
+
+tr_status tr_material_init(tr_material* m, uint32_t model)
+{
+    if (!m) return fail(TR_ERROR_INVALID_ARG, "null material");
+    memset(m, 0, sizeof(*m));
+    m->model = model;
+    m->fresnel[0] = 1.0f;
+    m->fresnel[1] = 0.0f;
+    m->sggx_depth = 2;
+    m->sggx_sample_count = 256;
+    m->sggx_basis[3] = 1.0f;
+    return TR_OK;
+}
+
+/* RTMaterial::setup -> DefaultMicrofacetMaterialSetup (illumination-models.hh:599-614)
+ * for the models whose setup entry is that function (:373-409). */
+tr_status tr_material_setup(tr_material* m)
+{
+    if (!m) return fail(TR_ERROR_INVALID_ARG, "null material");
+    uint32_t model = m->model;
+    bool default_setup = model == TR_MODEL_BLINN_PHONG || model == TR_MODEL_ASHIKHMIN_SHIRLEY ||
+                         (model >= TR_MODEL_GGX_MICROFACET && model <= TR_MODEL_GGX_MICROFACET_DIELECTRIC) ||
+                         (model >= TR_MODEL_GGX_MICROFACET_ANISOTROPIC &&
+                          model <= TR_MODEL_GGX_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
+                         model == TR_MODEL_SGGX_SURFACE || model == TR_MODEL_SGGX_PSEUDO_VOLUME ||
+                         (model >= TR_MODEL_BECKMANN_MICROFACET &&
+                          model <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC_ANISOTROPIC);
+    if (!default_setup) return TR_OK;
+    /* SampleIncidentLightLookup is non-null for all of these models */
+    float sum[3] = {m->diffuse[0] + m->specular[0], m->diffuse[1] + m->specular[1], m->diffuse[2] + m->specular[2]};
+    float max_value = std::max(std::max(sum[0], sum[1]), sum[2]);
+    if (max_value > 1.0f) {
+        float r = 1.0f / max_value;
+        for (int i = 0; i < 3; ++i) { m->diffuse[i] *= r; m->specular[i] *= r; }
+    }
+    m->normalization = 1.0f;
+    return TR_OK;
+}
+
+tr_status tr_scene_create(uint32_t width, uint32_t height, const float vpi[16], const tr_settings* settings,
+                          tr_scene** out)
+{
+    if (!out || !vpi || width == 0 || height == 0) return fail(TR_ERROR_INVALID_ARG, "invalid scene arguments");
+    tr_scene* s = new (std::nothrow) tr_scene();
+    if (!s) return fail(TR_ERROR_OUT_OF_MEMORY, "scene allocation failed");
+    s->width = width;
+    s->height = height;
+    memcpy(s->vpi, vpi, sizeof(s->vpi));
+    if (settings) s->settings = *settings;
+    else { s->settings.data_pool_size = 32u << 20; s->settings.path_pool_size = 0; }
+    memset(&s->stats, 0, sizeof(s->stats));
+    *out = s;
+    return TR_OK;
+}
+
+void tr_scene_destroy(tr_scene* s) { delete s; }
+
+#define SETTER(name, field, type)                                               \
+    tr_status name(tr_scene* s, type v)                                         \
+    {                                                                           \
+        if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");         \
+        s->field = v;                                                           \
+        return TR_OK;                                                           \
+    }
+SETTER(tr_scene_set_chunk_size, chunk, uint32_t)
+SETTER(tr_scene_set_samples_camera, spp, uint32_t)
+SETTER(tr_scene_set_max_ray_depth, max_depth, uint32_t)
+SETTER(tr_scene_set_russian_roulette, rr, float)
+SETTER(tr_scene_set_picture_post_process, post, uint32_t)
+SETTER(tr_scene_set_transparent_background, transparent, int32_t)
+SETTER(tr_scene_set_render_mode, render_mode, uint32_t)
+SETTER(tr_scene_set_seed, seed, uint64_t)
+SETTER(tr_scene_set_frame_index, frame_index, uint32_t)
+#undef SETTER
+
+tr_status tr_scene_set_samples_global_illumination(tr_scene* s, uint32_t v)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    s->n_gi = v;
+    return TR_OK;
+}
+
+tr_status tr_scene_set_samples_local_area_light(tr_scene* s, uint32_t v)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    s->n_light = v;
+    return TR_OK;
+}
+
+tr_status tr_scene_set_estimator(tr_scene* s, uint32_t mode)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (mode != TR_ESTIMATOR_REFERENCE) return fail(TR_ERROR_UNSUPPORTED, "only the reference estimator is implemented");
+    s->estimator = mode;
+    return TR_OK;
+}
+
+tr_status tr_scene_set_counting(tr_scene* s, int32_t enabled)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    s->counting = enabled != 0;
+    return TR_OK;
+}
+
+tr_status tr_scene_set_background_spectrum(tr_scene* s, const float rgb[3])
+{
+    if (!valid(s) || !rgb) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    memcpy(s->background, rgb, sizeof(s->background));
+    return TR_OK;
+}
+
+tr_status tr_scene_repaint(tr_scene* s)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    s->accum_samples = 0;
+    return TR_OK;
+}
+
+tr_status tr_scene_add_light_source(tr_scene* s, const tr_light* light, uint32_t* out_index)
+{
+    if (!valid(s) || !light) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (light->type != TR_LIGHT_DIRECTIONAL && light->type != TR_LIGHT_POINT)
+        return fail(TR_ERROR_UNSUPPORTED, "addLightSource takes directional and point lights; use addSphereLightSource");
+    LightRec L;
+    memset(&L, 0, sizeof(L));
+    L.src = *light;
+    L.geom = TR_INVALID_GEOMETRY;
+    L.area = 0;
+    if (out_index) *out_index = (uint32_t)s->lights.size();
+    s->lights.push_back(L);
+    s->lights_dirty = true;
+    s->accum_samples = 0;
+    return TR_OK;
+}
+
+tr_status tr_scene_update_light_source(tr_scene* s, uint32_t index, const tr_light* light)
+{
+    if (!valid(s) || !light) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (index >= s->lights.size()) return fail(TR_ERROR_INVALID_ARG, "light index out of range");
+    LightRec& L = s->lights[index];
+    if (L.src.type != light->type) return fail(TR_ERROR_INVALID_ARG, "updateLightSource cannot change the light type");
+    uint32_t gid = L.src.geometry_id;
+    L.src = *light;
+    L.src.geometry_id = gid;
+    s->lights_dirty = true;
+    s->accum_samples = 0; /* repaint(), ray-tracing.hh:327 */
+    return TR_OK;
+}
+
+tr_status tr_scene_clear_light_sources(tr_scene* s)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    s->lights.clear();
+    s->lights_dirty = true;
+    s->accum_samples = 0;
+    return TR_OK;
+}
+
+static uint32_t add_sphere_prim(tr_scene* s, const float c[3], float r, uint32_t type, uint32_t material)
+{
+    GeomRec g;
+    memset(&g, 0, sizeof(g));
+    g.type = type;
+    g.material = material;
+    g.sphere[0] = c[0]; g.sphere[1] = c[1]; g.sphere[2] = c[2]; g.sphere[3] = r;
+    uint32_t slot = (uint32_t)s->geoms.size();
+    s->geoms.push_back(g);
+    PrimRec p;
+    memset(&p, 0, sizeof(p));
+    p.v0[0] = c[0]; p.v0[1] = c[1]; p.v0[2] = c[2];
+    p.v1[0] = r;
+    p.prim = 0;
+    p.geom = slot;
+    p.flags = PRIM_SPHERE;
+    s->prims_in.push_back(p);
+    return slot;
+}
+
+tr_status tr_scene_add_sphere_light_source(tr_scene* s, const tr_light* light, uint32_t* out_geom_id)
+{
+    if (!valid(s) || !light) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "scene already committed");
+    if (!(light->sphere_radius > 0.0f)) return fail(TR_ERROR_INVALID_ARG, "sphere light radius must be positive");
+    tr_material m;
+    tr_material_init(&m, TR_MODEL_EMISSIVE);
+    memcpy(m.diffuse, light->radiance, sizeof(m.diffuse));
+    s->mat_keys.push_back(nullptr);
+    s->mats.push_back(m);
+    uint32_t slot = add_sphere_prim(s, light->sphere_center, light->sphere_radius, GEOM_SPHERE_LIGHT,
+                                    (uint32_t)s->mats.size() - 1);
+    LightRec L;
+    memset(&L, 0, sizeof(L));
+    L.src = *light;
+    L.src.type = TR_LIGHT_SPHERE_AREA;
+    L.src.geometry_id = slot;
+    L.geom = slot;
+    L.area = 1;
+    s->lights.push_back(L);
+    s->lights_dirty = true;
+    if (out_geom_id) *out_geom_id = slot;
+    return TR_OK;
+}
+
+tr_status tr_scene_add_triangle_mesh(tr_scene* s, const float world[16], size_t submesh_count, const tr_submesh* subs,
+                                     size_t index_count, const int32_t* tris, size_t vert_size, const void* verts,
+                                     const tr_mesh_options* opts, uint64_t* geom_ids)
+{
+    if (!valid(s) || !world || (submesh_count && (!subs || !tris || !verts)))
+        return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "scene already committed");
+    for (size_t si = 0; si < submesh_count; ++si) {
+        const tr_submesh& sm = subs[si];
+        if (!sm.material) return fail(TR_ERROR_INVALID_ARG, "submesh without material");
+        if (!model_supported(sm.material->model)) return fail(TR_ERROR_UNSUPPORTED, "illumination model not supported by the GPU tracer");
+        if (sm.stride < 24 || sm.stride % 4) return fail(TR_ERROR_INVALID_ARG, "vertex stride must be a multiple of 4, at least 24");
+        if (sm.vertex_count % 3) return fail(TR_ERROR_INVALID_ARG, "submesh index count must be a multiple of 3");
+        if ((size_t)sm.base_index + sm.vertex_count > index_count) return fail(TR_ERROR_INVALID_ARG, "submesh indices out of range");
+        if (sm.vertex_offset > vert_size) return fail(TR_ERROR_INVALID_ARG, "vertex offset out of range");
+        size_t nverts = (vert_size - sm.vertex_offset) / sm.stride;
+        for (uint32_t k = 0; k < sm.vertex_count; ++k) {
+            int32_t idx = tris[sm.base_index + k];
+            if (idx < 0 || (size_t)idx >= nverts) return fail(TR_ERROR_INVALID_ARG, "vertex index out of range");
+        }
+    }
+    uint64_t vbase = s->vpool.size();
+    s->vpool.insert(s->vpool.end(), (const uint8_t*)verts, (const uint8_t*)verts + vert_size);
+    while (s->vpool.size() % 16) s->vpool.push_back(0);
+    uint32_t inst = s->instances++;
+    for (size_t si = 0; si < submesh_count; ++si) {
+        const tr_submesh& sm = subs[si];
+        GeomRec g;
+        memset(&g, 0, sizeof(g));
+        g.type = GEOM_MESH;
+        g.material = intern_material(s, sm.material);
+        g.two_sided = opts && opts->two_sided ? 1u : 0u;
+        g.stride = sm.stride;
+        g.vbase = vbase + sm.vertex_offset;
+        g.index_base = (uint32_t)s->ipool.size();
+        g.tri_count = sm.vertex_count / 3;
+        s->ipool.insert(s->ipool.end(), tris + sm.base_index, tris + sm.base_index + sm.vertex_count);
+        uint32_t slot = (uint32_t)s->geoms.size();
+        const uint8_t* base = (const uint8_t*)verts + sm.vertex_offset;
+        for (uint32_t t = 0; t < g.tri_count; ++t) {
+            PrimRec p;
+            memset(&p, 0, sizeof(p));
+            float* dst[3] = {p.v0, p.v1, p.v2};
+            for (int k = 0; k < 3; ++k) {
+                const float* f = reinterpret_cast<const float*>(base + (size_t)tris[sm.base_index + t * 3 + k] * sm.stride);
+                for (int r = 0; r < 3; ++r)
+                    dst[k][r] = ((world[r] * f[0] + world[4 + r] * f[1]) + world[8 + r] * f[2]) + world[12 + r];
+            }
+            p.prim = t;
+            p.geom = slot;
+            p.flags = g.two_sided ? PRIM_TWO_SIDED : 0u;
+            s->prims_in.push_back(p);
+        }
+        s->geoms.push_back(g);
+        if (s->mats[g.material].model == TR_MODEL_EMISSIVE) {
+            LightRec L;
+            memset(&L, 0, sizeof(L));
+            L.src.type = TR_LIGHT_MESH_AREA;
+            L.src.geometry_id = slot;
+            L.geom = slot;
+            L.area = 0;
+            s->lights.push_back(L);
+            s->lights_dirty = true;
+        }
+        if (geom_ids) geom_ids[si] = ((uint64_t)inst << 32) | (uint64_t)si;
+    }
+    return TR_OK;
+}
+
+tr_status tr_scene_add_sphere(tr_scene* s, const float center[3], float radius, const tr_material* m, uint64_t* out)
+{
+    if (!valid(s) || !center || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "scene already committed");
+    if (!(radius > 0.0f)) return fail(TR_ERROR_INVALID_ARG, "sphere radius must be positive");
+    if (!model_supported(m->model)) return fail(TR_ERROR_UNSUPPORTED, "illumination model not supported by the GPU tracer");
+    uint32_t slot = add_sphere_prim(s, center, radius, GEOM_SPHERE, intern_material(s, m));
+    if (out) *out = slot;
+    return TR_OK;
+}
+
+tr_status tr_scene_commit(tr_scene* s)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (!s->committed) {
+        auto t0 = std::chrono::steady_clock::now();
+        std::string err;
+        if (!build_bvh8(s->prims_in, s->nodes, s->prims, &s->bvh_stats, &err)) return fail(TR_ERROR_INVALID_ARG, err);
+        auto t1 = std::chrono::steady_clock::now();
+        s->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        s->committed = true;
+        s->lights_dirty = true;
+    }
+    tr_status st = ensure_device(s);
+    if (st != TR_OK) return st;
+    if ((st = upload(s, s->d_nodes, s->nodes)) != TR_OK) return st;
+    if ((st = upload(s, s->d_prims, s->prims)) != TR_OK) return st;
+    if ((st = upload(s, s->d_geoms, s->geoms)) != TR_OK) return st;
+    if ((st = upload(s, s->d_mats, s->mats)) != TR_OK) return st;
+    if ((st = upload(s, s->d_vpool, s->vpool)) != TR_OK) return st;
+    if ((st = upload(s, s->d_ipool, s->ipool)) != TR_OK) return st;
+    s->lights_dirty = true;
+    if ((st = sync_lights(s)) != TR_OK) return st;
+    TR_HIP(hipStreamSynchronize(s->stream));
+    s->accum_samples = 0;
+    return TR_OK;
+}
+
+tr_status tr_scene_get_info(const tr_scene* s, tr_scene_info* info)
+{
+    if (!valid(s) || !info) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    memset(info, 0, sizeof(*info));
+    info->primitives = s->prims_in.size();
+    info->bvh_nodes = s->nodes.size();
+    info->bvh_bytes = s->nodes.size() * sizeof(Node8);
+    info->prim_bytes = s->prims.size() * sizeof(PrimRec);
+    info->bvh_depth = s->bvh_stats.depth;
+    info->geometries = (uint32_t)s->geoms.size();
+    info->lights = (uint32_t)s->lights.size();
+    info->materials = (uint32_t)s->mats.size();
+    info->build_ms = s->build_ms;
+    return TR_OK;
+}
+
+tr_status tr_scene_draw_once(tr_scene* s)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    tr_status st = start_frame(s);
+    if (st != TR_OK) return st;
+    return finish_frame(s, t0, (uint64_t)s->width * s->height * s->spp);
+}
+
+static std::chrono::steady_clock::time_point g_draw_start;
+
+tr_status tr_scene_draw(tr_scene* s, uint32_t width, uint32_t height, const float vpi[16])
+{
+    if (!valid(s) || !vpi) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    if (width != s->width || height != s->height || memcmp(vpi, s->vpi, sizeof(s->vpi)) != 0) {
+        s->width = width;
+        s->height = height;
+        memcpy(s->vpi, vpi, sizeof(s->vpi));
+        s->accum_samples = 0;
+    }
+    g_draw_start = std::chrono::steady_clock::now();
+    return start_frame(s);
+}
+
+tr_status tr_scene_frame_wait(tr_scene* s)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (!s->frame_pending) return TR_OK;
+    return finish_frame(s, g_draw_start, (uint64_t)s->width * s->height * s->spp);
+}
+
+tr_status tr_scene_read_backbuffer(tr_scene* s, uint32_t* rgba, size_t count)
+{
+    if (!valid(s) || !rgba) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    size_t npx = (size_t)s->width * s->height;
+    if (count < npx) return fail(TR_ERROR_INVALID_ARG, "buffer too small");
+    if (s->accum_samples == 0 || !s->rgba.ptr) return fail(TR_ERROR_NOT_COMMITTED, "no frame rendered");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    TR_HIP(hipMemcpy(rgba, s->rgba.ptr, npx * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return TR_OK;
+}
+
+tr_status tr_scene_read_accum(tr_scene* s, double* rgb_sum, size_t count, uint64_t* spp_out)
+{
+    if (!valid(s) || !rgb_sum) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    size_t npx = (size_t)s->width * s->height;
+    if (count < npx) return fail(TR_ERROR_INVALID_ARG, "buffer too small");
+    if (s->accum_samples == 0 || !s->accum.ptr) return fail(TR_ERROR_NOT_COMMITTED, "no frame rendered");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    TR_HIP(hipMemcpy(rgb_sum, s->accum.ptr, npx * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (spp_out) *spp_out = s->accum_samples;
+    return TR_OK;
+}
+
+tr_status tr_scene_trace_rays(tr_scene* s, size_t n, const tr_ray* rays, tr_hit* hits)
+{
+    if (!valid(s) || (n && (!rays || !hits))) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed || !s->device_ready) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    if (n == 0) return TR_OK;
+    if (n > 0xffffffffull - 64) return fail(TR_ERROR_INVALID_ARG, "too many rays");
+    DevBuf<tr_ray> d_rays;
+    DevBuf<tr_hit> d_hits;
+    DevBuf<uint32_t> d_work;
+    if (d_rays.reserve(n) != hipSuccess || d_hits.reserve(n) != hipSuccess || d_work.reserve(1) != hipSuccess) {
+        d_rays.release(); d_hits.release(); d_work.release();
+        return fail(TR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+    }
+    tr_status st = TR_OK;
+    if (s->spill.reserve((size_t)persistent_grid(s) * kTraceBlock * kStackSpill) != hipSuccess)
+        st = fail(TR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+    if (st == TR_OK && (hipMemcpyAsync(d_rays.ptr, rays, n * sizeof(tr_ray), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+                        hipMemsetAsync(d_work.ptr, 0, sizeof(uint32_t), s->stream) != hipSuccess))
+        st = fail(TR_ERROR_DEVICE, "ray upload failed");
+    if (st == TR_OK) {
+        launch_trace_rays(s->stream, scene_view(s), d_rays.ptr, d_hits.ptr, (uint32_t)n, s->spill.ptr, d_work.ptr,
+                          persistent_grid(s));
+        if (hipGetLastError() != hipSuccess) st = fail(TR_ERROR_DEVICE, "trace launch failed");
+    }
+    if (st == TR_OK && hipMemcpyAsync(hits, d_hits.ptr, n * sizeof(tr_hit), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+        st = fail(TR_ERROR_DEVICE, "hit download failed");
+    if (hipStreamSynchronize(s->stream) != hipSuccess && st == TR_OK) st = fail(TR_ERROR_DEVICE, "trace failed");
+    d_rays.release(); d_hits.release(); d_work.release();
+    return st;
+}
+
+tr_status tr_scene_render_paths(tr_scene* s, size_t pixel_count, const uint32_t* pixels, float* out_rgb,
+                                uint32_t* out_hash)
+{
+    if (!valid(s) || (pixel_count && (!pixels || !out_rgb))) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed || !s->device_ready) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    if (pixel_count == 0) return TR_OK;
+    uint64_t npx = (uint64_t)s->width * s->height;
+    for (size_t i = 0; i < pixel_count; ++i)
+        if (pixels[i] >= npx) return fail(TR_ERROR_INVALID_ARG, "pixel index out of range");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    size_t nsamples = pixel_count * s->spp;
+    DevBuf<uint32_t> d_pix, d_hash;
+    DevBuf<float> d_rgb;
+    if (d_pix.reserve(pixel_count) != hipSuccess || d_rgb.reserve(nsamples * 3) != hipSuccess ||
+        d_hash.reserve(nsamples) != hipSuccess) {
+        d_pix.release(); d_rgb.release(); d_hash.release();
+        return fail(TR_ERROR_OUT_OF_MEMORY, "device allocation failed");
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    tr_status st = TR_OK;
+    if (hipMemcpyAsync(d_pix.ptr, pixels, pixel_count * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        st = fail(TR_ERROR_DEVICE, "pixel upload failed");
+    BatchOut out;
+    memset(&out, 0, sizeof(out));
+    out.path_rgb = d_rgb.ptr;
+    out.path_hash = d_hash.ptr;
+    out.pixel_list = d_pix.ptr;
+    if (st == TR_OK) st = enqueue_frame(s, pixel_count, out);
+    if (st == TR_OK) {
+        if (hipMemcpyAsync(out_rgb, d_rgb.ptr, nsamples * 3 * sizeof(float), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            st = fail(TR_ERROR_DEVICE, "download failed");
+        if (out_hash && st == TR_OK &&
+            hipMemcpyAsync(out_hash, d_hash.ptr, nsamples * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            st = fail(TR_ERROR_DEVICE, "download failed");
+    }
+    if (hipStreamSynchronize(s->stream) != hipSuccess && st == TR_OK) st = fail(TR_ERROR_DEVICE, "render failed");
+    if (st == TR_OK) {
+        collect_stats(s, nsamples);
+        s->stats.frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    d_pix.release(); d_rgb.release(); d_hash.release();
+    return st;
+}
+
+tr_status tr_scene_get_stats(const tr_scene* s, tr_stats* out)
+{
+    if (!valid(s) || !out) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    *out = s->stats;
+    return TR_OK;
+}
+
+/* ---------------- debug exports (tempest_rt_debug.h) ---------------- */
+
+tr_status tr_debug_build_scene_host(tr_scene* s)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (s->committed) return TR_OK;
+    std::string err;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!build_bvh8(s->prims_in, s->nodes, s->prims, &s->bvh_stats, &err)) return fail(TR_ERROR_INVALID_ARG, err);
+    s->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->committed = true;
+    return TR_OK;
+}
+
+tr_status tr_debug_trace_rays_host(const tr_scene* s, size_t n, const tr_ray* rays, tr_hit* hits, uint64_t* node_visits,
+                                   uint64_t* prim_tests)
+{
+    if (!valid(s) || (n && (!rays || !hits))) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed) return fail(TR_ERROR_NOT_COMMITTED, "scene not built");
+    uint64_t nv = 0, np = 0;
+    HostStack* stack = new HostStack();
+    for (size_t i = 0; i < n; ++i) {
+        RayPre r = ray_setup(rays[i].org, rays[i].dir);
+        stack->sp = 0;
+        uint32_t v = 0, p = 0;
+        Hit h = traverse_closest(s->nodes.data(), s->prims.data(), r, rays[i].tnear, rays[i].tfar, *stack, &v, &p);
+        nv += v;
+        np += p;
+        hits[i].t = h.t; hits[i].u = h.u; hits[i].v = h.v; hits[i].geom = h.geom; hits[i].prim = h.prim;
+    }
+    delete stack;
+    if (node_visits) *node_visits = nv;
+    if (prim_tests) *prim_tests = np;
+    return TR_OK;
+}
+
+tr_status tr_debug_bvh_check(const tr_scene* s, uint64_t* violations)
+{
+    if (!valid(s) || !violations) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed) return fail(TR_ERROR_NOT_COMMITTED, "scene not built");
+    /* every primitive's box lies inside its leaf slot's decoded box, every
+     * internal slot points at an existing node, every primitive is
+     * referenced exactly once */
+    uint64_t bad = 0;
+    std::vector<char> seen(s->prims.size(), 0);
+    for (const Node8& n : s->nodes) {
+        float sc[3] = {exp2i(n.exp[0]), exp2i(n.exp[1]), exp2i(n.exp[2])};
+        uint32_t rank = 0;
+        for (int slot = 0; slot < 8; ++slot) {
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = n.origin[a] + (float)n.qlo[a][slot] * sc[a];
+                hi[a] = n.origin[a] + (float)n.qhi[a][slot] * sc[a];
+            }
+            if ((n.imask >> slot) & 1u) {
+                if (n.child_base + rank >= s->nodes.size()) ++bad;
+                ++rank;
+            } else if (n.meta[slot]) {
+                uint32_t first = n.prim_base + (n.meta[slot] & 31u), cnt = n.meta[slot] >> 5u;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    if (first + k >= s->prims.size()) { ++bad; continue; }
+                    seen[first + k]++;
+                    const PrimRec& p = s->prims[first + k];
+                    for (int a = 0; a < 3; ++a) {
+                        float pl, ph;
+                        if (p.flags & PRIM_SPHERE) { pl = p.v0[a] - p.v1[0]; ph = p.v0[a] + p.v1[0]; }
+                        else {
+                            pl = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
+                            ph = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));
+                        }
+                        if (pl < lo[a] || ph > hi[a]) ++bad;
+                    }
+                }
+            }
+        }
+    }
+    for (char c : seen)
+        if (c != 1) ++bad;
+    *violations = bad;
+    return TR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+/*
+ * tr_kernels.h - parameters of the wavefront kernels and their launchers.
+ *
+ * One frame is processed in batches of path slots.  A slot is one
+ * (pixel, camera sample, GI split) triple; slots are numbered pixel-major,
+ * so a batch holds whole pixels and the accumulation of a pixel happens in
+ * one thread, in a fixed order.  Per batch: ray generation, then for each
+ * vertex b = 0..MaxRayDepth: closest-hit trace of the live queue, shading
+ * (emission, next-event shadow rays, BSDF sample, queue compaction) and the
+ * shadow-ray trace; finally accumulation into FP64 per-pixel sums.
+ */
+#ifndef TR_KERNELS_H
+#define TR_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include "tr_scene.h"
+
+namespace trk {
+
+struct FrameParams {
+    SceneView sc;
+    uint32_t width, height;
+    uint32_t spp;        /* SamplesCamera */
+    uint32_t n_gi;       /* SamplesGlobalIllumination */
+    uint32_t n_light;    /* SamplesLocalAreaLight */
+    uint32_t max_depth;  /* MaxRayDepth */
+    uint32_t rays_sq;    /* sqrt(SamplesGlobalIllumination) */
+    uint32_t max_shadow; /* shadow entries per slot */
+    float rr;            /* RussianRoulette */
+    float background[3];
+    uint32_t key0, key1;
+    uint32_t frame;
+    float vpi[16];
+};
+
+/* Structure-of-arrays path state; every array has `capacity` entries
+ * (shadow arrays: max_shadow * capacity). */
+struct PathPool {
+    float4* ray_o;      /* origin, .w unused */
+    float4* ray_d;      /* direction, .w unused */
+    float4* hit;        /* t, u, v, geom (bits) */
+    uint32_t* hit_prim;
+    float4* thr;        /* throughput, .w = pending balance weight */
+    float4* local;      /* radiance of the camera vertex */
+    float4* global;     /* radiance of later vertices */
+    uint32_t* hash;     /* FNV-1a of the hit sequence */
+    uint32_t* queue[2];
+    uint32_t* shadow_queue;
+    float4* sh_o;       /* origin, .w = tfar */
+    float4* sh_d;       /* direction, .w = light geometry (bits) */
+    float4* sh_c;       /* contribution, .w = light index (bits) */
+    uint32_t* sh_n;     /* entries per slot */
+    uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size, [3],[4] work counters */
+    uint2* spill;       /* traversal stack spill, per persistent thread */
+    uint32_t capacity;
+};
+
+/* Output of one batch. */
+struct BatchOut {
+    double* accum;        /* per pixel RGB sums (frame accumulation), may be null */
+    float* path_rgb;      /* per camera sample radiance, may be null */
+    uint32_t* path_hash;  /* per camera sample hit hash, may be null */
+    const uint32_t* pixel_list; /* explicit pixel list, or null for pixel-major order */
+    uint64_t* count_out;  /* node/prim counters when counting, may be null */
+};
+
+constexpr int kTraceBlock = 64;       /* one wave per block */
+constexpr int kStackLds = 16;         /* stack entries per lane kept in LDS */
+constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
+
+void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                   uint32_t n_slots);
+void launch_trace(hipStream_t s, const SceneView& sc, const PathPool& pool, int cur, int nxt, uint32_t grid,
+                  uint64_t* counts);
+void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                  uint32_t vertex, int cur, int nxt, uint32_t n_slots);
+void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, uint32_t vertex, uint32_t grid,
+                   uint64_t* counts);
+void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
+                       uint64_t slot_base, uint32_t n_pixels);
+void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
+                       uint2* spill, uint32_t* work, uint32_t grid);
+void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,
+                    uint32_t n_pixels, uint32_t post);
+
+}  // namespace trk
+
+#endif

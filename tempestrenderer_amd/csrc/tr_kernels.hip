@@ -1,0 +1,519 @@
+/*
+ * tr_kernels.hip - wavefront path-tracing kernels for gfx950.
+ *
+ * The path loop restated here is the reference CPU integrator
+ * RayTracerScene::subdraw (src/graphics/ray-tracing/ray-tracing.cc:
+ * 1981-2379) with next-event estimation EvaluateLightSources (:1809-1930)
+ * and the balance heuristic EvaluateTotalBalanceHeuristic (:1791-1806),
+ * split into stages over a pool of path slots instead of a per-pixel loop:
+ *   raygen -> { trace -> shade -> shadow } per vertex -> accumulate.
+ * Traversal kernels are persistent: a fixed grid of one-wave blocks pulls
+ * 64 rays at a time from a work counter; each lane walks the 8-wide BVH
+ * with a node-group stack whose first kStackLds entries live in LDS.
+ *
+ * Philox counter of every random number: (pixel, camera sample,
+ * split << 16 | vertex << 8 | stream, sub).  The CPU oracle uses the same
+ * layout, so a path is identical on both sides.
+ */
+#include "tr_kernels.h"
+#include "tr_shading.h"
+#include "tr_traverse.h"
+
+namespace trk {
+
+namespace {
+
+constexpr uint32_t kFnvBasis = 2166136261u;
+constexpr uint32_t kFnvPrime = 16777619u;
+
+__device__ inline uint32_t fnv_hit(uint32_t h, uint32_t geom, uint32_t prim)
+{
+    h = (h ^ geom) * kFnvPrime;
+    return (h ^ prim) * kFnvPrime;
+}
+
+struct DevStack {
+    uint2* lds;     /* &stk[0][lane], stride kTraceBlock */
+    uint2* spill;   /* kStackSpill entries of this lane */
+    int sp;
+    __device__ void push(uint32_t b, uint32_t m)
+    {
+        uint2 e = make_uint2(b, m);
+        if (sp < kStackLds) lds[sp * kTraceBlock] = e;
+        else spill[sp - kStackLds] = e;
+        ++sp;
+    }
+    __device__ void pop(uint32_t* b, uint32_t* m)
+    {
+        --sp;
+        uint2 e = sp < kStackLds ? lds[sp * kTraceBlock] : spill[sp - kStackLds];
+        *b = e.x;
+        *m = e.y;
+    }
+    __device__ bool empty() const { return sp == 0; }
+};
+
+/* Wave-aggregated append to a queue. */
+__device__ inline void queue_push(bool pred, uint32_t value, uint32_t* queue, uint32_t* count)
+{
+    uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    uint32_t lane = __lane_id();
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    if (pred) queue[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+__device__ inline void slot_coords(const FrameParams& p, const BatchOut& out, uint64_t slot, uint32_t* pixel,
+                                   uint32_t* cs, uint32_t* split, uint64_t* pseq)
+{
+    uint64_t per_pixel = (uint64_t)p.spp * p.n_gi;
+    uint64_t ps = slot / per_pixel;
+    *pseq = ps;
+    *pixel = out.pixel_list ? out.pixel_list[ps] : (uint32_t)ps;
+    *cs = p.frame * p.spp + (uint32_t)((slot / p.n_gi) % p.spp);
+    *split = (uint32_t)(slot % p.n_gi);
+}
+
+__device__ inline V3 ld4(float4 v) { return mk(v.x, v.y, v.z); }
+__device__ inline float4 st4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+/* ---------------- ray generation ---------------- */
+
+__global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                                uint32_t n_slots)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        pool.counters[0] = n_slots;
+        pool.counters[3] = 0;
+    }
+    if (i >= n_slots) return;
+    uint32_t pixel, cs, split;
+    uint64_t pseq;
+    slot_coords(p, out, slot_base + i, &pixel, &cs, &split, &pseq);
+    uint32_t x = pixel % p.width, y = pixel / p.width;
+    float sx, sy;
+    if (p.spp > 1) {
+        /* camera jitter draws with split 0: all GI splits share the camera ray */
+        U4 r = philox(pixel, cs, (uint32_t)STREAM_CAMERA, 0u, p.key0, p.key1);
+        sx = 2.0f * ((float)x + u01(r.a)) / (float)p.width - 1.0f;
+        sy = 2.0f * ((float)y + u01(r.b)) / (float)p.height - 1.0f;
+    } else {
+        sx = 2.0f * (float)x / (float)(p.width - 1) - 1.0f;
+        sy = 2.0f * (float)y / (float)(p.height - 1) - 1.0f;
+    }
+    /* Matrix4 * Vector4 (matrix4.hh:858-880), ToVector3 (vector4.hh:125-130) */
+    const float* m = p.vpi;
+    float p0[4], p1[4];
+    for (int r = 0; r < 4; ++r) {
+        p0[r] = m[r] * sx;
+        p1[r] = m[r] * sx;
+        p0[r] = p0[r] + m[4 + r] * sy;
+        p1[r] = p1[r] + m[4 + r] * sy;
+        p0[r] = p0[r] + m[8 + r] * -1.0f;
+        p1[r] = p1[r] + m[8 + r] * 1.0f;
+        p0[r] = p0[r] + m[12 + r] * 1.0f;
+        p1[r] = p1[r] + m[12 + r] * 1.0f;
+    }
+    float w0 = 1.0f / p0[3], w1 = 1.0f / p1[3];
+    V3 start = mk(p0[0] * w0, p0[1] * w0, p0[2] * w0);
+    V3 end = mk(p1[0] * w1, p1[1] * w1, p1[2] * w1);
+    V3 dir = normalize(sub(end, start));
+    pool.ray_o[i] = st4(start, 0.0f);
+    pool.ray_d[i] = st4(dir, 0.0f);
+    pool.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    pool.local[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pool.global[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pool.hash[i] = kFnvBasis;
+    pool.queue[0][i] = i;
+}
+
+/* ---------------- closest-hit trace (persistent) ---------------- */
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool pool, int cur, int nxt,
+                                                       unsigned long long* counts)
+{
+    __shared__ uint2 stk[kStackLds][kTraceBlock];
+    __shared__ uint32_t s_base;
+    uint32_t lane = threadIdx.x;
+    uint32_t gtid = blockIdx.x * kTraceBlock + lane;
+    if (gtid == 0) {
+        pool.counters[nxt] = 0;
+        pool.counters[2] = 0;
+        pool.counters[4] = 0;
+    }
+    const uint32_t n = pool.counters[cur];
+    const uint32_t* queue = pool.queue[cur];
+    DevStack st;
+    st.lds = &stk[0][lane];
+    st.spill = pool.spill + (size_t)gtid * kStackSpill;
+    uint32_t nv = 0, np = 0;
+    for (;;) {
+        if (lane == 0) s_base = atomicAdd(&pool.counters[3], (uint32_t)kTraceBlock);
+        __syncthreads();
+        uint32_t base = s_base;
+        __syncthreads();
+        if (base >= n) break;
+        uint32_t qi = base + lane;
+        if (qi < n) {
+            uint32_t slot = queue[qi];
+            float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
+            float org[3] = {o.x, o.y, o.z}, dir[3] = {d.x, d.y, d.z};
+            RayPre r = ray_setup(org, dir);
+            st.sp = 0;
+            Hit h = traverse_closest(sc.nodes, sc.prims, r, 0.0f, INFINITY, st, COUNT ? &nv : nullptr,
+                                     COUNT ? &np : nullptr);
+            pool.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.geom));
+            pool.hit_prim[slot] = h.prim;
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nv);
+        atomicAdd(&counts[1], (unsigned long long)np);
+    }
+}
+
+/* ---------------- shading ---------------- */
+
+__global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                               uint32_t vertex, int cur, int nxt)
+{
+    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = pool.counters[cur];
+    bool active = qi < n;
+    bool extend = false;
+    bool shadow = false;
+    uint32_t slot = active ? pool.queue[cur][qi] : 0u;
+    if (active) {
+        const SceneView& sc = p.sc;
+        uint32_t pixel, cs, split;
+        uint64_t pseq;
+        slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+        Rng rng;
+        rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
+        float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = pool.hit[slot], t4 = pool.thr[slot];
+        V3 org = ld4(o4), dir = ld4(d4);
+        V3 T = ld4(t4);
+        float bal = t4.w;
+        uint32_t geom = __float_as_uint(h4.w);
+        uint32_t hash = pool.hash[slot];
+        V3 local = mk(0.0f, 0.0f, 0.0f);
+        V3 global = ld4(pool.global[slot]);
+        bool write_local = false;
+        if (geom == TR_INVALID_GEOMETRY) {
+            hash = (hash ^ 0xffffffffu) * kFnvPrime;
+            V3 env = ld3(p.background);
+            if (vertex == 0) { local = env; write_local = true; }
+            else global = add(global, mulv(env, T));
+        } else {
+            uint32_t prim = pool.hit_prim[slot];
+            hash = fnv_hit(hash, geom, prim);
+            Sample sd;
+            sd.dir_density = 0.0f;
+            sample_geometry(sc, geom, prim, h4.x, h4.y, h4.z, org, dir, &sd);
+            V3 pos = add(org, smul(h4.x, dir));
+            sd.wi = dir;
+            material_cache(&sd);
+            if (is_emissive(sd.mat->model)) {
+                V3 le = ld3(sd.mat->diffuse);
+                if (vertex == 0) { local = mulv(T, le); write_local = true; }
+                else global = add(global, mulv(muls(T, bal), le));
+            } else {
+                sd.wo = neg(dir);
+                uint32_t ls = vertex == 0 ? p.n_light : 1u;
+                uint32_t ss = vertex == 0 ? p.n_gi : 1u;
+                bool apply_balance = vertex == 0 ? (p.max_depth != 0) : (vertex != p.max_depth);
+                /* next-event estimation: the camera vertex's is evaluated once, by split 0 */
+                if (vertex != 0 || split == 0) {
+                    uint32_t ne = 0;
+                    for (uint32_t li = 0; li < sc.n_lights; ++li) {
+                        const LightRec& L = sc.lights[li];
+                        uint32_t samples = L.area ? ls : 1u;
+                        for (uint32_t k = 0; k < samples; ++k) {
+                            U4 xr = rng.draw(vertex, STREAM_LIGHT, (li << 16) | k);
+                            LightSample smp = sample_light(sc, L, pos, xr.a, xr.b, xr.c);
+                            if (smp.pdf == 0.0f) continue;
+                            Sample lsd = sd;
+                            lsd.wi = smp.dir;
+                            V3 curT = T;
+                            if (apply_balance && L.area) {
+                                float total = total_balance(sc, (float)ls, (float)ss, lsd, pos);
+                                float balance = (light_pdf(sc, L, pos, lsd.wi) * (float)samples) / total;
+                                curT = muls(curT, balance);
+                            }
+                            V3 contrib = divs(mulv(mulv(smp.radiance, curT), bsdf_eval(lsd)), smp.pdf);
+                            size_t e = (size_t)ne * pool.capacity + slot;
+                            pool.sh_o[e] = st4(pos, smp.dist);
+                            pool.sh_d[e] = st4(smp.dir, __uint_as_float(smp.geom));
+                            pool.sh_c[e] = st4(contrib, __uint_as_float(li));
+                            ++ne;
+                        }
+                    }
+                    pool.sh_n[slot] = ne;
+                    shadow = ne > 0;
+                }
+                bool alive = true;
+                if (vertex > 0) {
+                    T = muls(T, bal / p.rr);
+                    if (p.rr < 1.0f) {
+                        U4 xq = rng.draw(vertex, STREAM_RR, 0u);
+                        if (u01(xq.a) > p.rr) alive = false;
+                    }
+                }
+                if (alive && vertex < p.max_depth) {
+                    Strata sta;
+                    if (vertex == 0) {
+                        sta.x = (float)(split % p.rays_sq);
+                        sta.y = (float)(split / p.rays_sq);
+                        sta.tx = sta.ty = (float)p.rays_sq;
+                    } else {
+                        sta.x = sta.y = 0.0f;
+                        sta.tx = sta.ty = 1.0f;
+                    }
+                    U4 xb = rng.draw(vertex, STREAM_BSDF, 0u);
+                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c));
+                    if (sd.pdf != 0.0f) {
+                        float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
+                        float nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
+                        T = mulv(T, divs(bsdf_eval(sd), sd.pdf));
+                        pool.ray_o[slot] = st4(pos, 0.0f);
+                        pool.ray_d[slot] = st4(sd.wi, 0.0f);
+                        pool.thr[slot] = st4(T, nb);
+                        extend = true;
+                    }
+                }
+            }
+        }
+        pool.hash[slot] = hash;
+        pool.global[slot] = st4(global, 0.0f);
+        if (write_local) pool.local[slot] = st4(local, 0.0f);
+    }
+    queue_push(extend, slot, pool.queue[nxt], &pool.counters[nxt]);
+    queue_push(shadow, slot, pool.shadow_queue, &pool.counters[2]);
+}
+
+/* ---------------- shadow rays (persistent) ---------------- */
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool pool, uint32_t vertex,
+                                                        unsigned long long* counts)
+{
+    __shared__ uint2 stk[kStackLds][kTraceBlock];
+    __shared__ uint32_t s_base;
+    uint32_t lane = threadIdx.x;
+    uint32_t gtid = blockIdx.x * kTraceBlock + lane;
+    if (gtid == 0) pool.counters[3] = 0;
+    const uint32_t n = pool.counters[2];
+    const SceneView& sc = p.sc;
+    DevStack st;
+    st.lds = &stk[0][lane];
+    st.spill = pool.spill + (size_t)gtid * kStackSpill;
+    uint32_t nv = 0, np = 0;
+    for (;;) {
+        if (lane == 0) s_base = atomicAdd(&pool.counters[4], (uint32_t)kTraceBlock);
+        __syncthreads();
+        uint32_t base = s_base;
+        __syncthreads();
+        if (base >= n) break;
+        uint32_t qi = base + lane;
+        if (qi < n) {
+            uint32_t slot = pool.shadow_queue[qi];
+            uint32_t ne = pool.sh_n[slot];
+            uint32_t ls = vertex == 0 ? p.n_light : 1u;
+            V3 radiance = mk(0.0f, 0.0f, 0.0f);
+            uint32_t e = 0;
+            for (uint32_t li = 0; li < sc.n_lights; ++li) {
+                uint32_t samples = sc.lights[li].area ? ls : 1u;
+                V3 color = mk(0.0f, 0.0f, 0.0f);
+                while (e < ne) {
+                    size_t idx = (size_t)e * pool.capacity + slot;
+                    float4 c4 = pool.sh_c[idx];
+                    if (__float_as_uint(c4.w) != li) break;
+                    float4 o4 = pool.sh_o[idx], d4 = pool.sh_d[idx];
+                    float org[3] = {o4.x, o4.y, o4.z}, dir[3] = {d4.x, d4.y, d4.z};
+                    RayPre r = ray_setup(org, dir);
+                    st.sp = 0;
+                    Hit h = traverse_closest(sc.nodes, sc.prims, r, 0.0f, o4.w, st, COUNT ? &nv : nullptr,
+                                             COUNT ? &np : nullptr);
+                    if (h.geom == TR_INVALID_GEOMETRY || h.geom == __float_as_uint(d4.w)) color = add(color, ld4(c4));
+                    ++e;
+                }
+                radiance = add(radiance, divs(color, (float)samples));
+            }
+            if (vertex == 0) {
+                pool.local[slot] = st4(radiance, 0.0f);
+            } else {
+                float4 g = pool.global[slot];
+                pool.global[slot] = st4(add(ld4(g), radiance), 0.0f);
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nv);
+        atomicAdd(&counts[1], (unsigned long long)np);
+    }
+}
+
+/* ---------------- accumulation ---------------- */
+
+__global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                                    uint32_t n_pixels)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pixels) return;
+    uint64_t per_pixel = (uint64_t)p.spp * p.n_gi;
+    uint64_t first_slot = slot_base + (uint64_t)i * per_pixel;
+    uint64_t pseq = first_slot / per_pixel;
+    uint32_t pixel = out.pixel_list ? out.pixel_list[pseq] : (uint32_t)pseq;
+    double sum[3] = {0.0, 0.0, 0.0};
+    float rcp_gi = 1.0f / (float)p.n_gi;
+    for (uint32_t c = 0; c < p.spp; ++c) {
+        uint32_t s0 = (uint32_t)(first_slot - slot_base) + c * p.n_gi;
+        V3 g = ld4(pool.global[s0]);
+        uint32_t h = pool.hash[s0];
+        for (uint32_t j = 1; j < p.n_gi; ++j) {
+            g = add(g, ld4(pool.global[s0 + j]));
+            h = (h * 0x9E3779B1u) ^ pool.hash[s0 + j];
+        }
+        V3 v = add(ld4(pool.local[s0]), muls(g, rcp_gi));
+        sum[0] += (double)v.x;
+        sum[1] += (double)v.y;
+        sum[2] += (double)v.z;
+        if (out.path_rgb) {
+            size_t k = (size_t)pseq * p.spp + c;
+            out.path_rgb[k * 3 + 0] = v.x;
+            out.path_rgb[k * 3 + 1] = v.y;
+            out.path_rgb[k * 3 + 2] = v.z;
+            if (out.path_hash) out.path_hash[k] = h;
+        }
+    }
+    if (out.accum) {
+        out.accum[(size_t)pixel * 3 + 0] += sum[0];
+        out.accum[(size_t)pixel * 3 + 1] += sum[1];
+        out.accum[(size_t)pixel * 3 + 2] += sum[2];
+    }
+}
+
+/* ---------------- fixed ray sets ---------------- */
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
+                                                            uint2* spill, uint32_t* work)
+{
+    __shared__ uint2 stk[kStackLds][kTraceBlock];
+    __shared__ uint32_t s_base;
+    uint32_t lane = threadIdx.x;
+    uint32_t gtid = blockIdx.x * kTraceBlock + lane;
+    DevStack st;
+    st.lds = &stk[0][lane];
+    st.spill = spill + (size_t)gtid * kStackSpill;
+    for (;;) {
+        if (lane == 0) s_base = atomicAdd(work, (uint32_t)kTraceBlock);
+        __syncthreads();
+        uint32_t base = s_base;
+        __syncthreads();
+        if (base >= n) break;
+        uint32_t i = base + lane;
+        if (i < n) {
+            tr_ray ry = rays[i];
+            RayPre r = ray_setup(ry.org, ry.dir);
+            st.sp = 0;
+            Hit h = traverse_closest(sc.nodes, sc.prims, r, ry.tnear, ry.tfar, st, (uint32_t*)nullptr,
+                                     (uint32_t*)nullptr);
+            tr_hit o;
+            o.t = h.t; o.u = h.u; o.v = h.v; o.geom = h.geom; o.prim = h.prim;
+            hits[i] = o;
+        }
+    }
+}
+
+/* ---------------- backbuffer ---------------- */
+
+/* ConvertLinearToSLuminance (spectrum.hh:146-152; pow via tr_fmath) and
+ * ToColor (spectrum.hh:792-798). */
+__device__ inline float to_srgb(float v)
+{
+    if (v <= 0.0031308f) return v * 12.92f;
+    return 1.055f * tr_expf(tr_logf(v) * (1.0f / 2.4f)) - 0.055f;
+}
+__device__ inline uint32_t to_color(float x, float y, float z)
+{
+    x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+    y = y < 0.0f ? 0.0f : (y > 1.0f ? 1.0f : y);
+    z = z < 0.0f ? 0.0f : (z > 1.0f ? 1.0f : z);
+    return (uint32_t)(255.0f * x) | ((uint32_t)(255.0f * y) << 8u) | ((uint32_t)(255.0f * z) << 16u) | (255u << 24u);
+}
+
+__global__ __launch_bounds__(256) void k_resolve(const double* accum, double rcp_count, uint32_t* rgba, uint32_t n,
+                                                 uint32_t post)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float r = (float)(accum[(size_t)i * 3 + 0] * rcp_count);
+    float g = (float)(accum[(size_t)i * 3 + 1] * rcp_count);
+    float b = (float)(accum[(size_t)i * 3 + 2] * rcp_count);
+    if (post == TR_POST_LINEAR) rgba[i] = to_color(r, g, b);
+    else rgba[i] = to_color(to_srgb(r), to_srgb(g), to_srgb(b));
+}
+
+}  // namespace
+
+void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                   uint32_t n_slots)
+{
+    uint32_t grid = (n_slots + 255) / 256;
+    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, p, pool, out, slot_base, n_slots);
+}
+
+void launch_trace(hipStream_t s, const SceneView& sc, const PathPool& pool, int cur, int nxt, uint32_t grid,
+                  uint64_t* counts)
+{
+    if (counts)
+        hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(kTraceBlock), 0, s, sc, pool, cur, nxt,
+                           (unsigned long long*)counts);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(kTraceBlock), 0, s, sc, pool, cur, nxt,
+                           (unsigned long long*)nullptr);
+}
+
+void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                  uint32_t vertex, int cur, int nxt, uint32_t n_slots)
+{
+    uint32_t grid = (n_slots + 255) / 256;
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, p, pool, out, slot_base, vertex, cur, nxt);
+}
+
+void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, uint32_t vertex, uint32_t grid,
+                   uint64_t* counts)
+{
+    if (counts)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kTraceBlock), 0, s, p, pool, vertex,
+                           (unsigned long long*)counts);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kTraceBlock), 0, s, p, pool, vertex,
+                           (unsigned long long*)nullptr);
+}
+
+void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
+                       uint64_t slot_base, uint32_t n_pixels)
+{
+    uint32_t grid = (n_pixels + 255) / 256;
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, p, pool, out, slot_base, n_pixels);
+}
+
+void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n, uint2* spill,
+                       uint32_t* work, uint32_t grid)
+{
+    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill, work);
+}
+
+void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba, uint32_t n, uint32_t post)
+{
+    uint32_t grid = (n + 255) / 256;
+    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, s, accum, rcp_count, rgba, n, post);
+}
+
+}  // namespace trk

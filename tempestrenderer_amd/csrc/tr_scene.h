@@ -1,0 +1,92 @@
+/*
+ * tr_scene.h - the scene as it lives in HBM, and the 8-wide BVH format.
+ *
+ * Layout (all arrays 256-byte aligned, read-only during a frame):
+ *   Node8[]     128 B per node: quantised child boxes of an 8-wide BVH.
+ *   PrimRec[]   48 B per primitive, in BVH leaf order: the three world-space
+ *               vertices of a triangle (or centre and radius of a sphere),
+ *               its primitive id, geometry slot and flags.  Traversal reads
+ *               nothing else.
+ *   GeomRec[]   one per geometry slot (submesh, sphere, sphere light).
+ *   tr_material[], LightRec[], vertex bytes and int32 index triplets for
+ *               shading normals / tangents and mesh-light sampling.
+ */
+#ifndef TR_SCENE_H
+#define TR_SCENE_H
+
+#include <stdint.h>
+#include "../../include/tempest_rt.h"
+
+namespace trk {
+
+/* 8-wide BVH node, one 128-byte cache line.
+ * Child slot s (0..7) has the box
+ *   lo = origin + q_lo[s] * 2^exp,  hi = origin + q_hi[s] * 2^exp  (per axis)
+ * computed in FP32 exactly as written; the builder rounds outward and
+ * checks that the decoded box contains the child.  imask bit s marks an
+ * internal child: its node index is child_base + popcount(imask & ((1<<s)-1)).
+ * A leaf child has meta[s] = count << 5 | offset (count 1..7): primitives
+ * prim_base + offset .. + count - 1.  meta[s] = 0 and imask bit clear marks
+ * an empty slot.  Slots are assigned so that slot s lies towards the
+ * (+x if bit0, +y if bit1, +z if bit2) side of the node; a ray visits
+ * slots in the order i ^ octant(ray direction), i = 0..7, i.e. roughly
+ * front to back. */
+struct alignas(128) Node8 {
+    float origin[3];
+    int8_t exp[3];
+    uint8_t imask;
+    uint32_t child_base;
+    uint32_t prim_base;
+    uint8_t meta[8];
+    uint16_t qlo[3][8];
+    uint16_t qhi[3][8];
+};
+static_assert(sizeof(Node8) == 128, "Node8 must be one 128-byte line");
+
+enum { PRIM_SPHERE = 1u, PRIM_TWO_SIDED = 2u };
+
+struct alignas(16) PrimRec {
+    float v0[3];
+    uint32_t prim;
+    float v1[3];
+    uint32_t geom;
+    float v2[3];
+    uint32_t flags;
+};
+static_assert(sizeof(PrimRec) == 48, "PrimRec must be 48 bytes");
+
+enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2 };
+
+struct GeomRec {
+    uint32_t type;
+    uint32_t material;
+    uint32_t two_sided;
+    uint32_t stride;
+    uint64_t vbase;       /* byte offset of the submesh's vertex 0 in the vertex pool */
+    uint32_t index_base;  /* offset of the submesh's first index in the index pool */
+    uint32_t tri_count;
+    float sphere[4];
+};
+
+struct LightRec {
+    tr_light src;
+    uint32_t geom;        /* mesh and sphere lights: geometry slot */
+    uint32_t area;        /* IsAreaLight: sphere lights only (ray-tracing-common.hh:546-551) */
+};
+
+/* Device pointers of a committed scene. */
+struct SceneView {
+    const Node8* nodes;
+    const PrimRec* prims;
+    const GeomRec* geoms;
+    const tr_material* mats;
+    const LightRec* lights;
+    const uint8_t* vpool;
+    const int32_t* ipool;
+    uint32_t n_lights;
+    uint32_t n_prims;
+};
+
+}  // namespace trk
+
+#endif

@@ -1,0 +1,509 @@
+/*
+ * tr_shading.h - BSDFs, light sampling and geometry sampling of the kernels.
+ *
+ * Each function restates the reference function named beside it, in the
+ * reference's FP32 operation order.  Deviations from the reference, all
+ * deliberate and matched by the CPU oracle:
+ *  - random numbers come from Philox (tr_math.h) instead of FastFloatRand;
+ *  - sin/cos/log/exp come from tr_fmath.h instead of FastSinCos / libm, and
+ *    angles are taken in turns (sin(2*pi*u) is tr_sincos_turns(u));
+ *  - pow(x, 5) in FresnelSchlick is x^2*x^2*x;
+ *  - a mesh light's triangle is picked as floor(u32 * count / 2^32) instead
+ *    of Hash32(seed) % count (functions.hh:243-247).
+ */
+#ifndef TR_SHADING_H
+#define TR_SHADING_H
+
+#include "tr_math.h"
+#include "tr_scene.h"
+
+namespace trk {
+
+/* SampleData (ray-tracing-common.hh:184-198), the parts the models read. */
+struct Sample {
+    const tr_material* mat;
+    V3 wo, wi, tangent, binormal, normal;
+    float dir_density;
+    float pdf;
+};
+
+struct Strata {
+    float x, y, tx, ty;
+};
+
+TR_HD float geom_factor_sq(float power) { return 2.0f / (power + 2.0f); } /* microfacet-common.hh:109-112 */
+TR_HD float fresnel_schlick(float refl, float c) { return refl + (1.0f - refl) * pow5(1.0f - c); } /* functions.hh:262 */
+TR_HD float cos_transmittance(float eta, float ci) { return sqrtf(1.0f - (1.0f - ci * ci) / (eta * eta)); } /* :268 */
+
+/* FresnelConductor, functions.hh:275-294 */
+TR_HD float fresnel_conductor(float eta, float k, float cos_inc)
+{
+    float cos_inc2 = cos_inc * cos_inc;
+    float sin_inc2 = 1.0f - cos_inc2;
+    float eta2 = eta * eta;
+    float etak2 = k * k;
+    float t0 = eta2 - etak2 - sin_inc2;
+    float a2plusb2 = sqrtf(t0 * t0 + 4.0f * eta2 * etak2);
+    float t1 = a2plusb2 + cos_inc2;
+    float a = sqrtf(0.5f * (a2plusb2 + t0));
+    float t2 = 2.0f * a * cos_inc;
+    float r_perp = (t1 - t2) / (t1 + t2);
+    float t3 = cos_inc2 * a2plusb2 + sin_inc2 * sin_inc2;
+    float t4 = t2 * sin_inc2;
+    float r_parallel = r_perp * (t3 - t4) / (t3 + t4);
+    return 0.5f * (r_perp + r_parallel);
+}
+
+/* FresnelDielectric, functions.hh:296-301 */
+TR_HD float fresnel_dielectric(float eta, float ci, float ct)
+{
+    float r_parallel = (eta * ci - ct) / (eta * ci + ct);
+    float r_perp = (ci - eta * ct) / (ci + eta * ct);
+    return 0.5f * (r_parallel * r_parallel + r_perp * r_perp);
+}
+
+/* GGXMicrofacetDistribution, ggx-models-impl.hh:126-139 */
+TR_HD float ggx_d(float g2, float cos_nh)
+{
+    if (cos_nh <= 0.0f) return 0.0f;
+    float c2 = cos_nh * cos_nh;
+    float c4 = c2 * c2;
+    float tan2 = (1.0f - c2) / c2;
+    float den = g2 + tan2;
+    return g2 / (kPi * c4 * den * den);
+}
+
+/* G1_GGX, ggx-models-impl.hh:141-151 */
+TR_HD float ggx_g1(float g2, float cos_vm, float cos_vn)
+{
+    float ratio = cos_vm / cos_vn;
+    if (ratio <= 0.0f) return 0.0f;
+    float c2 = cos_vn * cos_vn;
+    float tan2 = (1.0f - c2) / c2;
+    return 2.0f / (1.0f + sqrtf(1.0f + g2 * tan2));
+}
+
+/* BeckmannMicrofacetDistribution, beckmann-models-impl.hh:57-70 */
+TR_HD float beckmann_d(float g2, float cos_nh)
+{
+    if (cos_nh <= 0.0f) return 0.0f;
+    float c2 = cos_nh * cos_nh;
+    float c4 = c2 * c2;
+    float tan2 = (1.0f - c2) / c2;
+    return tr_expf(-tan2 / g2) / (kPi * g2 * c4);
+}
+
+/* G1_Beckmann, rational approximation, beckmann-models-impl.hh:34-55 */
+TR_HD float beckmann_g1(float g2, float cos_vm, float cos_vn)
+{
+    float ratio = cos_vm / cos_vn;
+    if (ratio <= 0.0f) return 0.0f;
+    float c2 = cos_vn * cos_vn;
+    float tan2 = (1.0f - c2) / c2;
+    float a2 = 1.0f / (g2 * tan2);
+    float a = sqrtf(a2);
+    return a < 1.6f ? (3.535f * a + 2.181f * a2) / (1.0f + 2.276f * a + 2.577f * a2) : 1.0f;
+}
+
+TR_HD bool is_beckmann(uint32_t m) { return m >= TR_MODEL_BECKMANN_MICROFACET && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC; }
+TR_HD bool is_ggx(uint32_t m) { return m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC; }
+TR_HD bool is_microfacet(uint32_t m) { return is_ggx(m) || is_beckmann(m); }
+
+/* GGXMicrofacet{,Conductor,Dielectric}BRDF (ggx-models-impl.hh:156-209) and
+ * the Beckmann twins (beckmann-models-impl.hh:121-172): BRDF * cos. */
+TR_HD V3 microfacet_brdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    uint32_t model = m->model;
+    bool beck = is_beckmann(model);
+    /* ComputeMicrofacetAngles, microfacet-common.hh:74-84 */
+    V3 h = normalize(add(s.wi, s.wo));
+    float cos_in = dot(s.wi, s.normal);
+    float cos_on = dot(s.wo, s.normal);
+    float cos_im = dot(s.wi, h);
+    float cos_mn = dot(s.normal, h);
+    uint32_t variant = model - (beck ? (uint32_t)TR_MODEL_BECKMANN_MICROFACET : (uint32_t)TR_MODEL_GGX_MICROFACET);
+    float fresnel;
+    if (variant == 0) {
+        fresnel = fresnel_schlick(m->fresnel[0], cos_im);
+    } else if (variant == 1) {
+        fresnel = fresnel_conductor(m->fresnel[0], m->fresnel[1], cos_im);
+    } else {
+        float ct = cos_transmittance(m->fresnel[0], cos_im);
+        fresnel = fresnel_dielectric(m->fresnel[0], cos_im, ct);
+    }
+    float g2 = geom_factor_sq(m->specular_power[0]);
+    float shadow, d;
+    if (beck) {
+        shadow = beckmann_g1(g2, cos_im, cos_in) * beckmann_g1(g2, cos_im, cos_on);
+        d = beckmann_d(g2, cos_mn);
+    } else {
+        shadow = ggx_g1(g2, cos_im, cos_in) * ggx_g1(g2, cos_im, cos_on);
+        d = ggx_d(g2, cos_mn);
+    }
+    float spec = shadow * d / (4.0f * fabsf(cos_in) * fabsf(cos_on));
+    float brdf = fresnel * spec;
+    return muls(add(muls(ld3(m->diffuse), 1.0f / kPi), muls(ld3(m->specular), brdf)), cos_in);
+}
+
+/* GGXMicrofacetPDF / BeckmannMicrofacetPDF (ggx-models-impl.hh:211-237,
+ * beckmann-models-impl.hh:240-266) */
+TR_HD float microfacet_pdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    float cos_ln = dot(s.wi, s.normal);
+    if (cos_ln < 0.0f) return 0.0f;
+    V3 h = normalize(add(s.wi, s.wo));
+    float cos_hn = dot(h, s.normal);
+    float g2 = geom_factor_sq(m->specular_power[0]);
+    float norm_pdf = (is_beckmann(m->model) ? beckmann_d(g2, cos_hn) : ggx_d(g2, cos_hn)) * cos_hn;
+    if (cos_hn <= 0.0f || norm_pdf <= 0.0f) return 0.0f;
+    float jacobian = 1.0f / (4.0f * dot(h, s.wo));
+    float pdf0 = norm_pdf * jacobian;
+    float mv = max_value(add(ld3(m->diffuse), ld3(m->specular)));
+    float ratio = mv != 0.0f ? max_value(ld3(m->specular)) / mv : 0.0f;
+    float pdf1 = cos_ln * (1.0f / kPi);
+    return (pdf0 - pdf1) * ratio + pdf1;
+}
+
+/* GGXMicrofacetSampleIncidentLight / BeckmannMicrofacetSampleIncidentLight
+ * (ggx-models-impl.hh:239-282, beckmann-models-impl.hh:268-311) with
+ * CosineSampleHemisphere (sampling3.hh:69-76). */
+TR_HD void microfacet_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2)
+{
+    const tr_material* m = s->mat;
+    float x_ray = (st.x + xi0) / st.tx;
+    float y_ray = (st.y + xi1) / st.ty;
+    float mv = max_value(add(ld3(m->diffuse), ld3(m->specular)));
+    float ratio = mv != 0.0f ? max_value(ld3(m->specular)) / mv : 0.0f;
+    M3 basis = make_basis(s->normal);
+    float sin_phi, cos_phi;
+    tr_sincos_turns(y_ray, &sin_phi, &cos_phi);
+    if (xi2 <= ratio) {
+        float g2 = geom_factor_sq(m->specular_power[0]);
+        float g = sqrtf(g2);
+        float r = is_beckmann(m->model) ? -tr_logf(x_ray) : x_ray / (1.0f - x_ray);
+        float sqrt_ratio = sqrtf(r);
+        float rcp_len = 1.0f / sqrtf(r * g2 + 1.0f);
+        float common = sqrt_ratio * g * rcp_len;
+        V3 mn = rcp_len > 1e-6f ? mk(-common * cos_phi, -common * sin_phi, rcp_len) : mk(0.0f, 0.0f, 1.0f);
+        mn = transform(basis, mn);
+        s->wi = reflect(s->wo, mn);
+    } else {
+        float r = sqrtf(x_ray);
+        V3 d = mk(cos_phi * r, sin_phi * r, sqrtf(fmax_(0.0f, 1.0f - x_ray)));
+        s->wi = normalize(transform(basis, d));
+    }
+    s->pdf = microfacet_pdf(*s);
+}
+
+/* UniformHemispherePDF, illumination-models.hh:723-727 */
+TR_HD float uniform_hemisphere_pdf(const Sample& s)
+{
+    return dot(s.normal, s.wi) >= 0.0f ? 1.0f / (2.0f * kPi) : 0.0f;
+}
+
+/* UniformSampleIncidentLight, illumination-models.hh:729-738, with
+ * UniformSampleHemisphere (sampling3.hh:55-62) */
+TR_HD void uniform_hemisphere_sample(const Strata& st, Sample* s, float xi0, float xi1)
+{
+    M3 basis = make_basis(s->normal);
+    float ra = (st.x + xi0) / st.tx;
+    float rb = (st.y + xi1) / st.ty;
+    float r = sqrtf(1.0f - ra * ra);
+    float sin_phi, cos_phi;
+    tr_sincos_turns(rb, &sin_phi, &cos_phi);
+    V3 d = mk(cos_phi * r, sin_phi * r, ra);
+    s->pdf = 1.0f / (2.0f * kPi);
+    s->wi = normalize(transform(basis, d));
+}
+
+/* SGGXProjectedArea, ggx-models-impl.hh:99-102 */
+TR_HD float sggx_projected_area(V3 stddev, const M3& basis, V3 dir)
+{
+    return length(mulv(transform_rotation_inverse(basis, dir), stddev));
+}
+
+/* SGGXMicroFlakeNDF(stddev, basis, normal), ggx-models-impl.hh:48-57 */
+TR_HD float sggx_ndf(V3 stddev, const M3& basis, V3 n)
+{
+    V3 sq = mulv(stddev, stddev);
+    M3 inv_cov = transform_covariance(basis, mk(1.0f / sq.x, 1.0f / sq.y, 1.0f / sq.z));
+    float det = determinant(inv_cov);
+    if (det < 1e-6f) return 0.0f;
+    float den = dot(n, transform(inv_cov, n));
+    return 0.5f * sqrtf(det) / (kPi * den * den);
+}
+
+/* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619 */
+TR_HD float sggx_surface_density(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    M3 orient = m3(s.tangent, s.binormal, s.normal);
+    M3 basis = mul(orient, quat_to_m3(m->sggx_basis[0], m->sggx_basis[1], m->sggx_basis[2], m->sggx_basis[3]));
+    return sggx_projected_area(mk(m->specular_power[0], m->specular_power[1], 1.0f), basis, s.wi);
+}
+
+/* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551 (no texture maps) */
+TR_HD V3 sggx_surface_brdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    float din = dot(s.normal, s.wi);
+    float don = dot(s.normal, s.wo);
+    if (din < 0.0f || don < 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    V3 result = muls(muls(ld3(m->diffuse), din), 1.0f / kPi);
+    V3 stddev = mk(m->specular_power[0], m->specular_power[1], 1.0f);
+    if (stddev.x >= 1e-6f && stddev.y >= 1e-6f && stddev.z >= 1e-6f && s.dir_density >= 1e-6f) {
+        V3 mn = normalize(add(s.wi, s.wo));
+        M3 orient = m3(s.tangent, s.binormal, s.normal);
+        float qx = m->sggx_basis[0], qy = m->sggx_basis[1], qz = m->sggx_basis[2], qw = m->sggx_basis[3];
+        float ql = sqrtf(qx * qx + qy * qy + qz * qz + qw * qw);
+        if (ql == 0.0f) { qx = qy = qz = qw = 0.0f; }
+        else { qx = qx / ql; qy = qy / ql; qz = qz / ql; qw = qw / ql; }
+        M3 basis = mul(orient, quat_to_m3(qx, qy, qz, qw));
+        float cos_micro = dot(mn, s.wi);
+        float fresnel;
+        float eta = m->fresnel[0];
+        if (eta == 1.0f) {
+            fresnel = 1.0f;
+        } else {
+            float ct = cos_transmittance(eta, cos_micro);
+            fresnel = fresnel_dielectric(eta, cos_micro, ct);
+            fresnel = fresnel < 0.0f ? 0.0f : (fresnel > 1.0f ? 1.0f : fresnel);
+        }
+        float in_density = sggx_projected_area(stddev, basis, s.wi);
+        float contrib = 1.0f / (s.dir_density + in_density * don / din);
+        result = add(result, muls(muls(muls(muls(smul(contrib, ld3(m->specular)), sggx_ndf(stddev, basis, mn)), fresnel), 2.0f), 0.25f));
+    }
+    return result;
+}
+
+TR_HD bool is_emissive(uint32_t m) { return m == TR_MODEL_EMISSIVE; }
+
+/* TransmittanceLookup / PDFLookup / SampleIncidentLightLookup /
+ * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
+ * models: GGX and Beckmann (plain, conductor, dielectric), SGGX surface. */
+TR_HD V3 bsdf_eval(const Sample& s)
+{
+    uint32_t m = s.mat->model;
+    if (is_microfacet(m)) return microfacet_brdf(s);
+    if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
+    return mk(0.0f, 0.0f, 0.0f);
+}
+TR_HD float bsdf_pdf(const Sample& s)
+{
+    uint32_t m = s.mat->model;
+    if (is_microfacet(m)) return microfacet_pdf(s);
+    if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
+    return 0.0f;
+}
+TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2)
+{
+    uint32_t m = s->mat->model;
+    if (is_microfacet(m)) microfacet_sample(st, s, xi0, xi1, xi2);
+    else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi0, xi1);
+    else s->pdf = 0.0f;
+}
+TR_HD void material_cache(Sample* s)
+{
+    if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s);
+}
+
+/* ---------------- geometry sampling ---------------- */
+
+/* IntersectSphere's normal (intersect.hh:181-182) */
+TR_HD V3 sphere_normal(V3 org, V3 dir, const float* c, float t)
+{
+    V3 dc = sub(ld3(c), org);
+    return normalize(sub(smul(t, dir), dc));
+}
+
+/* RTMesh::sample + DefaultMeshSample (ray-tracing.cc:375-461),
+ * SphereGeometry::sample (:1166-1180), SphereAreaLightGeometry::sample
+ * (:1775-1783).  Tangent frames of vertex formats without tangents are
+ * zero (the reference leaves the previous values in place). */
+TR_HD void sample_geometry(const SceneView& sc, uint32_t geom, uint32_t prim, float t, float u, float v, V3 org, V3 dir,
+                           Sample* sd)
+{
+    const GeomRec& g = sc.geoms[geom];
+    sd->mat = &sc.mats[g.material];
+    if (g.type == GEOM_MESH) {
+        const int32_t* ib = sc.ipool + g.index_base + (size_t)prim * 3;
+        const uint8_t* base = sc.vpool + g.vbase;
+        const uint8_t* p0 = base + (size_t)ib[0] * g.stride;
+        const uint8_t* p1 = base + (size_t)ib[1] * g.stride;
+        const uint8_t* p2 = base + (size_t)ib[2] * g.stride;
+        uint32_t no = g.stride - 12;
+        float w = (1.0f - u) - v;
+        V3 n0 = ld3(reinterpret_cast<const float*>(p0 + no));
+        V3 n1 = ld3(reinterpret_cast<const float*>(p1 + no));
+        V3 n2 = ld3(reinterpret_cast<const float*>(p2 + no));
+        V3 norm = normalize(add(add(muls(n0, w), muls(n1, u)), muls(n2, v)));
+        sd->tangent = mk(0.0f, 0.0f, 0.0f);
+        sd->binormal = mk(0.0f, 0.0f, 0.0f);
+        if (g.stride >= 44) {
+            V3 t0 = ld3(reinterpret_cast<const float*>(p0 + 20));
+            V3 t1 = ld3(reinterpret_cast<const float*>(p1 + 20));
+            V3 t2 = ld3(reinterpret_cast<const float*>(p2 + 20));
+            V3 tan = add(add(muls(t0, w), muls(t1, u)), muls(t2, v));
+            M3 b = make_basis_orthogonalize(tan, norm);
+            sd->tangent = b.c0;
+            sd->binormal = b.c1;
+        }
+        sd->normal = norm;
+    } else if (g.type == GEOM_SPHERE) {
+        V3 n = sphere_normal(org, dir, g.sphere, t);
+        sd->normal = n;
+        sd->binormal = normalize(cross(mk(0.0f, 0.0f, 1.0f), n));
+        sd->tangent = cross(sd->binormal, sd->normal);
+    } else {
+        sd->normal = normalize(sub(add(org, muls(dir, t)), ld3(g.sphere)));
+        sd->tangent = mk(0.0f, 0.0f, 0.0f);
+        sd->binormal = mk(0.0f, 0.0f, 0.0f);
+    }
+}
+
+/* ---------------- lights ---------------- */
+
+struct LightSample {
+    uint32_t geom;
+    V3 dir;
+    float dist;
+    V3 radiance;
+    float pdf;
+};
+
+/* TriangleArea of projected vertices, sampling3.hh:206-212 */
+TR_HD float projected_triangle_area(V3 pos, V3 p0, V3 p1, V3 p2)
+{
+    V3 a = normalize(sub(p0, pos)), b = normalize(sub(p1, pos)), c = normalize(sub(p2, pos));
+    return 0.5f * length(cross(sub(b, a), sub(c, a)));
+}
+
+TR_HD V3 light_vertex(const SceneView& sc, const GeomRec& g, uint32_t tri, int k)
+{
+    int32_t idx = sc.ipool[g.index_base + (size_t)tri * 3 + k];
+    return ld3(reinterpret_cast<const float*>(sc.vpool + g.vbase + (size_t)idx * g.stride));
+}
+
+/* SampleLightSourceFunctionLookup (ray-tracing-common.hh:567-647) with
+ * SampleSphereArea / UniformSampleSphereArea / UniformSampleTriangleArea
+ * (sampling3.hh:142-227).  x0: triangle pick, x1/x2: the two coordinates. */
+TR_HD LightSample sample_light(const SceneView& sc, const LightRec& L, V3 pos, uint32_t x0, uint32_t x1, uint32_t x2)
+{
+    LightSample r;
+    const tr_light& l = L.src;
+    float ra = u01(x1), rb = u01(x2);
+    if (l.type == TR_LIGHT_DIRECTIONAL) {
+        r.dist = INFINITY;
+        r.dir = ld3(l.direction);
+        r.radiance = ld3(l.radiance);
+        r.pdf = 1.0f;
+        r.geom = TR_INVALID_GEOMETRY;
+    } else if (l.type == TR_LIGHT_POINT) {
+        V3 dv = sub(ld3(l.position), pos);
+        float dist = length(dv);
+        r.dist = INFINITY;
+        r.dir = divs(dv, dist);
+        r.radiance = divs(ld3(l.radiance), kTau * dist * dist);
+        r.pdf = 1.0f;
+        r.geom = TR_INVALID_GEOMETRY;
+    } else if (l.type == TR_LIGHT_SPHERE_AREA) {
+        V3 c = ld3(l.sphere_center);
+        float radius = l.sphere_radius;
+        V3 dc = sub(c, pos);
+        float dist2 = dot(dc, dc);
+        float dist = sqrtf(dist2);
+        r.radiance = ld3(l.radiance);
+        r.geom = L.geom;
+        if (dist <= radius) {
+            float sin_phi, cos_phi;
+            tr_sincos_turns(rb, &sin_phi, &cos_phi);
+            float r2 = 2.0f * radius;
+            float q = sqrtf(ra * (1.0f - ra));
+            V3 res = add(sub(c, pos), mk(r2 * cos_phi * q, r2 * sin_phi * q, radius * (1.0f - 2.0f * ra)));
+            float t = length(res);
+            r.dir = divs(res, t);
+            r.dist = t;
+            r.pdf = 1.0f / (4.0f * kPi);
+            return r;
+        }
+        V3 to_c = divs(dc, dist);
+        float sin_rc = radius / dist;
+        float cos_rc = sqrtf(1.0f - sin_rc * sin_rc);
+        float sin_phi, cos_phi;
+        tr_sincos_turns(ra, &sin_phi, &cos_phi);
+        float cos_theta = 1.0f - rb + rb * cos_rc;
+        float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+        M3 basis = make_basis(to_c);
+        V3 d = normalize(transform(basis, mk(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta)));
+        float b = dist / cos_theta;
+        float cc = dist2 - radius * radius;
+        float D = b * b - cc;
+        float t = b;
+        if (D >= 0.0f) t -= sqrtf(D);
+        r.dir = d;
+        r.dist = t;
+        r.pdf = 1.0f / (2.0f * kPi * (1.0f - cos_rc));
+    } else {
+        const GeomRec& g = sc.geoms[L.geom];
+        uint32_t tri = mulhi32(x0, g.tri_count);
+        V3 p0 = light_vertex(sc, g, tri, 0), p1 = light_vertex(sc, g, tri, 1), p2 = light_vertex(sc, g, tri, 2);
+        float sqrt_r = sqrtf(ra);
+        V3 e0 = sub(p1, p0), e1 = sub(p2, p0);
+        V3 point = add(add(p0, smul(sqrt_r * rb, e0)), smul(1.0f - sqrt_r, e1));
+        V3 du = sub(point, pos);
+        float len = length(du);
+        r.dir = divs(du, len);
+        r.dist = len;
+        float pdf = dot(cross(e0, e1), r.dir) < -1e-6f ? 1.0f / projected_triangle_area(pos, p0, p1, p2) : 0.0f;
+        r.pdf = pdf / (float)g.tri_count;
+        r.radiance = ld3(sc.mats[g.material].diffuse);
+        r.geom = L.geom;
+    }
+    return r;
+}
+
+/* LightSourceComputePDFLookup (ray-tracing-common.hh:582-687) */
+TR_HD float light_pdf(const SceneView& sc, const LightRec& L, V3 pos, V3 dir)
+{
+    const tr_light& l = L.src;
+    if (l.type == TR_LIGHT_DIRECTIONAL || l.type == TR_LIGHT_POINT) return 1.0f;
+    if (l.type == TR_LIGHT_SPHERE_AREA) {
+        V3 dc = sub(ld3(l.sphere_center), pos);
+        float dist = sqrtf(dot(dc, dc));
+        if (dist <= l.sphere_radius) return 1.0f / (4.0f * kPi);
+        V3 to_c = divs(dc, dist);
+        float sin_rc = l.sphere_radius / dist;
+        float cos_rc = sqrtf(1.0f - sin_rc * sin_rc);
+        float cos_theta = dot(dir, to_c);
+        if (cos_theta < cos_rc) return 0.0f;
+        return 1.0f / (2.0f * kPi * (1.0f - cos_rc));
+    }
+    const GeomRec& g = sc.geoms[L.geom];
+    float total = 0.0f;
+    for (uint32_t tri = 0; tri < g.tri_count; ++tri) {
+        V3 p0 = light_vertex(sc, g, tri, 0), p1 = light_vertex(sc, g, tri, 1), p2 = light_vertex(sc, g, tri, 2);
+        V3 un = cross(sub(p2, p1), sub(p0, p1));
+        if (dot(dir, un) >= 0.0f) continue;
+        total = total + projected_triangle_area(pos, p0, p1, p2);
+    }
+    return total != 0.0f ? 1.0f / total : 0.0f;
+}
+
+/* EvaluateTotalBalanceHeuristic (ray-tracing.cc:1791-1806) */
+TR_HD float total_balance(const SceneView& sc, float light_samples, float surface_samples, const Sample& s, V3 pos)
+{
+    float total = 0.0f;
+    for (uint32_t i = 0; i < sc.n_lights; ++i) {
+        const LightRec& L = sc.lights[i];
+        float n = L.area ? light_samples : 1.0f;
+        total = total + light_pdf(sc, L, pos, s.wi) * n;
+    }
+    total = total + bsdf_pdf(s) * surface_samples;
+    return total;
+}
+
+}  // namespace trk
+
+#endif

@@ -1,0 +1,258 @@
+/*
+ * tr_traverse.h - closest-hit traversal of the 8-wide BVH with watertight
+ * triangle and sphere tests.
+ *
+ * Closest hit = the smallest (t, geom, prim), lexicographically, over all
+ * primitive hits with tnear < t < tfar.  The boxes are conservative (the
+ * builder rounds outward, the slab test scales its far bound by 1 + 2^-20),
+ * so any traversal order returns the same ids as a brute-force loop.
+ *
+ * Triangle test: S. Woop, C. Benthin, I. Wald, "Watertight Ray/Triangle
+ * Intersection", JCGT 2(1), 2013, with the FP64 recomputation of zero edge
+ * functions.  Culling follows the reference's Embree filters:
+ * BackfaceCulling keeps a hit iff N_ccw . d < 0 (ray-tracing.cc:1303-1307);
+ * TwoSidedBias drops hits closer than 1e-3 (ray-tracing.cc:1309-1313).
+ * Sphere test: IntersectSphere (include/tempest/math/intersect.hh:163-190)
+ * plus the (tnear, tfar) range check.
+ */
+#ifndef TR_TRAVERSE_H
+#define TR_TRAVERSE_H
+
+#include "tr_math.h"
+#include "tr_scene.h"
+
+namespace trk {
+
+struct Hit {
+    float t, u, v;
+    uint32_t geom, prim;
+};
+
+struct RayPre {
+    float org[3], dir[3];
+    float inv[3];
+    int kx, ky, kz;
+    float sx, sy, sz;
+    uint32_t octant;
+};
+
+TR_HD RayPre ray_setup(const float org[3], const float dir[3])
+{
+    RayPre r;
+    for (int a = 0; a < 3; ++a) {
+        r.org[a] = org[a];
+        r.dir[a] = dir[a];
+        float d = dir[a];
+        if (fabsf(d) < 1e-20f) d = d < 0.0f ? -1e-20f : 1e-20f;
+        r.inv[a] = 1.0f / d;
+    }
+    float ax = fabsf(dir[0]), ay = fabsf(dir[1]), az = fabsf(dir[2]);
+    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
+    r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
+    if (dir[r.kz] < 0.0f) { int t = r.kx; r.kx = r.ky; r.ky = t; }
+    r.sx = dir[r.kx] / dir[r.kz];
+    r.sy = dir[r.ky] / dir[r.kz];
+    r.sz = 1.0f / dir[r.kz];
+    r.octant = (dir[0] < 0.0f ? 1u : 0u) | (dir[1] < 0.0f ? 2u : 0u) | (dir[2] < 0.0f ? 4u : 0u);
+    return r;
+}
+
+TR_HD float pick(const float* a, int k) { return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]); }
+
+TR_HD bool watertight_triangle(const RayPre& r, const float* v0, const float* v1, const float* v2, bool two_sided,
+                               float tnear, float tfar, float* t_out, float* u_out, float* v_out)
+{
+    float A[3] = {v0[0] - r.org[0], v0[1] - r.org[1], v0[2] - r.org[2]};
+    float B[3] = {v1[0] - r.org[0], v1[1] - r.org[1], v1[2] - r.org[2]};
+    float Cc[3] = {v2[0] - r.org[0], v2[1] - r.org[1], v2[2] - r.org[2]};
+    float Akz = pick(A, r.kz), Bkz = pick(B, r.kz), Ckz = pick(Cc, r.kz);
+    float Ax = pick(A, r.kx) - r.sx * Akz;
+    float Ay = pick(A, r.ky) - r.sy * Akz;
+    float Bx = pick(B, r.kx) - r.sx * Bkz;
+    float By = pick(B, r.ky) - r.sy * Bkz;
+    float Cx = pick(Cc, r.kx) - r.sx * Ckz;
+    float Cy = pick(Cc, r.ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if (two_sided) {
+        if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    } else {
+        if (U < 0.0f || V < 0.0f || W < 0.0f) return false;
+    }
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.sz * Akz;
+    float Bz = r.sz * Bkz;
+    float Cz = r.sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t > tnear) || !(t < tfar)) return false;
+    if (two_sided && t < 1e-3f) return false;
+    *t_out = t;
+    *u_out = V * rcp;
+    *v_out = W * rcp;
+    return true;
+}
+
+TR_HD bool sphere_hit(const RayPre& r, const float* c, float radius, float tnear, float tfar, float* t_out)
+{
+    float dc[3] = {c[0] - r.org[0], c[1] - r.org[1], c[2] - r.org[2]};
+    float b = dc[0] * r.dir[0] + dc[1] * r.dir[1] + dc[2] * r.dir[2];
+    float cc = (dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) - radius * radius;
+    float D = b * b - cc;
+    if (D < 0.0f || b < 0.0f) return false;
+    float sqrt_D = sqrtf(D);
+    float t = b + (2.0f * (float)(b < sqrt_D) - 1.0f) * sqrt_D;
+    float n[3] = {t * r.dir[0] - dc[0], t * r.dir[1] - dc[1], t * r.dir[2] - dc[2]};
+    float l = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l != 0.0f) {
+        float rl = 1.0f / l;
+        n[0] *= rl; n[1] *= rl; n[2] *= rl;
+    }
+    if (n[0] * r.dir[0] + n[1] * r.dir[1] + n[2] * r.dir[2] > 0.0f) return false;
+    if (!(t > tnear) || !(t < tfar)) return false;
+    *t_out = t;
+    return true;
+}
+
+TR_HD bool hit_better(float t, uint32_t g, uint32_t p, const Hit& b)
+{
+    if (b.geom == TR_INVALID_GEOMETRY) return true;
+    if (t != b.t) return t < b.t;
+    if (g != b.geom) return g < b.geom;
+    return p < b.prim;
+}
+
+TR_HD void test_prim(const PrimRec& p, const RayPre& r, float tnear, float tfar, Hit* best)
+{
+    float t, u = 0.0f, v = 0.0f;
+    bool hit;
+    if (p.flags & PRIM_SPHERE) hit = sphere_hit(r, p.v0, p.v1[0], tnear, tfar, &t);
+    else hit = watertight_triangle(r, p.v0, p.v1, p.v2, (p.flags & PRIM_TWO_SIDED) != 0, tnear, tfar, &t, &u, &v);
+    if (hit && hit_better(t, p.geom, p.prim, *best)) {
+        best->t = t; best->u = u; best->v = v; best->geom = p.geom; best->prim = p.prim;
+    }
+}
+
+TR_HD float exp2i(int e) { return tr_u2f((uint32_t)(e + 127) << 23); }
+
+/* Slab test of the 8 children of a node; returns the hit mask in slot
+ * order.  tmax is the current far bound (closest hit so far or tfar). */
+TR_HD uint32_t intersect_children(const Node8& n, const RayPre& r, float tnear, float tmax)
+{
+    uint32_t mask = 0;
+    float sx = exp2i(n.exp[0]), sy = exp2i(n.exp[1]), sz = exp2i(n.exp[2]);
+    for (int s = 0; s < 8; ++s) {
+        if (!((n.imask >> s) & 1u) && n.meta[s] == 0) continue;
+        float lx = n.origin[0] + (float)n.qlo[0][s] * sx, hx = n.origin[0] + (float)n.qhi[0][s] * sx;
+        float ly = n.origin[1] + (float)n.qlo[1][s] * sy, hy = n.origin[1] + (float)n.qhi[1][s] * sy;
+        float lz = n.origin[2] + (float)n.qlo[2][s] * sz, hz = n.origin[2] + (float)n.qhi[2][s] * sz;
+        float tx0 = (lx - r.org[0]) * r.inv[0], tx1 = (hx - r.org[0]) * r.inv[0];
+        float ty0 = (ly - r.org[1]) * r.inv[1], ty1 = (hy - r.org[1]) * r.inv[1];
+        float tz0 = (lz - r.org[2]) * r.inv[2], tz1 = (hz - r.org[2]) * r.inv[2];
+        float t0 = fmax_(fmax_(fmin_(tx0, tx1), fmin_(ty0, ty1)), fmax_(fmin_(tz0, tz1), tnear));
+        float t1 = fmin_(fmin_(fmax_(tx0, tx1), fmax_(ty0, ty1)), fmin_(fmax_(tz0, tz1), tmax));
+        t1 = t1 * 1.00000095f; /* 1 + 2^-20: keeps boxes touching a hit at t */
+        if (t0 <= t1) mask |= 1u << s;
+    }
+    return mask;
+}
+
+/* Permutes a slot-order mask into visiting order: bit i <-> slot i ^ octant. */
+TR_HD uint32_t to_visit_order(uint32_t slot_mask, uint32_t oct)
+{
+    uint32_t m = 0;
+    for (uint32_t s = 0; s < 8; ++s)
+        if ((slot_mask >> s) & 1u) m |= 1u << (s ^ oct);
+    return m;
+}
+
+TR_HD uint32_t popc8(uint32_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint32_t)__popc(x);
+#else
+    return (uint32_t)__builtin_popcount(x);
+#endif
+}
+TR_HD uint32_t ctz32(uint32_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint32_t)(__ffs(x) - 1);
+#else
+    return (uint32_t)__builtin_ctz(x);
+#endif
+}
+
+/* Stack of node groups.  An entry is (child_base, visit-order hit mask |
+ * imask << 8); its depth is bounded by the tree depth.  The kernels keep
+ * the first entries in LDS and spill deeper ones to global memory; this
+ * host/device template takes any stack object with push/pop/empty. */
+template <class Stack>
+TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const RayPre& r, float tnear, float tfar,
+                           Stack& stack, uint32_t* node_visits, uint32_t* prim_tests)
+{
+    Hit best;
+    best.t = tfar; best.u = 0.0f; best.v = 0.0f; best.geom = TR_INVALID_GEOMETRY; best.prim = TR_INVALID_GEOMETRY;
+    uint32_t oct = r.octant;
+    /* virtual root group: node 0 sits in slot 0 of an imask=1 group */
+    stack.push(0u, (1u << oct) | (1u << 8));
+    while (!stack.empty()) {
+        uint32_t base, masks;
+        stack.pop(&base, &masks);
+        uint32_t vis = masks & 0xffu;
+        uint32_t imask = masks >> 8;
+        uint32_t i = ctz32(vis);
+        vis &= vis - 1u;
+        if (vis) stack.push(base, vis | (imask << 8));
+        uint32_t slot = i ^ oct;
+        uint32_t node_index = base + popc8(imask & ((1u << slot) - 1u));
+        const Node8& n = nodes[node_index];
+        if (node_visits) ++*node_visits;
+        float tmax = best.geom == TR_INVALID_GEOMETRY ? tfar : best.t;
+        uint32_t hit = intersect_children(n, r, tnear, tmax);
+        uint32_t leaves = hit & ~(uint32_t)n.imask;
+        uint32_t inner = hit & (uint32_t)n.imask;
+        uint32_t lv = to_visit_order(leaves, oct);
+        while (lv) {
+            uint32_t li = ctz32(lv);
+            lv &= lv - 1u;
+            uint32_t s = li ^ oct;
+            uint32_t meta = n.meta[s];
+            uint32_t first = n.prim_base + (meta & 31u);
+            uint32_t count = meta >> 5;
+            for (uint32_t k = 0; k < count; ++k) {
+                if (prim_tests) ++*prim_tests;
+                test_prim(prims[first + k], r, tnear, tfar, &best);
+            }
+        }
+        if (inner) stack.push(n.child_base, to_visit_order(inner, oct) | ((uint32_t)n.imask << 8));
+    }
+    if (best.geom == TR_INVALID_GEOMETRY) best.t = tfar;
+    return best;
+}
+
+/* A plain array stack for host-side use. */
+struct HostStack {
+    uint32_t base[256];
+    uint32_t mask[256];
+    int sp = 0;
+    TR_HD void push(uint32_t b, uint32_t m) { base[sp] = b; mask[sp] = m; ++sp; }
+    TR_HD void pop(uint32_t* b, uint32_t* m) { --sp; *b = base[sp]; *m = mask[sp]; }
+    TR_HD bool empty() const { return sp == 0; }
+};
+
+}  // namespace trk
+
+#endif

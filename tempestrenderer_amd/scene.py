@@ -1,0 +1,249 @@
+"""Host-side mirror of the reference ray-trace scene API over the C ABI.
+
+`RayTracerScene` keeps the method names and argument meaning of the
+reference classes RayTracerScene (include/tempest/graphics/ray-tracing/
+ray-tracing.hh:241-395) and RayTracerCuda (include/tempest/compute/
+ray-tracing-cuda.hh:80-172), so code written against them reads the same.
+Every call goes through libtempest_rt.so; failures raise TempestError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import check
+
+_f = ctypes.c_float
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(_f))
+
+
+class Material:
+    """RTMaterial / RTMicrofacetMaterial / RTSGGXSurface as one record."""
+
+    def __init__(self, model, **fields):
+        self.rec = _abi.tr_material()
+        check(_abi.load().tr_material_init(ctypes.byref(self.rec), int(model)))
+        for k, v in fields.items():
+            self.set(k, v)
+
+    def set(self, name, value):
+        cur = getattr(self.rec, name)
+        if hasattr(cur, "__len__"):
+            for i, x in enumerate(value):
+                cur[i] = float(x)
+        else:
+            setattr(self.rec, name, value)
+        return self
+
+    def setup(self):
+        """RTMaterial::setup (ray-tracing.cc:80-87)."""
+        check(_abi.load().tr_material_setup(ctypes.byref(self.rec)))
+        return self
+
+
+class Submesh:
+    """RTSubmesh (ray-tracing-common.hh:240-247)."""
+
+    def __init__(self, material, vertex_count, vertex_offset=0, base_index=0, stride=24):
+        self.material = material
+        self.vertex_count = int(vertex_count)
+        self.vertex_offset = int(vertex_offset)
+        self.base_index = int(base_index)
+        self.stride = int(stride)
+
+
+def make_light(kind, radiance=(0, 0, 0), direction=(0, 0, 1), position=(0, 0, 0), center=(0, 0, 0), radius=0.0):
+    rec = _abi.tr_light()
+    rec.type = int(kind)
+    for i in range(3):
+        rec.radiance[i] = radiance[i]
+        rec.direction[i] = direction[i]
+        rec.position[i] = position[i]
+        rec.sphere_center[i] = center[i]
+    rec.sphere_radius = radius
+    return rec
+
+
+def submesh_array(submeshes):
+    arr = (_abi.tr_submesh * max(1, len(submeshes)))()
+    for i, sm in enumerate(submeshes):
+        arr[i].material = ctypes.pointer(sm.material.rec)
+        arr[i].vertex_count = sm.vertex_count
+        arr[i].vertex_offset = sm.vertex_offset
+        arr[i].base_index = sm.base_index
+        arr[i].stride = sm.stride
+    return arr
+
+
+class RayTracerScene:
+    """The MI355X tracer behind the reference scene API."""
+
+    def __init__(self, width, height, view_proj_inv, path_pool_size=0):
+        self.lib = _abi.load()
+        vpi = np.ascontiguousarray(np.asarray(view_proj_inv, dtype=np.float32).reshape(16))
+        settings = _abi.tr_settings(32 << 20, int(path_pool_size))
+        h = ctypes.c_void_p()
+        check(self.lib.tr_scene_create(int(width), int(height), _fptr(vpi), ctypes.byref(settings), ctypes.byref(h)))
+        self.h = h
+        self.width, self.height = int(width), int(height)
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.tr_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    make_light = staticmethod(make_light)
+
+    # setters (ray-tracing.hh:305-321)
+    def setChunkSize(self, n): check(self.lib.tr_scene_set_chunk_size(self.h, int(n)))
+    def setSamplesLocalAreaLight(self, n): check(self.lib.tr_scene_set_samples_local_area_light(self.h, int(n)))
+    def setSamplesGlobalIllumination(self, n): check(self.lib.tr_scene_set_samples_global_illumination(self.h, int(n)))
+    def setSamplesCamera(self, n): check(self.lib.tr_scene_set_samples_camera(self.h, int(n)))
+    def setMaxRayDepth(self, n): check(self.lib.tr_scene_set_max_ray_depth(self.h, int(n)))
+    def setRussianRoulette(self, r): check(self.lib.tr_scene_set_russian_roulette(self.h, float(r)))
+    def setPicturePostProcess(self, m): check(self.lib.tr_scene_set_picture_post_process(self.h, int(m)))
+    def setTransparentBackground(self, e): check(self.lib.tr_scene_set_transparent_background(self.h, int(bool(e))))
+    def setRenderMode(self, m): check(self.lib.tr_scene_set_render_mode(self.h, int(m)))
+    def repaint(self): check(self.lib.tr_scene_repaint(self.h))
+
+    def setBackgroundSpectrum(self, rgb):
+        a = np.asarray(rgb, dtype=np.float32)
+        check(self.lib.tr_scene_set_background_spectrum(self.h, _fptr(a)))
+
+    # additions beyond the reference
+    def setSeed(self, seed): check(self.lib.tr_scene_set_seed(self.h, int(seed)))
+    def setEstimator(self, mode): check(self.lib.tr_scene_set_estimator(self.h, int(mode)))
+    def setFrameIndex(self, f): check(self.lib.tr_scene_set_frame_index(self.h, int(f)))
+    def setCounting(self, on): check(self.lib.tr_scene_set_counting(self.h, int(bool(on))))
+
+    # lights (ray-tracing.hh:326-331)
+    def addLightSource(self, light):
+        idx = ctypes.c_uint32()
+        check(self.lib.tr_scene_add_light_source(self.h, ctypes.byref(light), ctypes.byref(idx)))
+        return idx.value
+
+    def updateLightSource(self, index, light):
+        check(self.lib.tr_scene_update_light_source(self.h, int(index), ctypes.byref(light)))
+
+    def clearLightSources(self):
+        check(self.lib.tr_scene_clear_light_sources(self.h))
+
+    def addSphereLightSource(self, light):
+        gid = ctypes.c_uint32()
+        check(self.lib.tr_scene_add_sphere_light_source(self.h, ctypes.byref(light), ctypes.byref(gid)))
+        light.geometry_id = gid.value
+        return gid.value
+
+    # geometry
+    def addTriangleMesh(self, world, submeshes, tris, verts, two_sided=False):
+        w = np.ascontiguousarray(np.asarray(world, dtype=np.float32).reshape(16))
+        t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).reshape(-1))
+        v = np.ascontiguousarray(verts)
+        arr = submesh_array(submeshes)
+        opts = _abi.tr_mesh_options(1 if two_sided else 0)
+        ids = (ctypes.c_uint64 * max(1, len(submeshes)))()
+        check(self.lib.tr_scene_add_triangle_mesh(
+            self.h, _fptr(w), len(submeshes), arr, t.size, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            v.nbytes, v.ctypes.data_as(ctypes.c_void_p), ctypes.byref(opts), ids))
+        self._keep.append(submeshes)
+        return [ids[i] for i in range(len(submeshes))]
+
+    def addSphere(self, center, radius, material):
+        c = np.asarray(center, dtype=np.float32)
+        gid = ctypes.c_uint64()
+        check(self.lib.tr_scene_add_sphere(self.h, _fptr(c), float(radius), ctypes.byref(material.rec),
+                                           ctypes.byref(gid)))
+        self._keep.append(material)
+        return gid.value
+
+    def commitScene(self):
+        check(self.lib.tr_scene_commit(self.h))
+
+    def buildHostOnly(self):
+        """Builds the BVH on the host only (no device); for CPU-side checks."""
+        check(self.lib.tr_debug_build_scene_host(self.h))
+
+    def info(self):
+        inf = _abi.tr_scene_info()
+        check(self.lib.tr_scene_get_info(self.h, ctypes.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in inf._fields_}
+
+    # rendering
+    def drawOnce(self):
+        check(self.lib.tr_scene_draw_once(self.h))
+
+    def draw(self, width, height, view_proj_inv):
+        vpi = np.ascontiguousarray(np.asarray(view_proj_inv, dtype=np.float32).reshape(16))
+        check(self.lib.tr_scene_draw(self.h, int(width), int(height), _fptr(vpi)))
+        self.width, self.height = int(width), int(height)
+
+    def frameWait(self):
+        check(self.lib.tr_scene_frame_wait(self.h))
+
+    def backbuffer(self):
+        n = self.width * self.height
+        out = np.zeros(n, dtype=np.uint32)
+        check(self.lib.tr_scene_read_backbuffer(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n))
+        return out.reshape(self.height, self.width)
+
+    def accumulation(self):
+        """(per-pixel RGB sums as float64 [H, W, 3], samples per pixel)."""
+        n = self.width * self.height
+        out = np.zeros(n * 3, dtype=np.float64)
+        spp = ctypes.c_uint64()
+        check(self.lib.tr_scene_read_accum(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                           ctypes.byref(spp)))
+        return out.reshape(self.height, self.width, 3), spp.value
+
+    def traceRays(self, rays):
+        """rays: float32 [N, 8] = org xyz, tnear, dir xyz, tfar -> structured hits."""
+        r = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(-1, 8))
+        n = r.shape[0]
+        hits = np.zeros(n, dtype=HIT_DTYPE)
+        check(self.lib.tr_scene_trace_rays(self.h, n, r.ctypes.data_as(ctypes.POINTER(_abi.tr_ray)),
+                                           hits.ctypes.data_as(ctypes.POINTER(_abi.tr_hit))))
+        return hits
+
+    def traceRaysHost(self, rays):
+        """The same closest-hit query through the host copy of the BVH8."""
+        r = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(-1, 8))
+        n = r.shape[0]
+        hits = np.zeros(n, dtype=HIT_DTYPE)
+        nv, npt = ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.lib.tr_debug_trace_rays_host(self.h, n, r.ctypes.data_as(ctypes.POINTER(_abi.tr_ray)),
+                                                hits.ctypes.data_as(ctypes.POINTER(_abi.tr_hit)),
+                                                ctypes.byref(nv), ctypes.byref(npt)))
+        return hits, nv.value, npt.value
+
+    def bvhViolations(self):
+        v = ctypes.c_uint64()
+        check(self.lib.tr_debug_bvh_check(self.h, ctypes.byref(v)))
+        return v.value
+
+    def renderPaths(self, pixels, samples_camera):
+        px = np.ascontiguousarray(np.asarray(pixels, dtype=np.uint32))
+        n = px.size * samples_camera
+        rgb = np.zeros(n * 3, dtype=np.float32)
+        hsh = np.zeros(n, dtype=np.uint32)
+        check(self.lib.tr_scene_render_paths(self.h, px.size, px.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                             _fptr(rgb), hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return rgb.reshape(n, 3), hsh
+
+    def stats(self):
+        st = _abi.tr_stats()
+        check(self.lib.tr_scene_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+HIT_DTYPE = np.dtype([("t", np.float32), ("u", np.float32), ("v", np.float32), ("geom", np.uint32),
+                      ("prim", np.uint32)])

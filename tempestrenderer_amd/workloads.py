@@ -1,0 +1,288 @@
+"""Synthetic workloads of the benchmark configurations, as scene descriptions.
+
+A `SceneDesc` is plain data (numpy arrays, material dictionaries, lights,
+camera, render settings).  `populate(target)` replays it through the
+reference scene API method names, so the same description feeds the GPU
+tracer (`RayTracerScene`) and any other object with those methods.
+
+Workloads:
+  box(...)         5 axis-aligned quads (10 triangles) forming an open box,
+                   2 GGX-conductor spheres r=0.3, a 0.25x0.25 emissive quad
+                   L=15 in the same mesh, Lambertian walls with albedo
+                   U[0.2, 0.8]; 256x256, 16 camera samples, depth 4.
+  heightfield(...) an n x n vertex grid of fBm value noise (6 octaves,
+                   amplitude 0.1; 2*(n-1)^2 triangles) + 64 spheres with GGX,
+                   Beckmann and SGGX-surface materials; constant environment
+                   light 1.0; 1920x1080, 64 camera samples, depth 8.
+Scene randomness comes from numpy's PCG64 seeded with `seed`; the path
+tracer's own random numbers come from Philox keyed by the render seed.
+"""
+import math
+
+import numpy as np
+
+# IlluminationModel values (ray-tracing-common.hh:68-105)
+EMISSIVE = 0
+GGX = 11
+GGX_CONDUCTOR = 12
+GGX_DIELECTRIC = 13
+SGGX_SURFACE = 24
+BECKMANN = 28
+BECKMANN_CONDUCTOR = 29
+
+LIGHT_DIRECTIONAL = 0
+LIGHT_SPHERE_AREA = 1
+LIGHT_POINT = 3
+
+
+def alpha_to_specular_power(alpha):
+    """SpecularPowerToGeometryFactorSq(n) = 2/(n+2) = alpha^2 (microfacet-common.hh:109-112)."""
+    return 2.0 / (alpha * alpha) - 2.0
+
+
+def perspective_view_proj_inv(eye, target, up, fovy_deg, aspect, near=0.1, far=100.0):
+    """Inverse of a GL-style projection*view, as 16 column-major floats."""
+    eye, target, up = (np.asarray(v, dtype=np.float64) for v in (eye, target, up))
+    f = target - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    view[:3, 3] = -view[:3, :3] @ eye
+    t = 1.0 / math.tan(math.radians(fovy_deg) / 2.0)
+    proj = np.zeros((4, 4))
+    proj[0, 0] = t / aspect
+    proj[1, 1] = t
+    proj[2, 2] = (far + near) / (near - far)
+    proj[2, 3] = 2.0 * far * near / (near - far)
+    proj[3, 2] = -1.0
+    inv = np.linalg.inv(proj @ view)
+    return np.ascontiguousarray(inv.T.reshape(16).astype(np.float32))  # column-major
+
+
+IDENTITY = np.eye(4, dtype=np.float32).reshape(16)
+
+
+class SceneDesc:
+    def __init__(self, name, width, height, vpi):
+        self.name = name
+        self.width, self.height = int(width), int(height)
+        self.vpi = np.asarray(vpi, dtype=np.float32)
+        self.meshes = []       # (verts float32 [V, stride/4], tris int32 [T*3], [(material, first_index, count)], two_sided)
+        self.spheres = []      # (center, radius, material)
+        self.lights = []       # dict(kind=..., ...)
+        self.sphere_lights = []
+        self.settings = dict(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=1.0,
+                             background=(0.0, 0.0, 0.0), seed=1)
+
+    @property
+    def pixels(self):
+        return self.width * self.height
+
+    def populate(self, target, material_factory):
+        """Replays the scene into `target` (RayTracerScene-like); materials are
+        made by material_factory(model, fields) so each target gets its own."""
+        cache = {}
+
+        def mat(m):
+            key = id(m)
+            if key not in cache:
+                fields = {k: v for k, v in m.items() if k != "model"}
+                cache[key] = material_factory(m["model"], fields)
+            return cache[key]
+
+        st = self.settings
+        target.setSamplesCamera(st["samples_camera"])
+        target.setSamplesGlobalIllumination(st["samples_gi"])
+        target.setSamplesLocalAreaLight(st["samples_light"])
+        target.setMaxRayDepth(st["max_depth"])
+        target.setRussianRoulette(st["rr"])
+        target.setBackgroundSpectrum(st["background"])
+        target.setSeed(st["seed"])
+        from .scene import Submesh
+        for verts, tris, parts, two_sided in self.meshes:
+            stride = verts.shape[1] * 4
+            subs = [Submesh(mat(m), count, 0, first, stride) for (m, first, count) in parts]
+            target.addTriangleMesh(IDENTITY, subs, tris, verts, two_sided=two_sided)
+        for center, radius, m in self.spheres:
+            target.addSphere(center, radius, mat(m))
+        for L in self.sphere_lights:
+            target.addSphereLightSource(target.make_light(**L))
+        for L in self.lights:
+            target.addLightSource(target.make_light(**L))
+
+
+def _quad(center, u, v):
+    c, u, v = (np.asarray(x, dtype=np.float32) for x in (center, u, v))
+    n = np.cross(u, v)
+    n = n / np.linalg.norm(n)
+    p = [c - u - v, c + u - v, c + u + v, c - u + v]
+    verts = np.array([np.concatenate([q, n]) for q in p], dtype=np.float32)
+    tris = np.array([0, 1, 2, 0, 2, 3], dtype=np.int32)
+    return verts, tris
+
+
+def box(width=256, height=256, samples_camera=16, max_depth=4, seed=1):
+    """Box scene: open-front Cornell-style box, two GGX conductor spheres,
+    a small emissive quad under the ceiling."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vpi = perspective_view_proj_inv((0.0, 0.0, 3.5), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, width / height)
+    d = SceneDesc("box", width, height, vpi)
+    X, Y, Z = (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)
+    quads = [
+        ((0, -1, 0), Z, X),   # floor, normal +y
+        ((0, 1, 0), X, Z),    # ceiling, normal -y
+        ((0, 0, -1), X, Y),   # back, normal +z
+        ((-1, 0, 0), Y, Z),   # left, normal +x
+        ((1, 0, 0), Z, Y),    # right, normal -x
+    ]
+    verts, tris, parts = [], [], []
+    for i, (c, u, v) in enumerate(quads):
+        qv, qt = _quad(c, u, v)
+        albedo = rng.uniform(0.2, 0.8, size=3)
+        m = dict(model=GGX, diffuse=tuple(float(a) for a in albedo), specular=(0.0, 0.0, 0.0),
+                 specular_power=(1.0, 1.0))
+        parts.append((m, len(tris) * 6, 6))
+        tris.append(qt + 4 * i)
+        verts.append(qv)
+    h = 0.125
+    ev, et = _quad((0.0, 0.999, 0.0), (h, 0.0, 0.0), (0.0, 0.0, h))  # x cross z = -y: faces down
+    emitter = dict(model=EMISSIVE, diffuse=(15.0, 15.0, 15.0))
+    parts.append((emitter, len(tris) * 6, 6))
+    tris.append(et + 4 * len(quads))
+    verts.append(ev)
+    d.meshes.append((np.concatenate(verts), np.concatenate(tris), parts, False))
+    conductor = dict(model=GGX_CONDUCTOR, diffuse=(0.0, 0.0, 0.0), specular=(1.0, 1.0, 1.0),
+                     specular_power=(alpha_to_specular_power(0.2),) * 2, fresnel=(0.2, 3.0))
+    d.spheres.append(((-0.45, -0.7, 0.0), 0.3, conductor))
+    d.spheres.append(((0.45, -0.7, 0.0), 0.3, conductor))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(0.0, 0.0, 0.0), seed=1)
+    return d
+
+
+def value_noise_fbm(n, octaves=6, amplitude=0.1, seed=1):
+    """fBm of bilinear value noise on an n x n grid over [-1, 1]^2
+    (lacunarity 2, gain 0.5), heights in roughly [-amplitude, amplitude]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs = np.linspace(0.0, 1.0, n)
+    h = np.zeros((n, n), dtype=np.float64)
+    amp, freq, norm = 1.0, 4, 0.0
+    for _ in range(octaves):
+        lattice = rng.uniform(-1.0, 1.0, size=(freq + 1, freq + 1))
+        g = xs * freq
+        i0 = np.minimum(g.astype(np.int64), freq - 1)
+        f = g - i0
+        f = f * f * (3.0 - 2.0 * f)
+        a = lattice[i0][:, i0]
+        b = lattice[i0 + 1][:, i0]
+        c = lattice[i0][:, i0 + 1]
+        e = lattice[i0 + 1][:, i0 + 1]
+        fy = f[:, None]
+        fx = f[None, :]
+        h += amp * ((a * (1 - fy) + b * fy) * (1 - fx) + (c * (1 - fy) + e * fy) * fx)
+        norm += amp
+        amp *= 0.5
+        freq *= 2
+    return (h / norm * amplitude).astype(np.float32)
+
+
+def heightfield_mesh(n, amplitude=0.1, seed=1):
+    """Vertices (position + normal, stride 24) and CCW-up triangles of an
+    n x n heightfield y = h(x, z) over [-1, 1]^2."""
+    h = value_noise_fbm(n, amplitude=amplitude, seed=seed)
+    coords = np.linspace(-1.0, 1.0, n, dtype=np.float32)
+    zz, xx = np.meshgrid(coords, coords, indexing="ij")  # row = z, col = x
+    step = 2.0 / (n - 1)
+    dhdx = np.gradient(h, step, axis=1)
+    dhdz = np.gradient(h, step, axis=0)
+    nrm = np.stack([-dhdx, np.ones_like(h), -dhdz], axis=-1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    verts = np.concatenate([np.stack([xx, h, zz], axis=-1), nrm], axis=-1).reshape(-1, 6).astype(np.float32)
+    r, c = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    v00 = (r * n + c).reshape(-1)
+    v01 = v00 + 1
+    v10 = v00 + n
+    v11 = v10 + 1
+    # (v00, v10, v11): (dz, 0) x (dz, dx) -> normal +y
+    tris = np.stack([v00, v10, v11, v00, v11, v01], axis=-1).reshape(-1).astype(np.int32)
+    return verts, tris, h
+
+
+def _random_quaternion(rng):
+    q = rng.normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+def heightfield(width=1920, height=1080, samples_camera=64, max_depth=8, grid=1024, n_spheres=64, seed=1):
+    rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    verts, tris, h = heightfield_mesh(grid, seed=seed)
+    vpi = perspective_view_proj_inv((0.0, 1.1, 2.1), (0.0, -0.1, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
+    d = SceneDesc("heightfield", width, height, vpi)
+    alpha = float(rng.uniform(0.05, 0.6))
+    ground = dict(model=GGX, diffuse=(0.45, 0.40, 0.35), specular=(0.5, 0.5, 0.5),
+                  specular_power=(alpha_to_specular_power(alpha),) * 2, fresnel=(0.04, 0.0))
+    d.meshes.append((verts, tris, [(ground, 0, tris.size)], False))
+    n = grid
+    for i in range(n_spheres):
+        r = float(rng.uniform(0.02, 0.08))
+        x, z = (float(v) for v in rng.uniform(-0.9, 0.9, size=2))
+        ci = min(n - 1, max(0, int(round((x + 1.0) * 0.5 * (n - 1)))))
+        ri = min(n - 1, max(0, int(round((z + 1.0) * 0.5 * (n - 1)))))
+        y = float(h[ri, ci]) + r
+        kind = i % 3
+        a = float(rng.uniform(0.05, 0.6))
+        col = tuple(float(v) for v in rng.uniform(0.2, 0.8, size=3))
+        if kind == 0:
+            m = dict(model=BECKMANN, diffuse=tuple(0.5 * c for c in col), specular=(0.5, 0.5, 0.5),
+                     specular_power=(alpha_to_specular_power(a),) * 2, fresnel=(0.04, 0.0))
+        elif kind == 1:
+            sig = tuple(float(v) for v in rng.uniform(0.2, 0.8, size=2))
+            m = dict(model=SGGX_SURFACE, diffuse=tuple(0.6 * c for c in col), specular=(0.4, 0.4, 0.4),
+                     specular_power=sig, fresnel=(1.0, 0.0),
+                     sggx_basis=tuple(float(v) for v in _random_quaternion(rng)))
+        else:
+            m = dict(model=GGX_CONDUCTOR, diffuse=(0.0, 0.0, 0.0), specular=col,
+                     specular_power=(alpha_to_specular_power(a),) * 2, fresnel=(0.2, 3.0))
+        d.spheres.append(((x, y, z), r, m))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(1.0, 1.0, 1.0), seed=1)
+    return d
+
+
+def random_rays(desc, n, seed=7, kind="sphere"):
+    """A fixed ray set: origins uniform in the scene's box [-1.2, 1.2]^3 (or
+    the camera for kind='camera'), directions uniform on the sphere."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    org = rng.uniform(-1.2, 1.2, size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), dtype=np.float32)
+    rays[:, 0:3] = org
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = v
+    rays[:, 7] = np.inf
+    return rays
+
+
+def primary_rays(desc, stride=1):
+    """Pixel-centre primary rays of the description's camera (one per pixel
+    when stride=1), computed in float64 from the inverse view-projection."""
+    W, H = desc.width, desc.height
+    m = desc.vpi.astype(np.float64).reshape(4, 4).T
+    ys, xs = np.mgrid[0:H:stride, 0:W:stride]
+    sx = 2.0 * (xs.reshape(-1) + 0.5) / W - 1.0
+    sy = 2.0 * (ys.reshape(-1) + 0.5) / H - 1.0
+    p0 = m @ np.stack([sx, sy, -np.ones_like(sx), np.ones_like(sx)])
+    p1 = m @ np.stack([sx, sy, np.ones_like(sx), np.ones_like(sx)])
+    a = (p0[:3] / p0[3]).T
+    b = (p1[:3] / p1[3]).T
+    dvec = b - a
+    dvec /= np.linalg.norm(dvec, axis=1, keepdims=True)
+    rays = np.zeros((a.shape[0], 8), dtype=np.float32)
+    rays[:, 0:3] = a
+    rays[:, 4:7] = dvec
+    rays[:, 7] = np.inf
+    return rays

@@ -1,0 +1,23 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+def gpu_material(model, fields):
+    from tempestrenderer_amd.scene import Material
+    return Material(model, **fields).setup()
+
+
+@pytest.fixture(scope="session")
+def hip_lib():
+    from tempestrenderer_amd import _abi
+    return _abi.load()

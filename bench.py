@@ -1,0 +1,183 @@
+"""Benchmark of the path-tracer hot path on one MI355X.
+
+Workload (one step = one drawOnce frame): the seeded heightfield scene -
+a 1024 x 1024 vertex grid of fBm noise (2,093,058 triangles) plus 64
+spheres with GGX / Beckmann / SGGX-surface materials, constant environment
+light, 1920 x 1080 pixels, 64 camera samples per pixel, max depth 8, FP32
+paths, FP64 accumulation.  Inputs (scene, BVH, path pool) are resident in
+HBM before the timed region.
+
+Prints one JSON line: Msamples/s over the timed steps, the roofline of the
+dominant kernel (closest-hit BVH traversal, HBM-bound; algorithmic bytes per
+ray segment = 128 B per node fetched + 48 B per primitive tested + 64 B of
+ray state and hit, counted by the kernels in a separate counting frame),
+and the CPU oracle timed on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_material(model, fields):
+    from tempestrenderer_amd.scene import Material
+    return Material(model, **fields).setup()
+
+
+def load_traffic(path):
+    """Measured HBM bytes per trace launch from a committed rocprofv3 PMC
+    summary (profiles/), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(desc, rows, threads):
+    import oracle
+    o = oracle.OracleScene(desc.width, desc.height, desc.vpi)
+    desc.populate(o, oracle.material)
+    o.commitScene()
+    _, _, secs = o.renderRows(0, rows, frame=0, fp64=False, threads=threads)
+    samples = rows * desc.width * desc.settings["samples_camera"]
+    return samples / secs / 1e6, secs, samples
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="reduced scene for a quick check (not the metric)")
+    ap.add_argument("--cpu-rows", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the tracer runs on a single MI355X in this build "
+              "(multi-GPU sharding by sample index is described, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    from tempestrenderer_amd import workloads
+    from tempestrenderer_amd.scene import RayTracerScene
+
+    if args.small:
+        desc = workloads.heightfield(480, 270, samples_camera=16, grid=256)
+    else:
+        desc = workloads.heightfield()
+    st = desc.settings
+    t0 = time.time()
+    gpu = RayTracerScene(desc.width, desc.height, desc.vpi)
+    desc.populate(gpu, gpu_material)
+    gpu.commitScene()
+    info = gpu.info()
+    log("scene: %d prims, %d BVH8 nodes (%.1f MB), depth %d, build %.0f ms, setup %.1f s" % (
+        info["primitives"], info["bvh_nodes"], info["bvh_bytes"] / 1e6, info["bvh_depth"], info["build_ms"],
+        time.time() - t0))
+
+    # counting frame (not timed): node / primitive visits for the roofline numerator
+    gpu.setCounting(True)
+    gpu.drawOnce()
+    cnt = gpu.stats()
+    gpu.setCounting(False)
+    gpu.repaint()
+    log("counting frame: %d segments, %.2f nodes/seg, %.2f prims/seg" % (
+        cnt["segments"], cnt["node_visits"] / max(1, cnt["segments"]), cnt["prim_tests"] / max(1, cnt["segments"])))
+
+    for _ in range(args.warmup):
+        gpu.drawOnce()
+    trace_ms = shade_ms = shadow_ms = other_ms = 0.0
+    launches = 0
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        gpu.drawOnce()
+        s = gpu.stats()
+        trace_ms += s["trace_ms"]
+        shade_ms += s["shade_ms"]
+        shadow_ms += s["shadow_ms"]
+        other_ms += s["other_ms"]
+        launches += s["trace_launches"]
+        log("step %d: %.1f ms (trace %.1f, shade %.1f, shadow %.1f)" % (
+            i, s["frame_ms"], s["trace_ms"], s["shade_ms"], s["shadow_ms"]))
+    elapsed = time.perf_counter() - t_start
+    samples = desc.pixels * st["samples_camera"]
+    value = samples * args.steps / elapsed / 1e6
+    ms_per_step = elapsed * 1e3 / args.steps
+
+    segs = cnt["segments"]
+    alg_bytes = 128.0 * cnt["node_visits"] + 48.0 * cnt["prim_tests"] + 64.0 * segs
+    trace_s = trace_ms / 1e3 / args.steps
+    achieved = alg_bytes / trace_s / 1e9 if trace_s > 0 else 0.0
+    traffic = load_traffic(os.path.join(ROOT, "profiles", "trace_kernel_heightfield_pmc.txt"))
+    acc, spp = gpu.accumulation()
+    img = acc / spp
+    log("image mean %s, finite %s" % (img.mean(axis=(0, 1)), bool(np.isfinite(img).all())))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu_value, cpu_secs, cpu_samples = cpu_baseline(desc, args.cpu_rows, threads)
+        cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": "rows 0-%d of the same frame (%d px x %d camera samples = %d paths), %.1f s" % (
+                   args.cpu_rows - 1, args.cpu_rows * desc.width, st["samples_camera"], cpu_samples, cpu_secs)}
+
+    out = {
+        "metric": "Msamples/s (plain path tracer, heightfield scene, 1 GPU)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded fBm heightfield + 64 spheres, Philox seed 1)",
+        "config": {
+            "workload": "heightfield %dx%d grid (%d tris) + 64 spheres, %dx%d, %d spp, depth %d, env light" % (
+                1024 if not args.small else 256, 1024 if not args.small else 256, info["primitives"] - 64,
+                desc.width, desc.height, st["samples_camera"], st["max_depth"]),
+            "samples_per_step": samples,
+            "estimator": "reference",
+        },
+        "mrays_per_s": round(segs / (ms_per_step / 1e3) / 1e6, 2),
+        "trace_ms_per_step": round(trace_ms / args.steps, 3),
+        "shade_ms_per_step": round(shade_ms / args.steps, 3),
+        "shadow_ms_per_step": round(shadow_ms / args.steps, 3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "kernel": "k_trace (closest hit, BVH8)",
+            "alg_bytes_per_step": alg_bytes,
+            "segments_per_step": segs,
+            "launches_per_step": launches // max(1, args.steps),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,6 +67,7 @@ def _declare(lib):
         "orc_add_sphere_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
         "orc_commit": (ctypes.c_int, [vp]),
         "orc_trace_rays": (None, [vp, sz, P(_abi.tr_ray), P(_abi.tr_hit), ctypes.c_int, ctypes.c_int]),
+        "orc_trace_path": (None, [vp, u32, u32]),
         "orc_render_paths": (None, [vp, sz, P(u32), u32, P(f), P(u32), ctypes.c_int]),
         "orc_render_rows": (ctypes.c_double, [vp, u32, u32, u32, ctypes.c_int, P(ctypes.c_double),
                                               P(ctypes.c_double), ctypes.c_int]),
@@ -219,6 +220,10 @@ class OracleScene:
         self.lib.orc_render_paths(self.h, px.size, px.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(frame),
                                   _fp(rgb), hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), threads)
         return rgb.reshape(n, 3), hsh
+
+    def tracePath(self, pixel, camera_sample):
+        """Prints every vertex of one camera sample to stderr."""
+        self.lib.orc_trace_path(self.h, int(pixel), int(camera_sample))
 
     def renderRows(self, row_begin, row_end, frame=0, fp64=False, threads=0):
         """Per-pixel sums and sums of squares over camera samples for a row

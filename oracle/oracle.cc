@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <chrono>
 #include <cstring>
 #include <thread>
@@ -37,6 +38,10 @@
 using namespace orc;
 
 namespace {
+
+/* When set, render_camera_sample prints every vertex of the path. */
+thread_local FILE* g_trace = nullptr;
+#define ORC_TRACE(...) do { if (g_trace) fprintf(g_trace, __VA_ARGS__); } while (0)
 
 enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2 };
 enum { STREAM_CAMERA = 0, STREAM_BSDF = 1, STREAM_LIGHT = 2, STREAM_RR = 3 };
@@ -344,12 +349,28 @@ template <class T> V3<T> vpos(const orc_scene* s, const Geom& g, uint32_t tri, i
 }
 
 /* Light sampling (ray-tracing-common.hh:567-647).  xi: triangle pick, ra, rb. */
+/* UniformSampleTriangleArea (sampling3.hh:214-227): direction, distance and
+ * (facing ? 1/projected area : 0). */
 template <class T>
-LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, const uint32_t xi_raw[4])
+void triangle_area_sample(V3<T> pos, V3<T> p0, V3<T> p1, V3<T> p2, T ra, T rb, V3<T>* dir, T* dist, T* pdf)
+{
+    T sqrt_r = xsqrt(ra);
+    V3<T> e0 = p1 - p0, e1 = p2 - p0;
+    V3<T> point = p0 + (sqrt_r * rb) * e0 + ((T)1 - sqrt_r) * e1;
+    V3<T> du = point - pos;
+    T len = length(du);
+    *dir = du / len;
+    *dist = len;
+    *pdf = dot(cross(e0, e1), *dir) < -(T)1e-6f ? (T)1 / projected_triangle_area(pos, p0, p1, p2) : (T)0;
+}
+
+/* Light sampling (ray-tracing-common.hh:567-647).  pick: the triangle
+ * selector word of mesh lights; ra, rb: the two sample coordinates. */
+template <class T>
+LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, uint32_t pick, T ra, T rb)
 {
     LightSample<T> r;
     const tr_light& l = L.src;
-    T ra = (T)orc_u01(xi_raw[1]), rb = (T)orc_u01(xi_raw[2]);
     if (l.type == TR_LIGHT_DIRECTIONAL) {
         r.dist = (T)INFINITY;
         r.dir = spec3<T>(l.direction);
@@ -407,16 +428,10 @@ LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, const
         /* MeshAreaSampleLightSource + UniformSampleTriangleArea
          * (ray-tracing-common.hh:622-647, sampling3.hh:214-227) */
         const Geom& g = s->geoms[L.geom];
-        uint32_t tri = (uint32_t)(((uint64_t)xi_raw[0] * g.tri_count) >> 32);
+        uint32_t tri = (uint32_t)(((uint64_t)pick * g.tri_count) >> 32);
         V3<T> p0 = vpos<T>(s, g, tri, 0), p1 = vpos<T>(s, g, tri, 1), p2 = vpos<T>(s, g, tri, 2);
-        T sqrt_r = xsqrt(ra);
-        V3<T> e0 = p1 - p0, e1 = p2 - p0;
-        V3<T> point = p0 + (sqrt_r * rb) * e0 + ((T)1 - sqrt_r) * e1;
-        V3<T> du = point - pos;
-        T len = length(du);
-        r.dir = du / len;
-        r.dist = len;
-        T pdf = dot(cross(e0, e1), r.dir) < -(T)1e-6f ? (T)1 / projected_triangle_area(pos, p0, p1, p2) : (T)0;
+        T pdf;
+        triangle_area_sample(pos, p0, p1, p2, ra, rb, &r.dir, &r.dist, &pdf);
         r.pdf = pdf / (T)g.tri_count;
         r.radiance = spec3<T>(s->materials[g.material].diffuse);
         r.geom = L.geom;
@@ -498,7 +513,7 @@ V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const
         for (uint32_t k = 0; k < samples; ++k) {
             uint32_t xi[4];
             rng.draw(vertex, STREAM_LIGHT, (li << 16) | k, xi);
-            LightSample<T> smp = sample_light(s, L, pos, xi);
+            LightSample<T> smp = sample_light(s, L, pos, xi[0], (T)orc_u01(xi[1]), (T)orc_u01(xi[2]));
             if (smp.pdf == (T)0) continue;
             lsd.wi = smp.dir;
             V3<T> cur = throughput;
@@ -608,12 +623,18 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             srng.draw(d, STREAM_BSDF, 0, xr);
             T xi[3] = {(T)orc_u01(xr[0]), (T)orc_u01(xr[1]), (T)orc_u01(xr[2])};
             bsdf_sample(st, &cur, xi);
+            ORC_TRACE("d=%u model=%u n=(%g %g %g) wo=(%g %g %g) xi=(%g %g %g) wi=(%g %g %g) pdf=%g\n", d,
+                      cur.mat->model, (double)cur.normal.x, (double)cur.normal.y, (double)cur.normal.z,
+                      (double)cur.wo.x, (double)cur.wo.y, (double)cur.wo.z, (double)xi[0], (double)xi[1],
+                      (double)xi[2], (double)cur.wi.x, (double)cur.wi.y, (double)cur.wi.z, (double)cur.pdf);
             if (cur.pdf == (T)0) break;
             T balance = (T)1;
             {
                 T total = total_balance(s, (T)ls, (T)ss, cur, cpos);
                 balance = total > (T)0 ? ((T)ss * cur.pdf) / total : (T)0;
                 ct *= bsdf_eval(cur) / cur.pdf;
+                ORC_TRACE("   balance=%g T=(%g %g %g) f=(%g %g %g)\n", (double)balance, (double)ct.x, (double)ct.y,
+                          (double)ct.z, (double)bsdf_eval(cur).x, (double)bsdf_eval(cur).y, (double)bsdf_eval(cur).z);
             }
             float o2[3] = {(float)cpos.x, (float)cpos.y, (float)cpos.z};
             float d2[3] = {(float)cur.wi.x, (float)cur.wi.y, (float)cur.wi.z};
@@ -624,6 +645,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
                 break;
             }
             hash_hit(&hash, hh);
+            ORC_TRACE("   hit geom=%u prim=%u t=%g\n", hh.geom, hh.prim, (double)hh.t);
             sample_geometry(s, hh, cpos, cur.wi, &cur);
             cpos = cpos + (T)hh.t * cur.wi;
             cur.position = cpos;
@@ -838,6 +860,15 @@ void orc_trace_rays(const orc_scene* s, size_t n, const tr_ray* rays, tr_hit* hi
     });
 }
 
+void orc_trace_path(const orc_scene* s, uint32_t pixel, uint32_t cs)
+{
+    g_trace = stderr;
+    uint32_t hash;
+    V3<float> c = render_camera_sample<float>(s, pixel, cs, &hash);
+    fprintf(stderr, "result (%g %g %g) hash %08x\n", (double)c.x, (double)c.y, (double)c.z, hash);
+    g_trace = nullptr;
+}
+
 void orc_render_paths(const orc_scene* s, size_t pixel_count, const uint32_t* pixels, uint32_t frame,
                       float* out_rgb, uint32_t* out_hash, int nthreads)
 {
@@ -884,3 +915,132 @@ double orc_render_rows(const orc_scene* s, uint32_t row_begin, uint32_t row_end,
 }
 
 }  // extern "C"
+
+/* ---------------- unit exports for the test suite ---------------- */
+
+extern "C" {
+
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { orc_philox4x32_10(ctr, key, out); }
+
+float orc_u01f(uint32_t x) { return orc_u01(x); }
+
+/* op: 0 sin(2 pi x), 1 cos(2 pi x), 2 log, 3 exp */
+void orc_fmath(int op, size_t n, const float* in, float* out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        float s, c;
+        switch (op) {
+        case 0: tr_sincos_turns(in[i], &s, &c); out[i] = s; break;
+        case 1: tr_sincos_turns(in[i], &s, &c); out[i] = c; break;
+        case 2: out[i] = tr_logf(in[i]); break;
+        default: out[i] = tr_expf(in[i]); break;
+        }
+    }
+}
+
+static Sample<float> unit_sample(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
+                                 const float* wi, float dir_density)
+{
+    Sample<float> s;
+    memset(&s, 0, sizeof(s));
+    s.mat = m;
+    s.normal = v3(n[0], n[1], n[2]);
+    s.tangent = t ? v3(t[0], t[1], t[2]) : v3(0.0f, 0.0f, 0.0f);
+    s.binormal = b ? v3(b[0], b[1], b[2]) : v3(0.0f, 0.0f, 0.0f);
+    s.wo = v3(wo[0], wo[1], wo[2]);
+    s.wi = wi ? v3(wi[0], wi[1], wi[2]) : v3(0.0f, 0.0f, 0.0f);
+    s.dir_density = dir_density;
+    return s;
+}
+
+/* BRDF*cos (out[0..2]) and PDF (out[3]) of a material; when dir_density < 0
+ * the material cache (SGGX surface density) is evaluated with wi = -wo. */
+void orc_bsdf_eval(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
+                   const float* wi, float dir_density, float* out)
+{
+    Sample<float> s = unit_sample(m, n, t, b, wo, wi, dir_density);
+    if (dir_density < 0.0f) {
+        Sample<float> c = s;
+        c.wi = -s.wo;
+        material_cache(&c);
+        s.dir_density = c.dir_density;
+    }
+    V3<float> f = bsdf_eval(s);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+    out[3] = bsdf_pdf(s);
+}
+
+/* Sampled direction (out[0..2]) and PDF (out[3]). */
+void orc_bsdf_sample(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
+                     const float* strata, const float* xi, float dir_density, float* out)
+{
+    Sample<float> s = unit_sample(m, n, t, b, wo, nullptr, dir_density);
+    Strata<float> st;
+    st.x = strata[0]; st.y = strata[1]; st.tx = strata[2]; st.ty = strata[3];
+    bsdf_sample(st, &s, xi);
+    out[0] = s.wi.x; out[1] = s.wi.y; out[2] = s.wi.z; out[3] = s.pdf;
+}
+
+float orc_fresnel_conductor(float eta, float k, float c) { return fresnel_conductor(eta, k, c); }
+float orc_fresnel_schlick(float r, float c) { return fresnel_schlick(r, c); }
+float orc_fresnel_dielectric(float eta, float ci, float ct) { return fresnel_dielectric(eta, ci, ct); }
+
+void orc_make_basis(const float* n, float* out9)
+{
+    M3<float> m = make_basis(v3(n[0], n[1], n[2]));
+    out9[0] = m.c0.x; out9[1] = m.c0.y; out9[2] = m.c0.z;
+    out9[3] = m.c1.x; out9[4] = m.c1.y; out9[5] = m.c1.z;
+    out9[6] = m.c2.x; out9[7] = m.c2.y; out9[8] = m.c2.z;
+}
+
+/* Sphere-light sample (SampleSphereArea) at pos: out = dir[3], dist, pdf. */
+void orc_sample_sphere_light(const float* pos, const float* c, float r, float ra, float rb, float* out)
+{
+    Light L;
+    memset(&L, 0, sizeof(L));
+    L.src.type = TR_LIGHT_SPHERE_AREA;
+    L.src.sphere_center[0] = c[0]; L.src.sphere_center[1] = c[1]; L.src.sphere_center[2] = c[2];
+    L.src.sphere_radius = r;
+    LightSample<float> smp = sample_light<float>(nullptr, L, v3(pos[0], pos[1], pos[2]), 0u, ra, rb);
+    out[0] = smp.dir.x; out[1] = smp.dir.y; out[2] = smp.dir.z; out[3] = smp.dist; out[4] = smp.pdf;
+}
+
+float orc_sphere_light_pdf(const float* pos, const float* dir, const float* c, float r)
+{
+    Light L;
+    memset(&L, 0, sizeof(L));
+    L.src.type = TR_LIGHT_SPHERE_AREA;
+    L.src.sphere_center[0] = c[0]; L.src.sphere_center[1] = c[1]; L.src.sphere_center[2] = c[2];
+    L.src.sphere_radius = r;
+    return light_pdf<float>(nullptr, L, v3(pos[0], pos[1], pos[2]), v3(dir[0], dir[1], dir[2]));
+}
+
+void orc_sample_triangle_area(const float* pos, const float* p0, const float* p1, const float* p2, float ra, float rb,
+                              float* out)
+{
+    V3<float> d;
+    float dist, pdf;
+    triangle_area_sample(v3(pos[0], pos[1], pos[2]), v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]),
+                         v3(p2[0], p2[1], p2[2]), ra, rb, &d, &dist, &pdf);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z; out[3] = dist; out[4] = pdf;
+}
+
+float orc_projected_triangle_area(const float* pos, const float* p0, const float* p1, const float* p2)
+{
+    return projected_triangle_area(v3(pos[0], pos[1], pos[2]), v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]),
+                                   v3(p2[0], p2[1], p2[2]));
+}
+
+/* SGGX helpers: out[0] = SGGXMicroFlakeNDF(stddev, basis, dir),
+ * out[1] = SGGXProjectedArea(stddev, basis, dir); basis from a quaternion. */
+void orc_sggx(const float* stddev, const float* quat, const float* dir, float* out)
+{
+    M3<float> basis = quat_to_m3(quat);
+    V3<float> sd = v3(stddev[0], stddev[1], stddev[2]);
+    V3<float> d = v3(dir[0], dir[1], dir[2]);
+    out[0] = sggx_ndf(sd, basis, d);
+    out[1] = sggx_projected_area(sd, basis, d);
+}
+
+}  // extern "C"
+

@@ -50,6 +50,9 @@ void orc_trace_rays(const orc_scene* scene, size_t n, const tr_ray* rays, tr_hit
 void orc_render_paths(const orc_scene* scene, size_t pixel_count, const uint32_t* pixels,
                       uint32_t frame, float* out_rgb, uint32_t* out_hash, int nthreads);
 
+/* Renders one camera sample and prints every vertex to stderr. */
+void orc_trace_path(const orc_scene* scene, uint32_t pixel, uint32_t camera_sample);
+
 /* Rows [row_begin, row_end) of one frame: per pixel the sum and the sum of
  * squares of the camera-sample radiance.  use_fp64 selects the FP64
  * instantiation of the integrator.  Returns the wall time in seconds. */

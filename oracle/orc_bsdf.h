@@ -154,7 +154,11 @@ template <class T> inline T specular_brdf(const Sample<T>& s, const Angles<T>& a
         shadow = ggx_g1(g2, a.cos_im, a.cos_in) * ggx_g1(g2, a.cos_im, a.cos_on);
         d = ggx_d(g2, a.cos_mn);
     }
-    return shadow * d / ((T)4 * xabs(a.cos_in) * xabs(a.cos_on));
+    /* Deviation: at exactly grazing incidence (cos = 0) the reference's
+     * shadow*D/(4|cos_in||cos_on|) is 0/0 = NaN; G1 is 0 there, so the limit
+     * value 0 is returned instead (the GPU kernels do the same). */
+    T den = (T)4 * xabs(a.cos_in) * xabs(a.cos_on);
+    return den > (T)0 ? shadow * d / den : (T)0;
 }
 
 /* GGXMicrofacet{,Conductor,Dielectric}BRDF (ggx-models-impl.hh:168-209) and

@@ -272,7 +272,7 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->sh_c.reserve((size_t)cap * msh));
     TR_HIP(s->counters.reserve(8));
     TR_HIP(s->spill.reserve((size_t)persistent_grid(s) * kTraceBlock * kStackSpill));
-    TR_HIP(s->counts.reserve(2));
+    TR_HIP(s->counts.reserve(8));
     s->capacity = cap;
     s->max_shadow = msh;
     return TR_OK;
@@ -334,7 +334,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
     uint64_t* counts = nullptr;
     if (s->counting) {
         counts = s->counts.ptr;
-        TR_HIP(hipMemsetAsync(counts, 0, 2 * sizeof(uint64_t), s->stream));
+        TR_HIP(hipMemsetAsync(counts, 0, 8 * sizeof(uint64_t), s->stream));
     }
     for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += ppb) {
         uint32_t npx = (uint32_t)std::min<uint64_t>(ppb, n_pixels - pix0);
@@ -356,8 +356,10 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
             launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots);
             TR_HIP(hipGetLastError());
             TR_HIP(hipEventRecord(e2, s->stream));
-            launch_shadow(s->stream, p, pool, v, grid, counts);
-            TR_HIP(hipGetLastError());
+            if (p.sc.n_lights > 0) {
+                launch_shadow(s->stream, p, pool, v, grid, counts);
+                TR_HIP(hipGetLastError());
+            }
             TR_HIP(hipEventRecord(e3, s->stream));
             record(s, EV_TRACE, e0, e1);
             record(s, EV_SHADE, e1, e2);
@@ -393,12 +395,18 @@ void collect_stats(tr_scene* s, uint64_t paths)
     s->stats.shadow_ms = ms[EV_SHADOW];
     s->stats.other_ms = ms[EV_OTHER];
     s->stats.trace_launches = traces;
-    s->stats.node_visits = s->stats.prim_tests = 0;
+    s->stats.node_visits = s->stats.prim_tests = s->stats.counted_rays = 0;
+    s->stats.segments = s->stats.shadow_rays = 0;
     if (s->counting) {
-        uint64_t c[2] = {0, 0};
+        /* [0..2] closest-hit kernel: nodes, primitives, rays;
+         * [3] shadow rays, [4..5] shadow kernel: nodes, primitives */
+        uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (hipMemcpy(c, s->counts.ptr, sizeof(c), hipMemcpyDeviceToHost) == hipSuccess) {
             s->stats.node_visits = c[0];
             s->stats.prim_tests = c[1];
+            s->stats.counted_rays = c[2];
+            s->stats.segments = c[2];
+            s->stats.shadow_rays = c[3];
         }
     }
 }
@@ -842,26 +850,24 @@ tr_status tr_scene_trace_rays(tr_scene* s, size_t n, const tr_ray* rays, tr_hit*
     if (n > 0xffffffffull - 64) return fail(TR_ERROR_INVALID_ARG, "too many rays");
     DevBuf<tr_ray> d_rays;
     DevBuf<tr_hit> d_hits;
-    DevBuf<uint32_t> d_work;
-    if (d_rays.reserve(n) != hipSuccess || d_hits.reserve(n) != hipSuccess || d_work.reserve(1) != hipSuccess) {
-        d_rays.release(); d_hits.release(); d_work.release();
+    if (d_rays.reserve(n) != hipSuccess || d_hits.reserve(n) != hipSuccess) {
+        d_rays.release(); d_hits.release();
         return fail(TR_ERROR_OUT_OF_MEMORY, "device allocation failed");
     }
     tr_status st = TR_OK;
     if (s->spill.reserve((size_t)persistent_grid(s) * kTraceBlock * kStackSpill) != hipSuccess)
         st = fail(TR_ERROR_OUT_OF_MEMORY, "device allocation failed");
-    if (st == TR_OK && (hipMemcpyAsync(d_rays.ptr, rays, n * sizeof(tr_ray), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-                        hipMemsetAsync(d_work.ptr, 0, sizeof(uint32_t), s->stream) != hipSuccess))
+    if (st == TR_OK && hipMemcpyAsync(d_rays.ptr, rays, n * sizeof(tr_ray), hipMemcpyHostToDevice, s->stream) != hipSuccess)
         st = fail(TR_ERROR_DEVICE, "ray upload failed");
     if (st == TR_OK) {
-        launch_trace_rays(s->stream, scene_view(s), d_rays.ptr, d_hits.ptr, (uint32_t)n, s->spill.ptr, d_work.ptr,
+        launch_trace_rays(s->stream, scene_view(s), d_rays.ptr, d_hits.ptr, (uint32_t)n, s->spill.ptr,
                           persistent_grid(s));
         if (hipGetLastError() != hipSuccess) st = fail(TR_ERROR_DEVICE, "trace launch failed");
     }
     if (st == TR_OK && hipMemcpyAsync(hits, d_hits.ptr, n * sizeof(tr_hit), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
         st = fail(TR_ERROR_DEVICE, "hit download failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess && st == TR_OK) st = fail(TR_ERROR_DEVICE, "trace failed");
-    d_rays.release(); d_hits.release(); d_work.release();
+    d_rays.release(); d_hits.release();
     return st;
 }
 

@@ -50,7 +50,7 @@ struct PathPool {
     float4* sh_d;       /* direction, .w = light geometry (bits) */
     float4* sh_c;       /* contribution, .w = light index (bits) */
     uint32_t* sh_n;     /* entries per slot */
-    uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size, [3],[4] work counters */
+    uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size */
     uint2* spill;       /* traversal stack spill, per persistent thread */
     uint32_t capacity;
 };
@@ -79,7 +79,7 @@ void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, ui
 void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
                        uint64_t slot_base, uint32_t n_pixels);
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
-                       uint2* spill, uint32_t* work, uint32_t grid);
+                       uint2* spill, uint32_t grid);
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,
                     uint32_t n_pixels, uint32_t post);
 

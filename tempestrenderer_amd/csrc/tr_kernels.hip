@@ -7,9 +7,11 @@
  * and the balance heuristic EvaluateTotalBalanceHeuristic (:1791-1806),
  * split into stages over a pool of path slots instead of a per-pixel loop:
  *   raygen -> { trace -> shade -> shadow } per vertex -> accumulate.
- * Traversal kernels are persistent: a fixed grid of one-wave blocks pulls
- * 64 rays at a time from a work counter; each lane walks the 8-wide BVH
- * with a node-group stack whose first kStackLds entries live in LDS.
+ * Traversal kernels are persistent: a fixed grid of one-wave blocks walks
+ * the ray queue with a static interleaved stride (no shared work counter,
+ * whose single-address atomics would serialise thousands of waves); each
+ * lane walks the 8-wide BVH with a node-group stack whose first kStackLds
+ * entries live in LDS.
  *
  * Philox counter of every random number: (pixel, camera sample,
  * split << 16 | vertex << 8 | stream, sub).  The CPU oracle uses the same
@@ -86,10 +88,7 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, Ba
                                                 uint32_t n_slots)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        pool.counters[0] = n_slots;
-        pool.counters[3] = 0;
-    }
+    if (i == 0) pool.counters[0] = n_slots;
     if (i >= n_slots) return;
     uint32_t pixel, cs, split;
     uint64_t pseq;
@@ -138,26 +137,21 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool po
                                                        unsigned long long* counts)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
-    __shared__ uint32_t s_base;
     uint32_t lane = threadIdx.x;
     uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     if (gtid == 0) {
         pool.counters[nxt] = 0;
         pool.counters[2] = 0;
-        pool.counters[4] = 0;
     }
     const uint32_t n = pool.counters[cur];
     const uint32_t* queue = pool.queue[cur];
     DevStack st;
     st.lds = &stk[0][lane];
     st.spill = pool.spill + (size_t)gtid * kStackSpill;
-    uint32_t nv = 0, np = 0;
-    for (;;) {
-        if (lane == 0) s_base = atomicAdd(&pool.counters[3], (uint32_t)kTraceBlock);
-        __syncthreads();
-        uint32_t base = s_base;
-        __syncthreads();
-        if (base >= n) break;
+    uint32_t nv = 0, np = 0, nr = 0;
+    /* static interleaved assignment: wave w takes rays w*64 + k*grid*64 */
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
         uint32_t qi = base + lane;
         if (qi < n) {
             uint32_t slot = queue[qi];
@@ -169,11 +163,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool po
                                      COUNT ? &np : nullptr);
             pool.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.geom));
             pool.hit_prim[slot] = h.prim;
+            ++nr;
         }
     }
     if (COUNT) {
         atomicAdd(&counts[0], (unsigned long long)nv);
         atomicAdd(&counts[1], (unsigned long long)np);
+        atomicAdd(&counts[2], (unsigned long long)nr);
     }
 }
 
@@ -303,22 +299,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                                                         unsigned long long* counts)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
-    __shared__ uint32_t s_base;
     uint32_t lane = threadIdx.x;
     uint32_t gtid = blockIdx.x * kTraceBlock + lane;
-    if (gtid == 0) pool.counters[3] = 0;
     const uint32_t n = pool.counters[2];
     const SceneView& sc = p.sc;
     DevStack st;
     st.lds = &stk[0][lane];
     st.spill = pool.spill + (size_t)gtid * kStackSpill;
-    uint32_t nv = 0, np = 0;
-    for (;;) {
-        if (lane == 0) s_base = atomicAdd(&pool.counters[4], (uint32_t)kTraceBlock);
-        __syncthreads();
-        uint32_t base = s_base;
-        __syncthreads();
-        if (base >= n) break;
+    uint32_t nv = 0, np = 0, nr = 0;
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
         uint32_t qi = base + lane;
         if (qi < n) {
             uint32_t slot = pool.shadow_queue[qi];
@@ -341,6 +331,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                                              COUNT ? &np : nullptr);
                     if (h.geom == TR_INVALID_GEOMETRY || h.geom == __float_as_uint(d4.w)) color = add(color, ld4(c4));
                     ++e;
+                    ++nr;
                 }
                 radiance = add(radiance, divs(color, (float)samples));
             }
@@ -353,8 +344,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
         }
     }
     if (COUNT) {
-        atomicAdd(&counts[0], (unsigned long long)nv);
-        atomicAdd(&counts[1], (unsigned long long)np);
+        atomicAdd(&counts[4], (unsigned long long)nv);
+        atomicAdd(&counts[5], (unsigned long long)np);
+        atomicAdd(&counts[3], (unsigned long long)nr);
     }
 }
 
@@ -401,21 +393,16 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
 /* ---------------- fixed ray sets ---------------- */
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
-                                                            uint2* spill, uint32_t* work)
+                                                            uint2* spill)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
-    __shared__ uint32_t s_base;
     uint32_t lane = threadIdx.x;
     uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     DevStack st;
     st.lds = &stk[0][lane];
     st.spill = spill + (size_t)gtid * kStackSpill;
-    for (;;) {
-        if (lane == 0) s_base = atomicAdd(work, (uint32_t)kTraceBlock);
-        __syncthreads();
-        uint32_t base = s_base;
-        __syncthreads();
-        if (base >= n) break;
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
         uint32_t i = base + lane;
         if (i < n) {
             tr_ray ry = rays[i];
@@ -505,9 +492,9 @@ void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool
 }
 
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n, uint2* spill,
-                       uint32_t* work, uint32_t grid)
+                       uint32_t grid)
 {
-    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill, work);
+    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill);
 }
 
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba, uint32_t n, uint32_t post)

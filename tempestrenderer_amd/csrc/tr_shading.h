@@ -9,7 +9,9 @@
  *    angles are taken in turns (sin(2*pi*u) is tr_sincos_turns(u));
  *  - pow(x, 5) in FresnelSchlick is x^2*x^2*x;
  *  - a mesh light's triangle is picked as floor(u32 * count / 2^32) instead
- *    of Hash32(seed) % count (functions.hh:243-247).
+ *    of Hash32(seed) % count (functions.hh:243-247);
+ *  - the microfacet specular term returns 0 instead of 0/0 = NaN when a
+ *    direction is exactly grazing (cos = 0).
  */
 #ifndef TR_SHADING_H
 #define TR_SHADING_H
@@ -141,7 +143,11 @@ TR_HD V3 microfacet_brdf(const Sample& s)
         shadow = ggx_g1(g2, cos_im, cos_in) * ggx_g1(g2, cos_im, cos_on);
         d = ggx_d(g2, cos_mn);
     }
-    float spec = shadow * d / (4.0f * fabsf(cos_in) * fabsf(cos_on));
+    /* At exactly grazing incidence (cos = 0) the reference's
+     * shadow*D/(4|cos_in||cos_on|) is 0/0; G1 is 0 there and the limit 0 is
+     * used instead of NaN. */
+    float den = 4.0f * fabsf(cos_in) * fabsf(cos_on);
+    float spec = den > 0.0f ? shadow * d / den : 0.0f;
     float brdf = fresnel * spec;
     return muls(add(muls(ld3(m->diffuse), 1.0f / kPi), muls(ld3(m->specular), brdf)), cos_in);
 }

@@ -230,7 +230,12 @@ class RayTracerScene:
         check(self.lib.tr_debug_bvh_check(self.h, ctypes.byref(v)))
         return v.value
 
-    def renderPaths(self, pixels, samples_camera):
+    def renderPaths(self, pixels, samples_camera, frame=None):
+        """Every camera sample of the listed pixels at the scene's frame index
+        (or `frame`, which then becomes the frame index): radiance [N, 3] and
+        hit-sequence hash per sample.  drawOnce advances the frame index."""
+        if frame is not None:
+            self.setFrameIndex(frame)
         px = np.ascontiguousarray(np.asarray(pixels, dtype=np.uint32))
         n = px.size * samples_camera
         rgb = np.zeros(n * 3, dtype=np.float32)

@@ -176,10 +176,10 @@ typedef struct tr_stats {
     uint64_t segments;           /* extension rays traced */
     uint64_t shadow_rays;        /* next-event shadow rays traced */
     double frame_ms;             /* host wall time of the call */
-    double trace_ms;             /* summed HIP-event time of closest-hit kernels */
-    double shade_ms;             /* summed HIP-event time of shading kernels */
-    double shadow_ms;            /* summed HIP-event time of shadow kernels */
-    double other_ms;             /* ray generation and accumulation */
+    double trace_ms;             /* timing mode: summed HIP-event time of closest-hit kernels */
+    double shade_ms;             /* timing mode: summed HIP-event time of shading kernels */
+    double shadow_ms;            /* timing mode: summed HIP-event time of shadow kernels */
+    double other_ms;             /* timing mode: ray generation and accumulation */
     uint64_t trace_launches;     /* closest-hit launches behind trace_ms */
     uint64_t node_visits;        /* BVH8 nodes fetched (only if counting on) */
     uint64_t prim_tests;         /* primitives tested (only if counting on) */
@@ -284,6 +284,10 @@ tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint3
 tr_status tr_scene_get_stats(const tr_scene* scene, tr_stats* stats);
 /* Turns on per-launch node/primitive counting (slower; for measurement). */
 tr_status tr_scene_set_counting(tr_scene* scene, int32_t enabled);
+/* Turns on per-kernel HIP-event timing: each launch is bracketed by events
+ * on the scene's stream and waited for (slower; for measurement).  The
+ *_ms fields of tr_stats are filled only in this mode. */
+tr_status tr_scene_set_timing(tr_scene* scene, int32_t enabled);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,7 @@ SIGNATURES = {
     "tr_scene_render_paths": (_st, [_vp, _sz, P(_u32), P(ctypes.c_float), P(_u32)]),
     "tr_scene_get_stats": (_st, [_vp, P(tr_stats)]),
     "tr_scene_set_counting": (_st, [_vp, ctypes.c_int32]),
+    "tr_scene_set_timing": (_st, [_vp, ctypes.c_int32]),
     "tr_debug_build_scene_host": (_st, [_vp]),
     "tr_debug_trace_rays_host": (_st, [_vp, _sz, P(tr_ray), P(tr_hit), P(_u64), P(_u64)]),
     "tr_debug_bvh_check": (_st, [_vp, P(_u64)]),

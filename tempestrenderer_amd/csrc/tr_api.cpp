@@ -81,6 +81,7 @@ struct tr_scene {
     uint32_t estimator = TR_ESTIMATOR_REFERENCE;
     uint32_t frame_index = 0;
     bool counting = false;
+    bool timing = false;
 
     /* host copy of the scene */
     std::vector<tr_material> mats;
@@ -336,42 +337,48 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         counts = s->counts.ptr;
         TR_HIP(hipMemsetAsync(counts, 0, 8 * sizeof(uint64_t), s->stream));
     }
+    /* Per-stage HIP events are recorded only in timing mode; there every
+     * end event is waited on before the next launch, so each event pair
+     * brackets exactly one kernel (the runtime may otherwise defer kernel
+     * submission and attach an event to an earlier command). */
+    auto stage = [&](int cat, auto&& launch) -> tr_status {
+        if (!s->timing) {
+            launch();
+            TR_HIP(hipGetLastError());
+            return TR_OK;
+        }
+        hipEvent_t a = get_event(s), b = get_event(s);
+        if (!a || !b) return fail(TR_ERROR_DEVICE, "event creation failed");
+        TR_HIP(hipEventRecord(a, s->stream));
+        launch();
+        TR_HIP(hipGetLastError());
+        TR_HIP(hipEventRecord(b, s->stream));
+        TR_HIP(hipEventSynchronize(b));
+        record(s, cat, a, b);
+        return TR_OK;
+    };
     for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += ppb) {
         uint32_t npx = (uint32_t)std::min<uint64_t>(ppb, n_pixels - pix0);
         uint32_t nslots = (uint32_t)(npx * per_pixel);
         uint64_t slot_base = pix0 * per_pixel;
-        hipEvent_t a = get_event(s), b = get_event(s);
-        TR_HIP(hipEventRecord(a, s->stream));
-        launch_raygen(s->stream, p, pool, out, slot_base, nslots);
-        TR_HIP(hipGetLastError());
-        TR_HIP(hipEventRecord(b, s->stream));
-        record(s, EV_OTHER, a, b);
+        tr_status st2;
+        if ((st2 = stage(EV_OTHER, [&] { launch_raygen(s->stream, p, pool, out, slot_base, nslots); })) != TR_OK)
+            return st2;
         int cur = 0, nxt = 1;
         for (uint32_t v = 0; v <= s->max_depth; ++v) {
-            hipEvent_t e0 = get_event(s), e1 = get_event(s), e2 = get_event(s), e3 = get_event(s);
-            TR_HIP(hipEventRecord(e0, s->stream));
-            launch_trace(s->stream, p.sc, pool, cur, nxt, grid, counts);
-            TR_HIP(hipGetLastError());
-            TR_HIP(hipEventRecord(e1, s->stream));
-            launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots);
-            TR_HIP(hipGetLastError());
-            TR_HIP(hipEventRecord(e2, s->stream));
+            if ((st2 = stage(EV_TRACE, [&] { launch_trace(s->stream, p.sc, pool, cur, nxt, grid, counts); })) != TR_OK)
+                return st2;
+            if ((st2 = stage(EV_SHADE, [&] { launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots); })) !=
+                TR_OK)
+                return st2;
             if (p.sc.n_lights > 0) {
-                launch_shadow(s->stream, p, pool, v, grid, counts);
-                TR_HIP(hipGetLastError());
+                if ((st2 = stage(EV_SHADOW, [&] { launch_shadow(s->stream, p, pool, v, grid, counts); })) != TR_OK)
+                    return st2;
             }
-            TR_HIP(hipEventRecord(e3, s->stream));
-            record(s, EV_TRACE, e0, e1);
-            record(s, EV_SHADE, e1, e2);
-            record(s, EV_SHADOW, e2, e3);
             std::swap(cur, nxt);
         }
-        hipEvent_t c = get_event(s), d = get_event(s);
-        TR_HIP(hipEventRecord(c, s->stream));
-        launch_accumulate(s->stream, p, pool, out, slot_base, npx);
-        TR_HIP(hipGetLastError());
-        TR_HIP(hipEventRecord(d, s->stream));
-        record(s, EV_OTHER, c, d);
+        if ((st2 = stage(EV_OTHER, [&] { launch_accumulate(s->stream, p, pool, out, slot_base, npx); })) != TR_OK)
+            return st2;
     }
     return TR_OK;
 }
@@ -549,6 +556,13 @@ tr_status tr_scene_set_counting(tr_scene* s, int32_t enabled)
 {
     if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
     s->counting = enabled != 0;
+    return TR_OK;
+}
+
+tr_status tr_scene_set_timing(tr_scene* s, int32_t enabled)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    s->timing = enabled != 0;
     return TR_OK;
 }
 

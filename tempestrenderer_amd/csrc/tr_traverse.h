@@ -147,17 +147,72 @@ TR_HD void test_prim(const PrimRec& p, const RayPre& r, float tnear, float tfar,
 
 TR_HD float exp2i(int e) { return tr_u2f((uint32_t)(e + 127) << 23); }
 
+/* A node as 32 dwords, fetched with eight 16-byte loads.  Field offsets
+ * (little-endian): 0-2 origin, 3 exp[0..2] | imask << 24, 4 child_base,
+ * 5 prim_base, 6-7 meta[8], 8-19 qlo[3][8], 20-31 qhi[3][8]. */
+struct NodeRegs {
+    uint32_t d[32];
+};
+
+TR_HD NodeRegs load_node(const Node8* nodes, uint32_t idx)
+{
+    NodeRegs r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + idx);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint4 v = p[k];
+        r.d[4 * k + 0] = v.x;
+        r.d[4 * k + 1] = v.y;
+        r.d[4 * k + 2] = v.z;
+        r.d[4 * k + 3] = v.w;
+    }
+#else
+    memcpy(r.d, nodes + idx, sizeof(r.d));
+#endif
+    return r;
+}
+
+TR_HD uint32_t node_imask(const NodeRegs& n) { return n.d[3] >> 24; }
+/* meta[s] without indexing the register array by a run-time value (which
+ * would move the whole node to scratch memory) */
+TR_HD uint32_t node_meta(const NodeRegs& n, uint32_t s)
+{
+    uint64_t m = (uint64_t)n.d[6] | ((uint64_t)n.d[7] << 32);
+    return (uint32_t)(m >> (8u * s)) & 0xffu;
+}
+
 /* Slab test of the 8 children of a node; returns the hit mask in slot
  * order.  tmax is the current far bound (closest hit so far or tfar). */
-TR_HD uint32_t intersect_children(const Node8& n, const RayPre& r, float tnear, float tmax)
+TR_HD uint32_t intersect_children(const NodeRegs& n, const RayPre& r, float tnear, float tmax)
 {
+    float ox = tr_u2f(n.d[0]), oy = tr_u2f(n.d[1]), oz = tr_u2f(n.d[2]);
+    float sx = exp2i((int)(int8_t)(n.d[3] & 0xffu));
+    float sy = exp2i((int)(int8_t)((n.d[3] >> 8) & 0xffu));
+    float sz = exp2i((int)(int8_t)((n.d[3] >> 16) & 0xffu));
+    /* slots with an internal child or a non-empty leaf */
+    uint32_t valid = node_imask(n);
+    uint32_t mlo = n.d[6], mhi = n.d[7];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (uint32_t s = 0; s < 4; ++s) {
+        if ((mlo >> (8u * s)) & 0xffu) valid |= 1u << s;
+        if ((mhi >> (8u * s)) & 0xffu) valid |= 1u << (s + 4);
+    }
     uint32_t mask = 0;
-    float sx = exp2i(n.exp[0]), sy = exp2i(n.exp[1]), sz = exp2i(n.exp[2]);
-    for (int s = 0; s < 8; ++s) {
-        if (!((n.imask >> s) & 1u) && n.meta[s] == 0) continue;
-        float lx = n.origin[0] + (float)n.qlo[0][s] * sx, hx = n.origin[0] + (float)n.qhi[0][s] * sx;
-        float ly = n.origin[1] + (float)n.qlo[1][s] * sy, hy = n.origin[1] + (float)n.qhi[1][s] * sy;
-        float lz = n.origin[2] + (float)n.qlo[2][s] * sz, hz = n.origin[2] + (float)n.qhi[2][s] * sz;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (uint32_t s = 0; s < 8; ++s) {
+        uint32_t sh = 16u * (s & 1u);
+        uint32_t w = s >> 1;
+        float lx = ox + (float)((n.d[8 + w] >> sh) & 0xffffu) * sx;
+        float ly = oy + (float)((n.d[12 + w] >> sh) & 0xffffu) * sy;
+        float lz = oz + (float)((n.d[16 + w] >> sh) & 0xffffu) * sz;
+        float hx = ox + (float)((n.d[20 + w] >> sh) & 0xffffu) * sx;
+        float hy = oy + (float)((n.d[24 + w] >> sh) & 0xffffu) * sy;
+        float hz = oz + (float)((n.d[28 + w] >> sh) & 0xffffu) * sz;
         float tx0 = (lx - r.org[0]) * r.inv[0], tx1 = (hx - r.org[0]) * r.inv[0];
         float ty0 = (ly - r.org[1]) * r.inv[1], ty1 = (hy - r.org[1]) * r.inv[1];
         float tz0 = (lz - r.org[2]) * r.inv[2], tz1 = (hz - r.org[2]) * r.inv[2];
@@ -166,15 +221,16 @@ TR_HD uint32_t intersect_children(const Node8& n, const RayPre& r, float tnear, 
         t1 = t1 * 1.00000095f; /* 1 + 2^-20: keeps boxes touching a hit at t */
         if (t0 <= t1) mask |= 1u << s;
     }
-    return mask;
+    return mask & valid;
 }
 
-/* Permutes a slot-order mask into visiting order: bit i <-> slot i ^ octant. */
-TR_HD uint32_t to_visit_order(uint32_t slot_mask, uint32_t oct)
+/* Permutes a slot-order mask into visiting order: bit i <-> slot i ^ octant
+ * (XOR by each octant bit swaps bits, bit pairs or nibbles). */
+TR_HD uint32_t to_visit_order(uint32_t m, uint32_t oct)
 {
-    uint32_t m = 0;
-    for (uint32_t s = 0; s < 8; ++s)
-        if ((slot_mask >> s) & 1u) m |= 1u << (s ^ oct);
+    if (oct & 1u) m = ((m & 0x55u) << 1) | ((m & 0xAAu) >> 1);
+    if (oct & 2u) m = ((m & 0x33u) << 2) | ((m & 0xCCu) >> 2);
+    if (oct & 4u) m = ((m & 0x0Fu) << 4) | ((m & 0xF0u) >> 4);
     return m;
 }
 
@@ -195,10 +251,26 @@ TR_HD uint32_t ctz32(uint32_t x)
 #endif
 }
 
+/* A primitive fetched with three 16-byte loads. */
+TR_HD PrimRec load_prim(const PrimRec* prims, uint32_t idx)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4* p = reinterpret_cast<const uint4*>(prims + idx);
+    uint4 a = p[0], b = p[1], c = p[2];
+    PrimRec r;
+    r.v0[0] = __uint_as_float(a.x); r.v0[1] = __uint_as_float(a.y); r.v0[2] = __uint_as_float(a.z); r.prim = a.w;
+    r.v1[0] = __uint_as_float(b.x); r.v1[1] = __uint_as_float(b.y); r.v1[2] = __uint_as_float(b.z); r.geom = b.w;
+    r.v2[0] = __uint_as_float(c.x); r.v2[1] = __uint_as_float(c.y); r.v2[2] = __uint_as_float(c.z); r.flags = c.w;
+    return r;
+#else
+    return prims[idx];
+#endif
+}
+
 /* Stack of node groups.  An entry is (child_base, visit-order hit mask |
- * imask << 8); its depth is bounded by the tree depth.  The kernels keep
- * the first entries in LDS and spill deeper ones to global memory; this
- * host/device template takes any stack object with push/pop/empty. */
+ * imask << 8); its depth is bounded by the tree depth plus one.  The
+ * kernels keep the first entries in LDS and spill deeper ones to global
+ * memory; this host/device template takes any stack with push/pop/empty. */
 template <class Stack>
 TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const RayPre& r, float tnear, float tfar,
                            Stack& stack, uint32_t* node_visits, uint32_t* prim_tests)
@@ -218,26 +290,26 @@ TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const RayPr
         if (vis) stack.push(base, vis | (imask << 8));
         uint32_t slot = i ^ oct;
         uint32_t node_index = base + popc8(imask & ((1u << slot) - 1u));
-        const Node8& n = nodes[node_index];
+        NodeRegs n = load_node(nodes, node_index);
         if (node_visits) ++*node_visits;
         float tmax = best.geom == TR_INVALID_GEOMETRY ? tfar : best.t;
         uint32_t hit = intersect_children(n, r, tnear, tmax);
-        uint32_t leaves = hit & ~(uint32_t)n.imask;
-        uint32_t inner = hit & (uint32_t)n.imask;
+        uint32_t nimask = node_imask(n);
+        uint32_t leaves = hit & ~nimask;
+        uint32_t inner = hit & nimask;
         uint32_t lv = to_visit_order(leaves, oct);
         while (lv) {
             uint32_t li = ctz32(lv);
             lv &= lv - 1u;
-            uint32_t s = li ^ oct;
-            uint32_t meta = n.meta[s];
-            uint32_t first = n.prim_base + (meta & 31u);
+            uint32_t meta = node_meta(n, li ^ oct);
+            uint32_t first = n.d[5] + (meta & 31u);
             uint32_t count = meta >> 5;
             for (uint32_t k = 0; k < count; ++k) {
                 if (prim_tests) ++*prim_tests;
-                test_prim(prims[first + k], r, tnear, tfar, &best);
+                test_prim(load_prim(prims, first + k), r, tnear, tfar, &best);
             }
         }
-        if (inner) stack.push(n.child_base, to_visit_order(inner, oct) | ((uint32_t)n.imask << 8));
+        if (inner) stack.push(n.d[4], to_visit_order(inner, oct) | (nimask << 8));
     }
     if (best.geom == TR_INVALID_GEOMETRY) best.t = tfar;
     return best;

@@ -125,6 +125,7 @@ class RayTracerScene:
     def setEstimator(self, mode): check(self.lib.tr_scene_set_estimator(self.h, int(mode)))
     def setFrameIndex(self, f): check(self.lib.tr_scene_set_frame_index(self.h, int(f)))
     def setCounting(self, on): check(self.lib.tr_scene_set_counting(self.h, int(bool(on))))
+    def setTiming(self, on): check(self.lib.tr_scene_set_timing(self.h, int(bool(on))))
 
     # lights (ray-tracing.hh:326-331)
     def addLightSource(self, light):

@@ -64,19 +64,40 @@ def cpu_baseline(desc, rows, threads):
     return samples / secs / 1e6, secs, samples, r0
 
 
+def calibrate_copy(mib=512, reps=4):
+    """A plain device-to-device copy of known size, run under
+    `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale those counters into
+    bytes: each copy reads and writes exactly `mib` MiB."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--small", action="store_true", help="reduced scene for a quick check (not the metric)")
-    ap.add_argument("--cpu-rows", type=int, default=96)
+    ap.add_argument("--cpu-rows", type=int, default=1080, help="rows of the frame the CPU oracle renders (capped at the height)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 512 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the tracer runs on a single MI355X in this build "
               "(multi-GPU sharding by sample index is described, not built).", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    args.steps = max(1, args.steps)
 
     from tempestrenderer_amd import workloads
     from tempestrenderer_amd.scene import RayTracerScene
@@ -137,6 +158,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
+        args.cpu_rows = min(args.cpu_rows, desc.height)
         cpu_value, cpu_secs, cpu_samples, r0 = cpu_baseline(desc, args.cpu_rows, threads)
         cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
                "sample": "rows %d-%d of the same frame (%d px x %d camera samples = %d paths), %.1f s wall" % (

@@ -184,6 +184,7 @@ typedef struct tr_stats {
     uint64_t node_visits;        /* BVH8 nodes fetched (only if counting on) */
     uint64_t prim_tests;         /* primitives tested (only if counting on) */
     uint64_t counted_rays;       /* rays behind node_visits / prim_tests */
+    uint64_t volume_steps;       /* tentative collisions of delta / ratio tracking (only if counting on) */
 } tr_stats;
 
 /* Scene-construction information. */
@@ -246,6 +247,37 @@ tr_status tr_scene_add_triangle_mesh(tr_scene* scene, const float world[16],
 tr_status tr_scene_add_sphere(tr_scene* scene, const float center[3], float radius,
                               const tr_material* material, uint64_t* out_geom_id);
 
+/* A heterogeneous participating medium sampled on a regular grid.  It
+ * replaces addHierarchicalVolume (ray-tracing.hh:381, ray-tracing.cc:
+ * 1448-1504), whose bricks hold u8 density and u8 (theta, phi) fibre
+ * angles: a caller flattens its bricks into one grid (INTEGRATION shows the
+ * conversion).  Samples sit on the grid nodes: sample (i, j, k) is at
+ * min_corner + (i, j, k) * (max_corner - min_corner) / (dims - 1), x
+ * fastest, and values between them are trilinear, as RTVolumeSet::
+ * sampleScattering interpolates (ray-tracing.cc:283-325).
+ *   density    IEEE binary16 per sample: extinction per unit length of a
+ *              flake whose projected area sigma(w) is 1 (>= 0, finite).
+ *   direction  three binary16 per sample: the fibre (SGGX tangent)
+ *              direction; its length does not matter.  The orientation at
+ *              a point is the dominant eigenvector of the trilinearly
+ *              weighted structure tensor (SampleVolumePoint, :110-182).
+ *   max_density  the majorant; 0 takes the grid's largest density, a value
+ *              below it is rejected.
+ * The material must be SGGXMicroFlake (albedo, sggx_stddev).  Extinction
+ * at x towards w is density(x) * sigma(w), tracked with the global majorant
+ * max_density * max(sggx_stddev). */
+typedef struct tr_grid_volume {
+    uint32_t dims[3];        /* >= 2 each */
+    float min_corner[3];
+    float max_corner[3];
+    const uint16_t* density;
+    const uint16_t* direction;
+    float max_density;
+} tr_grid_volume;
+
+tr_status tr_scene_add_grid_volume(tr_scene* scene, const tr_grid_volume* volume,
+                                   const tr_material* material, uint64_t* out_geom_id);
+
 /* commitScene (ray-tracing.cc:1633-1636): packs the scene into HBM and
  * builds the 8-wide BVH. */
 tr_status tr_scene_commit(tr_scene* scene);
@@ -284,9 +316,10 @@ tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint3
 tr_status tr_scene_get_stats(const tr_scene* scene, tr_stats* stats);
 /* Turns on per-launch node/primitive counting (slower; for measurement). */
 tr_status tr_scene_set_counting(tr_scene* scene, int32_t enabled);
-/* Turns on per-kernel HIP-event timing: each launch is bracketed by events
- * on the scene's stream and waited for (slower; for measurement).  The
- *_ms fields of tr_stats are filled only in this mode. */
+/* Turns on per-kernel timing: every launch carries a start and a stop
+ * event in its dispatch (hipExtLaunchKernelGGL) and is waited for before
+ * the next one (slower; for measurement).  The *_ms fields of tr_stats are
+ * filled only in this mode. */
 tr_status tr_scene_set_timing(tr_scene* scene, int32_t enabled);
 
 #ifdef __cplusplus

@@ -65,7 +65,11 @@ def _declare(lib):
         "orc_add_sphere": (ctypes.c_int, [vp, P(f), f, P(_abi.tr_material), P(u64)]),
         "orc_add_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
         "orc_add_sphere_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
+        "orc_add_grid_volume": (ctypes.c_int, [vp, P(_abi.tr_grid_volume), P(_abi.tr_material), P(u64)]),
         "orc_commit": (ctypes.c_int, [vp]),
+        "orc_grid_volume_lookup": (None, [vp, P(f), P(f), f, P(f)]),
+        "orc_trace_segments": (None, [vp, sz, P(_abi.tr_ray), u32, P(_abi.tr_hit), ctypes.c_int]),
+        "orc_shadow_transmittance": (None, [vp, sz, P(_abi.tr_ray), u32, u32, P(f), ctypes.c_int]),
         "orc_trace_rays": (None, [vp, sz, P(_abi.tr_ray), P(_abi.tr_hit), ctypes.c_int, ctypes.c_int]),
         "orc_trace_path": (None, [vp, u32, u32]),
         "orc_render_paths": (None, [vp, sz, P(u32), u32, P(f), P(u32), ctypes.c_int]),
@@ -200,8 +204,45 @@ class OracleScene:
             raise RuntimeError("oracle addSphereLightSource failed")
         return gid.value
 
+    def addGridVolume(self, dims, min_corner, max_corner, density, direction, material, max_density=0.0):
+        from tempestrenderer_amd.scene import grid_volume_record
+        rec, keep = grid_volume_record(dims, min_corner, max_corner, density, direction, max_density)
+        gid = ctypes.c_uint64()
+        if self.lib.orc_add_grid_volume(self.h, ctypes.byref(rec), ctypes.byref(material.rec), ctypes.byref(gid)):
+            raise RuntimeError("oracle addGridVolume failed")
+        self._keep.append(material)
+        return gid.value
+
     def commitScene(self):
         self.lib.orc_commit(self.h)
+
+    def gridVolumeLookup(self, pos, direction, threshold):
+        """First volume at `pos`: (density, frame 3x3 columns, sigma(direction),
+        decision 0 rejected / 1 accepted / 2 zero density)."""
+        out = np.zeros(12, dtype=np.float32)
+        p = np.asarray(pos, dtype=np.float32)
+        d = np.asarray(direction, dtype=np.float32)
+        self.lib.orc_grid_volume_lookup(self.h, _fp(p), _fp(d), float(threshold), _fp(out))
+        return float(out[0]), out[1:10].reshape(3, 3), float(out[10]), int(out[11])
+
+    def traceSegments(self, rays, sample=0, threads=0):
+        """Closest hit of each ray after delta tracking (Philox pixel = ray index)."""
+        from tempestrenderer_amd import _abi
+        from tempestrenderer_amd.scene import HIT_DTYPE
+        r = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(-1, 8))
+        hits = np.zeros(r.shape[0], dtype=HIT_DTYPE)
+        self.lib.orc_trace_segments(self.h, r.shape[0], r.ctypes.data_as(ctypes.POINTER(_abi.tr_ray)), int(sample),
+                                    hits.ctypes.data_as(ctypes.POINTER(_abi.tr_hit)), threads)
+        return hits
+
+    def shadowTransmittance(self, rays, sample=0, light_geom=0xFFFFFFFF, threads=0):
+        """Woodcock transmittance estimates of shadow rays (tfar = distance)."""
+        from tempestrenderer_amd import _abi
+        r = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(-1, 8))
+        out = np.zeros(r.shape[0], dtype=np.float32)
+        self.lib.orc_shadow_transmittance(self.h, r.shape[0], r.ctypes.data_as(ctypes.POINTER(_abi.tr_ray)),
+                                          int(sample), int(light_geom), _fp(out), threads)
+        return out
 
     def traceRays(self, rays, brute_force=False, threads=0):
         from tempestrenderer_amd import _abi

@@ -19,13 +19,20 @@
  *   ctr = (pixel, camera sample index, split<<16 | vertex<<8 | stream, sub)
  *   key = (seed low, seed high)
  * Streams: 0 camera jitter, 1 BSDF sample (x, y, lobe), 2 light sample
- * (sub = light<<16 | sample; triangle pick, ra, rb), 3 Russian roulette.
+ * (sub = light<<16 | sample; triangle pick, ra, rb), 3 Russian roulette,
+ * 4 delta tracking of a path vertex (sub = step pair), 5 Woodcock tracking
+ * of a shadow ray (sub = light<<24 | sample<<16 | step pair).
+ *
+ * Grid volumes (orc_volume.h) are one box primitive each; a closest hit on
+ * a box runs delta tracking along the ray and either scatters there or
+ * traces on beyond the box (the free-path retrace of subdraw).
  */
 #include "oracle.h"
 #include "orc_philox.h"
 #include "orc_math.h"
 #include "orc_bsdf.h"
 #include "orc_geom.h"
+#include "orc_volume.h"
 
 #include <algorithm>
 #include <atomic>
@@ -43,7 +50,7 @@ namespace {
 thread_local FILE* g_trace = nullptr;
 #define ORC_TRACE(...) do { if (g_trace) fprintf(g_trace, __VA_ARGS__); } while (0)
 
-enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2 };
+enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2, GEOM_VOLUME = 3 };
 enum { STREAM_CAMERA = 0, STREAM_BSDF = 1, STREAM_LIGHT = 2, STREAM_RR = 3 };
 
 struct Geom {
@@ -57,12 +64,13 @@ struct Geom {
     uint32_t tri_count;
     float sphere[4];
     uint32_t light;         /* sphere light: light index */
+    uint32_t volume;        /* grid volume: index into orc_scene::volumes */
 };
 
 struct Prim {
-    float v0[3], v1[3], v2[3]; /* triangle; spheres: v0 = centre, v1[0] = radius */
+    float v0[3], v1[3], v2[3]; /* triangle; spheres: v0 = centre, v1[0] = radius; volumes: box v0..v1 */
     uint32_t geom, prim;
-    uint32_t is_sphere, two_sided;
+    uint32_t is_sphere, two_sided, is_volume;
 };
 
 struct Light {
@@ -96,6 +104,7 @@ struct orc_scene {
     std::vector<uint32_t> prim_order;
     std::vector<BNode> nodes;
     std::vector<Light> lights;
+    std::vector<GridVolume> volumes;
     uint32_t instances = 0;
     bool committed = false;
 };
@@ -122,6 +131,8 @@ void prim_bounds(const Prim& p, float lo[3], float hi[3])
 {
     if (p.is_sphere) {
         for (int a = 0; a < 3; ++a) { lo[a] = p.v0[a] - p.v1[0]; hi[a] = p.v0[a] + p.v1[0]; }
+    } else if (p.is_volume) {
+        for (int a = 0; a < 3; ++a) { lo[a] = p.v0[a]; hi[a] = p.v1[a]; }
     } else {
         for (int a = 0; a < 3; ++a) {
             lo[a] = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
@@ -194,6 +205,30 @@ inline bool box_hit(const BNode& n, const double o[3], const double inv[3], doub
     return t0 <= t1;
 }
 
+/* The box of a grid volume: FP32 slab test with the reciprocal direction
+ * (components below 1e-20 in magnitude taken as +-1e-20).  Entry =
+ * max(slab entry, tnear) if the slab exit lies beyond tnear and the entry
+ * before tfar; the exit is returned as the hit's u. */
+inline int volume_box_hit(const float org[3], const float dir[3], const float lo[3], const float hi[3], float tnear,
+                          float tfar, float* t_out, float* t_exit)
+{
+    float t0 = -INFINITY, t1 = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        float d = dir[a];
+        if (fabsf(d) < 1e-20f) d = d < 0.0f ? -1e-20f : 1e-20f;
+        float inv = 1.0f / d;
+        float ta = (lo[a] - org[a]) * inv, tb = (hi[a] - org[a]) * inv;
+        t0 = std::max(t0, std::min(ta, tb));
+        t1 = std::min(t1, std::max(ta, tb));
+    }
+    if (!(t0 <= t1) || !(t1 > tnear)) return 0;
+    float t = std::max(t0, tnear);
+    if (!(t < tfar)) return 0;
+    *t_out = t;
+    *t_exit = t1;
+    return 1;
+}
+
 inline void test_prim(const orc_scene* s, uint32_t pi, const WatertightRay& wr, const float org[3],
                       const float dir[3], float tnear, float tfar, tr_hit* best)
 {
@@ -201,6 +236,7 @@ inline void test_prim(const orc_scene* s, uint32_t pi, const WatertightRay& wr, 
     float t, u = 0.0f, v = 0.0f;
     int hit;
     if (p.is_sphere) hit = sphere_hit(org, dir, p.v0, p.v1[0], tnear, tfar, &t);
+    else if (p.is_volume) hit = volume_box_hit(org, dir, p.v0, p.v1, tnear, tfar, &t, &u);
     else hit = watertight_triangle(wr, org, p.v0, p.v1, p.v2, (int)p.two_sided, tnear, tfar, &t, &u, &v);
     if (!hit) return;
     if (best->geom != TR_INVALID_GEOMETRY && !hit_better(t, p.geom, p.prim, best->t, best->geom, best->prim)) return;
@@ -261,7 +297,49 @@ struct Rng {
         uint32_t ctr[4] = {pixel, sample, (split << 16) | (vertex << 8) | stream, sub};
         orc_philox4x32_10(ctr, key, out);
     }
+    StepDraws steps(uint32_t vertex, uint32_t stream) const
+    {
+        StepDraws d;
+        d.ctr[0] = pixel; d.ctr[1] = sample; d.ctr[2] = (split << 16) | (vertex << 8) | stream; d.ctr[3] = 0;
+        d.key[0] = key[0]; d.key[1] = key[1];
+        return d;
+    }
 };
+
+bool is_volume_geom(const orc_scene* s, uint32_t geom)
+{
+    return geom != TR_INVALID_GEOMETRY && s->geoms[geom].type == GEOM_VOLUME;
+}
+
+/* Closest hit of a path segment with delta tracking through grid volumes:
+ * the primary and secondary tracking loops of subdraw (ray-tracing.cc:
+ * 2040-2128, 2212-2272).  A volume hit becomes a collision at distance t
+ * (u = v = 0), or the trace continues beyond the point where the walk left
+ * the volume. */
+tr_hit trace_segment(const orc_scene* s, const float org[3], const float dir[3], const Rng& rng, uint32_t vertex)
+{
+    tr_hit h = trace_closest(s, org, dir, 0.0f, INFINITY, false);
+    if (s->volumes.empty()) return h;
+    StepDraws dr = rng.steps(vertex, ORC_STREAM_VOLUME);
+    uint32_t step = 0, crossings = 0;
+    V3<float> o = v3(org[0], org[1], org[2]), d = v3(dir[0], dir[1], dir[2]);
+    while (is_volume_geom(s, h.geom)) {
+        const GridVolume& g = s->volumes[s->geoms[h.geom].volume];
+        float tc;
+        if (delta_track(g, o, d, h.t, h.u, dr, &step, &tc)) {
+            h.t = tc;
+            h.u = h.v = 0.0f;
+            break;
+        }
+        if (++crossings >= kOrcMaxVolumeCrossings) {
+            h.geom = h.prim = TR_INVALID_GEOMETRY;
+            h.t = INFINITY;
+            break;
+        }
+        h = trace_closest(s, org, dir, tc, INFINITY, false);
+    }
+    return h;
+}
 
 /* ---------------- geometry sampling ---------------- */
 
@@ -497,7 +575,7 @@ inline void hash_hit(uint32_t* h, const tr_hit& hit)
     *h = (*h ^ hit.prim) * 16777619u;
 }
 
-/* EvaluateLightSources (ray-tracing.cc:1809-1930), surfaces only. */
+/* EvaluateLightSources (ray-tracing.cc:1809-1930). */
 template <class T>
 V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const Sample<T>& sd,
                       uint32_t max_light_samples, uint32_t max_surface_samples, V3<T> pos, V3<T> throughput,
@@ -519,14 +597,33 @@ V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const
             V3<T> cur = throughput;
             float of[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
             float df[3] = {(float)smp.dir.x, (float)smp.dir.y, (float)smp.dir.z};
-            tr_hit h = trace_closest(s, of, df, 0.0f, (float)smp.dist, false);
-            if (h.geom == TR_INVALID_GEOMETRY || h.geom == smp.geom) {
+            float dist = (float)smp.dist;
+            tr_hit h = trace_closest(s, of, df, 0.0f, dist, false);
+            /* occlusion loop: volumes in the way scale the throughput by the
+             * Woodcock success ratio (ray-tracing.cc:1845-1926) */
+            float tf = 1.0f, step_through = 0.0f;
+            uint32_t step = 0, crossings = 0;
+            bool reached = false;
+            StepDraws dr = rng.steps(vertex, ORC_STREAM_SHADOW_VOLUME);
+            T tmax = max_value(throughput);
+            for (;;) {
+                if (h.geom == TR_INVALID_GEOMETRY || h.geom == smp.geom) { reached = true; break; }
+                if (!is_volume_geom(s, h.geom) || ++crossings > kOrcMaxVolumeCrossings) break;
+                const GridVolume& g = s->volumes[s->geoms[h.geom].volume];
+                uint32_t succ = woodcock_crossing(g, v3(of[0], of[1], of[2]), v3(df[0], df[1], df[2]), h.t, h.u, dr,
+                                                  (li << 24) | ((k & 0xffu) << 16), &step, &step_through);
+                tf = tf * ((float)succ / 2.0f);
+                if ((float)tmax * tf < 1e-3f) break;
+                if (step_through >= dist - 1e-3f) { reached = true; break; }
+                h = trace_closest(s, of, df, std::min(dist - 1e-3f, step_through), dist, false);
+            }
+            if (reached) {
                 if (apply_balance && area) {
                     T total = total_balance(s, (T)max_light_samples, (T)max_surface_samples, lsd, pos);
                     T balance = (light_pdf(s, L, pos, lsd.wi) * (T)samples) / total;
                     cur *= balance;
                 }
-                color += smp.radiance * cur * bsdf_eval(lsd) / smp.pdf;
+                color += (smp.radiance * cur * bsdf_eval(lsd) / smp.pdf) * (T)tf;
             }
         }
         radiance += color / (T)samples;
@@ -535,6 +632,40 @@ V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const
 }
 
 template <class T> inline V3<T> to_v3(const float* f) { return v3((T)f[0], (T)f[1], (T)f[2]); }
+
+/* Spectrum < TransmittanceThreshold: every component below 1e-3
+ * (vector3.hh:205, ray-tracing.cc:58). */
+template <class T> inline bool below_threshold(V3<T> t)
+{
+    return t.x < (T)1e-3f && t.y < (T)1e-3f && t.z < (T)1e-3f;
+}
+
+/* The shading point of a hit: surfaces through geom_desc->sample and the
+ * material cache; a volume collision through the fibre frame at the
+ * collision and DirectionalDensity = sigma(ray direction)
+ * (ray-tracing.cc:327-337).  Moves *pos to the hit; returns true for a
+ * volume. */
+template <class T> bool surface_or_volume(const orc_scene* s, const tr_hit& h, V3<T>* pos, Sample<T>* sd)
+{
+    const Geom& g = s->geoms[h.geom];
+    if (g.type == GEOM_VOLUME) {
+        const GridVolume& gv = s->volumes[g.volume];
+        sd->mat = &s->materials[g.material];
+        *pos = *pos + (T)h.t * sd->wi;
+        M3<T> b = grid_frame(gv, *pos);
+        sd->tangent = b.c0;
+        sd->binormal = b.c1;
+        sd->normal = b.c2;
+        sd->dir_density = sggx_projected_area(spec3<T>(sd->mat->sggx_stddev), b, sd->wi);
+        sd->position = *pos;
+        return true;
+    }
+    sample_geometry(s, h, *pos, sd->wi, sd);
+    *pos = *pos + (T)h.t * sd->wi;
+    sd->position = *pos;
+    material_cache(sd);
+    return false;
+}
 
 /* One camera sample of one pixel: the body of the camera loop of subdraw
  * (ray-tracing.cc:2011-2345) for surfaces. */
@@ -581,24 +712,29 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
     sd.total_density = (T)0;
     float of[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
     float df[3] = {(float)sd.wi.x, (float)sd.wi.y, (float)sd.wi.z};
-    tr_hit h = trace_closest(s, of, df, 0.0f, INFINITY, false);
+    tr_hit h = trace_segment(s, of, df, rng, 0);
     if (h.geom == TR_INVALID_GEOMETRY) {
         hash = (hash ^ 0xffffffffu) * 16777619u;
         *hash_out = repeat_hash(hash, s->n_gi);
         return to_v3<T>(s->background);
     }
     hash_hit(&hash, h);
-    sample_geometry(s, h, pos, sd.wi, &sd);
-    pos = pos + (T)h.t * sd.wi;
-    sd.position = pos;
-    material_cache(&sd);
+    bool volume = surface_or_volume(s, h, &pos, &sd);
     V3<T> start_T = v3((T)1, (T)1, (T)1);
+    if (volume) {
+        /* albedo * density / density (ray-tracing.cc:2134-2142) */
+        start_T *= spec3<T>(sd.mat->albedo);
+        if (below_threshold(start_T)) {
+            *hash_out = repeat_hash(hash, s->n_gi);
+            return v3((T)0, (T)0, (T)0);
+        }
+    }
     sd.wo = -sd.wi;
     if (is_emissive(sd.mat->model)) {
         *hash_out = repeat_hash(hash, s->n_gi);
         return start_T * spec3<T>(sd.mat->diffuse);
     }
-    V3<T> local = evaluate_lights(s, rng, 0, sd, s->n_light, s->n_gi, pos, start_T, s->max_depth != 0);
+    V3<T> local = evaluate_lights(s, rng, 0, sd, s->n_light, s->n_gi, pos, start_T, s->max_depth != 0 && !volume);
     V3<T> global = v3((T)0, (T)0, (T)0);
     uint32_t camera_hash = hash;
     uint32_t combined = 0;
@@ -611,6 +747,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
         Rng srng = rng;
         srng.split = split;
         Sample<T> cur = sd;
+        bool cur_volume = volume;
         uint32_t ss = s->n_gi, ls = s->n_light;
         V3<T> cpos = pos;
         V3<T> ct = start_T;
@@ -629,7 +766,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
                       (double)xi[2], (double)cur.wi.x, (double)cur.wi.y, (double)cur.wi.z, (double)cur.pdf);
             if (cur.pdf == (T)0) break;
             T balance = (T)1;
-            {
+            if (!cur_volume) {
                 T total = total_balance(s, (T)ls, (T)ss, cur, cpos);
                 balance = total > (T)0 ? ((T)ss * cur.pdf) / total : (T)0;
                 ct *= bsdf_eval(cur) / cur.pdf;
@@ -638,7 +775,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             }
             float o2[3] = {(float)cpos.x, (float)cpos.y, (float)cpos.z};
             float d2[3] = {(float)cur.wi.x, (float)cur.wi.y, (float)cur.wi.z};
-            tr_hit hh = trace_closest(s, o2, d2, 0.0f, INFINITY, false);
+            tr_hit hh = trace_segment(s, o2, d2, srng, d + 1);
             if (hh.geom == TR_INVALID_GEOMETRY) {
                 hash = (hash ^ 0xffffffffu) * 16777619u;
                 gsplit += to_v3<T>(s->background) * ct;
@@ -646,10 +783,11 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             }
             hash_hit(&hash, hh);
             ORC_TRACE("   hit geom=%u prim=%u t=%g\n", hh.geom, hh.prim, (double)hh.t);
-            sample_geometry(s, hh, cpos, cur.wi, &cur);
-            cpos = cpos + (T)hh.t * cur.wi;
-            cur.position = cpos;
-            material_cache(&cur);
+            cur_volume = surface_or_volume(s, hh, &cpos, &cur);
+            if (cur_volume) {
+                ct *= spec3<T>(cur.mat->albedo);
+                if (below_threshold(ct)) break;
+            }
             if (is_emissive(cur.mat->model)) {
                 gsplit += ct * balance * spec3<T>(cur.mat->diffuse);
                 break;
@@ -796,6 +934,52 @@ int orc_add_sphere(orc_scene* s, const float center[3], float radius, const tr_m
 {
     if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
     uint32_t slot = add_sphere_geom(s, center, radius, GEOM_SPHERE, intern_material(s, m), 0);
+    if (out) *out = slot;
+    return 0;
+}
+
+/* A grid volume (the build's addGridVolume): samples decoded from
+ * binary16, majorant = largest density (or max_density) * max(stddev). */
+int orc_add_grid_volume(orc_scene* s, const tr_grid_volume* v, const tr_material* m, uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    GridVolume g;
+    size_t n = (size_t)v->dims[0] * v->dims[1] * v->dims[2];
+    g.rho.resize(n);
+    g.dir.resize(n * 3);
+    float rho_max = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        g.rho[i] = half_bits_to_float(v->density[i]);
+        rho_max = std::max(rho_max, g.rho[i]);
+        for (int a = 0; a < 3; ++a) g.dir[i * 3 + a] = half_bits_to_float(v->direction[i * 3 + a]);
+    }
+    if (v->max_density != 0.0f) rho_max = v->max_density;
+    float smax = std::max(m->sggx_stddev[0], std::max(m->sggx_stddev[1], m->sggx_stddev[2]));
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = v->min_corner[a];
+        g.hi[a] = v->max_corner[a];
+        g.scale[a] = (float)(v->dims[a] - 1) / (v->max_corner[a] - v->min_corner[a]);
+        g.dims[a] = v->dims[a];
+        g.stddev[a] = m->sggx_stddev[a];
+    }
+    g.rcp_rho_max = 1.0f / rho_max;
+    g.rcp_majorant = 1.0f / (rho_max * smax);
+    g.rcp_smax = 1.0f / smax;
+    Geom gm;
+    memset(&gm, 0, sizeof(gm));
+    gm.type = GEOM_VOLUME;
+    gm.material = intern_material(s, m);
+    gm.volume = (uint32_t)s->volumes.size();
+    uint32_t slot = (uint32_t)s->geoms.size();
+    s->geoms.push_back(gm);
+    s->volumes.push_back(std::move(g));
+    Prim p;
+    memset(&p, 0, sizeof(p));
+    for (int a = 0; a < 3; ++a) { p.v0[a] = v->min_corner[a]; p.v1[a] = v->max_corner[a]; }
+    p.geom = slot;
+    p.prim = 0;
+    p.is_volume = 1;
+    s->prims.push_back(p);
     if (out) *out = slot;
     return 0;
 }
@@ -1029,6 +1213,108 @@ float orc_projected_triangle_area(const float* pos, const float* p0, const float
 {
     return projected_triangle_area(v3(pos[0], pos[1], pos[2]), v3(p0[0], p0[1], p0[2]), v3(p1[0], p1[1], p1[2]),
                                    v3(p2[0], p2[1], p2[2]));
+}
+
+/* Grid-volume lookups of the first volume of a committed scene at pos:
+ * out[0] density, out[1..9] fibre frame (tangent, binormal, normal),
+ * out[10] sigma(dir), out[11] sampleScattering's decision for `threshold`
+ * (0 rejected, 1 accepted, 2 zero density). */
+void orc_grid_volume_lookup(const orc_scene* s, const float* pos, const float* dir, float threshold, float* out)
+{
+    const GridVolume& g = s->volumes[0];
+    V3<float> p = v3(pos[0], pos[1], pos[2]), d = v3(dir[0], dir[1], dir[2]);
+    GridCell<float> c = grid_cell(g, p);
+    out[0] = grid_density(g, c);
+    M3<float> b = make_basis_tangent(grid_orientation(g, c));
+    float* o = out + 1;
+    o[0] = b.c0.x; o[1] = b.c0.y; o[2] = b.c0.z; o[3] = b.c1.x; o[4] = b.c1.y; o[5] = b.c1.z;
+    o[6] = b.c2.x; o[7] = b.c2.y; o[8] = b.c2.z;
+    out[10] = sggx_projected_area(v3(g.stddev[0], g.stddev[1], g.stddev[2]), b, d);
+    out[11] = (float)grid_accept(g, p, d, threshold);
+}
+
+/* Delta tracking of n rays through the scene with the Philox stream of
+ * (pixel = ray index, sample, split 0, vertex 0): the segment's closest hit
+ * after tracking (collision: geom of the volume, t = collision distance). */
+void orc_trace_segments(const orc_scene* s, size_t n, const tr_ray* rays, uint32_t sample, tr_hit* hits, int nthreads)
+{
+    parallel_for((uint32_t)n, nthreads, 64, [&](uint32_t i) {
+        Rng rng;
+        rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
+        rng.pixel = i; rng.sample = sample; rng.split = 0;
+        hits[i] = trace_segment(s, rays[i].org, rays[i].dir, rng, 0);
+    });
+}
+
+/* Woodcock transmittance estimate (0, 0.5 or 1 per crossing) of shadow
+ * rays org -> org + dist * dir through the scene's volumes, with throughput
+ * 1 and light geometry `light_geom` (TR_INVALID_GEOMETRY for none): the
+ * occlusion loop of EvaluateLightSources with the streams of (pixel = ray
+ * index, sample, split 0, vertex 0, light 0, sample 0). */
+void orc_shadow_transmittance(const orc_scene* s, size_t n, const tr_ray* rays, uint32_t sample, uint32_t light_geom,
+                              float* out, int nthreads)
+{
+    parallel_for((uint32_t)n, nthreads, 64, [&](uint32_t i) {
+        Rng rng;
+        rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
+        rng.pixel = i; rng.sample = sample; rng.split = 0;
+        const float* of = rays[i].org;
+        const float* df = rays[i].dir;
+        float dist = rays[i].tfar;
+        tr_hit h = trace_closest(s, of, df, 0.0f, dist, false);
+        float tf = 1.0f, step_through = 0.0f;
+        uint32_t step = 0, crossings = 0;
+        bool reached = false;
+        StepDraws dr = rng.steps(0, ORC_STREAM_SHADOW_VOLUME);
+        for (;;) {
+            if (h.geom == TR_INVALID_GEOMETRY || h.geom == light_geom) { reached = true; break; }
+            if (!is_volume_geom(s, h.geom) || ++crossings > kOrcMaxVolumeCrossings) break;
+            const GridVolume& g = s->volumes[s->geoms[h.geom].volume];
+            uint32_t succ = woodcock_crossing(g, v3(of[0], of[1], of[2]), v3(df[0], df[1], df[2]), h.t, h.u, dr, 0u,
+                                              &step, &step_through);
+            tf = tf * ((float)succ / 2.0f);
+            if (tf < 1e-3f) break;
+            if (step_through >= dist - 1e-3f) { reached = true; break; }
+            h = trace_closest(s, of, df, std::min(dist - 1e-3f, step_through), dist, false);
+        }
+        out[i] = reached ? tf : 0.0f;
+    });
+}
+
+/* Matrix3 operations of the phase sampling; matrices are 9 floats,
+ * column by column. */
+static M3<float> m9(const float* f) { return m3(v3(f[0], f[1], f[2]), v3(f[3], f[4], f[5]), v3(f[6], f[7], f[8])); }
+static void w9(const M3<float>& m, float* f)
+{
+    const V3<float>* c[3] = {&m.c0, &m.c1, &m.c2};
+    for (int i = 0; i < 3; ++i) { f[3 * i] = c[i]->x; f[3 * i + 1] = c[i]->y; f[3 * i + 2] = c[i]->z; }
+}
+void orc_make_basis_tangent(const float* t, float* out9) { w9(make_basis_tangent(v3(t[0], t[1], t[2])), out9); }
+void orc_cholesky(const float* a9, float* out9) { w9(cholesky(m9(a9)), out9); }
+void orc_transpose(const float* a9, float* out9) { w9(transpose(m9(a9)), out9); }
+void orc_mat_mul(const float* a9, const float* b9, float* out9) { w9(mul(m9(a9), m9(b9)), out9); }
+void orc_transform_covariance(const float* a9, const float* var, float* out9)
+{
+    w9(transform_covariance(m9(a9), v3(var[0], var[1], var[2])), out9);
+}
+void orc_transform_rotation_inverse(const float* a9, const float* v, float* out)
+{
+    V3<float> r = transform_rotation_inverse(m9(a9), v3(v[0], v[1], v[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* bsdf_eval / bsdf_pdf of one shading frame for `count` incident
+ * directions: out[4 i .. 4 i + 3] = BRDF*cos (RGB), PDF. */
+void orc_bsdf_eval_batch(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
+                         size_t count, const float* wi, float dir_density, float* out)
+{
+    Sample<float> s = unit_sample(m, n, t, b, wo, nullptr, dir_density);
+    for (size_t i = 0; i < count; ++i) {
+        s.wi = v3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        V3<float> f = bsdf_eval(s);
+        out[4 * i] = f.x; out[4 * i + 1] = f.y; out[4 * i + 2] = f.z;
+        out[4 * i + 3] = bsdf_pdf(s);
+    }
 }
 
 /* SGGX helpers: out[0] = SGGXMicroFlakeNDF(stddev, basis, dir),

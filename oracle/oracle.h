@@ -37,6 +37,8 @@ int orc_add_triangle_mesh(orc_scene* scene, const float world[16], size_t submes
                           uint64_t* geom_ids);
 int orc_add_sphere(orc_scene* scene, const float center[3], float radius,
                    const tr_material* material, uint64_t* out_geom_id);
+int orc_add_grid_volume(orc_scene* scene, const tr_grid_volume* volume, const tr_material* material,
+                        uint64_t* out_geom_id);
 int orc_add_light_source(orc_scene* scene, const tr_light* light, uint32_t* out_index);
 int orc_add_sphere_light_source(orc_scene* scene, const tr_light* light, uint32_t* out_geom_id);
 int orc_commit(orc_scene* scene);

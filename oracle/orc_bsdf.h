@@ -322,15 +322,77 @@ template <class T> inline V3<T> sggx_surface_brdf(const Sample<T>& s)
     return result;
 }
 
+/* ---------------- SGGX micro-flake phase function ---------------- */
+
+template <class T> inline M3<T> transpose(const M3<T>& m)
+{
+    return m3(v3(m.c0.x, m.c1.x, m.c2.x), v3(m.c0.y, m.c1.y, m.c2.y), v3(m.c0.z, m.c1.z, m.c2.z));
+}
+
+/* Matrix3::choleskyDecomposition (matrix3.hh:173-191) */
+template <class T> inline M3<T> cholesky(const M3<T>& m)
+{
+    T l00 = xsqrt(m.c0.x);
+    T rcp_l00 = (T)1 / l00;
+    T l10 = rcp_l00 * m.c0.y;
+    T l20 = rcp_l00 * m.c0.z;
+    T l11 = xsqrt(m.c1.y - l10 * l10);
+    T rcp_l11 = (T)1 / l11;
+    T l21 = rcp_l11 * (m.c1.z - l20 * l10);
+    T l22 = xsqrt(m.c2.z - l21 * l21 - l20 * l20);
+    return m3(v3(l00, l10, l20), v3((T)0, l11, l21), v3((T)0, (T)0, l22));
+}
+
+template <class T> inline M3<T> sample_frame(const Sample<T>& s) { return m3(s.tangent, s.binormal, s.normal); }
+
+/* SGGXMicroFlakeBRDF (ggx-models-impl.hh:60-68, 466-474) */
+template <class T> inline T sggx_flake_brdf(const Sample<T>& s)
+{
+    V3<T> mn = normalize(s.wi + s.wo);
+    return sggx_ndf(spec3<T>(s.mat->sggx_stddev), sample_frame(s), mn) * (T)2 / ((T)4 * s.dir_density);
+}
+
+/* SGGXMicroFlakePDF (ggx-models-impl.hh:447-464) */
+template <class T> inline T sggx_flake_pdf(T dir_density, V3<T> stddev, const M3<T>& basis, V3<T> mn)
+{
+    T ndf = sggx_ndf(stddev, basis, mn);
+    T proj_area = (T)1 / ((T)4 * dir_density);
+    return (T)2 * ndf * proj_area;
+}
+
+/* SGGXMicroFlakeSampleIncidentLight (ggx-models-impl.hh:554-580) with
+ * UniformSampleSphere (sampling3.hh:106-116). */
+template <class T> inline void sggx_flake_sample(Sample<T>* s, T ra, T rb)
+{
+    T sin_phi, cos_phi;
+    sincos_turns(ra, &sin_phi, &cos_phi);
+    T cos_theta = (T)2 * rb - (T)1;
+    T sin_theta = xsqrt((T)1 - cos_theta * cos_theta);
+    V3<T> us = v3(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta);
+    V3<T> sd = spec3<T>(s->mat->sggx_stddev);
+    M3<T> basis = sample_frame(*s);
+    M3<T> S = transform_covariance(basis, sd * sd);
+    M3<T> ob = make_basis(s->wo);
+    S = mul(mul(ob, S), transpose(ob));
+    V3<T> vn = normalize(transform_rotation_inverse(cholesky(S), us));
+    vn = transform(ob, vn);
+    s->wi = reflect(s->wo, vn);
+    s->pdf = sggx_flake_pdf(s->dir_density, sd, basis, normalize(s->wi + s->wo));
+}
+
 inline bool is_emissive(uint32_t m) { return m == TR_MODEL_EMISSIVE; }
 
 /* TransmittanceLookup (illumination-models.hh:279-324) for the models the
- * tracer supports. */
+ * tracer supports (SGGXMicroFlake only inside grid volumes). */
 template <class T> inline V3<T> bsdf_eval(const Sample<T>& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
+    if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
+        T p = sggx_flake_brdf(s);
+        return v3(p, p, p);
+    }
     return v3((T)0, (T)0, (T)0);
 }
 
@@ -340,6 +402,8 @@ template <class T> inline T bsdf_pdf(const Sample<T>& s)
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
+    if (m == TR_MODEL_SGGX_MICRO_FLAKE)
+        return sggx_flake_pdf(s.dir_density, spec3<T>(s.mat->sggx_stddev), sample_frame(s), normalize(s.wi + s.wo));
     return (T)0;
 }
 
@@ -349,6 +413,7 @@ template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, co
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi);
+    else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi[0], xi[1]);
     else s->pdf = (T)0;
 }
 

@@ -76,6 +76,18 @@ class tr_stats(ctypes.Structure):
         ("frame_ms", ctypes.c_double), ("trace_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
         ("shadow_ms", ctypes.c_double), ("other_ms", ctypes.c_double), ("trace_launches", ctypes.c_uint64),
         ("node_visits", ctypes.c_uint64), ("prim_tests", ctypes.c_uint64), ("counted_rays", ctypes.c_uint64),
+        ("volume_steps", ctypes.c_uint64),
+    ]
+
+
+class tr_grid_volume(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_uint32 * 3),
+        ("min_corner", ctypes.c_float * 3),
+        ("max_corner", ctypes.c_float * 3),
+        ("density", ctypes.POINTER(ctypes.c_uint16)),
+        ("direction", ctypes.POINTER(ctypes.c_uint16)),
+        ("max_density", ctypes.c_float),
     ]
 
 
@@ -128,6 +140,7 @@ SIGNATURES = {
     "tr_scene_add_triangle_mesh": (_st, [_vp, P(ctypes.c_float), _sz, P(tr_submesh), _sz, P(ctypes.c_int32),
                                          _sz, _vp, P(tr_mesh_options), P(_u64)]),
     "tr_scene_add_sphere": (_st, [_vp, P(ctypes.c_float), ctypes.c_float, P(tr_material), P(_u64)]),
+    "tr_scene_add_grid_volume": (_st, [_vp, P(tr_grid_volume), P(tr_material), P(_u64)]),
     "tr_scene_commit": (_st, [_vp]),
     "tr_scene_get_info": (_st, [_vp, P(tr_scene_info)]),
     "tr_scene_draw_once": (_st, [_vp]),

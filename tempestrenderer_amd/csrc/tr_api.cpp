@@ -15,6 +15,7 @@
 #include "tr_bvh8.h"
 #include "tr_kernels.h"
 #include "tr_traverse.h"
+#include "tr_volume.h"
 
 #include <hip/hip_runtime.h>
 
@@ -91,6 +92,8 @@ struct tr_scene {
     std::vector<GeomRec> geoms;
     std::vector<PrimRec> prims_in;
     std::vector<LightRec> lights;
+    std::vector<VolumeRec> vols;
+    std::vector<uint64_t> voxels;   /* binary16 x4 samples (dir x, y, z, density) */
     uint32_t instances = 0;
     bool committed = false;
     bool lights_dirty = false;
@@ -112,12 +115,15 @@ struct tr_scene {
     DevBuf<LightRec> d_lights;
     DevBuf<uint8_t> d_vpool;
     DevBuf<int32_t> d_ipool;
+    DevBuf<VolumeRec> d_vols;
+    DevBuf<uint64_t> d_voxels;
 
     /* path pool */
     uint32_t capacity = 0;
     uint32_t max_shadow = 0;
     DevBuf<float4> ray_o, ray_d, hit, thr, local, global, sh_o, sh_d, sh_c;
     DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, sh_n, counters;
+    DevBuf<float> sh_tmax;
     DevBuf<uint2> spill;
     DevBuf<uint64_t> counts;
 
@@ -166,7 +172,7 @@ tr_scene::~tr_scene()
     ev_used.clear();
     release_events(this);
     d_nodes.release(); d_prims.release(); d_geoms.release(); d_mats.release(); d_lights.release();
-    d_vpool.release(); d_ipool.release();
+    d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); sh_tmax.release();
     ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
     sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
     shq.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release();
@@ -184,6 +190,9 @@ uint32_t intern_material(tr_scene* s, const tr_material* m)
     return (uint32_t)(s->mats.size() - 1);
 }
 
+/* Surface models of meshes and spheres.  Volume models belong to grid
+ * volumes only: the reference treats a hit on a volume-model geometry as a
+ * volume set (ray-tracing.cc:2108-2110). */
 bool model_supported(uint32_t m)
 {
     return m == TR_MODEL_EMISSIVE || (m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC) ||
@@ -222,8 +231,11 @@ SceneView scene_view(const tr_scene* s)
     v.lights = s->d_lights.ptr;
     v.vpool = s->d_vpool.ptr;
     v.ipool = s->d_ipool.ptr;
+    v.volumes = s->d_vols.ptr;
+    v.voxels = s->d_voxels.ptr;
     v.n_lights = (uint32_t)s->lights.size();
     v.n_prims = (uint32_t)s->prims.size();
+    v.n_volumes = (uint32_t)s->vols.size();
     return v;
 }
 
@@ -268,6 +280,7 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->q1.reserve(cap));
     TR_HIP(s->shq.reserve(cap));
     TR_HIP(s->sh_n.reserve(cap));
+    TR_HIP(s->sh_tmax.reserve(cap));
     TR_HIP(s->sh_o.reserve((size_t)cap * msh));
     TR_HIP(s->sh_d.reserve((size_t)cap * msh));
     TR_HIP(s->sh_c.reserve((size_t)cap * msh));
@@ -285,7 +298,7 @@ PathPool path_pool(const tr_scene* s)
     p.ray_o = s->ray_o.ptr; p.ray_d = s->ray_d.ptr; p.hit = s->hit.ptr; p.hit_prim = s->hit_prim.ptr;
     p.thr = s->thr.ptr; p.local = s->local.ptr; p.global = s->global.ptr; p.hash = s->hash.ptr;
     p.queue[0] = s->q0.ptr; p.queue[1] = s->q1.ptr; p.shadow_queue = s->shq.ptr;
-    p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr;
+    p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr; p.sh_tmax = s->sh_tmax.ptr;
     p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity;
     return p;
 }
@@ -323,6 +336,9 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
     if (rsq * rsq != s->n_gi) return fail(TR_ERROR_INVALID_ARG, "SamplesGlobalIllumination must be a perfect square");
     if (s->bvh_stats.depth + 2 > (uint32_t)(kStackLds + kStackSpill))
         return fail(TR_ERROR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    if (s->n_light > 65535) return fail(TR_ERROR_UNSUPPORTED, "SamplesLocalAreaLight above 65535");
+    if (!s->vols.empty() && (s->lights.size() > 256 || s->n_light > 256))
+        return fail(TR_ERROR_UNSUPPORTED, "scenes with volumes take at most 256 lights and 256 samples per area light");
     tr_status st = sync_lights(s);
     if (st != TR_OK) return st;
     uint64_t per_pixel = (uint64_t)s->spp * s->n_gi;
@@ -337,24 +353,23 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         counts = s->counts.ptr;
         TR_HIP(hipMemsetAsync(counts, 0, 8 * sizeof(uint64_t), s->stream));
     }
-    /* Per-stage HIP events are recorded only in timing mode; there every
-     * end event is waited on before the next launch, so each event pair
-     * brackets exactly one kernel (the runtime may otherwise defer kernel
-     * submission and attach an event to an earlier command). */
+    /* In timing mode every launch carries its own start / stop events in
+     * the dispatch (hipExtLaunchKernelGGL) and is waited for, so the pair
+     * measures the kernel alone, without launch gaps. */
     auto stage = [&](int cat, auto&& launch) -> tr_status {
         if (!s->timing) {
-            launch();
+            launch((const LaunchEvents*)nullptr);
             TR_HIP(hipGetLastError());
             return TR_OK;
         }
-        hipEvent_t a = get_event(s), b = get_event(s);
-        if (!a || !b) return fail(TR_ERROR_DEVICE, "event creation failed");
-        TR_HIP(hipEventRecord(a, s->stream));
-        launch();
+        LaunchEvents ev;
+        ev.start = get_event(s);
+        ev.stop = get_event(s);
+        if (!ev.start || !ev.stop) return fail(TR_ERROR_DEVICE, "event creation failed");
+        launch(&ev);
         TR_HIP(hipGetLastError());
-        TR_HIP(hipEventRecord(b, s->stream));
-        TR_HIP(hipEventSynchronize(b));
-        record(s, cat, a, b);
+        TR_HIP(hipEventSynchronize(ev.stop));
+        record(s, cat, ev.start, ev.stop);
         return TR_OK;
     };
     for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += ppb) {
@@ -362,22 +377,31 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         uint32_t nslots = (uint32_t)(npx * per_pixel);
         uint64_t slot_base = pix0 * per_pixel;
         tr_status st2;
-        if ((st2 = stage(EV_OTHER, [&] { launch_raygen(s->stream, p, pool, out, slot_base, nslots); })) != TR_OK)
+        if ((st2 = stage(EV_OTHER, [&](const LaunchEvents* ev) {
+                 launch_raygen(s->stream, p, pool, out, slot_base, nslots, ev);
+             })) != TR_OK)
             return st2;
         int cur = 0, nxt = 1;
         for (uint32_t v = 0; v <= s->max_depth; ++v) {
-            if ((st2 = stage(EV_TRACE, [&] { launch_trace(s->stream, p.sc, pool, cur, nxt, grid, counts); })) != TR_OK)
+            if ((st2 = stage(EV_TRACE, [&](const LaunchEvents* ev) {
+                     launch_trace(s->stream, p, pool, out, slot_base, v, cur, nxt, grid, counts, ev);
+                 })) != TR_OK)
                 return st2;
-            if ((st2 = stage(EV_SHADE, [&] { launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots); })) !=
-                TR_OK)
+            if ((st2 = stage(EV_SHADE, [&](const LaunchEvents* ev) {
+                     launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots, ev);
+                 })) != TR_OK)
                 return st2;
             if (p.sc.n_lights > 0) {
-                if ((st2 = stage(EV_SHADOW, [&] { launch_shadow(s->stream, p, pool, v, grid, counts); })) != TR_OK)
+                if ((st2 = stage(EV_SHADOW, [&](const LaunchEvents* ev) {
+                         launch_shadow(s->stream, p, pool, out, slot_base, v, grid, counts, ev);
+                     })) != TR_OK)
                     return st2;
             }
             std::swap(cur, nxt);
         }
-        if ((st2 = stage(EV_OTHER, [&] { launch_accumulate(s->stream, p, pool, out, slot_base, npx); })) != TR_OK)
+        if ((st2 = stage(EV_OTHER, [&](const LaunchEvents* ev) {
+                 launch_accumulate(s->stream, p, pool, out, slot_base, npx, ev);
+             })) != TR_OK)
             return st2;
     }
     return TR_OK;
@@ -403,10 +427,11 @@ void collect_stats(tr_scene* s, uint64_t paths)
     s->stats.other_ms = ms[EV_OTHER];
     s->stats.trace_launches = traces;
     s->stats.node_visits = s->stats.prim_tests = s->stats.counted_rays = 0;
-    s->stats.segments = s->stats.shadow_rays = 0;
+    s->stats.segments = s->stats.shadow_rays = s->stats.volume_steps = 0;
     if (s->counting) {
         /* [0..2] closest-hit kernel: nodes, primitives, rays;
-         * [3] shadow rays, [4..5] shadow kernel: nodes, primitives */
+         * [3] shadow rays, [4..5] shadow kernel: nodes, primitives;
+         * [6] tentative volume collisions of both kernels */
         uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (hipMemcpy(c, s->counts.ptr, sizeof(c), hipMemcpyDeviceToHost) == hipSuccess) {
             s->stats.node_visits = c[0];
@@ -414,6 +439,7 @@ void collect_stats(tr_scene* s, uint64_t paths)
             s->stats.counted_rays = c[2];
             s->stats.segments = c[2];
             s->stats.shadow_rays = c[3];
+            s->stats.volume_steps = c[6];
         }
     }
 }
@@ -745,6 +771,75 @@ tr_status tr_scene_add_sphere(tr_scene* s, const float center[3], float radius, 
     return TR_OK;
 }
 
+tr_status tr_scene_add_grid_volume(tr_scene* s, const tr_grid_volume* v, const tr_material* m, uint64_t* out)
+{
+    if (!valid(s) || !v || !m || !v->density || !v->direction) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "scene already committed");
+    if (m->model != TR_MODEL_SGGX_MICRO_FLAKE)
+        return fail(TR_ERROR_UNSUPPORTED, "grid volumes take the SGGXMicroFlake volume model");
+    for (int a = 0; a < 3; ++a) {
+        if (v->dims[a] < 2) return fail(TR_ERROR_INVALID_ARG, "volume dims must be at least 2 per axis");
+        if (!(v->max_corner[a] > v->min_corner[a]) || !std::isfinite(v->min_corner[a]) ||
+            !std::isfinite(v->max_corner[a]))
+            return fail(TR_ERROR_INVALID_ARG, "volume bounds must be finite with min < max");
+        if (!(m->sggx_stddev[a] > 0.0f) || !std::isfinite(m->sggx_stddev[a]))
+            return fail(TR_ERROR_INVALID_ARG, "sggx_stddev must be positive and finite");
+    }
+    uint64_t n = (uint64_t)v->dims[0] * v->dims[1] * v->dims[2];
+    if (n > (1ull << 31)) return fail(TR_ERROR_INVALID_ARG, "volume larger than 2^31 samples");
+    float rho_max = 0.0f;
+    for (uint64_t i = 0; i < n; ++i) {
+        float r = h2f(v->density[i]);
+        if (!(r >= 0.0f) || !std::isfinite(r))
+            return fail(TR_ERROR_INVALID_ARG, "volume density must be finite and non-negative");
+        rho_max = std::max(rho_max, r);
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(h2f(v->direction[i * 3 + a])))
+                return fail(TR_ERROR_INVALID_ARG, "volume direction must be finite");
+    }
+    if (v->max_density != 0.0f) {
+        if (!(v->max_density >= rho_max) || !std::isfinite(v->max_density))
+            return fail(TR_ERROR_INVALID_ARG, "max_density below the grid's largest density");
+        rho_max = v->max_density;
+    }
+    if (!(rho_max > 0.0f)) return fail(TR_ERROR_INVALID_ARG, "volume density is zero everywhere");
+    VolumeRec r;
+    memset(&r, 0, sizeof(r));
+    float smax = std::max(m->sggx_stddev[0], std::max(m->sggx_stddev[1], m->sggx_stddev[2]));
+    for (int a = 0; a < 3; ++a) {
+        r.lo[a] = v->min_corner[a];
+        r.hi[a] = v->max_corner[a];
+        r.scale[a] = (float)(v->dims[a] - 1u) / (v->max_corner[a] - v->min_corner[a]);
+        r.stddev[a] = m->sggx_stddev[a];
+        r.dims[a] = v->dims[a];
+    }
+    r.rcp_rho_max = 1.0f / rho_max;
+    r.rcp_majorant = 1.0f / (rho_max * smax);
+    r.rcp_smax = 1.0f / smax;
+    r.voxel_base = s->voxels.size();
+    s->voxels.reserve(s->voxels.size() + n);
+    for (uint64_t i = 0; i < n; ++i)
+        s->voxels.push_back((uint64_t)v->direction[i * 3] | ((uint64_t)v->direction[i * 3 + 1] << 16) |
+                            ((uint64_t)v->direction[i * 3 + 2] << 32) | ((uint64_t)v->density[i] << 48));
+    GeomRec g;
+    memset(&g, 0, sizeof(g));
+    g.type = GEOM_VOLUME;
+    g.material = intern_material(s, m);
+    g.tri_count = (uint32_t)s->vols.size();
+    uint32_t slot = (uint32_t)s->geoms.size();
+    s->geoms.push_back(g);
+    s->vols.push_back(r);
+    PrimRec p;
+    memset(&p, 0, sizeof(p));
+    for (int a = 0; a < 3; ++a) { p.v0[a] = r.lo[a]; p.v1[a] = r.hi[a]; }
+    p.prim = 0;
+    p.geom = slot;
+    p.flags = PRIM_VOLUME;
+    s->prims_in.push_back(p);
+    if (out) *out = slot;
+    return TR_OK;
+}
+
 tr_status tr_scene_commit(tr_scene* s)
 {
     if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
@@ -765,6 +860,8 @@ tr_status tr_scene_commit(tr_scene* s)
     if ((st = upload(s, s->d_mats, s->mats)) != TR_OK) return st;
     if ((st = upload(s, s->d_vpool, s->vpool)) != TR_OK) return st;
     if ((st = upload(s, s->d_ipool, s->ipool)) != TR_OK) return st;
+    if ((st = upload(s, s->d_vols, s->vols)) != TR_OK) return st;
+    if ((st = upload(s, s->d_voxels, s->voxels)) != TR_OK) return st;
     s->lights_dirty = true;
     if ((st = sync_lights(s)) != TR_OK) return st;
     TR_HIP(hipStreamSynchronize(s->stream));
@@ -1005,6 +1102,7 @@ tr_status tr_debug_bvh_check(const tr_scene* s, uint64_t* violations)
                     for (int a = 0; a < 3; ++a) {
                         float pl, ph;
                         if (p.flags & PRIM_SPHERE) { pl = p.v0[a] - p.v1[0]; ph = p.v0[a] + p.v1[0]; }
+                        else if (p.flags & PRIM_VOLUME) { pl = p.v0[a]; ph = p.v1[a]; }
                         else {
                             pl = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
                             ph = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));

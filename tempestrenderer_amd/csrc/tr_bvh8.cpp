@@ -262,6 +262,8 @@ bool build_bvh8(const std::vector<PrimRec>& in, std::vector<Node8>& nodes_out, s
         Box bx;
         if (p.flags & PRIM_SPHERE) {
             for (int a = 0; a < 3; ++a) { bx.lo[a] = p.v0[a] - p.v1[0]; bx.hi[a] = p.v0[a] + p.v1[0]; }
+        } else if (p.flags & PRIM_VOLUME) {
+            for (int a = 0; a < 3; ++a) { bx.lo[a] = p.v0[a]; bx.hi[a] = p.v1[a]; }
         } else {
             for (int a = 0; a < 3; ++a) {
                 bx.lo[a] = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));

@@ -50,6 +50,7 @@ struct PathPool {
     float4* sh_d;       /* direction, .w = light geometry (bits) */
     float4* sh_c;       /* contribution, .w = light index (bits) */
     uint32_t* sh_n;     /* entries per slot */
+    float* sh_tmax;     /* largest path-throughput component at the shading point */
     uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size */
     uint2* spill;       /* traversal stack spill, per persistent thread */
     uint32_t capacity;
@@ -64,20 +65,28 @@ struct BatchOut {
     uint64_t* count_out;  /* node/prim counters when counting, may be null */
 };
 
+/* Timing mode: the launch records `start` and `stop` as part of the kernel
+ * dispatch itself (hipExtLaunchKernelGGL), so the pair measures exactly the
+ * kernel's execution, as a kernel-trace profiler does. */
+struct LaunchEvents {
+    hipEvent_t start, stop;
+};
+
 constexpr int kTraceBlock = 64;       /* one wave per block */
 constexpr int kStackLds = 16;         /* stack entries per lane kept in LDS */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
 
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                   uint32_t n_slots);
-void launch_trace(hipStream_t s, const SceneView& sc, const PathPool& pool, int cur, int nxt, uint32_t grid,
-                  uint64_t* counts);
+                   uint32_t n_slots, const LaunchEvents* ev = nullptr);
+void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                  uint32_t vertex, int cur, int nxt, uint32_t grid, uint64_t* counts, const LaunchEvents* ev = nullptr);
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                  uint32_t vertex, int cur, int nxt, uint32_t n_slots);
-void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, uint32_t vertex, uint32_t grid,
-                   uint64_t* counts);
+                  uint32_t vertex, int cur, int nxt, uint32_t n_slots, const LaunchEvents* ev = nullptr);
+void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
+                   uint64_t slot_base, uint32_t vertex, uint32_t grid, uint64_t* counts,
+                   const LaunchEvents* ev = nullptr);
 void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
-                       uint64_t slot_base, uint32_t n_pixels);
+                       uint64_t slot_base, uint32_t n_pixels, const LaunchEvents* ev = nullptr);
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
                        uint2* spill, uint32_t grid);
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,

@@ -16,10 +16,19 @@
  * Philox counter of every random number: (pixel, camera sample,
  * split << 16 | vertex << 8 | stream, sub).  The CPU oracle uses the same
  * layout, so a path is identical on both sides.
+ *
+ * Scenes with grid volumes use the VOL variants of the traversal kernels:
+ * a closest hit on a volume box runs delta tracking in the same thread and
+ * either turns into a scattering vertex or re-traces beyond the volume;
+ * shadow rays through volumes take the reference's two-repetition Woodcock
+ * estimate.
  */
 #include "tr_kernels.h"
 #include "tr_shading.h"
 #include "tr_traverse.h"
+#include "tr_volume.h"
+
+#include <hip/hip_ext.h>
 
 namespace trk {
 
@@ -132,11 +141,12 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, Ba
 
 /* ---------------- closest-hit trace (persistent) ---------------- */
 
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool pool, int cur, int nxt,
-                                                       unsigned long long* counts)
+template <bool COUNT, bool VOL>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                                       uint32_t vertex, int cur, int nxt, unsigned long long* counts)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
+    const SceneView& sc = p.sc;
     uint32_t lane = threadIdx.x;
     uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     if (gtid == 0) {
@@ -148,7 +158,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool po
     DevStack st;
     st.lds = &stk[0][lane];
     st.spill = pool.spill + (size_t)gtid * kStackSpill;
-    uint32_t nv = 0, np = 0, nr = 0;
+    uint32_t nv = 0, np = 0, nr = 0, ns = 0;
     /* static interleaved assignment: wave w takes rays w*64 + k*grid*64 */
     const uint32_t stride = gridDim.x * kTraceBlock;
     for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
@@ -161,6 +171,37 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool po
             st.sp = 0;
             Hit h = traverse_closest(sc.nodes, sc.prims, r, 0.0f, INFINITY, st, COUNT ? &nv : nullptr,
                                      COUNT ? &np : nullptr);
+            if (VOL) {
+                /* the tracking loops of subdraw (ray-tracing.cc:2040-2128, 2212-2272) */
+                uint32_t step = 0, crossings = 0;
+                Rng rng;
+                bool have_rng = false;
+                while (h.geom != TR_INVALID_GEOMETRY && sc.geoms[h.geom].type == GEOM_VOLUME) {
+                    if (!have_rng) {
+                        uint32_t pixel, cs, split;
+                        uint64_t pseq;
+                        slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+                        rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
+                        have_rng = true;
+                    }
+                    const VolumeRec& V = sc.volumes[sc.geoms[h.geom].tri_count];
+                    float tc;
+                    if (delta_track(sc.voxels, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
+                                    COUNT ? &ns : nullptr)) {
+                        h.t = tc;
+                        h.u = h.v = 0.0f;
+                        break;
+                    }
+                    if (++crossings >= kMaxVolumeCrossings) {
+                        h.geom = h.prim = TR_INVALID_GEOMETRY;
+                        h.t = INFINITY;
+                        break;
+                    }
+                    st.sp = 0;
+                    h = traverse_closest(sc.nodes, sc.prims, r, tc, INFINITY, st, COUNT ? &nv : nullptr,
+                                         COUNT ? &np : nullptr);
+                }
+            }
             pool.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.geom));
             pool.hit_prim[slot] = h.prim;
             ++nr;
@@ -170,11 +211,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sc, PathPool po
         atomicAdd(&counts[0], (unsigned long long)nv);
         atomicAdd(&counts[1], (unsigned long long)np);
         atomicAdd(&counts[2], (unsigned long long)nr);
+        if (VOL) atomicAdd(&counts[6], (unsigned long long)ns);
     }
 }
 
 /* ---------------- shading ---------------- */
 
+template <bool VOL>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
                                                uint32_t vertex, int cur, int nxt)
 {
@@ -210,11 +253,33 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
             hash = fnv_hit(hash, geom, prim);
             Sample sd;
             sd.dir_density = 0.0f;
-            sample_geometry(sc, geom, prim, h4.x, h4.y, h4.z, org, dir, &sd);
-            V3 pos = add(org, smul(h4.x, dir));
-            sd.wi = dir;
-            material_cache(&sd);
-            if (is_emissive(sd.mat->model)) {
+            V3 pos;
+            const GeomRec& G = sc.geoms[geom];
+            bool volume = VOL && G.type == GEOM_VOLUME;
+            if (volume) {
+                /* RTVolumeSet::sampleScattering's frame and DirectionalDensity
+                 * at the accepted collision (ray-tracing.cc:327-337) */
+                const VolumeRec& V = sc.volumes[G.tri_count];
+                pos = ray_point(org, dir, h4.x);
+                M3 b = volume_frame(fetch_cell(sc.voxels, V, pos));
+                sd.mat = &sc.mats[G.material];
+                sd.tangent = b.c0;
+                sd.binormal = b.c1;
+                sd.normal = b.c2;
+                sd.wi = dir;
+                sd.dir_density = sggx_projected_area(ld3(V.stddev), b, dir);
+                /* throughput *= albedo; a path below TransmittanceThreshold
+                 * ends (ray-tracing.cc:2136-2142, 2277-2284) */
+                T = mulv(T, ld3(sd.mat->albedo));
+            } else {
+                sample_geometry(sc, geom, prim, h4.x, h4.y, h4.z, org, dir, &sd);
+                pos = add(org, smul(h4.x, dir));
+                sd.wi = dir;
+                material_cache(&sd);
+            }
+            if (volume && T.x < 1e-3f && T.y < 1e-3f && T.z < 1e-3f) {
+                /* absorbed: no contribution from this vertex on */
+            } else if (is_emissive(sd.mat->model)) {
                 V3 le = ld3(sd.mat->diffuse);
                 if (vertex == 0) { local = mulv(T, le); write_local = true; }
                 else global = add(global, mulv(muls(T, bal), le));
@@ -222,7 +287,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 sd.wo = neg(dir);
                 uint32_t ls = vertex == 0 ? p.n_light : 1u;
                 uint32_t ss = vertex == 0 ? p.n_gi : 1u;
-                bool apply_balance = vertex == 0 ? (p.max_depth != 0) : (vertex != p.max_depth);
+                bool apply_balance = vertex == 0 ? (p.max_depth != 0 && !volume) : (vertex != p.max_depth);
                 /* next-event estimation: the camera vertex's is evaluated once, by split 0 */
                 if (vertex != 0 || split == 0) {
                     uint32_t ne = 0;
@@ -245,11 +310,12 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                             size_t e = (size_t)ne * pool.capacity + slot;
                             pool.sh_o[e] = st4(pos, smp.dist);
                             pool.sh_d[e] = st4(smp.dir, __uint_as_float(smp.geom));
-                            pool.sh_c[e] = st4(contrib, __uint_as_float(li));
+                            pool.sh_c[e] = st4(contrib, __uint_as_float((li << 16) | k));
                             ++ne;
                         }
                     }
                     pool.sh_n[slot] = ne;
+                    if (VOL) pool.sh_tmax[slot] = max_value(T);
                     shadow = ne > 0;
                 }
                 bool alive = true;
@@ -273,9 +339,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                     U4 xb = rng.draw(vertex, STREAM_BSDF, 0u);
                     bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c));
                     if (sd.pdf != 0.0f) {
-                        float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
-                        float nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
-                        T = mulv(T, divs(bsdf_eval(sd), sd.pdf));
+                        /* volume vertices keep balance 1 and the throughput
+                         * (phase / pdf = 1, ray-tracing.cc:2182-2201) */
+                        float nb = 1.0f;
+                        if (!volume) {
+                            float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
+                            nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
+                            T = mulv(T, divs(bsdf_eval(sd), sd.pdf));
+                        }
                         pool.ray_o[slot] = st4(pos, 0.0f);
                         pool.ray_d[slot] = st4(sd.wi, 0.0f);
                         pool.thr[slot] = st4(T, nb);
@@ -294,9 +365,9 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
 
 /* ---------------- shadow rays (persistent) ---------------- */
 
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool pool, uint32_t vertex,
-                                                        unsigned long long* counts)
+template <bool COUNT, bool VOL>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                                        uint32_t vertex, unsigned long long* counts)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
     uint32_t lane = threadIdx.x;
@@ -306,13 +377,22 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
     DevStack st;
     st.lds = &stk[0][lane];
     st.spill = pool.spill + (size_t)gtid * kStackSpill;
-    uint32_t nv = 0, np = 0, nr = 0;
+    uint32_t nv = 0, np = 0, nr = 0, ns = 0;
     const uint32_t stride = gridDim.x * kTraceBlock;
     for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
         uint32_t qi = base + lane;
         if (qi < n) {
             uint32_t slot = pool.shadow_queue[qi];
             uint32_t ne = pool.sh_n[slot];
+            Rng rng;
+            float tmax_T = 0.0f;
+            if (VOL) {
+                uint32_t pixel, cs, split;
+                uint64_t pseq;
+                slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+                rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
+                tmax_T = pool.sh_tmax[slot];
+            }
             uint32_t ls = vertex == 0 ? p.n_light : 1u;
             V3 radiance = mk(0.0f, 0.0f, 0.0f);
             uint32_t e = 0;
@@ -322,14 +402,47 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                 while (e < ne) {
                     size_t idx = (size_t)e * pool.capacity + slot;
                     float4 c4 = pool.sh_c[idx];
-                    if (__float_as_uint(c4.w) != li) break;
+                    uint32_t tag = __float_as_uint(c4.w);
+                    if ((tag >> 16) != li) break;
                     float4 o4 = pool.sh_o[idx], d4 = pool.sh_d[idx];
                     float org[3] = {o4.x, o4.y, o4.z}, dir[3] = {d4.x, d4.y, d4.z};
                     RayPre r = ray_setup(org, dir);
+                    uint32_t light_geom = __float_as_uint(d4.w);
                     st.sp = 0;
                     Hit h = traverse_closest(sc.nodes, sc.prims, r, 0.0f, o4.w, st, COUNT ? &nv : nullptr,
                                              COUNT ? &np : nullptr);
-                    if (h.geom == TR_INVALID_GEOMETRY || h.geom == __float_as_uint(d4.w)) color = add(color, ld4(c4));
+                    if (!VOL) {
+                        if (h.geom == TR_INVALID_GEOMETRY || h.geom == light_geom) color = add(color, ld4(c4));
+                    } else {
+                        /* EvaluateLightSources' occlusion loop with Woodcock
+                         * transmittance (ray-tracing.cc:1845-1926) */
+                        float dist = o4.w, tf = 1.0f, step_through = 0.0f;
+                        uint32_t step = 0, crossings = 0;
+                        uint32_t sub_hi = ((tag >> 16) << 24) | ((tag & 0xffu) << 16);
+                        bool reached = false;
+                        for (;;) {
+                            if (h.geom == TR_INVALID_GEOMETRY || h.geom == light_geom) {
+                                reached = true;
+                                break;
+                            }
+                            const GeomRec& G = sc.geoms[h.geom];
+                            if (G.type != GEOM_VOLUME || ++crossings > kMaxVolumeCrossings) break;
+                            const VolumeRec& V = sc.volumes[G.tri_count];
+                            uint32_t succ = woodcock(sc.voxels, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
+                                                     &step, &step_through, COUNT ? &ns : nullptr);
+                            tf = tf * ((float)succ / 2.0f);
+                            if (tmax_T * tf < 1e-3f) break;
+                            if (step_through >= dist - 1e-3f) {
+                                reached = true;
+                                break;
+                            }
+                            float tn = dist - 1e-3f < step_through ? dist - 1e-3f : step_through;
+                            st.sp = 0;
+                            h = traverse_closest(sc.nodes, sc.prims, r, tn, dist, st, COUNT ? &nv : nullptr,
+                                                 COUNT ? &np : nullptr);
+                        }
+                        if (reached) color = add(color, muls(ld4(c4), tf));
+                    }
                     ++e;
                     ++nr;
                 }
@@ -347,6 +460,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
         atomicAdd(&counts[4], (unsigned long long)nv);
         atomicAdd(&counts[5], (unsigned long long)np);
         atomicAdd(&counts[3], (unsigned long long)nr);
+        if (VOL) atomicAdd(&counts[6], (unsigned long long)ns);
     }
 }
 
@@ -405,11 +519,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView sc, const 
     for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
         uint32_t i = base + lane;
         if (i < n) {
-            tr_ray ry = rays[i];
-            RayPre r = ray_setup(ry.org, ry.dir);
+            const float4* rv = reinterpret_cast<const float4*>(rays + i);
+            float4 a = rv[0], b = rv[1];
+            float org[3] = {a.x, a.y, a.z}, dir[3] = {b.x, b.y, b.z};
+            RayPre r = ray_setup(org, dir);
             st.sp = 0;
-            Hit h = traverse_closest(sc.nodes, sc.prims, r, ry.tnear, ry.tfar, st, (uint32_t*)nullptr,
-                                     (uint32_t*)nullptr);
+            Hit h = traverse_closest(sc.nodes, sc.prims, r, a.w, b.w, st, (uint32_t*)nullptr, (uint32_t*)nullptr);
             tr_hit o;
             o.t = h.t; o.u = h.u; o.v = h.v; o.geom = h.geom; o.prim = h.prim;
             hits[i] = o;
@@ -448,47 +563,61 @@ __global__ __launch_bounds__(256) void k_resolve(const double* accum, double rcp
 
 }  // namespace
 
-void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                   uint32_t n_slots)
+/* Plain launch, or one whose dispatch records the timing events. */
+template <typename K, typename... A>
+void launch_k(K kernel, uint32_t grid, uint32_t block, hipStream_t s, const LaunchEvents* ev, A... args)
 {
-    uint32_t grid = (n_slots + 255) / 256;
-    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, p, pool, out, slot_base, n_slots);
+    if (ev) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, ev->start, ev->stop, 0u, args...);
+    else kernel<<<dim3(grid), dim3(block), 0, s>>>(args...);
 }
 
-void launch_trace(hipStream_t s, const SceneView& sc, const PathPool& pool, int cur, int nxt, uint32_t grid,
-                  uint64_t* counts)
+void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                   uint32_t n_slots, const LaunchEvents* ev)
 {
-    if (counts)
-        hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(kTraceBlock), 0, s, sc, pool, cur, nxt,
-                           (unsigned long long*)counts);
-    else
-        hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(kTraceBlock), 0, s, sc, pool, cur, nxt,
-                           (unsigned long long*)nullptr);
+    launch_k(k_raygen, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_slots);
+}
+
+void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
+                  uint32_t vertex, int cur, int nxt, uint32_t grid, uint64_t* counts, const LaunchEvents* ev)
+{
+    unsigned long long* c = (unsigned long long*)counts;
+    bool vol = p.sc.n_volumes > 0;
+    if (counts) {
+        if (vol) launch_k(k_trace<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+        else launch_k(k_trace<true, false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+    } else {
+        if (vol) launch_k(k_trace<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+        else launch_k(k_trace<false, false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+    }
 }
 
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                  uint32_t vertex, int cur, int nxt, uint32_t n_slots)
+                  uint32_t vertex, int cur, int nxt, uint32_t n_slots, const LaunchEvents* ev)
 {
-    uint32_t grid = (n_slots + 255) / 256;
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, p, pool, out, slot_base, vertex, cur, nxt);
+    if (p.sc.n_volumes > 0)
+        launch_k(k_shade<true>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt);
+    else
+        launch_k(k_shade<false>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt);
 }
 
-void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, uint32_t vertex, uint32_t grid,
-                   uint64_t* counts)
+void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
+                   uint64_t slot_base, uint32_t vertex, uint32_t grid, uint64_t* counts, const LaunchEvents* ev)
 {
-    if (counts)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kTraceBlock), 0, s, p, pool, vertex,
-                           (unsigned long long*)counts);
-    else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kTraceBlock), 0, s, p, pool, vertex,
-                           (unsigned long long*)nullptr);
+    unsigned long long* c = (unsigned long long*)counts;
+    bool vol = p.sc.n_volumes > 0;
+    if (counts) {
+        if (vol) launch_k(k_shadow<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, c);
+        else launch_k(k_shadow<true, false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, c);
+    } else {
+        if (vol) launch_k(k_shadow<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, c);
+        else launch_k(k_shadow<false, false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, c);
+    }
 }
 
 void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
-                       uint64_t slot_base, uint32_t n_pixels)
+                       uint64_t slot_base, uint32_t n_pixels, const LaunchEvents* ev)
 {
-    uint32_t grid = (n_pixels + 255) / 256;
-    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, p, pool, out, slot_base, n_pixels);
+    launch_k(k_accumulate, (n_pixels + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_pixels);
 }
 
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n, uint2* spill,

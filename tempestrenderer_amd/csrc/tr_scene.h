@@ -7,7 +7,8 @@
  *               vertices of a triangle (or centre and radius of a sphere),
  *               its primitive id, geometry slot and flags.  Traversal reads
  *               nothing else.
- *   GeomRec[]   one per geometry slot (submesh, sphere, sphere light).
+ *   GeomRec[]   one per geometry slot (submesh, sphere, sphere light, volume).
+ *   VolumeRec[] and the packed binary16 samples of grid volumes.
  *   tr_material[], LightRec[], vertex bytes and int32 index triplets for
  *               shading normals / tangents and mesh-light sampling.
  */
@@ -43,7 +44,9 @@ struct alignas(128) Node8 {
 };
 static_assert(sizeof(Node8) == 128, "Node8 must be one 128-byte line");
 
-enum { PRIM_SPHERE = 1u, PRIM_TWO_SIDED = 2u };
+/* Primitive kinds.  A grid volume is one PRIM_VOLUME primitive: its box
+ * lo = v0, hi = v1. */
+enum { PRIM_SPHERE = 1u, PRIM_TWO_SIDED = 2u, PRIM_VOLUME = 4u };
 
 struct alignas(16) PrimRec {
     float v0[3];
@@ -55,7 +58,7 @@ struct alignas(16) PrimRec {
 };
 static_assert(sizeof(PrimRec) == 48, "PrimRec must be 48 bytes");
 
-enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2 };
+enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2, GEOM_VOLUME = 3 };
 
 struct GeomRec {
     uint32_t type;
@@ -64,9 +67,28 @@ struct GeomRec {
     uint32_t stride;
     uint64_t vbase;       /* byte offset of the submesh's vertex 0 in the vertex pool */
     uint32_t index_base;  /* offset of the submesh's first index in the index pool */
-    uint32_t tri_count;
+    uint32_t tri_count;   /* volumes: index into SceneView::volumes */
     float sphere[4];
 };
+
+/* A grid volume as the tracking code reads it.  Samples are packed per
+ * grid node as four binary16 values (direction x, y, z, density): one
+ * 8-byte load per trilinear corner.  */
+struct VolumeRec {
+    float lo[3];
+    float rcp_rho_max;    /* 1 / max density */
+    float hi[3];
+    float rcp_majorant;   /* 1 / (max density * max(stddev)) */
+    float scale[3];       /* (dims - 1) / (hi - lo) */
+    float rcp_smax;       /* 1 / max(stddev) */
+    float stddev[3];      /* SGGX standard deviation of the fibre frame (tangent, binormal, normal) */
+    uint32_t pad0;
+    uint32_t dims[3];
+    uint32_t pad1;
+    uint64_t voxel_base;  /* first sample in SceneView::voxels */
+    uint64_t pad2;
+};
+static_assert(sizeof(VolumeRec) == 96, "VolumeRec layout");
 
 struct LightRec {
     tr_light src;
@@ -83,8 +105,11 @@ struct SceneView {
     const LightRec* lights;
     const uint8_t* vpool;
     const int32_t* ipool;
+    const VolumeRec* volumes;
+    const uint64_t* voxels; /* packed binary16 x4 samples of every grid volume */
     uint32_t n_lights;
     uint32_t n_prims;
+    uint32_t n_volumes;
 };
 
 }  // namespace trk

@@ -284,16 +284,87 @@ TR_HD V3 sggx_surface_brdf(const Sample& s)
     return result;
 }
 
+/* ---------------- SGGX micro-flake volume phase function ---------------- */
+
+/* Matrix3::transpose and choleskyDecomposition (matrix3.hh:92-97, 173-191). */
+TR_HD M3 transpose(const M3& m)
+{
+    return m3(mk(m.c0.x, m.c1.x, m.c2.x), mk(m.c0.y, m.c1.y, m.c2.y), mk(m.c0.z, m.c1.z, m.c2.z));
+}
+TR_HD M3 cholesky(const M3& m)
+{
+    float l00 = sqrtf(m.c0.x);
+    float rcp_l00 = 1.0f / l00;
+    float l10 = rcp_l00 * m.c0.y;
+    float l20 = rcp_l00 * m.c0.z;
+    float l11 = sqrtf(m.c1.y - l10 * l10);
+    float rcp_l11 = 1.0f / l11;
+    float l21 = rcp_l11 * (m.c1.z - l20 * l10);
+    float l22 = sqrtf(m.c2.z - l21 * l21 - l20 * l20);
+    return m3(mk(l00, l10, l20), mk(0.0f, l11, l21), mk(0.0f, 0.0f, l22));
+}
+
+TR_HD M3 frame_of(const Sample& s) { return m3(s.tangent, s.binormal, s.normal); }
+
+/* SGGXMicroFlakeBRDF (ggx-models-impl.hh:60-68, 466-474): the phase
+ * function 2 D(h) / (4 sigma(wo)), h = normalize(wi + wo). */
+TR_HD float sggx_flake_phase(const Sample& s)
+{
+    V3 mn = normalize(add(s.wi, s.wo));
+    return sggx_ndf(ld3(s.mat->sggx_stddev), frame_of(s), mn) * 2.0f / (4.0f * s.dir_density);
+}
+
+/* SGGXMicroFlakePDF (ggx-models-impl.hh:447-464): the same expression,
+ * evaluated as 2 * D * (1 / (4 sigma)). */
+TR_HD float sggx_flake_pdf_h(float dir_density, V3 stddev, const M3& basis, V3 mn)
+{
+    float ndf = sggx_ndf(stddev, basis, mn);
+    float proj_area = 1.0f / (4.0f * dir_density);
+    return 2.0f * ndf * proj_area;
+}
+TR_HD float sggx_flake_pdf(const Sample& s)
+{
+    return sggx_flake_pdf_h(s.dir_density, ld3(s.mat->sggx_stddev), frame_of(s), normalize(add(s.wi, s.wo)));
+}
+
+/* SGGXMicroFlakeSampleIncidentLight (ggx-models-impl.hh:554-580): a
+ * uniform sphere sample (UniformSampleSphere, sampling3.hh:106-116) mapped
+ * through the Cholesky factor of S expressed in the frame of wo, reflected
+ * about the resulting flake normal. */
+TR_HD void sggx_flake_sample(Sample* s, float ra, float rb)
+{
+    float sin_phi, cos_phi;
+    tr_sincos_turns(ra, &sin_phi, &cos_phi);
+    float cos_theta = 2.0f * rb - 1.0f;
+    float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    V3 u = mk(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta);
+    V3 sd = ld3(s->mat->sggx_stddev);
+    M3 basis = frame_of(*s);
+    M3 S = transform_covariance(basis, mulv(sd, sd));
+    M3 ob = make_basis(s->wo);
+    S = mul(mul(ob, S), transpose(ob));
+    V3 vn = normalize(transform_rotation_inverse(cholesky(S), u));
+    vn = transform(ob, vn);
+    s->wi = reflect(s->wo, vn);
+    s->pdf = sggx_flake_pdf_h(s->dir_density, sd, basis, normalize(add(s->wi, s->wo)));
+}
+
 TR_HD bool is_emissive(uint32_t m) { return m == TR_MODEL_EMISSIVE; }
+TR_HD bool is_volume_model(uint32_t m) { return m == TR_MODEL_SGGX_MICRO_FLAKE; }
 
 /* TransmittanceLookup / PDFLookup / SampleIncidentLightLookup /
  * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
- * models: GGX and Beckmann (plain, conductor, dielectric), SGGX surface. */
+ * models: GGX and Beckmann (plain, conductor, dielectric), SGGX surface,
+ * SGGX micro-flake volume. */
 TR_HD V3 bsdf_eval(const Sample& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
+    if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
+        float p = sggx_flake_phase(s);
+        return mk(p, p, p);
+    }
     return mk(0.0f, 0.0f, 0.0f);
 }
 TR_HD float bsdf_pdf(const Sample& s)
@@ -301,6 +372,7 @@ TR_HD float bsdf_pdf(const Sample& s)
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
+    if (m == TR_MODEL_SGGX_MICRO_FLAKE) return sggx_flake_pdf(s);
     return 0.0f;
 }
 TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2)
@@ -308,6 +380,7 @@ TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float 
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi0, xi1, xi2);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi0, xi1);
+    else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi0, xi1);
     else s->pdf = 0.0f;
 }
 TR_HD void material_cache(Sample* s)

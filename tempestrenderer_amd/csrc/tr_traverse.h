@@ -134,11 +134,37 @@ TR_HD bool hit_better(float t, uint32_t g, uint32_t p, const Hit& b)
     return p < b.prim;
 }
 
+/* A grid volume's box: the ray enters at max(slab entry, tnear) if the
+ * slab interval ends beyond tnear and the entry lies before tfar.  The
+ * entry is the hit distance and the slab exit is returned in *t_exit (the
+ * hit's u), where delta tracking stops.  A ray that starts inside enters
+ * at tnear, as the reference's volume intersection does at tnear
+ * (ray-tracing.cc:551-571). */
+TR_HD bool volume_box_hit(const RayPre& r, const float* lo, const float* hi, float tnear, float tfar, float* t_out,
+                          float* t_exit)
+{
+    float t0 = -INFINITY, t1 = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        float ta = (lo[a] - r.org[a]) * r.inv[a];
+        float tb = (hi[a] - r.org[a]) * r.inv[a];
+        float mn = ta < tb ? ta : tb, mx = ta < tb ? tb : ta;
+        t0 = mn > t0 ? mn : t0;
+        t1 = mx < t1 ? mx : t1;
+    }
+    if (!(t0 <= t1) || !(t1 > tnear)) return false;
+    float t = t0 > tnear ? t0 : tnear;
+    if (!(t < tfar)) return false;
+    *t_out = t;
+    *t_exit = t1;
+    return true;
+}
+
 TR_HD void test_prim(const PrimRec& p, const RayPre& r, float tnear, float tfar, Hit* best)
 {
     float t, u = 0.0f, v = 0.0f;
     bool hit;
     if (p.flags & PRIM_SPHERE) hit = sphere_hit(r, p.v0, p.v1[0], tnear, tfar, &t);
+    else if (p.flags & PRIM_VOLUME) hit = volume_box_hit(r, p.v0, p.v1, tnear, tfar, &t, &u);
     else hit = watertight_triangle(r, p.v0, p.v1, p.v2, (p.flags & PRIM_TWO_SIDED) != 0, tnear, tfar, &t, &u, &v);
     if (hit && hit_better(t, p.geom, p.prim, *best)) {
         best->t = t; best->u = u; best->v = v; best->geom = p.geom; best->prim = p.prim;

@@ -1,0 +1,255 @@
+/*
+ * tr_volume.h - grid volumes: sample lookup, fibre orientation, delta
+ * tracking and the ratio-tracking shadow estimate.
+ *
+ * Restates for the build's grid volumes (one box primitive per volume, the
+ * samples in HBM as binary16 x4 per grid node) the reference's
+ *   RTVolumeSet::sampleScattering  (src/graphics/ray-tracing/ray-tracing.cc:245-340)
+ *   SampleVolumePoint              (:110-182, structure tensor + power iteration)
+ *   the tracking loops of subdraw  (:2108-2123, 2259-2270)
+ *   the shadow-ray Woodcock loop   (:1876-1926)
+ * Tracking runs in the ray parameter t of the ray that entered the volume:
+ * the tentative collision is org + t * dir, and the walk leaves the volume
+ * when t reaches the box exit.  A point of zero density continues the walk
+ * (the reference re-traces from it, which with a box primitive returns the
+ * same volume at distance 0).
+ */
+#ifndef TR_VOLUME_H
+#define TR_VOLUME_H
+
+#include "tr_math.h"
+#include "tr_scene.h"
+
+namespace trk {
+
+/* binary16 -> binary32, exact. */
+TR_HD float h2f(uint32_t h)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+#else
+    uint32_t s = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0) {
+        float v = (float)m * 5.9604644775390625e-08f;
+        return s ? -v : v;
+    }
+    if (e == 31) return tr_u2f(s | 0x7f800000u | (m << 13));
+    return tr_u2f(s | ((e + 112u) << 23) | (m << 13));
+#endif
+}
+
+/* The eight grid samples around a point and its position in the cell. */
+struct VoxelCell {
+    uint64_t s[8];   /* corner k: x from bit 0, y from bit 1, z from bit 2 */
+    float f[3];      /* fractional position in the cell */
+};
+
+TR_HD VoxelCell fetch_cell(const uint64_t* voxels, const VolumeRec& V, V3 pos)
+{
+    VoxelCell c;
+    float p[3] = {(pos.x - V.lo[0]) * V.scale[0], (pos.y - V.lo[1]) * V.scale[1], (pos.z - V.lo[2]) * V.scale[2]};
+    uint32_t i0[3], i1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float dm = (float)(V.dims[a] - 1u);
+        float v = p[a] > 0.0f ? p[a] : 0.0f;
+        v = v < dm ? v : dm;
+        float fl = floorf(v);
+        i0[a] = (uint32_t)fl;
+        i1[a] = i0[a] + 1u < V.dims[a] ? i0[a] + 1u : i0[a];
+        c.f[a] = v - fl;
+    }
+    const uint64_t* base = voxels + V.voxel_base;
+    uint64_t sx = V.dims[0], sxy = (uint64_t)V.dims[0] * V.dims[1];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint64_t idx = (uint64_t)((k & 4) ? i1[2] : i0[2]) * sxy + (uint64_t)((k & 2) ? i1[1] : i0[1]) * sx +
+                       (uint64_t)((k & 1) ? i1[0] : i0[0]);
+        c.s[k] = base[idx];
+    }
+    return c;
+}
+
+/* Trilinear density in the reference's evaluation order
+ * (TrilinearInterpolation, ray-tracing.cc:89-108): corner weights
+ * (wx * wy) * wz, the two z layers summed per (x, y) corner, then the four
+ * corner sums pairwise. */
+TR_HD float cell_density(const VoxelCell& c)
+{
+    float fx = c.f[0], fy = c.f[1], fz = c.f[2];
+    float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    float wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+    float lane[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float r0 = h2f((uint32_t)(c.s[k] >> 48)), r1 = h2f((uint32_t)(c.s[k + 4] >> 48));
+        lane[k] = r0 * (wxy[k] * gz) + r1 * (wxy[k] * fz);
+    }
+    return (lane[0] + lane[1]) + (lane[2] + lane[3]);
+}
+
+/* Fibre direction at the point: dominant eigenvector of the structure
+ * tensor sum_k w_k d_k d_k^T by eight power iterations from (1,1,1)/sqrt(3)
+ * (SampleVolumePoint, ray-tracing.cc:110-182).  The corner weights are the
+ * reference's: the lower corner of an axis weighs f, the upper 1 - f.  A
+ * zero result (zero tensor) falls back to +z. */
+TR_HD V3 cell_orientation(const VoxelCell& c)
+{
+    float lf[3] = {c.f[0], c.f[1], c.f[2]};
+    float uf[3] = {1.0f - c.f[0], 1.0f - c.f[1], 1.0f - c.f[2]};
+    float t00 = 0.0f, t01 = 0.0f, t02 = 0.0f, t11 = 0.0f, t12 = 0.0f, t22 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float w = ((k & 4) ? uf[2] : lf[2]) * ((k & 2) ? uf[1] : lf[1]) * ((k & 1) ? uf[0] : lf[0]);
+        uint64_t s = c.s[k];
+        float d0 = h2f((uint32_t)(s & 0xffffu)), d1 = h2f((uint32_t)((s >> 16) & 0xffffu)),
+              d2 = h2f((uint32_t)((s >> 32) & 0xffffu));
+        t00 += w * d0 * d0;
+        t01 += w * d0 * d1;
+        t02 += w * d0 * d2;
+        t11 += w * d1 * d1;
+        t12 += w * d1 * d2;
+        t22 += w * d2 * d2;
+    }
+    V3 v = mk(0.0f, 0.0f, 0.0f);
+    if (t00 != 0.0f || t01 != 0.0f || t02 != 0.0f || t11 != 0.0f || t12 != 0.0f || t22 != 0.0f) {
+        const float r3 = 0.5773502691f;
+        v = mk(r3, r3, r3);
+        for (int i = 0; i < 8; ++i) {
+            v = mk(t00 * v.x + t01 * v.y + t02 * v.z, t01 * v.x + t11 * v.y + t12 * v.z,
+                   t02 * v.x + t12 * v.y + t22 * v.z);
+            float l = length(v);
+            if (l != 0.0f) {
+                float rl = 1.0f / l;
+                v = mk(v.x * rl, v.y * rl, v.z * rl);
+            }
+        }
+    }
+    if (v.x == 0.0f && v.y == 0.0f && v.z == 0.0f) v = mk(0.0f, 0.0f, 1.0f);
+    return v;
+}
+
+/* Matrix3::makeBasisTangent (matrix3.hh:110-120). */
+TR_HD M3 make_basis_tangent(V3 tan)
+{
+    V3 d0 = cross(mk(0.0f, 1.0f, 0.0f), tan), d1 = cross(mk(0.0f, 0.0f, 1.0f), tan);
+    float d0_len = dot(d0, d0), d1_len = dot(d1, d1);
+    V3 b = d0_len > d1_len ? divs(d0, sqrtf(d0_len)) : divs(d1, sqrtf(d1_len));
+    return m3(tan, b, cross(tan, b));
+}
+
+/* Position along a ray, the same expression everywhere. */
+TR_HD V3 ray_point(V3 org, V3 dir, float t) { return add(org, smul(t, dir)); }
+
+/* The fibre frame at a point and sigma(dir) of the SGGX micro-flakes. */
+TR_HD M3 volume_frame(const VoxelCell& c) { return make_basis_tangent(cell_orientation(c)); }
+
+/* One tentative collision: 0 = rejected, 1 = accepted (scatter / absorb),
+ * 2 = zero density (the reference's sampleScattering returns false). */
+TR_HD int volume_accept(const uint64_t* voxels, const VolumeRec& V, V3 pos, V3 dir, float threshold)
+{
+    VoxelCell c = fetch_cell(voxels, V, pos);
+    float rho = cell_density(c);
+    if (rho == 0.0f) return 2;
+    float dens = rho * V.rcp_rho_max;
+    if (dens < threshold) return 0;
+    M3 basis = volume_frame(c);
+    float sigma = length(mulv(transform_rotation_inverse(basis, dir), ld3(V.stddev)));
+    return dens * sigma * V.rcp_smax >= threshold ? 1 : 0;
+}
+
+/* Tracking budget: a walk that makes this many tentative collisions in one
+ * vertex scatters at the last one (the step length is 1 / majorant, so this
+ * is only reached by rays that cross more than 2^20 mean free paths). */
+constexpr uint32_t kMaxVolumeSteps = 1u << 20;
+/* Shadow rays: a Woodcock walk beyond 2^17 steps counts as absorbed, and a
+ * shadow ray crosses at most 256 volume boxes. */
+constexpr uint32_t kMaxShadowSteps = 1u << 17;
+constexpr uint32_t kMaxVolumeCrossings = 256u;
+
+/* Philox streams of the volume walks: delta tracking of the path vertex
+ * (sub = step pair) and ratio tracking of shadow rays (sub = light << 24 |
+ * light sample << 16 | step pair).  Each draw covers two steps: words
+ * (a, b) then (c, d) = (free-flight, acceptance). */
+enum { STREAM_VOLUME = 4, STREAM_SHADOW_VOLUME = 5 };
+
+/* Delta tracking from the volume entry t0 to its exit t1.  Returns true and
+ * the collision distance in *t when a collision is accepted, false and the
+ * distance beyond the exit where the walk left the volume otherwise.
+ * `step` counts tentative collisions of this path vertex across volumes. */
+TR_HD bool delta_track(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+                       const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
+{
+    float tc = t0;
+    U4 r;
+    bool have = false;
+    for (;;) {
+        uint32_t k = *step;
+        if (!have || (k & 1u) == 0u) r = rng.draw(vertex, STREAM_VOLUME, k >> 1);
+        have = true;
+        uint32_t xa = (k & 1u) ? r.c : r.a, xb = (k & 1u) ? r.d : r.b;
+        *step = k + 1u;
+        if (nsteps) ++*nsteps;
+        /* d = logf(xi) * RcpMaxExtinction; pos -= dir * d */
+        tc = tc - tr_logf(u01(xa)) * V.rcp_majorant;
+        if (tc >= t1) {
+            *t = tc;
+            return false;
+        }
+        if (k + 1u >= kMaxVolumeSteps) {
+            *t = tc;
+            return true;
+        }
+        int acc = volume_accept(voxels, V, ray_point(org, dir, tc), dir, u01(xb));
+        if (acc == 1) {
+            *t = tc;
+            return true;
+        }
+    }
+}
+
+/* The two Woodcock repetitions of a shadow ray through one volume crossing
+ * (ray-tracing.cc:1889-1916).  A repetition succeeds when its walk leaves
+ * the volume (or meets zero density) and records where in *step_through;
+ * one that accepts a collision is absorbed.  Each repetition starts at
+ * max(*step_through, t0): the reference starts at *step_through, measured
+ * from the shadow-ray origin, which skips a volume lying further than one
+ * free path from the shading point.  Returns the number of successes. */
+TR_HD uint32_t woodcock(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+                        const Rng& rng, uint32_t vertex, uint32_t sub_hi, uint32_t* step, float* step_through,
+                        uint32_t* nsteps)
+{
+    uint32_t success = 0;
+    U4 r;
+    bool have = false;
+    for (int rep = 0; rep < 2; ++rep) {
+        float sd = *step_through > t0 ? *step_through : t0;
+        for (;;) {
+            uint32_t k = *step;
+            if (k >= kMaxShadowSteps) break;
+            if (!have || (k & 1u) == 0u) r = rng.draw(vertex, STREAM_SHADOW_VOLUME, sub_hi | (k >> 1));
+            have = true;
+            uint32_t xa = (k & 1u) ? r.c : r.a, xb = (k & 1u) ? r.d : r.b;
+            *step = k + 1u;
+            if (nsteps) ++*nsteps;
+            sd = sd - tr_logf(u01(xa)) * V.rcp_majorant;
+            if (sd >= t1) {
+                *step_through = sd;
+                ++success;
+                break;
+            }
+            int acc = volume_accept(voxels, V, ray_point(org, dir, sd), dir, u01(xb));
+            if (acc == 2) {
+                *step_through = sd;
+                ++success;
+                break;
+            }
+            if (acc == 1) break;
+        }
+    }
+    return success;
+}
+
+}  // namespace trk
+
+#endif

@@ -78,6 +78,25 @@ def submesh_array(submeshes):
     return arr
 
 
+def grid_volume_record(dims, min_corner, max_corner, density, direction, max_density=0.0):
+    """A tr_grid_volume over binary16 copies of the arrays (returned too, to
+    keep them alive for the call)."""
+    d = np.ascontiguousarray(np.asarray(density).astype(np.float16, copy=False).reshape(-1)).view(np.uint16)
+    v = np.ascontiguousarray(np.asarray(direction).astype(np.float16, copy=False).reshape(-1)).view(np.uint16)
+    n = int(dims[0]) * int(dims[1]) * int(dims[2])
+    if d.size != n or v.size != 3 * n:
+        raise ValueError("density / direction sizes do not match dims")
+    rec = _abi.tr_grid_volume()
+    for a in range(3):
+        rec.dims[a] = int(dims[a])
+        rec.min_corner[a] = float(min_corner[a])
+        rec.max_corner[a] = float(max_corner[a])
+    rec.density = d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+    rec.direction = v.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+    rec.max_density = float(max_density)
+    return rec, (d, v)
+
+
 class RayTracerScene:
     """The MI355X tracer behind the reference scene API."""
 
@@ -164,6 +183,17 @@ class RayTracerScene:
         gid = ctypes.c_uint64()
         check(self.lib.tr_scene_add_sphere(self.h, _fptr(c), float(radius), ctypes.byref(material.rec),
                                            ctypes.byref(gid)))
+        self._keep.append(material)
+        return gid.value
+
+    def addGridVolume(self, dims, min_corner, max_corner, density, direction, material, max_density=0.0):
+        """A grid volume (tr_scene_add_grid_volume): `density` binary16
+        [nz, ny, nx] (or flat, x fastest), `direction` binary16 [..., 3];
+        float arrays are converted to binary16.  Returns the geometry slot."""
+        rec, keep = grid_volume_record(dims, min_corner, max_corner, density, direction, max_density)
+        gid = ctypes.c_uint64()
+        check(self.lib.tr_scene_add_grid_volume(self.h, ctypes.byref(rec), ctypes.byref(material.rec),
+                                                ctypes.byref(gid)))
         self._keep.append(material)
         return gid.value
 

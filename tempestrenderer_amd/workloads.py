@@ -14,6 +14,13 @@ Workloads:
                    amplitude 0.1; 2*(n-1)^2 triangles) + 64 spheres with GGX,
                    Beckmann and SGGX-surface materials; constant environment
                    light 1.0; 1920x1080, 64 camera samples, depth 8.
+  volume(...)      a unit-cube grid volume, n^3 binary16 density from 3-D fBm
+                   value noise scaled to a mean extinction of 4 per unit, a
+                   per-sample fibre direction from the gradient of a second
+                   noise field, SGGX micro-flakes with standard deviation
+                   (0.1, 1, 1) in the fibre frame, albedo 0.9; constant
+                   environment light 1.0; 1024x1024, 128 camera samples,
+                   depth 32 (optionally a floor and a directional light).
 Scene randomness comes from numpy's PCG64 seeded with `seed`; the path
 tracer's own random numbers come from Philox keyed by the render seed.
 """
@@ -26,6 +33,7 @@ EMISSIVE = 0
 GGX = 11
 GGX_CONDUCTOR = 12
 GGX_DIELECTRIC = 13
+SGGX_MICRO_FLAKE = 22
 SGGX_SURFACE = 24
 BECKMANN = 28
 BECKMANN_CONDUCTOR = 29
@@ -74,6 +82,7 @@ class SceneDesc:
         self.spheres = []      # (center, radius, material)
         self.lights = []       # dict(kind=..., ...)
         self.sphere_lights = []
+        self.volumes = []      # (dims, min, max, density f16 [nz, ny, nx], direction f16 [nz, ny, nx, 3], material)
         self.settings = dict(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=1.0,
                              background=(0.0, 0.0, 0.0), seed=1)
 
@@ -108,6 +117,8 @@ class SceneDesc:
             target.addTriangleMesh(IDENTITY, subs, tris, verts, two_sided=two_sided)
         for center, radius, m in self.spheres:
             target.addSphere(center, radius, mat(m))
+        for dims, lo, hi, dens, dirs, m in self.volumes:
+            target.addGridVolume(dims, lo, hi, dens, dirs, mat(m))
         for L in self.sphere_lights:
             target.addSphereLightSource(target.make_light(**L))
         for L in self.lights:
@@ -247,6 +258,59 @@ def heightfield(width=1920, height=1080, samples_camera=64, max_depth=8, grid=10
             m = dict(model=GGX_CONDUCTOR, diffuse=(0.0, 0.0, 0.0), specular=col,
                      specular_power=(alpha_to_specular_power(a),) * 2, fresnel=(0.2, 3.0))
         d.spheres.append(((x, y, z), r, m))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(1.0, 1.0, 1.0), seed=1)
+    return d
+
+
+def value_noise_3d(n, octaves=6, base_freq=4, seed=1):
+    """fBm of trilinear value noise (smoothstep weights) on an n^3 grid over
+    the unit cube, lacunarity 2, gain 0.5, normalised to about [-1, 1];
+    float32 [z, y, x]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs = np.linspace(0.0, 1.0, n)
+    out = np.zeros((n, n, n), dtype=np.float32)
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for _ in range(octaves):
+        lat = rng.uniform(-1.0, 1.0, size=(freq + 1, freq + 1, freq + 1)).astype(np.float32)
+        g = xs * freq
+        i0 = np.minimum(g.astype(np.int64), freq - 1)
+        f = g - i0
+        w = (f * f * (3.0 - 2.0 * f)).astype(np.float32)
+        a = lat[:, :, i0] * (1.0 - w)[None, None, :] + lat[:, :, i0 + 1] * w[None, None, :]   # x
+        b = a[:, i0, :] * (1.0 - w)[None, :, None] + a[:, i0 + 1, :] * w[None, :, None]       # y
+        out += amp * (b[i0] * (1.0 - w)[:, None, None] + b[i0 + 1] * w[:, None, None])       # z
+        norm += amp
+        amp *= 0.5
+        freq *= 2
+    return out / np.float32(norm)
+
+
+def volume_grid(n, mean_density=4.0, seed=1):
+    """Density (binary16, mean `mean_density` per unit for sigma = 1) and
+    unit fibre directions (binary16 x3) of the volume workload."""
+    noise = value_noise_3d(n, seed=seed)
+    raw = np.maximum(noise * 1.5 + 0.35, 0.0)
+    dens = (raw * np.float32(mean_density / max(float(raw.mean()), 1e-6))).astype(np.float16)
+    field = value_noise_3d(n, octaves=3, base_freq=3, seed=seed + 17)
+    g = np.stack(np.gradient(field), axis=-1)[..., ::-1]          # (d/dx, d/dy, d/dz)
+    ln = np.linalg.norm(g, axis=-1, keepdims=True)
+    g = np.where(ln > 1e-12, g / np.maximum(ln, 1e-12), np.array([0.0, 0.0, 1.0], dtype=np.float32))
+    return dens, g.astype(np.float16)
+
+
+def volume(width=1024, height=1024, samples_camera=128, max_depth=32, grid=256, seed=1, with_floor_light=False):
+    vpi = perspective_view_proj_inv((0.3, 0.45, 2.1), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, width / height)
+    d = SceneDesc("volume", width, height, vpi)
+    dens, dirs = volume_grid(grid, seed=seed)
+    flakes = dict(model=SGGX_MICRO_FLAKE, albedo=(0.9, 0.9, 0.9), sggx_stddev=(0.1, 1.0, 1.0))
+    d.volumes.append(((grid, grid, grid), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), dens, dirs, flakes))
+    if with_floor_light:
+        qv, qt = _quad((0.0, -0.6, 0.0), (0.0, 0.0, 1.5), (1.5, 0.0, 0.0))
+        floor = dict(model=GGX, diffuse=(0.6, 0.6, 0.6), specular=(0.0, 0.0, 0.0), specular_power=(1.0, 1.0))
+        d.meshes.append((qv, qt, [(floor, 0, 6)], False))
+        ld = np.array([0.3, 0.9, 0.2]) / np.linalg.norm([0.3, 0.9, 0.2])
+        d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
     d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
                       background=(1.0, 1.0, 1.0), seed=1)
     return d

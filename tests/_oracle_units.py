@@ -30,6 +30,14 @@ def lib():
         lb.orc_projected_triangle_area.restype = F
         lb.orc_projected_triangle_area.argtypes = [PF, PF, PF, PF]
         lb.orc_sggx.argtypes = [PF, PF, PF, PF]
+        lb.orc_make_basis_tangent.argtypes = [PF, PF]
+        lb.orc_cholesky.argtypes = [PF, PF]
+        lb.orc_transpose.argtypes = [PF, PF]
+        lb.orc_mat_mul.argtypes = [PF, PF, PF]
+        lb.orc_transform_covariance.argtypes = [PF, PF, PF]
+        lb.orc_transform_rotation_inverse.argtypes = [PF, PF, PF]
+        lb.orc_bsdf_eval_batch.argtypes = [ctypes.POINTER(_abi.tr_material), PF, PF, PF, PF, ctypes.c_size_t, PF, F,
+                                           PF]
         lb._units_declared = True
     return lb
 

@@ -1,5 +1,6 @@
-"""Writes tests/golden/ref_sampling.npz: input/output vectors of the
-reference's own sampling warps, Fresnel terms and basis construction.
+"""Writes tests/golden/ref_sampling.npz and ref_flake.npz: input/output
+vectors of the reference's own sampling warps, Fresnel terms, basis
+construction, Matrix3 operations and SGGX micro-flake phase sampling.
 
 Run in a checkout that has the reference sources, after `make -C oracle ref`
 has compiled them into oracle/_ref/libtempest_ref.so (see oracle/ref_shim.cc
@@ -104,6 +105,61 @@ def main():
     out.update(basis_normal=nv, basis=basis, reflect_inc=inc, reflect=refl)
     np.savez_compressed(os.path.join(HERE, "ref_sampling.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_sampling.npz"), "with", len(out), "arrays")
+    flake_fixture(lib)
+
+
+def flake_fixture(lib):
+    """tests/golden/ref_flake.npz: Matrix3 makeBasisTangent, transpose,
+    product, transformCovariance, transformRotationInverse and Cholesky, and
+    the SGGX micro-flake phase sample composed from them (ref_shim.cc)."""
+    F = ctypes.c_float
+    P = ctypes.POINTER(F)
+
+    def fp(a):
+        return a.ctypes.data_as(P)
+
+    rng = np.random.Generator(np.random.PCG64(4242))
+    n = 512
+    out = {}
+    tan = rng.normal(size=(n, 3))
+    tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+    tan = tan.astype(np.float32)
+    tan[:6] = [[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1]]
+    bt = np.zeros((n, 9), dtype=np.float32)
+    for i in range(n):
+        lib.ref_make_basis_tangent(fp(tan[i]), fp(bt[i]))
+    out.update(tangent=tan, basis_tangent=bt)
+    a = rng.normal(size=(n, 9)).astype(np.float32)
+    b = rng.normal(size=(n, 9)).astype(np.float32)
+    var = rng.uniform(0.01, 2.0, size=(n, 3)).astype(np.float32)
+    vec = rng.normal(size=(n, 3)).astype(np.float32)
+    prod = np.zeros((n, 9), dtype=np.float32)
+    tr = np.zeros((n, 9), dtype=np.float32)
+    cov = np.zeros((n, 9), dtype=np.float32)
+    rot_inv = np.zeros((n, 3), dtype=np.float32)
+    for i in range(n):
+        lib.ref_mat_mul(fp(a[i]), fp(b[i]), fp(prod[i]))
+        lib.ref_transpose(fp(a[i]), fp(tr[i]))
+        lib.ref_transform_covariance(fp(bt[i]), fp(var[i]), fp(cov[i]))
+        lib.ref_transform_rotation_inverse(fp(a[i]), fp(vec[i]), fp(rot_inv[i]))
+    # symmetric positive definite inputs for the Cholesky factor
+    chol = np.zeros((n, 9), dtype=np.float32)
+    for i in range(n):
+        lib.ref_cholesky(fp(cov[i]), fp(chol[i]))
+    out.update(mat_a=a, mat_b=b, mat_prod=prod, mat_transpose=tr, cov_var=var, cov=cov, rot_vec=vec,
+               rot_inv=rot_inv, cholesky=chol)
+    wo = rng.normal(size=(n, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wo = wo.astype(np.float32)
+    sd = rng.uniform(0.05, 1.0, size=(n, 3)).astype(np.float32)
+    sd[: n // 2] = [0.1, 1.0, 1.0]
+    uv = rng.uniform(0.0, 1.0, size=(n, 2)).astype(np.float32)
+    wi = np.zeros((n, 3), dtype=np.float32)
+    for i in range(n):
+        lib.ref_sggx_flake_sample(fp(wo[i]), fp(sd[i]), fp(bt[i]), F(uv[i, 0]), F(uv[i, 1]), fp(wi[i]))
+    out.update(flake_wo=wo, flake_stddev=sd, flake_frame=bt, flake_uv=uv, flake_wi=wi)
+    np.savez_compressed(os.path.join(HERE, "ref_flake.npz"), **out)
+    print("wrote", os.path.join(HERE, "ref_flake.npz"), "with", len(out), "arrays")
 
 
 if __name__ == "__main__":

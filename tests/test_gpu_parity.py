@@ -85,3 +85,56 @@ def test_draw_once_accumulates_paths():
     acc2, spp2 = g.accumulation()
     assert spp2 == 8
     assert np.all(np.isfinite(acc2))
+
+
+def test_trace_ids_volume_boxes():
+    """Fixed rays against a scene with a grid volume: the volume box is a
+    primitive whose hit is the entry distance with the exit in u."""
+    print("case: trace ids volume", flush=True)
+    desc = workloads.volume(32, 32, samples_camera=1, max_depth=2, grid=16, with_floor_light=True)
+    g, o = _pair(desc)
+    rays = np.concatenate([workloads.primary_rays(desc), workloads.random_rays(desc, 1 << 14)])
+    hb = o.traceRays(rays, brute_force=True)
+    hg = g.traceRays(rays)
+    assert _ids_equal(hb, hg).all()
+    hit = hb["geom"] != 0xFFFFFFFF
+    assert np.array_equal(hb["t"][hit], hg["t"][hit]) and np.array_equal(hb["u"][hit], hg["u"][hit])
+
+
+def test_paths_volume():
+    print("case: paths volume 48x48 spp4 depth12 grid32", flush=True)
+    desc = workloads.volume(48, 48, samples_camera=4, max_depth=12, grid=32)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
+def test_paths_volume_floor_light():
+    """Next-event estimation through the volume: Woodcock shadow rays."""
+    print("case: paths volume + floor + directional light", flush=True)
+    desc = workloads.volume(40, 40, samples_camera=4, max_depth=6, grid=24, with_floor_light=True)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
+def test_timing_mode_renders_the_same_frame():
+    """Launches that carry timing events compute what plain launches do."""
+    print("case: timing mode box", flush=True)
+    desc = workloads.box(32, 32, samples_camera=2, max_depth=3)
+    g, _ = _pair(desc)
+    g.setFrameIndex(0)
+    g.drawOnce()
+    a, _ = g.accumulation()
+    g.repaint()
+    g.setFrameIndex(0)
+    g.setTiming(True)
+    g.drawOnce()
+    st = g.stats()
+    b, _ = g.accumulation()
+    assert np.array_equal(a, b)
+    assert st["trace_ms"] > 0.0 and st["shade_ms"] > 0.0 and st["trace_launches"] == 4

@@ -84,9 +84,14 @@ enum {
 /* Estimator of the path loop.  REFERENCE reproduces the CPU reference's
  * balance-heuristic weighting (ray-tracing.cc:1791-1930, 2182-2332),
  * including its quirks; it is the mode of the per-path parity contract.
- * UNBIASED weights next-event estimation of every area light (mesh lights
- * included) with the balance heuristic and evaluates mesh-light PDFs only
- * along directions that hit the light. */
+ * UNBIASED is multi-sample MIS with the balance heuristic between each
+ * area light (sphere and mesh lights, SamplesLocalAreaLight samples each)
+ * and the BSDF (SamplesGlobalIllumination at the camera vertex, 1 later):
+ * the weight applies to the NEE term and to the emitter hit only, not to
+ * the continuing throughput, the denominator holds only the strategies
+ * that can produce the same path, and a mesh light's density is the true
+ * solid-angle density dist^2 / (cos * area * triangles).  Scenes with
+ * volumes take the REFERENCE estimator only. */
 enum {
     TR_ESTIMATOR_REFERENCE = 0,
     TR_ESTIMATOR_UNBIASED = 1

@@ -404,6 +404,37 @@ void sample_geometry(const orc_scene* s, const tr_hit& h, V3<T> org, V3<T> dir, 
 
 bool is_area_light(uint32_t type) { return type == TR_LIGHT_SPHERE_AREA; }
 
+/* Lights sampled SamplesLocalAreaLight times and balance-weighted in NEE:
+ * sphere lights in the reference estimator (IsAreaLight), sphere and mesh
+ * lights in the unbiased one. */
+bool multi_sample(uint32_t type, bool unbiased)
+{
+    return type == TR_LIGHT_SPHERE_AREA || (unbiased && type == TR_LIGHT_MESH_AREA);
+}
+
+/* Unbiased estimator: solid-angle density dist^2 / (cos * area) / count of
+ * area-uniform sampling of a triangle with n = e0 x e1, zero from behind. */
+template <class T> T mesh_light_pdf_u(V3<T> n, V3<T> dir, T dist, uint32_t tri_count)
+{
+    T c = dot(n, dir);
+    if (!(c < (T)-1e-6f)) return (T)0;
+    return (dist * dist) / ((T)-0.5 * c) / (T)tri_count;
+}
+
+/* ComputeSphereAreaPDF (sampling3.hh:229-262) */
+template <class T> T sphere_light_pdf(V3<T> c, T radius, V3<T> pos, V3<T> dir)
+{
+    V3<T> dc = c - pos;
+    T dist = xsqrt(dot(dc, dc));
+    if (dist <= radius) return (T)1 / ((T)4 * C<T>::pi);
+    V3<T> to_c = dc / dist;
+    T sin_rc = radius / dist;
+    T cos_rc = xsqrt((T)1 - sin_rc * sin_rc);
+    T cos_theta = dot(dir, to_c);
+    if (cos_theta < cos_rc) return (T)0;
+    return (T)1 / ((T)2 * C<T>::pi * ((T)1 - cos_rc));
+}
+
 template <class T> struct LightSample {
     uint32_t geom;
     V3<T> dir;
@@ -445,7 +476,8 @@ void triangle_area_sample(V3<T> pos, V3<T> p0, V3<T> p1, V3<T> p2, T ra, T rb, V
 /* Light sampling (ray-tracing-common.hh:567-647).  pick: the triangle
  * selector word of mesh lights; ra, rb: the two sample coordinates. */
 template <class T>
-LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, uint32_t pick, T ra, T rb)
+LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, uint32_t pick, T ra, T rb,
+                            bool unbiased = false)
 {
     LightSample<T> r;
     const tr_light& l = L.src;
@@ -510,7 +542,8 @@ LightSample<T> sample_light(const orc_scene* s, const Light& L, V3<T> pos, uint3
         V3<T> p0 = vpos<T>(s, g, tri, 0), p1 = vpos<T>(s, g, tri, 1), p2 = vpos<T>(s, g, tri, 2);
         T pdf;
         triangle_area_sample(pos, p0, p1, p2, ra, rb, &r.dir, &r.dist, &pdf);
-        r.pdf = pdf / (T)g.tri_count;
+        r.pdf = unbiased ? mesh_light_pdf_u(cross(p1 - p0, p2 - p0), r.dir, r.dist, g.tri_count)
+                         : pdf / (T)g.tri_count;
         r.radiance = spec3<T>(s->materials[g.material].diffuse);
         r.geom = L.geom;
     }
@@ -522,20 +555,7 @@ template <class T> T light_pdf(const orc_scene* s, const Light& L, V3<T> pos, V3
 {
     const tr_light& l = L.src;
     if (l.type == TR_LIGHT_DIRECTIONAL || l.type == TR_LIGHT_POINT) return (T)1;
-    if (l.type == TR_LIGHT_SPHERE_AREA) {
-        /* ComputeSphereAreaPDF, sampling3.hh:229-262 */
-        V3<T> c = spec3<T>(l.sphere_center);
-        T radius = (T)l.sphere_radius;
-        V3<T> dc = c - pos;
-        T dist = xsqrt(dot(dc, dc));
-        if (dist <= radius) return (T)1 / ((T)4 * C<T>::pi);
-        V3<T> to_c = dc / dist;
-        T sin_rc = radius / dist;
-        T cos_rc = xsqrt((T)1 - sin_rc * sin_rc);
-        T cos_theta = dot(dir, to_c);
-        if (cos_theta < cos_rc) return (T)0;
-        return (T)1 / ((T)2 * C<T>::pi * ((T)1 - cos_rc));
-    }
+    if (l.type == TR_LIGHT_SPHERE_AREA) return sphere_light_pdf(spec3<T>(l.sphere_center), (T)l.sphere_radius, pos, dir);
     /* MeshAreaLightComputePDF, ray-tracing-common.hh:650-687 */
     const Geom& g = s->geoms[L.geom];
     T total = (T)0;
@@ -547,6 +567,19 @@ template <class T> T light_pdf(const orc_scene* s, const Light& L, V3<T> pos, V3
         total += projected_triangle_area(pos, p0, p1, p2);
     }
     return total != (T)0 ? (T)1 / total : (T)0;
+}
+
+/* Unbiased estimator: density with which the light owning an emitter hit
+ * produces the hit point from `pos` along dir at distance t (0 for
+ * emitters that are not lights). */
+template <class T> T hit_light_pdf(const orc_scene* s, const tr_hit& h, V3<T> pos, V3<T> dir)
+{
+    const Geom& g = s->geoms[h.geom];
+    if (g.type == GEOM_SPHERE_LIGHT)
+        return sphere_light_pdf(v3((T)g.sphere[0], (T)g.sphere[1], (T)g.sphere[2]), (T)g.sphere[3], pos, dir);
+    if (g.type != GEOM_MESH) return (T)0;
+    V3<T> p0 = vpos<T>(s, g, h.prim, 0), p1 = vpos<T>(s, g, h.prim, 1), p2 = vpos<T>(s, g, h.prim, 2);
+    return mesh_light_pdf_u(cross(p1 - p0, p2 - p0), dir, (T)h.t, g.tri_count);
 }
 
 /* EvaluateTotalBalanceHeuristic (ray-tracing.cc:1791-1806) */
@@ -584,14 +617,15 @@ V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const
     V3<T> radiance = v3((T)0, (T)0, (T)0);
     for (uint32_t li = 0; li < s->lights.size(); ++li) {
         const Light& L = s->lights[li];
-        bool area = is_area_light(L.src.type);
+        bool unbiased = s->estimator == TR_ESTIMATOR_UNBIASED;
+        bool area = multi_sample(L.src.type, unbiased);
         uint32_t samples = area ? max_light_samples : 1;
         V3<T> color = v3((T)0, (T)0, (T)0);
         Sample<T> lsd = sd;
         for (uint32_t k = 0; k < samples; ++k) {
             uint32_t xi[4];
             rng.draw(vertex, STREAM_LIGHT, (li << 16) | k, xi);
-            LightSample<T> smp = sample_light(s, L, pos, xi[0], (T)orc_u01(xi[1]), (T)orc_u01(xi[2]));
+            LightSample<T> smp = sample_light(s, L, pos, xi[0], (T)orc_u01(xi[1]), (T)orc_u01(xi[2]), unbiased);
             if (smp.pdf == (T)0) continue;
             lsd.wi = smp.dir;
             V3<T> cur = throughput;
@@ -619,8 +653,15 @@ V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const
             }
             if (reached) {
                 if (apply_balance && area) {
-                    T total = total_balance(s, (T)max_light_samples, (T)max_surface_samples, lsd, pos);
-                    T balance = (light_pdf(s, L, pos, lsd.wi) * (T)samples) / total;
+                    T balance;
+                    if (unbiased) {
+                        /* balance heuristic of this light against the BSDF */
+                        T a = (T)samples * smp.pdf;
+                        balance = a / (a + (T)max_surface_samples * bsdf_pdf(lsd));
+                    } else {
+                        T total = total_balance(s, (T)max_light_samples, (T)max_surface_samples, lsd, pos);
+                        balance = (light_pdf(s, L, pos, lsd.wi) * (T)samples) / total;
+                    }
                     cur *= balance;
                 }
                 color += (smp.radiance * cur * bsdf_eval(lsd) / smp.pdf) * (T)tf;
@@ -736,6 +777,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
     }
     V3<T> local = evaluate_lights(s, rng, 0, sd, s->n_light, s->n_gi, pos, start_T, s->max_depth != 0 && !volume);
     V3<T> global = v3((T)0, (T)0, (T)0);
+    const bool unbiased = s->estimator == TR_ESTIMATOR_UNBIASED;
     uint32_t camera_hash = hash;
     uint32_t combined = 0;
     uint32_t rays_sq = (uint32_t)sqrtf((float)s->n_gi);
@@ -767,8 +809,12 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             if (cur.pdf == (T)0) break;
             T balance = (T)1;
             if (!cur_volume) {
-                T total = total_balance(s, (T)ls, (T)ss, cur, cpos);
-                balance = total > (T)0 ? ((T)ss * cur.pdf) / total : (T)0;
+                if (unbiased) {
+                    balance = (T)ss * cur.pdf;   /* the BSDF side of the emitter-hit weight */
+                } else {
+                    T total = total_balance(s, (T)ls, (T)ss, cur, cpos);
+                    balance = total > (T)0 ? ((T)ss * cur.pdf) / total : (T)0;
+                }
                 ct *= bsdf_eval(cur) / cur.pdf;
                 ORC_TRACE("   balance=%g T=(%g %g %g) f=(%g %g %g)\n", (double)balance, (double)ct.x, (double)ct.y,
                           (double)ct.z, (double)bsdf_eval(cur).x, (double)bsdf_eval(cur).y, (double)bsdf_eval(cur).z);
@@ -783,20 +829,27 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             }
             hash_hit(&hash, hh);
             ORC_TRACE("   hit geom=%u prim=%u t=%g\n", hh.geom, hh.prim, (double)hh.t);
+            V3<T> prev_pos = cpos;
             cur_volume = surface_or_volume(s, hh, &cpos, &cur);
             if (cur_volume) {
                 ct *= spec3<T>(cur.mat->albedo);
                 if (below_threshold(ct)) break;
             }
             if (is_emissive(cur.mat->model)) {
-                gsplit += ct * balance * spec3<T>(cur.mat->diffuse);
+                T w = balance;
+                if (unbiased) {
+                    T lp = hit_light_pdf(s, hh, prev_pos, cur.wi);
+                    T ls_prev = d == 0 ? (T)s->n_light : (T)1;
+                    w = balance / (balance + ls_prev * lp);
+                }
+                gsplit += ct * w * spec3<T>(cur.mat->diffuse);
                 break;
             }
             cur.wo = -cur.wi;
             ss = 1;
             ls = 1;
             gsplit += evaluate_lights(s, srng, d + 1, cur, ls, ss, cpos, ct, d + 1 != s->max_depth);
-            ct *= balance / (T)s->rr;
+            ct *= (unbiased ? (T)1 : balance) / (T)s->rr;
             st.x = st.y = (T)0;
             st.tx = st.ty = (T)1;
             if (s->rr < 1.0f) {

@@ -14,6 +14,7 @@
 #include "../../include/tempest_rt_debug.h"
 #include "tr_bvh8.h"
 #include "tr_kernels.h"
+#include "tr_shading.h"
 #include "tr_traverse.h"
 #include "tr_volume.h"
 
@@ -253,7 +254,8 @@ tr_status sync_lights(tr_scene* s)
 uint32_t shadow_entries(const tr_scene* s)
 {
     uint32_t n = 0;
-    for (const LightRec& L : s->lights) n += L.area ? std::max(1u, s->n_light) : 1u;
+    for (const LightRec& L : s->lights)
+        n += multi_sample_light(L, s->estimator == TR_ESTIMATOR_UNBIASED) ? std::max(1u, s->n_light) : 1u;
     return n;
 }
 
@@ -321,6 +323,7 @@ FrameParams frame_params(const tr_scene* s)
     p.key0 = (uint32_t)s->seed;
     p.key1 = (uint32_t)(s->seed >> 32);
     p.frame = s->frame_index;
+    p.estimator = s->estimator;
     memcpy(p.vpi, s->vpi, sizeof(p.vpi));
     return p;
 }
@@ -337,6 +340,9 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
     if (s->bvh_stats.depth + 2 > (uint32_t)(kStackLds + kStackSpill))
         return fail(TR_ERROR_UNSUPPORTED, "BVH deeper than the traversal stack");
     if (s->n_light > 65535) return fail(TR_ERROR_UNSUPPORTED, "SamplesLocalAreaLight above 65535");
+    if (!s->vols.empty() && s->estimator == TR_ESTIMATOR_UNBIASED)
+        return fail(TR_ERROR_UNSUPPORTED, "the unbiased estimator does not take volumes: the reference's micro-flake "
+                                          "sampler has no closed-form density");
     if (!s->vols.empty() && (s->lights.size() > 256 || s->n_light > 256))
         return fail(TR_ERROR_UNSUPPORTED, "scenes with volumes take at most 256 lights and 256 samples per area light");
     tr_status st = sync_lights(s);
@@ -573,7 +579,8 @@ tr_status tr_scene_set_samples_local_area_light(tr_scene* s, uint32_t v)
 tr_status tr_scene_set_estimator(tr_scene* s, uint32_t mode)
 {
     if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
-    if (mode != TR_ESTIMATOR_REFERENCE) return fail(TR_ERROR_UNSUPPORTED, "only the reference estimator is implemented");
+    if (mode != TR_ESTIMATOR_REFERENCE && mode != TR_ESTIMATOR_UNBIASED)
+        return fail(TR_ERROR_INVALID_ARG, "unknown estimator");
     s->estimator = mode;
     return TR_OK;
 }

@@ -30,6 +30,7 @@ struct FrameParams {
     float background[3];
     uint32_t key0, key1;
     uint32_t frame;
+    uint32_t estimator;  /* TR_ESTIMATOR_REFERENCE or TR_ESTIMATOR_UNBIASED */
     float vpi[16];
 };
 

@@ -234,6 +234,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
         Rng rng;
         rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
+        const bool unbiased = p.estimator == TR_ESTIMATOR_UNBIASED;
         float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = pool.hit[slot], t4 = pool.thr[slot];
         V3 org = ld4(o4), dir = ld4(d4);
         V3 T = ld4(t4);
@@ -281,8 +282,20 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 /* absorbed: no contribution from this vertex on */
             } else if (is_emissive(sd.mat->model)) {
                 V3 le = ld3(sd.mat->diffuse);
-                if (vertex == 0) { local = mulv(T, le); write_local = true; }
-                else global = add(global, mulv(muls(T, bal), le));
+                if (vertex == 0) {
+                    local = mulv(T, le);
+                    write_local = true;
+                } else {
+                    float w = bal;
+                    if (unbiased) {
+                        /* balance heuristic of the BSDF strategy against the
+                         * light that owns the hit point; bal holds ss * pdf */
+                        float lp = hit_light_pdf(sc, geom, prim, org, dir, h4.x);
+                        float ls_prev = vertex == 1 ? (float)p.n_light : 1.0f;
+                        w = bal / (bal + ls_prev * lp);
+                    }
+                    global = add(global, mulv(muls(T, w), le));
+                }
             } else {
                 sd.wo = neg(dir);
                 uint32_t ls = vertex == 0 ? p.n_light : 1u;
@@ -293,17 +306,24 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                     uint32_t ne = 0;
                     for (uint32_t li = 0; li < sc.n_lights; ++li) {
                         const LightRec& L = sc.lights[li];
-                        uint32_t samples = L.area ? ls : 1u;
+                        bool multi = multi_sample_light(L, unbiased);
+                        uint32_t samples = multi ? ls : 1u;
                         for (uint32_t k = 0; k < samples; ++k) {
                             U4 xr = rng.draw(vertex, STREAM_LIGHT, (li << 16) | k);
-                            LightSample smp = sample_light(sc, L, pos, xr.a, xr.b, xr.c);
+                            LightSample smp = sample_light(sc, L, pos, xr.a, xr.b, xr.c, unbiased);
                             if (smp.pdf == 0.0f) continue;
                             Sample lsd = sd;
                             lsd.wi = smp.dir;
                             V3 curT = T;
-                            if (apply_balance && L.area) {
-                                float total = total_balance(sc, (float)ls, (float)ss, lsd, pos);
-                                float balance = (light_pdf(sc, L, pos, lsd.wi) * (float)samples) / total;
+                            if (apply_balance && multi) {
+                                float balance;
+                                if (unbiased) {
+                                    float a = (float)samples * smp.pdf;
+                                    balance = a / (a + (float)ss * bsdf_pdf(lsd));
+                                } else {
+                                    float total = total_balance(sc, (float)ls, (float)ss, lsd, pos);
+                                    balance = (light_pdf(sc, L, pos, lsd.wi) * (float)samples) / total;
+                                }
                                 curT = muls(curT, balance);
                             }
                             V3 contrib = divs(mulv(mulv(smp.radiance, curT), bsdf_eval(lsd)), smp.pdf);
@@ -320,7 +340,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 }
                 bool alive = true;
                 if (vertex > 0) {
-                    T = muls(T, bal / p.rr);
+                    T = muls(T, (unbiased ? 1.0f : bal) / p.rr);
                     if (p.rr < 1.0f) {
                         U4 xq = rng.draw(vertex, STREAM_RR, 0u);
                         if (u01(xq.a) > p.rr) alive = false;
@@ -343,8 +363,12 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                          * (phase / pdf = 1, ray-tracing.cc:2182-2201) */
                         float nb = 1.0f;
                         if (!volume) {
-                            float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
-                            nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
+                            if (unbiased) {
+                                nb = (float)ss * sd.pdf;
+                            } else {
+                                float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
+                                nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
+                            }
                             T = mulv(T, divs(bsdf_eval(sd), sd.pdf));
                         }
                         pool.ray_o[slot] = st4(pos, 0.0f);
@@ -397,7 +421,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
             V3 radiance = mk(0.0f, 0.0f, 0.0f);
             uint32_t e = 0;
             for (uint32_t li = 0; li < sc.n_lights; ++li) {
-                uint32_t samples = sc.lights[li].area ? ls : 1u;
+                uint32_t samples = multi_sample_light(sc.lights[li], p.estimator == TR_ESTIMATOR_UNBIASED) ? ls : 1u;
                 V3 color = mk(0.0f, 0.0f, 0.0f);
                 while (e < ne) {
                     size_t idx = (size_t)e * pool.capacity + slot;

@@ -465,10 +465,46 @@ TR_HD V3 light_vertex(const SceneView& sc, const GeomRec& g, uint32_t tri, int k
     return ld3(reinterpret_cast<const float*>(sc.vpool + g.vbase + (size_t)idx * g.stride));
 }
 
+/* Unbiased estimator: solid-angle density of area-uniform sampling of a
+ * mesh-light triangle with n = e0 x e1 (twice its area), seen along dir at
+ * distance dist: dist^2 / (cos * area) / triangles; 0 from behind, by the
+ * reference's facing test (sampling3.hh:220-226). */
+TR_HD float mesh_light_pdf_u(V3 n, V3 dir, float dist, uint32_t tri_count)
+{
+    float c = dot(n, dir);
+    if (!(c < -1e-6f)) return 0.0f;
+    return (dist * dist) / (-0.5f * c) / (float)tri_count;
+}
+
+/* ComputeSphereAreaPDF (sampling3.hh:229-262) */
+TR_HD float sphere_light_pdf(V3 center, float radius, V3 pos, V3 dir)
+{
+    V3 dc = sub(center, pos);
+    float dist = sqrtf(dot(dc, dc));
+    if (dist <= radius) return 1.0f / (4.0f * kPi);
+    V3 to_c = divs(dc, dist);
+    float sin_rc = radius / dist;
+    float cos_rc = sqrtf(1.0f - sin_rc * sin_rc);
+    float cos_theta = dot(dir, to_c);
+    if (cos_theta < cos_rc) return 0.0f;
+    return 1.0f / (2.0f * kPi * (1.0f - cos_rc));
+}
+
+/* Lights sampled SamplesLocalAreaLight times: IsAreaLight (sphere lights,
+ * ray-tracing-common.hh:546-551) in the reference estimator; sphere and
+ * mesh lights in the unbiased one. */
+TR_HD bool multi_sample_light(const LightRec& L, bool unbiased)
+{
+    return L.area || (unbiased && L.src.type == TR_LIGHT_MESH_AREA);
+}
+
 /* SampleLightSourceFunctionLookup (ray-tracing-common.hh:567-647) with
  * SampleSphereArea / UniformSampleSphereArea / UniformSampleTriangleArea
- * (sampling3.hh:142-227).  x0: triangle pick, x1/x2: the two coordinates. */
-TR_HD LightSample sample_light(const SceneView& sc, const LightRec& L, V3 pos, uint32_t x0, uint32_t x1, uint32_t x2)
+ * (sampling3.hh:142-227).  x0: triangle pick, x1/x2: the two coordinates.
+ * The unbiased estimator reports the mesh-light triangle's true
+ * solid-angle density. */
+TR_HD LightSample sample_light(const SceneView& sc, const LightRec& L, V3 pos, uint32_t x0, uint32_t x1, uint32_t x2,
+                               bool unbiased = false)
 {
     LightSample r;
     const tr_light& l = L.src;
@@ -535,8 +571,12 @@ TR_HD LightSample sample_light(const SceneView& sc, const LightRec& L, V3 pos, u
         float len = length(du);
         r.dir = divs(du, len);
         r.dist = len;
-        float pdf = dot(cross(e0, e1), r.dir) < -1e-6f ? 1.0f / projected_triangle_area(pos, p0, p1, p2) : 0.0f;
-        r.pdf = pdf / (float)g.tri_count;
+        if (unbiased) {
+            r.pdf = mesh_light_pdf_u(cross(e0, e1), r.dir, r.dist, g.tri_count);
+        } else {
+            float pdf = dot(cross(e0, e1), r.dir) < -1e-6f ? 1.0f / projected_triangle_area(pos, p0, p1, p2) : 0.0f;
+            r.pdf = pdf / (float)g.tri_count;
+        }
         r.radiance = ld3(sc.mats[g.material].diffuse);
         r.geom = L.geom;
     }
@@ -548,17 +588,7 @@ TR_HD float light_pdf(const SceneView& sc, const LightRec& L, V3 pos, V3 dir)
 {
     const tr_light& l = L.src;
     if (l.type == TR_LIGHT_DIRECTIONAL || l.type == TR_LIGHT_POINT) return 1.0f;
-    if (l.type == TR_LIGHT_SPHERE_AREA) {
-        V3 dc = sub(ld3(l.sphere_center), pos);
-        float dist = sqrtf(dot(dc, dc));
-        if (dist <= l.sphere_radius) return 1.0f / (4.0f * kPi);
-        V3 to_c = divs(dc, dist);
-        float sin_rc = l.sphere_radius / dist;
-        float cos_rc = sqrtf(1.0f - sin_rc * sin_rc);
-        float cos_theta = dot(dir, to_c);
-        if (cos_theta < cos_rc) return 0.0f;
-        return 1.0f / (2.0f * kPi * (1.0f - cos_rc));
-    }
+    if (l.type == TR_LIGHT_SPHERE_AREA) return sphere_light_pdf(ld3(l.sphere_center), l.sphere_radius, pos, dir);
     const GeomRec& g = sc.geoms[L.geom];
     float total = 0.0f;
     for (uint32_t tri = 0; tri < g.tri_count; ++tri) {
@@ -568,6 +598,18 @@ TR_HD float light_pdf(const SceneView& sc, const LightRec& L, V3 pos, V3 dir)
         total = total + projected_triangle_area(pos, p0, p1, p2);
     }
     return total != 0.0f ? 1.0f / total : 0.0f;
+}
+
+/* Unbiased estimator: density with which the light of an emitter hit
+ * would have produced the hit point from `pos` (ray origin) along dir at
+ * distance t; 0 for emitters that are not lights. */
+TR_HD float hit_light_pdf(const SceneView& sc, uint32_t geom, uint32_t prim, V3 pos, V3 dir, float t)
+{
+    const GeomRec& g = sc.geoms[geom];
+    if (g.type == GEOM_SPHERE_LIGHT) return sphere_light_pdf(mk(g.sphere[0], g.sphere[1], g.sphere[2]), g.sphere[3], pos, dir);
+    if (g.type != GEOM_MESH) return 0.0f;
+    V3 p0 = light_vertex(sc, g, prim, 0), p1 = light_vertex(sc, g, prim, 1), p2 = light_vertex(sc, g, prim, 2);
+    return mesh_light_pdf_u(cross(sub(p1, p0), sub(p2, p0)), dir, t, g.tri_count);
 }
 
 /* EvaluateTotalBalanceHeuristic (ray-tracing.cc:1791-1806) */

@@ -84,7 +84,7 @@ class SceneDesc:
         self.sphere_lights = []
         self.volumes = []      # (dims, min, max, density f16 [nz, ny, nx], direction f16 [nz, ny, nx, 3], material)
         self.settings = dict(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=1.0,
-                             background=(0.0, 0.0, 0.0), seed=1)
+                             background=(0.0, 0.0, 0.0), seed=1, estimator=0)
 
     @property
     def pixels(self):
@@ -110,6 +110,7 @@ class SceneDesc:
         target.setRussianRoulette(st["rr"])
         target.setBackgroundSpectrum(st["background"])
         target.setSeed(st["seed"])
+        target.setEstimator(st.get("estimator", 0))
         from .scene import Submesh
         for verts, tris, parts, two_sided in self.meshes:
             stride = verts.shape[1] * 4

@@ -138,3 +138,29 @@ def test_timing_mode_renders_the_same_frame():
     b, _ = g.accumulation()
     assert np.array_equal(a, b)
     assert st["trace_ms"] > 0.0 and st["shade_ms"] > 0.0 and st["trace_launches"] == 4
+
+
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_paths_box_multi_sample(estimator):
+    """4 GI splits and 2 samples per area light, both estimators: the
+    unbiased one MIS-weights the mesh light's NEE and the emitter hits."""
+    print("case: paths box multi-sample estimator", estimator, flush=True)
+    desc = workloads.box(40, 40, samples_camera=2, max_depth=4)
+    desc.settings.update(estimator=estimator, samples_gi=4, samples_light=2)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 2)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
+def test_unbiased_rejects_volumes():
+    desc = workloads.volume(16, 16, samples_camera=1, max_depth=2, grid=8)
+    desc.settings.update(estimator=1)
+    g = RayTracerScene(desc.width, desc.height, desc.vpi)
+    desc.populate(g, gpu_material)
+    g.commitScene()
+    from tempestrenderer_amd import _abi
+    with pytest.raises(_abi.TempestError) as e:
+        g.drawOnce()
+    assert e.value.status == _abi.TR_ERROR_UNSUPPORTED

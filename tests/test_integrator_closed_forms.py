@@ -70,3 +70,60 @@ def test_background_and_emitter_at_camera_vertex():
     assert miss.sum() > 0 and emit.sum() > 0
     assert (miss | emit).all()
     assert abs(emit.mean() - np.mean(hits["geom"] == 0)) < 0.05
+
+
+def _rect_form_factor(x, y):
+    """Point-to-parallel-rectangle form factor, the normal through one
+    corner of an x by y rectangle (in units of the distance)."""
+    a = x / math.sqrt(1 + x * x) * math.atan(y / math.sqrt(1 + x * x))
+    b = y / math.sqrt(1 + y * y) * math.atan(x / math.sqrt(1 + y * y))
+    return (a + b) / (2 * math.pi)
+
+
+def _direct_desc(estimator, light, rho=0.5, spp=40000):
+    h = 0.5
+    vpi = workloads.perspective_view_proj_inv((0.0, h / 2, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, -1.0), 0.2, 1.0)
+    d = workloads.SceneDesc("direct", 1, 1, vpi)
+    qv, qt = workloads._quad((0.0, 0.0, 0.0), (0.0, 0.0, 50.0), (50.0, 0.0, 0.0))  # normal +y
+    plane = dict(model=workloads.GGX, diffuse=(rho, rho, rho), specular=(0, 0, 0))
+    parts = [(plane, 0, 6)]
+    verts, tris = [qv], [qt]
+    if light == "quad":
+        lv, lt = workloads._quad((0.0, h, 0.0), (0.5, 0.0, 0.0), (0.0, 0.0, 0.5))   # 1 x 1, faces -y
+        parts.append((dict(model=workloads.EMISSIVE, diffuse=(3.0, 3.0, 3.0)), 6, 6))
+        verts.append(lv)
+        tris.append(lt + 4)
+    else:
+        d.sphere_lights.append(dict(kind=workloads.LIGHT_SPHERE_AREA, radiance=(3.0, 3.0, 3.0), center=(0.0, 2.0, 0.0),
+                                    radius=0.5))
+    d.meshes.append((np.concatenate(verts), np.concatenate(tris), parts, False))
+    d.settings.update(samples_camera=spp, samples_gi=4, samples_light=2, max_depth=1, background=(0.0, 0.0, 0.0),
+                      estimator=estimator)
+    return d
+
+
+def _mean_and_error(o):
+    rgb, _ = o.renderPaths(np.array([0], dtype=np.uint32), threads=8)
+    est = rgb[:, 0].astype(np.float64)
+    return est.mean(), est.std() / math.sqrt(len(est))
+
+
+def test_unbiased_mesh_light_direct_illumination():
+    """Unbiased estimator, depth 1 (NEE and BSDF sampling both active, MIS
+    with 2 light samples and 4 BSDF samples): a Lambertian point under a
+    1 x 1 emitting quad at height 0.5 reflects rho * L * 4 F(1, 1).  The
+    reference estimator, whose mesh-light NEE is not MIS-weighted while its
+    BSDF hits are, over-counts the same light."""
+    expect = 0.5 * 3.0 * 4.0 * _rect_form_factor(1.0, 1.0)
+    m, e = _mean_and_error(_oracle(_direct_desc(1, "quad")))
+    assert abs(m - expect) < 4.0 * e + 1e-4 * expect, (m, expect, e)
+    m_ref, e_ref = _mean_and_error(_oracle(_direct_desc(0, "quad")))
+    assert m_ref - expect > 6.0 * e_ref, (m_ref, expect, e_ref)
+
+
+def test_unbiased_sphere_light_direct_illumination():
+    """Sphere light of radius 0.5 at height 2 above the point, MIS active:
+    rho * L * (R / h)^2."""
+    expect = 0.5 * 3.0 * (0.5 / 2.0) ** 2
+    m, e = _mean_and_error(_oracle(_direct_desc(1, "sphere")))
+    assert abs(m - expect) < 4.0 * e + 1e-4 * expect, (m, expect, e)

@@ -327,6 +327,35 @@ tr_status tr_scene_set_counting(tr_scene* scene, int32_t enabled);
  * filled only in this mode. */
 tr_status tr_scene_set_timing(tr_scene* scene, int32_t enabled);
 
+/* ---- radiance-cache network (new; no reference counterpart) ----
+ * The MLP that caches outgoing radiance for the path tracer's cache mode:
+ * 10 inputs in [0, 1] (position in the scene box, (w_o + 1) / 2,
+ * (n + 1) / 2, roughness), 12-band sin/cos encoding (240 features),
+ * 4 hidden ReLU layers of 64, 3 linear outputs; 28,099 FP32 parameters in
+ * the order W0 (64 x 240, row major), b0, W1, b1, W2, b2, W3, b3, W4
+ * (3 x 64), b4.  Products run in BF16 on the matrix cores with FP32
+ * accumulation; training minimises the relative L2 loss
+ * mean_samples sum_rgb (y - t)^2 / (y^2 + 0.01) (y^2 held constant) with
+ * Adam (lr 1e-2, betas 0.9 / 0.99, eps 1e-8) on FP32 master weights and a
+ * loss scale of 128.  The host-pointer entry points below copy their
+ * arrays (inputs n x 10, outputs and targets n x 3, valid n weights 0/1,
+ * row major) and are synchronous. */
+typedef struct tr_mlp tr_mlp;
+enum { TR_MLP_INPUTS = 10, TR_MLP_OUTPUTS = 3, TR_MLP_PARAMS = 28099 };
+
+tr_status tr_mlp_create(uint64_t seed, tr_mlp** out_mlp);
+void tr_mlp_destroy(tr_mlp* mlp);
+tr_status tr_mlp_get_params(tr_mlp* mlp, float* params, size_t count);
+/* Replaces the parameters and resets the Adam state. */
+tr_status tr_mlp_set_params(tr_mlp* mlp, const float* params, size_t count);
+tr_status tr_mlp_infer(tr_mlp* mlp, size_t n, const float* inputs, float* outputs);
+/* Gradient of the loss at the current parameters (no update) and the loss. */
+tr_status tr_mlp_gradient(tr_mlp* mlp, size_t n, const float* inputs, const float* targets,
+                          const float* valid, float* grad, float* loss);
+/* One Adam step on the n samples; returns the loss before the step. */
+tr_status tr_mlp_train_step(tr_mlp* mlp, size_t n, const float* inputs, const float* targets,
+                            const float* valid, float* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,135 @@
+"""FP64 CPU restatement of the radiance-cache MLP (test infrastructure).
+
+ORACLE: only tests/ may import this module, as the checker of the HIP
+kernels in tempestrenderer_amd/csrc/tr_mlp.hip.  The network has no
+counterpart in the reference; it is specified by the build:
+
+  input   10 scalars in [0, 1]: position (3, normalised to the scene box),
+          outgoing direction (3, (w + 1) / 2), shading normal (3, (n + 1) / 2),
+          roughness (1)
+  encode  per scalar x and band k = 0..11: sin(2^k pi x), cos(2^k pi x);
+          feature index (i * 12 + k) * 2 + {0: sin, 1: cos}  -> 240 features
+  layers  240 -> 64 -> 64 -> 64 -> 64 -> 3, ReLU after the four hidden
+          layers, linear output; parameters in the order W0 (64 x 240, row
+          major), b0, W1, b1, W2, b2, W3, b3, W4 (3 x 64), b4: 28,099
+  loss    relative L2: mean over samples of sum over RGB of
+          (y - t)^2 / (stop_grad(y)^2 + 0.01), samples of weight 0 ignored
+          (the mean divides by the number of samples of weight 1)
+  optim   Adam, lr 1e-2, beta1 0.9, beta2 0.99, eps 1e-8, bias-corrected
+
+`bf16=True` rounds what the GPU rounds to bfloat16 (round to nearest even):
+the weights, the encoded features, every hidden activation and the
+gradients that enter a matrix product; sums stay in FP64 (the GPU sums in
+FP32 inside the matrix cores).
+"""
+import numpy as np
+
+N_IN, BANDS = 10, 12
+FEAT = N_IN * BANDS * 2
+HIDDEN = 64
+N_OUT = 3
+LAYERS = [(FEAT, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, N_OUT)]
+N_PARAMS = sum(o * i + o for i, o in LAYERS)
+LR, BETA1, BETA2, EPS = 1e-2, 0.9, 0.99, 1e-8
+REL_EPS = 0.01
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    x = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    u = x.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    out = r.view(np.float32).astype(np.float64)
+    return np.where(np.isnan(x), np.nan, out)
+
+
+def unpack(theta):
+    """[(W, b)] per layer, views into theta."""
+    out, o = [], 0
+    for fan_in, fan_out in LAYERS:
+        W = theta[o:o + fan_out * fan_in].reshape(fan_out, fan_in)
+        o += fan_out * fan_in
+        b = theta[o:o + fan_out]
+        o += fan_out
+        out.append((W, b))
+    return out
+
+
+def init(seed=1):
+    """Uniform(-sqrt(6/fan_in), +) weights (He-uniform), zero biases."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    theta = np.zeros(N_PARAMS)
+    for W, b in unpack(theta):
+        lim = np.sqrt(6.0 / W.shape[1])
+        W[...] = rng.uniform(-lim, lim, size=W.shape)
+    return theta
+
+
+def encode(x):
+    """[N, 10] inputs -> [N, 240] features, in FP64."""
+    x = np.asarray(x, dtype=np.float64)
+    k = 2.0 ** np.arange(BANDS)
+    ang = np.pi * x[:, :, None] * k[None, None, :]            # [N, 10, 12]
+    f = np.stack([np.sin(ang), np.cos(ang)], axis=-1)          # [N, 10, 12, 2]
+    return f.reshape(x.shape[0], FEAT)
+
+
+def forward(theta, x, bf16_mode=False, features=None):
+    """Returns (y [N, 3], activations [X, H1..H4])."""
+    rnd = bf16 if bf16_mode else (lambda a: a)
+    h = rnd(encode(x) if features is None else features)
+    acts = [h]
+    layers = unpack(theta)
+    for li, (W, b) in enumerate(layers):
+        z = h @ rnd(W).T + b
+        if li < len(layers) - 1:
+            h = rnd(np.maximum(z, 0.0))
+            acts.append(h)
+        else:
+            h = z
+    return h, acts
+
+
+def loss_weights(y, valid=None):
+    """Per-element weights of the relative L2 loss at the current prediction
+    (the stop-gradient denominator), divided by the number of samples."""
+    w = 1.0 / (y * y + REL_EPS)
+    v = np.ones(len(y)) if valid is None else np.asarray(valid, dtype=np.float64)
+    n = max(1.0, float(v.sum()))
+    return w * (v / n)[:, None]
+
+
+def loss(theta, x, t, valid=None, bf16_mode=False):
+    y, _ = forward(theta, x, bf16_mode)
+    w = loss_weights(y, valid)
+    return float(np.sum(w * (y - t) ** 2))
+
+
+def gradient(theta, x, t, valid=None, bf16_mode=False, weights=None, loss_scale=1.0):
+    """d loss / d theta with the stop-gradient weights (or the given fixed
+    weights).  In bf16 mode the output gradient is scaled by loss_scale,
+    rounded to bfloat16 where the GPU rounds it, and unscaled at the end."""
+    rnd = bf16 if bf16_mode else (lambda a: a)
+    y, acts = forward(theta, x, bf16_mode)
+    w = loss_weights(y, valid) if weights is None else weights
+    dz = 2.0 * w * (y - t) * loss_scale
+    layers = unpack(theta)
+    grads = []
+    for li in range(len(layers) - 1, -1, -1):
+        W, b = layers[li]
+        dz_r = rnd(dz)
+        grads.append((dz_r.T @ acts[li], dz_r.sum(axis=0)))
+        if li > 0:
+            dh = dz_r @ rnd(W)
+            dz = dh * (acts[li] > 0.0)
+    g = np.concatenate([np.concatenate([gw.reshape(-1), gb]) for gw, gb in reversed(grads)])
+    return g / loss_scale
+
+
+def adam(theta, m, v, g, step, lr=LR):
+    """One Adam step in place; `step` counts from 1."""
+    m[...] = BETA1 * m + (1.0 - BETA1) * g
+    v[...] = BETA2 * v + (1.0 - BETA2) * g * g
+    mh = m / (1.0 - BETA1 ** step)
+    vh = v / (1.0 - BETA2 ** step)
+    theta -= lr * mh / (np.sqrt(vh) + EPS)

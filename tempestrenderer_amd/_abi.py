@@ -156,6 +156,15 @@ SIGNATURES = {
     "tr_scene_get_stats": (_st, [_vp, P(tr_stats)]),
     "tr_scene_set_counting": (_st, [_vp, ctypes.c_int32]),
     "tr_scene_set_timing": (_st, [_vp, ctypes.c_int32]),
+    "tr_mlp_create": (_st, [_u64, P(_vp)]),
+    "tr_mlp_destroy": (None, [_vp]),
+    "tr_mlp_get_params": (_st, [_vp, P(ctypes.c_float), _sz]),
+    "tr_mlp_set_params": (_st, [_vp, P(ctypes.c_float), _sz]),
+    "tr_mlp_infer": (_st, [_vp, _sz, P(ctypes.c_float), P(ctypes.c_float)]),
+    "tr_mlp_gradient": (_st, [_vp, _sz, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                              P(ctypes.c_float)]),
+    "tr_mlp_train_step": (_st, [_vp, _sz, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                P(ctypes.c_float)]),
     "tr_debug_build_scene_host": (_st, [_vp]),
     "tr_debug_trace_rays_host": (_st, [_vp, _sz, P(tr_ray), P(tr_hit), P(_u64), P(_u64)]),
     "tr_debug_bvh_check": (_st, [_vp, P(_u64)]),

@@ -13,6 +13,7 @@
 #include "../../include/tempest_rt.h"
 #include "../../include/tempest_rt_debug.h"
 #include "tr_bvh8.h"
+#include "tr_host.h"
 #include "tr_kernels.h"
 #include "tr_shading.h"
 #include "tr_traverse.h"
@@ -29,41 +30,11 @@
 
 using namespace trk;
 
-namespace {
-
+namespace trk {
 thread_local std::string g_error;
-
-tr_status fail(tr_status st, const std::string& msg)
-{
-    g_error = msg;
-    return st;
 }
 
-#define TR_HIP(call)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess) return fail(TR_ERROR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-template <class T> struct DevBuf {
-    T* ptr = nullptr;
-    size_t count = 0;
-    void release()
-    {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        count = 0;
-    }
-    hipError_t reserve(size_t n)
-    {
-        if (n <= count && ptr) return hipSuccess;
-        release();
-        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-        hipError_t e = hipMalloc((void**)&ptr, bytes);
-        if (e == hipSuccess) count = n;
-        return e;
-    }
-};
+namespace {
 
 enum { EV_TRACE = 0, EV_SHADE = 1, EV_SHADOW = 2, EV_OTHER = 3 };
 

@@ -1,0 +1,107 @@
+/*
+ * tr_mlp.h - the radiance-cache network: layout of its parameters, the
+ * frequency encoding, and the launchers of its HIP kernels (tr_mlp.hip).
+ * The network has no reference counterpart.
+ *
+ *   input    10 scalars in [0, 1]: position (scene box), outgoing direction
+ *            ((w + 1) / 2), shading normal ((n + 1) / 2), roughness
+ *   encoding per scalar x, band k = 0..11: sin(2^k pi x), cos(2^k pi x),
+ *            feature (i * 12 + k) * 2 + {sin, cos}: 240 features (padded
+ *            to 256 with zeros for the matrix cores)
+ *   network  240 -> 64 -> 64 -> 64 -> 64 -> 3, ReLU on hidden layers
+ *   params   FP32 master vector W0 (64 x 240 row major), b0, W1, b1, W2,
+ *            b2, W3, b3, W4 (3 x 64), b4: 28,099
+ *
+ * The matrix products run on v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+ * accumulation.  Weights live as BF16 copies packed per layer with padded
+ * shapes (forward: [out_pad][in_pad]; backward: the transposes of layers
+ * 1..4), biases as FP32.
+ */
+#ifndef TR_MLP_H
+#define TR_MLP_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tr_fmath.h"
+
+namespace trk {
+
+constexpr int kMlpIn = 10;
+constexpr int kMlpBands = 12;
+constexpr int kMlpFeat = kMlpIn * kMlpBands * 2;  /* 240 */
+constexpr int kMlpFeatPad = 256;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpOut = 3;
+constexpr int kMlpParams = 28099;
+
+/* offsets in the FP32 parameter vector */
+constexpr int kMlpOffW[5] = {0, 15424, 19584, 23744, 27904};
+constexpr int kMlpOffB[5] = {15360, 19520, 23680, 27840, 28096};
+constexpr int kMlpFanIn[5] = {240, 64, 64, 64, 64};
+constexpr int kMlpFanOut[5] = {64, 64, 64, 64, 3};
+
+/* packed BF16 forward weights, row major [out_pad][in_pad] */
+constexpr int kPackOff[5] = {0, 16384, 20480, 24576, 28672};
+constexpr int kPackInPad[5] = {256, 64, 64, 64, 64};
+constexpr int kPackTotal = 28672 + 16 * 64;       /* layer 4 padded to 16 rows */
+/* packed BF16 transposes of layers 1..4, row major [in][out_pad] */
+constexpr int kPackTOff[5] = {0, 0, 4096, 8192, 12288};
+constexpr int kPackTOutPad[5] = {0, 64, 64, 64, 32};
+constexpr int kPackTTotal = 12288 + 64 * 32;
+/* FP32 biases: b0..b3 at 64 l, b4 at 256 (padded to 16) */
+constexpr int kBiasTotal = 272;
+
+/* A cache query: network inputs, the path throughput that scales the
+ * prediction, and the path slot whose radiance receives it. */
+struct MlpQuery {
+    float in[kMlpIn];
+    float thr[3];
+    uint32_t slot;
+};
+
+/* One training sample: inputs, target radiance, weight 1 (valid) or 0. */
+struct MlpSampleSoA {
+    float* in;        /* [kMlpIn][stride] */
+    float* target;    /* [3][stride] */
+    float* valid;     /* [stride] */
+    uint32_t stride;
+};
+
+struct MlpWeights {
+    const uint16_t* w;    /* [kPackTotal] */
+    const uint16_t* wt;   /* [kPackTTotal] */
+    const float* bias;    /* [kBiasTotal] */
+};
+
+/* Feature of an input scalar: band k, sin (c = 0) or cos (c = 1) of
+ * 2^k pi x = 2 pi (2^(k-1) x), the scaling exact in binary32. */
+__host__ __device__ inline float mlp_feature(float x, int k, int c)
+{
+    float t = x * __builtin_ldexpf(1.0f, k - 1);
+    float s, co;
+    tr_sincos_turns(t, &s, &co);
+    return c ? co : s;
+}
+
+/* Device buffers of one training step (sized for `capacity` samples). */
+struct MlpTrainBufs {
+    uint16_t* ht;       /* [4][64][capacity]: H1..H4, [neuron][sample] */
+    uint16_t* dzt;      /* [5][64][capacity]: dZ of layers 0..4 (layer 4: 16 rows used) */
+    float* partial;     /* [slices][kMlpParams] */
+    float* loss_part;   /* [grid] */
+    uint32_t capacity;
+};
+
+void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQuery* q, uint32_t n, float* out,
+                      float4* radiance, uint32_t grid);
+void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,
+                      float inv_valid, const MlpTrainBufs& tb, uint32_t grid);
+void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb, uint32_t slices);
+void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* partial, uint32_t slices,
+                     float grad_scale, float lr, float bc1, float bc2, uint16_t* w, uint16_t* wt, float* bias,
+                     int update);
+
+}  // namespace trk
+
+#endif

@@ -283,3 +283,63 @@ class RayTracerScene:
 
 HIT_DTYPE = np.dtype([("t", np.float32), ("u", np.float32), ("v", np.float32), ("geom", np.uint32),
                       ("prim", np.uint32)])
+
+
+class RadianceCacheMLP:
+    """The radiance-cache network over tr_mlp_* (host arrays in and out)."""
+
+    N_INPUTS, N_OUTPUTS, N_PARAMS = 10, 3, 28099
+
+    def __init__(self, seed=1):
+        self.lib = _abi.load()
+        self.h = ctypes.c_void_p()
+        check(self.lib.tr_mlp_create(int(seed), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.tr_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(self.N_PARAMS, dtype=np.float32)
+        check(self.lib.tr_mlp_get_params(self.h, _fptr(p), p.size))
+        return p
+
+    @params.setter
+    def params(self, value):
+        p = np.ascontiguousarray(np.asarray(value, dtype=np.float32))
+        check(self.lib.tr_mlp_set_params(self.h, _fptr(p), p.size))
+
+    def infer(self, inputs):
+        x = np.ascontiguousarray(np.asarray(inputs, dtype=np.float32).reshape(-1, self.N_INPUTS))
+        y = np.zeros((x.shape[0], self.N_OUTPUTS), dtype=np.float32)
+        check(self.lib.tr_mlp_infer(self.h, x.shape[0], _fptr(x), _fptr(y)))
+        return y
+
+    def _arrays(self, inputs, targets, valid):
+        x = np.ascontiguousarray(np.asarray(inputs, dtype=np.float32).reshape(-1, self.N_INPUTS))
+        t = np.ascontiguousarray(np.asarray(targets, dtype=np.float32).reshape(-1, self.N_OUTPUTS))
+        v = None if valid is None else np.ascontiguousarray(np.asarray(valid, dtype=np.float32).reshape(-1))
+        return x, t, v
+
+    def gradient(self, inputs, targets, valid=None):
+        x, t, v = self._arrays(inputs, targets, valid)
+        g = np.zeros(self.N_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        check(self.lib.tr_mlp_gradient(self.h, x.shape[0], _fptr(x), _fptr(t), _fptr(v) if v is not None else None,
+                                       _fptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, inputs, targets, valid=None):
+        x, t, v = self._arrays(inputs, targets, valid)
+        loss = ctypes.c_float()
+        check(self.lib.tr_mlp_train_step(self.h, x.shape[0], _fptr(x), _fptr(t), _fptr(v) if v is not None else None,
+                                         ctypes.byref(loss)))
+        return loss.value

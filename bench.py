@@ -17,6 +17,7 @@ and the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -64,6 +65,57 @@ def cpu_baseline(desc, rows, threads):
     return samples / secs / 1e6, secs, samples, r0
 
 
+REF_IMAGE = os.path.join(ROOT, "tests", "golden", "box_unbiased_fp64.npz")
+
+
+def time_to_rmse(cached, target=1e-2, max_seconds=30.0, warmup_frames=8, frame_spp=16):
+    """Wall time of drawOnce calls until the de-biased RMSE of the running
+    image against the converged FP64 CPU render of the box workload
+    (256 x 256, depth 4, unbiased estimator; tests/golden/make_converged.py)
+    reaches `target`: RMSE^2 = mean over pixels and RGB of (I - I_ref)^2,
+    de-biased by the reference's own squared standard error.  The cached
+    run first trains for `warmup_frames` frames whose image is discarded;
+    that time counts."""
+    from tempestrenderer_amd import workloads
+    from tempestrenderer_amd.scene import RayTracerScene
+    ref = np.load(REF_IMAGE)
+    ref_mean, ref_se = ref["mean"].astype(np.float64), ref["stderr"].astype(np.float64)
+    ref_var = float(np.mean(ref_se * ref_se))
+    desc = workloads.box(256, 256, samples_camera=frame_spp, max_depth=4)
+    desc.settings.update(estimator=1)
+    sc = RayTracerScene(desc.width, desc.height, desc.vpi)
+    desc.populate(sc, gpu_material)
+    sc.commitScene()
+    elapsed = 0.0
+    if cached:
+        sc.setRadianceCache(True, train=True, train_paths=65536, batch=16384, rough_alpha=0.2, seed=1)
+        for _ in range(warmup_frames):
+            t0 = time.perf_counter()
+            sc.drawOnce()
+            elapsed += time.perf_counter() - t0
+        sc.repaint()
+    frames, rmse = 0, float("inf")
+    trace = []
+    while elapsed < max_seconds:
+        t0 = time.perf_counter()
+        sc.drawOnce()
+        elapsed += time.perf_counter() - t0
+        frames += 1
+        acc, n = sc.accumulation()
+        err = acc / n - ref_mean
+        rmse = math.sqrt(max(float(np.mean(err * err)) - ref_var, 0.0))
+        trace.append((round(elapsed, 4), n, round(rmse, 5)))
+        if rmse <= target:
+            break
+    out = {"seconds": round(elapsed, 4) if rmse <= target else None, "frames": frames, "spp": frames * frame_spp,
+           "final_rmse": round(rmse, 5), "ms_per_frame": round(elapsed * 1e3 / max(1, frames + (warmup_frames if cached else 0)), 3)}
+    if cached:
+        out["warmup_frames"] = warmup_frames
+        out["last_loss"] = round(sc.stats()["cache_loss"], 5)
+    sc.close()
+    return out, trace
+
+
 def calibrate_copy(mib=512, reps=4):
     """A plain device-to-device copy of known size, run under
     `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale those counters into
@@ -87,6 +139,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="reduced scene for a quick check (not the metric)")
     ap.add_argument("--cpu-rows", type=int, default=1080, help="rows of the frame the CPU oracle renders (capped at the height)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE runs (plain vs radiance cache)")
+    ap.add_argument("--rmse-only", action="store_true", help="only the time-to-RMSE runs, printed with their traces")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -98,6 +152,12 @@ def main():
         calibrate_copy()
         return
     args.steps = max(1, args.steps)
+    if args.rmse_only:
+        for cached in (False, True):
+            res, trace = time_to_rmse(cached)
+            log("cached" if cached else "plain", json.dumps(res))
+            log("  trace (seconds, spp, rmse):", trace)
+        return
 
     from tempestrenderer_amd import workloads
     from tempestrenderer_amd.scene import RayTracerScene
@@ -155,6 +215,20 @@ def main():
     img = acc / spp
     log("image mean %s, finite %s" % (img.mean(axis=(0, 1)), bool(np.isfinite(img).all())))
 
+    ttr = None
+    if not args.no_rmse and os.path.exists(REF_IMAGE):
+        plain, _ = time_to_rmse(False)
+        log("time-to-RMSE plain: %s" % json.dumps(plain))
+        cached, _ = time_to_rmse(True)
+        log("time-to-RMSE cached: %s" % json.dumps(cached))
+        ref = np.load(REF_IMAGE)
+        ttr = {"workload": "box 256x256, depth 4, unbiased estimator, 16 camera samples per frame",
+               "target_rmse": 0.01, "reference": "FP64 CPU oracle render, %d spp, seed %d" % (int(ref["spp"]),
+                                                                                           int(ref["seed"])),
+               "plain": plain, "cached": cached,
+               "speedup": (round(plain["seconds"] / cached["seconds"], 3)
+                           if plain["seconds"] and cached["seconds"] else None)}
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -202,6 +276,7 @@ def main():
             "avg_launch_us": round(tim["trace_ms"] * 1e3 / max(1, tim["trace_launches"]), 2),
         },
         "cpu_baseline": cpu,
+        "time_to_rmse": ttr,
     }
     print(json.dumps(out), flush=True)
 

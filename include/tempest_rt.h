@@ -190,6 +190,8 @@ typedef struct tr_stats {
     uint64_t prim_tests;         /* primitives tested (only if counting on) */
     uint64_t counted_rays;       /* rays behind node_visits / prim_tests */
     uint64_t volume_steps;       /* tentative collisions of delta / ratio tracking (only if counting on) */
+    double cache_ms;             /* timing mode: radiance-cache inference, targets and training */
+    double cache_loss;           /* loss of the last training step of the call (cache mode) */
 } tr_stats;
 
 /* Scene-construction information. */
@@ -319,6 +321,35 @@ tr_status tr_scene_trace_rays(tr_scene* scene, size_t ray_count, const tr_ray* r
 tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint32_t* pixels,
                                 float* out_rgb, uint32_t* out_hit_hash);
 tr_status tr_scene_get_stats(const tr_scene* scene, tr_stats* stats);
+/* Radiance cache (new; no reference counterpart).  With `enabled`, a path
+ * ends at the vertex after its first rough vertex (microfacet alpha >=
+ * rough_alpha, a lobe without specular part, or an SGGX surface): the
+ * network's prediction of the radiance scattered there, times the path
+ * throughput, replaces next-event estimation and the rest of the path.
+ * About train_paths paths per frame (every k-th path) are training paths:
+ * they run to MaxRayDepth, record the network inputs at their query vertex
+ * and, as target, the radiance they gathered after it divided by their
+ * throughput there; with `train` the frame ends with Adam steps of `batch`
+ * samples over these records.  Needs the unbiased estimator,
+ * SamplesGlobalIllumination 1 and no volumes. */
+typedef struct tr_cache_settings {
+    int32_t enabled;
+    int32_t train;
+    uint32_t train_paths;
+    uint32_t batch;
+    float rough_alpha;
+    uint64_t seed;           /* network initialisation */
+} tr_cache_settings;
+
+tr_status tr_scene_set_radiance_cache(tr_scene* scene, const tr_cache_settings* settings);
+/* The scene's network (owned by the scene). */
+tr_status tr_scene_get_radiance_cache(tr_scene* scene, struct tr_mlp** out_mlp);
+/* Training records of the last frame: inputs count x 10, targets count x 3,
+ * weights count (record r belongs to path p of the frame with
+ * (p + frame) = r * stride, stride = ceil(paths / train_paths)). */
+tr_status tr_scene_read_cache_records(tr_scene* scene, size_t count, float* inputs, float* targets,
+                                      float* valid);
+
 /* Turns on per-launch node/primitive counting (slower; for measurement). */
 tr_status tr_scene_set_counting(tr_scene* scene, int32_t enabled);
 /* Turns on per-kernel timing: every launch carries a start and a stop

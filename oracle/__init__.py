@@ -67,6 +67,8 @@ def _declare(lib):
         "orc_add_sphere_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
         "orc_add_grid_volume": (ctypes.c_int, [vp, P(_abi.tr_grid_volume), P(_abi.tr_material), P(u64)]),
         "orc_commit": (ctypes.c_int, [vp]),
+        "orc_set_cache": (None, [vp, ctypes.c_int, f, u32, u32, P(f), u32]),
+        "orc_cache_records": (None, [vp, sz, P(f), P(f), P(f)]),
         "orc_grid_volume_lookup": (None, [vp, P(f), P(f), f, P(f)]),
         "orc_trace_segments": (None, [vp, sz, P(_abi.tr_ray), u32, P(_abi.tr_hit), ctypes.c_int]),
         "orc_shadow_transmittance": (None, [vp, sz, P(_abi.tr_ray), u32, u32, P(f), ctypes.c_int]),
@@ -215,6 +217,19 @@ class OracleScene:
 
     def commitScene(self):
         self.lib.orc_commit(self.h)
+
+    def setCache(self, enabled, rough_alpha, train_stride, frame, value, records):
+        """Cache mode with a constant network output `value` (RGB)."""
+        v = np.asarray(value, dtype=np.float32)
+        self.lib.orc_set_cache(self.h, int(bool(enabled)), float(rough_alpha), int(train_stride), int(frame), _fp(v),
+                               int(records))
+
+    def cacheRecords(self, count):
+        x = np.zeros((count, 10), dtype=np.float32)
+        t = np.zeros((count, 3), dtype=np.float32)
+        v = np.zeros(count, dtype=np.float32)
+        self.lib.orc_cache_records(self.h, count, _fp(x), _fp(t), _fp(v))
+        return x, t, v
 
     def gridVolumeLookup(self, pos, direction, threshold):
         """First volume at `pos`: (density, frame 3x3 columns, sigma(direction),

@@ -95,6 +95,17 @@ struct orc_scene {
     float background[3] = {0, 0, 0};
     uint32_t key[2] = {1, 0};
     uint32_t estimator = TR_ESTIMATOR_REFERENCE;
+    /* radiance-cache mode with a constant network output (the checker of
+     * the path logic of cache mode) and the training records it writes */
+    struct {
+        bool enabled = false;
+        float rough_alpha = 0.2f;
+        uint32_t train_stride = 1;
+        uint32_t frame = 0;
+        float value[3] = {0, 0, 0};
+        float box_lo[3] = {0, 0, 0}, box_rcp[3] = {1, 1, 1};
+        std::vector<float> in, target, valid;   /* [r][10], [r][3], [r] */
+    } cache;
     std::vector<tr_material> materials;
     std::vector<const tr_material*> material_keys;
     std::vector<std::vector<uint8_t>> vbufs;
@@ -708,11 +719,47 @@ template <class T> bool surface_or_volume(const orc_scene* s, const tr_hit& h, V
     return false;
 }
 
+/* Cache mode: roughness input (microfacet alpha = sqrt(2 / (power + 2)),
+ * 1 without a specular part or for SGGX surfaces) and the network inputs
+ * of a shading point. */
+inline float cache_roughness(const tr_material* m)
+{
+    if (!is_microfacet(m->model)) return 1.0f;
+    if (m->specular[0] == 0.0f && m->specular[1] == 0.0f && m->specular[2] == 0.0f) return 1.0f;
+    float a = sqrtf(2.0f / (m->specular_power[0] + 2.0f));
+    return a < 1.0f ? a : 1.0f;
+}
+
+inline float unit_clamp(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+template <class T> void cache_inputs(const orc_scene* s, V3<T> pos, V3<T> dir, const Sample<T>& sd, float in[10])
+{
+    float p[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
+    for (int a = 0; a < 3; ++a) in[a] = unit_clamp((p[a] - s->cache.box_lo[a]) * s->cache.box_rcp[a]);
+    in[3] = (1.0f - (float)dir.x) * 0.5f;
+    in[4] = (1.0f - (float)dir.y) * 0.5f;
+    in[5] = (1.0f - (float)dir.z) * 0.5f;
+    in[6] = ((float)sd.normal.x + 1.0f) * 0.5f;
+    in[7] = ((float)sd.normal.y + 1.0f) * 0.5f;
+    in[8] = ((float)sd.normal.z + 1.0f) * 0.5f;
+    in[9] = cache_roughness(sd.mat);
+}
+
 /* One camera sample of one pixel: the body of the camera loop of subdraw
  * (ray-tracing.cc:2011-2345) for surfaces. */
 template <class T>
-V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint32_t* hash_out)
+V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint32_t* hash_out, uint64_t pseq = 0)
 {
+    /* cache mode: training path iff (path index + frame) % stride == 0 */
+    const bool cache = s->cache.enabled;
+    uint32_t record = 0xffffffffu;
+    if (cache) {
+        uint64_t path = pseq * s->spp + (cs - s->cache.frame * s->spp) + s->cache.frame;
+        if (path % s->cache.train_stride == 0) record = (uint32_t)(path / s->cache.train_stride);
+    }
+    bool rough = false, recorded = false;
+    float rec_in[10];
+    V3<T> rec_thr = v3((T)0, (T)0, (T)0), rec_before = rec_thr;
     uint32_t x = pixel % s->width, y = pixel / s->width;
     Rng rng;
     rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
@@ -819,6 +866,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
                 ORC_TRACE("   balance=%g T=(%g %g %g) f=(%g %g %g)\n", (double)balance, (double)ct.x, (double)ct.y,
                           (double)ct.z, (double)bsdf_eval(cur).x, (double)bsdf_eval(cur).y, (double)bsdf_eval(cur).z);
             }
+            if (cache && !recorded && cache_roughness(cur.mat) >= s->cache.rough_alpha) rough = true;
             float o2[3] = {(float)cpos.x, (float)cpos.y, (float)cpos.z};
             float d2[3] = {(float)cur.wi.x, (float)cur.wi.y, (float)cur.wi.z};
             tr_hit hh = trace_segment(s, o2, d2, srng, d + 1);
@@ -846,6 +894,20 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
                 break;
             }
             cur.wo = -cur.wi;
+            if (cache && rough) {
+                /* the query vertex: a constant prediction ends the path; a
+                 * training path records it and goes on */
+                if (record == 0xffffffffu) {
+                    gsplit += ct * v3((T)xmax(s->cache.value[0], 0.0f), (T)xmax(s->cache.value[1], 0.0f),
+                                      (T)xmax(s->cache.value[2], 0.0f));
+                    break;
+                }
+                cache_inputs(s, cpos, cur.wi, cur, rec_in);
+                rec_thr = ct;
+                rec_before = gsplit;
+                rough = false;
+                recorded = true;
+            }
             ss = 1;
             ls = 1;
             gsplit += evaluate_lights(s, srng, d + 1, cur, ls, ss, cpos, ct, d + 1 != s->max_depth);
@@ -857,6 +919,26 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
                 srng.draw(d + 1, STREAM_RR, 0, xq);
                 if ((T)orc_u01(xq[0]) > (T)s->rr) break;
             }
+        }
+        if (cache && record != 0xffffffffu && record < s->cache.valid.size()) {
+            orc_scene* ms = const_cast<orc_scene*>(s);
+            float* tg = &ms->cache.target[(size_t)record * 3];
+            float vt = 0.0f;
+            if (recorded) {
+                V3<T> after = gsplit;
+                T th[3] = {rec_thr.x, rec_thr.y, rec_thr.z}, g1[3] = {after.x, after.y, after.z},
+                  g0[3] = {rec_before.x, rec_before.y, rec_before.z};
+                bool ok = true;
+                for (int c = 0; c < 3; ++c) {
+                    ok = ok && (float)th[c] > 1e-8f;
+                    tg[c] = ok ? ((float)g1[c] - (float)g0[c]) / (float)th[c] : 0.0f;
+                }
+                vt = ok ? 1.0f : 0.0f;
+                memcpy(&ms->cache.in[(size_t)record * 10], rec_in, sizeof(rec_in));
+            } else {
+                tg[0] = tg[1] = tg[2] = 0.0f;
+            }
+            ms->cache.valid[record] = vt;
         }
         if (split == 0) { global = gsplit; combined = hash; }
         else { global += gsplit; combined = (combined * 0x9E3779B1u) ^ hash; }
@@ -908,6 +990,31 @@ void orc_set_russian_roulette(orc_scene* s, float rr) { s->rr = rr; }
 void orc_set_background_spectrum(orc_scene* s, const float rgb[3]) { memcpy(s->background, rgb, 12); }
 void orc_set_seed(orc_scene* s, uint64_t seed) { s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32); }
 void orc_set_estimator(orc_scene* s, uint32_t mode) { s->estimator = mode; }
+
+/* Cache mode with a constant network output `value`; the scene box of the
+ * position input is the box of all primitives (set at commit).  Training
+ * records for `records` training paths are kept for orc_cache_records. */
+void orc_set_cache(orc_scene* s, int enabled, float rough_alpha, uint32_t train_stride, uint32_t frame,
+                   const float value[3], uint32_t records)
+{
+    s->cache.enabled = enabled != 0;
+    s->cache.rough_alpha = rough_alpha;
+    s->cache.train_stride = train_stride ? train_stride : 1;
+    s->cache.frame = frame;
+    memcpy(s->cache.value, value, sizeof(s->cache.value));
+    s->cache.in.assign((size_t)records * 10, 0.0f);
+    s->cache.target.assign((size_t)records * 3, 0.0f);
+    s->cache.valid.assign(records, 0.0f);
+}
+
+void orc_cache_records(const orc_scene* s, size_t count, float* in, float* target, float* valid)
+{
+    for (size_t r = 0; r < count && r < s->cache.valid.size(); ++r) {
+        memcpy(in + r * 10, &s->cache.in[r * 10], 10 * sizeof(float));
+        memcpy(target + r * 3, &s->cache.target[r * 3], 3 * sizeof(float));
+        valid[r] = s->cache.valid[r];
+    }
+}
 
 int orc_add_triangle_mesh(orc_scene* s, const float world[16], size_t submesh_count, const tr_submesh* subs,
                           size_t index_count, const int32_t* tris, size_t vert_size, const void* verts,
@@ -1075,6 +1182,20 @@ int orc_add_sphere_light_source(orc_scene* s, const tr_light* light, uint32_t* o
 int orc_commit(orc_scene* s)
 {
     uint32_t n = (uint32_t)s->prims.size();
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (const Prim& p : s->prims) {
+            float pl[3], ph[3];
+            prim_bounds(p, pl, ph);
+            for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], pl[a]); hi[a] = std::max(hi[a], ph[a]); }
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (!(lo[a] <= hi[a])) { lo[a] = 0.0f; hi[a] = 1.0f; }
+            float ext = hi[a] - lo[a];
+            s->cache.box_lo[a] = lo[a];
+            s->cache.box_rcp[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
+        }
+    }
     s->prim_order.resize(n);
     s->nodes.clear();
     if (n == 0) { s->committed = true; return 0; }
@@ -1113,7 +1234,7 @@ void orc_render_paths(const orc_scene* s, size_t pixel_count, const uint32_t* pi
     parallel_for(total, nthreads, 16, [&](uint32_t i) {
         uint32_t pi = i / s->spp, k = i % s->spp;
         uint32_t hash;
-        V3<float> c = render_camera_sample<float>(s, pixels[pi], frame * s->spp + k, &hash);
+        V3<float> c = render_camera_sample<float>(s, pixels[pi], frame * s->spp + k, &hash, pi);
         out_rgb[i * 3 + 0] = c.x;
         out_rgb[i * 3 + 1] = c.y;
         out_rgb[i * 3 + 2] = c.z;

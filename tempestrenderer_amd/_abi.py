@@ -76,8 +76,13 @@ class tr_stats(ctypes.Structure):
         ("frame_ms", ctypes.c_double), ("trace_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
         ("shadow_ms", ctypes.c_double), ("other_ms", ctypes.c_double), ("trace_launches", ctypes.c_uint64),
         ("node_visits", ctypes.c_uint64), ("prim_tests", ctypes.c_uint64), ("counted_rays", ctypes.c_uint64),
-        ("volume_steps", ctypes.c_uint64),
+        ("volume_steps", ctypes.c_uint64), ("cache_ms", ctypes.c_double), ("cache_loss", ctypes.c_double),
     ]
+
+
+class tr_cache_settings(ctypes.Structure):
+    _fields_ = [("enabled", ctypes.c_int32), ("train", ctypes.c_int32), ("train_paths", ctypes.c_uint32),
+                ("batch", ctypes.c_uint32), ("rough_alpha", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 
 class tr_grid_volume(ctypes.Structure):
@@ -154,6 +159,9 @@ SIGNATURES = {
     "tr_scene_trace_rays": (_st, [_vp, _sz, P(tr_ray), P(tr_hit)]),
     "tr_scene_render_paths": (_st, [_vp, _sz, P(_u32), P(ctypes.c_float), P(_u32)]),
     "tr_scene_get_stats": (_st, [_vp, P(tr_stats)]),
+    "tr_scene_set_radiance_cache": (_st, [_vp, P(tr_cache_settings)]),
+    "tr_scene_get_radiance_cache": (_st, [_vp, P(_vp)]),
+    "tr_scene_read_cache_records": (_st, [_vp, _sz, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float)]),
     "tr_scene_set_counting": (_st, [_vp, ctypes.c_int32]),
     "tr_scene_set_timing": (_st, [_vp, ctypes.c_int32]),
     "tr_mlp_create": (_st, [_u64, P(_vp)]),

@@ -13,6 +13,7 @@
 #include "../../include/tempest_rt.h"
 #include "../../include/tempest_rt_debug.h"
 #include "tr_bvh8.h"
+#include "tr_cache.h"
 #include "tr_host.h"
 #include "tr_kernels.h"
 #include "tr_shading.h"
@@ -36,7 +37,7 @@ thread_local std::string g_error;
 
 namespace {
 
-enum { EV_TRACE = 0, EV_SHADE = 1, EV_SHADOW = 2, EV_OTHER = 3 };
+enum { EV_TRACE = 0, EV_SHADE = 1, EV_SHADOW = 2, EV_OTHER = 3, EV_CACHE = 4 };
 
 }  // namespace
 
@@ -55,6 +56,18 @@ struct tr_scene {
     uint32_t frame_index = 0;
     bool counting = false;
     bool timing = false;
+    bool cache_trained = false;
+    uint32_t cache_steps = 0;
+    float cache_step_w[64];
+
+    /* radiance cache */
+    tr_cache_settings cache;
+    tr_mlp* net = nullptr;
+    float box_lo[3] = {0, 0, 0}, box_hi[3] = {1, 1, 1};
+    uint32_t rec_cap = 0;
+    DevBuf<float> rec_thr, rec_before;
+    DevBuf<uint32_t> rec_slot, pflags;
+    DevBuf<MlpQuery> queries;
 
     /* host copy of the scene */
     std::vector<tr_material> mats;
@@ -111,6 +124,7 @@ struct tr_scene {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     tr_stats stats;
 
+    tr_scene() { memset(&cache, 0, sizeof(cache)); }
     ~tr_scene();
 };
 
@@ -148,6 +162,7 @@ tr_scene::~tr_scene()
     ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
     sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
     shq.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release();
+    if (net) tr_mlp_destroy(net);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -254,6 +269,8 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->shq.reserve(cap));
     TR_HIP(s->sh_n.reserve(cap));
     TR_HIP(s->sh_tmax.reserve(cap));
+    TR_HIP(s->pflags.reserve(cap));
+    if (s->cache.enabled) TR_HIP(s->queries.reserve(cap));
     TR_HIP(s->sh_o.reserve((size_t)cap * msh));
     TR_HIP(s->sh_d.reserve((size_t)cap * msh));
     TR_HIP(s->sh_c.reserve((size_t)cap * msh));
@@ -272,7 +289,7 @@ PathPool path_pool(const tr_scene* s)
     p.thr = s->thr.ptr; p.local = s->local.ptr; p.global = s->global.ptr; p.hash = s->hash.ptr;
     p.queue[0] = s->q0.ptr; p.queue[1] = s->q1.ptr; p.shadow_queue = s->shq.ptr;
     p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr; p.sh_tmax = s->sh_tmax.ptr;
-    p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity;
+    p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity; p.pflags = s->pflags.ptr;
     return p;
 }
 
@@ -295,6 +312,14 @@ FrameParams frame_params(const tr_scene* s)
     p.key1 = (uint32_t)(s->seed >> 32);
     p.frame = s->frame_index;
     p.estimator = s->estimator;
+    p.cache = s->cache.enabled ? 1u : 0u;
+    p.train_stride = 1;
+    p.rough_alpha = s->cache.rough_alpha;
+    for (int a = 0; a < 3; ++a) {
+        p.box_lo[a] = s->box_lo[a];
+        float ext = s->box_hi[a] - s->box_lo[a];
+        p.box_rcp[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
     memcpy(p.vpi, s->vpi, sizeof(p.vpi));
     return p;
 }
@@ -316,13 +341,45 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
                                           "sampler has no closed-form density");
     if (!s->vols.empty() && (s->lights.size() > 256 || s->n_light > 256))
         return fail(TR_ERROR_UNSUPPORTED, "scenes with volumes take at most 256 lights and 256 samples per area light");
+    if (s->cache.enabled) {
+        if (s->estimator != TR_ESTIMATOR_UNBIASED)
+            return fail(TR_ERROR_UNSUPPORTED, "the radiance cache needs the unbiased estimator");
+        if (s->n_gi != 1) return fail(TR_ERROR_UNSUPPORTED, "the radiance cache needs SamplesGlobalIllumination = 1");
+        if (!s->vols.empty()) return fail(TR_ERROR_UNSUPPORTED, "the radiance cache does not take volumes");
+    }
     tr_status st = sync_lights(s);
     if (st != TR_OK) return st;
     uint64_t per_pixel = (uint64_t)s->spp * s->n_gi;
     st = ensure_pool(s, n_pixels * per_pixel);
     if (st != TR_OK) return st;
+    if (s->cache.enabled && s->queries.count < s->capacity) TR_HIP(s->queries.reserve(s->capacity));
     FrameParams p = frame_params(s);
     PathPool pool = path_pool(s);
+    /* cache mode: every train_stride-th path of the frame trains */
+    uint64_t total_paths = n_pixels * per_pixel;
+    CacheRecords rec;
+    memset(&rec, 0, sizeof(rec));
+    uint32_t n_records = 0;
+    if (s->cache.enabled) {
+        uint32_t want = std::max(1u, s->cache.train_paths);
+        p.train_stride = (uint32_t)std::max<uint64_t>(1, (total_paths + want - 1) / want);
+        n_records = (uint32_t)((total_paths + p.frame + p.train_stride - 1) / p.train_stride);
+        MlpState& ms = s->net->st;
+        if ((st = ms.ensure_train(n_records)) != TR_OK) return st;
+        if (s->rec_cap < ms.train_cap) {
+            TR_HIP(s->rec_thr.reserve((size_t)3 * ms.train_cap));
+            TR_HIP(s->rec_before.reserve((size_t)3 * ms.train_cap));
+            TR_HIP(s->rec_slot.reserve(ms.train_cap));
+            s->rec_cap = ms.train_cap;
+        }
+        rec.smp = ms.samples();
+        rec.thr = s->rec_thr.ptr;
+        rec.before = s->rec_before.ptr;
+        rec.slot = s->rec_slot.ptr;
+        rec.queries = s->queries.ptr;
+        /* records beyond the frame's paths stay at weight 0 */
+        TR_HIP(hipMemsetAsync(ms.s_valid.ptr, 0, (size_t)ms.train_cap * sizeof(float), s->stream));
+    }
     uint32_t ppb = (uint32_t)std::max<uint64_t>(1, s->capacity / per_pixel);
     uint32_t grid = persistent_grid(s);
     uint64_t* counts = nullptr;
@@ -349,13 +406,31 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         record(s, cat, ev.start, ev.stop);
         return TR_OK;
     };
+    /* A stage of several launches: in timing mode bracketed by events
+     * recorded on the stream, after the previous stage has completed. */
+    auto stage_multi = [&](int cat, auto&& launch) -> tr_status {
+        if (!s->timing) {
+            launch();
+            TR_HIP(hipGetLastError());
+            return TR_OK;
+        }
+        hipEvent_t a = get_event(s), b = get_event(s);
+        if (!a || !b) return fail(TR_ERROR_DEVICE, "event creation failed");
+        TR_HIP(hipEventRecord(a, s->stream));
+        launch();
+        TR_HIP(hipGetLastError());
+        TR_HIP(hipEventRecord(b, s->stream));
+        TR_HIP(hipEventSynchronize(b));
+        record(s, cat, a, b);
+        return TR_OK;
+    };
     for (uint64_t pix0 = 0; pix0 < n_pixels; pix0 += ppb) {
         uint32_t npx = (uint32_t)std::min<uint64_t>(ppb, n_pixels - pix0);
         uint32_t nslots = (uint32_t)(npx * per_pixel);
         uint64_t slot_base = pix0 * per_pixel;
         tr_status st2;
         if ((st2 = stage(EV_OTHER, [&](const LaunchEvents* ev) {
-                 launch_raygen(s->stream, p, pool, out, slot_base, nslots, ev);
+                 launch_raygen(s->stream, p, pool, out, slot_base, nslots, rec, ev);
              })) != TR_OK)
             return st2;
         int cur = 0, nxt = 1;
@@ -365,9 +440,16 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
                  })) != TR_OK)
                 return st2;
             if ((st2 = stage(EV_SHADE, [&](const LaunchEvents* ev) {
-                     launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots, ev);
+                     launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots, rec, ev);
                  })) != TR_OK)
                 return st2;
+            if (s->cache.enabled && v > 0) {
+                MlpWeights w = s->net->st.weights();
+                if ((st2 = stage_multi(EV_CACHE, [&]() {
+                         launch_cache_infer(s->stream, w, pool, rec, (uint32_t)std::max(1, s->cu_count) * 2);
+                     })) != TR_OK)
+                    return st2;
+            }
             if (p.sc.n_lights > 0) {
                 if ((st2 = stage(EV_SHADOW, [&](const LaunchEvents* ev) {
                          launch_shadow(s->stream, p, pool, out, slot_base, v, grid, counts, ev);
@@ -376,10 +458,38 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
             }
             std::swap(cur, nxt);
         }
+        if (s->cache.enabled) {
+            /* records of the paths of this batch: path q = slot_base + i */
+            uint64_t q0 = slot_base + p.frame, q1 = slot_base + nslots + p.frame;
+            uint32_t r0 = (uint32_t)((q0 + p.train_stride - 1) / p.train_stride);
+            uint32_t r1 = (uint32_t)((q1 + p.train_stride - 1) / p.train_stride);
+            if ((st2 = stage(EV_CACHE, [&](const LaunchEvents* ev) {
+                     launch_cache_targets(s->stream, pool, rec, r0, r1, ev);
+                 })) != TR_OK)
+                return st2;
+        }
         if ((st2 = stage(EV_OTHER, [&](const LaunchEvents* ev) {
                  launch_accumulate(s->stream, p, pool, out, slot_base, npx, ev);
              })) != TR_OK)
             return st2;
+    }
+    if (s->cache.enabled && s->cache.train) {
+        /* online training on this frame's records, in batches */
+        MlpState& ms = s->net->st;
+        uint32_t batch = std::max(MlpState::kSlice, s->cache.batch / MlpState::kSlice * MlpState::kSlice);
+        /* records below ceil(frame / stride) belong to no path of the frame */
+        uint32_t first = (uint32_t)((p.frame + p.train_stride - 1) / p.train_stride) / MlpState::kSlice * MlpState::kSlice;
+        s->cache_steps = 0;
+        for (uint32_t off = first; off < n_records; off += batch) {
+            uint32_t nb = std::min(batch, n_records - off);
+            tr_status st3 = TR_OK;
+            uint32_t slot = s->cache_steps;
+            tr_status st2 = stage_multi(EV_CACHE, [&]() { st3 = ms.enqueue_step(nb, nb, s->stream, true, off, slot); });
+            if (st2 != TR_OK) return st2;
+            if (st3 != TR_OK) return st3;
+            if (s->cache_steps < MlpState::kMaxLossSlots) s->cache_step_w[s->cache_steps++] = (float)nb;
+        }
+        s->cache_trained = true;
     }
     return TR_OK;
 }
@@ -387,7 +497,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
 /* Collects event timings after the stream has been synchronised. */
 void collect_stats(tr_scene* s, uint64_t paths)
 {
-    double ms[4] = {0, 0, 0, 0};
+    double ms[5] = {0, 0, 0, 0, 0};
     uint64_t traces = 0;
     for (auto& u : s->ev_used) {
         float t = 0.0f;
@@ -402,6 +512,10 @@ void collect_stats(tr_scene* s, uint64_t paths)
     s->stats.shade_ms = ms[EV_SHADE];
     s->stats.shadow_ms = ms[EV_SHADOW];
     s->stats.other_ms = ms[EV_OTHER];
+    s->stats.cache_ms = ms[EV_CACHE];
+    s->stats.cache_loss = 0.0;
+    if (s->cache_trained && s->net) s->stats.cache_loss = s->net->st.read_loss(s->cache_steps, s->cache_step_w);
+    s->cache_trained = false;
     s->stats.trace_launches = traces;
     s->stats.node_visits = s->stats.prim_tests = s->stats.counted_rays = 0;
     s->stats.segments = s->stats.shadow_rays = s->stats.volume_steps = 0;
@@ -452,6 +566,30 @@ tr_status start_frame(tr_scene* s)
 }
 
 bool valid(const tr_scene* s) { return s != nullptr; }
+
+/* Bounding box of all primitives: the cache's position input is the
+ * position within it. */
+void scene_box(tr_scene* s)
+{
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const PrimRec& p : s->prims_in)
+        for (int a = 0; a < 3; ++a) {
+            float pl, ph;
+            if (p.flags & PRIM_SPHERE) { pl = p.v0[a] - p.v1[0]; ph = p.v0[a] + p.v1[0]; }
+            else if (p.flags & PRIM_VOLUME) { pl = p.v0[a]; ph = p.v1[a]; }
+            else {
+                pl = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
+                ph = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));
+            }
+            lo[a] = std::min(lo[a], pl);
+            hi[a] = std::max(hi[a], ph);
+        }
+    for (int a = 0; a < 3; ++a) {
+        if (!(lo[a] <= hi[a])) { lo[a] = 0.0f; hi[a] = 1.0f; }
+        s->box_lo[a] = lo[a];
+        s->box_hi[a] = hi[a];
+    }
+}
 
 }  // namespace
 
@@ -553,6 +691,48 @@ tr_status tr_scene_set_estimator(tr_scene* s, uint32_t mode)
     if (mode != TR_ESTIMATOR_REFERENCE && mode != TR_ESTIMATOR_UNBIASED)
         return fail(TR_ERROR_INVALID_ARG, "unknown estimator");
     s->estimator = mode;
+    return TR_OK;
+}
+
+tr_status tr_scene_set_radiance_cache(tr_scene* s, const tr_cache_settings* c)
+{
+    if (!valid(s) || !c) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (c->enabled) {
+        if (!(c->rough_alpha >= 0.0f) || c->train_paths == 0 || c->batch == 0)
+            return fail(TR_ERROR_INVALID_ARG, "cache settings need train_paths > 0, batch > 0, rough_alpha >= 0");
+        if (!s->net) {
+            tr_status st = tr_mlp_create(c->seed, &s->net);
+            if (st != TR_OK) return st;
+        }
+    }
+    s->cache = *c;
+    return TR_OK;
+}
+
+tr_status tr_scene_get_radiance_cache(tr_scene* s, tr_mlp** out)
+{
+    if (!valid(s) || !out) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->net) return fail(TR_ERROR_NOT_COMMITTED, "the radiance cache has not been enabled");
+    *out = s->net;
+    return TR_OK;
+}
+
+tr_status tr_scene_read_cache_records(tr_scene* s, size_t count, float* inputs, float* targets, float* valid_out)
+{
+    if (!valid(s) || !inputs || !targets || !valid_out) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->net || s->net->st.train_cap < count) return fail(TR_ERROR_INVALID_ARG, "fewer records than requested");
+    TR_HIP(hipStreamSynchronize(s->stream));
+    MlpState& ms = s->net->st;
+    size_t cap = ms.train_cap;
+    std::vector<float> in((size_t)kMlpIn * cap), tg((size_t)3 * cap), vd(cap);
+    TR_HIP(hipMemcpy(in.data(), ms.s_in.ptr, in.size() * sizeof(float), hipMemcpyDeviceToHost));
+    TR_HIP(hipMemcpy(tg.data(), ms.s_target.ptr, tg.size() * sizeof(float), hipMemcpyDeviceToHost));
+    TR_HIP(hipMemcpy(vd.data(), ms.s_valid.ptr, vd.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < count; ++r) {
+        for (int k = 0; k < kMlpIn; ++k) inputs[r * kMlpIn + k] = in[(size_t)k * cap + r];
+        for (int c = 0; c < 3; ++c) targets[r * 3 + c] = tg[(size_t)c * cap + r];
+        valid_out[r] = vd[r];
+    }
     return TR_OK;
 }
 
@@ -825,6 +1005,7 @@ tr_status tr_scene_commit(tr_scene* s)
         auto t0 = std::chrono::steady_clock::now();
         std::string err;
         if (!build_bvh8(s->prims_in, s->nodes, s->prims, &s->bvh_stats, &err)) return fail(TR_ERROR_INVALID_ARG, err);
+        scene_box(s);
         auto t1 = std::chrono::steady_clock::now();
         s->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         s->committed = true;

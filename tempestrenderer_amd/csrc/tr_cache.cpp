@@ -64,7 +64,7 @@ tr_status MlpState::ensure_train(uint32_t n)
     TR_HIP(dzt.reserve((size_t)5 * 64 * cap));
     TR_HIP(hipMemsetAsync(dzt.ptr, 0, (size_t)5 * 64 * cap * sizeof(uint16_t), stream));
     TR_HIP(partial.reserve((size_t)(cap / kSlice) * kMlpParams));
-    TR_HIP(loss_part.reserve(kTrainGrid));
+    TR_HIP(loss_part.reserve((size_t)kTrainGrid * kMaxLossSlots));
     TR_HIP(s_in.reserve((size_t)kMlpIn * cap));
     TR_HIP(s_target.reserve((size_t)3 * cap));
     TR_HIP(s_valid.reserve(cap));
@@ -105,13 +105,21 @@ MlpTrainBufs MlpState::train_bufs() const
 /* One pass over the `n` resident samples (padded with weight-0 samples to
  * a multiple of kSlice): forward, loss gradient, backward, weight gradients
  * into per-slice partials; with `update`, Adam. */
-tr_status MlpState::enqueue_step(uint32_t n, uint32_t n_valid, hipStream_t s, bool update)
+tr_status MlpState::enqueue_step(uint32_t n, uint32_t n_valid, hipStream_t s, bool update, uint32_t offset,
+                                 uint32_t loss_slot)
 {
+    if (loss_slot >= kMaxLossSlots) loss_slot = kMaxLossSlots - 1;
     uint32_t padded = (n + kSlice - 1) / kSlice * kSlice;
+    if (offset + padded > train_cap) return fail(TR_ERROR_INVALID_ARG, "training batch beyond the sample buffer");
     float inv_valid = 1.0f / (float)std::max(1u, n_valid);
     MlpTrainBufs tb = train_bufs();
-    launch_mlp_train(s, weights(), samples(), padded, loss_scale, inv_valid, tb, kTrainGrid);
-    launch_mlp_wgrad(s, samples(), padded, tb, padded / kSlice);
+    tb.loss_part += (size_t)loss_slot * kTrainGrid;
+    MlpSampleSoA smp = samples();
+    smp.in += offset;
+    smp.target += offset;
+    smp.valid += offset;
+    launch_mlp_train(s, weights(), smp, padded, loss_scale, inv_valid, tb, kTrainGrid);
+    launch_mlp_wgrad(s, smp, padded, tb, padded / kSlice);
     last_slices = padded / kSlice;
     if (update) {
         ++step;
@@ -144,22 +152,24 @@ tr_status MlpState::upload_samples(size_t n, const float* inputs, const float* t
     return TR_OK;
 }
 
-float MlpState::read_loss()
+float MlpState::read_loss(uint32_t n_slots, const float* w)
 {
-    std::vector<float> lp(kTrainGrid);
-    if (hipMemcpy(lp.data(), loss_part.ptr, kTrainGrid * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return NAN;
-    double s = 0.0;
-    for (float v : lp) s += v;
-    return (float)s;
+    n_slots = std::max(1u, std::min(n_slots, kMaxLossSlots));
+    std::vector<float> lp((size_t)kTrainGrid * n_slots);
+    if (hipMemcpy(lp.data(), loss_part.ptr, lp.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return NAN;
+    double total = 0.0, wsum = 0.0;
+    for (uint32_t k = 0; k < n_slots; ++k) {
+        double s = 0.0;
+        for (uint32_t i = 0; i < kTrainGrid; ++i) s += lp[(size_t)k * kTrainGrid + i];
+        double wk = w ? w[k] : 1.0;
+        total += wk * s;
+        wsum += wk;
+    }
+    return wsum > 0.0 ? (float)(total / wsum) : 0.0f;
 }
 
 }  // namespace trk
 
-struct tr_mlp {
-    MlpState st;
-    DevBuf<MlpQuery> q;
-    DevBuf<float> out;
-};
 
 extern "C" {
 

@@ -13,6 +13,7 @@ namespace trk {
 struct MlpState {
     static constexpr uint32_t kSlice = 512;      /* samples per weight-gradient slice */
     static constexpr uint32_t kTrainGrid = 256;  /* blocks of the training kernel */
+    static constexpr uint32_t kMaxLossSlots = 64; /* per-step loss partials kept per frame */
 
     hipStream_t stream = nullptr;
     int cu_count = 0;
@@ -32,13 +33,21 @@ struct MlpState {
     tr_status ensure_train(uint32_t n);
     tr_status upload_samples(size_t n, const float* inputs, const float* targets, const float* valid,
                              uint32_t* n_valid);
-    tr_status enqueue_step(uint32_t n, uint32_t n_valid, hipStream_t s, bool update);
-    float read_loss();
+    tr_status enqueue_step(uint32_t n, uint32_t n_valid, hipStream_t s, bool update, uint32_t offset = 0,
+                           uint32_t loss_slot = 0);
+    /* loss of one step (slot), or the mean of slots [0, n) weighted by `w` */
+    float read_loss(uint32_t n_slots = 1, const float* w = nullptr);
     MlpWeights weights() const;
     MlpSampleSoA samples() const;
     MlpTrainBufs train_bufs() const;
 };
 
 }  // namespace trk
+
+struct tr_mlp {
+    trk::MlpState st;
+    trk::DevBuf<trk::MlpQuery> q;
+    trk::DevBuf<float> out;
+};
 
 #endif

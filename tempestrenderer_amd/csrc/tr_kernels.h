@@ -13,6 +13,7 @@
 #define TR_KERNELS_H
 
 #include <hip/hip_runtime.h>
+#include "tr_mlp.h"
 #include "tr_scene.h"
 
 namespace trk {
@@ -32,6 +33,26 @@ struct FrameParams {
     uint32_t frame;
     uint32_t estimator;  /* TR_ESTIMATOR_REFERENCE or TR_ESTIMATOR_UNBIASED */
     float vpi[16];
+    /* radiance cache (cache mode only) */
+    uint32_t cache;          /* 1: paths end at the vertex after their first rough one */
+    uint32_t train_stride;   /* path p of a frame trains iff (p + frame) % train_stride == 0 */
+    float rough_alpha;       /* roughness threshold */
+    float box_lo[3];         /* scene box for the position input */
+    float box_rcp[3];
+};
+
+/* Path flags of cache mode. */
+enum { PF_TRAIN = 1u, PF_ROUGH = 2u, PF_RECORDED = 4u };
+
+/* Training records of one frame, indexed by training path: the network
+ * inputs at the path's query vertex, the throughput there, the path
+ * radiance gathered before it and the path's slot. */
+struct CacheRecords {
+    MlpSampleSoA smp;     /* inputs, targets, valid (capacity = smp.stride) */
+    float* thr;           /* [3][stride] */
+    float* before;        /* [3][stride] */
+    uint32_t* slot;       /* [stride]: batch-local slot, or ~0 if none */
+    MlpQuery* queries;    /* query queue of the current vertex */
 };
 
 /* Structure-of-arrays path state; every array has `capacity` entries
@@ -52,7 +73,8 @@ struct PathPool {
     float4* sh_c;       /* contribution, .w = light index (bits) */
     uint32_t* sh_n;     /* entries per slot */
     float* sh_tmax;     /* largest path-throughput component at the shading point */
-    uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size */
+    uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size, [3] cache queries */
+    uint32_t* pflags;   /* cache mode path flags */
     uint2* spill;       /* traversal stack spill, per persistent thread */
     uint32_t capacity;
 };
@@ -78,11 +100,19 @@ constexpr int kStackLds = 16;         /* stack entries per lane kept in LDS */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
 
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                   uint32_t n_slots, const LaunchEvents* ev = nullptr);
+                   uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev = nullptr);
 void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                   uint32_t vertex, int cur, int nxt, uint32_t grid, uint64_t* counts, const LaunchEvents* ev = nullptr);
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                  uint32_t vertex, int cur, int nxt, uint32_t n_slots, const LaunchEvents* ev = nullptr);
+                  uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec,
+                  const LaunchEvents* ev = nullptr);
+/* Cache mode: network predictions for the queries of the last shade, added
+ * to their paths' radiance. */
+void launch_cache_infer(hipStream_t s, const MlpWeights& w, const PathPool& pool, const CacheRecords& rec,
+                        uint32_t grid, const LaunchEvents* ev = nullptr);
+/* Cache mode: targets of the training records [r0, r1) of a batch. */
+void launch_cache_targets(hipStream_t s, const PathPool& pool, const CacheRecords& rec, uint32_t r0, uint32_t r1,
+                          const LaunchEvents* ev = nullptr);
 void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
                    uint64_t slot_base, uint32_t vertex, uint32_t grid, uint64_t* counts,
                    const LaunchEvents* ev = nullptr);

@@ -91,10 +91,62 @@ __device__ inline void slot_coords(const FrameParams& p, const BatchOut& out, ui
 __device__ inline V3 ld4(float4 v) { return mk(v.x, v.y, v.z); }
 __device__ inline float4 st4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+/* ---------------- radiance cache helpers ---------------- */
+
+/* Roughness input of the cache: the microfacet alpha (sqrt of
+ * SpecularPowerToGeometryFactorSq, microfacet-common.hh:109-112), 1 for
+ * lobes without a specular part and for SGGX surfaces. */
+__device__ inline float cache_roughness(const tr_material* m)
+{
+    if (!is_microfacet(m->model)) return 1.0f;
+    if (m->specular[0] == 0.0f && m->specular[1] == 0.0f && m->specular[2] == 0.0f) return 1.0f;
+    float a = sqrtf(geom_factor_sq(m->specular_power[0]));
+    return a < 1.0f ? a : 1.0f;
+}
+
+__device__ inline float unit_clamp(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+__device__ inline void cache_inputs(const FrameParams& p, V3 pos, V3 dir, const Sample& sd, float in[kMlpIn])
+{
+    in[0] = unit_clamp((pos.x - p.box_lo[0]) * p.box_rcp[0]);
+    in[1] = unit_clamp((pos.y - p.box_lo[1]) * p.box_rcp[1]);
+    in[2] = unit_clamp((pos.z - p.box_lo[2]) * p.box_rcp[2]);
+    in[3] = (1.0f - dir.x) * 0.5f;
+    in[4] = (1.0f - dir.y) * 0.5f;
+    in[5] = (1.0f - dir.z) * 0.5f;
+    in[6] = (sd.normal.x + 1.0f) * 0.5f;
+    in[7] = (sd.normal.y + 1.0f) * 0.5f;
+    in[8] = (sd.normal.z + 1.0f) * 0.5f;
+    in[9] = cache_roughness(sd.mat);
+}
+
+/* Training record of a path, or ~0 if it is not a training path: path
+ * p = pixel sequence * spp + camera sample of the frame trains iff
+ * (p + frame) % train_stride == 0, as record (p + frame) / train_stride. */
+__device__ inline uint32_t train_record(const FrameParams& p, uint64_t pseq, uint32_t cs)
+{
+    uint64_t path = pseq * p.spp + (cs - p.frame * p.spp) + p.frame;
+    if (path % p.train_stride != 0) return 0xffffffffu;
+    return (uint32_t)(path / p.train_stride);
+}
+
+/* Wave-aggregated append of a query record. */
+__device__ inline void query_push(bool pred, const MlpQuery& q, MlpQuery* queue, uint32_t* count)
+{
+    uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    uint32_t lane = __lane_id();
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    if (pred) queue[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = q;
+}
+
 /* ---------------- ray generation ---------------- */
 
 __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
-                                                uint32_t n_slots)
+                                                uint32_t n_slots, CacheRecords rec)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) pool.counters[0] = n_slots;
@@ -137,6 +189,14 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, Ba
     pool.global[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pool.hash[i] = kFnvBasis;
     pool.queue[0][i] = i;
+    if (p.cache) {
+        uint32_t r = train_record(p, pseq, cs);
+        pool.pflags[i] = r != 0xffffffffu ? PF_TRAIN : 0u;
+        if (r != 0xffffffffu) {
+            rec.slot[r] = i;
+            rec.smp.valid[r] = 0.0f;
+        }
+    }
 }
 
 /* ---------------- closest-hit trace (persistent) ---------------- */
@@ -152,6 +212,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
     if (gtid == 0) {
         pool.counters[nxt] = 0;
         pool.counters[2] = 0;
+        pool.counters[3] = 0;
     }
     const uint32_t n = pool.counters[cur];
     const uint32_t* queue = pool.queue[cur];
@@ -219,13 +280,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
 
 template <bool VOL>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
-                                               uint32_t vertex, int cur, int nxt)
+                                               uint32_t vertex, int cur, int nxt, CacheRecords rec)
 {
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = pool.counters[cur];
     bool active = qi < n;
     bool extend = false;
     bool shadow = false;
+    bool cache_query = false;
+    MlpQuery qrec;
     uint32_t slot = active ? pool.queue[cur][qi] : 0u;
     if (active) {
         const SceneView& sc = p.sc;
@@ -298,9 +361,44 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 }
             } else {
                 sd.wo = neg(dir);
+                /* radiance cache: the vertex after a path's first rough one
+                 * is its query vertex; a training path records it instead
+                 * and goes on */
+                uint32_t pf = 0;
+                if (p.cache) {
+                    pf = pool.pflags[slot];
+                    if ((pf & PF_ROUGH) && vertex > 0) {
+                        float in[kMlpIn];
+                        cache_inputs(p, pos, dir, sd, in);
+                        if (!(pf & PF_TRAIN)) {
+                            cache_query = true;
+#pragma unroll
+                            for (int k = 0; k < kMlpIn; ++k) qrec.in[k] = in[k];
+                            qrec.thr[0] = T.x;
+                            qrec.thr[1] = T.y;
+                            qrec.thr[2] = T.z;
+                            qrec.slot = slot;
+                        } else {
+                            uint32_t r = train_record(p, pseq, cs);
+                            uint32_t st = rec.smp.stride;
+#pragma unroll
+                            for (int k = 0; k < kMlpIn; ++k) rec.smp.in[(size_t)k * st + r] = in[k];
+                            rec.thr[r] = T.x;
+                            rec.thr[st + r] = T.y;
+                            rec.thr[2 * st + r] = T.z;
+                            rec.before[r] = global.x;
+                            rec.before[st + r] = global.y;
+                            rec.before[2 * st + r] = global.z;
+                            rec.smp.valid[r] = 1.0f;
+                            pf = (pf & ~PF_ROUGH) | PF_RECORDED;
+                            pool.pflags[slot] = pf;
+                        }
+                    }
+                }
                 uint32_t ls = vertex == 0 ? p.n_light : 1u;
                 uint32_t ss = vertex == 0 ? p.n_gi : 1u;
                 bool apply_balance = vertex == 0 ? (p.max_depth != 0 && !volume) : (vertex != p.max_depth);
+                if (!cache_query) {
                 /* next-event estimation: the camera vertex's is evaluated once, by split 0 */
                 if (vertex != 0 || split == 0) {
                     uint32_t ne = 0;
@@ -375,7 +473,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                         pool.ray_d[slot] = st4(sd.wi, 0.0f);
                         pool.thr[slot] = st4(T, nb);
                         extend = true;
+                        if (p.cache && !(pf & PF_RECORDED) && cache_roughness(sd.mat) >= p.rough_alpha)
+                            pool.pflags[slot] = pf | PF_ROUGH;
                     }
+                }
                 }
             }
         }
@@ -385,6 +486,34 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
     }
     queue_push(extend, slot, pool.queue[nxt], &pool.counters[nxt]);
     queue_push(shadow, slot, pool.shadow_queue, &pool.counters[2]);
+    if (p.cache) query_push(cache_query, qrec, rec.queries, &pool.counters[3]);
+}
+
+/* ---------------- radiance cache: training targets ---------------- */
+
+/* Target of a training record: the radiance its path gathered after the
+ * query vertex, divided by the throughput there (the quantity a query
+ * replaces).  Records whose path never reached a query vertex, or whose
+ * throughput is zero in a channel, get weight 0. */
+__global__ __launch_bounds__(256) void k_cache_targets(PathPool pool, CacheRecords rec, uint32_t r0, uint32_t r1)
+{
+    uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= r1) return;
+    uint32_t st = rec.smp.stride;
+    if (rec.smp.valid[r] == 0.0f) {
+        rec.smp.target[r] = rec.smp.target[st + r] = rec.smp.target[2 * st + r] = 0.0f;
+        return;
+    }
+    float4 g = pool.global[rec.slot[r]];
+    float gc[3] = {g.x, g.y, g.z};
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float t = rec.thr[(size_t)c * st + r];
+        ok = ok && t > 1e-8f;
+        rec.smp.target[(size_t)c * st + r] = ok ? (gc[c] - rec.before[(size_t)c * st + r]) / t : 0.0f;
+    }
+    if (!ok) rec.smp.valid[r] = 0.0f;
 }
 
 /* ---------------- shadow rays (persistent) ---------------- */
@@ -596,9 +725,9 @@ void launch_k(K kernel, uint32_t grid, uint32_t block, hipStream_t s, const Laun
 }
 
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                   uint32_t n_slots, const LaunchEvents* ev)
+                   uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev)
 {
-    launch_k(k_raygen, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_slots);
+    launch_k(k_raygen, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_slots, rec);
 }
 
 void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
@@ -616,12 +745,26 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 }
 
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                  uint32_t vertex, int cur, int nxt, uint32_t n_slots, const LaunchEvents* ev)
+                  uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev)
 {
     if (p.sc.n_volumes > 0)
-        launch_k(k_shade<true>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt);
+        launch_k(k_shade<true>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
     else
-        launch_k(k_shade<false>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt);
+        launch_k(k_shade<false>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+}
+
+void launch_cache_infer(hipStream_t s, const MlpWeights& w, const PathPool& pool, const CacheRecords& rec,
+                        uint32_t grid, const LaunchEvents* ev)
+{
+    (void)ev;
+    launch_mlp_infer_counted(s, w, rec.queries, &pool.counters[3], pool.global, grid);
+}
+
+void launch_cache_targets(hipStream_t s, const PathPool& pool, const CacheRecords& rec, uint32_t r0, uint32_t r1,
+                          const LaunchEvents* ev)
+{
+    if (r1 <= r0) return;
+    launch_k(k_cache_targets, (r1 - r0 + 255) / 256, 256, s, ev, pool, rec, r0, r1);
 }
 
 void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,

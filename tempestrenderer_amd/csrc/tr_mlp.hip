@@ -83,9 +83,11 @@ constexpr int kBlockTile = kWaves * kTile;
 
 /* ---------------- inference ---------------- */
 
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery* q, uint32_t n, float* out,
-                                                   float4* radiance)
+__global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery* q, uint32_t n_arg,
+                                                   const uint32_t* count, float* out, float4* radiance)
 {
+    const uint32_t n = count ? *count : n_arg;
+    if (n == 0) return;
     __shared__ __attribute__((aligned(16))) uint16_t sw[kPackTotal];
     __shared__ float sb[kBiasTotal];
     __shared__ float qin[kWaves][kTile][kMlpIn];
@@ -417,7 +419,14 @@ void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQuery* q, uin
 {
     if (n == 0) return;
     uint32_t need = (n + kBlockTile - 1) / kBlockTile;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(need < grid ? need : grid), dim3(256), 0, s, w, q, n, out, radiance);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(need < grid ? need : grid), dim3(256), 0, s, w, q, n, (const uint32_t*)nullptr,
+                       out, radiance);
+}
+
+void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* count,
+                              float4* radiance, uint32_t grid)
+{
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, s, w, q, 0u, count, (float*)nullptr, radiance);
 }
 
 void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,

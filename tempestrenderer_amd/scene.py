@@ -275,6 +275,28 @@ class RayTracerScene:
                                              _fptr(rgb), hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
         return rgb.reshape(n, 3), hsh
 
+    # radiance cache
+    def setRadianceCache(self, enabled=True, train=True, train_paths=65536, batch=16384, rough_alpha=0.2, seed=1):
+        c = _abi.tr_cache_settings(int(bool(enabled)), int(bool(train)), int(train_paths), int(batch),
+                                   float(rough_alpha), int(seed))
+        check(self.lib.tr_scene_set_radiance_cache(self.h, ctypes.byref(c)))
+
+    def radianceCache(self):
+        """The scene's network as a RadianceCacheMLP (owned by the scene)."""
+        h = ctypes.c_void_p()
+        check(self.lib.tr_scene_get_radiance_cache(self.h, ctypes.byref(h)))
+        net = RadianceCacheMLP.__new__(RadianceCacheMLP)
+        net.lib, net.h, net._borrowed = self.lib, h, True
+        return net
+
+    def cacheRecords(self, count):
+        """Training records of the last frame: inputs [n, 10], targets [n, 3], weights [n]."""
+        x = np.zeros((count, 10), dtype=np.float32)
+        t = np.zeros((count, 3), dtype=np.float32)
+        v = np.zeros(count, dtype=np.float32)
+        check(self.lib.tr_scene_read_cache_records(self.h, count, _fptr(x), _fptr(t), _fptr(v)))
+        return x, t, v
+
     def stats(self):
         st = _abi.tr_stats()
         check(self.lib.tr_scene_get_stats(self.h, ctypes.byref(st)))
@@ -296,9 +318,9 @@ class RadianceCacheMLP:
         check(self.lib.tr_mlp_create(int(seed), ctypes.byref(self.h)))
 
     def close(self):
-        if self.h:
+        if self.h and not getattr(self, "_borrowed", False):
             self.lib.tr_mlp_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:

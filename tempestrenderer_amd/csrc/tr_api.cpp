@@ -107,7 +107,7 @@ struct tr_scene {
     uint32_t capacity = 0;
     uint32_t max_shadow = 0;
     DevBuf<float4> ray_o, ray_d, hit, thr, local, global, sh_o, sh_d, sh_c;
-    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, sh_n, counters;
+    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, qq, seg, seg_n, sh_n, counters;
     DevBuf<float> sh_tmax;
     DevBuf<uint2> spill;
     DevBuf<uint64_t> counts;
@@ -161,7 +161,7 @@ tr_scene::~tr_scene()
     d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); sh_tmax.release();
     ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
     sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
-    shq.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release();
+    shq.release(); qq.release(); seg.release(); seg_n.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release();
     if (net) tr_mlp_destroy(net);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -267,6 +267,10 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->q0.reserve(cap));
     TR_HIP(s->q1.reserve(cap));
     TR_HIP(s->shq.reserve(cap));
+    TR_HIP(s->qq.reserve(cap));
+    uint32_t seg_cap = (cap + kSegment - 1) / kSegment;
+    TR_HIP(s->seg.reserve((size_t)3 * seg_cap * kSegment));
+    TR_HIP(s->seg_n.reserve((size_t)3 * seg_cap));
     TR_HIP(s->sh_n.reserve(cap));
     TR_HIP(s->sh_tmax.reserve(cap));
     TR_HIP(s->pflags.reserve(cap));
@@ -290,6 +294,8 @@ PathPool path_pool(const tr_scene* s)
     p.queue[0] = s->q0.ptr; p.queue[1] = s->q1.ptr; p.shadow_queue = s->shq.ptr;
     p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr; p.sh_tmax = s->sh_tmax.ptr;
     p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity; p.pflags = s->pflags.ptr;
+    p.query_queue = s->qq.ptr; p.seg = s->seg.ptr; p.seg_n = s->seg_n.ptr;
+    p.seg_cap = (s->capacity + kSegment - 1) / kSegment;
     return p;
 }
 
@@ -441,6 +447,10 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
                 return st2;
             if ((st2 = stage(EV_SHADE, [&](const LaunchEvents* ev) {
                      launch_shade(s->stream, p, pool, out, slot_base, v, cur, nxt, nslots, rec, ev);
+                 })) != TR_OK)
+                return st2;
+            if ((st2 = stage(EV_SHADE, [&](const LaunchEvents* ev) {
+                     launch_compact(s->stream, pool, nslots, nxt, s->cache.enabled, ev);
                  })) != TR_OK)
                 return st2;
             if (s->cache.enabled && v > 0) {

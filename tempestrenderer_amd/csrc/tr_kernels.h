@@ -52,7 +52,7 @@ struct CacheRecords {
     float* thr;           /* [3][stride] */
     float* before;        /* [3][stride] */
     uint32_t* slot;       /* [stride]: batch-local slot, or ~0 if none */
-    MlpQuery* queries;    /* query queue of the current vertex */
+    MlpQuery* queries;    /* query of each path slot (valid for the slots of the query queue) */
 };
 
 /* Structure-of-arrays path state; every array has `capacity` entries
@@ -73,7 +73,16 @@ struct PathPool {
     float4* sh_c;       /* contribution, .w = light index (bits) */
     uint32_t* sh_n;     /* entries per slot */
     float* sh_tmax;     /* largest path-throughput component at the shading point */
+    uint32_t* query_queue; /* cache mode: slots that query the network */
     uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size, [3] cache queries */
+    /* Segment-local queues written by the shading kernel: block b appends
+     * queue k's entries of its threads, in thread order, at
+     * seg[k * seg_cap * kSegment + b * kSegment ...] and their number at
+     * seg_n[k * seg_cap + b]; k_compact concatenates the segments into the
+     * dense queues (k = 0 next path queue, 1 shadow queue, 2 query queue). */
+    uint32_t* seg;
+    uint32_t* seg_n;
+    uint32_t seg_cap;   /* segments per queue */
     uint32_t* pflags;   /* cache mode path flags */
     uint2* spill;       /* traversal stack spill, per persistent thread */
     uint32_t capacity;
@@ -96,6 +105,7 @@ struct LaunchEvents {
 };
 
 constexpr int kTraceBlock = 64;       /* one wave per block */
+constexpr uint32_t kSegment = 256;    /* shading block = queue segment */
 constexpr int kStackLds = 16;         /* stack entries per lane kept in LDS */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
 
@@ -106,6 +116,11 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                   uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec,
                   const LaunchEvents* ev = nullptr);
+/* Concatenates the segment-local queues of the last shade into the next
+ * path queue, the shadow queue and (cache mode) the query queue, in
+ * segment order, and writes their sizes to the counters. */
+void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int nxt, bool queries,
+                    const LaunchEvents* ev = nullptr);
 /* Cache mode: network predictions for the queries of the last shade, added
  * to their paths' radiance. */
 void launch_cache_infer(hipStream_t s, const MlpWeights& w, const PathPool& pool, const CacheRecords& rec,

@@ -64,17 +64,38 @@ struct DevStack {
     __device__ bool empty() const { return sp == 0; }
 };
 
-/* Wave-aggregated append to a queue. */
-__device__ inline void queue_push(bool pred, uint32_t value, uint32_t* queue, uint32_t* count)
+/* Segment-local append of one entry to each of the three shading queues
+ * (next path, shadow, cache query) for the threads whose predicate holds:
+ * the block's entries go to its own segment in thread order and the
+ * block's first threads write the three counts.  Every thread of the
+ * block must call it.  No atomics: one counter bumped by every wave of a
+ * launch serialises (tens of thousands of same-address atomics per
+ * launch), while k_compact concatenates the segments in a few
+ * microseconds. */
+__device__ inline void segment_push(const PathPool& pool, bool e0, bool e1, bool e2, uint32_t value)
 {
-    uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    uint32_t lane = __lane_id();
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    if (pred) queue[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = value;
+    __shared__ uint32_t wc[3][kSegment / 64];
+    const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+    const uint64_t m[3] = {__ballot(e0), __ballot(e1), __ballot(e2)};
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) wc[k][w] = (uint32_t)__popcll(m[k]);
+    }
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const bool e[3] = {e0, e1, e2};
+    const size_t seg_stride = (size_t)pool.seg_cap * kSegment;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint32_t o = 0;
+        for (uint32_t j = 0; j < w; ++j) o += wc[k][j];
+        if (e[k]) pool.seg[k * seg_stride + (size_t)blockIdx.x * kSegment + o + (uint32_t)__popcll(m[k] & lt)] = value;
+    }
+    if (threadIdx.x < 3) {
+        uint32_t t = 0;
+        for (uint32_t j = 0; j < kSegment / 64; ++j) t += wc[threadIdx.x][j];
+        pool.seg_n[threadIdx.x * pool.seg_cap + blockIdx.x] = t;
+    }
 }
 
 __device__ inline void slot_coords(const FrameParams& p, const BatchOut& out, uint64_t slot, uint32_t* pixel,
@@ -128,19 +149,6 @@ __device__ inline uint32_t train_record(const FrameParams& p, uint64_t pseq, uin
     uint64_t path = pseq * p.spp + (cs - p.frame * p.spp) + p.frame;
     if (path % p.train_stride != 0) return 0xffffffffu;
     return (uint32_t)(path / p.train_stride);
-}
-
-/* Wave-aggregated append of a query record. */
-__device__ inline void query_push(bool pred, const MlpQuery& q, MlpQuery* queue, uint32_t* count)
-{
-    uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    uint32_t lane = __lane_id();
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    if (pred) queue[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = q;
 }
 
 /* ---------------- ray generation ---------------- */
@@ -209,11 +217,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
     const SceneView& sc = p.sc;
     uint32_t lane = threadIdx.x;
     uint32_t gtid = blockIdx.x * kTraceBlock + lane;
-    if (gtid == 0) {
-        pool.counters[nxt] = 0;
-        pool.counters[2] = 0;
-        pool.counters[3] = 0;
-    }
     const uint32_t n = pool.counters[cur];
     const uint32_t* queue = pool.queue[cur];
     DevStack st;
@@ -288,7 +291,6 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
     bool extend = false;
     bool shadow = false;
     bool cache_query = false;
-    MlpQuery qrec;
     uint32_t slot = active ? pool.queue[cur][qi] : 0u;
     if (active) {
         const SceneView& sc = p.sc;
@@ -372,12 +374,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                         cache_inputs(p, pos, dir, sd, in);
                         if (!(pf & PF_TRAIN)) {
                             cache_query = true;
+                            MlpQuery qrec;
 #pragma unroll
                             for (int k = 0; k < kMlpIn; ++k) qrec.in[k] = in[k];
                             qrec.thr[0] = T.x;
                             qrec.thr[1] = T.y;
                             qrec.thr[2] = T.z;
                             qrec.slot = slot;
+                            rec.queries[slot] = qrec;
                         } else {
                             uint32_t r = train_record(p, pseq, cs);
                             uint32_t st = rec.smp.stride;
@@ -484,9 +488,70 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         pool.global[slot] = st4(global, 0.0f);
         if (write_local) pool.local[slot] = st4(local, 0.0f);
     }
-    queue_push(extend, slot, pool.queue[nxt], &pool.counters[nxt]);
-    queue_push(shadow, slot, pool.shadow_queue, &pool.counters[2]);
-    if (p.cache) query_push(cache_query, qrec, rec.queries, &pool.counters[3]);
+    segment_push(pool, extend, shadow, cache_query, slot);
+}
+
+/* ---------------- queue compaction ---------------- */
+
+/* Exclusive prefix sum over the block (blockDim.x == kSegment) of one value
+ * per thread; *total receives the block's sum. */
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
+{
+    __shared__ uint32_t ws[kSegment / 64];
+    const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kSegment / 64; ++j) {
+        if (j < w) before += ws[j];
+        sum += ws[j];
+    }
+    __syncthreads();
+    *total = sum;
+    return before + x - v;
+}
+
+/* Block (x, k) copies queue k's segments [x * group, (x + 1) * group) to
+ * the dense queue at the offset given by the counts of all segments before
+ * them (summed by the block itself: the counts are a few KB and L2
+ * resident); the last block writes the queue size. */
+__global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_seg, uint32_t group, int nxt)
+{
+    const uint32_t k = blockIdx.y;
+    const uint32_t* cnt = pool.seg_n + (size_t)k * pool.seg_cap;
+    const uint32_t* src = pool.seg + (size_t)k * pool.seg_cap * kSegment;
+    uint32_t* dst = k == 0 ? pool.queue[nxt] : (k == 1 ? pool.shadow_queue : pool.query_queue);
+    const uint32_t s0 = blockIdx.x * group;
+    const uint32_t s1 = min(s0 + group, n_seg);
+    uint32_t part = 0;
+    for (uint32_t j = threadIdx.x; j < s0; j += kSegment) part += cnt[j];
+    uint32_t base;
+    (void)block_exclusive_scan(part, &base);
+    for (uint32_t c0 = s0; c0 < s1; c0 += kSegment) {
+        __shared__ uint32_t sc[kSegment], so[kSegment];
+        uint32_t c = c0 + threadIdx.x < s1 ? cnt[c0 + threadIdx.x] : 0u;
+        uint32_t sum;
+        uint32_t o = block_exclusive_scan(c, &sum);
+        sc[threadIdx.x] = c;
+        so[threadIdx.x] = o;
+        __syncthreads();
+        const uint32_t nb = min(s1 - c0, kSegment);
+        for (uint32_t j = 0; j < nb; ++j)
+            if (threadIdx.x < sc[j]) dst[base + so[j] + threadIdx.x] = src[(size_t)(c0 + j) * kSegment + threadIdx.x];
+        __syncthreads();
+        base += sum;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t slot = k == 0 ? (uint32_t)nxt : k + 1;
+        pool.counters[slot] = base;
+    }
 }
 
 /* ---------------- radiance cache: training targets ---------------- */
@@ -753,11 +818,25 @@ void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, con
         launch_k(k_shade<false>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
 }
 
+void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int nxt, bool queries,
+                    const LaunchEvents* ev)
+{
+    uint32_t n_seg = (n_slots + kSegment - 1) / kSegment;
+    /* about 256 blocks per queue: the per-block prefix reads stay a small
+     * multiple of the count array */
+    uint32_t group = (n_seg + 255) / 256;
+    if (group < 16) group = 16;
+    uint32_t blocks = (n_seg + group - 1) / group;
+    dim3 grid(blocks, queries ? 3 : 2);
+    if (ev) hipExtLaunchKernelGGL(k_compact, grid, dim3(kSegment), 0, s, ev->start, ev->stop, 0u, pool, n_seg, group, nxt);
+    else k_compact<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, nxt);
+}
+
 void launch_cache_infer(hipStream_t s, const MlpWeights& w, const PathPool& pool, const CacheRecords& rec,
                         uint32_t grid, const LaunchEvents* ev)
 {
     (void)ev;
-    launch_mlp_infer_counted(s, w, rec.queries, &pool.counters[3], pool.global, grid);
+    launch_mlp_infer_counted(s, w, rec.queries, pool.query_queue, &pool.counters[3], pool.global, grid);
 }
 
 void launch_cache_targets(hipStream_t s, const PathPool& pool, const CacheRecords& rec, uint32_t r0, uint32_t r1,

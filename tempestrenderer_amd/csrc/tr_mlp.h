@@ -95,9 +95,10 @@ struct MlpTrainBufs {
 
 void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQuery* q, uint32_t n, float* out,
                       float4* radiance, uint32_t grid);
-/* The same with the query count read on the device. */
-void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* count,
-                              float4* radiance, uint32_t grid);
+/* The same for the queries q[index[i]], i < *count (count read on the
+ * device). */
+void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* index,
+                              const uint32_t* count, float4* radiance, uint32_t grid);
 void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,
                       float inv_valid, const MlpTrainBufs& tb, uint32_t grid);
 void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb, uint32_t slices);

@@ -83,8 +83,9 @@ constexpr int kBlockTile = kWaves * kTile;
 
 /* ---------------- inference ---------------- */
 
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery* q, uint32_t n_arg,
-                                                   const uint32_t* count, float* out, float4* radiance)
+__global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery* q, const uint32_t* index,
+                                                   uint32_t n_arg, const uint32_t* count, float* out,
+                                                   float4* radiance)
 {
     const uint32_t n = count ? *count : n_arg;
     if (n == 0) return;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery*
         bool valid = qi < n;
         if (g == 0) {
 #pragma unroll
-            for (int i = 0; i < kMlpIn; ++i) qin[wave][col][i] = valid ? q[qi].in[i] : 0.0f;
+            for (int i = 0; i < kMlpIn; ++i) qin[wave][col][i] = valid ? q[index ? index[qi] : qi].in[i] : 0.0f;
         }
         __syncthreads();
         /* layer 0: 4 neuron tiles x 8 feature steps */
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery*
                 out[qi * 3 + 2] = y2;
             }
             if (radiance) {
-                const MlpQuery& r = q[qi];
+                const MlpQuery& r = q[index ? index[qi] : qi];
                 float4 cur = radiance[r.slot];
                 cur.x = cur.x + r.thr[0] * fmaxf(y0, 0.0f);
                 cur.y = cur.y + r.thr[1] * fmaxf(y1, 0.0f);
@@ -419,14 +420,14 @@ void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQuery* q, uin
 {
     if (n == 0) return;
     uint32_t need = (n + kBlockTile - 1) / kBlockTile;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(need < grid ? need : grid), dim3(256), 0, s, w, q, n, (const uint32_t*)nullptr,
-                       out, radiance);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(need < grid ? need : grid), dim3(256), 0, s, w, q, (const uint32_t*)nullptr,
+                       n, (const uint32_t*)nullptr, out, radiance);
 }
 
-void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* count,
-                              float4* radiance, uint32_t grid)
+void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* index,
+                              const uint32_t* count, float4* radiance, uint32_t grid)
 {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, s, w, q, 0u, count, (float*)nullptr, radiance);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, s, w, q, index, 0u, count, (float*)nullptr, radiance);
 }
 
 void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,

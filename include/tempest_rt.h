@@ -338,6 +338,10 @@ typedef struct tr_cache_settings {
     uint32_t train_paths;
     uint32_t batch;
     float rough_alpha;
+    /* 1: the query vertex keeps its next-event estimate and the network
+     * predicts the radiance beyond it; 0: the network predicts the whole
+     * radiance leaving the query vertex (no shadow rays there) */
+    int32_t query_nee;
     uint64_t seed;           /* network initialisation */
 } tr_cache_settings;
 

@@ -67,7 +67,7 @@ def _declare(lib):
         "orc_add_sphere_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
         "orc_add_grid_volume": (ctypes.c_int, [vp, P(_abi.tr_grid_volume), P(_abi.tr_material), P(u64)]),
         "orc_commit": (ctypes.c_int, [vp]),
-        "orc_set_cache": (None, [vp, ctypes.c_int, f, u32, u32, P(f), u32]),
+        "orc_set_cache": (None, [vp, ctypes.c_int, f, u32, u32, P(f), u32, ctypes.c_int]),
         "orc_cache_records": (None, [vp, sz, P(f), P(f), P(f)]),
         "orc_grid_volume_lookup": (None, [vp, P(f), P(f), f, P(f)]),
         "orc_trace_segments": (None, [vp, sz, P(_abi.tr_ray), u32, P(_abi.tr_hit), ctypes.c_int]),
@@ -218,11 +218,11 @@ class OracleScene:
     def commitScene(self):
         self.lib.orc_commit(self.h)
 
-    def setCache(self, enabled, rough_alpha, train_stride, frame, value, records):
+    def setCache(self, enabled, rough_alpha, train_stride, frame, value, records, query_nee=False):
         """Cache mode with a constant network output `value` (RGB)."""
         v = np.asarray(value, dtype=np.float32)
         self.lib.orc_set_cache(self.h, int(bool(enabled)), float(rough_alpha), int(train_stride), int(frame), _fp(v),
-                               int(records))
+                               int(records), int(bool(query_nee)))
 
     def cacheRecords(self, count):
         x = np.zeros((count, 10), dtype=np.float32)

@@ -20,7 +20,11 @@ counterpart in the reference; it is specified by the build:
 `bf16=True` rounds what the GPU rounds to bfloat16 (round to nearest even):
 the weights, the encoded features, every hidden activation and the
 gradients that enter a matrix product; sums stay in FP64 (the GPU sums in
-FP32 inside the matrix cores).
+FP32 inside the matrix cores).  Its features follow the GPU's recipe
+(encode_gpu): per scalar and group of four bands k0..k0+3, band k0 from
+sin/cos and the next three by FP32 angle doubling,
+sin 2a = (2 sin a) cos a, cos 2a = 1 - (2 sin a) sin a, each operation
+rounded to binary32 as the GPU rounds it (no fused multiply-add).
 """
 import numpy as np
 
@@ -74,10 +78,34 @@ def encode(x):
     return f.reshape(x.shape[0], FEAT)
 
 
+def encode_gpu(x):
+    """[N, 10] inputs -> [N, 240] features by the GPU's FP32 recipe: the
+    first band of each group of four from sin/cos of the binary32 angle
+    argument 2^(k0-1) x turns (rounded to binary32), the next three by
+    binary32 angle doubling."""
+    x = np.asarray(x, dtype=np.float32)
+    n = x.shape[0]
+    f = np.zeros((n, N_IN, BANDS, 2), dtype=np.float64)
+    for k0 in range(0, BANDS, 4):
+        t = (x * np.float32(2.0 ** (k0 - 1))).astype(np.float32)   # exact scaling
+        ang = 2.0 * np.pi * t.astype(np.float64)
+        s = np.sin(ang).astype(np.float32)
+        c = np.cos(ang).astype(np.float32)
+        for b in range(4):
+            if b:
+                tt = np.float32(2.0) * s
+                s, c = tt * c, np.float32(1.0) - tt * s
+            f[:, :, k0 + b, 0] = s
+            f[:, :, k0 + b, 1] = c
+    return f.reshape(n, FEAT)
+
+
 def forward(theta, x, bf16_mode=False, features=None):
     """Returns (y [N, 3], activations [X, H1..H4])."""
     rnd = bf16 if bf16_mode else (lambda a: a)
-    h = rnd(encode(x) if features is None else features)
+    if features is None:
+        features = encode_gpu(x) if bf16_mode else encode(x)
+    h = rnd(features)
     acts = [h]
     layers = unpack(theta)
     for li, (W, b) in enumerate(layers):

@@ -103,6 +103,7 @@ struct orc_scene {
         uint32_t train_stride = 1;
         uint32_t frame = 0;
         float value[3] = {0, 0, 0};
+        bool query_nee = false;
         float box_lo[3] = {0, 0, 0}, box_rcp[3] = {1, 1, 1};
         std::vector<float> in, target, valid;   /* [r][10], [r][3], [r] */
     } cache;
@@ -894,23 +895,31 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
                 break;
             }
             cur.wo = -cur.wi;
-            if (cache && rough) {
-                /* the query vertex: a constant prediction ends the path; a
-                 * training path records it and goes on */
+            ss = 1;
+            ls = 1;
+            bool capture = false;
+            if (cache && rough && d + 1 < s->max_depth) {
+                /* the query vertex: the constant prediction (of the radiance
+                 * leaving the vertex, or with query_nee of the radiance
+                 * beyond its next-event estimate, which then follows; the
+                 * GPU adds them in this order) ends the path; a training
+                 * path records the vertex and goes on */
                 if (record == 0xffffffffu) {
                     gsplit += ct * v3((T)xmax(s->cache.value[0], 0.0f), (T)xmax(s->cache.value[1], 0.0f),
                                       (T)xmax(s->cache.value[2], 0.0f));
+                    if (s->cache.query_nee)
+                        gsplit += evaluate_lights(s, srng, d + 1, cur, ls, ss, cpos, ct, d + 1 != s->max_depth);
                     break;
                 }
                 cache_inputs(s, cpos, cur.wi, cur, rec_in);
                 rec_thr = ct;
-                rec_before = gsplit;
                 rough = false;
                 recorded = true;
+                if (s->cache.query_nee) capture = true;
+                else rec_before = gsplit;
             }
-            ss = 1;
-            ls = 1;
             gsplit += evaluate_lights(s, srng, d + 1, cur, ls, ss, cpos, ct, d + 1 != s->max_depth);
+            if (capture) rec_before = gsplit;
             ct *= (unbiased ? (T)1 : balance) / (T)s->rr;
             st.x = st.y = (T)0;
             st.tx = st.ty = (T)1;
@@ -995,9 +1004,10 @@ void orc_set_estimator(orc_scene* s, uint32_t mode) { s->estimator = mode; }
  * position input is the box of all primitives (set at commit).  Training
  * records for `records` training paths are kept for orc_cache_records. */
 void orc_set_cache(orc_scene* s, int enabled, float rough_alpha, uint32_t train_stride, uint32_t frame,
-                   const float value[3], uint32_t records)
+                   const float value[3], uint32_t records, int query_nee)
 {
     s->cache.enabled = enabled != 0;
+    s->cache.query_nee = query_nee != 0;
     s->cache.rough_alpha = rough_alpha;
     s->cache.train_stride = train_stride ? train_stride : 1;
     s->cache.frame = frame;

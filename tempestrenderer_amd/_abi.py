@@ -82,7 +82,8 @@ class tr_stats(ctypes.Structure):
 
 class tr_cache_settings(ctypes.Structure):
     _fields_ = [("enabled", ctypes.c_int32), ("train", ctypes.c_int32), ("train_paths", ctypes.c_uint32),
-                ("batch", ctypes.c_uint32), ("rough_alpha", ctypes.c_float), ("seed", ctypes.c_uint64)]
+                ("batch", ctypes.c_uint32), ("rough_alpha", ctypes.c_float), ("query_nee", ctypes.c_int32),
+                ("seed", ctypes.c_uint64)]
 
 
 class tr_grid_volume(ctypes.Structure):

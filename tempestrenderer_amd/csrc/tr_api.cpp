@@ -67,7 +67,8 @@ struct tr_scene {
     uint32_t rec_cap = 0;
     DevBuf<float> rec_thr, rec_before;
     DevBuf<uint32_t> rec_slot, pflags;
-    DevBuf<MlpQuery> queries;
+    DevBuf<float> q_in, q_thr;   /* cache queries, [kMlpIn | 3][seg_cap * kSegment] */
+    DevBuf<uint32_t> q_slot;
 
     /* host copy of the scene */
     std::vector<tr_material> mats;
@@ -274,7 +275,6 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->sh_n.reserve(cap));
     TR_HIP(s->sh_tmax.reserve(cap));
     TR_HIP(s->pflags.reserve(cap));
-    if (s->cache.enabled) TR_HIP(s->queries.reserve(cap));
     TR_HIP(s->sh_o.reserve((size_t)cap * msh));
     TR_HIP(s->sh_d.reserve((size_t)cap * msh));
     TR_HIP(s->sh_c.reserve((size_t)cap * msh));
@@ -321,6 +321,7 @@ FrameParams frame_params(const tr_scene* s)
     p.cache = s->cache.enabled ? 1u : 0u;
     p.train_stride = 1;
     p.rough_alpha = s->cache.rough_alpha;
+    p.cache_nee = s->cache.query_nee ? 1u : 0u;
     for (int a = 0; a < 3; ++a) {
         p.box_lo[a] = s->box_lo[a];
         float ext = s->box_hi[a] - s->box_lo[a];
@@ -358,7 +359,12 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
     uint64_t per_pixel = (uint64_t)s->spp * s->n_gi;
     st = ensure_pool(s, n_pixels * per_pixel);
     if (st != TR_OK) return st;
-    if (s->cache.enabled && s->queries.count < s->capacity) TR_HIP(s->queries.reserve(s->capacity));
+    const size_t q_stride = (size_t)(s->capacity + kSegment - 1) / kSegment * kSegment;
+    if (s->cache.enabled && s->q_slot.count < q_stride) {
+        TR_HIP(s->q_in.reserve(q_stride * kMlpIn));
+        TR_HIP(s->q_thr.reserve(q_stride * 3));
+        TR_HIP(s->q_slot.reserve(q_stride));
+    }
     FrameParams p = frame_params(s);
     PathPool pool = path_pool(s);
     /* cache mode: every train_stride-th path of the frame trains */
@@ -382,7 +388,10 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         rec.thr = s->rec_thr.ptr;
         rec.before = s->rec_before.ptr;
         rec.slot = s->rec_slot.ptr;
-        rec.queries = s->queries.ptr;
+        rec.queries.in = s->q_in.ptr;
+        rec.queries.thr = s->q_thr.ptr;
+        rec.queries.slot = s->q_slot.ptr;
+        rec.queries.stride = (uint32_t)q_stride;
         /* records beyond the frame's paths stay at weight 0 */
         TR_HIP(hipMemsetAsync(ms.s_valid.ptr, 0, (size_t)ms.train_cap * sizeof(float), s->stream));
     }
@@ -456,7 +465,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
             if (s->cache.enabled && v > 0) {
                 MlpWeights w = s->net->st.weights();
                 if ((st2 = stage_multi(EV_CACHE, [&]() {
-                         launch_cache_infer(s->stream, w, pool, rec, (uint32_t)std::max(1, s->cu_count) * 2);
+                         launch_cache_infer(s->stream, w, pool, rec, (uint32_t)std::max(1, s->cu_count));
                      })) != TR_OK)
                     return st2;
             }
@@ -490,8 +499,11 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         /* records below ceil(frame / stride) belong to no path of the frame */
         uint32_t first = (uint32_t)((p.frame + p.train_stride - 1) / p.train_stride) / MlpState::kSlice * MlpState::kSlice;
         s->cache_steps = 0;
-        for (uint32_t off = first; off < n_records; off += batch) {
-            uint32_t nb = std::min(batch, n_records - off);
+        for (uint32_t off = first, nb = 0; off < n_records; off += nb) {
+            nb = std::min(batch, n_records - off);
+            /* a remainder below a quarter batch joins the step before it
+             * instead of paying for a step of its own */
+            if (n_records - off - nb > 0 && n_records - off - nb < batch / 4) nb = n_records - off;
             tr_status st3 = TR_OK;
             uint32_t slot = s->cache_steps;
             tr_status st2 = stage_multi(EV_CACHE, [&]() { st3 = ms.enqueue_step(nb, nb, s->stream, true, off, slot); });

@@ -218,17 +218,19 @@ tr_status tr_mlp_infer(tr_mlp* m, size_t n, const float* inputs, float* outputs)
     if (!m || (n && (!inputs || !outputs))) return fail(TR_ERROR_INVALID_ARG, "null argument");
     if (n == 0) return TR_OK;
     if (n > (1u << 26)) return fail(TR_ERROR_INVALID_ARG, "too many queries");
-    std::vector<MlpQuery> q(n);
-    for (size_t i = 0; i < n; ++i) {
-        memcpy(q[i].in, inputs + i * kMlpIn, sizeof(q[i].in));
-        q[i].thr[0] = q[i].thr[1] = q[i].thr[2] = 0.0f;
-        q[i].slot = 0xffffffffu;
-    }
-    TR_HIP(m->q.reserve(n));
+    std::vector<float> q(n * kMlpIn);
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < kMlpIn; ++k) q[(size_t)k * n + i] = inputs[i * kMlpIn + k];
+    TR_HIP(m->q.reserve(n * kMlpIn));
     TR_HIP(m->out.reserve(n * 3));
     hipStream_t s = m->st.stream;
-    TR_HIP(hipMemcpyAsync(m->q.ptr, q.data(), n * sizeof(MlpQuery), hipMemcpyHostToDevice, s));
-    launch_mlp_infer(s, m->st.weights(), m->q.ptr, (uint32_t)n, m->out.ptr, nullptr, (uint32_t)m->st.cu_count * 2);
+    TR_HIP(hipMemcpyAsync(m->q.ptr, q.data(), q.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    MlpQueries mq;
+    mq.in = m->q.ptr;
+    mq.thr = nullptr;
+    mq.slot = nullptr;
+    mq.stride = (uint32_t)n;
+    launch_mlp_infer(s, m->st.weights(), mq, (uint32_t)n, m->out.ptr, (uint32_t)m->st.cu_count);
     TR_HIP(hipGetLastError());
     TR_HIP(hipMemcpyAsync(outputs, m->out.ptr, n * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
     TR_HIP(hipStreamSynchronize(s));

@@ -46,7 +46,7 @@ struct MlpState {
 
 struct tr_mlp {
     trk::MlpState st;
-    trk::DevBuf<trk::MlpQuery> q;
+    trk::DevBuf<float> q;   /* query inputs, [kMlpIn][n] */
     trk::DevBuf<float> out;
 };
 

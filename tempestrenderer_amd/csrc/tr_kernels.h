@@ -37,22 +37,26 @@ struct FrameParams {
     uint32_t cache;          /* 1: paths end at the vertex after their first rough one */
     uint32_t train_stride;   /* path p of a frame trains iff (p + frame) % train_stride == 0 */
     float rough_alpha;       /* roughness threshold */
+    uint32_t cache_nee;      /* 1: next-event estimate at the query vertex, network predicts the rest */
     float box_lo[3];         /* scene box for the position input */
     float box_rcp[3];
 };
 
-/* Path flags of cache mode. */
-enum { PF_TRAIN = 1u, PF_ROUGH = 2u, PF_RECORDED = 4u };
+/* Path flags of cache mode: training path, next vertex is the query
+ * vertex, record written, record waits for its radiance-before value
+ * (captured when the path reaches the vertex after the query vertex). */
+enum { PF_TRAIN = 1u, PF_ROUGH = 2u, PF_RECORDED = 4u, PF_BEFORE = 8u };
 
 /* Training records of one frame, indexed by training path: the network
  * inputs at the path's query vertex, the throughput there, the path
- * radiance gathered before it and the path's slot. */
+ * radiance gathered up to and including that vertex's next-event
+ * estimate, and the path's slot. */
 struct CacheRecords {
     MlpSampleSoA smp;     /* inputs, targets, valid (capacity = smp.stride) */
     float* thr;           /* [3][stride] */
     float* before;        /* [3][stride] */
     uint32_t* slot;       /* [stride]: batch-local slot, or ~0 if none */
-    MlpQuery* queries;    /* query of each path slot (valid for the slots of the query queue) */
+    MlpQueries queries;   /* the shade's queries at their queue-segment positions (stride seg_cap * kSegment) */
 };
 
 /* Structure-of-arrays path state; every array has `capacity` entries
@@ -73,7 +77,7 @@ struct PathPool {
     float4* sh_c;       /* contribution, .w = light index (bits) */
     uint32_t* sh_n;     /* entries per slot */
     float* sh_tmax;     /* largest path-throughput component at the shading point */
-    uint32_t* query_queue; /* cache mode: slots that query the network */
+    uint32_t* query_queue; /* cache mode: positions of the queries in CacheRecords::queries */
     uint32_t* counters; /* [0],[1] queue sizes, [2] shadow queue size, [3] cache queries */
     /* Segment-local queues written by the shading kernel: block b appends
      * queue k's entries of its threads, in thread order, at

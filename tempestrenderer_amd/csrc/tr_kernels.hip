@@ -67,12 +67,14 @@ struct DevStack {
 /* Segment-local append of one entry to each of the three shading queues
  * (next path, shadow, cache query) for the threads whose predicate holds:
  * the block's entries go to its own segment in thread order and the
- * block's first threads write the three counts.  Every thread of the
- * block must call it.  No atomics: one counter bumped by every wave of a
- * launch serialises (tens of thousands of same-address atomics per
- * launch), while k_compact concatenates the segments in a few
- * microseconds. */
-__device__ inline void segment_push(const PathPool& pool, bool e0, bool e1, bool e2, uint32_t value)
+ * block's first threads write the three counts.  Queues 0 and 1 receive
+ * `value` (the path slot); queue 2 receives the entry's own position,
+ * which is returned: the query record is stored there, so the queries of
+ * a launch lie densely in segment order.  Every thread of the block must
+ * call it.  No atomics: one counter bumped by every wave of a launch
+ * serialises (tens of thousands of same-address atomics per launch),
+ * while k_compact concatenates the segments in a few microseconds. */
+__device__ inline uint32_t segment_push(const PathPool& pool, bool e0, bool e1, bool e2, uint32_t value)
 {
     __shared__ uint32_t wc[3][kSegment / 64];
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
@@ -85,17 +87,21 @@ __device__ inline void segment_push(const PathPool& pool, bool e0, bool e1, bool
     const uint64_t lt = (1ull << lane) - 1ull;
     const bool e[3] = {e0, e1, e2};
     const size_t seg_stride = (size_t)pool.seg_cap * kSegment;
+    uint32_t pos2 = 0xffffffffu;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         uint32_t o = 0;
         for (uint32_t j = 0; j < w; ++j) o += wc[k][j];
-        if (e[k]) pool.seg[k * seg_stride + (size_t)blockIdx.x * kSegment + o + (uint32_t)__popcll(m[k] & lt)] = value;
+        const uint32_t pos = blockIdx.x * kSegment + o + (uint32_t)__popcll(m[k] & lt);
+        if (k == 2) pos2 = pos;
+        if (e[k]) pool.seg[k * seg_stride + pos] = k == 2 ? pos : value;
     }
     if (threadIdx.x < 3) {
         uint32_t t = 0;
         for (uint32_t j = 0; j < kSegment / 64; ++j) t += wc[threadIdx.x][j];
         pool.seg_n[threadIdx.x * pool.seg_cap + blockIdx.x] = t;
     }
+    return pos2;
 }
 
 __device__ inline void slot_coords(const FrameParams& p, const BatchOut& out, uint64_t slot, uint32_t* pixel,
@@ -291,6 +297,8 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
     bool extend = false;
     bool shadow = false;
     bool cache_query = false;
+    float qin[kMlpIn];
+    V3 qthr = mk(0.0f, 0.0f, 0.0f);
     uint32_t slot = active ? pool.queue[cur][qi] : 0u;
     if (active) {
         const SceneView& sc = p.sc;
@@ -309,6 +317,22 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         V3 local = mk(0.0f, 0.0f, 0.0f);
         V3 global = ld4(pool.global[slot]);
         bool write_local = false;
+        uint32_t pf = 0;
+        if (p.cache) {
+            pf = pool.pflags[slot];
+            if (pf & PF_BEFORE) {
+                /* the vertex after a training path's query vertex: the
+                 * radiance so far includes the query vertex's next-event
+                 * estimate, the part the network does not predict */
+                uint32_t r = train_record(p, pseq, cs);
+                uint32_t st = rec.smp.stride;
+                rec.before[r] = global.x;
+                rec.before[st + r] = global.y;
+                rec.before[2 * st + r] = global.z;
+                pf &= ~PF_BEFORE;
+                pool.pflags[slot] = pf;
+            }
+        }
         if (geom == TR_INVALID_GEOMETRY) {
             hash = (hash ^ 0xffffffffu) * kFnvPrime;
             V3 env = ld3(p.background);
@@ -364,24 +388,20 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
             } else {
                 sd.wo = neg(dir);
                 /* radiance cache: the vertex after a path's first rough one
-                 * is its query vertex; a training path records it instead
+                 * (below the last bounce) is its query vertex, where the
+                 * network predicts the radiance leaving it (with cache_nee:
+                 * the radiance beyond the vertex's own next-event estimate)
+                 * and the path ends; a training path records it instead
                  * and goes on */
-                uint32_t pf = 0;
                 if (p.cache) {
-                    pf = pool.pflags[slot];
-                    if ((pf & PF_ROUGH) && vertex > 0) {
+                    if ((pf & PF_ROUGH) && vertex > 0 && vertex < p.max_depth) {
                         float in[kMlpIn];
                         cache_inputs(p, pos, dir, sd, in);
                         if (!(pf & PF_TRAIN)) {
                             cache_query = true;
-                            MlpQuery qrec;
 #pragma unroll
-                            for (int k = 0; k < kMlpIn; ++k) qrec.in[k] = in[k];
-                            qrec.thr[0] = T.x;
-                            qrec.thr[1] = T.y;
-                            qrec.thr[2] = T.z;
-                            qrec.slot = slot;
-                            rec.queries[slot] = qrec;
+                            for (int k = 0; k < kMlpIn; ++k) qin[k] = in[k];
+                            qthr = T;
                         } else {
                             uint32_t r = train_record(p, pseq, cs);
                             uint32_t st = rec.smp.stride;
@@ -390,11 +410,15 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                             rec.thr[r] = T.x;
                             rec.thr[st + r] = T.y;
                             rec.thr[2 * st + r] = T.z;
-                            rec.before[r] = global.x;
-                            rec.before[st + r] = global.y;
-                            rec.before[2 * st + r] = global.z;
                             rec.smp.valid[r] = 1.0f;
                             pf = (pf & ~PF_ROUGH) | PF_RECORDED;
+                            if (p.cache_nee) {
+                                pf |= PF_BEFORE;
+                            } else {
+                                rec.before[r] = global.x;
+                                rec.before[st + r] = global.y;
+                                rec.before[2 * st + r] = global.z;
+                            }
                             pool.pflags[slot] = pf;
                         }
                     }
@@ -402,9 +426,9 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 uint32_t ls = vertex == 0 ? p.n_light : 1u;
                 uint32_t ss = vertex == 0 ? p.n_gi : 1u;
                 bool apply_balance = vertex == 0 ? (p.max_depth != 0 && !volume) : (vertex != p.max_depth);
-                if (!cache_query) {
-                /* next-event estimation: the camera vertex's is evaluated once, by split 0 */
-                if (vertex != 0 || split == 0) {
+                /* next-event estimation: the camera vertex's is evaluated once, by split 0;
+                 * a query vertex has one only with cache_nee */
+                if ((vertex != 0 || split == 0) && (!cache_query || p.cache_nee)) {
                     uint32_t ne = 0;
                     for (uint32_t li = 0; li < sc.n_lights; ++li) {
                         const LightRec& L = sc.lights[li];
@@ -440,7 +464,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                     if (VOL) pool.sh_tmax[slot] = max_value(T);
                     shadow = ne > 0;
                 }
-                bool alive = true;
+                bool alive = !cache_query;
                 if (vertex > 0) {
                     T = muls(T, (unbiased ? 1.0f : bal) / p.rr);
                     if (p.rr < 1.0f) {
@@ -481,14 +505,22 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                             pool.pflags[slot] = pf | PF_ROUGH;
                     }
                 }
-                }
             }
         }
         pool.hash[slot] = hash;
         pool.global[slot] = st4(global, 0.0f);
         if (write_local) pool.local[slot] = st4(local, 0.0f);
     }
-    segment_push(pool, extend, shadow, cache_query, slot);
+    const uint32_t qpos = segment_push(pool, extend, shadow, cache_query, slot);
+    if (cache_query) {
+        const MlpQueries& q = rec.queries;
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) q.in[(size_t)k * q.stride + qpos] = qin[k];
+        q.thr[qpos] = qthr.x;
+        q.thr[q.stride + qpos] = qthr.y;
+        q.thr[2 * (size_t)q.stride + qpos] = qthr.z;
+        q.slot[qpos] = slot;
+    }
 }
 
 /* ---------------- queue compaction ---------------- */
@@ -556,9 +588,10 @@ __global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_
 
 /* ---------------- radiance cache: training targets ---------------- */
 
-/* Target of a training record: the radiance its path gathered after the
- * query vertex, divided by the throughput there (the quantity a query
- * replaces).  Records whose path never reached a query vertex, or whose
+/* Target of a training record: the radiance its path gathered beyond the
+ * query vertex's next-event estimate, divided by the throughput at the
+ * query vertex (the quantity a query supplies); 0 when the path ended
+ * there.  Records whose path never reached a query vertex, or whose
  * throughput is zero in a channel, get weight 0. */
 __global__ __launch_bounds__(256) void k_cache_targets(PathPool pool, CacheRecords rec, uint32_t r0, uint32_t r1)
 {
@@ -569,14 +602,18 @@ __global__ __launch_bounds__(256) void k_cache_targets(PathPool pool, CacheRecor
         rec.smp.target[r] = rec.smp.target[st + r] = rec.smp.target[2 * st + r] = 0.0f;
         return;
     }
-    float4 g = pool.global[rec.slot[r]];
+    uint32_t slot = rec.slot[r];
+    const bool ended = (pool.pflags[slot] & PF_BEFORE) != 0;
+    float4 g = pool.global[slot];
     float gc[3] = {g.x, g.y, g.z};
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         float t = rec.thr[(size_t)c * st + r];
         ok = ok && t > 1e-8f;
-        rec.smp.target[(size_t)c * st + r] = ok ? (gc[c] - rec.before[(size_t)c * st + r]) / t : 0.0f;
+        float v = 0.0f;
+        if (ok && !ended) v = (gc[c] - rec.before[(size_t)c * st + r]) / t;
+        rec.smp.target[(size_t)c * st + r] = v;
     }
     if (!ok) rec.smp.valid[r] = 0.0f;
 }

@@ -52,12 +52,15 @@ constexpr int kPackTTotal = 12288 + 64 * 32;
 /* FP32 biases: b0..b3 at 64 l, b4 at 256 (padded to 16) */
 constexpr int kBiasTotal = 272;
 
-/* A cache query: network inputs, the path throughput that scales the
- * prediction, and the path slot whose radiance receives it. */
-struct MlpQuery {
-    float in[kMlpIn];
-    float thr[3];
-    uint32_t slot;
+/* Cache queries as structure of arrays (`stride` entries per field): the
+ * network inputs, the path throughput that scales the prediction and the
+ * path slot whose radiance receives it (thr and slot may be null when
+ * only the outputs are wanted). */
+struct MlpQueries {
+    float* in;        /* [kMlpIn][stride] */
+    float* thr;       /* [3][stride] */
+    uint32_t* slot;   /* [stride] */
+    uint32_t stride;
 };
 
 /* One training sample: inputs, target radiance, weight 1 (valid) or 0. */
@@ -74,13 +77,35 @@ struct MlpWeights {
     const float* bias;    /* [kBiasTotal] */
 };
 
-/* Feature of an input scalar: band k, sin (c = 0) or cos (c = 1) of
- * 2^k pi x = 2 pi (2^(k-1) x), the scaling exact in binary32. */
+/* sin and cos of 2^k pi x for the four bands k = k0 .. k0 + 3 of one
+ * input scalar (k0 a multiple of 4): band k0 from the shared sin/cos in
+ * turns (2^k0 pi x = 2 pi (2^(k0-1) x), the scaling exact in binary32),
+ * each next band by angle doubling, sin 2a = (2 sin a) cos a and
+ * cos 2a = 1 - (2 sin a) sin a.  Three doublings keep the error within a
+ * few units in the last place of 1, far below the BF16 rounding of the
+ * features, at a third of the cost of four sin/cos evaluations. */
+__host__ __device__ inline void mlp_band_quad(float x, int k0, float s[4], float c[4])
+{
+    tr_sincos_turns(x * __builtin_ldexpf(1.0f, k0 - 1), &s[0], &c[0]);
+    for (int b = 1; b < 4; ++b) {
+        float t = 2.0f * s[b - 1];
+        s[b] = t * c[b - 1];
+        c[b] = 1.0f - t * s[b - 1];
+    }
+}
+
+/* Feature of an input scalar: band k, sin (c = 0) or cos (c = 1), by the
+ * same operations as mlp_band_quad (bit-identical to it). */
 __host__ __device__ inline float mlp_feature(float x, int k, int c)
 {
-    float t = x * __builtin_ldexpf(1.0f, k - 1);
     float s, co;
-    tr_sincos_turns(t, &s, &co);
+    tr_sincos_turns(x * __builtin_ldexpf(1.0f, (k & ~3) - 1), &s, &co);
+    for (int b = 0; b < (k & 3); ++b) {
+        float t = 2.0f * s;
+        float s2 = t * co;
+        co = 1.0f - t * s;
+        s = s2;
+    }
     return c ? co : s;
 }
 
@@ -93,11 +118,12 @@ struct MlpTrainBufs {
     uint32_t capacity;
 };
 
-void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQuery* q, uint32_t n, float* out,
-                      float4* radiance, uint32_t grid);
-/* The same for the queries q[index[i]], i < *count (count read on the
- * device). */
-void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* index,
+/* Outputs of queries 0..n-1 to out[3 i + c]. */
+void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQueries& q, uint32_t n, float* out,
+                      uint32_t grid);
+/* Queries index[i], i < *count (count read on the device): radiance of
+ * their slots += thr * max(prediction, 0). */
+void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQueries& q, const uint32_t* index,
                               const uint32_t* count, float4* radiance, uint32_t grid);
 void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,
                       float inv_valid, const MlpTrainBufs& tb, uint32_t grid);

@@ -21,6 +21,8 @@
  */
 #include "tr_mlp.h"
 
+#include <type_traits>
+
 namespace trk {
 
 namespace {
@@ -37,6 +39,18 @@ __device__ inline uint32_t f2bf(float x)
 }
 __device__ inline float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
 
+/* Two binary32 values to a packed BF16 pair (a in the low half), round to
+ * nearest even, in one v_cvt_pk_bf16_f32. */
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t cvt2(float a, float b)
+{
+    bf16x2 r = __builtin_convertvector((f32x2){a, b}, bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &r, 4);
+    return u;
+}
+
 __device__ inline bf16x8 ld8(const uint16_t* p)
 {
     bf16x8 v;
@@ -47,7 +61,7 @@ __device__ inline bf16x8 pack8(const float f[8])
 {
     uint32_t w[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) w[j] = f2bf(f[2 * j]) | (f2bf(f[2 * j + 1]) << 16);
+    for (int j = 0; j < 4; ++j) w[j] = cvt2(f[2 * j], f[2 * j + 1]);
     bf16x8 v;
     __builtin_memcpy(&v, w, 16);
     return v;
@@ -67,11 +81,12 @@ __device__ inline bf16x8 feature_chunk(float x, int c)
 #pragma unroll
         for (int j = 0; j < 8; ++j) f[j] = 0.0f;
     } else {
-        int k0 = (c % 3) * 4;
+        float s[4], co[4];
+        mlp_band_quad(x, (c % 3) * 4, s, co);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            float t = x * __builtin_ldexpf(1.0f, k0 + b - 1);
-            tr_sincos_turns(t, &f[2 * b], &f[2 * b + 1]);
+            f[2 * b] = s[b];
+            f[2 * b + 1] = co[b];
         }
     }
     return pack8(f);
@@ -81,97 +96,231 @@ constexpr int kWaves = 4;          /* waves per block */
 constexpr int kTile = 16;          /* samples per wave tile */
 constexpr int kBlockTile = kWaves * kTile;
 
+/* ---------------- LDS weight layout ---------------- */
+
+/* The packed weights staged in LDS with padded rows: a 16-row MFMA
+ * fragment read (16 bytes per lane) then spreads over all 64 banks instead
+ * of hitting the same banks 16 times (unpadded rows are 128 or 512 bytes
+ * apart).  Row strides in halves: layer 0 264, layers 1..4 72; transposes
+ * of layers 1..3 72, of layer 4 (32 outputs padded) 40. */
+constexpr int kLdsStride[5] = {264, 72, 72, 72, 72};
+constexpr int kLdsOff[5] = {0, 64 * 264, 64 * 264 + 64 * 72, 64 * 264 + 2 * 64 * 72, 64 * 264 + 3 * 64 * 72};
+constexpr int kLdsTotal = 64 * 264 + 3 * 64 * 72 + 16 * 72;
+constexpr int kLdsTStride[5] = {0, 72, 72, 72, 40};
+constexpr int kLdsTOff[5] = {0, 0, 64 * 72, 2 * 64 * 72, 3 * 64 * 72};
+constexpr int kLdsTTotal = 3 * 64 * 72 + 64 * 40;
+
+/* Copies `rows` rows of `row_len` halves (multiple of 8) from packed
+ * global memory to LDS rows of `stride` halves. */
+__device__ inline void stage_rows(uint16_t* dst, const uint16_t* src, int rows, int row_len, int stride)
+{
+    const int per_row = row_len / 8;
+    for (int i = threadIdx.x; i < rows * per_row; i += blockDim.x) {
+        const int r = i / per_row, c = i - r * per_row;
+        *reinterpret_cast<uint4*>(dst + r * stride + 8 * c) = reinterpret_cast<const uint4*>(src + r * row_len)[c];
+    }
+}
+
+__device__ inline void stage_forward(uint16_t* sw, const uint16_t* w)
+{
+#pragma unroll
+    for (int L = 0; L < 5; ++L) stage_rows(sw + kLdsOff[L], w + kPackOff[L], L == 4 ? 16 : 64, kPackInPad[L], kLdsStride[L]);
+}
+
+__device__ inline void stage_transposed(uint16_t* swt, const uint16_t* wt)
+{
+#pragma unroll
+    for (int L = 1; L < 5; ++L) stage_rows(swt + kLdsTOff[L], wt + kPackTOff[L], 64, kPackTOutPad[L], kLdsTStride[L]);
+}
+
 /* ---------------- inference ---------------- */
 
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery* q, const uint32_t* index,
+constexpr int kInferWaves = 8;                     /* waves per block (one weight copy) */
+constexpr int kInferCols = 2;                      /* 16-query column tiles per wave */
+constexpr int kInferWaveQ = 16 * kInferCols;       /* queries per wave iteration */
+constexpr int kInferBlockQ = kInferWaves * kInferWaveQ;
+constexpr int kHqStride = kMlpHidden + 8;          /* padded activation row (halves) */
+
+/* Orders a wave's LDS writes before later LDS reads of other lanes of the
+ * same wave: the inference kernel's waves each own their LDS tiles, so no
+ * block barrier is needed between layers. */
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* The network inputs, throughput and slot of query i of the list. */
+struct InferIn {
+    float in[kMlpIn];
+    float thr[3];
+    uint32_t slot;
+};
+
+__device__ inline void infer_load(const MlpQueries& q, const uint32_t* index, uint32_t qi, uint32_t n, InferIn* d)
+{
+    if (qi < n) {
+        const uint32_t k = index ? index[qi] : qi;
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) d->in[i] = q.in[(size_t)i * q.stride + k];
+        if (q.thr) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) d->thr[i] = q.thr[(size_t)i * q.stride + k];
+            d->slot = q.slot[k];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) d->in[i] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) d->thr[i] = 0.0f;
+        d->slot = 0;
+    }
+}
+
+/* Input scalar c / 3 (c = 4 s + g, chunk of k-step s) from the lane's
+ * registers: for a fixed step s it is one of two neighbouring scalars. */
+template <int S>
+__device__ inline float chunk_input(const InferIn& d, int g)
+{
+    constexpr int lo = (4 * S) / 3;
+    constexpr int hi = lo + 1 < kMlpIn ? lo + 1 : kMlpIn - 1;
+    return (4 * S + g) / 3 == lo ? d.in[lo] : d.in[hi];
+}
+
+/* Writes relu(acc + bias) of neuron tile mt, column tile ct as BF16
+ * activations (four consecutive neurons per lane, one 8-byte store). */
+__device__ inline void store_act(uint16_t (*h)[kHqStride], const float* bias, int mt, int ct, int g, int col,
+                                 f32x4 acc)
+{
+    int m = 16 * mt + 4 * g;
+    uint32_t lo = cvt2(fmaxf(acc[0] + bias[m], 0.0f), fmaxf(acc[1] + bias[m + 1], 0.0f));
+    uint32_t hi = cvt2(fmaxf(acc[2] + bias[m + 2], 0.0f), fmaxf(acc[3] + bias[m + 3], 0.0f));
+    *reinterpret_cast<uint2*>(&h[16 * ct + col][m]) = make_uint2(lo, hi);
+}
+
+/* Forward pass for queries: the packed BF16 weights and the biases are
+ * staged in LDS once per block; each wave then runs 32 queries per
+ * iteration (two 16-query column tiles sharing every weight fragment),
+ * its activations in its own LDS tile, and prefetches the next
+ * iteration's inputs while the matrix cores work. */
+__global__ __launch_bounds__(kInferWaves * 64) void k_mlp_infer(MlpWeights W, MlpQueries q, const uint32_t* index,
                                                    uint32_t n_arg, const uint32_t* count, float* out,
                                                    float4* radiance)
 {
     const uint32_t n = count ? *count : n_arg;
     if (n == 0) return;
-    __shared__ __attribute__((aligned(16))) uint16_t sw[kPackTotal];
+    __shared__ __attribute__((aligned(16))) uint16_t sw[kLdsTotal];
     __shared__ float sb[kBiasTotal];
-    __shared__ float qin[kWaves][kTile][kMlpIn];
-    __shared__ __attribute__((aligned(16))) uint16_t hq[kWaves][kTile][kMlpHidden];
+    __shared__ __attribute__((aligned(16))) uint16_t hq[kInferWaves][kInferWaveQ][kHqStride];
 
-    for (int i = threadIdx.x; i < kPackTotal / 8; i += blockDim.x)
-        reinterpret_cast<uint4*>(sw)[i] = reinterpret_cast<const uint4*>(W.w)[i];
+    stage_forward(sw, W.w);
     for (int i = threadIdx.x; i < kBiasTotal; i += blockDim.x) sb[i] = W.bias[i];
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-    for (uint32_t base = blockIdx.x * kBlockTile; base < n; base += gridDim.x * kBlockTile) {
-        uint32_t qi = base + wave * kTile + col;
-        bool valid = qi < n;
-        if (g == 0) {
+    uint16_t (*h)[kHqStride] = hq[wave];
+    const uint32_t step = gridDim.x * kInferBlockQ;
+    uint32_t base = blockIdx.x * kInferBlockQ + wave * kInferWaveQ;
+    InferIn cur[kInferCols];
 #pragma unroll
-            for (int i = 0; i < kMlpIn; ++i) qin[wave][col][i] = valid ? q[index ? index[qi] : qi].in[i] : 0.0f;
-        }
-        __syncthreads();
-        /* layer 0: 4 neuron tiles x 8 feature steps */
-        f32x4 acc[4];
+    for (int ct = 0; ct < kInferCols; ++ct) infer_load(q, index, base + 16 * ct + col, n, &cur[ct]);
+    for (; base < n; base += step) {
+        InferIn nxt[kInferCols];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            int c = 4 * s + g;
-            bf16x8 b = feature_chunk(qin[wave][col][c < 30 ? c / 3 : 0], c);
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-                acc[mt] = mfma(ld8(&sw[(16 * mt + col) * 256 + 32 * s + 8 * g]), b, acc[mt]);
-        }
+        for (int ct = 0; ct < kInferCols; ++ct) infer_load(q, index, base + step + 16 * ct + col, n, &nxt[ct]);
+        /* layer 0: 4 neuron tiles x 8 feature steps, features made in registers */
+        f32x4 acc[4][kInferCols];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int m = 16 * mt + 4 * g + i;
-                hq[wave][col][m] = (uint16_t)f2bf(fmaxf(acc[mt][i] + sb[m], 0.0f));
+            for (int ct = 0; ct < kInferCols; ++ct) acc[mt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        auto layer0_step = [&](auto S) {
+            constexpr int s = decltype(S)::value;
+            const int c = 4 * s + g;
+            bf16x8 b[kInferCols];
+#pragma unroll
+            for (int ct = 0; ct < kInferCols; ++ct) b[ct] = feature_chunk(chunk_input<s>(cur[ct], g), c);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                bf16x8 a = ld8(&sw[(16 * mt + col) * kLdsStride[0] + 32 * s + 8 * g]);
+#pragma unroll
+                for (int ct = 0; ct < kInferCols; ++ct) acc[mt][ct] = mfma(a, b[ct], acc[mt][ct]);
             }
-        __syncthreads();
+        };
+        layer0_step(std::integral_constant<int, 0>{});
+        layer0_step(std::integral_constant<int, 1>{});
+        layer0_step(std::integral_constant<int, 2>{});
+        layer0_step(std::integral_constant<int, 3>{});
+        layer0_step(std::integral_constant<int, 4>{});
+        layer0_step(std::integral_constant<int, 5>{});
+        layer0_step(std::integral_constant<int, 6>{});
+        layer0_step(std::integral_constant<int, 7>{});
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int ct = 0; ct < kInferCols; ++ct) store_act(h, sb, mt, ct, g, col, acc[mt][ct]);
+        wave_lds_sync();
         /* hidden layers 1..3 */
 #pragma unroll
         for (int L = 1; L <= 3; ++L) {
-            const uint16_t* WL = sw + kPackOff[L];
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                bf16x8 b = ld8(&hq[wave][col][32 * s + 8 * g]);
-#pragma unroll
-                for (int mt = 0; mt < 4; ++mt) acc[mt] = mfma(ld8(&WL[(16 * mt + col) * 64 + 32 * s + 8 * g]), b, acc[mt]);
-            }
-            __syncthreads();
+            const uint16_t* WL = sw + kLdsOff[L];
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    int m = 16 * mt + 4 * g + i;
-                    hq[wave][col][m] = (uint16_t)f2bf(fmaxf(acc[mt][i] + sb[64 * L + m], 0.0f));
-                }
-            __syncthreads();
-        }
-        /* output layer: one 16-row tile, rows 0..2 used */
-        f32x4 o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int ct = 0; ct < kInferCols; ++ct) acc[mt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
-            o = mfma(ld8(&sw[kPackOff[4] + col * 64 + 32 * s + 8 * g]), ld8(&hq[wave][col][32 * s + 8 * g]), o);
-        if (g == 0 && valid) {
-            float y0 = o[0] + sb[256], y1 = o[1] + sb[257], y2 = o[2] + sb[258];
-            if (out) {
-                out[qi * 3 + 0] = y0;
-                out[qi * 3 + 1] = y1;
-                out[qi * 3 + 2] = y2;
+            for (int s = 0; s < 2; ++s) {
+                bf16x8 b[kInferCols];
+#pragma unroll
+                for (int ct = 0; ct < kInferCols; ++ct) b[ct] = ld8(&h[16 * ct + col][32 * s + 8 * g]);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    bf16x8 a = ld8(&WL[(16 * mt + col) * kLdsStride[L] + 32 * s + 8 * g]);
+#pragma unroll
+                    for (int ct = 0; ct < kInferCols; ++ct) acc[mt][ct] = mfma(a, b[ct], acc[mt][ct]);
+                }
             }
-            if (radiance) {
-                const MlpQuery& r = q[index ? index[qi] : qi];
-                float4 cur = radiance[r.slot];
-                cur.x = cur.x + r.thr[0] * fmaxf(y0, 0.0f);
-                cur.y = cur.y + r.thr[1] * fmaxf(y1, 0.0f);
-                cur.z = cur.z + r.thr[2] * fmaxf(y2, 0.0f);
-                radiance[r.slot] = cur;
+            wave_lds_sync();
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int ct = 0; ct < kInferCols; ++ct) store_act(h, sb + 64 * L, mt, ct, g, col, acc[mt][ct]);
+            wave_lds_sync();
+        }
+        /* output layer: one 16-row tile per column tile, rows 0..2 used */
+        f32x4 o[kInferCols];
+#pragma unroll
+        for (int ct = 0; ct < kInferCols; ++ct) o[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            bf16x8 a = ld8(&sw[kLdsOff[4] + col * kLdsStride[4] + 32 * s + 8 * g]);
+#pragma unroll
+            for (int ct = 0; ct < kInferCols; ++ct) o[ct] = mfma(a, ld8(&h[16 * ct + col][32 * s + 8 * g]), o[ct]);
+        }
+        wave_lds_sync();
+        if (g == 0) {
+#pragma unroll
+            for (int ct = 0; ct < kInferCols; ++ct) {
+                uint32_t qi = base + 16 * ct + col;
+                if (qi >= n) continue;
+                float y0 = o[ct][0] + sb[256], y1 = o[ct][1] + sb[257], y2 = o[ct][2] + sb[258];
+                if (out) {
+                    out[qi * 3 + 0] = y0;
+                    out[qi * 3 + 1] = y1;
+                    out[qi * 3 + 2] = y2;
+                }
+                if (radiance) {
+                    float4 r = radiance[cur[ct].slot];
+                    r.x = r.x + cur[ct].thr[0] * fmaxf(y0, 0.0f);
+                    r.y = r.y + cur[ct].thr[1] * fmaxf(y1, 0.0f);
+                    r.z = r.z + cur[ct].thr[2] * fmaxf(y2, 0.0f);
+                    radiance[cur[ct].slot] = r;
+                }
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int ct = 0; ct < kInferCols; ++ct) cur[ct] = nxt[ct];
     }
 }
 
@@ -179,36 +328,83 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpWeights W, const MlpQuery*
 
 __device__ inline bool bf_positive(uint16_t h) { return h != 0 && (h & 0x8000u) == 0; }
 
-__global__ __launch_bounds__(256) void k_mlp_train(MlpWeights W, MlpSampleSoA smp, uint32_t n, float loss_scale,
-                                                   float inv_valid, MlpTrainBufs tb)
+constexpr int kTrainWaves = 8;                      /* waves per training block */
+constexpr int kTrainBlockQ = kTrainWaves * kTile;   /* samples per block iteration */
+
+/* relu(acc + bias) of neuron tile mt as BF16: written to the wave's LDS
+ * tile (row = sample) and to ht / dzt-style global storage
+ * [neuron][sample]; returns the four positive-bits (bit i: neuron
+ * 16 mt + 4 g + i). */
+__device__ inline uint32_t store_h(uint16_t (*h)[kHqStride], uint16_t* glob, uint32_t cap, uint32_t b,
+                                   const float* bias, int mt, int g, int col, f32x4 acc)
 {
-    __shared__ float qin[kWaves][kTile][kMlpIn];
-    __shared__ __attribute__((aligned(16))) uint16_t hq[kWaves][4][kTile][kMlpHidden];  /* H1..H4 */
-    __shared__ __attribute__((aligned(16))) uint16_t dq[kWaves][kTile][kMlpHidden];
-    __shared__ float red[256];
+    const int m = 16 * mt + 4 * g;
+    const uint32_t lo = cvt2(fmaxf(acc[0] + bias[m], 0.0f), fmaxf(acc[1] + bias[m + 1], 0.0f));
+    const uint32_t hi = cvt2(fmaxf(acc[2] + bias[m + 2], 0.0f), fmaxf(acc[3] + bias[m + 3], 0.0f));
+    const uint32_t hv[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
+    uint32_t bits = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        glob[(size_t)(m + i) * cap + b] = (uint16_t)hv[i];
+        bits |= (bf_positive((uint16_t)hv[i]) ? 1u : 0u) << i;
+    }
+    *reinterpret_cast<uint2*>(&h[col][m]) = make_uint2(lo, hi);
+    return bits;
+}
+
+/* One training step's per-sample work for 16 samples per wave and
+ * iteration: forward pass (activations H1..H4 to global for the weight
+ * gradient, their positive-bits kept in registers), relative-L2 loss and
+ * its scaled gradient, backward data pass (pre-activation gradients dZ to
+ * global).  The packed forward and transposed weights are staged in LDS
+ * once per block; each wave exchanges layer outputs through its own LDS
+ * tile, so only the staging needs a block barrier. */
+__global__ __launch_bounds__(kTrainWaves * 64) void k_mlp_train(MlpWeights W, MlpSampleSoA smp, uint32_t n,
+                                                                float loss_scale, float inv_valid, MlpTrainBufs tb)
+{
+    __shared__ float red[kTrainWaves];
+    if (blockIdx.x * kTrainBlockQ >= n) {
+        if (threadIdx.x == 0) tb.loss_part[blockIdx.x] = 0.0f;
+        return;
+    }
+    __shared__ __attribute__((aligned(16))) uint16_t sw[kLdsTotal];
+    __shared__ __attribute__((aligned(16))) uint16_t swt[kLdsTTotal];
+    __shared__ float sb[kBiasTotal];
+    __shared__ __attribute__((aligned(16))) uint16_t hq[kTrainWaves][kTile][kHqStride];
+    stage_forward(sw, W.w);
+    stage_transposed(swt, W.wt);
+    for (int i = threadIdx.x; i < kBiasTotal; i += blockDim.x) sb[i] = W.bias[i];
+    __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    uint16_t (*h)[kHqStride] = hq[wave];
     const uint32_t cap = tb.capacity;
     float loss_acc = 0.0f;
-    for (uint32_t base = blockIdx.x * kBlockTile; base < n; base += gridDim.x * kBlockTile) {
-        uint32_t b = base + wave * kTile + col;   /* n is a multiple of 64: always < n */
-        if (g == 0) {
+    for (uint32_t base = blockIdx.x * kTrainBlockQ; base < n; base += gridDim.x * kTrainBlockQ) {
+        const uint32_t b = base + wave * kTile + col;   /* n is a multiple of 512: always < n */
+        InferIn x;
 #pragma unroll
-            for (int i = 0; i < kMlpIn; ++i) qin[wave][col][i] = smp.in[(size_t)i * smp.stride + b];
-        }
-        __syncthreads();
-        /* forward */
+        for (int i = 0; i < kMlpIn; ++i) x.in[i] = smp.in[(size_t)i * smp.stride + b];
+        /* forward, layer 0 */
         f32x4 acc[4];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            int c = 4 * s + g;
-            bf16x8 bv = feature_chunk(qin[wave][col][c < 30 ? c / 3 : 0], c);
+        auto layer0_step = [&](auto S) {
+            constexpr int s = decltype(S)::value;
+            bf16x8 bv = feature_chunk(chunk_input<s>(x, g), 4 * s + g);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
-                acc[mt] = mfma(ld8(W.w + (16 * mt + col) * 256 + 32 * s + 8 * g), bv, acc[mt]);
-        }
+                acc[mt] = mfma(ld8(&sw[(16 * mt + col) * kLdsStride[0] + 32 * s + 8 * g]), bv, acc[mt]);
+        };
+        layer0_step(std::integral_constant<int, 0>{});
+        layer0_step(std::integral_constant<int, 1>{});
+        layer0_step(std::integral_constant<int, 2>{});
+        layer0_step(std::integral_constant<int, 3>{});
+        layer0_step(std::integral_constant<int, 4>{});
+        layer0_step(std::integral_constant<int, 5>{});
+        layer0_step(std::integral_constant<int, 6>{});
+        layer0_step(std::integral_constant<int, 7>{});
+        uint32_t pos[4];   /* positive-bits of H1..H4: bit 4 mt + i */
 #pragma unroll
         for (int L = 0; L < 4; ++L) {
             if (L > 0) {
@@ -216,35 +412,33 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpWeights W, MlpSampleSoA sm
                 for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
-                    bf16x8 bv = ld8(&hq[wave][L - 1][col][32 * s + 8 * g]);
+                    bf16x8 bv = ld8(&h[col][32 * s + 8 * g]);
 #pragma unroll
                     for (int mt = 0; mt < 4; ++mt)
-                        acc[mt] = mfma(ld8(W.w + kPackOff[L] + (16 * mt + col) * 64 + 32 * s + 8 * g), bv, acc[mt]);
+                        acc[mt] = mfma(ld8(&sw[kLdsOff[L] + (16 * mt + col) * kLdsStride[L] + 32 * s + 8 * g]), bv,
+                                       acc[mt]);
                 }
+                wave_lds_sync();
             }
-            /* H_{L+1} = ReLU(Z_L + b_L): to LDS and to ht[L] */
+            /* H_{L+1} = ReLU(Z_L + b_L) */
+            uint16_t* glob = tb.ht + (size_t)L * 64 * cap;
+            pos[L] = 0;
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    int m = 16 * mt + 4 * g + i;
-                    uint16_t h = (uint16_t)f2bf(fmaxf(acc[mt][i] + W.bias[64 * L + m], 0.0f));
-                    hq[wave][L][col][m] = h;
-                    tb.ht[((size_t)L * 64 + m) * cap + b] = h;
-                }
-            __syncthreads();
+            for (int mt = 0; mt < 4; ++mt) pos[L] |= store_h(h, glob, cap, b, sb + 64 * L, mt, g, col, acc[mt]) << (4 * mt);
+            wave_lds_sync();
         }
         f32x4 o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = 0; s < 2; ++s)
-            o = mfma(ld8(W.w + kPackOff[4] + col * 64 + 32 * s + 8 * g), ld8(&hq[wave][3][col][32 * s + 8 * g]), o);
+            o = mfma(ld8(&sw[kLdsOff[4] + col * kLdsStride[4] + 32 * s + 8 * g]), ld8(&h[col][32 * s + 8 * g]), o);
+        wave_lds_sync();
         /* relative L2 loss gradient, scaled; rows 0..2 of lanes g == 0 */
         float d[3] = {0.0f, 0.0f, 0.0f};
         if (g == 0) {
             float vw = smp.valid[b];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                float y = o[c] + W.bias[256 + c];
+                float y = o[c] + sb[256 + c];
                 float t = smp.target[(size_t)c * smp.stride + b];
                 float w = vw * inv_valid / (y * y + 0.01f);
                 float e = y - t;
@@ -270,43 +464,52 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpWeights W, MlpSampleSoA sm
         }
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
-            acc[mt] = mfma(ld8(W.wt + kPackTOff[4] + (16 * mt + col) * 32 + 8 * g), bz, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            acc[mt] = mfma(ld8(&swt[kLdsTOff[4] + (16 * mt + col) * kLdsTStride[4] + 8 * g]), bz,
+                           f32x4{0.0f, 0.0f, 0.0f, 0.0f});
         /* backward through layers 3..0 */
 #pragma unroll
         for (int L = 3; L >= 0; --L) {
             /* dZ_L = dH_{L+1} * [H_{L+1} > 0] */
+            uint16_t* glob = tb.dzt + (size_t)L * 64 * cap;
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
+            for (int mt = 0; mt < 4; ++mt) {
+                const int m = 16 * mt + 4 * g;
+                float v[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    int m = 16 * mt + 4 * g + i;
-                    float v = bf_positive(hq[wave][L][col][m]) ? acc[mt][i] : 0.0f;
-                    uint16_t h = (uint16_t)f2bf(v);
-                    dq[wave][col][m] = h;
-                    tb.dzt[((size_t)L * 64 + m) * cap + b] = h;
-                }
-            __syncthreads();
+                for (int i = 0; i < 4; ++i) v[i] = (pos[L] >> (4 * mt + i)) & 1u ? acc[mt][i] : 0.0f;
+                const uint32_t lo = cvt2(v[0], v[1]), hi = cvt2(v[2], v[3]);
+                glob[(size_t)m * cap + b] = (uint16_t)(lo & 0xffffu);
+                glob[(size_t)(m + 1) * cap + b] = (uint16_t)(lo >> 16);
+                glob[(size_t)(m + 2) * cap + b] = (uint16_t)(hi & 0xffffu);
+                glob[(size_t)(m + 3) * cap + b] = (uint16_t)(hi >> 16);
+                if (L > 0) *reinterpret_cast<uint2*>(&h[col][m]) = make_uint2(lo, hi);
+            }
             if (L == 0) break;
+            wave_lds_sync();
             /* dH_L = W_L^T dZ_L */
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                bf16x8 bv = ld8(&dq[wave][col][32 * s + 8 * g]);
+                bf16x8 bv = ld8(&h[col][32 * s + 8 * g]);
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt)
-                    acc[mt] = mfma(ld8(W.wt + kPackTOff[L] + (16 * mt + col) * 64 + 32 * s + 8 * g), bv, acc[mt]);
+                    acc[mt] = mfma(ld8(&swt[kLdsTOff[L] + (16 * mt + col) * kLdsTStride[L] + 32 * s + 8 * g]), bv,
+                                   acc[mt]);
             }
-            __syncthreads();
+            wave_lds_sync();
         }
     }
-    red[threadIdx.x] = loss_acc;
+    /* block sum of the loss, in a fixed order */
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off);
+    if (lane == 0) red[wave] = loss_acc;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kTrainWaves; ++w) t += red[w];
+        tb.loss_part[blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) tb.loss_part[blockIdx.x] = red[0];
 }
 
 /* ---------------- weight gradients ---------------- */
@@ -332,31 +535,45 @@ __global__ __launch_bounds__(64) void k_mlp_wgrad(MlpSampleSoA smp, uint32_t n, 
     }
     const int mtiles = L == 4 ? 1 : 4;
     const uint32_t cap = tb.capacity;
-    const uint32_t per = n / slices;              /* multiple of 32 */
+    const uint32_t per = n / slices;              /* multiple of 128 */
     const uint32_t k_begin = slice * per, k_end = k_begin + per;
     f32x4 acc[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const uint16_t* dz = tb.dzt + (size_t)L * 64 * cap;
-    for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32) {
-        bf16x8 bv;
-        if (bias) {
-            uint32_t one = col == 0 ? 0x3f80u : 0u;
-            uint32_t w4[4] = {one | (one << 16), one | (one << 16), one | (one << 16), one | (one << 16)};
-            __builtin_memcpy(&bv, w4, 16);
-        } else if (L == 0) {
-            int f = 16 * nt + col;                 /* < 240 */
-            int in = f / 24, band = (f % 24) / 2, c = f & 1;
-            float v[8];
+    /* four 32-sample k-steps per iteration: all their loads are issued
+     * before the matrix-core work (the slice length is a multiple of 128) */
+    constexpr int kU = 4;
+    for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32 * kU) {
+        bf16x8 bv[kU], av[kU][4];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = mlp_feature(smp.in[(size_t)in * smp.stride + k0 + 8 * g + j], band, c);
-            bv = pack8(v);
-        } else {
-            bv = ld8(tb.ht + ((size_t)(L - 1) * 64 + 16 * nt + col) * cap + k0 + 8 * g);
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t k = k0 + 32 * u;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                if (mt < mtiles) av[u][mt] = ld8(dz + (size_t)(16 * mt + col) * cap + k + 8 * g);
+            if (bias) {
+                uint32_t one = col == 0 ? 0x3f80u : 0u;
+                uint32_t w4[4] = {one | (one << 16), one | (one << 16), one | (one << 16), one | (one << 16)};
+                __builtin_memcpy(&bv[u], w4, 16);
+            } else if (L == 0) {
+                int f = 16 * nt + col;                 /* < 240 */
+                int in = f / 24, band = (f % 24) / 2, c = f & 1;
+                const float4* src = reinterpret_cast<const float4*>(smp.in + (size_t)in * smp.stride + k + 8 * g);
+                float4 x0 = src[0], x1 = src[1];
+                float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = mlp_feature(v[j], band, c);
+                bv[u] = pack8(v);
+            } else {
+                bv[u] = ld8(tb.ht + ((size_t)(L - 1) * 64 + 16 * nt + col) * cap + k + 8 * g);
+            }
         }
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-            if (mt < mtiles) acc[mt] = mfma(ld8(dz + (size_t)(16 * mt + col) * cap + k0 + 8 * g), bv, acc[mt]);
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                if (mt < mtiles) acc[mt] = mfma(av[u][mt], bv[u], acc[mt]);
     }
     float* part = tb.partial + (size_t)slice * kMlpParams;
     const int fan_out = kMlpFanOut[L], fan_in = kMlpFanIn[L];
@@ -387,8 +604,17 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* master, float* mom, flo
     if (p >= kMlpParams) return;
     float x = master[p];
     if (update) {
+        /* slices summed in order; the loads of eight are in flight at once */
         float gsum = 0.0f;
-        for (uint32_t s = 0; s < slices; ++s) gsum += partial[(size_t)s * kMlpParams + p];
+        uint32_t s = 0;
+        for (; s + 8 <= slices; s += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(s + u) * kMlpParams + p];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) gsum += v[u];
+        }
+        for (; s < slices; ++s) gsum += partial[(size_t)s * kMlpParams + p];
         float gr = gsum * grad_scale;
         float m = 0.9f * mom[p] + 0.1f * gr;
         float v = 0.99f * vel[p] + 0.01f * gr * gr;
@@ -415,25 +641,26 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* master, float* mom, flo
 
 }  // namespace
 
-void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQuery* q, uint32_t n, float* out, float4* radiance,
-                      uint32_t grid)
+void launch_mlp_infer(hipStream_t s, const MlpWeights& w, const MlpQueries& q, uint32_t n, float* out, uint32_t grid)
 {
     if (n == 0) return;
-    uint32_t need = (n + kBlockTile - 1) / kBlockTile;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(need < grid ? need : grid), dim3(256), 0, s, w, q, (const uint32_t*)nullptr,
-                       n, (const uint32_t*)nullptr, out, radiance);
+    uint32_t need = (n + kInferBlockQ - 1) / kInferBlockQ;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(need < grid ? need : grid), dim3(kInferWaves * 64), 0, s, w, q,
+                       (const uint32_t*)nullptr,
+                       n, (const uint32_t*)nullptr, out, (float4*)nullptr);
 }
 
-void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQuery* q, const uint32_t* index,
+void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQueries& q, const uint32_t* index,
                               const uint32_t* count, float4* radiance, uint32_t grid)
 {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, s, w, q, index, 0u, count, (float*)nullptr, radiance);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kInferWaves * 64), 0, s, w, q, index, 0u, count, (float*)nullptr,
+                       radiance);
 }
 
 void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,
                       float inv_valid, const MlpTrainBufs& tb, uint32_t grid)
 {
-    hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(256), 0, s, w, smp, n, loss_scale, inv_valid, tb);
+    hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(kTrainWaves * 64), 0, s, w, smp, n, loss_scale, inv_valid, tb);
 }
 
 void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb, uint32_t slices)

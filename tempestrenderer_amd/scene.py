@@ -276,9 +276,10 @@ class RayTracerScene:
         return rgb.reshape(n, 3), hsh
 
     # radiance cache
-    def setRadianceCache(self, enabled=True, train=True, train_paths=65536, batch=16384, rough_alpha=0.2, seed=1):
+    def setRadianceCache(self, enabled=True, train=True, train_paths=65536, batch=16384, rough_alpha=0.2, seed=1,
+                         query_nee=False):
         c = _abi.tr_cache_settings(int(bool(enabled)), int(bool(train)), int(train_paths), int(batch),
-                                   float(rough_alpha), int(seed))
+                                   float(rough_alpha), int(bool(query_nee)), int(seed))
         check(self.lib.tr_scene_set_radiance_cache(self.h, ctypes.byref(c)))
 
     def radianceCache(self):

@@ -34,8 +34,9 @@ def _desc(w=48, h=48, spp=4, depth=4):
     return d
 
 
-def test_cache_paths_match_oracle_with_constant_network():
-    print("case: cache mode constant network box 48x48 spp4", flush=True)
+@pytest.mark.parametrize("query_nee", [False, True])
+def test_cache_paths_match_oracle_with_constant_network(query_nee):
+    print("case: cache mode constant network box 48x48 spp4 query_nee=%d" % query_nee, flush=True)
     d = _desc()
     train_paths = 1000
     total = d.pixels * d.settings["samples_camera"]
@@ -43,13 +44,13 @@ def test_cache_paths_match_oracle_with_constant_network():
     g = RayTracerScene(d.width, d.height, d.vpi)
     d.populate(g, gpu_material)
     g.commitScene()
-    g.setRadianceCache(True, train=False, train_paths=train_paths, batch=4096, rough_alpha=0.2)
+    g.setRadianceCache(True, train=False, train_paths=train_paths, batch=4096, rough_alpha=0.2, query_nee=query_nee)
     g.radianceCache().params = _constant_params()
     o = oracle.OracleScene(d.width, d.height, d.vpi)
     d.populate(o, oracle.material)
     o.commitScene()
     n_rec = total // stride + 1
-    o.setCache(True, 0.2, stride, 0, C, n_rec)
+    o.setCache(True, 0.2, stride, 0, C, n_rec, query_nee=query_nee)
     px = np.arange(d.pixels, dtype=np.uint32)
     g_rgb, g_hash = g.renderPaths(px, d.settings["samples_camera"], frame=0)
     o_rgb, o_hash = o.renderPaths(px, frame=0)
@@ -61,6 +62,8 @@ def test_cache_paths_match_oracle_with_constant_network():
     assert np.array_equal(gv, ov)
     assert gv.sum() > 0.5 * n_rec
     ok = gv > 0
+    # some targets are zero (paths ending at the query vertex), most are not
+    assert 0.05 * ok.sum() < (np.abs(gt[ok]).sum(axis=1) > 0).sum()
     assert np.array_equal(gx[ok], ox[ok])
     tol = 1e-4 * np.maximum(np.maximum(np.abs(gt[ok]), np.abs(ot[ok])), 1e-6)
     assert not (np.abs(gt[ok] - ot[ok]) > tol).any()

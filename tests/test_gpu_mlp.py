@@ -1,10 +1,12 @@
 """GPU radiance-cache MLP against the FP64 oracle (oracle/mlp.py).
 
 The GPU multiplies BF16 operands with FP32 accumulation; the oracle's bf16
-mode rounds the same operands in FP64.  They differ by the accumulation
-order and by rare one-ulp BF16 rounding flips (the GPU encodes features
-with the shared FP32 sin/cos, the oracle in FP64), so outputs are compared
-to 1e-2 of their scale and gradients to 1e-3 in norm, per layer.
+mode rounds the same operands in FP64, with features made by the GPU's
+FP32 recipe (oracle.mlp.encode_gpu).  They differ by the accumulation
+order and by rare one-ulp BF16 rounding flips (the GPU's first band of a
+group comes from the shared FP32 sin/cos, the oracle's from FP64), so
+outputs are compared to 1e-2 of their scale and gradients to 5e-4 in
+norm, per layer.
 """
 import numpy as np
 import pytest
@@ -66,7 +68,7 @@ def test_gradient_matches_oracle():
     for i, sl in enumerate(_layer_slices()):
         rel = np.linalg.norm(g[sl] - ref[sl]) / np.linalg.norm(ref[sl])
         print("layer %d gradient rel err %.2e" % (i, rel))
-        assert rel <= 1e-3, (i, rel)
+        assert rel <= 5e-4, (i, rel)
 
 
 def _smooth_targets(x):

@@ -88,3 +88,17 @@ def test_abi_declares_mlp_entry_points(hip_lib):
     for name in ("tr_mlp_create", "tr_mlp_destroy", "tr_mlp_get_params", "tr_mlp_set_params", "tr_mlp_infer",
                  "tr_mlp_gradient", "tr_mlp_train_step"):
         assert hasattr(hip_lib, name)
+
+
+def test_gpu_feature_recipe_matches_exact_encoding():
+    """The angle-doubling features the GPU computes stay within 2e-6 of the
+    exact sin/cos (three binary32 doublings from an exact base), far below
+    the BF16 spacing of the features; a wrong doubling formula or band
+    order would be off by O(1)."""
+    x = np.random.default_rng(3).uniform(0.0, 1.0, size=(4096, mlp.N_IN))
+    exact, gpu = mlp.encode(x.astype(np.float32)), mlp.encode_gpu(x)
+    assert np.abs(gpu - exact).max() < 2e-6
+    # the first band of each group of four is sin/cos itself
+    f = gpu.reshape(-1, mlp.N_IN, mlp.BANDS, 2)
+    e = exact.reshape(-1, mlp.N_IN, mlp.BANDS, 2)
+    assert np.abs(f[:, :, ::4] - e[:, :, ::4]).max() < 2e-7

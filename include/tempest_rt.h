@@ -321,6 +321,50 @@ tr_status tr_scene_trace_rays(tr_scene* scene, size_t ray_count, const tr_ray* r
 tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint32_t* pixels,
                                 float* out_rgb, uint32_t* out_hit_hash);
 tr_status tr_scene_get_stats(const tr_scene* scene, tr_stats* stats);
+
+/* Batched render requests.  Replaces the reference's serial request loop
+ * of samples/virtual-gonioreflectometer.cc:161-189 (per request:
+ * updateLightSource of the directional light, drawOnce, copy of the
+ * image) by one call: every request is a frame of its own camera, image
+ * size and camera sample count, with light 0 (a directional light) turned
+ * to the request's direction.  All requests' paths run through one path
+ * pool in batches that end at pixel boundaries; the bounce loop of a batch
+ * is a captured HIP graph replayed once per batch.  A request's pixel p,
+ * camera sample c draws the random numbers of frame pixel
+ * (first pixel of the request over all requests) + p, sample c, so the
+ * results do not depend on the batching. */
+typedef struct tr_request {
+    float view_proj_inv[16];
+    uint32_t width, height;    /* >= 2 each */
+    uint32_t samples_camera;   /* 1 .. 65536 */
+    float light_direction[3];  /* DirectionalLight::Direction of light 0 */
+} tr_request;
+
+typedef struct tr_request_options {
+    int32_t use_graph;           /* 1: replay one captured graph per batch; 0: plain launches */
+    int32_t autotune;            /* 1: time candidate pool sizes and traversal grids on the
+                                  * first batches and keep the fastest for the rest */
+    uint32_t pool_slots;         /* paths per batch without autotuning (0: 2^22) */
+    uint32_t trace_waves_per_cu; /* persistent traversal waves per CU without autotuning (0: 16) */
+} tr_request_options;
+
+typedef struct tr_request_stats {
+    uint64_t paths, pixels;
+    uint32_t batches;
+    uint32_t pool_slots, trace_waves_per_cu; /* configuration of the bulk of the batches */
+    uint32_t candidates;                     /* configurations timed by the autotuner */
+    double ms;                               /* wall time of the call */
+    double tune_ms;                          /* part of it spent on the timed trial batches */
+} tr_request_stats;
+
+/* Renders `count` requests.  out_mean (count x 3, may be null): mean
+ * radiance of each request over its pixels and samples; out_image (may be
+ * null): per-pixel mean radiance, all requests in order, width x height x
+ * 3 floats each.  Needs a committed scene whose light 0 is directional,
+ * no grid volumes and no radiance cache.  options and stats may be null. */
+tr_status tr_scene_render_requests(tr_scene* scene, size_t count, const tr_request* requests,
+                                   const tr_request_options* options, float* out_mean, float* out_image,
+                                   tr_request_stats* stats);
 /* Radiance cache (new; no reference counterpart).  With `enabled`, a path
  * ends at the vertex after its first rough vertex (microfacet alpha >=
  * rough_alpha, a lobe without specular part, or an SGGX surface): the

@@ -59,6 +59,7 @@ def _declare(lib):
         "orc_set_russian_roulette": (None, [vp, f]),
         "orc_set_background_spectrum": (None, [vp, P(f)]),
         "orc_set_seed": (None, [vp, u64]),
+        "orc_set_pixel_key_base": (None, [vp, u32]),
         "orc_set_estimator": (None, [vp, u32]),
         "orc_add_triangle_mesh": (ctypes.c_int, [vp, P(f), sz, P(_abi.tr_submesh), sz, P(ctypes.c_int32), sz, vp,
                                                  P(_abi.tr_mesh_options), P(u64)]),
@@ -165,6 +166,7 @@ class OracleScene:
     def setMaxRayDepth(self, n): self.lib.orc_set_max_ray_depth(self.h, int(n))
     def setRussianRoulette(self, r): self.lib.orc_set_russian_roulette(self.h, float(r))
     def setSeed(self, s): self.lib.orc_set_seed(self.h, int(s))
+    def setPixelKeyBase(self, b): self.lib.orc_set_pixel_key_base(self.h, int(b))
     def setEstimator(self, m): self.lib.orc_set_estimator(self.h, int(m))
 
     def setBackgroundSpectrum(self, rgb):

@@ -95,6 +95,10 @@ struct orc_scene {
     float background[3] = {0, 0, 0};
     uint32_t key[2] = {1, 0};
     uint32_t estimator = TR_ESTIMATOR_REFERENCE;
+    /* added to the pixel index in the random-number key: a request of a
+     * batched render draws the numbers of its pixels' position over all
+     * requests */
+    uint32_t pixel_key_base = 0;
     /* radiance-cache mode with a constant network output (the checker of
      * the path logic of cache mode) and the training records it writes */
     struct {
@@ -764,7 +768,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
     uint32_t x = pixel % s->width, y = pixel / s->width;
     Rng rng;
     rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
-    rng.pixel = pixel; rng.sample = cs; rng.split = 0;
+    rng.pixel = pixel + s->pixel_key_base; rng.sample = cs; rng.split = 0;
     uint32_t hash = 2166136261u;
     float sx, sy;
     if (s->spp > 1) {
@@ -998,6 +1002,7 @@ void orc_set_max_ray_depth(orc_scene* s, uint32_t d) { s->max_depth = d; }
 void orc_set_russian_roulette(orc_scene* s, float rr) { s->rr = rr; }
 void orc_set_background_spectrum(orc_scene* s, const float rgb[3]) { memcpy(s->background, rgb, 12); }
 void orc_set_seed(orc_scene* s, uint64_t seed) { s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32); }
+void orc_set_pixel_key_base(orc_scene* s, uint32_t base) { s->pixel_key_base = base; }
 void orc_set_estimator(orc_scene* s, uint32_t mode) { s->estimator = mode; }
 
 /* Cache mode with a constant network output `value`; the scene box of the

@@ -80,6 +80,22 @@ class tr_stats(ctypes.Structure):
     ]
 
 
+class tr_request(ctypes.Structure):
+    _fields_ = [("view_proj_inv", ctypes.c_float * 16), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("samples_camera", ctypes.c_uint32), ("light_direction", ctypes.c_float * 3)]
+
+
+class tr_request_options(ctypes.Structure):
+    _fields_ = [("use_graph", ctypes.c_int32), ("autotune", ctypes.c_int32), ("pool_slots", ctypes.c_uint32),
+                ("trace_waves_per_cu", ctypes.c_uint32)]
+
+
+class tr_request_stats(ctypes.Structure):
+    _fields_ = [("paths", ctypes.c_uint64), ("pixels", ctypes.c_uint64), ("batches", ctypes.c_uint32),
+                ("pool_slots", ctypes.c_uint32), ("trace_waves_per_cu", ctypes.c_uint32),
+                ("candidates", ctypes.c_uint32), ("ms", ctypes.c_double), ("tune_ms", ctypes.c_double)]
+
+
 class tr_cache_settings(ctypes.Structure):
     _fields_ = [("enabled", ctypes.c_int32), ("train", ctypes.c_int32), ("train_paths", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("rough_alpha", ctypes.c_float), ("query_nee", ctypes.c_int32),
@@ -160,6 +176,8 @@ SIGNATURES = {
     "tr_scene_trace_rays": (_st, [_vp, _sz, P(tr_ray), P(tr_hit)]),
     "tr_scene_render_paths": (_st, [_vp, _sz, P(_u32), P(ctypes.c_float), P(_u32)]),
     "tr_scene_get_stats": (_st, [_vp, P(tr_stats)]),
+    "tr_scene_render_requests": (_st, [_vp, _sz, P(tr_request), P(tr_request_options), P(ctypes.c_float),
+                                       P(ctypes.c_float), P(tr_request_stats)]),
     "tr_scene_set_radiance_cache": (_st, [_vp, P(tr_cache_settings)]),
     "tr_scene_get_radiance_cache": (_st, [_vp, P(_vp)]),
     "tr_scene_read_cache_records": (_st, [_vp, _sz, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float)]),

@@ -108,7 +108,7 @@ struct tr_scene {
     uint32_t capacity = 0;
     uint32_t max_shadow = 0;
     DevBuf<float4> ray_o, ray_d, hit, thr, local, global, sh_o, sh_d, sh_c;
-    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, qq, seg, seg_n, sh_n, counters;
+    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, qq, seg, seg_n, sh_n, counters, slot_req;
     DevBuf<float> sh_tmax;
     DevBuf<uint2> spill;
     DevBuf<uint64_t> counts;
@@ -289,12 +289,13 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
 PathPool path_pool(const tr_scene* s)
 {
     PathPool p;
+    memset(&p, 0, sizeof(p));
     p.ray_o = s->ray_o.ptr; p.ray_d = s->ray_d.ptr; p.hit = s->hit.ptr; p.hit_prim = s->hit_prim.ptr;
     p.thr = s->thr.ptr; p.local = s->local.ptr; p.global = s->global.ptr; p.hash = s->hash.ptr;
     p.queue[0] = s->q0.ptr; p.queue[1] = s->q1.ptr; p.shadow_queue = s->shq.ptr;
     p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr; p.sh_tmax = s->sh_tmax.ptr;
     p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity; p.pflags = s->pflags.ptr;
-    p.query_queue = s->qq.ptr; p.seg = s->seg.ptr; p.seg_n = s->seg_n.ptr;
+    p.query_queue = s->qq.ptr; p.seg = s->seg.ptr; p.seg_n = s->seg_n.ptr; p.slot_req = s->slot_req.ptr;
     p.seg_cap = (s->capacity + kSegment - 1) / kSegment;
     return p;
 }
@@ -302,6 +303,7 @@ PathPool path_pool(const tr_scene* s)
 FrameParams frame_params(const tr_scene* s)
 {
     FrameParams p;
+    memset(&p, 0, sizeof(p));
     p.sc = scene_view(s);
     p.width = s->width;
     p.height = s->height;
@@ -1208,6 +1210,281 @@ tr_status tr_scene_render_paths(tr_scene* s, size_t pixel_count, const uint32_t*
     }
     d_pix.release(); d_rgb.release(); d_hash.release();
     return st;
+}
+
+namespace {
+
+/* Cuts the next batches of at most `pool` paths, ending at pixel
+ * boundaries, from position (request r, pixel lp of it), appending up to
+ * `max_batches` to `out`.  Returns false if a single pixel's paths exceed
+ * the pool. */
+bool cut_batches(const std::vector<ReqRec>& reqs, const std::vector<uint64_t>& path_off,
+                 const std::vector<uint64_t>& pix_off, uint32_t n_gi, uint32_t pool, uint32_t max_batches,
+                 uint32_t* r, uint64_t* lp, std::vector<ReqBatch>* out)
+{
+    const uint32_t n = (uint32_t)reqs.size();
+    for (uint32_t k = 0; k < max_batches && *r < n; ++k) {
+        ReqBatch b;
+        const uint64_t per0 = (uint64_t)reqs[*r].spp * n_gi;
+        b.path_base = path_off[*r] + *lp * per0;
+        b.pix_base = pix_off[*r] + *lp;
+        b.n_slots = 0;
+        b.n_pix = 0;
+        uint64_t room = pool;
+        while (*r < n) {
+            const uint64_t per = (uint64_t)reqs[*r].spp * n_gi;
+            const uint64_t pixels = (uint64_t)reqs[*r].width * reqs[*r].height;
+            const uint64_t take = std::min(room / per, pixels - *lp);
+            if (take == 0) break;
+            b.n_pix += (uint32_t)take;
+            b.n_slots += (uint32_t)(take * per);
+            room -= take * per;
+            *lp += take;
+            if (*lp < pixels) break;
+            ++*r;
+            *lp = 0;
+        }
+        if (b.n_slots == 0) return false;
+        out->push_back(b);
+    }
+    return true;
+}
+
+}  // namespace
+
+tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* req, const tr_request_options* opt,
+                                   float* out_mean, float* out_image, tr_request_stats* out_stats)
+{
+    if (!valid(s) || (count && !req)) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed || !s->device_ready) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    if (count == 0) return TR_OK;
+    if (count >= (1u << 24)) return fail(TR_ERROR_INVALID_ARG, "at most 2^24 - 1 requests");
+    if (s->lights.empty() || s->lights[0].src.type != TR_LIGHT_DIRECTIONAL)
+        return fail(TR_ERROR_INVALID_ARG, "render requests turn light 0, which must be a directional light");
+    if (!s->vols.empty()) return fail(TR_ERROR_UNSUPPORTED, "render requests do not take grid volumes");
+    if (s->cache.enabled) return fail(TR_ERROR_UNSUPPORTED, "render requests do not take the radiance cache");
+    tr_request_options o;
+    memset(&o, 0, sizeof(o));
+    o.use_graph = 1;
+    if (opt) o = *opt;
+    if (o.pool_slots == 0) o.pool_slots = 1u << 22;
+    if (o.trace_waves_per_cu == 0) o.trace_waves_per_cu = 16;
+    if (o.trace_waves_per_cu > 32) return fail(TR_ERROR_INVALID_ARG, "trace_waves_per_cu above 32");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    /* request table and the path / pixel index spaces */
+    std::vector<ReqRec> reqs(count);
+    std::vector<uint64_t> path_off(count + 1), pix_off(count + 1);
+    uint32_t max_per_pixel = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const tr_request& q = req[i];
+        if (q.width < 2 || q.height < 2 || q.samples_camera == 0 || q.samples_camera > 65536)
+            return fail(TR_ERROR_INVALID_ARG, "requests need width, height >= 2 and 1..65536 camera samples");
+        for (int k = 0; k < 16; ++k)
+            if (!std::isfinite(q.view_proj_inv[k])) return fail(TR_ERROR_INVALID_ARG, "non-finite camera matrix");
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(q.light_direction[k])) return fail(TR_ERROR_INVALID_ARG, "non-finite light direction");
+        ReqRec& r = reqs[i];
+        memcpy(r.vpi, q.view_proj_inv, sizeof(r.vpi));
+        r.width = q.width;
+        r.height = q.height;
+        r.spp = q.samples_camera;
+        memcpy(r.light_dir, q.light_direction, sizeof(r.light_dir));
+        const uint64_t px = (uint64_t)q.width * q.height;
+        pix_off[i + 1] = pix_off[i] + px;
+        path_off[i + 1] = path_off[i] + px * q.samples_camera * s->n_gi;
+        max_per_pixel = std::max(max_per_pixel, q.samples_camera * s->n_gi);
+    }
+    const uint64_t total_pix = pix_off[count], total_paths = path_off[count];
+    if (total_pix >= (1ull << 32)) return fail(TR_ERROR_INVALID_ARG, "more than 2^32 - 1 pixels over all requests");
+    /* candidate configurations: (paths per batch, traversal waves per CU) */
+    std::vector<std::pair<uint32_t, uint32_t>> cands;
+    if (o.autotune) {
+        for (uint32_t pool : {1u << 20, 1u << 21, 1u << 22})
+            for (uint32_t w : {8u, 16u, 32u}) cands.push_back({pool, w});
+    } else {
+        cands.push_back({o.pool_slots, o.trace_waves_per_cu});
+    }
+    for (auto& c : cands) c.first = std::max(c.first, max_per_pixel);
+    uint32_t pool_max = 0;
+    for (auto& c : cands) pool_max = std::max(pool_max, c.first);
+    tr_status st = sync_lights(s);
+    if (st != TR_OK) return st;
+    {
+        /* the pool sized for the largest candidate */
+        uint32_t keep = s->settings.path_pool_size;
+        s->settings.path_pool_size = pool_max;
+        st = ensure_pool(s, pool_max);
+        s->settings.path_pool_size = keep;
+        if (st != TR_OK) return st;
+    }
+    if (s->capacity < max_per_pixel) return fail(TR_ERROR_OUT_OF_MEMORY, "path pool below one pixel's samples");
+    for (auto& c : cands) c.first = std::min(c.first, s->capacity);
+    TR_HIP(s->slot_req.reserve(s->capacity));
+    /* traversal stack spill for the widest persistent grid */
+    uint32_t max_waves = 16;
+    for (auto& c : cands) max_waves = std::max(max_waves, c.second);
+    TR_HIP(s->spill.reserve((size_t)std::max(1, s->cu_count) * max_waves * kTraceBlock * kStackSpill));
+    /* batch table: the smallest candidate cuts the most batches; each
+     * trial phase may end one batch early */
+    uint32_t pool_min = s->capacity;
+    for (auto& c : cands) pool_min = std::min(pool_min, c.first);
+    uint64_t batch_cap = cands.size() + 2;
+    {
+        std::vector<ReqBatch> tmp;
+        uint32_t r = 0;
+        uint64_t lp = 0;
+        while (r < count) {
+            tmp.clear();
+            if (!cut_batches(reqs, path_off, pix_off, s->n_gi, pool_min, 1u << 16, &r, &lp, &tmp))
+                return fail(TR_ERROR_INVALID_ARG, "a pixel's samples exceed the batch size");
+            batch_cap += tmp.size();
+        }
+    }
+    DevBuf<ReqRec> d_reqs;
+    DevBuf<uint64_t> d_path_off, d_pix_off;
+    DevBuf<ReqBatch> d_batches;
+    DevBuf<uint32_t> d_cursor;
+    DevBuf<double> d_accum;
+    DevBuf<float> d_mean, d_image;
+    TR_HIP(d_reqs.reserve(count));
+    TR_HIP(d_path_off.reserve(count + 1));
+    TR_HIP(d_pix_off.reserve(count + 1));
+    TR_HIP(d_batches.reserve(batch_cap));
+    TR_HIP(d_cursor.reserve(1));
+    TR_HIP(d_accum.reserve(total_pix * 3));
+    TR_HIP(d_mean.reserve(count * 3));
+    if (out_image) TR_HIP(d_image.reserve(total_pix * 3));
+    hipStream_t stream = s->stream;
+    TR_HIP(hipMemcpyAsync(d_reqs.ptr, reqs.data(), count * sizeof(ReqRec), hipMemcpyHostToDevice, stream));
+    TR_HIP(hipMemcpyAsync(d_path_off.ptr, path_off.data(), (count + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    TR_HIP(hipMemcpyAsync(d_pix_off.ptr, pix_off.data(), (count + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    TR_HIP(hipMemsetAsync(d_cursor.ptr, 0, sizeof(uint32_t), stream));
+    TR_HIP(hipMemsetAsync(d_accum.ptr, 0, total_pix * 3 * sizeof(double), stream));
+
+    FrameParams p = frame_params(s);
+    p.req.reqs = d_reqs.ptr;
+    p.req.path_off = d_path_off.ptr;
+    p.req.pix_off = d_pix_off.ptr;
+    p.req.batches = d_batches.ptr;
+    p.req.cursor = d_cursor.ptr;
+    p.req.n = (uint32_t)count;
+    PathPool pool = path_pool(s);
+    BatchOut out;
+    memset(&out, 0, sizeof(out));
+    CacheRecords rec;
+    memset(&rec, 0, sizeof(rec));
+
+    /* the launches of one batch: every per-batch quantity is read on the
+     * device through the cursor, so the sequence can be captured once */
+    auto enqueue_batch = [&](uint32_t slots, uint32_t waves, uint32_t max_pix) {
+        const uint32_t grid = (uint32_t)std::max(1, s->cu_count) * waves;
+        launch_raygen(stream, p, pool, out, 0, slots, rec);
+        int cur = 0, nxt = 1;
+        for (uint32_t v = 0; v <= s->max_depth; ++v) {
+            launch_trace(stream, p, pool, out, 0, v, cur, nxt, grid, nullptr);
+            launch_shade(stream, p, pool, out, 0, v, cur, nxt, slots, rec);
+            launch_compact(stream, pool, slots, nxt, false);
+            if (p.sc.n_lights > 0) launch_shadow(stream, p, pool, out, 0, v, grid, nullptr);
+            std::swap(cur, nxt);
+        }
+        launch_accumulate_requests(stream, p, pool, d_accum.ptr, max_pix, d_cursor.ptr);
+    };
+    std::vector<ReqBatch> batches;
+    batches.reserve(batch_cap);   /* stable storage while uploads are in flight */
+    uint32_t pos_r = 0;
+    uint64_t pos_lp = 0;
+    uint32_t launched = 0;
+    /* runs the batches [launched, batches.size()) with one configuration */
+    auto run = [&](uint32_t slots, uint32_t waves) -> tr_status {
+        const uint32_t b0 = launched, b1 = (uint32_t)batches.size();
+        if (b1 <= b0) return TR_OK;
+        if (b1 > batch_cap) return fail(TR_ERROR_INVALID_ARG, "batch table overflow");
+        TR_HIP(hipMemcpyAsync(d_batches.ptr + b0, batches.data() + b0, (b1 - b0) * sizeof(ReqBatch),
+                              hipMemcpyHostToDevice, stream));
+        uint32_t max_pix = 1;
+        for (uint32_t k = b0; k < b1; ++k) max_pix = std::max(max_pix, batches[k].n_pix);
+        if (o.use_graph) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            TR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            enqueue_batch(slots, waves, max_pix);
+            hipError_t e = hipStreamEndCapture(stream, &graph);
+            if (e != hipSuccess) return fail(TR_ERROR_DEVICE, "graph capture failed");
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return fail(TR_ERROR_DEVICE, "graph instantiation failed");
+            }
+            for (uint32_t k = b0; k < b1 && e == hipSuccess; ++k) e = hipGraphLaunch(exec, stream);
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(TR_ERROR_DEVICE, "graph launch failed");
+        } else {
+            for (uint32_t k = b0; k < b1; ++k) enqueue_batch(slots, waves, max_pix);
+            TR_HIP(hipGetLastError());
+        }
+        launched = b1;
+        return TR_OK;
+    };
+    /* autotuning: two trial batches per candidate, timed with events */
+    size_t best = 0;
+    double tune_ms = 0.0;
+    uint32_t timed = 0;
+    if (cands.size() > 1) {
+        double best_rate = -1.0;
+        for (size_t c = 0; c < cands.size() && pos_r < count; ++c) {
+            const uint32_t b0 = (uint32_t)batches.size();
+            if (!cut_batches(reqs, path_off, pix_off, s->n_gi, cands[c].first, 2, &pos_r, &pos_lp, &batches))
+                return fail(TR_ERROR_INVALID_ARG, "a pixel's samples exceed the batch size");
+            uint64_t paths = 0;
+            for (uint32_t k = b0; k < batches.size(); ++k) paths += batches[k].n_slots;
+            hipEvent_t a = get_event(s), b = get_event(s);
+            if (!a || !b) return fail(TR_ERROR_DEVICE, "event creation failed");
+            TR_HIP(hipEventRecord(a, stream));
+            if ((st = run(cands[c].first, cands[c].second)) != TR_OK) return st;
+            TR_HIP(hipEventRecord(b, stream));
+            TR_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            TR_HIP(hipEventElapsedTime(&ms, a, b));
+            s->ev_pool.push_back(a);
+            s->ev_pool.push_back(b);
+            tune_ms += ms;
+            ++timed;
+            const double rate = (double)paths / std::max(1e-6, (double)ms);
+            if (rate > best_rate) {
+                best_rate = rate;
+                best = c;
+            }
+        }
+    }
+    /* the rest with the chosen configuration */
+    if (!cut_batches(reqs, path_off, pix_off, s->n_gi, cands[best].first, 0xffffffffu, &pos_r, &pos_lp, &batches))
+        return fail(TR_ERROR_INVALID_ARG, "a pixel's samples exceed the batch size");
+    if ((st = run(cands[best].first, cands[best].second)) != TR_OK) return st;
+    launch_request_outputs(stream, p.req, s->n_gi, d_accum.ptr, d_mean.ptr, out_image ? d_image.ptr : nullptr,
+                           total_pix);
+    TR_HIP(hipGetLastError());
+    if (out_mean)
+        TR_HIP(hipMemcpyAsync(out_mean, d_mean.ptr, count * 3 * sizeof(float), hipMemcpyDeviceToHost, stream));
+    if (out_image)
+        TR_HIP(hipMemcpyAsync(out_image, d_image.ptr, total_pix * 3 * sizeof(float), hipMemcpyDeviceToHost, stream));
+    TR_HIP(hipStreamSynchronize(stream));
+    if (out_stats) {
+        memset(out_stats, 0, sizeof(*out_stats));
+        out_stats->paths = total_paths;
+        out_stats->pixels = total_pix;
+        out_stats->batches = (uint32_t)batches.size();
+        out_stats->pool_slots = cands[best].first;
+        out_stats->trace_waves_per_cu = cands[best].second;
+        out_stats->candidates = timed;
+        out_stats->tune_ms = tune_ms;
+        out_stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return TR_OK;
 }
 
 tr_status tr_scene_get_stats(const tr_scene* s, tr_stats* out)

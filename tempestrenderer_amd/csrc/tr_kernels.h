@@ -18,6 +18,32 @@
 
 namespace trk {
 
+/* Batched render requests (tr_scene_render_requests): camera, image size,
+ * camera samples and the direction of light 0 per request.  The paths of
+ * all requests form one index space (request-major, then pixel, camera
+ * sample, GI split), cut into pool batches at pixel boundaries; the
+ * kernels read the current batch through a device-side cursor, so one
+ * captured graph serves every batch. */
+struct ReqRec {
+    float vpi[16];
+    uint32_t width, height, spp;
+    float light_dir[3];       /* DirectionalLight::Direction of light 0 */
+};
+struct ReqBatch {
+    uint64_t path_base;       /* first path of the batch */
+    uint64_t pix_base;        /* first pixel (over all requests) */
+    uint32_t n_slots;         /* paths of the batch */
+    uint32_t n_pix;           /* pixels of the batch */
+};
+struct ReqView {
+    const ReqRec* reqs;
+    const uint64_t* path_off; /* [n + 1]: first path of each request */
+    const uint64_t* pix_off;  /* [n + 1]: first pixel of each request */
+    const ReqBatch* batches;
+    const uint32_t* cursor;   /* index of the current batch */
+    uint32_t n;               /* requests; 0 outside request mode */
+};
+
 struct FrameParams {
     SceneView sc;
     uint32_t width, height;
@@ -40,6 +66,7 @@ struct FrameParams {
     uint32_t cache_nee;      /* 1: next-event estimate at the query vertex, network predicts the rest */
     float box_lo[3];         /* scene box for the position input */
     float box_rcp[3];
+    ReqView req;             /* request mode (req.n > 0) */
 };
 
 /* Path flags of cache mode: training path, next vertex is the query
@@ -88,6 +115,7 @@ struct PathPool {
     uint32_t* seg_n;
     uint32_t seg_cap;   /* segments per queue */
     uint32_t* pflags;   /* cache mode path flags */
+    uint32_t* slot_req; /* request mode: request of each slot */
     uint2* spill;       /* traversal stack spill, per persistent thread */
     uint32_t capacity;
 };
@@ -137,6 +165,16 @@ void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, co
                    const LaunchEvents* ev = nullptr);
 void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
                        uint64_t slot_base, uint32_t n_pixels, const LaunchEvents* ev = nullptr);
+/* Request mode: per-pixel sums of the current batch into accum (one
+ * thread per pixel, up to max_pix), then the cursor moves to the next
+ * batch. */
+void launch_accumulate_requests(hipStream_t s, const FrameParams& p, const PathPool& pool, double* accum,
+                                uint32_t max_pix, uint32_t* cursor, const LaunchEvents* ev = nullptr);
+/* Request mode outputs: mean RGB of each request over its pixels and
+ * samples; optionally the per-pixel mean image (float RGB, all requests
+ * in order). */
+void launch_request_outputs(hipStream_t s, const ReqView& rv, uint32_t n_gi, const double* accum, float* mean,
+                            float* image, uint64_t n_pix_total);
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
                        uint2* spill, uint32_t grid);
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,

@@ -104,9 +104,47 @@ __device__ inline uint32_t segment_push(const PathPool& pool, bool e0, bool e1, 
     return pos2;
 }
 
+/* Request mode: the request whose range of `off` (path or pixel offsets,
+ * increasing, every request non-empty) holds q. */
+__device__ inline uint32_t find_request(const uint64_t* off, uint32_t n, uint64_t q)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* Request mode coordinates of slot `slot` of the current batch, a path of
+ * request r: pixel over all requests (the random-number key), pixel within
+ * the request, camera sample and GI split. */
+__device__ inline void request_coords(const FrameParams& p, uint32_t r, uint32_t slot, uint32_t* gpix,
+                                      uint32_t* lpix, uint32_t* cs, uint32_t* split)
+{
+    const ReqBatch& b = p.req.batches[*p.req.cursor];
+    const uint64_t local = b.path_base + slot - p.req.path_off[r];
+    const uint64_t per_pixel = (uint64_t)p.req.reqs[r].spp * p.n_gi;
+    const uint64_t lp = local / per_pixel;
+    const uint32_t s = (uint32_t)(local - lp * per_pixel);
+    *lpix = (uint32_t)lp;
+    *gpix = (uint32_t)(p.req.pix_off[r] + lp);
+    *cs = s / p.n_gi;
+    *split = s % p.n_gi;
+}
+
 __device__ inline void slot_coords(const FrameParams& p, const BatchOut& out, uint64_t slot, uint32_t* pixel,
                                    uint32_t* cs, uint32_t* split, uint64_t* pseq)
 {
+    if (p.req.n) {
+        const ReqBatch& b = p.req.batches[*p.req.cursor];
+        const uint32_t r = find_request(p.req.path_off, p.req.n, b.path_base + slot);
+        uint32_t lp;
+        request_coords(p, r, (uint32_t)slot, pixel, &lp, cs, split);
+        *pseq = lp;
+        return;
+    }
     uint64_t per_pixel = (uint64_t)p.spp * p.n_gi;
     uint64_t ps = slot / per_pixel;
     *pseq = ps;
@@ -163,24 +201,41 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, Ba
                                                 uint32_t n_slots, CacheRecords rec)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p.req.n) n_slots = p.req.batches[*p.req.cursor].n_slots;
     if (i == 0) pool.counters[0] = n_slots;
     if (i >= n_slots) return;
-    uint32_t pixel, cs, split;
+    uint32_t pixel, cs, split, lpix;
     uint64_t pseq;
-    slot_coords(p, out, slot_base + i, &pixel, &cs, &split, &pseq);
-    uint32_t x = pixel % p.width, y = pixel / p.width;
+    uint32_t width = p.width, height = p.height, spp = p.spp;
+    const float* m = p.vpi;
+    if (p.req.n) {
+        /* request mode: the slot's request, its camera and image size */
+        const ReqBatch& b = p.req.batches[*p.req.cursor];
+        const uint32_t r = find_request(p.req.path_off, p.req.n, b.path_base + i);
+        pool.slot_req[i] = r;
+        const ReqRec& R = p.req.reqs[r];
+        request_coords(p, r, i, &pixel, &lpix, &cs, &split);
+        pseq = lpix;
+        width = R.width;
+        height = R.height;
+        spp = R.spp;
+        m = R.vpi;
+    } else {
+        slot_coords(p, out, slot_base + i, &pixel, &cs, &split, &pseq);
+        lpix = pixel;
+    }
+    uint32_t x = lpix % width, y = lpix / width;
     float sx, sy;
-    if (p.spp > 1) {
+    if (spp > 1) {
         /* camera jitter draws with split 0: all GI splits share the camera ray */
         U4 r = philox(pixel, cs, (uint32_t)STREAM_CAMERA, 0u, p.key0, p.key1);
-        sx = 2.0f * ((float)x + u01(r.a)) / (float)p.width - 1.0f;
-        sy = 2.0f * ((float)y + u01(r.b)) / (float)p.height - 1.0f;
+        sx = 2.0f * ((float)x + u01(r.a)) / (float)width - 1.0f;
+        sy = 2.0f * ((float)y + u01(r.b)) / (float)height - 1.0f;
     } else {
-        sx = 2.0f * (float)x / (float)(p.width - 1) - 1.0f;
-        sy = 2.0f * (float)y / (float)(p.height - 1) - 1.0f;
+        sx = 2.0f * (float)x / (float)(width - 1) - 1.0f;
+        sy = 2.0f * (float)y / (float)(height - 1) - 1.0f;
     }
     /* Matrix4 * Vector4 (matrix4.hh:858-880), ToVector3 (vector4.hh:125-130) */
-    const float* m = p.vpi;
     float p0[4], p1[4];
     for (int r = 0; r < 4; ++r) {
         p0[r] = m[r] * sx;
@@ -304,7 +359,15 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         const SceneView& sc = p.sc;
         uint32_t pixel, cs, split;
         uint64_t pseq;
-        slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+        uint32_t req = 0;
+        if (p.req.n) {
+            uint32_t lp;
+            req = pool.slot_req[slot];
+            request_coords(p, req, slot, &pixel, &lp, &cs, &split);
+            pseq = lp;
+        } else {
+            slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+        }
         Rng rng;
         rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
         const bool unbiased = p.estimator == TR_ESTIMATOR_UNBIASED;
@@ -431,7 +494,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 if ((vertex != 0 || split == 0) && (!cache_query || p.cache_nee)) {
                     uint32_t ne = 0;
                     for (uint32_t li = 0; li < sc.n_lights; ++li) {
-                        const LightRec& L = sc.lights[li];
+                        LightRec L = sc.lights[li];
+                        if (p.req.n && li == 0) {
+                            /* request mode: the request's direction of light 0 */
+                            const float* d = p.req.reqs[req].light_dir;
+                            L.src.direction[0] = d[0];
+                            L.src.direction[1] = d[1];
+                            L.src.direction[2] = d[2];
+                        }
                         bool multi = multi_sample_light(L, unbiased);
                         uint32_t samples = multi ? ls : 1u;
                         for (uint32_t k = 0; k < samples; ++k) {
@@ -521,6 +591,78 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         q.thr[2 * (size_t)q.stride + qpos] = qthr.z;
         q.slot[qpos] = slot;
     }
+}
+
+/* ---------------- request mode: accumulation and outputs ---------------- */
+
+/* Per-pixel sums of the current batch (pixels b.pix_base ..), in sample
+ * order, each added once to the pixel's FP64 sum: batches end at pixel
+ * boundaries, so the result does not depend on the batch size. */
+__global__ __launch_bounds__(256) void k_accumulate_req(FrameParams p, PathPool pool, double* accum,
+                                                        const uint32_t* cursor)
+{
+    const ReqBatch& b = p.req.batches[*cursor];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n_pix) return;
+    const uint64_t gp = b.pix_base + i;
+    const uint32_t r = find_request(p.req.pix_off, p.req.n, gp);
+    const uint32_t spp = p.req.reqs[r].spp;
+    const uint64_t per_pixel = (uint64_t)spp * p.n_gi;
+    const uint32_t s_first = (uint32_t)(p.req.path_off[r] + (gp - p.req.pix_off[r]) * per_pixel - b.path_base);
+    const float rcp_gi = 1.0f / (float)p.n_gi;
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (uint32_t c = 0; c < spp; ++c) {
+        const uint32_t s0 = s_first + c * p.n_gi;
+        V3 g = ld4(pool.global[s0]);
+        for (uint32_t j = 1; j < p.n_gi; ++j) g = add(g, ld4(pool.global[s0 + j]));
+        V3 v = add(ld4(pool.local[s0]), muls(g, rcp_gi));
+        sum[0] += (double)v.x;
+        sum[1] += (double)v.y;
+        sum[2] += (double)v.z;
+    }
+    accum[gp * 3 + 0] += sum[0];
+    accum[gp * 3 + 1] += sum[1];
+    accum[gp * 3 + 2] += sum[2];
+}
+
+/* Moves the request cursor to the next batch (after the batch's last
+ * kernel, in stream order). */
+__global__ void k_req_next(uint32_t* cursor)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) cursor[0] = cursor[0] + 1u;
+}
+
+/* Mean RGB of request blockIdx.x over its pixels and camera samples: the
+ * block sums its pixels in a fixed order (strided per thread, then a tree). */
+__global__ __launch_bounds__(256) void k_req_mean(ReqView rv, const double* accum, float* mean)
+{
+    __shared__ double red[3][256];
+    const uint32_t r = blockIdx.x;
+    const uint64_t p0 = rv.pix_off[r], p1 = rv.pix_off[r + 1];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint64_t q = p0 + threadIdx.x; q < p1; q += blockDim.x)
+        for (int k = 0; k < 3; ++k) s[k] += accum[q * 3 + k];
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n = (double)(p1 - p0) * (double)rv.reqs[r].spp;
+        for (int k = 0; k < 3; ++k) mean[r * 3 + k] = (float)(red[k][0] / n);
+    }
+}
+
+/* Per-pixel mean radiance of every request, in request order. */
+__global__ __launch_bounds__(256) void k_req_image(ReqView rv, const double* accum, float* image, uint64_t n_pix)
+{
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_pix) return;
+    const uint32_t r = find_request(rv.pix_off, rv.n, q);
+    const double rcp = 1.0 / (double)rv.reqs[r].spp;
+    for (int k = 0; k < 3; ++k) image[q * 3 + k] = (float)(accum[q * 3 + k] * rcp);
 }
 
 /* ---------------- queue compaction ---------------- */
@@ -901,6 +1043,21 @@ void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool
                        uint64_t slot_base, uint32_t n_pixels, const LaunchEvents* ev)
 {
     launch_k(k_accumulate, (n_pixels + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_pixels);
+}
+
+void launch_accumulate_requests(hipStream_t s, const FrameParams& p, const PathPool& pool, double* accum,
+                                uint32_t max_pix, uint32_t* cursor, const LaunchEvents* ev)
+{
+    launch_k(k_accumulate_req, (max_pix + 255) / 256, 256, s, ev, p, pool, accum, (const uint32_t*)cursor);
+    k_req_next<<<1, 64, 0, s>>>(cursor);
+}
+
+void launch_request_outputs(hipStream_t s, const ReqView& rv, uint32_t n_gi, const double* accum, float* mean,
+                            float* image, uint64_t n_pix_total)
+{
+    (void)n_gi;
+    if (mean) k_req_mean<<<rv.n, 256, 0, s>>>(rv, accum, mean);
+    if (image) k_req_image<<<(uint32_t)((n_pix_total + 255) / 256), 256, 0, s>>>(rv, accum, image, n_pix_total);
 }
 
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n, uint2* spill,

@@ -275,6 +275,37 @@ class RayTracerScene:
                                              _fptr(rgb), hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
         return rgb.reshape(n, 3), hsh
 
+    def renderRequests(self, requests, use_graph=True, autotune=False, pool_slots=0, trace_waves_per_cu=0,
+                       images=False):
+        """Batched render requests (samples/virtual-gonioreflectometer.cc:161-189
+        as one call).  `requests`: sequence of dicts with view_proj_inv (16),
+        width, height, samples_camera, light_direction (3).  Returns (mean
+        radiance [n, 3], per-request images [h, w, 3] or None, stats dict)."""
+        n = len(requests)
+        arr = (_abi.tr_request * n)()
+        total_px = 0
+        for i, q in enumerate(requests):
+            r = arr[i]
+            r.view_proj_inv[:] = [float(v) for v in np.asarray(q["view_proj_inv"], dtype=np.float32).reshape(16)]
+            r.width, r.height, r.samples_camera = int(q["width"]), int(q["height"]), int(q["samples_camera"])
+            r.light_direction[:] = [float(v) for v in q["light_direction"]]
+            total_px += r.width * r.height
+        opt = _abi.tr_request_options(int(bool(use_graph)), int(bool(autotune)), int(pool_slots),
+                                      int(trace_waves_per_cu))
+        mean = np.zeros(n * 3, dtype=np.float32)
+        img = np.zeros(total_px * 3, dtype=np.float32) if images else None
+        st = _abi.tr_request_stats()
+        check(self.lib.tr_scene_render_requests(self.h, n, arr, ctypes.byref(opt), _fptr(mean),
+                                                _fptr(img) if images else None, ctypes.byref(st)))
+        out_imgs = None
+        if images:
+            out_imgs, o = [], 0
+            for q in requests:
+                w, h = int(q["width"]), int(q["height"])
+                out_imgs.append(img[o * 3:(o + w * h) * 3].reshape(h, w, 3))
+                o += w * h
+        return mean.reshape(n, 3), out_imgs, {k: getattr(st, k) for k, _ in st._fields_}
+
     # radiance cache
     def setRadianceCache(self, enabled=True, train=True, train_paths=65536, batch=16384, rough_alpha=0.2, seed=1,
                          query_nee=False):

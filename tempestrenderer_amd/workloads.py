@@ -351,3 +351,75 @@ def primary_rays(desc, stride=1):
     rays[:, 4:7] = dvec
     rays[:, 7] = np.inf
     return rays
+
+
+def ortho_view_proj_inv(eye_dir, half_extent, dist=5.0, near=0.1, far=20.0):
+    """Inverse of an orthographic projection (GL clip z in [-1, 1], the
+    reference's OrthoMatrix) times a lookAt from dist * eye_dir towards the
+    origin, as 16 column-major floats; square view of half size
+    half_extent."""
+    e = np.asarray(eye_dir, dtype=np.float64)
+    e = e / np.linalg.norm(e)
+    eye = dist * e
+    f = -e
+    up = np.array([0.0, 1.0, 0.0]) if abs(f[1]) < 0.99 else np.array([0.0, 0.0, 1.0])
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    view[:3, 3] = -view[:3, :3] @ eye
+    proj = np.eye(4)
+    proj[0, 0] = proj[1, 1] = 1.0 / half_extent
+    proj[2, 2] = -2.0 / (far - near)
+    proj[2, 3] = -(far + near) / (far - near)
+    inv = np.linalg.inv(proj @ view)
+    return np.ascontiguousarray(inv.T.reshape(16).astype(np.float32))
+
+
+def hemisphere_grid(n=9):
+    """n x n directions over the upper (+y) hemisphere from the square grid
+    tc in [-1, 1]^2 (the reference's request grid,
+    samples/virtual-gonioreflectometer.cc:167-171): (tc.x, cos, tc.y)
+    normalised, cos = sqrt(max(1 - |tc|^2, 0.05))."""
+    out = []
+    for j in range(n):
+        for i in range(n):
+            tx, ty = 2.0 * i / (n - 1) - 1.0, 2.0 * j / (n - 1) - 1.0
+            c = math.sqrt(max(1.0 - tx * tx - ty * ty, 0.05))
+            v = np.array([tx, c, ty])
+            out.append(tuple(float(x) for x in v / np.linalg.norm(v)))
+    return out
+
+
+def gonio(grid=64, n_dirs=9, seed=1, max_requests=None):
+    """Batched virtual gonioreflectometer: a grid x grid fBm heightfield
+    patch (2 (grid-1)^2 triangles; GGX alpha 0.3 with the reference
+    sample's Diffuse 0.05 / Specular 0.95 / Fresnel of IOR 1.56,
+    samples/virtual-gonioreflectometer.cc:99-127), a directional light of
+    radiance 10, depth 1, Russian roulette 0.9; requests for every
+    (light, view) pair of the n_dirs^2 hemisphere grid, an orthographic
+    camera per view, image side 2^U{5..8} and camera samples from
+    {4, 8, 16, 32, 64} drawn by Philox seed `seed`.  Returns
+    (SceneDesc, requests)."""
+    verts, tris, _ = heightfield_mesh(grid, amplitude=0.1, seed=seed)
+    d = SceneDesc("gonio", 32, 32, ortho_view_proj_inv((0.0, 1.0, 0.0), 1.5))
+    f0 = ((1.56 - 1.0) / (1.56 + 1.0)) ** 2          # ComputeReflectionCoefficient(1.56)
+    m = dict(model=GGX, diffuse=(0.05, 0.05, 0.05), specular=(0.95, 0.95, 0.95),
+             specular_power=(alpha_to_specular_power(0.3),) * 2, fresnel=(f0, 0.0))
+    d.meshes.append((verts, tris, [(m, 0, tris.size)], False))
+    dirs = hemisphere_grid(n_dirs)
+    d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=dirs[len(dirs) // 2], radiance=(10.0, 10.0, 10.0)))
+    d.settings.update(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=0.9,
+                      background=(0.0, 0.0, 0.0), seed=1)
+    rng = np.random.Generator(np.random.Philox(seed))
+    reqs = []
+    for li, ld in enumerate(dirs):
+        for vi, vd in enumerate(dirs):
+            side = 1 << int(rng.integers(5, 9))
+            spp = int(rng.choice([4, 8, 16, 32, 64]))
+            reqs.append(dict(view_proj_inv=ortho_view_proj_inv(vd, 1.5), width=side, height=side,
+                             samples_camera=spp, light_direction=ld))
+            if max_requests is not None and len(reqs) >= max_requests:
+                return d, reqs
+    return d, reqs

@@ -65,6 +65,56 @@ def cpu_baseline(desc, rows, threads):
     return samples / secs / 1e6, secs, samples, r0
 
 
+def gonio_bench(cpu_paths=2.0e8, threads=16, cpu=True):
+    """The batched virtual gonioreflectometer: all 6561 requests in one
+    tr_scene_render_requests call (graph replay, autotuned), timed after a
+    small warm-up call; beside it the CPU oracle rendering the first
+    requests one after another, as the reference's request loop does, until
+    about `cpu_paths` paths."""
+    import copy
+    from tempestrenderer_amd import workloads
+    from tempestrenderer_amd.scene import RayTracerScene
+    d, reqs = workloads.gonio()
+    g = RayTracerScene(32, 32, d.vpi)
+    d.populate(g, gpu_material)
+    g.commitScene()
+    g.renderRequests(reqs[:64], use_graph=True)
+    t0 = time.perf_counter()
+    _, _, st = g.renderRequests(reqs, use_graph=True, autotune=True)
+    secs = time.perf_counter() - t0
+    g.close()
+    out = {"workload": "gonioreflectometer: 64x64 fBm patch (7938 tris), GGX alpha 0.3, 81 x 81 light/view "
+                       "requests, 32^2-256^2 px, 4-64 spp (Philox seed 1), depth 1, RR 0.9",
+           "requests": len(reqs), "paths": st["paths"], "seconds": round(secs, 4),
+           "requests_per_s": round(len(reqs) / secs, 1), "msamples_per_s": round(st["paths"] / secs / 1e6, 1),
+           "batches": st["batches"], "tuned": {"pool_slots": st["pool_slots"],
+                                               "trace_waves_per_cu": st["trace_waves_per_cu"],
+                                               "candidates": st["candidates"], "trial_ms": round(st["tune_ms"], 2)},
+           "graph": True}
+    if cpu:
+        import oracle
+        done, paths, t_cpu, base = 0, 0, 0.0, 0
+        while done < len(reqs) and paths < cpu_paths:
+            q = reqs[done]
+            d2 = copy.copy(d)
+            d2.lights = [dict(d.lights[0], direction=tuple(q["light_direction"]))]
+            o = oracle.OracleScene(q["width"], q["height"], q["view_proj_inv"])
+            d2.populate(o, oracle.material)
+            o.setSamplesCamera(q["samples_camera"])
+            o.setPixelKeyBase(base)
+            o.commitScene()
+            _, _, secs_r = o.renderRows(0, q["height"], frame=0, fp64=False, threads=threads)
+            t_cpu += secs_r
+            paths += q["width"] * q["height"] * q["samples_camera"]
+            base += q["width"] * q["height"]
+            done += 1
+        out["cpu_baseline"] = {"value": round(paths / t_cpu / 1e6, 3), "unit": "Msamples/s", "cores": threads,
+                               "kind": "port", "requests_per_s": round(done / t_cpu, 2),
+                               "sample": "the first %d requests one after another (%d paths), %.1f s" % (
+                                   done, paths, t_cpu)}
+    return out
+
+
 REF_IMAGE = os.path.join(ROOT, "tests", "golden", "box_unbiased_fp64.npz")
 
 
@@ -179,6 +229,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE runs (plain vs radiance cache)")
     ap.add_argument("--rmse-only", action="store_true", help="only the time-to-RMSE runs, printed with their traces")
+    ap.add_argument("--no-gonio", action="store_true", help="skip the batched gonioreflectometer run")
+    ap.add_argument("--gonio-only", action="store_true", help="only the batched gonioreflectometer run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -190,6 +242,9 @@ def main():
         calibrate_copy()
         return
     args.steps = max(1, args.steps)
+    if args.gonio_only:
+        log(json.dumps(gonio_bench(cpu=not args.no_cpu)))
+        return
     if args.rmse_only:
         for cached, qn in ((False, False), (True, False), (True, True)):
             res, trace = time_to_rmse(cached, query_nee=qn)
@@ -270,6 +325,11 @@ def main():
                                        if plain["off_emitter"]["seconds"] and cached["off_emitter"]["seconds"]
                                        else None)}
 
+    gonio = None
+    if not args.no_gonio:
+        gonio = gonio_bench(cpu=not args.no_cpu, threads=min(16, os.cpu_count() or 1))
+        log("gonioreflectometer: %s" % json.dumps(gonio))
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -318,6 +378,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
+        "gonioreflectometer": gonio,
     }
     print(json.dumps(out), flush=True)
 

@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
 
 /* ---------------- shading ---------------- */
 
-template <bool VOL>
+template <bool VOL, bool CACHE>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
                                                uint32_t vertex, int cur, int nxt, CacheRecords rec)
 {
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         V3 global = ld4(pool.global[slot]);
         bool write_local = false;
         uint32_t pf = 0;
-        if (p.cache) {
+        if (CACHE) {
             pf = pool.pflags[slot];
             if (pf & PF_BEFORE) {
                 /* the vertex after a training path's query vertex: the
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                  * the radiance beyond the vertex's own next-event estimate)
                  * and the path ends; a training path records it instead
                  * and goes on */
-                if (p.cache) {
+                if (CACHE) {
                     if ((pf & PF_ROUGH) && vertex > 0 && vertex < p.max_depth) {
                         float in[kMlpIn];
                         cache_inputs(p, pos, dir, sd, in);
@@ -494,19 +494,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 if ((vertex != 0 || split == 0) && (!cache_query || p.cache_nee)) {
                     uint32_t ne = 0;
                     for (uint32_t li = 0; li < sc.n_lights; ++li) {
-                        LightRec L = sc.lights[li];
-                        if (p.req.n && li == 0) {
-                            /* request mode: the request's direction of light 0 */
-                            const float* d = p.req.reqs[req].light_dir;
-                            L.src.direction[0] = d[0];
-                            L.src.direction[1] = d[1];
-                            L.src.direction[2] = d[2];
-                        }
+                        const LightRec& L = sc.lights[li];
+                        /* request mode: the request's direction of light 0 */
+                        const float* dir_override = p.req.n && li == 0 ? p.req.reqs[req].light_dir : nullptr;
                         bool multi = multi_sample_light(L, unbiased);
                         uint32_t samples = multi ? ls : 1u;
                         for (uint32_t k = 0; k < samples; ++k) {
                             U4 xr = rng.draw(vertex, STREAM_LIGHT, (li << 16) | k);
-                            LightSample smp = sample_light(sc, L, pos, xr.a, xr.b, xr.c, unbiased);
+                            LightSample smp = sample_light(sc, L, pos, xr.a, xr.b, xr.c, unbiased, dir_override);
                             if (smp.pdf == 0.0f) continue;
                             Sample lsd = sd;
                             lsd.wi = smp.dir;
@@ -571,7 +566,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                         pool.ray_d[slot] = st4(sd.wi, 0.0f);
                         pool.thr[slot] = st4(T, nb);
                         extend = true;
-                        if (p.cache && !(pf & PF_RECORDED) && cache_roughness(sd.mat) >= p.rough_alpha)
+                        if (CACHE && !(pf & PF_RECORDED) && cache_roughness(sd.mat) >= p.rough_alpha)
                             pool.pflags[slot] = pf | PF_ROUGH;
                     }
                 }
@@ -582,7 +577,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         if (write_local) pool.local[slot] = st4(local, 0.0f);
     }
     const uint32_t qpos = segment_push(pool, extend, shadow, cache_query, slot);
-    if (cache_query) {
+    if (CACHE && cache_query) {
         const MlpQueries& q = rec.queries;
 #pragma unroll
         for (int k = 0; k < kMlpIn; ++k) q.in[(size_t)k * q.stride + qpos] = qin[k];
@@ -991,10 +986,14 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                   uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev)
 {
-    if (p.sc.n_volumes > 0)
-        launch_k(k_shade<true>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-    else
-        launch_k(k_shade<false>, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+    const uint32_t grid = (n_slots + 255) / 256;
+    if (p.sc.n_volumes > 0) {
+        if (p.cache) launch_k(k_shade<true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        else launch_k(k_shade<true, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+    } else {
+        if (p.cache) launch_k(k_shade<false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        else launch_k(k_shade<false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+    }
 }
 
 void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int nxt, bool queries,

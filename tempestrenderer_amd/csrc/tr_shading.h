@@ -504,14 +504,14 @@ TR_HD bool multi_sample_light(const LightRec& L, bool unbiased)
  * The unbiased estimator reports the mesh-light triangle's true
  * solid-angle density. */
 TR_HD LightSample sample_light(const SceneView& sc, const LightRec& L, V3 pos, uint32_t x0, uint32_t x1, uint32_t x2,
-                               bool unbiased = false)
+                               bool unbiased = false, const float* dir_override = nullptr)
 {
     LightSample r;
     const tr_light& l = L.src;
     float ra = u01(x1), rb = u01(x2);
     if (l.type == TR_LIGHT_DIRECTIONAL) {
         r.dist = INFINITY;
-        r.dir = ld3(l.direction);
+        r.dir = ld3(dir_override ? dir_override : l.direction);
         r.radiance = ld3(l.radiance);
         r.pdf = 1.0f;
         r.geom = TR_INVALID_GEOMETRY;

@@ -115,6 +115,58 @@ def gonio_bench(cpu_paths=2.0e8, threads=16, cpu=True):
     return out
 
 
+def volume_bench(cpu_rows=128, threads=16, cpu=True):
+    """The heterogeneous SGGX volume: 256^3 binary16 grid (mean extinction
+    4), 1024x1024, 128 camera samples, depth 32, delta tracking.  One timed
+    frame; a counting frame (8 camera samples) gives tentative collisions
+    per path for the gather-byte numerator (8 corners x 8 B each)."""
+    from tempestrenderer_amd import workloads
+    from tempestrenderer_amd.scene import RayTracerScene
+    d = workloads.volume()
+    st = d.settings
+    g = RayTracerScene(d.width, d.height, d.vpi)
+    d.populate(g, gpu_material)
+    g.commitScene()
+    g.setSamplesCamera(8)
+    g.setCounting(True)
+    g.drawOnce()
+    cnt = g.stats()
+    g.setCounting(False)
+    g.setSamplesCamera(st["samples_camera"])
+    g.repaint()
+    g.drawOnce()                                 # warm-up
+    g.repaint()
+    g.setTiming(True)
+    g.drawOnce()
+    tim = g.stats()
+    g.setTiming(False)
+    g.repaint()
+    t0 = time.perf_counter()
+    g.drawOnce()
+    secs = time.perf_counter() - t0
+    g.close()
+    paths = d.width * d.height * st["samples_camera"]
+    steps_per_path = cnt["volume_steps"] / max(1, cnt["paths"])
+    gather = steps_per_path * paths * 64.0
+    out = {"workload": "SGGX fibre volume 256^3 binary16 (mean extinction 4), %dx%d, %d spp, depth %d, albedo 0.9, "
+                       "delta tracking, env light" % (d.width, d.height, st["samples_camera"], st["max_depth"]),
+           "msamples_per_s": round(paths / secs / 1e6, 1), "ms_per_frame": round(secs * 1e3, 2),
+           "tentative_collisions_per_path": round(steps_per_path, 2),
+           "trace_ms": round(tim["trace_ms"], 2), "shade_ms": round(tim["shade_ms"], 2),
+           "gather_gbs": round(gather / (tim["trace_ms"] / 1e3) / 1e9, 1)}
+    if cpu:
+        import oracle
+        o = oracle.OracleScene(d.width, d.height, d.vpi)
+        d.populate(o, oracle.material)
+        o.commitScene()
+        r0 = (d.height - cpu_rows) // 2
+        _, _, cs = o.renderRows(r0, r0 + cpu_rows, frame=0, fp64=False, threads=threads)
+        cp = cpu_rows * d.width * st["samples_camera"]
+        out["cpu_baseline"] = {"value": round(cp / cs / 1e6, 3), "unit": "Msamples/s", "cores": threads, "kind": "port",
+                               "sample": "rows %d-%d of the frame (%d paths), %.1f s" % (r0, r0 + cpu_rows - 1, cp, cs)}
+    return out
+
+
 REF_IMAGE = os.path.join(ROOT, "tests", "golden", "box_unbiased_fp64.npz")
 
 
@@ -231,6 +283,8 @@ def main():
     ap.add_argument("--rmse-only", action="store_true", help="only the time-to-RMSE runs, printed with their traces")
     ap.add_argument("--no-gonio", action="store_true", help="skip the batched gonioreflectometer run")
     ap.add_argument("--gonio-only", action="store_true", help="only the batched gonioreflectometer run")
+    ap.add_argument("--no-volume", action="store_true", help="skip the volume run")
+    ap.add_argument("--volume-only", action="store_true", help="only the volume run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -244,6 +298,9 @@ def main():
     args.steps = max(1, args.steps)
     if args.gonio_only:
         log(json.dumps(gonio_bench(cpu=not args.no_cpu)))
+        return
+    if args.volume_only:
+        log(json.dumps(volume_bench(cpu=not args.no_cpu)))
         return
     if args.rmse_only:
         for cached, qn in ((False, False), (True, False), (True, True)):
@@ -312,7 +369,7 @@ def main():
     if not args.no_rmse and os.path.exists(REF_IMAGE):
         plain, _ = time_to_rmse(False)
         log("time-to-RMSE plain: %s" % json.dumps(plain))
-        cached, _ = time_to_rmse(True)
+        cached, _ = time_to_rmse(True, query_nee=True)
         log("time-to-RMSE cached: %s" % json.dumps(cached))
         ref = np.load(REF_IMAGE)
         ttr = {"workload": "box 256x256, depth 4, unbiased estimator, 16 camera samples per frame",
@@ -324,6 +381,11 @@ def main():
                "speedup_off_emitter": (round(plain["off_emitter"]["seconds"] / cached["off_emitter"]["seconds"], 3)
                                        if plain["off_emitter"]["seconds"] and cached["off_emitter"]["seconds"]
                                        else None)}
+
+    vol = None
+    if not args.no_volume:
+        vol = volume_bench(cpu=not args.no_cpu, threads=min(16, os.cpu_count() or 1))
+        log("volume: %s" % json.dumps(vol))
 
     gonio = None
     if not args.no_gonio:
@@ -379,6 +441,7 @@ def main():
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
         "gonioreflectometer": gonio,
+        "volume": vol,
     }
     print(json.dumps(out), flush=True)
 

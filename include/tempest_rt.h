@@ -153,6 +153,7 @@ typedef struct tr_mesh_options {
 typedef struct tr_settings {
     uint32_t data_pool_size;   /* accepted for compatibility, unused */
     uint32_t path_pool_size;   /* paths resident per batch; 0 = automatic */
+    uint32_t trace_waves_per_cu; /* persistent traversal waves per CU, 1..32; 0 = 16 */
 } tr_settings;
 
 /* A ray of a fixed ray set and its closest hit. */

@@ -227,7 +227,11 @@ SceneView scene_view(const tr_scene* s)
     return v;
 }
 
-uint32_t persistent_grid(const tr_scene* s) { return (uint32_t)std::max(1, s->cu_count) * 16u; }
+uint32_t persistent_grid(const tr_scene* s)
+{
+    const uint32_t waves = s->settings.trace_waves_per_cu ? s->settings.trace_waves_per_cu : 16u;
+    return (uint32_t)std::max(1, s->cu_count) * waves;
+}
 
 tr_status sync_lights(tr_scene* s)
 {
@@ -668,8 +672,12 @@ tr_status tr_scene_create(uint32_t width, uint32_t height, const float vpi[16], 
     s->width = width;
     s->height = height;
     memcpy(s->vpi, vpi, sizeof(s->vpi));
+    if (settings && settings->trace_waves_per_cu > 32) {
+        delete s;
+        return fail(TR_ERROR_INVALID_ARG, "trace_waves_per_cu above 32");
+    }
     if (settings) s->settings = *settings;
-    else { s->settings.data_pool_size = 32u << 20; s->settings.path_pool_size = 0; }
+    else { s->settings.data_pool_size = 32u << 20; s->settings.path_pool_size = 0; s->settings.trace_waves_per_cu = 0; }
     memset(&s->stats, 0, sizeof(s->stats));
     *out = s;
     return TR_OK;

@@ -340,6 +340,112 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
     }
 }
 
+/* Closest hit for scenes without volumes, with ray refill: a lane whose
+ * ray has finished takes the wave's next ray once at least kRefill lanes
+ * are idle, so the traversal loop runs with mostly full waves instead of
+ * waiting for the slowest of a fixed group of 64 rays.  Wave w traces the
+ * same rays as in k_trace (the queue chunks w, w + grid, ... of 64), in
+ * the same per-ray arithmetic; each iteration visits one node of every
+ * active lane's ray and tests that node's leaves. */
+constexpr uint32_t kRefill = 16;
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_refill(FrameParams p, PathPool pool, int cur,
+                                                              unsigned long long* counts)
+{
+    __shared__ uint2 stk[kStackLds][kTraceBlock];
+    const SceneView& sc = p.sc;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
+    const uint32_t n = pool.counters[cur];
+    const uint32_t* queue = pool.queue[cur];
+    const uint32_t grid = gridDim.x;
+    DevStack st;
+    st.lds = &stk[0][lane];
+    st.spill = pool.spill + (size_t)gtid * kStackSpill;
+    st.sp = 0;
+    uint32_t nv = 0, np = 0, nr = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    /* queue index of the wave's k-th ray */
+    auto ray_index = [&](uint32_t k) -> uint32_t { return (blockIdx.x + (k >> 6) * grid) * kTraceBlock + (k & 63u); };
+    uint32_t next = 0;                /* the wave's next unassigned ray (uniform) */
+    bool active = false;
+    uint32_t slot = 0;
+    RayPre r;
+    Hit best;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const bool more = ray_index(next) < n;
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (more && (n_idle >= kRefill || idle == ~0ull)) {
+            if (!active) {
+                const uint32_t qi = ray_index(next + (uint32_t)__popcll(idle & lt));
+                if (qi < n) {
+                    slot = queue[qi];
+                    float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
+                    float org[3] = {o.x, o.y, o.z}, dir[3] = {d.x, d.y, d.z};
+                    r = ray_setup(org, dir);
+                    best.t = INFINITY;
+                    best.u = best.v = 0.0f;
+                    best.geom = best.prim = TR_INVALID_GEOMETRY;
+                    st.sp = 0;
+                    st.push(0u, (1u << r.octant) | (1u << 8));
+                    active = true;
+                    ++nr;
+                }
+            }
+            next += n_idle;
+        }
+        if (__ballot(active) == 0ull) {
+            if (ray_index(next) < n) continue;
+            break;
+        }
+        if (active) {
+            /* one node of the ray's traversal (traverse_closest's loop body) */
+            const uint32_t oct = r.octant;
+            uint32_t base, masks;
+            st.pop(&base, &masks);
+            uint32_t vis = masks & 0xffu;
+            const uint32_t imask = masks >> 8;
+            const uint32_t i = ctz32(vis);
+            vis &= vis - 1u;
+            if (vis) st.push(base, vis | (imask << 8));
+            const uint32_t cslot = i ^ oct;
+            const uint32_t node_index = base + popc8(imask & ((1u << cslot) - 1u));
+            NodeRegs nd = load_node(sc.nodes, node_index);
+            if (COUNT) ++nv;
+            const float tmax = best.geom == TR_INVALID_GEOMETRY ? INFINITY : best.t;
+            const uint32_t hit = intersect_children(nd, r, 0.0f, tmax);
+            const uint32_t nimask = node_imask(nd);
+            const uint32_t inner = hit & nimask;
+            uint32_t lv = to_visit_order(hit & ~nimask, oct);
+            while (lv) {
+                const uint32_t li = ctz32(lv);
+                lv &= lv - 1u;
+                const uint32_t meta = node_meta(nd, li ^ oct);
+                const uint32_t first = nd.d[5] + (meta & 31u);
+                const uint32_t count = meta >> 5;
+                for (uint32_t k = 0; k < count; ++k) {
+                    if (COUNT) ++np;
+                    test_prim(load_prim(sc.prims, first + k), r, 0.0f, INFINITY, &best);
+                }
+            }
+            if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
+            if (st.empty()) {
+                if (best.geom == TR_INVALID_GEOMETRY) best.t = INFINITY;
+                pool.hit[slot] = make_float4(best.t, best.u, best.v, __uint_as_float(best.geom));
+                pool.hit_prim[slot] = best.prim;
+                active = false;
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nv);
+        atomicAdd(&counts[1], (unsigned long long)np);
+        atomicAdd(&counts[2], (unsigned long long)nr);
+    }
+}
+
 /* ---------------- shading ---------------- */
 
 template <bool VOL, bool CACHE>
@@ -976,10 +1082,10 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
     bool vol = p.sc.n_volumes > 0;
     if (counts) {
         if (vol) launch_k(k_trace<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else launch_k(k_trace<true, false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+        else launch_k(k_trace_refill<true>, grid, kTraceBlock, s, ev, p, pool, cur, c);
     } else {
         if (vol) launch_k(k_trace<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else launch_k(k_trace<false, false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+        else launch_k(k_trace_refill<false>, grid, kTraceBlock, s, ev, p, pool, cur, c);
     }
 }
 

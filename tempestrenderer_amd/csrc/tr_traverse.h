@@ -209,13 +209,20 @@ TR_HD uint32_t node_meta(const NodeRegs& n, uint32_t s)
 }
 
 /* Slab test of the 8 children of a node; returns the hit mask in slot
- * order.  tmax is the current far bound (closest hit so far or tfar). */
+ * order.  tmax is the current far bound (closest hit so far or tfar).  A
+ * child plane at o + q s (q the 16-bit code, s a power of two) is crossed
+ * at t = q (s inv) + (o - org) inv: s inv is exact, so one fused
+ * multiply-add per plane replaces the decode, subtraction and product,
+ * with no more roundings than those three. */
 TR_HD uint32_t intersect_children(const NodeRegs& n, const RayPre& r, float tnear, float tmax)
 {
-    float ox = tr_u2f(n.d[0]), oy = tr_u2f(n.d[1]), oz = tr_u2f(n.d[2]);
-    float sx = exp2i((int)(int8_t)(n.d[3] & 0xffu));
-    float sy = exp2i((int)(int8_t)((n.d[3] >> 8) & 0xffu));
-    float sz = exp2i((int)(int8_t)((n.d[3] >> 16) & 0xffu));
+    const float ox = tr_u2f(n.d[0]), oy = tr_u2f(n.d[1]), oz = tr_u2f(n.d[2]);
+    const float ax = exp2i((int)(int8_t)(n.d[3] & 0xffu)) * r.inv[0];
+    const float ay = exp2i((int)(int8_t)((n.d[3] >> 8) & 0xffu)) * r.inv[1];
+    const float az = exp2i((int)(int8_t)((n.d[3] >> 16) & 0xffu)) * r.inv[2];
+    const float bx = (ox - r.org[0]) * r.inv[0];
+    const float by = (oy - r.org[1]) * r.inv[1];
+    const float bz = (oz - r.org[2]) * r.inv[2];
     /* slots with an internal child or a non-empty leaf */
     uint32_t valid = node_imask(n);
     uint32_t mlo = n.d[6], mhi = n.d[7];
@@ -233,15 +240,12 @@ TR_HD uint32_t intersect_children(const NodeRegs& n, const RayPre& r, float tnea
     for (uint32_t s = 0; s < 8; ++s) {
         uint32_t sh = 16u * (s & 1u);
         uint32_t w = s >> 1;
-        float lx = ox + (float)((n.d[8 + w] >> sh) & 0xffffu) * sx;
-        float ly = oy + (float)((n.d[12 + w] >> sh) & 0xffffu) * sy;
-        float lz = oz + (float)((n.d[16 + w] >> sh) & 0xffffu) * sz;
-        float hx = ox + (float)((n.d[20 + w] >> sh) & 0xffffu) * sx;
-        float hy = oy + (float)((n.d[24 + w] >> sh) & 0xffffu) * sy;
-        float hz = oz + (float)((n.d[28 + w] >> sh) & 0xffffu) * sz;
-        float tx0 = (lx - r.org[0]) * r.inv[0], tx1 = (hx - r.org[0]) * r.inv[0];
-        float ty0 = (ly - r.org[1]) * r.inv[1], ty1 = (hy - r.org[1]) * r.inv[1];
-        float tz0 = (lz - r.org[2]) * r.inv[2], tz1 = (hz - r.org[2]) * r.inv[2];
+        float tx0 = fmaf((float)((n.d[8 + w] >> sh) & 0xffffu), ax, bx);
+        float ty0 = fmaf((float)((n.d[12 + w] >> sh) & 0xffffu), ay, by);
+        float tz0 = fmaf((float)((n.d[16 + w] >> sh) & 0xffffu), az, bz);
+        float tx1 = fmaf((float)((n.d[20 + w] >> sh) & 0xffffu), ax, bx);
+        float ty1 = fmaf((float)((n.d[24 + w] >> sh) & 0xffffu), ay, by);
+        float tz1 = fmaf((float)((n.d[28 + w] >> sh) & 0xffffu), az, bz);
         float t0 = fmax_(fmax_(fmin_(tx0, tx1), fmin_(ty0, ty1)), fmax_(fmin_(tz0, tz1), tnear));
         float t1 = fmin_(fmin_(fmax_(tx0, tx1), fmax_(ty0, ty1)), fmin_(fmax_(tz0, tz1), tmax));
         t1 = t1 * 1.00000095f; /* 1 + 2^-20: keeps boxes touching a hit at t */

@@ -108,7 +108,7 @@ struct tr_scene {
     uint32_t capacity = 0;
     uint32_t max_shadow = 0;
     DevBuf<float4> ray_o, ray_d, hit, thr, local, global, sh_o, sh_d, sh_c;
-    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, qq, seg, seg_n, sh_n, counters, slot_req;
+    DevBuf<uint32_t> hit_prim, hash, q0, q1, shq, qq, seg, seg_n, sh_n, counters, slot_req, gsum;
     DevBuf<float> sh_tmax;
     DevBuf<uint2> spill;
     DevBuf<uint64_t> counts;
@@ -283,6 +283,7 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->sh_d.reserve((size_t)cap * msh));
     TR_HIP(s->sh_c.reserve((size_t)cap * msh));
     TR_HIP(s->counters.reserve(8));
+    TR_HIP(s->gsum.reserve(3 * kCompactGroups));
     TR_HIP(s->spill.reserve((size_t)persistent_grid(s) * kTraceBlock * kStackSpill));
     TR_HIP(s->counts.reserve(8));
     s->capacity = cap;
@@ -300,6 +301,7 @@ PathPool path_pool(const tr_scene* s)
     p.sh_o = s->sh_o.ptr; p.sh_d = s->sh_d.ptr; p.sh_c = s->sh_c.ptr; p.sh_n = s->sh_n.ptr; p.sh_tmax = s->sh_tmax.ptr;
     p.counters = s->counters.ptr; p.spill = s->spill.ptr; p.capacity = s->capacity; p.pflags = s->pflags.ptr;
     p.query_queue = s->qq.ptr; p.seg = s->seg.ptr; p.seg_n = s->seg_n.ptr; p.slot_req = s->slot_req.ptr;
+    p.group_sum = s->gsum.ptr;
     p.seg_cap = (s->capacity + kSegment - 1) / kSegment;
     return p;
 }

@@ -114,6 +114,7 @@ struct PathPool {
     uint32_t* seg;
     uint32_t* seg_n;
     uint32_t seg_cap;   /* segments per queue */
+    uint32_t* group_sum; /* k_compact: [3][kCompactGroups] segment-group sums */
     uint32_t* pflags;   /* cache mode path flags */
     uint32_t* slot_req; /* request mode: request of each slot */
     uint2* spill;       /* traversal stack spill, per persistent thread */
@@ -138,6 +139,7 @@ struct LaunchEvents {
 
 constexpr int kTraceBlock = 64;       /* one wave per block */
 constexpr uint32_t kSegment = 256;    /* shading block = queue segment */
+constexpr uint32_t kCompactGroups = 256; /* k_compact blocks per queue, at most */
 constexpr int kStackLds = 16;         /* stack entries per lane kept in LDS */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
 

@@ -712,7 +712,26 @@ __global__ __launch_bounds__(256) void k_accumulate_req(FrameParams p, PathPool 
     const uint32_t s_first = (uint32_t)(p.req.path_off[r] + (gp - p.req.pix_off[r]) * per_pixel - b.path_base);
     const float rcp_gi = 1.0f / (float)p.n_gi;
     double sum[3] = {0.0, 0.0, 0.0};
-    for (uint32_t c = 0; c < spp; ++c) {
+    uint32_t c = 0;
+    if (p.n_gi == 1) {
+        /* runs of 8 consecutive slots, as in k_accumulate */
+        for (; c + 8 <= spp; c += 8) {
+            float4 g[8], l[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                g[u] = pool.global[s_first + c + u];
+                l[u] = pool.local[s_first + c + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                V3 v = add(ld4(l[u]), muls(ld4(g[u]), rcp_gi));
+                sum[0] += (double)v.x;
+                sum[1] += (double)v.y;
+                sum[2] += (double)v.z;
+            }
+        }
+    }
+    for (; c < spp; ++c) {
         const uint32_t s0 = s_first + c * p.n_gi;
         V3 g = ld4(pool.global[s0]);
         for (uint32_t j = 1; j < p.n_gi; ++j) g = add(g, ld4(pool.global[s0 + j]));
@@ -793,11 +812,27 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
     return before + x - v;
 }
 
-/* Block (x, k) copies queue k's segments [x * group, (x + 1) * group) to
- * the dense queue at the offset given by the counts of all segments before
- * them (summed by the block itself: the counts are a few KB and L2
- * resident); the last block writes the queue size. */
-__global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_seg, uint32_t group, int nxt)
+/* Block (x, k) sums queue k's segment counts of group x (segments
+ * [x * group, (x + 1) * group)), in a fixed order. */
+__global__ __launch_bounds__(kSegment) void k_compact_sums(PathPool pool, uint32_t n_seg, uint32_t group,
+                                                            uint32_t* group_sum)
+{
+    const uint32_t k = blockIdx.y;
+    const uint32_t* cnt = pool.seg_n + (size_t)k * pool.seg_cap;
+    const uint32_t s0 = blockIdx.x * group;
+    const uint32_t s1 = min(s0 + group, n_seg);
+    uint32_t part = 0;
+    for (uint32_t j = s0 + threadIdx.x; j < s1; j += kSegment) part += cnt[j];
+    uint32_t total;
+    (void)block_exclusive_scan(part, &total);
+    if (threadIdx.x == 0) group_sum[k * gridDim.x + blockIdx.x] = total;
+}
+
+/* Block (x, k) copies queue k's segments of group x to the dense queue at
+ * the offset given by the group sums before it (at most a few hundred);
+ * the last block writes the queue size. */
+__global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_seg, uint32_t group, int nxt,
+                                                      const uint32_t* group_sum)
 {
     const uint32_t k = blockIdx.y;
     const uint32_t* cnt = pool.seg_n + (size_t)k * pool.seg_cap;
@@ -805,8 +840,9 @@ __global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_
     uint32_t* dst = k == 0 ? pool.queue[nxt] : (k == 1 ? pool.shadow_queue : pool.query_queue);
     const uint32_t s0 = blockIdx.x * group;
     const uint32_t s1 = min(s0 + group, n_seg);
+    const uint32_t* gs = group_sum + k * gridDim.x;
     uint32_t part = 0;
-    for (uint32_t j = threadIdx.x; j < s0; j += kSegment) part += cnt[j];
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kSegment) part += gs[j];
     uint32_t base;
     (void)block_exclusive_scan(part, &base);
     for (uint32_t c0 = s0; c0 < s1; c0 += kSegment) {
@@ -975,7 +1011,28 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
     uint32_t pixel = out.pixel_list ? out.pixel_list[pseq] : (uint32_t)pseq;
     double sum[3] = {0.0, 0.0, 0.0};
     float rcp_gi = 1.0f / (float)p.n_gi;
-    for (uint32_t c = 0; c < p.spp; ++c) {
+    uint32_t c = 0;
+    if (p.n_gi == 1 && !out.path_rgb) {
+        /* runs of 8 consecutive slots: each lane consumes whole cache lines
+         * of its pixel back to back (same sums, same order) */
+        const uint32_t s_first = (uint32_t)(first_slot - slot_base);
+        for (; c + 8 <= p.spp; c += 8) {
+            float4 g[8], l[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                g[u] = pool.global[s_first + c + u];
+                l[u] = pool.local[s_first + c + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                V3 v = add(ld4(l[u]), muls(ld4(g[u]), rcp_gi));
+                sum[0] += (double)v.x;
+                sum[1] += (double)v.y;
+                sum[2] += (double)v.z;
+            }
+        }
+    }
+    for (; c < p.spp; ++c) {
         uint32_t s0 = (uint32_t)(first_slot - slot_base) + c * p.n_gi;
         V3 g = ld4(pool.global[s0]);
         uint32_t h = pool.hash[s0];
@@ -1110,10 +1167,17 @@ void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int n
      * multiple of the count array */
     uint32_t group = (n_seg + 255) / 256;
     if (group < 16) group = 16;
-    uint32_t blocks = (n_seg + group - 1) / group;
+    uint32_t blocks = (n_seg + group - 1) / group;   /* at most kCompactGroups */
     dim3 grid(blocks, queries ? 3 : 2);
-    if (ev) hipExtLaunchKernelGGL(k_compact, grid, dim3(kSegment), 0, s, ev->start, ev->stop, 0u, pool, n_seg, group, nxt);
-    else k_compact<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, nxt);
+    if (ev) {
+        hipExtLaunchKernelGGL(k_compact_sums, grid, dim3(kSegment), 0, s, ev->start, nullptr, 0u, pool, n_seg, group,
+                              pool.group_sum);
+        hipExtLaunchKernelGGL(k_compact, grid, dim3(kSegment), 0, s, nullptr, ev->stop, 0u, pool, n_seg, group, nxt,
+                              (const uint32_t*)pool.group_sum);
+    } else {
+        k_compact_sums<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, pool.group_sum);
+        k_compact<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, nxt, (const uint32_t*)pool.group_sum);
+    }
 }
 
 void launch_cache_infer(hipStream_t s, const MlpWeights& w, const PathPool& pool, const CacheRecords& rec,

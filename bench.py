@@ -9,8 +9,9 @@ HBM before the timed region.
 
 Prints one JSON line: Msamples/s over the timed steps, the roofline of the
 dominant kernel (closest-hit BVH traversal, HBM-bound; algorithmic bytes per
-ray segment = 128 B per node fetched + 48 B per primitive tested + 64 B of
-ray state and hit, counted by the kernels in a separate counting frame),
+ray segment = one node record (128 B) per node fetched + 48 B per primitive
+tested + 64 B of ray state and hit, counted by the kernels in a separate
+counting frame),
 and the CPU oracle timed on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
@@ -357,7 +358,8 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
 
     segs = cnt["segments"]
-    alg_bytes = 128.0 * cnt["node_visits"] + 48.0 * cnt["prim_tests"] + 64.0 * segs
+    node_bytes = info["bvh_bytes"] / max(1, info["bvh_nodes"])
+    alg_bytes = node_bytes * cnt["node_visits"] + 48.0 * cnt["prim_tests"] + 64.0 * segs
     trace_s = tim["trace_ms"] / 1e3
     achieved = alg_bytes / trace_s / 1e9 if trace_s > 0 else 0.0
     traffic = load_traffic(os.path.join(ROOT, "profiles", "trace_kernel_heightfield_pmc.txt"))
@@ -432,7 +434,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
-            "kernel": "k_trace (closest hit, BVH8)",
+            "kernel": "k_trace_refill (closest hit, BVH8)",
             "alg_bytes_per_step": alg_bytes,
             "segments_per_step": segs,
             "launches_per_frame": tim["trace_launches"],

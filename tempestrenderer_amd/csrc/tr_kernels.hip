@@ -253,10 +253,9 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, Ba
     V3 dir = normalize(sub(end, start));
     pool.ray_o[i] = st4(start, 0.0f);
     pool.ray_d[i] = st4(dir, 0.0f);
-    pool.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    pool.local[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    pool.global[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    pool.hash[i] = kFnvBasis;
+    /* throughput 1, balance 1, zero radiance and the hash basis are the
+     * state of every path at its camera vertex: k_shade starts vertex 0
+     * from them instead of reading them back, and writes all four */
     pool.queue[0][i] = i;
     if (p.cache) {
         uint32_t r = train_record(p, pseq, cs);
@@ -477,15 +476,17 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         Rng rng;
         rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
         const bool unbiased = p.estimator == TR_ESTIMATOR_UNBIASED;
-        float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = pool.hit[slot], t4 = pool.thr[slot];
+        float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = pool.hit[slot];
+        const float4 t4 = vertex == 0 ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : pool.thr[slot];
         V3 org = ld4(o4), dir = ld4(d4);
         V3 T = ld4(t4);
         float bal = t4.w;
         uint32_t geom = __float_as_uint(h4.w);
-        uint32_t hash = pool.hash[slot];
+        uint32_t hash = vertex == 0 ? kFnvBasis : pool.hash[slot];
         V3 local = mk(0.0f, 0.0f, 0.0f);
-        V3 global = ld4(pool.global[slot]);
-        bool write_local = false;
+        V3 global = vertex == 0 ? mk(0.0f, 0.0f, 0.0f) : ld4(pool.global[slot]);
+        /* the camera vertex writes the slot's camera-vertex radiance, zero or not */
+        bool write_local = vertex == 0;
         uint32_t pf = 0;
         if (CACHE) {
             pf = pool.pflags[slot];

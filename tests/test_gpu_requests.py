@@ -42,12 +42,14 @@ def _gpu_scene(d):
     return g
 
 
-def test_requests_match_oracle_per_pixel():
-    print("case: render requests vs oracle, 6 requests, pool 512", flush=True)
+@pytest.mark.parametrize("gi,estimator", [(1, 0), (4, 1)])
+def test_requests_match_oracle_per_pixel(gi, estimator):
+    print("case: render requests vs oracle, 6 requests, pool 512, gi %d estimator %d" % (gi, estimator), flush=True)
     d, reqs = _scene_and_requests()
+    d.settings.update(samples_gi=gi, estimator=estimator)
     g = _gpu_scene(d)
     mean, imgs, st = g.renderRequests(reqs, use_graph=True, pool_slots=512, images=True)
-    total = sum(q["width"] * q["height"] * q["samples_camera"] for q in reqs)
+    total = sum(q["width"] * q["height"] * q["samples_camera"] for q in reqs) * gi
     assert st["paths"] == total and st["batches"] >= -(-total // 512) > 1   # batches cut through requests
     base = 0
     for q, img in zip(reqs, imgs):

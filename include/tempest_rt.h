@@ -193,6 +193,7 @@ typedef struct tr_stats {
     uint64_t volume_steps;       /* tentative collisions of delta / ratio tracking (only if counting on) */
     double cache_ms;             /* timing mode: radiance-cache inference, targets and training */
     double cache_loss;           /* loss of the last training step of the call (cache mode) */
+    double exposure;             /* auto-exposure modes: the factor applied to the frame's mean image */
 } tr_stats;
 
 /* Scene-construction information. */

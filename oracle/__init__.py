@@ -76,6 +76,7 @@ def _declare(lib):
         "orc_trace_rays": (None, [vp, sz, P(_abi.tr_ray), P(_abi.tr_hit), ctypes.c_int, ctypes.c_int]),
         "orc_trace_path": (None, [vp, u32, u32]),
         "orc_render_paths": (None, [vp, sz, P(u32), u32, P(f), P(u32), ctypes.c_int]),
+        "orc_resolve": (f, [sz, P(ctypes.c_double), ctypes.c_double, u32, f, P(u32)]),
         "orc_render_rows": (ctypes.c_double, [vp, u32, u32, u32, ctypes.c_int, P(ctypes.c_double),
                                               P(ctypes.c_double), ctypes.c_int]),
     }
@@ -128,6 +129,18 @@ def material(model, fields):
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def resolve(accum, rcp_count, post, exposure=0.0):
+    """Backbuffer of an FP64 radiance sum [n, 3] through a picture
+    post-process (ray-tracing.cc:2366-2375, 1638-1696): (uint32 RGBA [n],
+    the computed auto exposure).  exposure > 0 replaces the computed one."""
+    lib = load()
+    acc = np.ascontiguousarray(accum, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(acc.shape[0], dtype=np.uint32)
+    e = lib.orc_resolve(acc.shape[0], acc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(rcp_count),
+                        int(post), float(exposure), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out, float(e)
 
 
 class OracleScene:

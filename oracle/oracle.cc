@@ -1311,6 +1311,83 @@ void orc_fmath(int op, size_t n, const float* in, float* out)
     }
 }
 
+/* The backbuffer resolve, restated from the reference's per-pixel colour
+ * conversion (src/graphics/ray-tracing/ray-tracing.cc:2366-2375) and its
+ * auto-exposure post pass (:1638-1696): mean = accum * rcp_count in FP64,
+ * rounded to float; Linear -> ToColor; sRGB -> ToColor(sRGB); ACES ->
+ * ToColor(sRGB(ACESFilm)) (include/tempest/math/hdr.hh:76-84); the
+ * auto-exposure modes scale by 0.18 / RGBToLuminance(exp(mean of
+ * log(1e-3 + v) over the pixels)), with log taken over finite pixels only
+ * and the mean over all of them (spectrum.hh:328-331), then Reinhard
+ * x / (1 + x) for AutoExposureHDR (hdr.hh:32-35) and sRGB.  ToColor
+ * truncates 255 * clamp(v, 0, 1) (spectrum.hh:792-798); sRGB is
+ * ConvertLinearToSLuminance (spectrum.hh:147-153) with pow written as
+ * exp(log(v) / 2.4) in the shared tr_logf/tr_expf.  `exposure_in` > 0
+ * replaces the computed exposure (to compare pixels with a device that
+ * summed the logs in another order); the computed one is returned. */
+static float orc_to_srgb(float v)
+{
+    if (v <= 0.0031308f) return v * 12.92f;
+    return 1.055f * tr_expf(tr_logf(v) * (1.0f / 2.4f)) - 0.055f;
+}
+
+static uint32_t orc_to_color(const float c[3])
+{
+    uint32_t out = 255u << 24;
+    for (int k = 0; k < 3; ++k) {
+        float v = c[k] < 0.0f ? 0.0f : (c[k] > 1.0f ? 1.0f : c[k]);
+        out |= (uint32_t)(255.0f * v) << (8 * k);
+    }
+    return out;
+}
+
+static float orc_aces(float x)
+{
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    float v = (x * (a * x + b)) / (x * (c * x + d) + e);
+    return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+}
+
+float orc_resolve(size_t n, const double* accum, double rcp_count, uint32_t post, float exposure_in, uint32_t* rgba)
+{
+    auto mean = [&](size_t i, float c[3]) {
+        for (int k = 0; k < 3; ++k) c[k] = (float)(accum[i * 3 + k] * rcp_count);
+    };
+    const bool autoexp = post == TR_POST_AUTO_EXPOSURE_SRGB || post == TR_POST_AUTO_EXPOSURE_HDR;
+    float exposure = 0.0f;
+    if (autoexp) {
+        double log_sum[3] = {0.0, 0.0, 0.0};
+        for (size_t i = 0; i < n; ++i) {
+            float c[3];
+            mean(i, c);
+            if (!(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2]))) continue;
+            for (int k = 0; k < 3; ++k) log_sum[k] += (double)tr_logf(1e-3f + c[k]);
+        }
+        float m[3];
+        for (int k = 0; k < 3; ++k) m[k] = tr_expf((float)(log_sum[k] / (double)n));
+        exposure = 0.18f / (0.2126f * m[0] + 0.7152f * m[1] + 0.0722f * m[2]);
+    }
+    const float e = exposure_in > 0.0f ? exposure_in : exposure;
+    for (size_t i = 0; i < n; ++i) {
+        float c[3];
+        mean(i, c);
+        if (post == TR_POST_LINEAR) {
+            rgba[i] = orc_to_color(c);
+            continue;
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (post == TR_POST_ACES) c[k] = orc_aces(c[k]);
+            else if (autoexp) {
+                c[k] = e * c[k];
+                if (post == TR_POST_AUTO_EXPOSURE_HDR) c[k] = c[k] / (1.0f + c[k]);
+            }
+            c[k] = orc_to_srgb(c[k]);
+        }
+        rgba[i] = orc_to_color(c);
+    }
+    return exposure;
+}
+
 static Sample<float> unit_sample(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
                                  const float* wi, float dir_density)
 {

@@ -78,6 +78,7 @@ class tr_stats(ctypes.Structure):
         ("shadow_ms", ctypes.c_double), ("other_ms", ctypes.c_double), ("trace_launches", ctypes.c_uint64),
         ("node_visits", ctypes.c_uint64), ("prim_tests", ctypes.c_uint64), ("counted_rays", ctypes.c_uint64),
         ("volume_steps", ctypes.c_uint64), ("cache_ms", ctypes.c_double), ("cache_loss", ctypes.c_double),
+        ("exposure", ctypes.c_double),
     ]
 
 

@@ -49,6 +49,7 @@ struct tr_scene {
     float rr = 1.0f;
     float background[3] = {0.0f, 0.0f, 0.0f};
     uint32_t post = TR_POST_SRGB;
+    bool expo_used = false; /* the last frame ran the auto-exposure passes */
     int32_t transparent = 0;
     uint32_t render_mode = 0;
     uint64_t seed = 1;
@@ -116,6 +117,8 @@ struct tr_scene {
     /* frame accumulation */
     DevBuf<double> accum;
     DevBuf<uint32_t> rgba;
+    DevBuf<double> expo_partial;  /* auto exposure: 3 log sums per 256 pixels */
+    DevBuf<float> exposure;       /* auto exposure: the frame's factor */
     std::vector<uint32_t> rgba_host;
     uint64_t accum_samples = 0;
     bool frame_pending = false;
@@ -162,7 +165,7 @@ tr_scene::~tr_scene()
     d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); sh_tmax.release();
     ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
     sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
-    shq.release(); qq.release(); seg.release(); seg_n.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release();
+    shq.release(); qq.release(); seg.release(); seg_n.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release(); expo_partial.release(); exposure.release();
     if (net) tr_mlp_destroy(net);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -546,6 +549,11 @@ void collect_stats(tr_scene* s, uint64_t paths)
     s->stats.cache_loss = 0.0;
     if (s->cache_trained && s->net) s->stats.cache_loss = s->net->st.read_loss(s->cache_steps, s->cache_step_w);
     s->cache_trained = false;
+    s->stats.exposure = 0.0;
+    if (s->expo_used) {
+        float e = 0.0f;
+        if (hipMemcpy(&e, s->exposure.ptr, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess) s->stats.exposure = e;
+    }
     s->stats.trace_launches = traces;
     s->stats.node_visits = s->stats.prim_tests = s->stats.counted_rays = 0;
     s->stats.segments = s->stats.shadow_rays = s->stats.volume_steps = 0;
@@ -581,6 +589,11 @@ tr_status start_frame(tr_scene* s)
     uint64_t npx = (uint64_t)s->width * s->height;
     TR_HIP(s->accum.reserve(npx * 3));
     TR_HIP(s->rgba.reserve(npx));
+    s->expo_used = s->post == TR_POST_AUTO_EXPOSURE_SRGB || s->post == TR_POST_AUTO_EXPOSURE_HDR;
+    if (s->expo_used) {
+        TR_HIP(s->expo_partial.reserve((npx + 255) / 256 * 3));
+        TR_HIP(s->exposure.reserve(1));
+    }
     if (s->accum_samples == 0) TR_HIP(hipMemsetAsync(s->accum.ptr, 0, npx * 3 * sizeof(double), s->stream));
     BatchOut out;
     memset(&out, 0, sizeof(out));
@@ -589,7 +602,8 @@ tr_status start_frame(tr_scene* s)
     if (st != TR_OK) return st;
     s->accum_samples += s->spp;
     s->frame_index += 1;
-    launch_resolve(s->stream, s->accum.ptr, 1.0 / (double)s->accum_samples, s->rgba.ptr, (uint32_t)npx, s->post);
+    launch_resolve(s->stream, s->accum.ptr, 1.0 / (double)s->accum_samples, s->rgba.ptr, (uint32_t)npx, s->post,
+                   s->expo_partial.ptr, s->exposure.ptr);
     TR_HIP(hipGetLastError());
     s->frame_pending = true;
     return TR_OK;
@@ -698,7 +712,13 @@ SETTER(tr_scene_set_chunk_size, chunk, uint32_t)
 SETTER(tr_scene_set_samples_camera, spp, uint32_t)
 SETTER(tr_scene_set_max_ray_depth, max_depth, uint32_t)
 SETTER(tr_scene_set_russian_roulette, rr, float)
-SETTER(tr_scene_set_picture_post_process, post, uint32_t)
+tr_status tr_scene_set_picture_post_process(tr_scene* s, uint32_t mode)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (mode > TR_POST_ACES) return fail(TR_ERROR_INVALID_ARG, "unknown picture post-process");
+    s->post = mode;
+    return TR_OK;
+}
 SETTER(tr_scene_set_transparent_background, transparent, int32_t)
 SETTER(tr_scene_set_render_mode, render_mode, uint32_t)
 SETTER(tr_scene_set_seed, seed, uint64_t)

@@ -179,8 +179,11 @@ void launch_request_outputs(hipStream_t s, const ReqView& rv, uint32_t n_gi, con
                             float* image, uint64_t n_pix_total);
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
                        uint2* spill, uint32_t grid);
+/* The backbuffer of the frame's mean radiance through the picture
+ * post-process; the auto-exposure modes use `partial` (3 doubles per 256
+ * pixels) and write their exposure to `exposure`. */
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,
-                    uint32_t n_pixels, uint32_t post);
+                    uint32_t n_pixels, uint32_t post, double* partial, float* exposure);
 
 }  // namespace trk
 

@@ -1105,16 +1105,89 @@ __device__ inline uint32_t to_color(float x, float y, float z)
     return (uint32_t)(255.0f * x) | ((uint32_t)(255.0f * y) << 8u) | ((uint32_t)(255.0f * z) << 16u) | (255u << 24u);
 }
 
+/* ACESFilm (math/hdr.hh:76-84): x (a x + b) / (x (c x + d) + e), clamped
+ * to [0, 1]. */
+__device__ inline float aces_film(float x)
+{
+    float v = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+    return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+}
+
+/* Auto exposure, first pass: per block the FP64 sums of the per-channel
+ * log(Epsilon + pixel), Epsilon = 1e-3 (ray-tracing.cc:61, 2375), over
+ * finite pixels; second pass (one block): the blocks' sums in order, then
+ * the exposure 0.18 / RGBToLuminance(exp(sum / pixels)) (ray-tracing.cc:
+ * 1649-1659, spectrum.hh:328-331). */
+__global__ __launch_bounds__(256) void k_log_sum(const double* accum, double rcp_count, uint32_t n, double* partial)
+{
+    __shared__ double red[3][256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const float v[3] = {(float)(accum[(size_t)i * 3 + 0] * rcp_count), (float)(accum[(size_t)i * 3 + 1] * rcp_count),
+                            (float)(accum[(size_t)i * 3 + 2] * rcp_count)};
+        if (isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))
+            for (int k = 0; k < 3; ++k) s[k] = (double)tr_logf(1e-3f + v[k]);
+    }
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) partial[(size_t)blockIdx.x * 3 + k] = red[k][0];
+}
+
+__global__ __launch_bounds__(256) void k_exposure(const double* partial, uint32_t blocks, uint32_t n, float* exposure)
+{
+    __shared__ double red[3][256];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x)
+        for (int k = 0; k < 3; ++k) s[k] += partial[(size_t)b * 3 + k];
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float m[3];
+        for (int k = 0; k < 3; ++k) m[k] = tr_expf((float)(red[k][0] / (double)n));
+        const float lum = 0.2126f * m[0] + 0.7152f * m[1] + 0.0722f * m[2];
+        exposure[0] = 0.18f / lum;
+    }
+}
+
+/* The backbuffer: per-pixel mean radiance through the PicturePostProcess
+ * (ray-tracing.cc:2367-2375 and postprocess :1638-1696): Linear, sRGB,
+ * ACESFilm then sRGB, and the auto-exposure modes (exposure * value, with
+ * the Reinhard operator x / (1 + x) for AutoExposureHDR, math/hdr.hh:
+ * 32-35) then sRGB; ToColor truncates 255 * clamp(v, 0, 1)
+ * (spectrum.hh:792-798). */
 __global__ __launch_bounds__(256) void k_resolve(const double* accum, double rcp_count, uint32_t* rgba, uint32_t n,
-                                                 uint32_t post)
+                                                 uint32_t post, const float* exposure)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float r = (float)(accum[(size_t)i * 3 + 0] * rcp_count);
-    float g = (float)(accum[(size_t)i * 3 + 1] * rcp_count);
-    float b = (float)(accum[(size_t)i * 3 + 2] * rcp_count);
-    if (post == TR_POST_LINEAR) rgba[i] = to_color(r, g, b);
-    else rgba[i] = to_color(to_srgb(r), to_srgb(g), to_srgb(b));
+    float c[3] = {(float)(accum[(size_t)i * 3 + 0] * rcp_count), (float)(accum[(size_t)i * 3 + 1] * rcp_count),
+                  (float)(accum[(size_t)i * 3 + 2] * rcp_count)};
+    if (post == TR_POST_LINEAR) {
+        rgba[i] = to_color(c[0], c[1], c[2]);
+        return;
+    }
+    if (post == TR_POST_ACES) {
+        for (int k = 0; k < 3; ++k) c[k] = aces_film(c[k]);
+    } else if (post == TR_POST_AUTO_EXPOSURE_SRGB || post == TR_POST_AUTO_EXPOSURE_HDR) {
+        const float e = exposure[0];
+        for (int k = 0; k < 3; ++k) {
+            c[k] = e * c[k];
+            if (post == TR_POST_AUTO_EXPOSURE_HDR) c[k] = c[k] / (1.0f + c[k]);
+        }
+    }
+    rgba[i] = to_color(to_srgb(c[0]), to_srgb(c[1]), to_srgb(c[2]));
 }
 
 }  // namespace
@@ -1236,10 +1309,15 @@ void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, t
     hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill);
 }
 
-void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba, uint32_t n, uint32_t post)
+void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba, uint32_t n, uint32_t post,
+                    double* partial, float* exposure)
 {
     uint32_t grid = (n + 255) / 256;
-    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, s, accum, rcp_count, rgba, n, post);
+    if (post == TR_POST_AUTO_EXPOSURE_SRGB || post == TR_POST_AUTO_EXPOSURE_HDR) {
+        hipLaunchKernelGGL(k_log_sum, dim3(grid), dim3(256), 0, s, accum, rcp_count, n, partial);
+        hipLaunchKernelGGL(k_exposure, dim3(1), dim3(256), 0, s, (const double*)partial, grid, n, exposure);
+    }
+    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, s, accum, rcp_count, rgba, n, post, (const float*)exposure);
 }
 
 }  // namespace trk

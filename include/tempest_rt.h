@@ -30,7 +30,8 @@ typedef enum tr_status {
     TR_ERROR_OUT_OF_MEMORY = 2,
     TR_ERROR_DEVICE = 3,
     TR_ERROR_NOT_COMMITTED = 4,
-    TR_ERROR_UNSUPPORTED = 5
+    TR_ERROR_UNSUPPORTED = 5,
+    TR_ERROR_IO = 6
 } tr_status;
 
 /* IlluminationModel, numeric values kept (ray-tracing-common.hh:68-105). */
@@ -307,6 +308,23 @@ tr_status tr_scene_read_backbuffer(tr_scene* scene, uint32_t* rgba8, size_t pixe
  * all frames since the last reset, and the number of samples in it. */
 tr_status tr_scene_read_accum(tr_scene* scene, double* rgb_sum, size_t pixel_count,
                               uint64_t* samples_per_pixel);
+
+/* Image files of an RGBA8 picture whose row 0 is the bottom row (the
+ * backbuffer's layout). */
+enum { TR_IMAGE_TGA = 0, TR_IMAGE_PNG = 1 };
+/* The file bytes of one format: SaveTGAImage (src/image/tga-image.cc:
+ * 247-309) byte for byte; PNG holds the pixels SavePNGImage
+ * (src/image/png-image.cc:249-280) stores, deflated by zlib.  out = NULL
+ * asks for the size only. */
+tr_status tr_image_encode(uint32_t format, uint32_t width, uint32_t height, const uint32_t* rgba8,
+                          uint8_t* out, size_t capacity, size_t* size);
+/* SaveImage (src/image/image.cc:60-80): the format from the file's
+ * lower-cased extension, "tga" or "png"; "exr" and others fail with
+ * TR_ERROR_UNSUPPORTED (the reference's EXR writer has no RGBA8 case). */
+tr_status tr_image_save(const char* path, uint32_t width, uint32_t height, const uint32_t* rgba8);
+/* RayTracingSystem::saveImage (ray-tracing-system.hh:125-130): the last
+ * frame's backbuffer through tr_image_save. */
+tr_status tr_scene_save_image(tr_scene* scene, const char* path);
 
 /* Philox key of the counter-based RNG (new; the reference has no seed). */
 tr_status tr_scene_set_seed(tr_scene* scene, uint64_t seed);

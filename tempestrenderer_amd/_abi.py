@@ -130,6 +130,9 @@ TR_ERROR_OUT_OF_MEMORY = 2
 TR_ERROR_DEVICE = 3
 TR_ERROR_NOT_COMMITTED = 4
 TR_ERROR_UNSUPPORTED = 5
+TR_ERROR_IO = 6
+TR_IMAGE_TGA = 0
+TR_IMAGE_PNG = 1
 TR_INVALID_GEOMETRY = 0xFFFFFFFF
 
 P = ctypes.POINTER
@@ -172,6 +175,9 @@ SIGNATURES = {
     "tr_scene_frame_wait": (_st, [_vp]),
     "tr_scene_read_backbuffer": (_st, [_vp, P(_u32), _sz]),
     "tr_scene_read_accum": (_st, [_vp, P(ctypes.c_double), _sz, P(_u64)]),
+    "tr_image_encode": (_st, [_u32, _u32, _u32, P(_u32), P(ctypes.c_uint8), _sz, P(_sz)]),
+    "tr_image_save": (_st, [ctypes.c_char_p, _u32, _u32, P(_u32)]),
+    "tr_scene_save_image": (_st, [_vp, ctypes.c_char_p]),
     "tr_scene_set_seed": (_st, [_vp, _u64]),
     "tr_scene_set_estimator": (_st, [_vp, _u32]),
     "tr_scene_set_frame_index": (_st, [_vp, _u32]),

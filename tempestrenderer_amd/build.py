@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtempest_rt.so")
-SOURCES = ["tr_kernels.hip", "tr_mlp.hip", "tr_api.cpp", "tr_cache.cpp", "tr_bvh8.cpp"]
+SOURCES = ["tr_kernels.hip", "tr_mlp.hip", "tr_api.cpp", "tr_cache.cpp", "tr_bvh8.cpp", "tr_image.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -25,7 +25,7 @@ FLAGS = [
 
 
 def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lpthread"]
+    cmd = [HIPCC] + FLAGS + ["-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lpthread", "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

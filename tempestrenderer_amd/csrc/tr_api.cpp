@@ -1151,6 +1151,15 @@ tr_status tr_scene_read_backbuffer(tr_scene* s, uint32_t* rgba, size_t count)
     return TR_OK;
 }
 
+tr_status tr_scene_save_image(tr_scene* s, const char* path)
+{
+    if (!valid(s) || !path) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    std::vector<uint32_t> px((size_t)s->width * s->height);
+    tr_status st = tr_scene_read_backbuffer(s, px.data(), px.size());
+    if (st != TR_OK) return st;
+    return tr_image_save(path, s->width, s->height, px.data());
+}
+
 tr_status tr_scene_read_accum(tr_scene* s, double* rgb_sum, size_t count, uint64_t* spp_out)
 {
     if (!valid(s) || !rgb_sum) return fail(TR_ERROR_INVALID_ARG, "null argument");

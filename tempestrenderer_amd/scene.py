@@ -7,6 +7,7 @@ ray-tracing-cuda.hh:80-172), so code written against them reads the same.
 Every call goes through libtempest_rt.so; failures raise TempestError.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -18,6 +19,32 @@ _f = ctypes.c_float
 
 def _fptr(a):
     return a.ctypes.data_as(ctypes.POINTER(_f))
+
+
+def _rgba_arg(rgba):
+    a = np.ascontiguousarray(rgba, dtype=np.uint32)
+    if a.ndim != 2:
+        raise ValueError("rgba must be [height, width] uint32")
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def encode_image(rgba, fmt=_abi.TR_IMAGE_TGA):
+    """File bytes of an RGBA8 picture [height, width] (row 0 at the bottom)
+    as TGA (SaveTGAImage, src/image/tga-image.cc:247-309) or PNG."""
+    a, p = _rgba_arg(rgba)
+    lib = _abi.load()
+    n = ctypes.c_size_t()
+    check(lib.tr_image_encode(int(fmt), a.shape[1], a.shape[0], p, None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, dtype=np.uint8)
+    check(lib.tr_image_encode(int(fmt), a.shape[1], a.shape[0], p,
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n.value, ctypes.byref(n)))
+    return out.tobytes()
+
+
+def save_image(path, rgba):
+    """SaveImage (src/image/image.cc:60-80): format from the extension."""
+    a, p = _rgba_arg(rgba)
+    check(_abi.load().tr_image_save(os.fsencode(path), a.shape[1], a.shape[0], p))
 
 
 class Material:
@@ -226,6 +253,11 @@ class RayTracerScene:
         out = np.zeros(n, dtype=np.uint32)
         check(self.lib.tr_scene_read_backbuffer(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n))
         return out.reshape(self.height, self.width)
+
+    def saveImage(self, path):
+        """RayTracingSystem::saveImage (ray-tracing-system.hh:125-130): the
+        backbuffer as .tga (byte-exact) or .png."""
+        check(self.lib.tr_scene_save_image(self.h, os.fsencode(path)))
 
     def accumulation(self):
         """(per-pixel RGB sums as float64 [H, W, 3], samples per pixel)."""

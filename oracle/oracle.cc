@@ -724,12 +724,12 @@ template <class T> bool surface_or_volume(const orc_scene* s, const tr_hit& h, V
     return false;
 }
 
-/* Cache mode: roughness input (microfacet alpha = sqrt(2 / (power + 2)),
- * 1 without a specular part or for SGGX surfaces) and the network inputs
- * of a shading point. */
+/* Cache mode: roughness input (microfacet or Blinn-Phong alpha =
+ * sqrt(2 / (power + 2)), 1 without a specular part or for SGGX surfaces)
+ * and the network inputs of a shading point. */
 inline float cache_roughness(const tr_material* m)
 {
-    if (!is_microfacet(m->model)) return 1.0f;
+    if (!is_microfacet(m->model) && m->model != TR_MODEL_BLINN_PHONG) return 1.0f;
     if (m->specular[0] == 0.0f && m->specular[1] == 0.0f && m->specular[2] == 0.0f) return 1.0f;
     float a = sqrtf(2.0f / (m->specular_power[0] + 2.0f));
     return a < 1.0f ? a : 1.0f;

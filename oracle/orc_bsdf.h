@@ -236,6 +236,82 @@ template <class T> inline void microfacet_sample(const Strata<T>& st, Sample<T>*
     s->pdf = microfacet_pdf(*s);
 }
 
+/* powf as the Blinn-Phong lobes use it.  FP32: exp(p log x) in the shared
+ * tr_expf/tr_logf for x > 0; x = 0 gives 0 (p > 0) or 1 (p = 0); a negative
+ * base gives |x|^p signed by (-1)^p for integer p, NaN otherwise (C99
+ * pow).  FP64: libm pow. */
+inline float xpow(float x, float p)
+{
+    if (x > 0.0f) return tr_expf(p * tr_logf(x));
+    if (p == 0.0f) return 1.0f;
+    if (x == 0.0f) return p > 0.0f ? 0.0f : HUGE_VALF;
+    if (floorf(p) != p) return NAN;
+    float m = tr_expf(p * tr_logf(-x));
+    return fmodf(p, 2.0f) != 0.0f ? -m : m;
+}
+inline double xpow(double x, double p) { return pow(x, p); }
+
+template <class T> inline T lobe_ratio(const tr_material* m)
+{
+    T mv = max_value(spec3<T>(m->diffuse) + spec3<T>(m->specular));
+    return mv != (T)0 ? max_value(spec3<T>(m->specular)) / mv : (T)0;
+}
+
+/* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF times cos. */
+template <class T> inline V3<T> blinn_phong_brdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    T power = (T)m->specular_power[0];
+    T dot_nl = xmax((T)0, dot(s.normal, s.wi));
+    V3<T> h = normalize(s.wi + s.wo);
+    V3<T> spec = spec3<T>(m->specular) * xpow(xmax((T)0, dot(s.normal, h)), power);
+    T cos_theta = xmin(xmax(dot(s.wi, h), (T)0), (T)1);
+    T fresnel = fresnel_schlick((T)m->fresnel[0], cos_theta);
+    return (spec3<T>(m->diffuse) + spec * fresnel * (power + (T)8) * ((T)1 / (T)8)) * dot_nl * ((T)1 / C<T>::pi);
+}
+
+/* BlinnPhongPDF (illumination-models.hh:618-643) */
+template <class T> inline T blinn_phong_pdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    V3<T> h = normalize(s.wi + s.wo);
+    T dot_ln = dot(s.wi, s.normal);
+    if (dot_ln < (T)0) return (T)0;
+    T ratio = lobe_ratio<T>(m);
+    T power = (T)m->specular_power[0];
+    T pdf0 = (power + (T)1) * xpow(dot(h, s.normal), power) * ((T)1 / ((T)2 * C<T>::pi));
+    T transform4 = (T)4 * dot(h, s.wo);
+    pdf0 = transform4 > (T)0 && pdf0 > (T)0 ? pdf0 / transform4 : (T)0;
+    T pdf1 = dot_ln * ((T)1 / C<T>::pi);
+    return (pdf0 - pdf1) * ratio + pdf1;
+}
+
+/* BlinnPhongSampleIncidentLight (illumination-models.hh:645-677):
+ * PowerCosineSampleHemisphere (sampling3.hh:96-104) for the reflected
+ * half vector when xi[2] <= ratio, else CosineSampleHemisphere
+ * (sampling3.hh:69-76). */
+template <class T> inline void blinn_phong_sample(const Strata<T>& st, Sample<T>* s, const T xi[3])
+{
+    const tr_material* m = s->mat;
+    T x_ray = (st.x + xi[0]) / st.tx;
+    T y_ray = (st.y + xi[1]) / st.ty;
+    T ratio = lobe_ratio<T>(m);
+    M3<T> basis = make_basis(s->normal);
+    T sin_phi, cos_phi;
+    sincos_turns(y_ray, &sin_phi, &cos_phi);
+    if (xi[2] <= ratio) {
+        T cos_t = xpow(x_ray, (T)1 / ((T)m->specular_power[0] + (T)1));
+        T sin_t = xsqrt(xmax((T)0, (T)1 - cos_t * cos_t));
+        V3<T> mn = normalize(transform(basis, v3(cos_phi * sin_t, sin_phi * sin_t, cos_t)));
+        s->wi = reflect(s->wo, mn);
+    } else {
+        T r = xsqrt(x_ray);
+        V3<T> d = v3(cos_phi * r, sin_phi * r, xsqrt(xmax((T)0, (T)1 - x_ray)));
+        s->wi = normalize(transform(basis, d));
+    }
+    s->pdf = blinn_phong_pdf(*s);
+}
+
 /* UniformHemispherePDF, illumination-models.hh:723-727 */
 template <class T> inline T uniform_hemisphere_pdf(const Sample<T>& s)
 {
@@ -388,6 +464,7 @@ template <class T> inline V3<T> bsdf_eval(const Sample<T>& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
+    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
         T p = sggx_flake_brdf(s);
@@ -401,6 +478,7 @@ template <class T> inline T bsdf_pdf(const Sample<T>& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
+    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE)
         return sggx_flake_pdf(s.dir_density, spec3<T>(s.mat->sggx_stddev), sample_frame(s), normalize(s.wi + s.wo));
@@ -412,6 +490,7 @@ template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, co
 {
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi);
+    else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi[0], xi[1]);
     else s->pdf = (T)0;

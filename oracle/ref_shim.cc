@@ -28,6 +28,10 @@ static void W(const Vector3& v, float* f) { f[0] = v.x; f[1] = v.y; f[2] = v.z; 
 extern "C" {
 
 void ref_cosine_sample_hemisphere(float ra, float rb, float* out) { W(CosineSampleHemisphere(ra, rb), out); }
+void ref_power_cosine_sample_hemisphere(float ra, float rb, float power, float* out)
+{
+    W(PowerCosineSampleHemisphere(ra, rb, power), out);
+}
 void ref_uniform_sample_hemisphere(float ra, float rb, float* out) { W(UniformSampleHemisphere(ra, rb), out); }
 void ref_uniform_sample_sphere(float ra, float rb, float* out) { W(UniformSampleSphere(ra, rb), out); }
 

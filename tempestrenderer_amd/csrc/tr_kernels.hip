@@ -159,11 +159,12 @@ __device__ inline float4 st4(V3 v, float w) { return make_float4(v.x, v.y, v.z, 
 /* ---------------- radiance cache helpers ---------------- */
 
 /* Roughness input of the cache: the microfacet alpha (sqrt of
- * SpecularPowerToGeometryFactorSq, microfacet-common.hh:109-112), 1 for
- * lobes without a specular part and for SGGX surfaces. */
+ * SpecularPowerToGeometryFactorSq, microfacet-common.hh:109-112; the same
+ * map of the Blinn-Phong exponent), 1 for lobes without a specular part
+ * and for SGGX surfaces. */
 __device__ inline float cache_roughness(const tr_material* m)
 {
-    if (!is_microfacet(m->model)) return 1.0f;
+    if (!is_microfacet(m->model) && m->model != TR_MODEL_BLINN_PHONG) return 1.0f;
     if (m->specular[0] == 0.0f && m->specular[1] == 0.0f && m->specular[2] == 0.0f) return 1.0f;
     float a = sqrtf(geom_factor_sq(m->specular_power[0]));
     return a < 1.0f ? a : 1.0f;

@@ -203,6 +203,83 @@ TR_HD void microfacet_sample(const Strata& st, Sample* s, float xi0, float xi1, 
     s->pdf = microfacet_pdf(*s);
 }
 
+/* powf(x, p) for the Blinn-Phong lobes, in the shared exp/log: x > 0 gives
+ * exp(p log x); x = 0 gives 0 for p > 0 and 1 for p = 0; x < 0 gives
+ * |x|^p with the sign of (-1)^p for an integer p and NaN otherwise. */
+TR_HD float phong_pow(float x, float p)
+{
+    if (x > 0.0f) return tr_expf(p * tr_logf(x));
+    if (p == 0.0f) return 1.0f;
+    if (x == 0.0f) return p > 0.0f ? 0.0f : __builtin_inff();
+    float fl = floorf(p);
+    if (fl != p) return __builtin_nanf("");
+    float m = tr_expf(p * tr_logf(-x));
+    return fmodf(fl, 2.0f) != 0.0f ? -m : m;
+}
+
+/* Specular/diffuse lobe choice shared by the microfacet and Blinn-Phong
+ * models: MaxValue(Specular) / MaxValue(Diffuse + Specular), 0 for black. */
+TR_HD float lobe_ratio(const tr_material* m)
+{
+    float mv = max_value(add(ld3(m->diffuse), ld3(m->specular)));
+    return mv != 0.0f ? max_value(ld3(m->specular)) / mv : 0.0f;
+}
+
+/* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF * cos. */
+TR_HD V3 blinn_phong_brdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    float power = m->specular_power[0];
+    float cos_ln = fmax_(0.0f, dot(s.normal, s.wi));
+    V3 h = normalize(add(s.wi, s.wo));
+    float lobe = phong_pow(fmax_(0.0f, dot(s.normal, h)), power);
+    float cos_theta = fmin_(fmax_(dot(s.wi, h), 0.0f), 1.0f);
+    float fresnel = fresnel_schlick(m->fresnel[0], cos_theta);
+    V3 spec = muls(muls(muls(muls(ld3(m->specular), lobe), fresnel), power + 8.0f), 1.0f / 8.0f);
+    return muls(muls(add(ld3(m->diffuse), spec), cos_ln), 1.0f / kPi);
+}
+
+/* BlinnPhongPDF (illumination-models.hh:618-643) */
+TR_HD float blinn_phong_pdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    V3 h = normalize(add(s.wi, s.wo));
+    float cos_ln = dot(s.wi, s.normal);
+    if (cos_ln < 0.0f) return 0.0f;
+    float ratio = lobe_ratio(m);
+    float power = m->specular_power[0];
+    float pdf0 = (power + 1.0f) * phong_pow(dot(h, s.normal), power) * (1.0f / (2.0f * kPi));
+    float jac = 4.0f * dot(h, s.wo);
+    pdf0 = jac > 0.0f && pdf0 > 0.0f ? pdf0 / jac : 0.0f;
+    float pdf1 = cos_ln * (1.0f / kPi);
+    return (pdf0 - pdf1) * ratio + pdf1;
+}
+
+/* BlinnPhongSampleIncidentLight (illumination-models.hh:645-677) with
+ * PowerCosineSampleHemisphere (sampling3.hh:96-104) and
+ * CosineSampleHemisphere (:69-76). */
+TR_HD void blinn_phong_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2)
+{
+    const tr_material* m = s->mat;
+    float x_ray = (st.x + xi0) / st.tx;
+    float y_ray = (st.y + xi1) / st.ty;
+    float ratio = lobe_ratio(m);
+    M3 basis = make_basis(s->normal);
+    float sin_phi, cos_phi;
+    tr_sincos_turns(y_ray, &sin_phi, &cos_phi);
+    if (xi2 <= ratio) {
+        float cos_t = phong_pow(x_ray, 1.0f / (m->specular_power[0] + 1.0f));
+        float sin_t = sqrtf(fmax_(0.0f, 1.0f - cos_t * cos_t));
+        V3 mn = normalize(transform(basis, mk(cos_phi * sin_t, sin_phi * sin_t, cos_t)));
+        s->wi = reflect(s->wo, mn);
+    } else {
+        float r = sqrtf(x_ray);
+        V3 d = mk(cos_phi * r, sin_phi * r, sqrtf(fmax_(0.0f, 1.0f - x_ray)));
+        s->wi = normalize(transform(basis, d));
+    }
+    s->pdf = blinn_phong_pdf(*s);
+}
+
 /* UniformHemispherePDF, illumination-models.hh:723-727 */
 TR_HD float uniform_hemisphere_pdf(const Sample& s)
 {
@@ -354,12 +431,13 @@ TR_HD bool is_volume_model(uint32_t m) { return m == TR_MODEL_SGGX_MICRO_FLAKE; 
 
 /* TransmittanceLookup / PDFLookup / SampleIncidentLightLookup /
  * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
- * models: GGX and Beckmann (plain, conductor, dielectric), SGGX surface,
- * SGGX micro-flake volume. */
+ * models: Blinn-Phong, GGX and Beckmann (plain, conductor, dielectric),
+ * SGGX surface, SGGX micro-flake volume. */
 TR_HD V3 bsdf_eval(const Sample& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
+    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
         float p = sggx_flake_phase(s);
@@ -371,6 +449,7 @@ TR_HD float bsdf_pdf(const Sample& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
+    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) return sggx_flake_pdf(s);
     return 0.0f;
@@ -379,6 +458,7 @@ TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float 
 {
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi0, xi1, xi2);
+    else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi0, xi1, xi2);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi0, xi1);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi0, xi1);
     else s->pdf = 0.0f;

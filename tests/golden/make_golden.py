@@ -106,6 +106,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ref_sampling.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_sampling.npz"), "with", len(out), "arrays")
     flake_fixture(lib)
+    phong_fixture(lib)
 
 
 def flake_fixture(lib):
@@ -160,6 +161,22 @@ def flake_fixture(lib):
     out.update(flake_wo=wo, flake_stddev=sd, flake_frame=bt, flake_uv=uv, flake_wi=wi)
     np.savez_compressed(os.path.join(HERE, "ref_flake.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_flake.npz"), "with", len(out), "arrays")
+
+
+def phong_fixture(lib):
+    """tests/golden/ref_phong.npz: PowerCosineSampleHemisphere
+    (sampling3.hh:96-104), the half-vector warp of the Blinn-Phong sampler."""
+    F = ctypes.c_float
+    P = ctypes.POINTER(F)
+    rng = np.random.Generator(np.random.PCG64(77))
+    n = 512
+    uv = rng.uniform(0.0, 1.0, size=(n, 2)).astype(np.float32)
+    power = rng.choice(np.array([1.0, 4.0, 10.0, 48.0, 200.0], dtype=np.float32), size=n)
+    res = np.zeros((n, 3), dtype=np.float32)
+    for i in range(n):
+        lib.ref_power_cosine_sample_hemisphere(F(uv[i, 0]), F(uv[i, 1]), F(power[i]), res[i].ctypes.data_as(P))
+    np.savez_compressed(os.path.join(HERE, "ref_phong.npz"), uv=uv, power=power, power_cosine_sample_hemisphere=res)
+    print("wrote", os.path.join(HERE, "ref_phong.npz"))
 
 
 if __name__ == "__main__":

@@ -18,8 +18,14 @@ check, with the same or tighter tolerances:
   integral of D(w) |w.v| over the sphere equals the projected area
   sqrt(v^T S v) that SGGXProjectedArea returns, and the isotropic NDF is the
   constant 0.5/pi (sggx-test.cc:418-469 closed-form identities, +-1e-3);
-- the Lambertian case (Specular 0) gives BRDF*cos/pdf = albedo exactly.
+- the Lambertian case (Specular 0) gives BRDF*cos/pdf = albedo exactly;
+- Blinn-Phong (illumination-models.hh:618-698): the half-vector density
+  (n+1) cos^n(h) / (2 pi) of PowerCosineSampleHemisphere carried to the
+  reflected direction integrates, at normal incidence, to the mass of h
+  below 45 deg, 1 - 2^(-(n+1)/2); its half-vector warp equals the
+  reference's PowerCosineSampleHemisphere (tests/golden/ref_phong.npz).
 """
+import os
 import math
 
 import numpy as np
@@ -75,7 +81,7 @@ def test_white_lobe_energy(model, alpha, theta):
         assert fsum > 0.9, fsum
 
 
-@pytest.mark.parametrize("model,alpha", [(11, 0.3), (28, 0.3), (12, 0.2)])
+@pytest.mark.parametrize("model,alpha", [(11, 0.3), (28, 0.3), (12, 0.2), (1, 0.3), (1, 0.7)])
 def test_sampler_matches_its_pdf(model, alpha):
     mat = _spec_material(model, alpha, fresnel=(0.2, 3.0) if model == 12 else (1.0, 0.0))
     mat.rec.diffuse[0] = mat.rec.diffuse[1] = mat.rec.diffuse[2] = 0.3
@@ -98,8 +104,49 @@ def test_sampler_matches_its_pdf(model, alpha):
     assert abs(est.mean() - fsum) < 4.0 * sigma + 2e-3, (est.mean(), fsum, sigma)
 
 
-def test_lambert_weight_is_albedo():
-    mat = oracle.material(11, dict(diffuse=(0.7, 0.5, 0.3), specular=(0, 0, 0)))
+@pytest.mark.parametrize("power", [1.0, 10.0, 48.0, 400.0])
+def test_phong_pdf_normalised_at_normal_incidence(power):
+    mat = oracle.material(1, dict(diffuse=(0, 0, 0), specular=(1, 1, 1), specular_power=(power, power)))
+    _, psum = _integrate(mat, (0.0, 0.0, 1.0), n_theta=384, n_phi=64)
+    expect = 1.0 - 2.0 ** (-(power + 1.0) / 2.0)
+    assert abs(psum - expect) < 2e-3, (psum, expect)
+
+
+@pytest.mark.parametrize("power,theta", [(10.0, 0.0), (48.0, 0.0), (48.0, 30.0), (400.0, 60.0)])
+def test_phong_energy(power, theta):
+    """The white lobe with F = 1 (Schlick with R0 = 1): at normal incidence
+    wi makes twice the half-vector angle, so the albedo is
+    (n + 8) / 4 * int_0^{pi/2} cos^n(t / 2) cos t sin t dt (1.07478 for
+    n = 10, 1.03385 for n = 48); brdf-test.cc:66-85 bounds it by 1.1."""
+    from scipy.integrate import quad
+    t = math.radians(theta)
+    mat = oracle.material(1, dict(diffuse=(0, 0, 0), specular=(1, 1, 1), specular_power=(power, power)))
+    fsum, _ = _integrate(mat, (math.sin(t), 0.0, math.cos(t)), n_theta=384, n_phi=384)
+    assert fsum <= 1.1, fsum
+    if theta == 0.0:
+        want = (power + 8.0) / 4.0 * quad(lambda x: math.cos(x / 2) ** power * math.cos(x) * math.sin(x), 0.0,
+                                          0.5 * math.pi, limit=200)[0]
+        assert abs(fsum - want) < 2e-3, (fsum, want)
+
+
+def test_phong_half_vector_warp_matches_reference():
+    """With Diffuse 0 the sampler takes the specular lobe: the half vector is
+    PowerCosineSampleHemisphere(ra, rb, n) in the normal's basis (identity
+    for n = +z) and wi its reflection of wo = n."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_phong.npz"))
+    for (u, v), p, ref in zip(g["uv"], g["power"], g["power_cosine_sample_hemisphere"]):
+        mat = oracle.material(1, dict(diffuse=(0, 0, 0), specular=(1, 1, 1), specular_power=(float(p), float(p))))
+        wi, pdf = U.bsdf_sample(mat, (0, 0, 1), (0, 0, 1), (float(u), float(v), 0.0))
+        h = ref.astype(np.float64) / np.linalg.norm(ref)
+        want = 2.0 * h[2] * h - np.array([0.0, 0.0, 1.0])
+        assert np.abs(wi - want).max() < 2e-5, (u, v, p, wi, want)
+        # half vectors beyond 45 deg reflect below the horizon: PDF 0 there
+        assert (pdf > 0.0) == (want[2] >= 0.0) or abs(want[2]) < 1e-5
+
+
+@pytest.mark.parametrize("model", [11, 1])
+def test_lambert_weight_is_albedo(model):
+    mat = oracle.material(model, dict(diffuse=(0.7, 0.5, 0.3), specular=(0, 0, 0)))
     rng = np.random.Generator(np.random.PCG64(3))
     n = np.array([0.3, 0.4, 0.866], dtype=np.float32)
     n /= np.linalg.norm(n)

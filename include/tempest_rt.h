@@ -309,6 +309,34 @@ tr_status tr_scene_read_backbuffer(tr_scene* scene, uint32_t* rgba8, size_t pixe
 tr_status tr_scene_read_accum(tr_scene* scene, double* rgb_sum, size_t pixel_count,
                               uint64_t* samples_per_pixel);
 
+/* RTMeshBlob (mesh/obj-loader.hh:47-59): submeshes whose material
+ * pointers point into `materials`, interleaved vertices (byte offsets in
+ * the submeshes), group-local int32 index triplets.  `stride` is kept for
+ * the reference's field, which its loader never sets (0).  `ignored_maps`
+ * counts materials whose map_Kd the reference would bind (texture maps are
+ * not loaded here). */
+typedef struct tr_mesh_blob {
+    uint32_t submesh_count;
+    tr_submesh* submeshes;
+    uint8_t* vertex_data;
+    size_t vertex_bytes;
+    int32_t* index_data;
+    size_t index_count;
+    uint32_t stride;
+    tr_material* materials;
+    uint32_t material_count;
+    uint32_t ignored_maps;
+} tr_mesh_blob;
+/* TEMPEST_OBJ_LOADER_* flags (mesh/obj-loader.hh:36-40) */
+enum { TR_OBJ_GENERATE_TANGENTS = 1, TR_OBJ_GENERATE_CONSISTENT_NORMALS = 2 };
+/* LoadObjFileStaticRTGeometry(filename, nullptr, blob, flags)
+ * (src/mesh/obj-loader.cc:635-840): parses the OBJ file and the MTL files
+ * it names; any diagnostic fails the load (TR_ERROR_INVALID_ARG, the
+ * message names file and line).  Release with tr_mesh_blob_free. */
+tr_status tr_load_obj_rt_geometry(const char* filename, uint32_t flags, tr_mesh_blob* blob);
+/* RTMeshBlob::~RTMeshBlob (src/mesh/obj-loader.cc:51-59) */
+void tr_mesh_blob_free(tr_mesh_blob* blob);
+
 /* Image files of an RGBA8 picture whose row 0 is the bottom row (the
  * backbuffer's layout). */
 enum { TR_IMAGE_TGA = 0, TR_IMAGE_PNG = 1 };

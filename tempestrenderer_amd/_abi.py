@@ -82,6 +82,18 @@ class tr_stats(ctypes.Structure):
     ]
 
 
+class tr_mesh_blob(ctypes.Structure):
+    _fields_ = [("submesh_count", ctypes.c_uint32), ("submeshes", ctypes.POINTER(tr_submesh)),
+                ("vertex_data", ctypes.POINTER(ctypes.c_uint8)), ("vertex_bytes", ctypes.c_size_t),
+                ("index_data", ctypes.POINTER(ctypes.c_int32)), ("index_count", ctypes.c_size_t),
+                ("stride", ctypes.c_uint32), ("materials", ctypes.POINTER(tr_material)),
+                ("material_count", ctypes.c_uint32), ("ignored_maps", ctypes.c_uint32)]
+
+
+TR_OBJ_GENERATE_TANGENTS = 1
+TR_OBJ_GENERATE_CONSISTENT_NORMALS = 2
+
+
 class tr_request(ctypes.Structure):
     _fields_ = [("view_proj_inv", ctypes.c_float * 16), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("samples_camera", ctypes.c_uint32), ("light_direction", ctypes.c_float * 3)]
@@ -175,6 +187,8 @@ SIGNATURES = {
     "tr_scene_frame_wait": (_st, [_vp]),
     "tr_scene_read_backbuffer": (_st, [_vp, P(_u32), _sz]),
     "tr_scene_read_accum": (_st, [_vp, P(ctypes.c_double), _sz, P(_u64)]),
+    "tr_load_obj_rt_geometry": (_st, [ctypes.c_char_p, _u32, P(tr_mesh_blob)]),
+    "tr_mesh_blob_free": (None, [P(tr_mesh_blob)]),
     "tr_image_encode": (_st, [_u32, _u32, _u32, P(_u32), P(ctypes.c_uint8), _sz, P(_sz)]),
     "tr_image_save": (_st, [ctypes.c_char_p, _u32, _u32, P(_u32)]),
     "tr_scene_save_image": (_st, [_vp, ctypes.c_char_p]),

@@ -186,7 +186,7 @@ uint32_t intern_material(tr_scene* s, const tr_material* m)
  * volume set (ray-tracing.cc:2108-2110). */
 bool model_supported(uint32_t m)
 {
-    return m == TR_MODEL_EMISSIVE || (m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC) ||
+    return m == TR_MODEL_EMISSIVE || m == TR_MODEL_BLINN_PHONG || (m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC) ||
            (m >= TR_MODEL_BECKMANN_MICROFACET && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC) ||
            m == TR_MODEL_SGGX_SURFACE;
 }

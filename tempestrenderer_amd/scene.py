@@ -429,3 +429,60 @@ class RadianceCacheMLP:
         check(self.lib.tr_mlp_train_step(self.h, x.shape[0], _fptr(x), _fptr(t), _fptr(v) if v is not None else None,
                                          ctypes.byref(loss)))
         return loss.value
+
+
+class RTMeshBlob:
+    """RTMeshBlob (mesh/obj-loader.hh:47-59) as Python-owned copies:
+    Submeshes (Submesh list whose materials are entries of Materials),
+    VertexData (uint8 bytes), IndexData (int32), Materials (Material list).
+    IgnoredMaps counts materials whose map_Kd was not loaded."""
+
+    def __init__(self):
+        self.Submeshes, self.Materials = [], []
+        self.VertexData = np.zeros(0, dtype=np.uint8)
+        self.IndexData = np.zeros(0, dtype=np.int32)
+        self.Stride = 0
+        self.IgnoredMaps = 0
+
+    @property
+    def SubmeshCount(self):
+        return len(self.Submeshes)
+
+    @property
+    def MaterialCount(self):
+        return len(self.Materials)
+
+
+def LoadObjFileStaticRTGeometry(filename, flags=0):
+    """LoadObjFileStaticRTGeometry (src/mesh/obj-loader.cc:635-840): the
+    OBJ file and its MTL libraries as an RTMeshBlob; raises TempestError
+    with file and line on any diagnostic.  Pass the result to
+    RayTracerScene.addTriangleMesh(world, blob.Submeshes, blob.IndexData,
+    blob.VertexData)."""
+    lib = _abi.load()
+    raw = _abi.tr_mesh_blob()
+    check(lib.tr_load_obj_rt_geometry(os.fsencode(filename), int(flags), ctypes.byref(raw)))
+    try:
+        out = RTMeshBlob()
+        for i in range(raw.material_count):
+            m = Material.__new__(Material)
+            m.rec = _abi.tr_material()
+            ctypes.pointer(m.rec)[0] = raw.materials[i]
+            out.Materials.append(m)
+        base = ctypes.addressof(raw.materials.contents) if raw.material_count else 0
+        size = ctypes.sizeof(_abi.tr_material)
+        for i in range(raw.submesh_count):
+            sm = raw.submeshes[i]
+            mat = None
+            if sm.material:
+                mat = out.Materials[(ctypes.cast(sm.material, ctypes.c_void_p).value - base) // size]
+            out.Submeshes.append(Submesh(mat, sm.vertex_count, sm.vertex_offset, sm.base_index, sm.stride))
+        if raw.vertex_bytes:
+            out.VertexData = np.ctypeslib.as_array(raw.vertex_data, shape=(raw.vertex_bytes,)).copy()
+        if raw.index_count:
+            out.IndexData = np.ctypeslib.as_array(raw.index_data, shape=(raw.index_count,)).copy()
+        out.Stride = raw.stride
+        out.IgnoredMaps = raw.ignored_maps
+        return out
+    finally:
+        lib.tr_mesh_blob_free(ctypes.byref(raw))

@@ -8,7 +8,7 @@ longest-match / earliest-rule selection, the parsers
 (src/mesh/obj-loader-parser.cc:322-515, obj-mtl-parser.cc:148-247), the
 driver's index and material handling (obj-loader-driver.hh:51-83,
 obj-loader-driver.cc:71-86, obj-mtl-loader-driver.hh:48-54) and the
-per-group InterleaveInterm (obj-loader.cc:62-402) with InterleaveVertices
+per-group InterleaveInterm (obj-loader.cc:61-384) with InterleaveVertices
 (src/utils/interleave-vertices.cc:62-115), all FP32 arithmetic in numpy
 float32 scalars, in the reference's operation order.
 

@@ -7,7 +7,7 @@
  * parsers (src/mesh/obj-loader-parser.cc:322-515, obj-mtl-parser.cc:
  * 148-247), driver (obj-loader-driver.cc:36-86, obj-loader-driver.hh:
  * 51-83, obj-mtl-loader-driver.hh:48-54) and the per-group interleaving
- * InterleaveInterm (obj-loader.cc:62-402) over InterleaveVertices
+ * InterleaveInterm (obj-loader.cc:61-384) over InterleaveVertices
  * (src/utils/interleave-vertices.cc:62-115).
  *
  * The result is what the reference fills into RTMeshBlob: per group ("g"
@@ -691,7 +691,7 @@ void tangent_space(V3 p0, V3 p1, V3 p2, V2 t0, V2 t1, V2 t2, V3* tangent)
     *tangent = V3{x.x, y.x, z.x};
 }
 
-/* InterleaveInterm (obj-loader.cc:62-402) for one group; appends the
+/* InterleaveInterm (obj-loader.cc:61-384) for one group; appends the
  * group's vertices to `data` and its group-local indices to `inds`. */
 void interleave_group(const ObjData& d, const Group& g, size_t pos_n, size_t tc_n, size_t norm_n, uint32_t flags,
                       std::vector<uint8_t>* data, std::vector<int32_t>* inds, uint32_t* stride, const std::string& file)
@@ -809,7 +809,7 @@ void interleave_group(const ObjData& d, const Group& g, size_t pos_n, size_t tc_
     }
     if (gen_tan) {
         /* tangent orthogonalised against the vertex normal (obj-loader.cc:
-         * 375-391), over this group's vertices */
+         * 341-358), over this group's vertices */
         const uint32_t to = 20, no = *stride - 12;
         for (size_t o = data_start; o < data->size(); o += *stride) {
             V3 t, n;

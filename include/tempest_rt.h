@@ -288,6 +288,41 @@ typedef struct tr_grid_volume {
 tr_status tr_scene_add_grid_volume(tr_scene* scene, const tr_grid_volume* volume,
                                    const tr_material* material, uint64_t* out_geom_id);
 
+/* VolumeRoot / Volume (volume/volume.hh:65-83): a grid of dims bricks
+ * spanning [min_corner, max_corner]; brick (x, y, z) is
+ * bricks[(z * dims[1] + y) * dims[0] + x].  A brick's data is its u8
+ * densities (x fastest) followed by one (theta, phi) byte pair per voxel
+ * (the QuantizedDirections grid); data = NULL is an empty brick. */
+typedef struct tr_volume_brick {
+    int32_t dims[3];
+    uint8_t* data;
+} tr_volume_brick;
+typedef struct tr_volume_root {
+    float min_corner[3];
+    float max_corner[3];
+    int32_t dims[3];
+    tr_volume_brick* bricks;
+} tr_volume_root;
+
+/* ParseVolumeHierarchy (src/volume/volume.cc:361-420): reads
+ * <name>_description.vol and the <name>_XXX_YYY_ZZZ-density.vol /
+ * -orientation.vol pair of every brick it lists ("VOL\x3" grids;
+ * src/volume/volume.cc:93-359 converts Float32 and UInt8 inputs).
+ * Release with tr_volume_root_free. */
+tr_status tr_parse_volume_hierarchy(const char* name, tr_volume_root* root);
+/* ParseVolume (src/volume/volume.cc:422-453): one brick from a density and
+ * an orientation file; the bounds come from the density file. */
+tr_status tr_parse_volume(const char* density_file, const char* orientation_file, tr_volume_root* root);
+void tr_volume_root_free(tr_volume_root* root);
+/* addHierarchicalVolume (ray-tracing.cc:1448-1504): every non-empty brick
+ * becomes a volume box sampled as sampleScattering samples bricks (u8
+ * density trilinear, quantized fibre angles through cos/sin tables,
+ * majorant DensityScale = 100).  The material must be SGGXMicroFlake.
+ * *out_geom_id is the first brick's geometry slot (UINT64_MAX if every brick
+ * is empty). */
+tr_status tr_scene_add_hierarchical_volume(tr_scene* scene, const tr_volume_root* root,
+                                           const tr_material* material, uint64_t* out_geom_id);
+
 /* commitScene (ray-tracing.cc:1633-1636): packs the scene into HBM and
  * builds the 8-wide BVH. */
 tr_status tr_scene_commit(tr_scene* scene);

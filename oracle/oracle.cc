@@ -1159,6 +1159,76 @@ int orc_add_grid_volume(orc_scene* s, const tr_grid_volume* v, const tr_material
     return 0;
 }
 
+/* addHierarchicalVolume (ray-tracing.cc:1448-1504): one grid per non-empty
+ * brick at Box{x, y, z} * vol_size + MinCorner, u8 densities, directions
+ * (cos phi sin theta, sin phi sin theta, cos theta) from the s_cosPhi /
+ * s_sinPhi / s_cosTheta / s_sinTheta tables (:777-788), RcpMaxExtinction =
+ * 1 / DensityScale (:60, 1463). */
+int orc_add_hierarchical_volume(orc_scene* s, const tr_volume_root* root, const tr_material* m, uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    float cos_phi[256], sin_phi[256], cos_theta[256], sin_theta[256];
+    for (int i = 0; i < 255; ++i) {
+        float angle = (float)i * (3.141592f / 255.0f);
+        cos_phi[i] = std::cos(2.0f * angle);
+        sin_phi[i] = std::sin(2.0f * angle);
+        cos_theta[i] = std::cos(angle);
+        sin_theta[i] = std::sin(angle);
+    }
+    cos_phi[255] = sin_phi[255] = cos_theta[255] = sin_theta[255] = 0.0f;
+    float vs[3];
+    for (int a = 0; a < 3; ++a) vs[a] = (root->max_corner[a] - root->min_corner[a]) / (float)root->dims[a];
+    uint64_t first = UINT64_MAX;
+    const size_t nb = (size_t)root->dims[0] * root->dims[1] * root->dims[2];
+    const uint32_t mat = intern_material(s, m);
+    for (size_t b = 0; b < nb; ++b) {
+        const tr_volume_brick& br = root->bricks[b];
+        if (!br.data) continue;
+        size_t idx[3] = {b % (size_t)root->dims[0], (b / (size_t)root->dims[0]) % (size_t)root->dims[1],
+                         b / ((size_t)root->dims[0] * root->dims[1])};
+        GridVolume g;
+        g.brick = 1;
+        size_t n = (size_t)br.dims[0] * br.dims[1] * br.dims[2];
+        g.rho.resize(n);
+        g.dir.resize(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            g.rho[i] = (float)br.data[i];
+            uint8_t th = br.data[n + 2 * i], ph = br.data[n + 2 * i + 1];
+            g.dir[3 * i + 0] = cos_phi[ph] * sin_theta[th];
+            g.dir[3 * i + 1] = sin_phi[ph] * sin_theta[th];
+            g.dir[3 * i + 2] = cos_theta[th];
+        }
+        for (int a = 0; a < 3; ++a) {
+            g.lo[a] = (float)idx[a] * vs[a] + root->min_corner[a];
+            g.hi[a] = g.lo[a] + vs[a];
+            g.dims[a] = (uint32_t)br.dims[a];
+            g.scale[a] = (float)(br.dims[a] - 1) / (g.hi[a] - g.lo[a]);
+            g.stddev[a] = m->sggx_stddev[a];
+        }
+        g.rcp_rho_max = 1.0f / 255.0f;
+        g.rcp_majorant = 1.0f / 100.0f;
+        g.rcp_smax = 1.0f;
+        Geom gm;
+        memset(&gm, 0, sizeof(gm));
+        gm.type = GEOM_VOLUME;
+        gm.material = mat;
+        gm.volume = (uint32_t)s->volumes.size();
+        uint32_t slot = (uint32_t)s->geoms.size();
+        s->geoms.push_back(gm);
+        Prim p;
+        memset(&p, 0, sizeof(p));
+        for (int a = 0; a < 3; ++a) { p.v0[a] = g.lo[a]; p.v1[a] = g.hi[a]; }
+        s->volumes.push_back(std::move(g));
+        p.geom = slot;
+        p.prim = 0;
+        p.is_volume = 1;
+        s->prims.push_back(p);
+        if (first == UINT64_MAX) first = slot;
+    }
+    if (out) *out = first;
+    return 0;
+}
+
 int orc_add_light_source(orc_scene* s, const tr_light* light, uint32_t* out)
 {
     Light L;

@@ -41,6 +41,10 @@ struct GridVolume {
     uint32_t dims[3];
     float rcp_rho_max, rcp_majorant, rcp_smax;
     float stddev[3];
+    /* 1 for the reference's brick volumes (u8 density in rho, directions
+     * decoded from the quantized angles): sampleScattering's own decision
+     * arithmetic, ray-tracing.cc:316-337 */
+    uint32_t brick = 0;
     std::vector<float> rho;   /* per sample */
     std::vector<float> dir;   /* 3 per sample */
 };
@@ -144,6 +148,16 @@ inline int grid_accept(const GridVolume& g, V3<float> pos, V3<float> dir, float 
     GridCell<float> c = grid_cell(g, pos);
     float rho = grid_density(g, c);
     if (rho == 0.0f) return 2;
+    if (g.brick) {
+        /* max_density = density / 255 against the threshold, then
+         * TotalDensity = (DensityScale / 255) * density * dir_density times
+         * RcpMaxExtinction (the tracking loops) */
+        if (rho / 255.0f < threshold) return 0;
+        M3<float> basis = make_basis_tangent(grid_orientation(g, c));
+        float sigma = sggx_projected_area(v3(g.stddev[0], g.stddev[1], g.stddev[2]), basis, dir);
+        float total = (100.0f / 255.0f) * rho * sigma;
+        return total * g.rcp_majorant >= threshold ? 1 : 0;
+    }
     float dens = rho * g.rcp_rho_max;
     if (dens < threshold) return 0;
     M3<float> basis = make_basis_tangent(grid_orientation(g, c));

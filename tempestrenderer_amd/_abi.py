@@ -94,6 +94,15 @@ TR_OBJ_GENERATE_TANGENTS = 1
 TR_OBJ_GENERATE_CONSISTENT_NORMALS = 2
 
 
+class tr_volume_brick(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32 * 3), ("data", ctypes.POINTER(ctypes.c_uint8))]
+
+
+class tr_volume_root(ctypes.Structure):
+    _fields_ = [("min_corner", ctypes.c_float * 3), ("max_corner", ctypes.c_float * 3), ("dims", ctypes.c_int32 * 3),
+                ("bricks", ctypes.POINTER(tr_volume_brick))]
+
+
 class tr_request(ctypes.Structure):
     _fields_ = [("view_proj_inv", ctypes.c_float * 16), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("samples_camera", ctypes.c_uint32), ("light_direction", ctypes.c_float * 3)]
@@ -187,6 +196,10 @@ SIGNATURES = {
     "tr_scene_frame_wait": (_st, [_vp]),
     "tr_scene_read_backbuffer": (_st, [_vp, P(_u32), _sz]),
     "tr_scene_read_accum": (_st, [_vp, P(ctypes.c_double), _sz, P(_u64)]),
+    "tr_parse_volume_hierarchy": (_st, [ctypes.c_char_p, P(tr_volume_root)]),
+    "tr_parse_volume": (_st, [ctypes.c_char_p, ctypes.c_char_p, P(tr_volume_root)]),
+    "tr_volume_root_free": (None, [P(tr_volume_root)]),
+    "tr_scene_add_hierarchical_volume": (_st, [_vp, P(tr_volume_root), P(tr_material), P(_u64)]),
     "tr_load_obj_rt_geometry": (_st, [ctypes.c_char_p, _u32, P(tr_mesh_blob)]),
     "tr_mesh_blob_free": (None, [P(tr_mesh_blob)]),
     "tr_image_encode": (_st, [_u32, _u32, _u32, P(_u32), P(ctypes.c_uint8), _sz, P(_sz)]),

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtempest_rt.so")
-SOURCES = ["tr_kernels.hip", "tr_mlp.hip", "tr_api.cpp", "tr_cache.cpp", "tr_bvh8.cpp", "tr_image.cpp", "tr_obj.cpp"]
+SOURCES = ["tr_kernels.hip", "tr_mlp.hip", "tr_api.cpp", "tr_cache.cpp", "tr_bvh8.cpp", "tr_image.cpp", "tr_obj.cpp", "tr_volfile.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

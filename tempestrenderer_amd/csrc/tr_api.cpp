@@ -80,7 +80,8 @@ struct tr_scene {
     std::vector<PrimRec> prims_in;
     std::vector<LightRec> lights;
     std::vector<VolumeRec> vols;
-    std::vector<uint64_t> voxels;   /* binary16 x4 samples (dir x, y, z, density) */
+    std::vector<uint64_t> voxels;   /* voxel words of all volumes (kVoxelHalf / kVoxelQuantized) */
+    uint64_t dir_table = UINT64_MAX; /* brick volumes' direction tables in voxels */
     uint32_t instances = 0;
     bool committed = false;
     bool lights_dirty = false;
@@ -1049,6 +1050,121 @@ tr_status tr_scene_add_grid_volume(tr_scene* s, const tr_grid_volume* v, const t
     p.flags = PRIM_VOLUME;
     s->prims_in.push_back(p);
     if (out) *out = slot;
+    return TR_OK;
+}
+
+namespace {
+
+/* An integer 0..255 as binary16 bits (exact). */
+uint64_t u8_half_bits(uint32_t v)
+{
+    if (v == 0) return 0;
+    uint32_t e = 31u - (uint32_t)__builtin_clz(v);
+    return (uint64_t)(((e + 15u) << 10) | ((v << (10u - e)) & 0x3ffu));
+}
+
+/* s_cosPhi, s_sinPhi, s_cosTheta, s_sinTheta (ray-tracing.cc:777-788):
+ * angle i * pi / 255 (theta) and twice that (phi) for i < 255, 0 at 255,
+ * with the reference's truncated pi. */
+void append_direction_tables(std::vector<uint64_t>* voxels)
+{
+    float tab[1024];
+    for (int i = 0; i < 255; ++i) {
+        float angle = (float)i * (kPi / 255.0f);
+        tab[i] = std::cos(2.0f * angle);
+        tab[256 + i] = std::sin(2.0f * angle);
+        tab[512 + i] = std::cos(angle);
+        tab[768 + i] = std::sin(angle);
+    }
+    tab[255] = tab[511] = tab[767] = tab[1023] = 0.0f;
+    const size_t at = voxels->size();
+    voxels->resize(at + 512);
+    memcpy(voxels->data() + at, tab, sizeof(tab));
+}
+
+}  // namespace
+
+tr_status tr_scene_add_hierarchical_volume(tr_scene* s, const tr_volume_root* root, const tr_material* m,
+                                           uint64_t* out)
+{
+    if (!valid(s) || !root || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "scene already committed");
+    if (m->model != TR_MODEL_SGGX_MICRO_FLAKE)
+        return fail(TR_ERROR_UNSUPPORTED, "hierarchical volumes take the SGGXMicroFlake volume model");
+    for (int a = 0; a < 3; ++a) {
+        if (root->dims[a] < 1) return fail(TR_ERROR_INVALID_ARG, "volume root dims must be positive");
+        if (!(root->max_corner[a] > root->min_corner[a]) || !std::isfinite(root->min_corner[a]) ||
+            !std::isfinite(root->max_corner[a]))
+            return fail(TR_ERROR_INVALID_ARG, "volume bounds must be finite with min < max");
+        if (!(m->sggx_stddev[a] > 0.0f) || !std::isfinite(m->sggx_stddev[a]))
+            return fail(TR_ERROR_INVALID_ARG, "sggx_stddev must be positive and finite");
+    }
+    const uint64_t nb = (uint64_t)root->dims[0] * root->dims[1] * root->dims[2];
+    if (nb > (1u << 20)) return fail(TR_ERROR_INVALID_ARG, "more than 2^20 bricks");
+    if (nb && !root->bricks) return fail(TR_ERROR_INVALID_ARG, "null brick array");
+    uint64_t total = 0;
+    for (uint64_t b = 0; b < nb; ++b) {
+        const tr_volume_brick& br = root->bricks[b];
+        if (!br.data) continue;
+        if (br.dims[0] < 1 || br.dims[1] < 1 || br.dims[2] < 1) return fail(TR_ERROR_INVALID_ARG, "empty brick dims");
+        total += (uint64_t)br.dims[0] * br.dims[1] * br.dims[2];
+    }
+    if (total > (1ull << 31)) return fail(TR_ERROR_INVALID_ARG, "volume larger than 2^31 samples");
+    if (s->dir_table == UINT64_MAX) {
+        s->dir_table = s->voxels.size();
+        append_direction_tables(&s->voxels);
+    }
+    /* vol_size = (MaxCorner - MinCorner) / Dimensions (ray-tracing.cc:1455);
+     * brick corners Box{x, y, z} * vol_size + MinCorner (:1486-1487) */
+    float vs[3];
+    for (int a = 0; a < 3; ++a) vs[a] = (root->max_corner[a] - root->min_corner[a]) / (float)root->dims[a];
+    const uint32_t mat = intern_material(s, m);
+    uint64_t first = UINT64_MAX;
+    for (uint64_t b = 0; b < nb; ++b) {
+        const tr_volume_brick& br = root->bricks[b];
+        if (!br.data) continue;
+        const uint32_t idx[3] = {(uint32_t)(b % (uint64_t)root->dims[0]),
+                                 (uint32_t)((b / (uint64_t)root->dims[0]) % (uint64_t)root->dims[1]),
+                                 (uint32_t)(b / ((uint64_t)root->dims[0] * root->dims[1]))};
+        VolumeRec r;
+        memset(&r, 0, sizeof(r));
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = (float)idx[a] * vs[a] + root->min_corner[a];
+            r.hi[a] = r.lo[a] + vs[a];
+            r.dims[a] = (uint32_t)br.dims[a];
+            r.scale[a] = (float)(br.dims[a] - 1) / (r.hi[a] - r.lo[a]);
+            r.stddev[a] = m->sggx_stddev[a];
+        }
+        r.format = kVoxelQuantized;
+        r.rcp_rho_max = 1.0f / 255.0f;
+        r.rcp_majorant = 1.0f / kDensityScale;  /* RcpMaxExtinction (:1463) */
+        r.rcp_smax = 1.0f;
+        r.table_base = s->dir_table;
+        r.voxel_base = s->voxels.size();
+        const uint64_t n = (uint64_t)br.dims[0] * br.dims[1] * br.dims[2];
+        s->voxels.reserve(s->voxels.size() + n);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint8_t* q = br.data + n + 2 * i;
+            s->voxels.push_back((uint64_t)q[0] | ((uint64_t)q[1] << 8) | (u8_half_bits(br.data[i]) << 48));
+        }
+        GeomRec g;
+        memset(&g, 0, sizeof(g));
+        g.type = GEOM_VOLUME;
+        g.material = mat;
+        g.tri_count = (uint32_t)s->vols.size();
+        uint32_t slot = (uint32_t)s->geoms.size();
+        s->geoms.push_back(g);
+        s->vols.push_back(r);
+        PrimRec p;
+        memset(&p, 0, sizeof(p));
+        for (int a = 0; a < 3; ++a) { p.v0[a] = r.lo[a]; p.v1[a] = r.hi[a]; }
+        p.prim = 0;
+        p.geom = slot;
+        p.flags = PRIM_VOLUME;
+        s->prims_in.push_back(p);
+        if (first == UINT64_MAX) first = slot;
+    }
+    if (out) *out = first;
     return TR_OK;
 }
 

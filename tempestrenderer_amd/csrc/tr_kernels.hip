@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                  * at the accepted collision (ray-tracing.cc:327-337) */
                 const VolumeRec& V = sc.volumes[G.tri_count];
                 pos = ray_point(org, dir, h4.x);
-                M3 b = volume_frame(fetch_cell(sc.voxels, V, pos));
+                M3 b = volume_frame(fetch_cell(sc.voxels, V, pos), sc.voxels, V);
                 sd.mat = &sc.mats[G.material];
                 sd.tangent = b.c0;
                 sd.binormal = b.c1;

@@ -82,12 +82,20 @@ struct VolumeRec {
     float scale[3];       /* (dims - 1) / (hi - lo) */
     float rcp_smax;       /* 1 / max(stddev) */
     float stddev[3];      /* SGGX standard deviation of the fibre frame (tangent, binormal, normal) */
-    uint32_t pad0;
+    uint32_t format;      /* kVoxelHalf or kVoxelQuantized */
     uint32_t dims[3];
     uint32_t pad1;
     uint64_t voxel_base;  /* first sample in SceneView::voxels */
-    uint64_t pad2;
+    uint64_t table_base;  /* kVoxelQuantized: the direction tables in SceneView::voxels */
 };
+/* Voxel words.  kVoxelHalf: fibre x, y, z and density as binary16 (bits 0,
+ * 16, 32, 48).  kVoxelQuantized (the reference's brick volumes): the
+ * quantized theta and phi bytes at bits 0 and 8 and the u8 density as an
+ * exact binary16 at bit 48; the direction tables cos(phi), sin(phi),
+ * cos(theta), sin(theta) (256 floats each) sit at table_base. */
+enum { kVoxelHalf = 0, kVoxelQuantized = 1 };
+/* DensityScale (ray-tracing.cc:60): the brick volumes' majorant. */
+constexpr float kDensityScale = 100.0f;
 static_assert(sizeof(VolumeRec) == 96, "VolumeRec layout");
 
 struct LightRec {

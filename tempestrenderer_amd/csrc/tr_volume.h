@@ -93,17 +93,31 @@ TR_HD float cell_density(const VoxelCell& c)
  * (SampleVolumePoint, ray-tracing.cc:110-182).  The corner weights are the
  * reference's: the lower corner of an axis weighs f, the upper 1 - f.  A
  * zero result (zero tensor) falls back to +z. */
-TR_HD V3 cell_orientation(const VoxelCell& c)
+TR_HD V3 cell_orientation(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V)
 {
     float lf[3] = {c.f[0], c.f[1], c.f[2]};
     float uf[3] = {1.0f - c.f[0], 1.0f - c.f[1], 1.0f - c.f[2]};
     float t00 = 0.0f, t01 = 0.0f, t02 = 0.0f, t11 = 0.0f, t12 = 0.0f, t22 = 0.0f;
+    const float* tab = reinterpret_cast<const float*>(voxels + V.table_base);
+    const bool quant = V.format == kVoxelQuantized;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         float w = ((k & 4) ? uf[2] : lf[2]) * ((k & 2) ? uf[1] : lf[1]) * ((k & 1) ? uf[0] : lf[0]);
         uint64_t s = c.s[k];
-        float d0 = h2f((uint32_t)(s & 0xffffu)), d1 = h2f((uint32_t)((s >> 16) & 0xffffu)),
-              d2 = h2f((uint32_t)((s >> 32) & 0xffffu));
+        float d0, d1, d2;
+        if (quant) {
+            /* d = (cos phi sin theta, sin phi sin theta, cos theta) from the
+             * quantized angles (ray-tracing.cc:133-144) */
+            uint32_t th = (uint32_t)(s & 0xffu), ph = (uint32_t)((s >> 8) & 0xffu);
+            float sin_t = tab[768 + th];
+            d0 = tab[ph] * sin_t;
+            d1 = tab[256 + ph] * sin_t;
+            d2 = tab[512 + th];
+        } else {
+            d0 = h2f((uint32_t)(s & 0xffffu));
+            d1 = h2f((uint32_t)((s >> 16) & 0xffffu));
+            d2 = h2f((uint32_t)((s >> 32) & 0xffffu));
+        }
         t00 += w * d0 * d0;
         t01 += w * d0 * d1;
         t02 += w * d0 * d2;
@@ -142,19 +156,28 @@ TR_HD M3 make_basis_tangent(V3 tan)
 TR_HD V3 ray_point(V3 org, V3 dir, float t) { return add(org, smul(t, dir)); }
 
 /* The fibre frame at a point and sigma(dir) of the SGGX micro-flakes. */
-TR_HD M3 volume_frame(const VoxelCell& c) { return make_basis_tangent(cell_orientation(c)); }
+TR_HD M3 volume_frame(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V)
+{
+    return make_basis_tangent(cell_orientation(c, voxels, V));
+}
 
 /* One tentative collision: 0 = rejected, 1 = accepted (scatter / absorb),
- * 2 = zero density (the reference's sampleScattering returns false). */
+ * 2 = zero density (the reference's sampleScattering returns false).
+ * Brick volumes follow sampleScattering's own arithmetic (ray-tracing.cc:
+ * 316-337 and the loops' TotalDensity *= RcpMaxExtinction): u8 density / 255
+ * against the threshold, then (DensityScale / 255) * density * sigma *
+ * RcpMaxExtinction. */
 TR_HD int volume_accept(const uint64_t* voxels, const VolumeRec& V, V3 pos, V3 dir, float threshold)
 {
     VoxelCell c = fetch_cell(voxels, V, pos);
     float rho = cell_density(c);
     if (rho == 0.0f) return 2;
-    float dens = rho * V.rcp_rho_max;
+    const bool quant = V.format == kVoxelQuantized;
+    float dens = quant ? rho / 255.0f : rho * V.rcp_rho_max;
     if (dens < threshold) return 0;
-    M3 basis = volume_frame(c);
+    M3 basis = volume_frame(c, voxels, V);
     float sigma = length(mulv(transform_rotation_inverse(basis, dir), ld3(V.stddev)));
+    if (quant) return (kDensityScale / 255.0f) * rho * sigma * V.rcp_majorant >= threshold ? 1 : 0;
     return dens * sigma * V.rcp_smax >= threshold ? 1 : 0;
 }
 

@@ -124,6 +124,82 @@ def grid_volume_record(dims, min_corner, max_corner, density, direction, max_den
     return rec, (d, v)
 
 
+class VolumeRoot:
+    """VolumeRoot (volume/volume.hh:65-83): MinCorner, MaxCorner, Dimensions
+    (bricks per axis) and Volumes, a list in (z, y, x) order of None or
+    (dims (x, y, z), uint8 data: densities then (theta, phi) pairs)."""
+
+    def __init__(self, min_corner, max_corner, dims, volumes):
+        self.MinCorner = tuple(float(v) for v in min_corner)
+        self.MaxCorner = tuple(float(v) for v in max_corner)
+        self.Dimensions = tuple(int(v) for v in dims)
+        self.Volumes = list(volumes)
+
+
+def volume_root_record(root):
+    """A tr_volume_root over the VolumeRoot's arrays (returned too, to keep
+    them alive for the call)."""
+    rec = _abi.tr_volume_root()
+    for a in range(3):
+        rec.min_corner[a] = root.MinCorner[a]
+        rec.max_corner[a] = root.MaxCorner[a]
+        rec.dims[a] = root.Dimensions[a]
+    n = root.Dimensions[0] * root.Dimensions[1] * root.Dimensions[2]
+    if len(root.Volumes) != n:
+        raise ValueError("VolumeRoot.Volumes must hold one entry per brick")
+    arr = (_abi.tr_volume_brick * max(1, n))()
+    keep = [arr]
+    for i, v in enumerate(root.Volumes):
+        if v is None:
+            continue
+        dims, data = v
+        d = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+        if d.size != 3 * int(dims[0]) * int(dims[1]) * int(dims[2]):
+            raise ValueError("a brick holds 3 bytes per voxel")
+        keep.append(d)
+        for a in range(3):
+            arr[i].dims[a] = int(dims[a])
+        arr[i].data = d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    rec.bricks = arr
+    return rec, keep
+
+
+def _root_from_raw(raw):
+    n = raw.dims[0] * raw.dims[1] * raw.dims[2]
+    vols = []
+    for i in range(n):
+        b = raw.bricks[i]
+        if not b.data:
+            vols.append(None)
+            continue
+        dims = tuple(b.dims)
+        size = 3 * dims[0] * dims[1] * dims[2]
+        vols.append((dims, np.ctypeslib.as_array(b.data, shape=(size,)).copy()))
+    return VolumeRoot(tuple(raw.min_corner), tuple(raw.max_corner), tuple(raw.dims), vols)
+
+
+def ParseVolumeHierarchy(name):
+    """ParseVolumeHierarchy (src/volume/volume.cc:361-420)."""
+    lib = _abi.load()
+    raw = _abi.tr_volume_root()
+    check(lib.tr_parse_volume_hierarchy(os.fsencode(name), ctypes.byref(raw)))
+    try:
+        return _root_from_raw(raw)
+    finally:
+        lib.tr_volume_root_free(ctypes.byref(raw))
+
+
+def ParseVolume(density, orientation):
+    """ParseVolume (src/volume/volume.cc:422-453)."""
+    lib = _abi.load()
+    raw = _abi.tr_volume_root()
+    check(lib.tr_parse_volume(os.fsencode(density), os.fsencode(orientation), ctypes.byref(raw)))
+    try:
+        return _root_from_raw(raw)
+    finally:
+        lib.tr_volume_root_free(ctypes.byref(raw))
+
+
 class RayTracerScene:
     """The MI355X tracer behind the reference scene API."""
 
@@ -210,6 +286,16 @@ class RayTracerScene:
         gid = ctypes.c_uint64()
         check(self.lib.tr_scene_add_sphere(self.h, _fptr(c), float(radius), ctypes.byref(material.rec),
                                            ctypes.byref(gid)))
+        self._keep.append(material)
+        return gid.value
+
+    def addHierarchicalVolume(self, root, material):
+        """addHierarchicalVolume (ray-tracing.cc:1448-1504) of a VolumeRoot;
+        returns the first non-empty brick's geometry slot."""
+        rec, keep = volume_root_record(root)
+        gid = ctypes.c_uint64()
+        check(self.lib.tr_scene_add_hierarchical_volume(self.h, ctypes.byref(rec), ctypes.byref(material.rec),
+                                                        ctypes.byref(gid)))
         self._keep.append(material)
         return gid.value
 

@@ -83,6 +83,7 @@ class SceneDesc:
         self.lights = []       # dict(kind=..., ...)
         self.sphere_lights = []
         self.volumes = []      # (dims, min, max, density f16 [nz, ny, nx], direction f16 [nz, ny, nx, 3], material)
+        self.brick_volumes = []  # (VolumeRoot, material)
         self.settings = dict(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=1.0,
                              background=(0.0, 0.0, 0.0), seed=1, estimator=0)
 
@@ -120,6 +121,8 @@ class SceneDesc:
             target.addSphere(center, radius, mat(m))
         for dims, lo, hi, dens, dirs, m in self.volumes:
             target.addGridVolume(dims, lo, hi, dens, dirs, mat(m))
+        for root, m in self.brick_volumes:
+            target.addHierarchicalVolume(root, mat(m))
         for L in self.sphere_lights:
             target.addSphereLightSource(target.make_light(**L))
         for L in self.lights:
@@ -306,6 +309,50 @@ def volume(width=1024, height=1024, samples_camera=128, max_depth=32, grid=256, 
     dens, dirs = volume_grid(grid, seed=seed)
     flakes = dict(model=SGGX_MICRO_FLAKE, albedo=(0.9, 0.9, 0.9), sggx_stddev=(0.1, 1.0, 1.0))
     d.volumes.append(((grid, grid, grid), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), dens, dirs, flakes))
+    if with_floor_light:
+        qv, qt = _quad((0.0, -0.6, 0.0), (0.0, 0.0, 1.5), (1.5, 0.0, 0.0))
+        floor = dict(model=GGX, diffuse=(0.6, 0.6, 0.6), specular=(0.0, 0.0, 0.0), specular_power=(1.0, 1.0))
+        d.meshes.append((qv, qt, [(floor, 0, 6)], False))
+        ld = np.array([0.3, 0.9, 0.2]) / np.linalg.norm([0.3, 0.9, 0.2])
+        d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(1.0, 1.0, 1.0), seed=1)
+    return d
+
+
+def brick_volume_root(bricks=(2, 1, 2), n=12, seed=3, empty=((1, 0, 0),), lo=(-0.5, -0.5, -0.5), hi=(0.5, 0.5, 0.5)):
+    """A VolumeRoot of `bricks` bricks of n^3 voxels in the reference's brick
+    layout: u8 densities from 3-D value noise (about a third of the voxels
+    zero) followed by (theta, phi) byte pairs of a smooth fibre field;
+    the bricks listed in `empty` have no data."""
+    from .scene import VolumeRoot
+    vols = []
+    k = 0
+    for z in range(bricks[2]):
+        for y in range(bricks[1]):
+            for x in range(bricks[0]):
+                k += 1
+                if (x, y, z) in empty:
+                    vols.append(None)
+                    continue
+                noise = value_noise_3d(n, octaves=3, base_freq=3, seed=seed + k)
+                dens = np.clip((noise - 0.2) * 500.0, 0.0, 255.0).astype(np.uint8).reshape(-1)
+                f1 = value_noise_3d(n, octaves=2, base_freq=2, seed=seed + 100 + k).reshape(-1)
+                f2 = value_noise_3d(n, octaves=2, base_freq=2, seed=seed + 200 + k).reshape(-1)
+                theta = np.clip(f1 * 255.0, 0, 255).astype(np.uint8)
+                phi = np.clip(f2 * 255.0, 0, 255).astype(np.uint8)
+                data = np.concatenate([dens, np.stack([theta, phi], axis=1).reshape(-1)])
+                vols.append(((n, n, n), data))
+    return VolumeRoot(lo, hi, bricks, vols)
+
+
+def brick_volume(width=48, height=48, samples_camera=4, max_depth=8, seed=3, with_floor_light=True):
+    """A reference-format brick volume (addHierarchicalVolume) over a floor
+    lit by a directional light, in a white sky."""
+    vpi = perspective_view_proj_inv((0.3, 0.45, 2.1), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, width / height)
+    d = SceneDesc("brick_volume", width, height, vpi)
+    flakes = dict(model=SGGX_MICRO_FLAKE, albedo=(0.8, 0.85, 0.9), sggx_stddev=(0.2, 1.0, 1.0))
+    d.brick_volumes.append((brick_volume_root(seed=seed), flakes))
     if with_floor_light:
         qv, qt = _quad((0.0, -0.6, 0.0), (0.0, 0.0, 1.5), (1.5, 0.0, 0.0))
         floor = dict(model=GGX, diffuse=(0.6, 0.6, 0.6), specular=(0.0, 0.0, 0.0), specular_power=(1.0, 1.0))

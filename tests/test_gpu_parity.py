@@ -122,6 +122,32 @@ def test_paths_volume_floor_light():
     _check_paths(g_rgb, o_rgb, g_hash, o_hash)
 
 
+def test_trace_ids_brick_volume():
+    """The reference's brick volumes (addHierarchicalVolume): one box per
+    non-empty brick, the empty brick absent."""
+    print("case: trace ids brick volume", flush=True)
+    desc = workloads.brick_volume(32, 32, samples_camera=1, max_depth=2)
+    g, o = _pair(desc)
+    rays = np.concatenate([workloads.primary_rays(desc), workloads.random_rays(desc, 1 << 14)])
+    hb = o.traceRays(rays, brute_force=True)
+    hg = g.traceRays(rays)
+    assert _ids_equal(hb, hg).all()
+    hit = hb["geom"] != 0xFFFFFFFF
+    assert np.array_equal(hb["t"][hit], hg["t"][hit]) and np.array_equal(hb["u"][hit], hg["u"][hit])
+
+
+def test_paths_brick_volume():
+    """u8 densities, quantized fibre angles through the cos/sin tables and
+    sampleScattering's brick arithmetic, with Woodcock shadow rays."""
+    print("case: paths brick volume 40x40 spp4 depth8", flush=True)
+    desc = workloads.brick_volume(40, 40, samples_camera=4, max_depth=8)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
 def test_timing_mode_renders_the_same_frame():
     """Launches that carry timing events compute what plain launches do."""
     print("case: timing mode box", flush=True)

@@ -104,7 +104,12 @@ enum {
  * Emissive), fresnel {eta, k}, normalization.  RTSGGXSurface (:397-406):
  * sggx_basis quaternion (x, y, z, w), sggx_depth, sggx_sample_count.
  * RTVolumeMaterial / RTSGGXMicroFlakeMaterial (:379-395): albedo,
- * sggx_stddev.  tr_material_init fills the reference's defaults. */
+ * sggx_stddev.  The texture maps DiffuseMap, SpecularMap (:171-172),
+ * StandardDeviationMap and BasisMap (:404-405) are handles from
+ * tr_scene_bind_texture of the scene the material is added to (0 = none);
+ * Blinn-Phong reads DiffuseMap (illumination-models.hh:618-698), the SGGX
+ * surface all four (ggx-models-impl.hh:476-619).  tr_material_init fills
+ * the reference's defaults. */
 typedef struct tr_material {
     uint32_t model;
     float specular[3];
@@ -117,6 +122,10 @@ typedef struct tr_material {
     uint32_t sggx_sample_count;
     float albedo[3];
     float sggx_stddev[3];
+    uint32_t diffuse_map;
+    uint32_t specular_map;
+    uint32_t stddev_map;
+    uint32_t basis_map;
 } tr_material;
 
 /* LightSource and its subclasses (ray-tracing-common.hh:486-544). */
@@ -231,6 +240,39 @@ tr_status tr_scene_set_samples_camera(tr_scene* scene, uint32_t rays);
 tr_status tr_scene_set_max_ray_depth(tr_scene* scene, uint32_t depth);
 tr_status tr_scene_set_russian_roulette(tr_scene* scene, float rval);
 tr_status tr_scene_set_background_spectrum(tr_scene* scene, const float rgb[3]);
+
+/* Textures (graphics/texture.hh:373-600): texels row by row, x fastest;
+ * Texture::sample's bilinear (wrapping, texel centres at +0.5) or nearest
+ * filter.  Extractors: spectrum maps (DiffuseMap, SpecularMap) take RGBA8
+ * as sRGB (SRGBToSpectrum) or RGB32F / RGBA32F as linear; the
+ * standard-deviation map takes RG8 / RG32F / RGB32F / RGBA32F; the basis map
+ * RGBA8 / RGBA32F. */
+enum {
+    TR_TEX_R8_UNORM = 0,
+    TR_TEX_RG8_UNORM = 1,
+    TR_TEX_RGBA8_UNORM = 2,
+    TR_TEX_R32F = 3,
+    TR_TEX_RG32F = 4,
+    TR_TEX_RGB32F = 5,
+    TR_TEX_RGBA32F = 6
+};
+enum { TR_TEX_BILINEAR = 0, TR_TEX_NEAREST = 1 };
+typedef struct tr_texture_desc {
+    uint32_t width, height;   /* 1..65535, TextureDescription's uint16 */
+    uint32_t format;          /* TR_TEX_* */
+    uint32_t sampling;        /* TR_TEX_BILINEAR or TR_TEX_NEAREST */
+} tr_texture_desc;
+/* bindTexture (ray-tracing.hh:340-354): copies the texels; *out_handle
+ * (1, 2, ...) goes into tr_material map fields. */
+tr_status tr_scene_bind_texture(tr_scene* scene, const tr_texture_desc* desc, const void* texels,
+                                uint32_t* out_handle);
+/* setGlobalCubeMap (ray-tracing.hh:316; CubeMap, src/graphics/cube-map.cc):
+ * six square RGBA8 faces (+x, -x, +y, -y, +z, -z) of side > 16, each as
+ * the reference's Texture holds it (the faces are flipped vertically on
+ * copy, cube-map.cc:58-66).  Rays that miss look up the map (clamped
+ * bilinear, sRGB) instead of the background spectrum.  faces = NULL
+ * removes it. */
+tr_status tr_scene_set_global_cube_map(tr_scene* scene, uint32_t side, const uint32_t* const faces[6]);
 tr_status tr_scene_set_picture_post_process(tr_scene* scene, uint32_t mode);
 tr_status tr_scene_set_transparent_background(tr_scene* scene, int32_t enabled);
 tr_status tr_scene_set_render_mode(tr_scene* scene, uint32_t mode);

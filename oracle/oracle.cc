@@ -121,6 +121,7 @@ struct orc_scene {
     std::vector<BNode> nodes;
     std::vector<Light> lights;
     std::vector<GridVolume> volumes;
+    OrcTexSet texset;          /* bound textures and the global cube map */
     uint32_t instances = 0;
     bool committed = false;
 };
@@ -364,6 +365,8 @@ void sample_geometry(const orc_scene* s, const tr_hit& h, V3<T> org, V3<T> dir, 
 {
     const Geom& g = s->geoms[h.geom];
     sd->mat = &s->materials[g.material];
+    sd->tex = &s->texset;
+    sd->tcx = sd->tcy = (T)0;
     if (g.type == GEOM_MESH) {
         /* RTMesh::sample + DefaultMeshSample (ray-tracing.cc:375-461) */
         const int32_t* ib = &s->indices[g.index_base + (size_t)h.prim * 3];
@@ -383,6 +386,15 @@ void sample_geometry(const orc_scene* s, const tr_hit& h, V3<T> org, V3<T> dir, 
         V3<T> norm = normalize(N0 * w + N1 * u + N2 * v);
         sd->tangent = v3((T)0, (T)0, (T)0);
         sd->binormal = v3((T)0, (T)0, (T)0);
+        sd->tcx = sd->tcy = (T)0;
+        if (g.stride >= 32) {
+            /* TexCoord = tc0 * w + tc1 * u + tc2 * v (ray-tracing.cc:388-394) */
+            const float* c0 = reinterpret_cast<const float*>(p0 + 12);
+            const float* c1 = reinterpret_cast<const float*>(p1 + 12);
+            const float* c2 = reinterpret_cast<const float*>(p2 + 12);
+            sd->tcx = (T)c0[0] * w + (T)c1[0] * u + (T)c2[0] * v;
+            sd->tcy = (T)c0[1] * w + (T)c1[1] * u + (T)c2[1] * v;
+        }
         if (g.stride >= 44) {
             const float* t0 = reinterpret_cast<const float*>(p0 + 20);
             const float* t1 = reinterpret_cast<const float*>(p1 + 20);
@@ -690,6 +702,17 @@ V3<T> evaluate_lights(const orc_scene* s, const Rng& rng, uint32_t vertex, const
 
 template <class T> inline V3<T> to_v3(const float* f) { return v3((T)f[0], (T)f[1], (T)f[2]); }
 
+/* What a ray that leaves the scene sees: m_GlobalCubeMap->sampleSpectrum
+ * of its direction, or the background spectrum (ray-tracing.cc:2060,
+ * 2224). */
+template <class T> inline V3<T> env_radiance(const orc_scene* s, const float dir[3])
+{
+    if (!s->texset.cube_side) return to_v3<T>(s->background);
+    float c[3];
+    orc_cube_sample(s->texset, dir, c);
+    return to_v3<T>(c);
+}
+
 /* Spectrum < TransmittanceThreshold: every component below 1e-3
  * (vector3.hh:205, ray-tracing.cc:58). */
 template <class T> inline bool below_threshold(V3<T> t)
@@ -809,7 +832,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
     if (h.geom == TR_INVALID_GEOMETRY) {
         hash = (hash ^ 0xffffffffu) * 16777619u;
         *hash_out = repeat_hash(hash, s->n_gi);
-        return to_v3<T>(s->background);
+        return env_radiance<T>(s, df);
     }
     hash_hit(&hash, h);
     bool volume = surface_or_volume(s, h, &pos, &sd);
@@ -877,7 +900,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             tr_hit hh = trace_segment(s, o2, d2, srng, d + 1);
             if (hh.geom == TR_INVALID_GEOMETRY) {
                 hash = (hash ^ 0xffffffffu) * 16777619u;
-                gsplit += to_v3<T>(s->background) * ct;
+                gsplit += env_radiance<T>(s, d2) * ct;
                 break;
             }
             hash_hit(&hash, hh);
@@ -1229,6 +1252,52 @@ int orc_add_hierarchical_volume(orc_scene* s, const tr_volume_root* root, const 
     return 0;
 }
 
+/* bindTexture: a copy of the texels; handles count from 1. */
+int orc_bind_texture(orc_scene* s, const tr_texture_desc* d, const void* texels, uint32_t* out)
+{
+    if (s->committed || orc_texel_size(d->format) == 0) return -1;
+    OrcTexture t;
+    t.w = d->width;
+    t.h = d->height;
+    t.format = d->format;
+    t.sampling = d->sampling;
+    size_t bytes = (size_t)t.w * t.h * orc_texel_size(t.format);
+    t.data.assign((const uint8_t*)texels, (const uint8_t*)texels + bytes);
+    s->texset.tex.push_back(std::move(t));
+    *out = (uint32_t)s->texset.tex.size();
+    return 0;
+}
+
+/* CubeMap(Texture**) (src/graphics/cube-map.cc:32-71): six faces, each
+ * copied with its rows in reverse order. */
+int orc_set_global_cube_map(orc_scene* s, uint32_t side, const uint32_t* const faces[6])
+{
+    if (s->committed) return -1;
+    if (!faces) {
+        s->texset.cube_side = 0;
+        s->texset.cube.clear();
+        return 0;
+    }
+    size_t pitch = (size_t)side * 4;
+    s->texset.cube.assign(6 * pitch * side, 0);
+    for (int f = 0; f < 6; ++f)
+        for (size_t y = 0; y < side; ++y)
+            memcpy(s->texset.cube.data() + (size_t)f * pitch * side + y * pitch,
+                   (const uint8_t*)faces[f] + (side - 1 - y) * pitch, pitch);
+    s->texset.cube_side = side;
+    return 0;
+}
+
+/* One texture lookup (tests): kind 0 spectrum, 1 RG, 2 RGBA. */
+void orc_texture_sample(const orc_scene* s, uint32_t handle, float u, float v, int kind, float out[4])
+{
+    Rgba r = orc_tex_sample(s->texset, handle, u, v, kind);
+    memcpy(out, r.v, 16);
+}
+
+/* One cube-map lookup (tests). */
+void orc_cube_map_sample(const orc_scene* s, const float dir[3], float out[3]) { orc_cube_sample(s->texset, dir, out); }
+
 int orc_add_light_source(orc_scene* s, const tr_light* light, uint32_t* out)
 {
     Light L;
@@ -1461,8 +1530,7 @@ float orc_resolve(size_t n, const double* accum, double rcp_count, uint32_t post
 static Sample<float> unit_sample(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
                                  const float* wi, float dir_density)
 {
-    Sample<float> s;
-    memset(&s, 0, sizeof(s));
+    Sample<float> s{};
     s.mat = m;
     s.normal = v3(n[0], n[1], n[2]);
     s.tangent = t ? v3(t[0], t[1], t[2]) : v3(0.0f, 0.0f, 0.0f);

@@ -15,6 +15,7 @@
 #define ORC_BSDF_H
 
 #include "orc_math.h"
+#include "orc_texture.h"
 #include "../include/tempest_rt.h"
 
 namespace orc {
@@ -31,7 +32,17 @@ template <class T> struct Sample {
     T total_density;
     T dir_density;  /* DirectionalDensity */
     T pdf;
+    T tcx = 0, tcy = 0;             /* TexCoord */
+    const OrcTexSet* tex = nullptr; /* the scene's textures (none in unit calls) */
 };
+
+/* A spectrum map at the TexCoord (SampleSpectrum) or the constant. */
+template <class T> inline V3<T> map_spectrum(const Sample<T>& s, uint32_t map, const float* constant)
+{
+    if (map == 0 || !s.tex) return v3((T)constant[0], (T)constant[1], (T)constant[2]);
+    Rgba c = orc_tex_sample(*s.tex, map, (float)s.tcx, (float)s.tcy, ORC_TEX_SPECTRUM);
+    return v3((T)c.v[0], (T)c.v[1], (T)c.v[2]);
+}
 
 template <class T> struct Strata {
     T x, y, tx, ty;
@@ -251,23 +262,25 @@ inline float xpow(float x, float p)
 }
 inline double xpow(double x, double p) { return pow(x, p); }
 
-template <class T> inline T lobe_ratio(const tr_material* m)
+template <class T> inline T lobe_ratio(V3<T> albedo, V3<T> specular)
 {
-    T mv = max_value(spec3<T>(m->diffuse) + spec3<T>(m->specular));
-    return mv != (T)0 ? max_value(spec3<T>(m->specular)) / mv : (T)0;
+    T mv = max_value(albedo + specular);
+    return mv != (T)0 ? max_value(specular) / mv : (T)0;
 }
 
-/* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF times cos. */
+/* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF times cos, the
+ * albedo from DiffuseMap when there is one. */
 template <class T> inline V3<T> blinn_phong_brdf(const Sample<T>& s)
 {
     const tr_material* m = s.mat;
+    V3<T> albedo = map_spectrum(s, m->diffuse_map, m->diffuse);
     T power = (T)m->specular_power[0];
     T dot_nl = xmax((T)0, dot(s.normal, s.wi));
     V3<T> h = normalize(s.wi + s.wo);
     V3<T> spec = spec3<T>(m->specular) * xpow(xmax((T)0, dot(s.normal, h)), power);
     T cos_theta = xmin(xmax(dot(s.wi, h), (T)0), (T)1);
     T fresnel = fresnel_schlick((T)m->fresnel[0], cos_theta);
-    return (spec3<T>(m->diffuse) + spec * fresnel * (power + (T)8) * ((T)1 / (T)8)) * dot_nl * ((T)1 / C<T>::pi);
+    return (albedo + spec * fresnel * (power + (T)8) * ((T)1 / (T)8)) * dot_nl * ((T)1 / C<T>::pi);
 }
 
 /* BlinnPhongPDF (illumination-models.hh:618-643) */
@@ -277,7 +290,7 @@ template <class T> inline T blinn_phong_pdf(const Sample<T>& s)
     V3<T> h = normalize(s.wi + s.wo);
     T dot_ln = dot(s.wi, s.normal);
     if (dot_ln < (T)0) return (T)0;
-    T ratio = lobe_ratio<T>(m);
+    T ratio = lobe_ratio<T>(map_spectrum(s, m->diffuse_map, m->diffuse), spec3<T>(m->specular));
     T power = (T)m->specular_power[0];
     T pdf0 = (power + (T)1) * xpow(dot(h, s.normal), power) * ((T)1 / ((T)2 * C<T>::pi));
     T transform4 = (T)4 * dot(h, s.wo);
@@ -295,7 +308,7 @@ template <class T> inline void blinn_phong_sample(const Strata<T>& st, Sample<T>
     const tr_material* m = s->mat;
     T x_ray = (st.x + xi[0]) / st.tx;
     T y_ray = (st.y + xi[1]) / st.ty;
-    T ratio = lobe_ratio<T>(m);
+    T ratio = lobe_ratio<T>(map_spectrum(*s, m->diffuse_map, m->diffuse), spec3<T>(m->specular));
     M3<T> basis = make_basis(s->normal);
     T sin_phi, cos_phi;
     sincos_turns(y_ray, &sin_phi, &cos_phi);
@@ -350,33 +363,52 @@ template <class T> inline T sggx_ndf(V3<T> stddev, const M3<T>& basis, V3<T> n)
     return (T)0.5 * xsqrt(det) / (C<T>::pi * den * den);
 }
 
-template <class T> inline V3<T> sggx_surface_stddev(const tr_material* m)
+/* StandardDeviation (x, y, 1) from StandardDeviationMap (SampleRG) or
+ * the material; SGGXBasis from BasisMap (SampleRGBA) or the material
+ * (ggx-models-impl.hh:495-517, 593-608). */
+template <class T> inline V3<T> sggx_surface_stddev(const Sample<T>& s)
 {
-    return v3((T)m->specular_power[0], (T)m->specular_power[1], (T)1);
+    const tr_material* m = s.mat;
+    if (m->stddev_map == 0 || !s.tex) return v3((T)m->specular_power[0], (T)m->specular_power[1], (T)1);
+    Rgba c = orc_tex_sample(*s.tex, m->stddev_map, (float)s.tcx, (float)s.tcy, ORC_TEX_RG);
+    return v3((T)c.v[0], (T)c.v[1], (T)1);
+}
+template <class T> inline void sggx_surface_quat(const Sample<T>& s, T q[4])
+{
+    const tr_material* m = s.mat;
+    if (m->basis_map == 0 || !s.tex) {
+        for (int i = 0; i < 4; ++i) q[i] = (T)m->sggx_basis[i];
+        return;
+    }
+    Rgba c = orc_tex_sample(*s.tex, m->basis_map, (float)s.tcx, (float)s.tcy, ORC_TEX_RGBA);
+    for (int i = 0; i < 4; ++i) q[i] = (T)c.v[i];
 }
 
-/* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619 */
+/* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619
+ * (the quaternion is not normalised there) */
 template <class T> inline T sggx_surface_density(const Sample<T>& s)
 {
-    T q[4] = {(T)s.mat->sggx_basis[0], (T)s.mat->sggx_basis[1], (T)s.mat->sggx_basis[2], (T)s.mat->sggx_basis[3]};
+    T q[4];
+    sggx_surface_quat(s, q);
     M3<T> orient = m3(s.tangent, s.binormal, s.normal);
     M3<T> basis = mul(orient, quat_to_m3(q));
-    return sggx_projected_area(sggx_surface_stddev<T>(s.mat), basis, s.wi);
+    return sggx_projected_area(sggx_surface_stddev(s), basis, s.wi);
 }
 
-/* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551 (no texture maps). */
+/* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551. */
 template <class T> inline V3<T> sggx_surface_brdf(const Sample<T>& s)
 {
     const tr_material* m = s.mat;
     T din = dot(s.normal, s.wi);
     T don = dot(s.normal, s.wo);
     if (din < (T)0 || don < (T)0) return v3((T)0, (T)0, (T)0);
-    V3<T> result = spec3<T>(m->diffuse) * din * ((T)1 / C<T>::pi);
-    V3<T> stddev = sggx_surface_stddev<T>(m);
+    V3<T> result = map_spectrum(s, m->diffuse_map, m->diffuse) * din * ((T)1 / C<T>::pi);
+    V3<T> stddev = sggx_surface_stddev(s);
     if (stddev.x >= (T)1e-6f && stddev.y >= (T)1e-6f && stddev.z >= (T)1e-6f && s.dir_density >= (T)1e-6f) {
         V3<T> mn = normalize(s.wi + s.wo);
         M3<T> orient = m3(s.tangent, s.binormal, s.normal);
-        T q[4] = {(T)m->sggx_basis[0], (T)m->sggx_basis[1], (T)m->sggx_basis[2], (T)m->sggx_basis[3]};
+        T q[4];
+        sggx_surface_quat(s, q);
         T qn[4];
         quat_normalize(q, qn);
         M3<T> basis = mul(orient, quat_to_m3(qn));
@@ -393,7 +425,8 @@ template <class T> inline V3<T> sggx_surface_brdf(const Sample<T>& s)
         T out_density = s.dir_density;
         T in_density = sggx_projected_area(stddev, basis, s.wi);
         T contrib = (T)1 / (out_density + in_density * don / din);
-        result += contrib * spec3<T>(m->specular) * sggx_ndf(stddev, basis, mn) * fresnel * (T)2 * (T)0.25;
+        result += contrib * map_spectrum(s, m->specular_map, m->specular) * sggx_ndf(stddev, basis, mn) * fresnel *
+                  (T)2 * (T)0.25;
     }
     return result;
 }

@@ -27,6 +27,10 @@ class tr_material(ctypes.Structure):
         ("sggx_sample_count", ctypes.c_uint32),
         ("albedo", ctypes.c_float * 3),
         ("sggx_stddev", ctypes.c_float * 3),
+        ("diffuse_map", ctypes.c_uint32),
+        ("specular_map", ctypes.c_uint32),
+        ("stddev_map", ctypes.c_uint32),
+        ("basis_map", ctypes.c_uint32),
     ]
 
 
@@ -101,6 +105,15 @@ class tr_volume_brick(ctypes.Structure):
 class tr_volume_root(ctypes.Structure):
     _fields_ = [("min_corner", ctypes.c_float * 3), ("max_corner", ctypes.c_float * 3), ("dims", ctypes.c_int32 * 3),
                 ("bricks", ctypes.POINTER(tr_volume_brick))]
+
+
+class tr_texture_desc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
+                ("sampling", ctypes.c_uint32)]
+
+
+TR_TEX_R8_UNORM, TR_TEX_RG8_UNORM, TR_TEX_RGBA8_UNORM, TR_TEX_R32F, TR_TEX_RG32F, TR_TEX_RGB32F, TR_TEX_RGBA32F = range(7)
+TR_TEX_BILINEAR, TR_TEX_NEAREST = 0, 1
 
 
 class tr_request(ctypes.Structure):
@@ -196,6 +209,8 @@ SIGNATURES = {
     "tr_scene_frame_wait": (_st, [_vp]),
     "tr_scene_read_backbuffer": (_st, [_vp, P(_u32), _sz]),
     "tr_scene_read_accum": (_st, [_vp, P(ctypes.c_double), _sz, P(_u64)]),
+    "tr_scene_bind_texture": (_st, [_vp, P(tr_texture_desc), _vp, P(_u32)]),
+    "tr_scene_set_global_cube_map": (_st, [_vp, _u32, P(_vp)]),
     "tr_parse_volume_hierarchy": (_st, [ctypes.c_char_p, P(tr_volume_root)]),
     "tr_parse_volume": (_st, [ctypes.c_char_p, ctypes.c_char_p, P(tr_volume_root)]),
     "tr_volume_root_free": (None, [P(tr_volume_root)]),

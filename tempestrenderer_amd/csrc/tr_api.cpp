@@ -105,6 +105,15 @@ struct tr_scene {
     DevBuf<int32_t> d_ipool;
     DevBuf<VolumeRec> d_vols;
     DevBuf<uint64_t> d_voxels;
+    std::vector<TexRec> tex_recs;    /* bound textures */
+    std::vector<uint8_t> texels;     /* their texels, then the cube map's faces */
+    std::vector<uint8_t> cube;       /* six flipped RGBA8 faces */
+    uint32_t cube_side = 0;
+    DevBuf<TexRec> d_tex;
+    DevBuf<uint8_t> d_texels;
+    DevBuf<float> d_srgb;
+    std::vector<float> srgb;         /* host copy of the sRGB decode table */
+    uint64_t cube_offset = 0;
 
     /* path pool */
     uint32_t capacity = 0;
@@ -228,6 +237,12 @@ SceneView scene_view(const tr_scene* s)
     v.n_lights = (uint32_t)s->lights.size();
     v.n_prims = (uint32_t)s->prims.size();
     v.n_volumes = (uint32_t)s->vols.size();
+    v.tex.recs = s->d_tex.ptr;
+    v.tex.texels = s->d_texels.ptr;
+    v.tex.srgb = s->d_srgb.ptr;
+    v.tex.count = (uint32_t)s->tex_recs.size();
+    v.tex.cube_side = s->cube_side;
+    v.tex.cube_offset = s->cube_offset;
     return v;
 }
 
@@ -1168,10 +1183,103 @@ tr_status tr_scene_add_hierarchical_volume(tr_scene* s, const tr_volume_root* ro
     return TR_OK;
 }
 
+tr_status tr_scene_bind_texture(tr_scene* s, const tr_texture_desc* d, const void* texels, uint32_t* out)
+{
+    if (!valid(s) || !d || !texels || !out) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "textures are bound before commitScene");
+    if (d->width < 1 || d->height < 1 || d->width > 65535 || d->height > 65535)
+        return fail(TR_ERROR_INVALID_ARG, "texture sides are 1..65535");
+    const uint32_t tb = texel_bytes(d->format);
+    if (tb == 0 || d->format > TR_TEX_RGBA32F) return fail(TR_ERROR_INVALID_ARG, "unknown texture format");
+    if (d->sampling != TR_TEX_BILINEAR && d->sampling != TR_TEX_NEAREST)
+        return fail(TR_ERROR_INVALID_ARG, "unknown texture sampling");
+    const size_t bytes = (size_t)d->width * d->height * tb;
+    TexRec r;
+    memset(&r, 0, sizeof(r));
+    r.width = d->width;
+    r.height = d->height;
+    r.format = d->format;
+    r.sampling = d->sampling;
+    r.offset = (s->texels.size() + 15) & ~(size_t)15;
+    s->texels.resize(r.offset + bytes);
+    memcpy(s->texels.data() + r.offset, texels, bytes);
+    s->tex_recs.push_back(r);
+    *out = (uint32_t)s->tex_recs.size();
+    return TR_OK;
+}
+
+tr_status tr_scene_set_global_cube_map(tr_scene* s, uint32_t side, const uint32_t* const faces[6])
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "the cube map is set before commitScene");
+    if (!faces) {
+        s->cube.clear();
+        s->cube_side = 0;
+        return TR_OK;
+    }
+    if (side <= 16 || side > 16384) return fail(TR_ERROR_INVALID_ARG, "cube map sides are 17..16384");
+    for (int f = 0; f < 6; ++f)
+        if (!faces[f]) return fail(TR_ERROR_INVALID_ARG, "null cube map face");
+    const size_t pitch = (size_t)side * 4;
+    s->cube.resize(6 * pitch * side);
+    for (int f = 0; f < 6; ++f) {
+        uint8_t* dst = s->cube.data() + (size_t)f * pitch * side;
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(faces[f]);
+        for (size_t y = 0; y < side; ++y) memcpy(dst + y * pitch, src + (side - 1 - y) * pitch, pitch);
+    }
+    s->cube_side = side;
+    return TR_OK;
+}
+
+namespace {
+
+/* A material's texture handles against the bound textures and the
+ * extractor each model applies to them. */
+tr_status check_maps(const tr_scene* s, const tr_material& m)
+{
+    const uint32_t n = (uint32_t)s->tex_recs.size();
+    auto ok = [&](uint32_t h, int kind) {
+        return h == 0 || (h <= n && texel_kind_ok(s->tex_recs[h - 1].format, kind));
+    };
+    if (m.model == TR_MODEL_BLINN_PHONG && !ok(m.diffuse_map, TEX_SPECTRUM))
+        return fail(TR_ERROR_INVALID_ARG, "DiffuseMap: unknown texture or a format SampleSpectrum does not read");
+    if (m.model == TR_MODEL_SGGX_SURFACE) {
+        if (!ok(m.diffuse_map, TEX_SPECTRUM) || !ok(m.specular_map, TEX_SPECTRUM))
+            return fail(TR_ERROR_INVALID_ARG, "Diffuse/SpecularMap: unknown texture or unreadable format");
+        if (!ok(m.stddev_map, TEX_RG))
+            return fail(TR_ERROR_INVALID_ARG, "StandardDeviationMap: unknown texture or a format SampleRG does not read");
+        if (!ok(m.basis_map, TEX_RGBA))
+            return fail(TR_ERROR_INVALID_ARG, "BasisMap: unknown texture or a format SampleRGBA does not read");
+    }
+    return TR_OK;
+}
+
+/* SRGBToSpectrum of 8-bit channels: ToVector3's c * (1 / 255)
+ * (spectrum.hh:815-819), then ConvertSLuminanceToLinear (:156-162). */
+void srgb_table(std::vector<float>* t)
+{
+    t->resize(256);
+    for (int i = 0; i < 256; ++i) {
+        float c = (float)i * (1.0f / 255.0f);
+        (*t)[i] = c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+    }
+}
+
+}  // namespace
+
 tr_status tr_scene_commit(tr_scene* s)
 {
     if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
     if (!s->committed) {
+        for (const tr_material& m : s->mats) {
+            tr_status ms = check_maps(s, m);
+            if (ms != TR_OK) return ms;
+        }
+        if (s->cube_side) {
+            s->cube_offset = (s->texels.size() + 15) & ~(size_t)15;
+            s->texels.resize(s->cube_offset + s->cube.size());
+            memcpy(s->texels.data() + s->cube_offset, s->cube.data(), s->cube.size());
+        }
         auto t0 = std::chrono::steady_clock::now();
         std::string err;
         if (!build_bvh8(s->prims_in, s->nodes, s->prims, &s->bvh_stats, &err)) return fail(TR_ERROR_INVALID_ARG, err);
@@ -1191,6 +1299,10 @@ tr_status tr_scene_commit(tr_scene* s)
     if ((st = upload(s, s->d_ipool, s->ipool)) != TR_OK) return st;
     if ((st = upload(s, s->d_vols, s->vols)) != TR_OK) return st;
     if ((st = upload(s, s->d_voxels, s->voxels)) != TR_OK) return st;
+    if ((st = upload(s, s->d_tex, s->tex_recs)) != TR_OK) return st;
+    if ((st = upload(s, s->d_texels, s->texels)) != TR_OK) return st;
+    srgb_table(&s->srgb);
+    if ((st = upload(s, s->d_srgb, s->srgb)) != TR_OK) return st;
     s->lights_dirty = true;
     if ((st = sync_lights(s)) != TR_OK) return st;
     TR_HIP(hipStreamSynchronize(s->stream));

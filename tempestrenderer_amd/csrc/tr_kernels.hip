@@ -506,7 +506,9 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         }
         if (geom == TR_INVALID_GEOMETRY) {
             hash = (hash ^ 0xffffffffu) * kFnvPrime;
-            V3 env = ld3(p.background);
+            /* m_GlobalCubeMap->sampleSpectrum(ray direction) or the
+             * background spectrum (ray-tracing.cc:2060, 2224) */
+            V3 env = sc.tex.cube_side ? cube_sample(sc.tex, dir) : ld3(p.background);
             if (vertex == 0) { local = env; write_local = true; }
             else global = add(global, mulv(env, T));
         } else {
@@ -528,6 +530,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 sd.binormal = b.c1;
                 sd.normal = b.c2;
                 sd.wi = dir;
+                sd.tcx = sd.tcy = 0.0f;
                 sd.dir_density = sggx_projected_area(ld3(V.stddev), b, dir);
                 /* throughput *= albedo; a path below TransmittanceThreshold
                  * ends (ray-tracing.cc:2136-2142, 2277-2284) */
@@ -536,7 +539,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 sample_geometry(sc, geom, prim, h4.x, h4.y, h4.z, org, dir, &sd);
                 pos = add(org, smul(h4.x, dir));
                 sd.wi = dir;
-                material_cache(&sd);
+                material_cache(&sd, sc.tex);
             }
             if (volume && T.x < 1e-3f && T.y < 1e-3f && T.z < 1e-3f) {
                 /* absorbed: no contribution from this vertex on */
@@ -618,14 +621,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                                 float balance;
                                 if (unbiased) {
                                     float a = (float)samples * smp.pdf;
-                                    balance = a / (a + (float)ss * bsdf_pdf(lsd));
+                                    balance = a / (a + (float)ss * bsdf_pdf(lsd, sc.tex));
                                 } else {
                                     float total = total_balance(sc, (float)ls, (float)ss, lsd, pos);
                                     balance = (light_pdf(sc, L, pos, lsd.wi) * (float)samples) / total;
                                 }
                                 curT = muls(curT, balance);
                             }
-                            V3 contrib = divs(mulv(mulv(smp.radiance, curT), bsdf_eval(lsd)), smp.pdf);
+                            V3 contrib = divs(mulv(mulv(smp.radiance, curT), bsdf_eval(lsd, sc.tex)), smp.pdf);
                             size_t e = (size_t)ne * pool.capacity + slot;
                             pool.sh_o[e] = st4(pos, smp.dist);
                             pool.sh_d[e] = st4(smp.dir, __uint_as_float(smp.geom));
@@ -656,7 +659,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                         sta.tx = sta.ty = 1.0f;
                     }
                     U4 xb = rng.draw(vertex, STREAM_BSDF, 0u);
-                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c));
+                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c), sc.tex);
                     if (sd.pdf != 0.0f) {
                         /* volume vertices keep balance 1 and the throughput
                          * (phase / pdf = 1, ray-tracing.cc:2182-2201) */
@@ -668,7 +671,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                                 float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
                                 nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
                             }
-                            T = mulv(T, divs(bsdf_eval(sd), sd.pdf));
+                            T = mulv(T, divs(bsdf_eval(sd, sc.tex), sd.pdf));
                         }
                         pool.ray_o[slot] = st4(pos, 0.0f);
                         pool.ray_d[slot] = st4(sd.wi, 0.0f);

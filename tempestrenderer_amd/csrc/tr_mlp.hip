@@ -92,9 +92,7 @@ __device__ inline bf16x8 feature_chunk(float x, int c)
     return pack8(f);
 }
 
-constexpr int kWaves = 4;          /* waves per block */
 constexpr int kTile = 16;          /* samples per wave tile */
-constexpr int kBlockTile = kWaves * kTile;
 
 /* ---------------- LDS weight layout ---------------- */
 

@@ -104,6 +104,26 @@ struct LightRec {
     uint32_t area;        /* IsAreaLight: sphere lights only (ray-tracing-common.hh:546-551) */
 };
 
+/* A bound texture: TR_TEX_* format and filter, texels at `offset` bytes
+ * into the texel pool (16-byte aligned). */
+struct TexRec {
+    uint32_t width, height, format, sampling;
+    uint64_t offset;
+    uint64_t pad;
+};
+
+/* The scene's textures, the sRGB decode table of 8-bit values and the
+ * global cube map (six RGBA8 faces of cube_side^2 texels at cube_offset,
+ * cube_side 0 = none). */
+struct TexView {
+    const TexRec* recs;
+    const uint8_t* texels;
+    const float* srgb;
+    uint32_t count;
+    uint32_t cube_side;
+    uint64_t cube_offset;
+};
+
 /* Device pointers of a committed scene. */
 struct SceneView {
     const Node8* nodes;
@@ -118,6 +138,7 @@ struct SceneView {
     uint32_t n_lights;
     uint32_t n_prims;
     uint32_t n_volumes;
+    TexView tex;
 };
 
 }  // namespace trk

@@ -18,6 +18,7 @@
 
 #include "tr_math.h"
 #include "tr_scene.h"
+#include "tr_texture.h"
 
 namespace trk {
 
@@ -27,6 +28,7 @@ struct Sample {
     V3 wo, wi, tangent, binormal, normal;
     float dir_density;
     float pdf;
+    float tcx, tcy;  /* TexCoord */
 };
 
 struct Strata {
@@ -217,18 +219,28 @@ TR_HD float phong_pow(float x, float p)
     return fmodf(fl, 2.0f) != 0.0f ? -m : m;
 }
 
-/* Specular/diffuse lobe choice shared by the microfacet and Blinn-Phong
- * models: MaxValue(Specular) / MaxValue(Diffuse + Specular), 0 for black. */
-TR_HD float lobe_ratio(const tr_material* m)
+/* Blinn-Phong's lobe choice: MaxValue(Specular) / MaxValue(albedo +
+ * Specular), 0 for black. */
+TR_HD float lobe_ratio(V3 albedo, V3 specular)
 {
-    float mv = max_value(add(ld3(m->diffuse), ld3(m->specular)));
-    return mv != 0.0f ? max_value(ld3(m->specular)) / mv : 0.0f;
+    float mv = max_value(add(albedo, specular));
+    return mv != 0.0f ? max_value(specular) / mv : 0.0f;
 }
 
-/* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF * cos. */
-TR_HD V3 blinn_phong_brdf(const Sample& s)
+/* A spectrum map (SampleSpectrum at the TexCoord) or the constant. */
+TR_HD V3 map_spectrum(const TexView& tv, uint32_t map, const float* constant, const Sample& s)
+{
+    if (map == 0u) return ld3(constant);
+    V4f c = tex_sample(tv, map, s.tcx, s.tcy, TEX_SPECTRUM);
+    return mk(c.x, c.y, c.z);
+}
+
+/* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF * cos; the albedo
+ * from DiffuseMap when there is one. */
+TR_HD V3 blinn_phong_brdf(const Sample& s, const TexView& tv)
 {
     const tr_material* m = s.mat;
+    V3 albedo = map_spectrum(tv, m->diffuse_map, m->diffuse, s);
     float power = m->specular_power[0];
     float cos_ln = fmax_(0.0f, dot(s.normal, s.wi));
     V3 h = normalize(add(s.wi, s.wo));
@@ -236,17 +248,18 @@ TR_HD V3 blinn_phong_brdf(const Sample& s)
     float cos_theta = fmin_(fmax_(dot(s.wi, h), 0.0f), 1.0f);
     float fresnel = fresnel_schlick(m->fresnel[0], cos_theta);
     V3 spec = muls(muls(muls(muls(ld3(m->specular), lobe), fresnel), power + 8.0f), 1.0f / 8.0f);
-    return muls(muls(add(ld3(m->diffuse), spec), cos_ln), 1.0f / kPi);
+    return muls(muls(add(albedo, spec), cos_ln), 1.0f / kPi);
 }
 
 /* BlinnPhongPDF (illumination-models.hh:618-643) */
-TR_HD float blinn_phong_pdf(const Sample& s)
+TR_HD float blinn_phong_pdf(const Sample& s, const TexView& tv)
 {
     const tr_material* m = s.mat;
     V3 h = normalize(add(s.wi, s.wo));
+    V3 albedo = map_spectrum(tv, m->diffuse_map, m->diffuse, s);
     float cos_ln = dot(s.wi, s.normal);
     if (cos_ln < 0.0f) return 0.0f;
-    float ratio = lobe_ratio(m);
+    float ratio = lobe_ratio(albedo, ld3(m->specular));
     float power = m->specular_power[0];
     float pdf0 = (power + 1.0f) * phong_pow(dot(h, s.normal), power) * (1.0f / (2.0f * kPi));
     float jac = 4.0f * dot(h, s.wo);
@@ -258,12 +271,12 @@ TR_HD float blinn_phong_pdf(const Sample& s)
 /* BlinnPhongSampleIncidentLight (illumination-models.hh:645-677) with
  * PowerCosineSampleHemisphere (sampling3.hh:96-104) and
  * CosineSampleHemisphere (:69-76). */
-TR_HD void blinn_phong_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2)
+TR_HD void blinn_phong_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TexView& tv)
 {
     const tr_material* m = s->mat;
     float x_ray = (st.x + xi0) / st.tx;
     float y_ray = (st.y + xi1) / st.ty;
-    float ratio = lobe_ratio(m);
+    float ratio = lobe_ratio(map_spectrum(tv, m->diffuse_map, m->diffuse, *s), ld3(m->specular));
     M3 basis = make_basis(s->normal);
     float sin_phi, cos_phi;
     tr_sincos_turns(y_ray, &sin_phi, &cos_phi);
@@ -277,7 +290,7 @@ TR_HD void blinn_phong_sample(const Strata& st, Sample* s, float xi0, float xi1,
         V3 d = mk(cos_phi * r, sin_phi * r, sqrtf(fmax_(0.0f, 1.0f - x_ray)));
         s->wi = normalize(transform(basis, d));
     }
-    s->pdf = blinn_phong_pdf(*s);
+    s->pdf = blinn_phong_pdf(*s, tv);
 }
 
 /* UniformHemispherePDF, illumination-models.hh:723-727 */
@@ -318,28 +331,50 @@ TR_HD float sggx_ndf(V3 stddev, const M3& basis, V3 n)
     return 0.5f * sqrtf(det) / (kPi * den * den);
 }
 
-/* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619 */
-TR_HD float sggx_surface_density(const Sample& s)
+/* The SGGX surface's StandardDeviation (x, y; z = 1) and SGGXBasis
+ * quaternion, from StandardDeviationMap (SampleRG) and BasisMap
+ * (SampleRGBA) when present (ggx-models-impl.hh:495-517, 593-608). */
+TR_HD V3 sggx_surface_stddev(const Sample& s, const TexView& tv)
 {
     const tr_material* m = s.mat;
-    M3 orient = m3(s.tangent, s.binormal, s.normal);
-    M3 basis = mul(orient, quat_to_m3(m->sggx_basis[0], m->sggx_basis[1], m->sggx_basis[2], m->sggx_basis[3]));
-    return sggx_projected_area(mk(m->specular_power[0], m->specular_power[1], 1.0f), basis, s.wi);
+    if (m->stddev_map == 0u) return mk(m->specular_power[0], m->specular_power[1], 1.0f);
+    V4f c = tex_sample(tv, m->stddev_map, s.tcx, s.tcy, TEX_RG);
+    return mk(c.x, c.y, 1.0f);
+}
+TR_HD V4f sggx_surface_quat(const Sample& s, const TexView& tv)
+{
+    const tr_material* m = s.mat;
+    if (m->basis_map == 0u) {
+        V4f q = {m->sggx_basis[0], m->sggx_basis[1], m->sggx_basis[2], m->sggx_basis[3]};
+        return q;
+    }
+    return tex_sample(tv, m->basis_map, s.tcx, s.tcy, TEX_RGBA);
 }
 
-/* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551 (no texture maps) */
-TR_HD V3 sggx_surface_brdf(const Sample& s)
+/* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619
+ * (the quaternion is used as it is, not normalised) */
+TR_HD float sggx_surface_density(const Sample& s, const TexView& tv)
+{
+    M3 orient = m3(s.tangent, s.binormal, s.normal);
+    V4f q = sggx_surface_quat(s, tv);
+    M3 basis = mul(orient, quat_to_m3(q.x, q.y, q.z, q.w));
+    return sggx_projected_area(sggx_surface_stddev(s, tv), basis, s.wi);
+}
+
+/* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551 */
+TR_HD V3 sggx_surface_brdf(const Sample& s, const TexView& tv)
 {
     const tr_material* m = s.mat;
     float din = dot(s.normal, s.wi);
     float don = dot(s.normal, s.wo);
     if (din < 0.0f || don < 0.0f) return mk(0.0f, 0.0f, 0.0f);
-    V3 result = muls(muls(ld3(m->diffuse), din), 1.0f / kPi);
-    V3 stddev = mk(m->specular_power[0], m->specular_power[1], 1.0f);
+    V3 result = muls(muls(map_spectrum(tv, m->diffuse_map, m->diffuse, s), din), 1.0f / kPi);
+    V3 stddev = sggx_surface_stddev(s, tv);
     if (stddev.x >= 1e-6f && stddev.y >= 1e-6f && stddev.z >= 1e-6f && s.dir_density >= 1e-6f) {
         V3 mn = normalize(add(s.wi, s.wo));
         M3 orient = m3(s.tangent, s.binormal, s.normal);
-        float qx = m->sggx_basis[0], qy = m->sggx_basis[1], qz = m->sggx_basis[2], qw = m->sggx_basis[3];
+        V4f q = sggx_surface_quat(s, tv);
+        float qx = q.x, qy = q.y, qz = q.z, qw = q.w;
         float ql = sqrtf(qx * qx + qy * qy + qz * qz + qw * qw);
         if (ql == 0.0f) { qx = qy = qz = qw = 0.0f; }
         else { qx = qx / ql; qy = qy / ql; qz = qz / ql; qw = qw / ql; }
@@ -356,7 +391,8 @@ TR_HD V3 sggx_surface_brdf(const Sample& s)
         }
         float in_density = sggx_projected_area(stddev, basis, s.wi);
         float contrib = 1.0f / (s.dir_density + in_density * don / din);
-        result = add(result, muls(muls(muls(muls(smul(contrib, ld3(m->specular)), sggx_ndf(stddev, basis, mn)), fresnel), 2.0f), 0.25f));
+        V3 spec = map_spectrum(tv, m->specular_map, m->specular, s);
+        result = add(result, muls(muls(muls(muls(smul(contrib, spec), sggx_ndf(stddev, basis, mn)), fresnel), 2.0f), 0.25f));
     }
     return result;
 }
@@ -433,39 +469,39 @@ TR_HD bool is_volume_model(uint32_t m) { return m == TR_MODEL_SGGX_MICRO_FLAKE; 
  * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
  * models: Blinn-Phong, GGX and Beckmann (plain, conductor, dielectric),
  * SGGX surface, SGGX micro-flake volume. */
-TR_HD V3 bsdf_eval(const Sample& s)
+TR_HD V3 bsdf_eval(const Sample& s, const TexView& tv)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
-    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s);
-    if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
+    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s, tv);
+    if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s, tv);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
         float p = sggx_flake_phase(s);
         return mk(p, p, p);
     }
     return mk(0.0f, 0.0f, 0.0f);
 }
-TR_HD float bsdf_pdf(const Sample& s)
+TR_HD float bsdf_pdf(const Sample& s, const TexView& tv)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
-    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s);
+    if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s, tv);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) return sggx_flake_pdf(s);
     return 0.0f;
 }
-TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2)
+TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TexView& tv)
 {
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi0, xi1, xi2);
-    else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi0, xi1, xi2);
+    else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi0, xi1, xi2, tv);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi0, xi1);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi0, xi1);
     else s->pdf = 0.0f;
 }
-TR_HD void material_cache(Sample* s)
+TR_HD void material_cache(Sample* s, const TexView& tv)
 {
-    if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s);
+    if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s, tv);
 }
 
 /* ---------------- geometry sampling ---------------- */
@@ -500,6 +536,16 @@ TR_HD void sample_geometry(const SceneView& sc, uint32_t geom, uint32_t prim, fl
         V3 norm = normalize(add(add(muls(n0, w), muls(n1, u)), muls(n2, v)));
         sd->tangent = mk(0.0f, 0.0f, 0.0f);
         sd->binormal = mk(0.0f, 0.0f, 0.0f);
+        sd->tcx = 0.0f;
+        sd->tcy = 0.0f;
+        if (g.stride >= 32) {
+            /* TexCoord = tc0 * w + tc1 * u + tc2 * v (PTNFormat and wider) */
+            const float* t0 = reinterpret_cast<const float*>(p0 + 12);
+            const float* t1 = reinterpret_cast<const float*>(p1 + 12);
+            const float* t2 = reinterpret_cast<const float*>(p2 + 12);
+            sd->tcx = (t0[0] * w + t1[0] * u) + t2[0] * v;
+            sd->tcy = (t0[1] * w + t1[1] * u) + t2[1] * v;
+        }
         if (g.stride >= 44) {
             V3 t0 = ld3(reinterpret_cast<const float*>(p0 + 20));
             V3 t1 = ld3(reinterpret_cast<const float*>(p1 + 20));
@@ -513,12 +559,16 @@ TR_HD void sample_geometry(const SceneView& sc, uint32_t geom, uint32_t prim, fl
     } else if (g.type == GEOM_SPHERE) {
         V3 n = sphere_normal(org, dir, g.sphere, t);
         sd->normal = n;
+        sd->tcx = 0.0f;
+        sd->tcy = 0.0f;
         sd->binormal = normalize(cross(mk(0.0f, 0.0f, 1.0f), n));
         sd->tangent = cross(sd->binormal, sd->normal);
     } else {
         sd->normal = normalize(sub(add(org, muls(dir, t)), ld3(g.sphere)));
         sd->tangent = mk(0.0f, 0.0f, 0.0f);
         sd->binormal = mk(0.0f, 0.0f, 0.0f);
+        sd->tcx = 0.0f;
+        sd->tcy = 0.0f;
     }
 }
 
@@ -701,7 +751,7 @@ TR_HD float total_balance(const SceneView& sc, float light_samples, float surfac
         float n = L.area ? light_samples : 1.0f;
         total = total + light_pdf(sc, L, pos, s.wi) * n;
     }
-    total = total + bsdf_pdf(s) * surface_samples;
+    total = total + bsdf_pdf(s, sc.tex) * surface_samples;
     return total;
 }
 

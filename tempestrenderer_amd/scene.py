@@ -164,6 +164,26 @@ def volume_root_record(root):
     return rec, keep
 
 
+def texture_record(texels, fmt, sampling=0):
+    """A tr_texture_desc and a contiguous byte view of the texels."""
+    a = np.ascontiguousarray(texels)
+    if a.ndim < 2:
+        raise ValueError("texels must be [height, width(, channels)]")
+    desc = _abi.tr_texture_desc(int(a.shape[1]), int(a.shape[0]), int(fmt), int(sampling))
+    sizes = {0: 1, 1: 2, 2: 4, 3: 4, 4: 8, 5: 12, 6: 16}
+    if int(fmt) in sizes and a.nbytes != a.shape[0] * a.shape[1] * sizes[int(fmt)]:
+        raise ValueError("texel array size does not match the format")
+    return desc, a.view(np.uint8).reshape(-1)
+
+
+def cube_faces(faces):
+    arrs = [np.ascontiguousarray(f, dtype=np.uint32) for f in faces]
+    if len(arrs) != 6 or any(a.shape != arrs[0].shape or a.shape[0] != a.shape[1] for a in arrs):
+        raise ValueError("six square faces of one size")
+    ptrs = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
 def _root_from_raw(raw):
     n = raw.dims[0] * raw.dims[1] * raw.dims[2]
     vols = []
@@ -288,6 +308,25 @@ class RayTracerScene:
                                            ctypes.byref(gid)))
         self._keep.append(material)
         return gid.value
+
+    def bindTexture(self, texels, fmt, sampling=0):
+        """bindTexture (ray-tracing.hh:340-354): texels [height, width(, c)]
+        in the layout of `fmt` (TR_TEX_*); returns the handle for the
+        material map fields."""
+        desc, buf = texture_record(texels, fmt, sampling)
+        h = ctypes.c_uint32()
+        check(self.lib.tr_scene_bind_texture(self.h, ctypes.byref(desc), buf.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.byref(h)))
+        return h.value
+
+    def setGlobalCubeMap(self, faces):
+        """setGlobalCubeMap: six square RGBA8 faces (+x, -x, +y, -y, +z, -z)
+        as uint32 [side, side] arrays, or None."""
+        if faces is None:
+            check(self.lib.tr_scene_set_global_cube_map(self.h, 0, None))
+            return
+        arrs, ptrs = cube_faces(faces)
+        check(self.lib.tr_scene_set_global_cube_map(self.h, arrs[0].shape[0], ptrs))
 
     def addHierarchicalVolume(self, root, material):
         """addHierarchicalVolume (ray-tracing.cc:1448-1504) of a VolumeRoot;

@@ -84,6 +84,8 @@ class SceneDesc:
         self.sphere_lights = []
         self.volumes = []      # (dims, min, max, density f16 [nz, ny, nx], direction f16 [nz, ny, nx, 3], material)
         self.brick_volumes = []  # (VolumeRoot, material)
+        self.textures = []     # (texels, TR_TEX_* format, sampling): handles 1, 2, ... in this order
+        self.cube_faces = None  # six uint32 RGBA8 faces
         self.settings = dict(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=1.0,
                              background=(0.0, 0.0, 0.0), seed=1, estimator=0)
 
@@ -104,6 +106,10 @@ class SceneDesc:
             return cache[key]
 
         st = self.settings
+        for texels, fmt, sampling in self.textures:
+            target.bindTexture(texels, fmt, sampling)
+        if self.cube_faces is not None:
+            target.setGlobalCubeMap(self.cube_faces)
         target.setSamplesCamera(st["samples_camera"])
         target.setSamplesGlobalIllumination(st["samples_gi"])
         target.setSamplesLocalAreaLight(st["samples_light"])
@@ -361,6 +367,72 @@ def brick_volume(width=48, height=48, samples_camera=4, max_depth=8, seed=3, wit
         d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
     d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
                       background=(1.0, 1.0, 1.0), seed=1)
+    return d
+
+
+def _textured_quad(center, u, v, tc_scale=1.0, tangent=False):
+    """A quad with texture coordinates 0..tc_scale: PTNFormat (stride 32)
+    or PTTNFormat (stride 44, tangent along u)."""
+    c, u, v = (np.asarray(x, dtype=np.float32) for x in (center, u, v))
+    n = np.cross(u, v)
+    n = n / np.linalg.norm(n)
+    t = u / np.linalg.norm(u)
+    pts = [c - u - v, c + u - v, c + u + v, c - u + v]
+    tcs = [(0.0, 0.0), (tc_scale, 0.0), (tc_scale, tc_scale), (0.0, tc_scale)]
+    rows = []
+    for p, q in zip(pts, tcs):
+        rows.append(np.concatenate([p, np.float32(q), t, n] if tangent else [p, np.float32(q), n]))
+    return np.array(rows, dtype=np.float32), np.array([0, 1, 2, 0, 2, 3], dtype=np.int32)
+
+
+def textured(width=48, height=48, samples_camera=4, max_depth=4, seed=5):
+    """Texture maps and a cube-map environment: a Blinn-Phong floor with an
+    sRGB checker DiffuseMap (bilinear, coordinates wrapping three times), an
+    SGGX-surface wall with Diffuse (RGB32F), Specular (RGBA8),
+    StandardDeviation (RG8, nearest) and Basis (RGBA32F quaternion) maps, a
+    Blinn-Phong sphere with a DiffuseMap, a directional light and a cube map
+    of six distinct gradient faces."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vpi = perspective_view_proj_inv((0.0, 0.6, 3.2), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
+    d = SceneDesc("textured", width, height, vpi)
+    checker = np.zeros((16, 16), dtype=np.uint32)
+    for y in range(16):
+        for x in range(16):
+            a, b = (200, 40) if (x // 4 + y // 4) % 2 == 0 else (30, 180)
+            checker[y, x] = a | (b << 8) | (((a + b) // 2) << 16) | (255 << 24)
+    d.textures.append((checker, 2, 0))                                      # 1: RGBA8 bilinear
+    grad = np.stack(np.meshgrid(np.linspace(0.1, 0.9, 8), np.linspace(0.2, 0.7, 8)), -1)
+    d.textures.append((np.concatenate([grad, 0.5 * np.ones((8, 8, 1))], -1).astype(np.float32), 5, 0))  # 2: RGB32F
+    spec = rng.integers(0, 256, size=(8, 8), dtype=np.uint32) * 0x010101 | 0xFF000000
+    d.textures.append((spec.astype(np.uint32), 2, 0))                       # 3: RGBA8
+    sd = rng.integers(20, 255, size=(8, 8, 2), dtype=np.uint8)
+    d.textures.append((sd, 1, 1))                                           # 4: RG8 nearest
+    q = rng.normal(size=(8, 8, 4)).astype(np.float32)
+    q[..., 3] = np.abs(q[..., 3]) + 1.0
+    d.textures.append((q, 6, 0))                                            # 5: RGBA32F
+    floor = dict(model=1, diffuse=(0.5, 0.5, 0.5), specular=(0.2, 0.2, 0.2), specular_power=(20.0, 20.0),
+                 diffuse_map=1)
+    wall = dict(model=SGGX_SURFACE, diffuse=(0.3, 0.3, 0.3), specular=(0.6, 0.6, 0.6), specular_power=(0.4, 0.4),
+                diffuse_map=2, specular_map=3, stddev_map=4, basis_map=5)
+    fv, ft = _textured_quad((0.0, -0.5, 0.0), (0.0, 0.0, 1.2), (1.2, 0.0, 0.0), tc_scale=3.0)
+    d.meshes.append((fv, ft, [(floor, 0, 6)], False))
+    wv, wt = _textured_quad((0.0, 0.3, -1.0), (1.2, 0.0, 0.0), (0.0, 0.8, 0.0), tangent=True)
+    d.meshes.append((wv, wt, [(wall, 0, 6)], False))
+    d.spheres.append(((0.4, -0.2, 0.2), 0.3, dict(model=1, diffuse=(0.2, 0.2, 0.2), specular=(0.5, 0.5, 0.5),
+                                                  specular_power=(40.0, 40.0), diffuse_map=1)))
+    side = 24
+    faces = []
+    for f in range(6):
+        yy, xx = np.mgrid[0:side, 0:side]
+        r = (40 * f + 4 * xx) % 256
+        g = (255 - 8 * yy) % 256
+        b = (30 * f + 2 * (xx + yy)) % 256
+        faces.append((r | (g << 8) | (b << 16) | (255 << 24)).astype(np.uint32))
+    d.cube_faces = faces
+    ld = np.array([0.4, 0.8, 0.45]) / np.linalg.norm([0.4, 0.8, 0.45])
+    d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(0.0, 0.0, 0.0), seed=1)
     return d
 
 

@@ -148,6 +148,40 @@ def test_paths_brick_volume():
     _check_paths(g_rgb, o_rgb, g_hash, o_hash)
 
 
+def test_paths_textured():
+    """DiffuseMap on Blinn-Phong (floor, sphere), the four SGGX-surface maps
+    (wall), wrapping bilinear and nearest filters, and the cube-map
+    environment."""
+    print("case: paths textured 48x48 spp4 depth4", flush=True)
+    desc = workloads.textured(48, 48, samples_camera=4, max_depth=4)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
+def test_texture_arguments_are_checked():
+    print("case: texture arguments", flush=True)
+    from tempestrenderer_amd._abi import TempestError
+    from tempestrenderer_amd.scene import Material, Submesh
+    desc = workloads.box(8, 8)
+    g = RayTracerScene(8, 8, desc.vpi)
+    with pytest.raises(TempestError):
+        g.bindTexture(np.zeros((4, 4), np.uint32), 9)
+    with pytest.raises(TempestError):
+        g.setGlobalCubeMap([np.zeros((16, 16), np.uint32)] * 6)
+    rg = g.bindTexture(np.zeros((4, 4, 2), np.uint8), 1)
+    qv, qt = workloads._textured_quad((0, 0, 0), (0, 0, 1), (1, 0, 0))
+    g.addTriangleMesh(workloads.IDENTITY, [Submesh(Material(1, diffuse_map=rg), 6, 0, 0, 32)], qt, qv)
+    with pytest.raises(TempestError):
+        g.commitScene()  # an RG8 texture cannot be a DiffuseMap
+    g2 = RayTracerScene(8, 8, desc.vpi)
+    g2.addTriangleMesh(workloads.IDENTITY, [Submesh(Material(1, diffuse_map=3), 6, 0, 0, 32)], qt, qv)
+    with pytest.raises(TempestError):
+        g2.commitScene()  # no texture 3
+
+
 def test_timing_mode_renders_the_same_frame():
     """Launches that carry timing events compute what plain launches do."""
     print("case: timing mode box", flush=True)

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
                     }
                     const VolumeRec& V = sc.volumes[sc.geoms[h.geom].tri_count];
                     float tc;
-                    if (delta_track(sc.voxels, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
+                    if (delta_track_any(sc.voxels, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
                                     COUNT ? &ns : nullptr)) {
                         h.t = tc;
                         h.u = h.v = 0.0f;
@@ -448,7 +448,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_refill(FrameParams p, Pat
 
 /* ---------------- shading ---------------- */
 
-template <bool VOL, bool CACHE>
+/* The texture view of k_shade: the scene's textures, or none. */
+template <bool TEX> struct TexSel;
+template <> struct TexSel<true> {
+    typedef TexView type;
+    __device__ static const TexView& get(const SceneView& sc) { return sc.tex; }
+};
+template <> struct TexSel<false> {
+    typedef NoTex type;
+    __device__ static NoTex get(const SceneView&) { return NoTex(); }
+};
+
+template <bool VOL, bool CACHE, bool TEX>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
                                                uint32_t vertex, int cur, int nxt, CacheRecords rec)
 {
@@ -463,6 +474,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
     uint32_t slot = active ? pool.queue[cur][qi] : 0u;
     if (active) {
         const SceneView& sc = p.sc;
+        const typename TexSel<TEX>::type tv = TexSel<TEX>::get(sc);
         uint32_t pixel, cs, split;
         uint64_t pseq;
         uint32_t req = 0;
@@ -508,7 +520,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
             hash = (hash ^ 0xffffffffu) * kFnvPrime;
             /* m_GlobalCubeMap->sampleSpectrum(ray direction) or the
              * background spectrum (ray-tracing.cc:2060, 2224) */
-            V3 env = sc.tex.cube_side ? cube_sample(sc.tex, dir) : ld3(p.background);
+            V3 env = (TEX && sc.tex.cube_side) ? cube_sample(sc.tex, dir) : ld3(p.background);
             if (vertex == 0) { local = env; write_local = true; }
             else global = add(global, mulv(env, T));
         } else {
@@ -524,7 +536,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                  * at the accepted collision (ray-tracing.cc:327-337) */
                 const VolumeRec& V = sc.volumes[G.tri_count];
                 pos = ray_point(org, dir, h4.x);
-                M3 b = volume_frame(fetch_cell(sc.voxels, V, pos), sc.voxels, V);
+                M3 b = volume_frame_any(fetch_cell(sc.voxels, V, pos), sc.voxels, V);
                 sd.mat = &sc.mats[G.material];
                 sd.tangent = b.c0;
                 sd.binormal = b.c1;
@@ -536,10 +548,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                  * ends (ray-tracing.cc:2136-2142, 2277-2284) */
                 T = mulv(T, ld3(sd.mat->albedo));
             } else {
-                sample_geometry(sc, geom, prim, h4.x, h4.y, h4.z, org, dir, &sd);
+                sample_geometry<TEX>(sc, geom, prim, h4.x, h4.y, h4.z, org, dir, &sd);
                 pos = add(org, smul(h4.x, dir));
                 sd.wi = dir;
-                material_cache(&sd, sc.tex);
+                material_cache(&sd, tv);
             }
             if (volume && T.x < 1e-3f && T.y < 1e-3f && T.z < 1e-3f) {
                 /* absorbed: no contribution from this vertex on */
@@ -621,14 +633,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                                 float balance;
                                 if (unbiased) {
                                     float a = (float)samples * smp.pdf;
-                                    balance = a / (a + (float)ss * bsdf_pdf(lsd, sc.tex));
+                                    balance = a / (a + (float)ss * bsdf_pdf(lsd, tv));
                                 } else {
-                                    float total = total_balance(sc, (float)ls, (float)ss, lsd, pos);
+                                    float total = total_balance(sc, (float)ls, (float)ss, lsd, pos, tv);
                                     balance = (light_pdf(sc, L, pos, lsd.wi) * (float)samples) / total;
                                 }
                                 curT = muls(curT, balance);
                             }
-                            V3 contrib = divs(mulv(mulv(smp.radiance, curT), bsdf_eval(lsd, sc.tex)), smp.pdf);
+                            V3 contrib = divs(mulv(mulv(smp.radiance, curT), bsdf_eval(lsd, tv)), smp.pdf);
                             size_t e = (size_t)ne * pool.capacity + slot;
                             pool.sh_o[e] = st4(pos, smp.dist);
                             pool.sh_d[e] = st4(smp.dir, __uint_as_float(smp.geom));
@@ -659,7 +671,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                         sta.tx = sta.ty = 1.0f;
                     }
                     U4 xb = rng.draw(vertex, STREAM_BSDF, 0u);
-                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c), sc.tex);
+                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c), tv);
                     if (sd.pdf != 0.0f) {
                         /* volume vertices keep balance 1 and the throughput
                          * (phase / pdf = 1, ray-tracing.cc:2182-2201) */
@@ -668,10 +680,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                             if (unbiased) {
                                 nb = (float)ss * sd.pdf;
                             } else {
-                                float total = total_balance(sc, (float)ls, (float)ss, sd, pos);
+                                float total = total_balance(sc, (float)ls, (float)ss, sd, pos, tv);
                                 nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
                             }
-                            T = mulv(T, divs(bsdf_eval(sd, sc.tex), sd.pdf));
+                            T = mulv(T, divs(bsdf_eval(sd, tv), sd.pdf));
                         }
                         pool.ray_o[slot] = st4(pos, 0.0f);
                         pool.ray_d[slot] = st4(sd.wi, 0.0f);
@@ -967,7 +979,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                             const GeomRec& G = sc.geoms[h.geom];
                             if (G.type != GEOM_VOLUME || ++crossings > kMaxVolumeCrossings) break;
                             const VolumeRec& V = sc.volumes[G.tri_count];
-                            uint32_t succ = woodcock(sc.voxels, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
+                            uint32_t succ = woodcock_any(sc.voxels, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
                                                      &step, &step_through, COUNT ? &ns : nullptr);
                             tf = tf * ((float)succ / 2.0f);
                             if (tmax_T * tf < 1e-3f) break;
@@ -1228,12 +1240,24 @@ void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, con
                   uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev)
 {
     const uint32_t grid = (n_slots + 255) / 256;
+    /* material maps are bound only to scenes that have textures */
+    const bool tex = p.sc.tex.count > 0 || p.sc.tex.cube_side > 0;
     if (p.sc.n_volumes > 0) {
-        if (p.cache) launch_k(k_shade<true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-        else launch_k(k_shade<true, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        if (tex) {
+            if (p.cache) launch_k(k_shade<true, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<true, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        } else {
+            if (p.cache) launch_k(k_shade<true, true, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<true, false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        }
     } else {
-        if (p.cache) launch_k(k_shade<false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-        else launch_k(k_shade<false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        if (tex) {
+            if (p.cache) launch_k(k_shade<false, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<false, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        } else {
+            if (p.cache) launch_k(k_shade<false, true, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<false, false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+        }
     }
 }
 

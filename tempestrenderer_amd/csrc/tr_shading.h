@@ -227,6 +227,12 @@ TR_HD float lobe_ratio(V3 albedo, V3 specular)
     return mv != 0.0f ? max_value(specular) / mv : 0.0f;
 }
 
+/* Scenes without texture maps and cube map shade through this view: every
+ * map lookup is the material's constant, at no cost. */
+struct NoTex {};
+
+TR_HD V3 map_spectrum(const NoTex&, uint32_t, const float* constant, const Sample&) { return ld3(constant); }
+
 /* A spectrum map (SampleSpectrum at the TexCoord) or the constant. */
 TR_HD V3 map_spectrum(const TexView& tv, uint32_t map, const float* constant, const Sample& s)
 {
@@ -237,7 +243,7 @@ TR_HD V3 map_spectrum(const TexView& tv, uint32_t map, const float* constant, co
 
 /* BlinnPhongBRDF (illumination-models.hh:679-698): BRDF * cos; the albedo
  * from DiffuseMap when there is one. */
-TR_HD V3 blinn_phong_brdf(const Sample& s, const TexView& tv)
+template <class TV> TR_HD V3 blinn_phong_brdf(const Sample& s, const TV& tv)
 {
     const tr_material* m = s.mat;
     V3 albedo = map_spectrum(tv, m->diffuse_map, m->diffuse, s);
@@ -252,7 +258,7 @@ TR_HD V3 blinn_phong_brdf(const Sample& s, const TexView& tv)
 }
 
 /* BlinnPhongPDF (illumination-models.hh:618-643) */
-TR_HD float blinn_phong_pdf(const Sample& s, const TexView& tv)
+template <class TV> TR_HD float blinn_phong_pdf(const Sample& s, const TV& tv)
 {
     const tr_material* m = s.mat;
     V3 h = normalize(add(s.wi, s.wo));
@@ -271,7 +277,8 @@ TR_HD float blinn_phong_pdf(const Sample& s, const TexView& tv)
 /* BlinnPhongSampleIncidentLight (illumination-models.hh:645-677) with
  * PowerCosineSampleHemisphere (sampling3.hh:96-104) and
  * CosineSampleHemisphere (:69-76). */
-TR_HD void blinn_phong_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TexView& tv)
+template <class TV>
+TR_HD void blinn_phong_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TV& tv)
 {
     const tr_material* m = s->mat;
     float x_ray = (st.x + xi0) / st.tx;
@@ -341,6 +348,15 @@ TR_HD V3 sggx_surface_stddev(const Sample& s, const TexView& tv)
     V4f c = tex_sample(tv, m->stddev_map, s.tcx, s.tcy, TEX_RG);
     return mk(c.x, c.y, 1.0f);
 }
+TR_HD V3 sggx_surface_stddev(const Sample& s, const NoTex&)
+{
+    return mk(s.mat->specular_power[0], s.mat->specular_power[1], 1.0f);
+}
+TR_HD V4f sggx_surface_quat(const Sample& s, const NoTex&)
+{
+    V4f q = {s.mat->sggx_basis[0], s.mat->sggx_basis[1], s.mat->sggx_basis[2], s.mat->sggx_basis[3]};
+    return q;
+}
 TR_HD V4f sggx_surface_quat(const Sample& s, const TexView& tv)
 {
     const tr_material* m = s.mat;
@@ -353,7 +369,7 @@ TR_HD V4f sggx_surface_quat(const Sample& s, const TexView& tv)
 
 /* SGGXMicroFlakeSurfaceDensity via SGGXSurfaceCache, ggx-models-impl.hh:589-619
  * (the quaternion is used as it is, not normalised) */
-TR_HD float sggx_surface_density(const Sample& s, const TexView& tv)
+template <class TV> TR_HD float sggx_surface_density(const Sample& s, const TV& tv)
 {
     M3 orient = m3(s.tangent, s.binormal, s.normal);
     V4f q = sggx_surface_quat(s, tv);
@@ -362,7 +378,7 @@ TR_HD float sggx_surface_density(const Sample& s, const TexView& tv)
 }
 
 /* SGGXMicroFlakeSurfaceBRDF, ggx-models-impl.hh:476-551 */
-TR_HD V3 sggx_surface_brdf(const Sample& s, const TexView& tv)
+template <class TV> TR_HD V3 sggx_surface_brdf(const Sample& s, const TV& tv)
 {
     const tr_material* m = s.mat;
     float din = dot(s.normal, s.wi);
@@ -469,7 +485,7 @@ TR_HD bool is_volume_model(uint32_t m) { return m == TR_MODEL_SGGX_MICRO_FLAKE; 
  * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
  * models: Blinn-Phong, GGX and Beckmann (plain, conductor, dielectric),
  * SGGX surface, SGGX micro-flake volume. */
-TR_HD V3 bsdf_eval(const Sample& s, const TexView& tv)
+template <class TV> TR_HD V3 bsdf_eval(const Sample& s, const TV& tv)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
@@ -481,7 +497,7 @@ TR_HD V3 bsdf_eval(const Sample& s, const TexView& tv)
     }
     return mk(0.0f, 0.0f, 0.0f);
 }
-TR_HD float bsdf_pdf(const Sample& s, const TexView& tv)
+template <class TV> TR_HD float bsdf_pdf(const Sample& s, const TV& tv)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
@@ -490,7 +506,7 @@ TR_HD float bsdf_pdf(const Sample& s, const TexView& tv)
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) return sggx_flake_pdf(s);
     return 0.0f;
 }
-TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TexView& tv)
+template <class TV> TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TV& tv)
 {
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi0, xi1, xi2);
@@ -499,7 +515,7 @@ TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float 
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi0, xi1);
     else s->pdf = 0.0f;
 }
-TR_HD void material_cache(Sample* s, const TexView& tv)
+template <class TV> TR_HD void material_cache(Sample* s, const TV& tv)
 {
     if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s, tv);
 }
@@ -517,6 +533,7 @@ TR_HD V3 sphere_normal(V3 org, V3 dir, const float* c, float t)
  * SphereGeometry::sample (:1166-1180), SphereAreaLightGeometry::sample
  * (:1775-1783).  Tangent frames of vertex formats without tangents are
  * zero (the reference leaves the previous values in place). */
+template <bool TC = true>
 TR_HD void sample_geometry(const SceneView& sc, uint32_t geom, uint32_t prim, float t, float u, float v, V3 org, V3 dir,
                            Sample* sd)
 {
@@ -538,7 +555,7 @@ TR_HD void sample_geometry(const SceneView& sc, uint32_t geom, uint32_t prim, fl
         sd->binormal = mk(0.0f, 0.0f, 0.0f);
         sd->tcx = 0.0f;
         sd->tcy = 0.0f;
-        if (g.stride >= 32) {
+        if (TC && g.stride >= 32) {
             /* TexCoord = tc0 * w + tc1 * u + tc2 * v (PTNFormat and wider) */
             const float* t0 = reinterpret_cast<const float*>(p0 + 12);
             const float* t1 = reinterpret_cast<const float*>(p1 + 12);
@@ -743,7 +760,9 @@ TR_HD float hit_light_pdf(const SceneView& sc, uint32_t geom, uint32_t prim, V3 
 }
 
 /* EvaluateTotalBalanceHeuristic (ray-tracing.cc:1791-1806) */
-TR_HD float total_balance(const SceneView& sc, float light_samples, float surface_samples, const Sample& s, V3 pos)
+template <class TV>
+TR_HD float total_balance(const SceneView& sc, float light_samples, float surface_samples, const Sample& s, V3 pos,
+                          const TV& tv)
 {
     float total = 0.0f;
     for (uint32_t i = 0; i < sc.n_lights; ++i) {
@@ -751,7 +770,7 @@ TR_HD float total_balance(const SceneView& sc, float light_samples, float surfac
         float n = L.area ? light_samples : 1.0f;
         total = total + light_pdf(sc, L, pos, s.wi) * n;
     }
-    total = total + bsdf_pdf(s, sc.tex) * surface_samples;
+    total = total + bsdf_pdf(s, tv) * surface_samples;
     return total;
 }
 

@@ -93,13 +93,14 @@ TR_HD float cell_density(const VoxelCell& c)
  * (SampleVolumePoint, ray-tracing.cc:110-182).  The corner weights are the
  * reference's: the lower corner of an axis weighs f, the upper 1 - f.  A
  * zero result (zero tensor) falls back to +z. */
+template <bool QUANT>
 TR_HD V3 cell_orientation(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V)
 {
     float lf[3] = {c.f[0], c.f[1], c.f[2]};
     float uf[3] = {1.0f - c.f[0], 1.0f - c.f[1], 1.0f - c.f[2]};
     float t00 = 0.0f, t01 = 0.0f, t02 = 0.0f, t11 = 0.0f, t12 = 0.0f, t22 = 0.0f;
     const float* tab = reinterpret_cast<const float*>(voxels + V.table_base);
-    const bool quant = V.format == kVoxelQuantized;
+    const bool quant = QUANT;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         float w = ((k & 4) ? uf[2] : lf[2]) * ((k & 2) ? uf[1] : lf[1]) * ((k & 1) ? uf[0] : lf[0]);
@@ -155,10 +156,16 @@ TR_HD M3 make_basis_tangent(V3 tan)
 /* Position along a ray, the same expression everywhere. */
 TR_HD V3 ray_point(V3 org, V3 dir, float t) { return add(org, smul(t, dir)); }
 
-/* The fibre frame at a point and sigma(dir) of the SGGX micro-flakes. */
-TR_HD M3 volume_frame(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V)
+/* The fibre frame at a point and sigma(dir) of the SGGX micro-flakes.  The
+ * voxel format is a template argument so that each walk loop carries only
+ * its own decoding (the registers of the two do not add up). */
+template <bool QUANT> TR_HD M3 volume_frame(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V)
 {
-    return make_basis_tangent(cell_orientation(c, voxels, V));
+    return make_basis_tangent(cell_orientation<QUANT>(c, voxels, V));
+}
+TR_HD M3 volume_frame_any(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V)
+{
+    return V.format == kVoxelQuantized ? volume_frame<true>(c, voxels, V) : volume_frame<false>(c, voxels, V);
 }
 
 /* One tentative collision: 0 = rejected, 1 = accepted (scatter / absorb),
@@ -167,15 +174,16 @@ TR_HD M3 volume_frame(const VoxelCell& c, const uint64_t* voxels, const VolumeRe
  * 316-337 and the loops' TotalDensity *= RcpMaxExtinction): u8 density / 255
  * against the threshold, then (DensityScale / 255) * density * sigma *
  * RcpMaxExtinction. */
+template <bool QUANT>
 TR_HD int volume_accept(const uint64_t* voxels, const VolumeRec& V, V3 pos, V3 dir, float threshold)
 {
     VoxelCell c = fetch_cell(voxels, V, pos);
     float rho = cell_density(c);
     if (rho == 0.0f) return 2;
-    const bool quant = V.format == kVoxelQuantized;
+    const bool quant = QUANT;
     float dens = quant ? rho / 255.0f : rho * V.rcp_rho_max;
     if (dens < threshold) return 0;
-    M3 basis = volume_frame(c, voxels, V);
+    M3 basis = volume_frame<QUANT>(c, voxels, V);
     float sigma = length(mulv(transform_rotation_inverse(basis, dir), ld3(V.stddev)));
     if (quant) return (kDensityScale / 255.0f) * rho * sigma * V.rcp_majorant >= threshold ? 1 : 0;
     return dens * sigma * V.rcp_smax >= threshold ? 1 : 0;
@@ -200,6 +208,7 @@ enum { STREAM_VOLUME = 4, STREAM_SHADOW_VOLUME = 5 };
  * the collision distance in *t when a collision is accepted, false and the
  * distance beyond the exit where the walk left the volume otherwise.
  * `step` counts tentative collisions of this path vertex across volumes. */
+template <bool QUANT>
 TR_HD bool delta_track(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
                        const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
 {
@@ -223,12 +232,18 @@ TR_HD bool delta_track(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 di
             *t = tc;
             return true;
         }
-        int acc = volume_accept(voxels, V, ray_point(org, dir, tc), dir, u01(xb));
+        int acc = volume_accept<QUANT>(voxels, V, ray_point(org, dir, tc), dir, u01(xb));
         if (acc == 1) {
             *t = tc;
             return true;
         }
     }
+}
+TR_HD bool delta_track_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+                           const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
+{
+    if (V.format == kVoxelQuantized) return delta_track<true>(voxels, V, org, dir, t0, t1, rng, vertex, step, t, nsteps);
+    return delta_track<false>(voxels, V, org, dir, t0, t1, rng, vertex, step, t, nsteps);
 }
 
 /* The two Woodcock repetitions of a shadow ray through one volume crossing
@@ -238,6 +253,7 @@ TR_HD bool delta_track(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 di
  * max(*step_through, t0): the reference starts at *step_through, measured
  * from the shadow-ray origin, which skips a volume lying further than one
  * free path from the shading point.  Returns the number of successes. */
+template <bool QUANT>
 TR_HD uint32_t woodcock(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
                         const Rng& rng, uint32_t vertex, uint32_t sub_hi, uint32_t* step, float* step_through,
                         uint32_t* nsteps)
@@ -261,7 +277,7 @@ TR_HD uint32_t woodcock(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 d
                 ++success;
                 break;
             }
-            int acc = volume_accept(voxels, V, ray_point(org, dir, sd), dir, u01(xb));
+            int acc = volume_accept<QUANT>(voxels, V, ray_point(org, dir, sd), dir, u01(xb));
             if (acc == 2) {
                 *step_through = sd;
                 ++success;
@@ -271,6 +287,14 @@ TR_HD uint32_t woodcock(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 d
         }
     }
     return success;
+}
+TR_HD uint32_t woodcock_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+                            const Rng& rng, uint32_t vertex, uint32_t sub_hi, uint32_t* step, float* step_through,
+                            uint32_t* nsteps)
+{
+    if (V.format == kVoxelQuantized)
+        return woodcock<true>(voxels, V, org, dir, t0, t1, rng, vertex, sub_hi, step, step_through, nsteps);
+    return woodcock<false>(voxels, V, org, dir, t0, t1, rng, vertex, sub_hi, step, step_through, nsteps);
 }
 
 }  // namespace trk

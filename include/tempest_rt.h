@@ -386,12 +386,19 @@ tr_status tr_scene_read_backbuffer(tr_scene* scene, uint32_t* rgba8, size_t pixe
 tr_status tr_scene_read_accum(tr_scene* scene, double* rgb_sum, size_t pixel_count,
                               uint64_t* samples_per_pixel);
 
+/* A decoded image: rows in the order the reference's loaders leave them,
+ * format TR_TEX_RGBA8_UNORM or TR_TEX_R8_UNORM. */
+typedef struct tr_image {
+    uint32_t width, height, format;
+    uint8_t* data;
+} tr_image;
+
 /* RTMeshBlob (mesh/obj-loader.hh:47-59): submeshes whose material
  * pointers point into `materials`, interleaved vertices (byte offsets in
- * the submeshes), group-local int32 index triplets.  `stride` is kept for
- * the reference's field, which its loader never sets (0).  `ignored_maps`
- * counts materials whose map_Kd the reference would bind (texture maps are
- * not loaded here). */
+ * the submeshes), group-local int32 index triplets, the decoded map_Kd
+ * images (TextureData).  `stride` is kept for the reference's field, which
+ * its loader never sets (0).  `ignored_maps` counts map_Kd files that did
+ * not load. */
 typedef struct tr_mesh_blob {
     uint32_t submesh_count;
     tr_submesh* submeshes;
@@ -403,6 +410,8 @@ typedef struct tr_mesh_blob {
     tr_material* materials;
     uint32_t material_count;
     uint32_t ignored_maps;
+    tr_image* textures;        /* the map_Kd images; materials' diffuse_map = index + 1 */
+    uint32_t texture_count;
 } tr_mesh_blob;
 /* TEMPEST_OBJ_LOADER_* flags (mesh/obj-loader.hh:36-40) */
 enum { TR_OBJ_GENERATE_TANGENTS = 1, TR_OBJ_GENERATE_CONSISTENT_NORMALS = 2 };
@@ -413,6 +422,11 @@ enum { TR_OBJ_GENERATE_TANGENTS = 1, TR_OBJ_GENERATE_CONSISTENT_NORMALS = 2 };
 tr_status tr_load_obj_rt_geometry(const char* filename, uint32_t flags, tr_mesh_blob* blob);
 /* RTMeshBlob::~RTMeshBlob (src/mesh/obj-loader.cc:51-59) */
 void tr_mesh_blob_free(tr_mesh_blob* blob);
+/* Binds the blob's textures to a scene (bilinear, as the reference's
+ * loaded textures sample) and rewrites its materials' diffuse_map from
+ * blob indices to that scene's handles; call once, before adding the
+ * blob's meshes. */
+tr_status tr_scene_bind_mesh_blob_textures(tr_scene* scene, tr_mesh_blob* blob);
 
 /* Image files of an RGBA8 picture whose row 0 is the bottom row (the
  * backbuffer's layout). */
@@ -430,6 +444,15 @@ tr_status tr_image_save(const char* path, uint32_t width, uint32_t height, const
 /* RayTracingSystem::saveImage (ray-tracing-system.hh:125-130): the last
  * frame's backbuffer through tr_image_save. */
 tr_status tr_scene_save_image(tr_scene* scene, const char* path);
+
+/* LoadImage (src/image/image.cc:36-58): "tga" (LoadTGAImage,
+ * tga-image.cc:98-242: true colour, 24 or 32 bits, plain or run-length;
+ * rows bottom-up as stored unless the descriptor says top-down) or "png"
+ * (LoadPNGImage, png-image.cc:125-246: 8-bit grey, grey-alpha, RGB, RGBA
+ * and palettes of 1-8 bits, tRNS alpha, non-interlaced; rows top-down as
+ * stored) by the lower-cased extension.  Release with tr_image_free. */
+tr_status tr_image_load(const char* path, tr_image* image);
+void tr_image_free(tr_image* image);
 
 /* Philox key of the counter-based RNG (new; the reference has no seed). */
 tr_status tr_scene_set_seed(tr_scene* scene, uint64_t seed);

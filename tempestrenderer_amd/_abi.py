@@ -86,12 +86,18 @@ class tr_stats(ctypes.Structure):
     ]
 
 
+class tr_image(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
+                ("data", ctypes.POINTER(ctypes.c_uint8))]
+
+
 class tr_mesh_blob(ctypes.Structure):
     _fields_ = [("submesh_count", ctypes.c_uint32), ("submeshes", ctypes.POINTER(tr_submesh)),
                 ("vertex_data", ctypes.POINTER(ctypes.c_uint8)), ("vertex_bytes", ctypes.c_size_t),
                 ("index_data", ctypes.POINTER(ctypes.c_int32)), ("index_count", ctypes.c_size_t),
                 ("stride", ctypes.c_uint32), ("materials", ctypes.POINTER(tr_material)),
-                ("material_count", ctypes.c_uint32), ("ignored_maps", ctypes.c_uint32)]
+                ("material_count", ctypes.c_uint32), ("ignored_maps", ctypes.c_uint32),
+                ("textures", ctypes.POINTER(tr_image)), ("texture_count", ctypes.c_uint32)]
 
 
 TR_OBJ_GENERATE_TANGENTS = 1
@@ -215,6 +221,9 @@ SIGNATURES = {
     "tr_parse_volume": (_st, [ctypes.c_char_p, ctypes.c_char_p, P(tr_volume_root)]),
     "tr_volume_root_free": (None, [P(tr_volume_root)]),
     "tr_scene_add_hierarchical_volume": (_st, [_vp, P(tr_volume_root), P(tr_material), P(_u64)]),
+    "tr_image_load": (_st, [ctypes.c_char_p, P(tr_image)]),
+    "tr_image_free": (None, [P(tr_image)]),
+    "tr_scene_bind_mesh_blob_textures": (_st, [_vp, P(tr_mesh_blob)]),
     "tr_load_obj_rt_geometry": (_st, [ctypes.c_char_p, _u32, P(tr_mesh_blob)]),
     "tr_mesh_blob_free": (None, [P(tr_mesh_blob)]),
     "tr_image_encode": (_st, [_u32, _u32, _u32, P(_u32), P(ctypes.c_uint8), _sz, P(_sz)]),

@@ -1208,6 +1208,23 @@ tr_status tr_scene_bind_texture(tr_scene* s, const tr_texture_desc* d, const voi
     return TR_OK;
 }
 
+tr_status tr_scene_bind_mesh_blob_textures(tr_scene* s, tr_mesh_blob* blob)
+{
+    if (!valid(s) || !blob) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    std::vector<uint32_t> handle(blob->texture_count + 1, 0);
+    for (uint32_t i = 0; i < blob->texture_count; ++i) {
+        const tr_image& im = blob->textures[i];
+        tr_texture_desc d = {im.width, im.height, im.format, TR_TEX_BILINEAR};
+        tr_status st = tr_scene_bind_texture(s, &d, im.data, &handle[i + 1]);
+        if (st != TR_OK) return st;
+    }
+    for (uint32_t i = 0; i < blob->material_count; ++i) {
+        uint32_t& m = blob->materials[i].diffuse_map;
+        m = m <= blob->texture_count ? handle[m] : 0;
+    }
+    return TR_OK;
+}
+
 tr_status tr_scene_set_global_cube_map(tr_scene* s, uint32_t side, const uint32_t* const faces[6])
 {
     if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");

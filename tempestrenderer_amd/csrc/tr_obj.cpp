@@ -24,10 +24,13 @@
  * error instead: indices outside the arrays, groups whose faces mix vertex
  * formats.  Material and file names after usemtl / mtllib / map_* are
  * taken without surrounding blanks and carriage returns (the reference
- * keeps them, so such files fail to find their materials there).  Texture
- * maps are not loaded; `ignored_maps` counts the materials whose map_Kd
- * the reference would have bound.  Like the reference, any diagnostic
- * fails the whole load; the first one is the error message.
+ * keeps them, so such files fail to find their materials there).  A
+ * material's map_Kd image is read with tr_image_load from the OBJ file's
+ * directory (obj-loader.cc:755-767) into the blob's textures; one that
+ * does not load leaves the material without a map and is counted in
+ * `ignored_maps`, as the reference logs it and goes on.  Like the
+ * reference, any diagnostic of the OBJ and MTL grammars fails the whole
+ * load; the first one is the error message.
  */
 #include <cmath>
 #include <cstdio>
@@ -872,6 +875,26 @@ tr_status tr_load_obj_rt_geometry(const char* filename, uint32_t flags, tr_mesh_
     if (data.size() > 0xFFFFFFFFull) return fail(TR_ERROR_UNSUPPORTED, "OBJ vertex data beyond 4 GiB");
 
     const uint32_t nm = (uint32_t)d.materials.size();
+    std::vector<tr_image> images;
+    std::vector<uint32_t> image_of(nm, 0);
+    const std::string dir = directory_of(filename);
+    for (uint32_t i = 0; i < nm; ++i) {
+        if (d.materials[i].map_kd.empty()) continue;
+        tr_image img;
+        if (tr_image_load((dir + "/" + d.materials[i].map_kd).c_str(), &img) == TR_OK) {
+            images.push_back(img);
+            image_of[i] = (uint32_t)images.size();
+        } else {
+            ++out->ignored_maps;
+        }
+    }
+    out->textures = images.empty() ? nullptr : (tr_image*)calloc(images.size(), sizeof(tr_image));
+    if (!images.empty() && !out->textures) {
+        for (auto& im : images) tr_image_free(&im);
+        return fail(TR_ERROR_OUT_OF_MEMORY, "out of memory for the mesh blob");
+    }
+    for (size_t i = 0; i < images.size(); ++i) out->textures[i] = images[i];
+    out->texture_count = (uint32_t)images.size();
     out->materials = nm ? (tr_material*)calloc(nm, sizeof(tr_material)) : nullptr;
     out->submeshes = subs.empty() ? nullptr : (tr_submesh*)calloc(subs.size(), sizeof(tr_submesh));
     out->vertex_data = data.empty() ? nullptr : (uint8_t*)malloc(data.size());
@@ -895,7 +918,7 @@ tr_status tr_load_obj_rt_geometry(const char* filename, uint32_t flags, tr_mesh_
             m->specular[0] = m->specular[1] = m->specular[2] = 0.0f;
             m->specular_power[0] = m->specular_power[1] = 1.0f;
         }
-        if (!in.map_kd.empty()) ++out->ignored_maps;
+        m->diffuse_map = image_of[i];
         if (all_nonzero(in.emission)) {
             m->model = TR_MODEL_EMISSIVE;
             const float ke[3] = {in.emission.x, in.emission.y, in.emission.z};
@@ -925,6 +948,8 @@ void tr_mesh_blob_free(tr_mesh_blob* blob)
     free(blob->vertex_data);
     free(blob->index_data);
     free(blob->materials);
+    for (uint32_t i = 0; i < blob->texture_count; ++i) tr_image_free(&blob->textures[i]);
+    free(blob->textures);
     memset(blob, 0, sizeof(*blob));
 }
 

@@ -568,6 +568,15 @@ class RTMeshBlob:
         self.IndexData = np.zeros(0, dtype=np.int32)
         self.Stride = 0
         self.IgnoredMaps = 0
+        self.Textures = []   # TextureData: (texels, TR_TEX_* format); materials' diffuse_map = index + 1
+
+    def bindTextures(self, scene):
+        """Binds the textures to `scene` (RayTracerScene or the oracle's) and
+        rewrites the materials' diffuse_map to its handles; once per blob."""
+        handles = [0] + [scene.bindTexture(t, fmt, 0) for t, fmt in self.Textures]
+        for m in self.Materials:
+            if 0 < m.rec.diffuse_map <= len(self.Textures):
+                m.rec.diffuse_map = handles[m.rec.diffuse_map]
 
     @property
     def SubmeshCount(self):
@@ -576,6 +585,27 @@ class RTMeshBlob:
     @property
     def MaterialCount(self):
         return len(self.Materials)
+
+
+def _image_from_raw(im):
+    if im.format == _abi.TR_TEX_R8_UNORM:
+        a = np.ctypeslib.as_array(im.data, shape=(im.height, im.width)).copy()
+    else:
+        a = np.ctypeslib.as_array(im.data, shape=(im.height * im.width * 4,)).copy().view(np.uint32)
+        a = a.reshape(im.height, im.width)
+    return a, int(im.format)
+
+
+def load_image(path):
+    """LoadImage (src/image/image.cc:36-58): (texels, TR_TEX_* format);
+    RGBA8 as uint32 [h, w] (R in the low byte), R8 as uint8 [h, w]."""
+    lib = _abi.load()
+    im = _abi.tr_image()
+    check(lib.tr_image_load(os.fsencode(path), ctypes.byref(im)))
+    try:
+        return _image_from_raw(im)
+    finally:
+        lib.tr_image_free(ctypes.byref(im))
 
 
 def LoadObjFileStaticRTGeometry(filename, flags=0):
@@ -608,6 +638,8 @@ def LoadObjFileStaticRTGeometry(filename, flags=0):
             out.IndexData = np.ctypeslib.as_array(raw.index_data, shape=(raw.index_count,)).copy()
         out.Stride = raw.stride
         out.IgnoredMaps = raw.ignored_maps
+        for i in range(raw.texture_count):
+            out.Textures.append(_image_from_raw(raw.textures[i]))
         return out
     finally:
         lib.tr_mesh_blob_free(ctypes.byref(raw))

@@ -50,3 +50,32 @@ def test_cornell_paths(name):
     bad = np.abs(g_rgb - o_rgb) > tol
     assert not bad.any(), "radiance differs on %d of %d values" % (bad.sum(), bad.size)
     assert g_rgb.mean() > 0.0  # the ceiling light is seen
+
+
+def test_textured_cube_in_cube_map():
+    """The reference's textured cube (map_Kd PNG through LoadImage) in the
+    Storforsen4 cube-map environment (six PNG faces)."""
+    print("case: textured cube in cube map", flush=True)
+    from tempestrenderer_amd.scene import load_image
+    root = os.path.join(os.path.dirname(__file__), "golden")
+    faces = [load_image(os.path.join(root, "storforsen4", "%s-256.png" % f))[0]
+             for f in ("posx", "negx", "posy", "negy", "posz", "negz")]
+    w, h = 48, 48
+    vpi = workloads.perspective_view_proj_inv((1.2, 1.0, 1.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, 1.0)
+    scenes = []
+    for make in (lambda: RayTracerScene(w, h, vpi), lambda: oracle.OracleScene(w, h, vpi)):
+        blob = LoadObjFileStaticRTGeometry(os.path.join(root, "cube", "cube.obj"))
+        s = make()
+        blob.bindTextures(s)
+        s.setGlobalCubeMap(faces)
+        s.addLightSource(s.make_light(kind=0, direction=(0.4, 0.8, 0.45), radiance=(1.5, 1.5, 1.5)))
+        _setup(s, blob, 4, 3)
+        scenes.append(s)
+    g, o = scenes
+    px = np.arange(w * h, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    assert np.array_equal(g_hash, o_hash), "hit sequences differ on %d paths" % np.sum(g_hash != o_hash)
+    tol = 1e-4 * np.maximum(np.maximum(np.abs(g_rgb), np.abs(o_rgb)), 1e-6)
+    bad = np.abs(g_rgb - o_rgb) > tol
+    assert not bad.any(), "radiance differs on %d of %d values" % (bad.sum(), bad.size)

@@ -86,3 +86,105 @@ def test_bad_arguments():
         encode_image(np.zeros((1, 70000), dtype=np.uint32), _abi.TR_IMAGE_TGA)
     with pytest.raises(TempestError):
         encode_image(np.zeros((2, 2), dtype=np.uint32), 7)
+
+
+# ---------------- readers ----------------
+
+from tempestrenderer_amd.scene import load_image  # noqa: E402
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def _rgba_bytes(a):
+    return a.view(np.uint8).reshape(a.shape[0], a.shape[1], 4)
+
+
+@pytest.mark.parametrize("name", ["cube/default.png", "storforsen4/posx-256.png", "storforsen4/negy-256.png"])
+def test_png_fixtures_match_pillow(name):
+    path = GOLDEN + "/" + name
+    a, fmt = load_image(path)
+    assert fmt == _abi.TR_TEX_RGBA8_UNORM
+    assert np.array_equal(_rgba_bytes(a), np.asarray(Image.open(path).convert("RGBA")))
+
+
+def _pil_png(tmp_path, img, name, **kw):
+    p = tmp_path / name
+    img.save(p, **kw)
+    return str(p)
+
+
+def test_png_colour_types_match_pillow(tmp_path):
+    """RGB (alpha 255), grey-alpha (grey in R, G, B), 8-bit palettes with and
+    without tRNS, a 4-bit palette and 1-bit grey, each against Pillow; every
+    row filter occurs in Pillow's adaptive filtering of these images."""
+    rng = np.random.default_rng(3)
+    rgb = rng.integers(0, 256, size=(37, 29, 3), dtype=np.uint8)
+    rgb[:, :10] = rgb[:1, :10]  # runs, so that filters differ per row
+    p = _pil_png(tmp_path, Image.fromarray(rgb, "RGB"), "rgb.png")
+    a, fmt = load_image(p)
+    want = np.concatenate([rgb, np.full((37, 29, 1), 255, np.uint8)], -1)
+    assert fmt == _abi.TR_TEX_RGBA8_UNORM and np.array_equal(_rgba_bytes(a), want)
+    la = rng.integers(0, 256, size=(20, 33, 2), dtype=np.uint8)
+    p = _pil_png(tmp_path, Image.fromarray(la, "LA"), "la.png")
+    a, _ = load_image(p)
+    assert np.array_equal(_rgba_bytes(a), np.asarray(Image.open(p).convert("RGBA")))
+    pal = Image.fromarray(rgb, "RGB").quantize(colors=200)
+    for name, kw in (("p8.png", {}), ("p8t.png", {"transparency": 5})):
+        p = _pil_png(tmp_path, pal, name, **kw)
+        a, _ = load_image(p)
+        assert np.array_equal(_rgba_bytes(a), np.asarray(Image.open(p).convert("RGBA")))
+    pal4 = Image.fromarray(rgb, "RGB").quantize(colors=12)
+    p = _pil_png(tmp_path, pal4, "p4.png", bits=4)
+    a, _ = load_image(p)
+    assert np.array_equal(_rgba_bytes(a), np.asarray(Image.open(p).convert("RGBA")))
+
+
+def test_png_grey_is_r8(tmp_path):
+    rng = np.random.default_rng(4)
+    g = rng.integers(0, 256, size=(17, 23), dtype=np.uint8)
+    a, fmt = load_image(_pil_png(tmp_path, Image.fromarray(g, "L"), "g.png"))
+    assert fmt == _abi.TR_TEX_R8_UNORM and np.array_equal(a, g)
+    bits = rng.integers(0, 2, size=(9, 13)).astype(bool)
+    a, fmt = load_image(_pil_png(tmp_path, Image.fromarray(bits), "b.png"))
+    assert fmt == _abi.TR_TEX_R8_UNORM and np.array_equal(a, bits.astype(np.uint8) * 255)
+
+
+def test_png_rejected_kinds(tmp_path):
+    img = Image.fromarray(np.zeros((8, 8, 3), np.uint8), "RGB")
+    with pytest.raises(TempestError):
+        load_image(str(tmp_path / "none.png"))
+    p16 = _pil_png(tmp_path, Image.fromarray(np.zeros((8, 8), np.uint16)), "w.png")
+    with pytest.raises(TempestError):
+        load_image(p16)
+    bad = tmp_path / "bad.png"
+    data = bytearray(open(_pil_png(tmp_path, img, "ok.png"), "rb").read())
+    data[40] ^= 0xFF  # inside IDAT: CRC mismatch
+    bad.write_bytes(bytes(data))
+    with pytest.raises(TempestError):
+        load_image(str(bad))
+
+
+def test_tga_round_trip_and_pillow(tmp_path):
+    px = _picture(h=21, w=34, seed=8)
+    save_image(tmp_path / "w.tga", px)
+    a, fmt = load_image(str(tmp_path / "w.tga"))
+    assert fmt == _abi.TR_TEX_RGBA8_UNORM and np.array_equal(a, px)  # bottom-up as written
+    rgb = np.random.default_rng(2).integers(0, 256, size=(12, 19, 3), dtype=np.uint8)
+    rgb[3:9] = rgb[3]  # runs for the run-length coder
+    for name, kw in (("p.tga", {}), ("r.tga", {"compression": "tga_rle"})):
+        Image.fromarray(rgb, "RGB").save(tmp_path / name, **kw)
+        a, _ = load_image(str(tmp_path / name))
+        got = _rgba_bytes(a)
+        # texture rows run bottom-up whatever the file's origin (a top-down
+        # file is flipped); Pillow's array is top-down
+        assert np.array_equal(got[..., :3], rgb[::-1]) and (got[..., 3] == 255).all()
+
+
+def test_obj_loads_map_kd():
+    from tempestrenderer_amd.scene import LoadObjFileStaticRTGeometry
+    b = LoadObjFileStaticRTGeometry(GOLDEN + "/cube/cube.obj")
+    assert b.IgnoredMaps == 0 and len(b.Textures) == 1
+    tex, fmt = b.Textures[0]
+    assert fmt == _abi.TR_TEX_RGBA8_UNORM and tex.shape == (128, 128)
+    assert all(m.rec.diffuse_map == 1 for m in b.Materials)
+    assert np.array_equal(_rgba_bytes(tex), np.asarray(Image.open(GOLDEN + "/cube/default.png").convert("RGBA")))

@@ -748,13 +748,15 @@ template <class T> bool surface_or_volume(const orc_scene* s, const tr_hit& h, V
 }
 
 /* Cache mode: roughness input (microfacet or Blinn-Phong alpha =
- * sqrt(2 / (power + 2)), 1 without a specular part or for SGGX surfaces)
+ * sqrt(2 / (power + 2)), the geometric mean of the two alphas of an
+ * anisotropic microfacet model, 1 without a specular part or for SGGX surfaces)
  * and the network inputs of a shading point. */
 inline float cache_roughness(const tr_material* m)
 {
-    if (!is_microfacet(m->model) && m->model != TR_MODEL_BLINN_PHONG) return 1.0f;
+    if (!is_microfacet(m->model) && !is_aniso(m->model) && m->model != TR_MODEL_BLINN_PHONG) return 1.0f;
     if (m->specular[0] == 0.0f && m->specular[1] == 0.0f && m->specular[2] == 0.0f) return 1.0f;
     float a = sqrtf(2.0f / (m->specular_power[0] + 2.0f));
+    if (is_aniso(m->model)) a = sqrtf(a * sqrtf(2.0f / (m->specular_power[1] + 2.0f)));
     return a < 1.0f ? a : 1.0f;
 }
 
@@ -875,7 +877,13 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
         for (uint32_t d = 0; d < s->max_depth; ++d) {
             uint32_t xr[4];
             srng.draw(d, STREAM_BSDF, 0, xr);
-            T xi[3] = {(T)orc_u01(xr[0]), (T)orc_u01(xr[1]), (T)orc_u01(xr[2])};
+            T xi[5] = {(T)orc_u01(xr[0]), (T)orc_u01(xr[1]), (T)orc_u01(xr[2]), (T)0, (T)0};
+            if (is_aniso(cur.mat->model)) {
+                /* the anisotropic specular lobe's second pair: sub-counter 1 */
+                srng.draw(d, STREAM_BSDF, 1, xr);
+                xi[3] = (T)orc_u01(xr[0]);
+                xi[4] = (T)orc_u01(xr[1]);
+            }
             bsdf_sample(st, &cur, xi);
             ORC_TRACE("d=%u model=%u n=(%g %g %g) wo=(%g %g %g) xi=(%g %g %g) wi=(%g %g %g) pdf=%g\n", d,
                       cur.mat->model, (double)cur.normal.x, (double)cur.normal.y, (double)cur.normal.z,
@@ -1558,7 +1566,8 @@ void orc_bsdf_eval(const tr_material* m, const float* n, const float* t, const f
     out[3] = bsdf_pdf(s);
 }
 
-/* Sampled direction (out[0..2]) and PDF (out[3]). */
+/* Sampled direction (out[0..2]) and PDF (out[3]); xi holds five uniforms
+ * (the last two read by the anisotropic microfacet models only). */
 void orc_bsdf_sample(const tr_material* m, const float* n, const float* t, const float* b, const float* wo,
                      const float* strata, const float* xi, float dir_density, float* out)
 {

@@ -6,7 +6,8 @@
  * Deviations, applied identically in the GPU kernels:
  *  - random numbers come from Philox (three per surface sample: the two
  *    stratified coordinates and the lobe selector, in the reference's draw
- *    order) instead of FastFloatRand;
+ *    order; the anisotropic microfacet models take their specular pair from
+ *    a second draw) instead of FastFloatRand;
  *  - sin/cos of 2*pi*u use tr_sincos_turns instead of FastSinCos
  *    (functions.hh:322-353), logf uses tr_logf, expf uses tr_expf,
  *    pow(x, 5) is x^2*x^2*x.
@@ -245,6 +246,150 @@ template <class T> inline void microfacet_sample(const Strata<T>& st, Sample<T>*
         s->wi = normalize(transform(basis, d));
     }
     s->pdf = microfacet_pdf(*s);
+}
+
+/* ---- anisotropic GGX and Beckmann (models 15-17 and 31-33) ---- */
+
+inline bool is_aniso(uint32_t m)
+{
+    return (m >= TR_MODEL_GGX_MICROFACET_ANISOTROPIC && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
+           (m >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC_ANISOTROPIC);
+}
+inline bool aniso_is_beckmann(uint32_t m) { return m >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC; }
+
+/* A Vector2 and its NormalizeSelf (math/vector2.hh:60-74). */
+template <class T> struct Dir2 {
+    T x, y;
+};
+template <class T> inline Dir2<T> unit2(T x, T y)
+{
+    T l = xsqrt(x * x + y * y);
+    if (l != (T)0) {
+        x = x / l;
+        y = y / l;
+    }
+    return Dir2<T>{x, y};
+}
+
+/* (alpha_x, alpha_y): the square roots of SpecularPowerToGeometryFactorSq of
+ * SpecularPower.x and .y */
+template <class T> inline Dir2<T> aniso_alpha(const tr_material* m)
+{
+    return Dir2<T>{xsqrt(geom_factor_sq((T)m->specular_power[0])), xsqrt(geom_factor_sq((T)m->specular_power[1]))};
+}
+
+/* GGXMicrofacetDistributionAnisotropic, ggx-models-impl.hh:111-127, and
+ * BeckmannMicrofacetDistributionAnisotropic, beckmann-models-impl.hh:69-85;
+ * `th` is the half vector's normalized tangent-plane direction.
+ * Deviation: 0 for cos_nh <= 0 (the reference has no guard; its callers
+ * multiply such a value by a zero shadowing term or drop the non-positive
+ * PDF, so only the NaN at cos_nh = 0 changes). */
+template <class T> inline T aniso_distribution(bool beckmann, Dir2<T> alpha, T cos_nh, Dir2<T> th)
+{
+    if (cos_nh <= (T)0) return (T)0;
+    T c2 = cos_nh * cos_nh;
+    T c4 = c2 * c2;
+    T tan2 = ((T)1 - c2) / c2;
+    T r1 = th.x / alpha.x;
+    T r2 = th.y / alpha.y;
+    T aniso = r1 * r1 + r2 * r2;
+    if (beckmann) return xexp(-tan2 * aniso) / (C<T>::pi * alpha.x * alpha.y * c4);
+    T den = (T)1 + tan2 * aniso;
+    return (T)1 / (C<T>::pi * c4 * alpha.x * alpha.y * den * den);
+}
+
+/* {GGX,Beckmann}MicrofacetAnisotropic{,Conductor,Dielectric}BRDF
+ * (ggx-models-impl.hh:283-341, beckmann-models-impl.hh:174-238) with
+ * ComputeMicrofacetAnglesAnisotropic and
+ * ComputeRoughnessProjectedOnDirectionSq (microfacet-common.hh:86-107):
+ * BRDF times cos. */
+template <class T> inline V3<T> microfacet_aniso_brdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    const bool beck = aniso_is_beckmann(m->model);
+    Angles<T> a = microfacet_angles(s);
+    V3<T> h = normalize(s.wi + s.wo);
+    Dir2<T> th = unit2(dot(s.tangent, h), dot(s.binormal, h));
+    T fresnel;
+    uint32_t variant =
+        m->model - (beck ? TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC : TR_MODEL_GGX_MICROFACET_ANISOTROPIC);
+    if (variant == 0) {
+        fresnel = fresnel_schlick((T)m->fresnel[0], a.cos_im);
+    } else if (variant == 1) {
+        fresnel = fresnel_conductor((T)m->fresnel[0], (T)m->fresnel[1], a.cos_im);
+    } else {
+        T cos_trans = cos_transmittance((T)m->fresnel[0], a.cos_im);
+        fresnel = fresnel_dielectric((T)m->fresnel[0], a.cos_im, cos_trans);
+    }
+    Dir2<T> to = unit2(dot(s.wo, s.tangent), dot(s.wo, s.binormal));
+    Dir2<T> alpha = aniso_alpha<T>(m);
+    T px = alpha.x * to.x, py = alpha.y * to.y;
+    T g2_out = xabs(px * px + py * py);
+    T shadow = beck ? beckmann_g1(g2_out, a.cos_im, a.cos_in) * beckmann_g1(g2_out, a.cos_im, a.cos_on)
+                    : ggx_g1(g2_out, a.cos_im, a.cos_in) * ggx_g1(g2_out, a.cos_im, a.cos_on);
+    T d = aniso_distribution(beck, alpha, a.cos_mn, th);
+    /* the grazing-incidence limit 0, as for the isotropic models */
+    T den = (T)4 * xabs(a.cos_in) * xabs(a.cos_on);
+    T spec = den > (T)0 ? shadow * d / den : (T)0;
+    T brdf = fresnel * spec;
+    return (spec3<T>(m->diffuse) * ((T)1 / C<T>::pi) + spec3<T>(m->specular) * brdf) * a.cos_in;
+}
+
+/* {GGX,Beckmann}MicrofacetAnisotropicPDF (ggx-models-impl.hh:343-375,
+ * beckmann-models-impl.hh:313-345) */
+template <class T> inline T microfacet_aniso_pdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    T cos_ln = dot(s.wi, s.normal);
+    if (cos_ln < (T)0) return (T)0;
+    V3<T> h = normalize(s.wi + s.wo);
+    T cos_hn = dot(h, s.normal);
+    Dir2<T> th = unit2(dot(s.tangent, h), dot(s.binormal, h));
+    T norm_pdf = aniso_distribution(aniso_is_beckmann(m->model), aniso_alpha<T>(m), cos_hn, th) * cos_hn;
+    if (norm_pdf <= (T)0) return (T)0;
+    T jacobian = (T)1 / ((T)4 * dot(h, s.wo));
+    T pdf0 = norm_pdf * jacobian;
+    T mv = max_value(spec3<T>(m->diffuse) + spec3<T>(m->specular));
+    T ratio = mv != (T)0 ? max_value(spec3<T>(m->specular)) / mv : (T)0;
+    T pdf1 = cos_ln * ((T)1 / C<T>::pi);
+    return (pdf0 - pdf1) * ratio + pdf1;
+}
+
+/* {GGX,Beckmann}MicrofacetAnisotropicSampleIncidentLight with
+ * {GGX,Beckmann}SampleMicrofacetAnisotropicDistribution
+ * (ggx-models-impl.hh:377-432, beckmann-models-impl.hh:347-398).
+ * xi[0], xi[1]: the stratified coordinates of the cosine lobe; xi[2]: the
+ * lobe selector; xi[3], xi[4]: the specular lobe's own stratified pair.
+ * The micro-normal lives in makeBasis(Normal), not in the tangent frame. */
+template <class T> inline void microfacet_aniso_sample(const Strata<T>& st, Sample<T>* s, const T xi[5])
+{
+    const tr_material* m = s->mat;
+    T x_ray = (st.x + xi[0]) / st.tx;
+    T y_ray = (st.y + xi[1]) / st.ty;
+    T mv = max_value(spec3<T>(m->diffuse) + spec3<T>(m->specular));
+    T ratio = mv != (T)0 ? max_value(spec3<T>(m->specular)) / mv : (T)0;
+    M3<T> basis = make_basis(s->normal);
+    if (xi[2] <= ratio) {
+        Dir2<T> alpha = aniso_alpha<T>(m);
+        T r0 = (st.x + xi[3]) / st.tx;
+        T r1 = (st.y + xi[4]) / st.ty;
+        T sin_phi, cos_phi;
+        sincos_turns(r1, &sin_phi, &cos_phi);
+        T sqrt_ratio = aniso_is_beckmann(m->model) ? xsqrt(-xlog(r0)) : xsqrt(r0 / ((T)1 - r0));
+        T x = alpha.x * sqrt_ratio * cos_phi;
+        T y = alpha.y * sqrt_ratio * sin_phi;
+        V3<T> mn = normalize(v3(-x, -y, (T)1));
+        mn = normalize(mn);  /* normalized once more by the caller */
+        mn = normalize(transform(basis, mn));
+        s->wi = reflect(s->wo, mn);
+    } else {
+        T sin_phi, cos_phi;
+        sincos_turns(y_ray, &sin_phi, &cos_phi);
+        T r = xsqrt(x_ray);
+        V3<T> d = v3(cos_phi * r, sin_phi * r, xsqrt(xmax((T)0, (T)1 - x_ray)));
+        s->wi = normalize(transform(basis, d));
+    }
+    s->pdf = microfacet_aniso_pdf(*s);
 }
 
 /* powf as the Blinn-Phong lobes use it.  FP32: exp(p log x) in the shared
@@ -497,6 +642,7 @@ template <class T> inline V3<T> bsdf_eval(const Sample<T>& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
+    if (is_aniso(m)) return microfacet_aniso_brdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
@@ -511,6 +657,7 @@ template <class T> inline T bsdf_pdf(const Sample<T>& s)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
+    if (is_aniso(m)) return microfacet_aniso_pdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE)
@@ -518,11 +665,13 @@ template <class T> inline T bsdf_pdf(const Sample<T>& s)
     return (T)0;
 }
 
-/* SampleIncidentLightLookup (illumination-models.hh:325-370) */
-template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, const T xi[3])
+/* SampleIncidentLightLookup (illumination-models.hh:325-370); xi[3] and
+ * xi[4] are read by the anisotropic microfacet models only. */
+template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, const T xi[5])
 {
     uint32_t m = s->mat->model;
     if (is_microfacet(m)) microfacet_sample(st, s, xi);
+    else if (is_aniso(m)) microfacet_aniso_sample(st, s, xi);
     else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi[0], xi[1]);

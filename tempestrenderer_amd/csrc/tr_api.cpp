@@ -198,6 +198,8 @@ bool model_supported(uint32_t m)
 {
     return m == TR_MODEL_EMISSIVE || m == TR_MODEL_BLINN_PHONG || (m >= TR_MODEL_GGX_MICROFACET && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC) ||
            (m >= TR_MODEL_BECKMANN_MICROFACET && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC) ||
+           (m >= TR_MODEL_GGX_MICROFACET_ANISOTROPIC && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
+           (m >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
            m == TR_MODEL_SGGX_SURFACE;
 }
 

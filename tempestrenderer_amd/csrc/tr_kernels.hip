@@ -160,13 +160,15 @@ __device__ inline float4 st4(V3 v, float w) { return make_float4(v.x, v.y, v.z, 
 
 /* Roughness input of the cache: the microfacet alpha (sqrt of
  * SpecularPowerToGeometryFactorSq, microfacet-common.hh:109-112; the same
- * map of the Blinn-Phong exponent), 1 for lobes without a specular part
+ * map of the Blinn-Phong exponent; the geometric mean of the two alphas of
+ * the anisotropic models), 1 for lobes without a specular part
  * and for SGGX surfaces. */
 __device__ inline float cache_roughness(const tr_material* m)
 {
-    if (!is_microfacet(m->model) && m->model != TR_MODEL_BLINN_PHONG) return 1.0f;
+    if (!is_microfacet(m->model) && !is_aniso(m->model) && m->model != TR_MODEL_BLINN_PHONG) return 1.0f;
     if (m->specular[0] == 0.0f && m->specular[1] == 0.0f && m->specular[2] == 0.0f) return 1.0f;
     float a = sqrtf(geom_factor_sq(m->specular_power[0]));
+    if (is_aniso(m->model)) a = sqrtf(a * sqrtf(geom_factor_sq(m->specular_power[1])));
     return a < 1.0f ? a : 1.0f;
 }
 
@@ -671,7 +673,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                         sta.tx = sta.ty = 1.0f;
                     }
                     U4 xb = rng.draw(vertex, STREAM_BSDF, 0u);
-                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c), tv);
+                    bsdf_sample(sta, &sd, u01(xb.a), u01(xb.b), u01(xb.c), tv, rng, vertex);
                     if (sd.pdf != 0.0f) {
                         /* volume vertices keep balance 1 and the throughput
                          * (phase / pdf = 1, ray-tracing.cc:2182-2201) */

@@ -205,6 +205,153 @@ TR_HD void microfacet_sample(const Strata& st, Sample* s, float xi0, float xi1, 
     s->pdf = microfacet_pdf(*s);
 }
 
+/* ---------------- anisotropic GGX / Beckmann ---------------- */
+
+TR_HD bool is_aniso(uint32_t m)
+{
+    return (m >= TR_MODEL_GGX_MICROFACET_ANISOTROPIC && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
+           (m >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC_ANISOTROPIC);
+}
+
+/* NormalizeSelf of a Vector2 (math/vector2.hh:60-74): divides by the length,
+ * leaves (0, 0) alone. */
+TR_HD void normalize2(float* x, float* y)
+{
+    float l = sqrtf(*x * *x + *y * *y);
+    if (l != 0.0f) {
+        *x /= l;
+        *y /= l;
+    }
+}
+
+/* GGXMicrofacetDistributionAnisotropic (ggx-models-impl.hh:111-127) and
+ * BeckmannMicrofacetDistributionAnisotropic (beckmann-models-impl.hh:69-85).
+ * (ct, st): the half vector's direction in the tangent plane.  The reference
+ * has no cos <= 0 guard; every caller then multiplies by a zero shadowing
+ * term or rejects the non-positive PDF, so the guard changes only the NaN of
+ * cos = 0 into 0. */
+TR_HD float aniso_d(bool beck, float ax, float ay, float cos_nh, float ct, float st)
+{
+    if (cos_nh <= 0.0f) return 0.0f;
+    float c2 = cos_nh * cos_nh;
+    float c4 = c2 * c2;
+    float tan2 = (1.0f - c2) / c2;
+    float r1 = ct / ax, r2 = st / ay;
+    float aniso = r1 * r1 + r2 * r2;
+    if (beck) return tr_expf(-tan2 * aniso) / (kPi * ax * ay * c4);
+    float den = 1.0f + tan2 * aniso;
+    return 1.0f / (kPi * c4 * ax * ay * den * den);
+}
+
+TR_HD void aniso_alphas(const tr_material* m, float* ax, float* ay)
+{
+    *ax = sqrtf(geom_factor_sq(m->specular_power[0]));
+    *ay = sqrtf(geom_factor_sq(m->specular_power[1]));
+}
+
+/* GGXMicrofacetAnisotropic{,Conductor,Dielectric}BRDF with
+ * GGXSpecularAnisotropicBRDF (ggx-models-impl.hh:283-341) and the Beckmann
+ * twins (beckmann-models-impl.hh:174-238); ComputeMicrofacetAnglesAnisotropic
+ * and ComputeRoughnessProjectedOnDirectionSq, microfacet-common.hh:86-107.
+ * Both G1 terms use the roughness projected on the outgoing direction. */
+TR_HD V3 microfacet_aniso_brdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    uint32_t model = m->model;
+    bool beck = model >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC;
+    V3 h = normalize(add(s.wi, s.wo));
+    float cos_in = dot(s.wi, s.normal);
+    float cos_on = dot(s.wo, s.normal);
+    float cos_im = dot(s.wi, h);
+    float cos_mn = dot(s.normal, h);
+    float ht = dot(s.tangent, h), hb = dot(s.binormal, h);
+    normalize2(&ht, &hb);
+    uint32_t variant =
+        model - (beck ? (uint32_t)TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC : (uint32_t)TR_MODEL_GGX_MICROFACET_ANISOTROPIC);
+    float fresnel;
+    if (variant == 0) {
+        fresnel = fresnel_schlick(m->fresnel[0], cos_im);
+    } else if (variant == 1) {
+        fresnel = fresnel_conductor(m->fresnel[0], m->fresnel[1], cos_im);
+    } else {
+        float ct = cos_transmittance(m->fresnel[0], cos_im);
+        fresnel = fresnel_dielectric(m->fresnel[0], cos_im, ct);
+    }
+    float ot = dot(s.wo, s.tangent), ob = dot(s.wo, s.binormal);
+    normalize2(&ot, &ob);
+    float ax, ay;
+    aniso_alphas(m, &ax, &ay);
+    float tt = ax * ot, bt = ay * ob;
+    float g2o = fabsf(tt * tt + bt * bt);
+    float shadow = beck ? beckmann_g1(g2o, cos_im, cos_in) * beckmann_g1(g2o, cos_im, cos_on)
+                        : ggx_g1(g2o, cos_im, cos_in) * ggx_g1(g2o, cos_im, cos_on);
+    float d = aniso_d(beck, ax, ay, cos_mn, ht, hb);
+    float den = 4.0f * fabsf(cos_in) * fabsf(cos_on);
+    float spec = den > 0.0f ? shadow * d / den : 0.0f;
+    float brdf = fresnel * spec;
+    return muls(add(muls(ld3(m->diffuse), 1.0f / kPi), muls(ld3(m->specular), brdf)), cos_in);
+}
+
+/* GGXMicrofacetAnisotropicPDF / BeckmannMicrofacetAnisotropicPDF
+ * (ggx-models-impl.hh:343-375, beckmann-models-impl.hh:313-345) */
+TR_HD float microfacet_aniso_pdf(const Sample& s)
+{
+    const tr_material* m = s.mat;
+    float cos_ln = dot(s.wi, s.normal);
+    if (cos_ln < 0.0f) return 0.0f;
+    V3 h = normalize(add(s.wi, s.wo));
+    float cos_hn = dot(h, s.normal);
+    float ht = dot(s.tangent, h), hb = dot(s.binormal, h);
+    normalize2(&ht, &hb);
+    float ax, ay;
+    aniso_alphas(m, &ax, &ay);
+    float norm_pdf = aniso_d(m->model >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC, ax, ay, cos_hn, ht, hb) * cos_hn;
+    if (norm_pdf <= 0.0f) return 0.0f;
+    float jacobian = 1.0f / (4.0f * dot(h, s.wo));
+    float pdf0 = norm_pdf * jacobian;
+    float mv = max_value(add(ld3(m->diffuse), ld3(m->specular)));
+    float ratio = mv != 0.0f ? max_value(ld3(m->specular)) / mv : 0.0f;
+    float pdf1 = cos_ln * (1.0f / kPi);
+    return (pdf0 - pdf1) * ratio + pdf1;
+}
+
+/* GGXMicrofacetAnisotropicSampleIncidentLight with
+ * GGXSampleMicrofacetAnisotropicDistribution (ggx-models-impl.hh:377-432)
+ * and the Beckmann twins (beckmann-models-impl.hh:347-398).  The specular
+ * branch draws a second stratified pair (xi3, xi4); the micro-normal is
+ * built in makeBasis(Normal), not the tangent frame, as in the reference. */
+TR_HD void microfacet_aniso_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, float xi3, float xi4)
+{
+    const tr_material* m = s->mat;
+    float x_ray = (st.x + xi0) / st.tx;
+    float y_ray = (st.y + xi1) / st.ty;
+    float mv = max_value(add(ld3(m->diffuse), ld3(m->specular)));
+    float ratio = mv != 0.0f ? max_value(ld3(m->specular)) / mv : 0.0f;
+    M3 basis = make_basis(s->normal);
+    if (xi2 <= ratio) {
+        float ax, ay;
+        aniso_alphas(m, &ax, &ay);
+        float xr = (st.x + xi3) / st.tx;
+        float yr = (st.y + xi4) / st.ty;
+        float sin_phi, cos_phi;
+        tr_sincos_turns(yr, &sin_phi, &cos_phi);
+        float sqrt_ratio =
+            m->model >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC ? sqrtf(-tr_logf(xr)) : sqrtf(xr / (1.0f - xr));
+        float x = ax * sqrt_ratio * cos_phi;
+        float y = ay * sqrt_ratio * sin_phi;
+        V3 mn = normalize(normalize(mk(-x, -y, 1.0f)));
+        mn = normalize(transform(basis, mn));
+        s->wi = reflect(s->wo, mn);
+    } else {
+        float sin_phi, cos_phi;
+        tr_sincos_turns(y_ray, &sin_phi, &cos_phi);
+        float r = sqrtf(x_ray);
+        V3 d = mk(cos_phi * r, sin_phi * r, sqrtf(fmax_(0.0f, 1.0f - x_ray)));
+        s->wi = normalize(transform(basis, d));
+    }
+    s->pdf = microfacet_aniso_pdf(*s);
+}
+
 /* powf(x, p) for the Blinn-Phong lobes, in the shared exp/log: x > 0 gives
  * exp(p log x); x = 0 gives 0 for p > 0 and 1 for p = 0; x < 0 gives
  * |x|^p with the sign of (-1)^p for an integer p and NaN otherwise. */
@@ -483,12 +630,13 @@ TR_HD bool is_volume_model(uint32_t m) { return m == TR_MODEL_SGGX_MICRO_FLAKE; 
 
 /* TransmittanceLookup / PDFLookup / SampleIncidentLightLookup /
  * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
- * models: Blinn-Phong, GGX and Beckmann (plain, conductor, dielectric),
- * SGGX surface, SGGX micro-flake volume. */
+ * models: Blinn-Phong, GGX and Beckmann (plain, conductor, dielectric;
+ * isotropic and anisotropic), SGGX surface, SGGX micro-flake volume. */
 template <class TV> TR_HD V3 bsdf_eval(const Sample& s, const TV& tv)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_brdf(s);
+    if (is_aniso(m)) return microfacet_aniso_brdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s, tv);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s, tv);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
@@ -501,15 +649,25 @@ template <class TV> TR_HD float bsdf_pdf(const Sample& s, const TV& tv)
 {
     uint32_t m = s.mat->model;
     if (is_microfacet(m)) return microfacet_pdf(s);
+    if (is_aniso(m)) return microfacet_aniso_pdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s, tv);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) return sggx_flake_pdf(s);
     return 0.0f;
 }
-template <class TV> TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TV& tv)
+/* xi0..xi2: the vertex's BSDF draw; the anisotropic microfacet models take
+ * their second stratified pair from the same stream's sub-counter 1. */
+template <class TV>
+TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float xi2, const TV& tv, const Rng& rng,
+                       uint32_t vertex)
 {
     uint32_t m = s->mat->model;
-    if (is_microfacet(m)) microfacet_sample(st, s, xi0, xi1, xi2);
+    if (is_microfacet(m)) {
+        microfacet_sample(st, s, xi0, xi1, xi2);
+    } else if (is_aniso(m)) {
+        U4 x2 = rng.draw(vertex, STREAM_BSDF, 1u);
+        microfacet_aniso_sample(st, s, xi0, xi1, xi2, u01(x2.a), u01(x2.b));
+    }
     else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi0, xi1, xi2, tv);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi0, xi1);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi0, xi1);

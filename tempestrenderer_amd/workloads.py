@@ -21,6 +21,8 @@ Workloads:
                    (0.1, 1, 1) in the fibre frame, albedo 0.9; constant
                    environment light 1.0; 1024x1024, 128 camera samples,
                    depth 32 (optionally a floor and a directional light).
+  anisotropic(...) the six anisotropic GGX / Beckmann models on tangent-frame
+                   quads, spheres and a quad without tangents; 48x48.
 Scene randomness comes from numpy's PCG64 seeded with `seed`; the path
 tracer's own random numbers come from Philox keyed by the render seed.
 """
@@ -433,6 +435,52 @@ def textured(width=48, height=48, samples_camera=4, max_depth=4, seed=5):
     d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
     d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
                       background=(0.0, 0.0, 0.0), seed=1)
+    return d
+
+
+GGX_ANISOTROPIC = 15
+GGX_CONDUCTOR_ANISOTROPIC = 16
+GGX_DIELECTRIC_ANISOTROPIC = 17
+BECKMANN_ANISOTROPIC = 31
+BECKMANN_CONDUCTOR_ANISOTROPIC = 32
+BECKMANN_DIELECTRIC_ANISOTROPIC = 33
+
+
+def anisotropic(width=48, height=48, samples_camera=4, max_depth=4):
+    """The six anisotropic microfacet models: a brushed-conductor GGX floor
+    and a Beckmann dielectric wall (PTTNFormat: tangent along the first
+    edge), four spheres (tangent frame from the sphere normal) with the other
+    variants, a PTNFormat quad (no tangent: the anisotropy collapses) with a
+    Beckmann conductor, a directional light and a point light."""
+    vpi = perspective_view_proj_inv((0.0, 0.7, 3.0), (0.0, -0.1, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
+    d = SceneDesc("anisotropic", width, height, vpi)
+
+    def mat(model, ax, ay, diffuse, specular, fresnel):
+        return dict(model=model, diffuse=diffuse, specular=specular,
+                    specular_power=(alpha_to_specular_power(ax), alpha_to_specular_power(ay)), fresnel=fresnel)
+
+    floor = mat(GGX_CONDUCTOR_ANISOTROPIC, 0.08, 0.45, (0.05, 0.05, 0.05), (0.9, 0.8, 0.6), (0.2, 3.0))
+    wall = mat(BECKMANN_DIELECTRIC_ANISOTROPIC, 0.5, 0.1, (0.4, 0.3, 0.2), (0.5, 0.5, 0.5), (1.5, 0.0))
+    fv, ft = _textured_quad((0.0, -0.5, 0.0), (0.0, 0.0, 1.3), (1.3, 0.0, 0.0), tangent=True)
+    d.meshes.append((fv, ft, [(floor, 0, 6)], False))
+    wv, wt = _textured_quad((0.0, 0.3, -1.0), (1.3, 0.0, 0.0), (0.0, 0.8, 0.0), tangent=True)
+    d.meshes.append((wv, wt, [(wall, 0, 6)], False))
+    side = mat(BECKMANN_CONDUCTOR_ANISOTROPIC, 0.3, 0.05, (0.1, 0.1, 0.1), (0.7, 0.7, 0.8), (0.3, 2.5))
+    sv, st = _textured_quad((-1.2, 0.0, -0.2), (0.0, 0.0, 0.8), (0.0, 0.6, 0.0))
+    d.meshes.append((sv, st, [(side, 0, 6)], False))
+    d.spheres.append(((-0.55, -0.2, 0.1), 0.3, mat(GGX_ANISOTROPIC, 0.1, 0.6, (0.3, 0.1, 0.1), (0.6, 0.6, 0.6),
+                                                     (0.04, 0.0))))
+    d.spheres.append(((0.0, -0.2, 0.3), 0.3, mat(GGX_DIELECTRIC_ANISOTROPIC, 0.4, 0.07, (0.1, 0.3, 0.1),
+                                                   (0.5, 0.5, 0.5), (1.6, 0.0))))
+    d.spheres.append(((0.55, -0.2, 0.1), 0.3, mat(BECKMANN_ANISOTROPIC, 0.06, 0.35, (0.1, 0.1, 0.3),
+                                                    (0.6, 0.6, 0.6), (0.05, 0.0))))
+    d.spheres.append(((0.3, 0.45, -0.3), 0.15, mat(GGX_CONDUCTOR_ANISOTROPIC, 0.2, 0.2, (0.0, 0.0, 0.0),
+                                                     (0.9, 0.9, 0.9), (0.2, 3.5))))
+    ld = np.array([0.3, 0.9, 0.4]) / np.linalg.norm([0.3, 0.9, 0.4])
+    d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(1.5, 1.5, 1.5)))
+    d.lights.append(dict(kind=LIGHT_POINT, position=(-0.4, 1.2, 0.8), radiance=(3.0, 2.5, 2.0)))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(0.2, 0.25, 0.3), seed=1)
     return d
 
 

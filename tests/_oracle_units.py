@@ -75,9 +75,12 @@ def bsdf_eval(mat, n, wo, wi, t=None, b=None, dir_density=-1.0):
 
 
 def bsdf_sample(mat, n, wo, xi, strata=(0, 0, 1, 1), t=None, b=None, dir_density=0.0):
+    """xi: three uniforms (cosine pair, lobe selector), optionally two more
+    (the anisotropic microfacet models' specular pair, 0.5 by default)."""
+    xi5 = tuple(xi) + (0.5, 0.5)[: 5 - len(xi)]
     out = np.zeros(4, dtype=np.float32)
     tt = f32(*(t if t is not None else (0, 0, 0)))
     bb = f32(*(b if b is not None else (0, 0, 0)))
     lib().orc_bsdf_sample(ctypes.byref(mat.rec), fp(f32(*n)), fp(tt), fp(bb), fp(f32(*wo)), fp(f32(*strata)),
-                          fp(f32(*xi)), float(dir_density), fp(out))
+                          fp(f32(*xi5)), float(dir_density), fp(out))
     return out[:3], float(out[3])

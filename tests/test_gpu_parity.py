@@ -161,6 +161,19 @@ def test_paths_textured():
     _check_paths(g_rgb, o_rgb, g_hash, o_hash)
 
 
+def test_paths_anisotropic():
+    """The six anisotropic GGX / Beckmann models on tangent-frame meshes,
+    spheres and a mesh without tangents, with their second stratified pair
+    in the specular lobe."""
+    print("case: paths anisotropic 48x48 spp4 depth4", flush=True)
+    desc = workloads.anisotropic(48, 48, samples_camera=4, max_depth=4)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
 def test_texture_arguments_are_checked():
     print("case: texture arguments", flush=True)
     from tempestrenderer_amd._abi import TempestError

@@ -1578,6 +1578,13 @@ void orc_bsdf_sample(const tr_material* m, const float* n, const float* t, const
     out[0] = s.wi.x; out[1] = s.wi.y; out[2] = s.wi.z; out[3] = s.pdf;
 }
 
+/* n values of the pseudo-volume's LCG (FloatIntelFastRand) from `seed`. */
+void orc_lcg15(uint32_t seed, uint32_t n, float* out)
+{
+    OrcLcg g{seed};
+    for (uint32_t i = 0; i < n; ++i) out[i] = g.next();
+}
+
 float orc_fresnel_conductor(float eta, float k, float c) { return fresnel_conductor(eta, k, c); }
 float orc_fresnel_schlick(float r, float c) { return fresnel_schlick(r, c); }
 float orc_fresnel_dielectric(float eta, float ci, float ct) { return fresnel_dielectric(eta, ci, ct); }

@@ -634,6 +634,113 @@ template <class T> inline void sggx_flake_sample(Sample<T>* s, T ra, T rb)
     s->pdf = sggx_flake_pdf(s->dir_density, sd, basis, normalize(s->wi + s->wo));
 }
 
+/* ---- SGGX pseudo-volume surface (model 25) ---- */
+
+/* The flake normal of SGGXMicroFlakeSampleIncidentLight(out_light, ...)
+ * (ggx-models-impl.hh:554-570) for the covariance S = basis
+ * transformCovariance(stddev^2): Cholesky factor of S in the frame of wo. */
+template <class T> inline V3<T> sggx_flake_normal(V3<T> wo, const M3<T>& S, T ra, T rb)
+{
+    T sin_phi, cos_phi;
+    sincos_turns(ra, &sin_phi, &cos_phi);
+    T cos_theta = (T)2 * rb - (T)1;
+    T sin_theta = xsqrt((T)1 - cos_theta * cos_theta);
+    V3<T> us = v3(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta);
+    M3<T> ob = make_basis(wo);
+    M3<T> Sw = mul(mul(ob, S), transpose(ob));
+    return transform(ob, normalize(transform_rotation_inverse(cholesky(Sw), us)));
+}
+
+/* IntelFastRand / FloatIntelFastRand, math/functions.hh:214-218, 249-252 */
+struct OrcLcg {
+    uint32_t seed;
+    float next()
+    {
+        seed = 214013u * seed + 2531011u;
+        return (float)((seed >> 16) & 0x7FFFu) / 32767.0f;
+    }
+};
+
+/* (uint32_t)(tc * USHRT_MAX) of the seed: 0 for negative and NaN, saturated
+ * past 2^32 - 1 (undefined in C++). */
+inline uint32_t orc_tc_seed_part(float x) { return x > 0.0f ? (x < 4294967296.0f ? (uint32_t)x : 0xFFFFFFFFu) : 0u; }
+
+/* surface frame * ToMatrix3(quaternion as given) */
+template <class T> inline M3<T> sggx_pseudo_basis(const Sample<T>& s)
+{
+    T q[4];
+    sggx_surface_quat(s, q);
+    return mul(sample_frame(s), quat_to_m3(q));
+}
+
+/* SGGXMicroFlakePseudoVolumeBRDF, ggx-models-impl.hh:621-758.  The
+ * internal random walks use the reference's LCG seeded from the TexCoord;
+ * the two uniforms of each flake sample are drawn left to right. */
+template <class T> inline V3<T> sggx_pseudo_volume_brdf(const Sample<T>& s)
+{
+    const tr_material* m = s.mat;
+    T din = dot(s.normal, s.wi);
+    T don = dot(s.normal, s.wo);
+    if (don < (T)0 || din < (T)0) return v3((T)0, (T)0, (T)0);
+    V3<T> sd = sggx_surface_stddev(s);
+    if (sd.x < (T)1e-6f || sd.y < (T)1e-6f) return v3((T)0, (T)0, (T)0);
+    M3<T> basis = sggx_pseudo_basis(s);
+    OrcLcg lcg{(orc_tc_seed_part((float)s.tcy * 65535.0f) << 16) + orc_tc_seed_part((float)s.tcx * 65535.0f)};
+    lcg.next();
+    const uint32_t depth = m->sggx_depth, count = m->sggx_sample_count;
+    const T start = s.dir_density;
+    const T in_density = sggx_projected_area(sd, basis, s.wi);
+    const V3<T> spec = spec3<T>(m->specular);
+    V3<T> mn = normalize(s.wo + s.wi);
+    T contrib = (T)(float)count / (start + in_density * don / din);
+    V3<T> radiance = spec * contrib * sggx_ndf(sd, basis, mn) * (T)2 * (T)0.25;
+    const M3<T> S = transform_covariance(basis, sd * sd);
+    for (uint32_t k = 0; k < count; ++k) {
+        V3<T> thr = spec;
+        V3<T> out = s.wo;
+        T pen = (-xlog((T)lcg.next()) / start) * don;
+        for (uint32_t it = 0; it < depth; ++it) {
+            if (!(thr.x > (T)1e-3f && thr.y > (T)1e-3f && thr.z > (T)1e-3f)) break;
+            T ra = (T)lcg.next();
+            T rb = (T)lcg.next();
+            V3<T> inc = reflect(out, sggx_flake_normal(out, S, ra, rb));
+            T dd = sggx_projected_area(sd, basis, inc);
+            out = inc * (T)-1;
+            pen = pen + (-xlog((T)lcg.next()) / dd) * dot(s.normal, out);
+            if (pen < (T)0) break;
+            T ext = xexp(-in_density * (pen / din));
+            if (ext < (T)1e-9f) break;
+            V3<T> mn2 = normalize(out + s.wi);
+            thr = v3(thr.x * spec.x, thr.y * spec.y, thr.z * spec.z);
+            T phase = sggx_ndf(sd, basis, mn2) * (T)2 / ((T)4 * dd);
+            radiance = radiance + thr * ext * phase;
+        }
+    }
+    T rc = (T)1 / (T)(float)count;
+    return radiance * rc + spec3<T>(m->diffuse) * din * ((T)1 / C<T>::pi);
+}
+
+/* SGGXMicroFlakeSurfacePDF, ggx-models-impl.hh:760-794 */
+template <class T> inline T sggx_pseudo_volume_pdf(const Sample<T>& s)
+{
+    if (dot(s.wi, s.normal) < (T)0 || dot(s.wo, s.normal) < (T)0) return (T)0;
+    V3<T> mn = normalize(s.wi + s.wo);
+    return (T)2 * sggx_flake_pdf(s.dir_density, sggx_surface_stddev(s), sggx_pseudo_basis(s), mn);
+}
+
+/* SGGXMicroFlakeSurfaceSampleIncidentLight, ggx-models-impl.hh:796-842,
+ * including its flip of any flake normal with a non-zero cosine to the
+ * normal (:836), which leaves the reflected direction unchanged. */
+template <class T> inline void sggx_pseudo_volume_sample(Sample<T>* s, T ra, T rb)
+{
+    V3<T> sd = sggx_surface_stddev(*s);
+    M3<T> basis = sggx_pseudo_basis(*s);
+    V3<T> mn = sggx_flake_normal(s->wo, transform_covariance(basis, sd * sd), ra, rb);
+    if (dot(mn, s->normal) != (T)0) mn = mn * (T)-1;
+    s->wi = reflect(s->wo, mn);
+    s->pdf = dot(s->wi, s->normal) > (T)0 ? (T)2 * sggx_flake_pdf(s->dir_density, sd, basis, mn) : (T)0;
+}
+
 inline bool is_emissive(uint32_t m) { return m == TR_MODEL_EMISSIVE; }
 
 /* TransmittanceLookup (illumination-models.hh:279-324) for the models the
@@ -645,6 +752,7 @@ template <class T> inline V3<T> bsdf_eval(const Sample<T>& s)
     if (is_aniso(m)) return microfacet_aniso_brdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s);
+    if (m == TR_MODEL_SGGX_PSEUDO_VOLUME) return sggx_pseudo_volume_brdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
         T p = sggx_flake_brdf(s);
         return v3(p, p, p);
@@ -660,6 +768,7 @@ template <class T> inline T bsdf_pdf(const Sample<T>& s)
     if (is_aniso(m)) return microfacet_aniso_pdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
+    if (m == TR_MODEL_SGGX_PSEUDO_VOLUME) return sggx_pseudo_volume_pdf(s);
     if (m == TR_MODEL_SGGX_MICRO_FLAKE)
         return sggx_flake_pdf(s.dir_density, spec3<T>(s.mat->sggx_stddev), sample_frame(s), normalize(s.wi + s.wo));
     return (T)0;
@@ -674,6 +783,7 @@ template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, co
     else if (is_aniso(m)) microfacet_aniso_sample(st, s, xi);
     else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi);
+    else if (m == TR_MODEL_SGGX_PSEUDO_VOLUME) sggx_pseudo_volume_sample(s, xi[0], xi[1]);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi[0], xi[1]);
     else s->pdf = (T)0;
 }
@@ -681,7 +791,8 @@ template <class T> inline void bsdf_sample(const Strata<T>& st, Sample<T>* s, co
 /* MaterialCacheLookup (illumination-models.hh:455-492) */
 template <class T> inline void material_cache(Sample<T>* s)
 {
-    if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s);
+    if (s->mat->model == TR_MODEL_SGGX_SURFACE || s->mat->model == TR_MODEL_SGGX_PSEUDO_VOLUME)
+        s->dir_density = sggx_surface_density(*s);
 }
 
 }  // namespace orc

@@ -200,7 +200,7 @@ bool model_supported(uint32_t m)
            (m >= TR_MODEL_BECKMANN_MICROFACET && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC) ||
            (m >= TR_MODEL_GGX_MICROFACET_ANISOTROPIC && m <= TR_MODEL_GGX_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
            (m >= TR_MODEL_BECKMANN_MICROFACET_ANISOTROPIC && m <= TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC_ANISOTROPIC) ||
-           m == TR_MODEL_SGGX_SURFACE;
+           m == TR_MODEL_SGGX_SURFACE || m == TR_MODEL_SGGX_PSEUDO_VOLUME;
 }
 
 tr_status ensure_device(tr_scene* s)
@@ -245,6 +245,8 @@ SceneView scene_view(const tr_scene* s)
     v.tex.count = (uint32_t)s->tex_recs.size();
     v.tex.cube_side = s->cube_side;
     v.tex.cube_offset = s->cube_offset;
+    v.tex.pseudo_volume = 0;
+    for (const tr_material& m : s->mats) v.tex.pseudo_volume |= m.model == TR_MODEL_SGGX_PSEUDO_VOLUME;
     return v;
 }
 
@@ -1262,8 +1264,15 @@ tr_status check_maps(const tr_scene* s, const tr_material& m)
     };
     if (m.model == TR_MODEL_BLINN_PHONG && !ok(m.diffuse_map, TEX_SPECTRUM))
         return fail(TR_ERROR_INVALID_ARG, "DiffuseMap: unknown texture or a format SampleSpectrum does not read");
-    if (m.model == TR_MODEL_SGGX_SURFACE) {
-        if (!ok(m.diffuse_map, TEX_SPECTRUM) || !ok(m.specular_map, TEX_SPECTRUM))
+    if (m.model == TR_MODEL_SGGX_PSEUDO_VOLUME) {
+        /* the BRDF runs SampleCount walks of up to Depth scatterings per
+         * evaluation: 0 walks divide by zero, and the bounds keep one
+         * evaluation within a kernel's time */
+        if (m.sggx_sample_count < 1 || m.sggx_sample_count > 4096 || m.sggx_depth > 64)
+            return fail(TR_ERROR_INVALID_ARG, "pseudo-volume SampleCount must be 1..4096 and Depth at most 64");
+    }
+    if (m.model == TR_MODEL_SGGX_SURFACE || m.model == TR_MODEL_SGGX_PSEUDO_VOLUME) {
+        if (m.model == TR_MODEL_SGGX_SURFACE && (!ok(m.diffuse_map, TEX_SPECTRUM) || !ok(m.specular_map, TEX_SPECTRUM)))
             return fail(TR_ERROR_INVALID_ARG, "Diffuse/SpecularMap: unknown texture or unreadable format");
         if (!ok(m.stddev_map, TEX_RG))
             return fail(TR_ERROR_INVALID_ARG, "StandardDeviationMap: unknown texture or a format SampleRG does not read");

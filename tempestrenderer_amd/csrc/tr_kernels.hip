@@ -1243,7 +1243,8 @@ void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 {
     const uint32_t grid = (n_slots + 255) / 256;
     /* material maps are bound only to scenes that have textures */
-    const bool tex = p.sc.tex.count > 0 || p.sc.tex.cube_side > 0;
+    /* the texture instances also carry the pseudo-volume BRDF's walks */
+    const bool tex = p.sc.tex.count > 0 || p.sc.tex.cube_side > 0 || p.sc.tex.pseudo_volume;
     if (p.sc.n_volumes > 0) {
         if (tex) {
             if (p.cache) launch_k(k_shade<true, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);

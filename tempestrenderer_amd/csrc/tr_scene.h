@@ -122,6 +122,7 @@ struct TexView {
     uint32_t count;
     uint32_t cube_side;
     uint64_t cube_offset;
+    uint32_t pseudo_volume;  /* a material of the scene is the SGGX pseudo-volume */
 };
 
 /* Device pointers of a committed scene. */

@@ -374,9 +374,17 @@ TR_HD float lobe_ratio(V3 albedo, V3 specular)
     return mv != 0.0f ? max_value(specular) / mv : 0.0f;
 }
 
-/* Scenes without texture maps and cube map shade through this view: every
- * map lookup is the material's constant, at no cost. */
+/* Scenes without texture maps, cube map and pseudo-volume materials shade
+ * through this view: every map lookup is the material's constant, at no
+ * cost, and the pseudo-volume BRDF (hundreds of random walks per call) is
+ * left out of those kernel instances. */
 struct NoTex {};
+template <class TV> struct FullModels {
+    static constexpr bool value = true;
+};
+template <> struct FullModels<NoTex> {
+    static constexpr bool value = false;
+};
 
 TR_HD V3 map_spectrum(const NoTex&, uint32_t, const float* constant, const Sample&) { return ld3(constant); }
 
@@ -603,26 +611,130 @@ TR_HD float sggx_flake_pdf(const Sample& s)
     return sggx_flake_pdf_h(s.dir_density, ld3(s.mat->sggx_stddev), frame_of(s), normalize(add(s.wi, s.wo)));
 }
 
-/* SGGXMicroFlakeSampleIncidentLight (ggx-models-impl.hh:554-580): a
- * uniform sphere sample (UniformSampleSphere, sampling3.hh:106-116) mapped
- * through the Cholesky factor of S expressed in the frame of wo, reflected
- * about the resulting flake normal. */
-TR_HD void sggx_flake_sample(Sample* s, float ra, float rb)
+/* The visible flake normal of SGGXMicroFlakeSampleIncidentLight
+ * (ggx-models-impl.hh:554-570): a uniform sphere sample (UniformSampleSphere,
+ * sampling3.hh:106-116) mapped through the Cholesky factor of the covariance
+ * S expressed in the frame of wo. */
+TR_HD V3 sggx_visible_normal(V3 wo, const M3& S, float ra, float rb)
 {
     float sin_phi, cos_phi;
     tr_sincos_turns(ra, &sin_phi, &cos_phi);
     float cos_theta = 2.0f * rb - 1.0f;
     float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
     V3 u = mk(cos_phi * sin_theta, sin_phi * sin_theta, cos_theta);
+    M3 ob = make_basis(wo);
+    M3 So = mul(mul(ob, S), transpose(ob));
+    V3 vn = normalize(transform_rotation_inverse(cholesky(So), u));
+    return transform(ob, vn);
+}
+
+/* SGGXMicroFlakeSampleIncidentLight (ggx-models-impl.hh:554-580): wo
+ * reflected about the visible flake normal. */
+TR_HD void sggx_flake_sample(Sample* s, float ra, float rb)
+{
     V3 sd = ld3(s->mat->sggx_stddev);
     M3 basis = frame_of(*s);
     M3 S = transform_covariance(basis, mulv(sd, sd));
-    M3 ob = make_basis(s->wo);
-    S = mul(mul(ob, S), transpose(ob));
-    V3 vn = normalize(transform_rotation_inverse(cholesky(S), u));
-    vn = transform(ob, vn);
-    s->wi = reflect(s->wo, vn);
+    s->wi = reflect(s->wo, sggx_visible_normal(s->wo, S, ra, rb));
     s->pdf = sggx_flake_pdf_h(s->dir_density, sd, basis, normalize(add(s->wi, s->wo)));
+}
+
+/* ---------------- SGGX pseudo-volume surface (model 25) ---------------- */
+
+/* FloatIntelFastRand over IntelFastRand (math/functions.hh:214-218,
+ * 249-252): the reference's 15-bit LCG.  The pseudo-volume BRDF seeds it
+ * from the TexCoord, so its sequence is a function of the hit point and is
+ * reproduced exactly. */
+struct Lcg15 {
+    uint32_t seed;
+    TR_HD float next()
+    {
+        seed = 214013u * seed + 2531011u;
+        return (float)((seed >> 16) & 0x7FFFu) / 32767.0f;
+    }
+};
+
+/* (uint32_t)x of the seed formula: negative and NaN give 0, values past
+ * 2^32 - 1 saturate (the C++ conversion is undefined there). */
+TR_HD uint32_t tc_to_u32(float x) { return x > 0.0f ? (x < 4294967296.0f ? (uint32_t)x : 0xFFFFFFFFu) : 0u; }
+
+/* sggx_surface_orientation * ToMatrix3(quat) with the quaternion as given
+ * (ggx-models-impl.hh:646-659, 780-792, 817-828). */
+template <class TV> TR_HD M3 sggx_pseudo_basis(const Sample& s, const TV& tv)
+{
+    V4f q = sggx_surface_quat(s, tv);
+    return mul(frame_of(s), quat_to_m3(q.x, q.y, q.z, q.w));
+}
+
+/* SGGXMicroFlakePseudoVolumeBRDF, ggx-models-impl.hh:621-758: single
+ * scattering of the surface's flakes plus SampleCount random walks of up to
+ * Depth flake scatterings below the surface, each scattering attenuated by
+ * the extinction back along the incident direction.  Specular and Diffuse
+ * are the material's constants here (the reference reads no maps for
+ * them); StandardDeviation and the basis come from their maps. */
+template <class TV> TR_HD V3 sggx_pseudo_volume_brdf(const Sample& s, const TV& tv)
+{
+    const tr_material* m = s.mat;
+    float din = dot(s.normal, s.wi);
+    float don = dot(s.normal, s.wo);
+    if (don < 0.0f || din < 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    V3 sd = sggx_surface_stddev(s, tv);
+    if (sd.x < 1e-6f || sd.y < 1e-6f) return mk(0.0f, 0.0f, 0.0f);
+    M3 basis = sggx_pseudo_basis(s, tv);
+    Lcg15 rng{(tc_to_u32(s.tcy * 65535.0f) << 16) + tc_to_u32(s.tcx * 65535.0f)};
+    rng.next();
+    const uint32_t depth = m->sggx_depth, count = m->sggx_sample_count;
+    const float start = s.dir_density;
+    const float in_density = sggx_projected_area(sd, basis, s.wi);
+    const V3 spec = ld3(m->specular);
+    V3 mn = normalize(add(s.wo, s.wi));
+    float contrib = (float)count / (start + in_density * don / din);
+    V3 radiance = muls(muls(muls(smul(contrib, spec), sggx_ndf(sd, basis, mn)), 2.0f), 0.25f);
+    const M3 S = transform_covariance(basis, mulv(sd, sd));
+    for (uint32_t k = 0; k < count; ++k) {
+        V3 thr = spec;
+        V3 out = s.wo;
+        float pen = (-tr_logf(rng.next()) / start) * don;
+        for (uint32_t it = 0; thr.x > 1e-3f && thr.y > 1e-3f && thr.z > 1e-3f && it < depth; ++it) {
+            float ra = rng.next();
+            float rb = rng.next();
+            V3 inc = reflect(out, sggx_visible_normal(out, S, ra, rb));
+            float dd = sggx_projected_area(sd, basis, inc);
+            out = neg(inc);
+            pen += (-tr_logf(rng.next()) / dd) * dot(s.normal, out);
+            if (pen < 0.0f) break;
+            float ext = tr_expf(-in_density * (pen / din));
+            if (ext < 1e-9f) break;
+            V3 mn2 = normalize(add(out, s.wi));
+            thr = mulv(thr, smul(1.0f, spec));
+            float phase = sggx_ndf(sd, basis, mn2) * 2.0f / (4.0f * dd);
+            radiance = add(radiance, smul(phase, smul(ext, thr)));
+        }
+    }
+    return add(divs(radiance, (float)count), muls(muls(ld3(m->diffuse), din), 1.0f / kPi));
+}
+
+/* SGGXMicroFlakeSurfacePDF, ggx-models-impl.hh:760-794 */
+template <class TV> TR_HD float sggx_pseudo_volume_pdf(const Sample& s, const TV& tv)
+{
+    if (dot(s.wi, s.normal) < 0.0f || dot(s.wo, s.normal) < 0.0f) return 0.0f;
+    V3 mn = normalize(add(s.wi, s.wo));
+    return 2.0f * sggx_flake_pdf_h(s.dir_density, sggx_surface_stddev(s, tv), sggx_pseudo_basis(s, tv), mn);
+}
+
+/* SGGXMicroFlakeSurfaceSampleIncidentLight, ggx-models-impl.hh:796-842,
+ * with its flip of every flake normal not exactly orthogonal to the normal
+ * (`if(Dot(micro_norm, Normal))`, :836; the reflection does not depend on
+ * the sign). */
+template <class TV> TR_HD void sggx_pseudo_volume_sample(Sample* s, float ra, float rb, const TV& tv)
+{
+    V3 sd = sggx_surface_stddev(*s, tv);
+    M3 basis = sggx_pseudo_basis(*s, tv);
+    M3 S = transform_covariance(basis, mulv(sd, sd));
+    V3 mn = sggx_visible_normal(s->wo, S, ra, rb);
+    if (dot(mn, s->normal) != 0.0f) mn = neg(mn);
+    s->wi = reflect(s->wo, mn);
+    s->pdf = dot(s->wi, s->normal) > 0.0f ? 2.0f * sggx_flake_pdf_h(s->dir_density, sd, basis, mn) : 0.0f;
 }
 
 TR_HD bool is_emissive(uint32_t m) { return m == TR_MODEL_EMISSIVE; }
@@ -631,7 +743,8 @@ TR_HD bool is_volume_model(uint32_t m) { return m == TR_MODEL_SGGX_MICRO_FLAKE; 
 /* TransmittanceLookup / PDFLookup / SampleIncidentLightLookup /
  * MaterialCacheLookup (illumination-models.hh:233-492) for the supported
  * models: Blinn-Phong, GGX and Beckmann (plain, conductor, dielectric;
- * isotropic and anisotropic), SGGX surface, SGGX micro-flake volume. */
+ * isotropic and anisotropic), SGGX surface and pseudo-volume, SGGX
+ * micro-flake volume. */
 template <class TV> TR_HD V3 bsdf_eval(const Sample& s, const TV& tv)
 {
     uint32_t m = s.mat->model;
@@ -639,6 +752,9 @@ template <class TV> TR_HD V3 bsdf_eval(const Sample& s, const TV& tv)
     if (is_aniso(m)) return microfacet_aniso_brdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_brdf(s, tv);
     if (m == TR_MODEL_SGGX_SURFACE) return sggx_surface_brdf(s, tv);
+    if constexpr (FullModels<TV>::value) {
+        if (m == TR_MODEL_SGGX_PSEUDO_VOLUME) return sggx_pseudo_volume_brdf(s, tv);
+    }
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) {
         float p = sggx_flake_phase(s);
         return mk(p, p, p);
@@ -652,6 +768,9 @@ template <class TV> TR_HD float bsdf_pdf(const Sample& s, const TV& tv)
     if (is_aniso(m)) return microfacet_aniso_pdf(s);
     if (m == TR_MODEL_BLINN_PHONG) return blinn_phong_pdf(s, tv);
     if (m == TR_MODEL_SGGX_SURFACE) return uniform_hemisphere_pdf(s);
+    if constexpr (FullModels<TV>::value) {
+        if (m == TR_MODEL_SGGX_PSEUDO_VOLUME) return sggx_pseudo_volume_pdf(s, tv);
+    }
     if (m == TR_MODEL_SGGX_MICRO_FLAKE) return sggx_flake_pdf(s);
     return 0.0f;
 }
@@ -670,12 +789,14 @@ TR_HD void bsdf_sample(const Strata& st, Sample* s, float xi0, float xi1, float 
     }
     else if (m == TR_MODEL_BLINN_PHONG) blinn_phong_sample(st, s, xi0, xi1, xi2, tv);
     else if (m == TR_MODEL_SGGX_SURFACE) uniform_hemisphere_sample(st, s, xi0, xi1);
+    else if (FullModels<TV>::value && m == TR_MODEL_SGGX_PSEUDO_VOLUME) sggx_pseudo_volume_sample(s, xi0, xi1, tv);
     else if (m == TR_MODEL_SGGX_MICRO_FLAKE) sggx_flake_sample(s, xi0, xi1);
     else s->pdf = 0.0f;
 }
 template <class TV> TR_HD void material_cache(Sample* s, const TV& tv)
 {
-    if (s->mat->model == TR_MODEL_SGGX_SURFACE) s->dir_density = sggx_surface_density(*s, tv);
+    if (s->mat->model == TR_MODEL_SGGX_SURFACE || s->mat->model == TR_MODEL_SGGX_PSEUDO_VOLUME)
+        s->dir_density = sggx_surface_density(*s, tv);
 }
 
 /* ---------------- geometry sampling ---------------- */

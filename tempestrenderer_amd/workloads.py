@@ -23,6 +23,7 @@ Workloads:
                    depth 32 (optionally a floor and a directional light).
   anisotropic(...) the six anisotropic GGX / Beckmann models on tangent-frame
                    quads, spheres and a quad without tangents; 48x48.
+  pseudo_volume(...) SGGX pseudo-volume surfaces (mapped and constant); 32x32.
 Scene randomness comes from numpy's PCG64 seeded with `seed`; the path
 tracer's own random numbers come from Philox keyed by the render seed.
 """
@@ -481,6 +482,46 @@ def anisotropic(width=48, height=48, samples_camera=4, max_depth=4):
     d.lights.append(dict(kind=LIGHT_POINT, position=(-0.4, 1.2, 0.8), radiance=(3.0, 2.5, 2.0)))
     d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
                       background=(0.2, 0.25, 0.3), seed=1)
+    return d
+
+
+SGGX_PSEUDO_VOLUME = 25
+
+
+def pseudo_volume(width=32, height=32, samples_camera=2, max_depth=3, seed=9):
+    """The SGGX pseudo-volume surface (model 25): a quad (PTTNFormat, texture
+    coordinates 0..1, so its BRDF's walks are seeded per point) with
+    StandardDeviation (RG8) and basis (RGBA32F) maps and 256 walks of depth
+    2, two spheres with constant parameters (64 walks of depth 3; 32 walks
+    of depth 1 and a specular channel under the walks' 1e-3 cut), a GGX floor
+    and a directional light."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vpi = perspective_view_proj_inv((0.0, 0.5, 2.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
+    d = SceneDesc("pseudo_volume", width, height, vpi)
+    sd = rng.integers(30, 255, size=(8, 8, 2), dtype=np.uint8)
+    d.textures.append((sd, 1, 0))                                           # 1: RG8 bilinear
+    q = rng.normal(size=(8, 8, 4)).astype(np.float32)
+    q /= np.linalg.norm(q, axis=-1, keepdims=True)
+    d.textures.append((q, 6, 1))                                            # 2: RGBA32F nearest
+    wall = dict(model=SGGX_PSEUDO_VOLUME, diffuse=(0.1, 0.1, 0.1), specular=(0.8, 0.7, 0.6),
+                specular_power=(0.3, 0.3), stddev_map=1, basis_map=2, sggx_depth=2, sggx_sample_count=256)
+    wv, wt = _textured_quad((0.0, 0.2, -0.8), (1.0, 0.0, 0.0), (0.0, 0.7, 0.0), tangent=True)
+    d.meshes.append((wv, wt, [(wall, 0, 6)], False))
+    floor = dict(model=GGX, diffuse=(0.4, 0.4, 0.4), specular=(0.2, 0.2, 0.2),
+                 specular_power=(alpha_to_specular_power(0.3),) * 2)
+    fv, ft = _quad((0.0, -0.5, 0.0), (0.0, 0.0, 1.2), (1.2, 0.0, 0.0))
+    d.meshes.append((fv, ft, [(floor, 0, 6)], False))
+    d.spheres.append(((-0.4, -0.2, 0.1), 0.3, dict(model=SGGX_PSEUDO_VOLUME, diffuse=(0.05, 0.1, 0.05),
+                                                     specular=(0.9, 0.9, 0.9), specular_power=(0.15, 0.6),
+                                                     sggx_basis=(0.0, 0.38268343, 0.0, 0.9238795),
+                                                     sggx_depth=3, sggx_sample_count=64)))
+    d.spheres.append(((0.45, -0.25, 0.2), 0.25, dict(model=SGGX_PSEUDO_VOLUME, diffuse=(0.2, 0.1, 0.1),
+                                                       specular=(0.7, 0.0005, 0.7), specular_power=(0.5, 0.2),
+                                                       sggx_depth=1, sggx_sample_count=32)))
+    ld = np.array([0.3, 0.8, 0.5]) / np.linalg.norm([0.3, 0.8, 0.5])
+    d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,
+                      background=(0.3, 0.3, 0.35), seed=1)
     return d
 
 

@@ -23,6 +23,7 @@ def lib():
             getattr(lb, name).restype = F
             getattr(lb, name).argtypes = [F, F] if name == "orc_fresnel_schlick" else [F, F, F]
         lb.orc_make_basis.argtypes = [PF, PF]
+        lb.orc_lcg15.argtypes = [ctypes.c_uint32, ctypes.c_uint32, PF]
         lb.orc_sample_sphere_light.argtypes = [PF, PF, F, F, F, PF]
         lb.orc_sphere_light_pdf.restype = F
         lb.orc_sphere_light_pdf.argtypes = [PF, PF, PF, F]

@@ -174,6 +174,30 @@ def test_paths_anisotropic():
     _check_paths(g_rgb, o_rgb, g_hash, o_hash)
 
 
+def test_paths_pseudo_volume():
+    """SGGX pseudo-volume surfaces: TexCoord-seeded random walks inside
+    the BRDF, mapped and constant parameters, a channel under the walks'
+    cut."""
+    print("case: paths pseudo-volume 32x32 spp2 depth3", flush=True)
+    desc = workloads.pseudo_volume(32, 32, samples_camera=2, max_depth=3)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 2)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
+def test_pseudo_volume_arguments_are_checked():
+    print("case: pseudo-volume arguments", flush=True)
+    from tempestrenderer_amd._abi import TempestError
+    desc = workloads.box(8, 8)
+    for bad in (dict(sggx_sample_count=0), dict(sggx_sample_count=5000), dict(sggx_depth=65)):
+        g = RayTracerScene(8, 8, desc.vpi)
+        g.addSphere((0, 0, 0), 0.5, gpu_material(workloads.SGGX_PSEUDO_VOLUME, bad))
+        with pytest.raises(TempestError):
+            g.commitScene()
+
+
 def test_texture_arguments_are_checked():
     print("case: texture arguments", flush=True)
     from tempestrenderer_amd._abi import TempestError

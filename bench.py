@@ -116,6 +116,29 @@ def gonio_bench(cpu_paths=2.0e8, threads=16, cpu=True):
     return out
 
 
+def pseudo_volume_bench(width=1920, height=1080, repeats=5):
+    """The reference's pseudo-volume microbenchmark (ComputeImage,
+    tests/benchmark/pseudo-volume-performance-test.cu): one SGGX pseudo-volume
+    BRDF evaluation per pixel of a 1080p image, SampleCount 256, Depth 1,
+    Specular 0.75, StandardDeviation (0.5, 0.5), identity basis, fixed 45
+    degree directions."""
+    import ctypes
+    from tempestrenderer_amd import _abi
+    from tempestrenderer_amd.scene import Material
+    m = Material(25, diffuse=(0.0, 0.0, 0.0), specular=(0.75, 0.75, 0.75), specular_power=(0.5, 0.5),
+                 sggx_basis=(0.0, 0.0, 0.0, 1.0), sggx_depth=1, sggx_sample_count=256).setup()
+    img = np.zeros(width * height, dtype=np.uint32)
+    probe = np.zeros(4, dtype=np.float32)
+    ms = ctypes.c_float()
+    _abi.check(_abi.load().tr_bench_pseudo_volume_image(
+        ctypes.byref(m.rec), width, height, repeats, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+        probe.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(ms)))
+    evals = width * height
+    return {"workload": "SGGX pseudo-volume BRDF, %dx%d evaluations, SampleCount 256, Depth 1" % (width, height),
+            "ms_per_image": round(ms.value, 3), "mevals_per_s": round(evals / ms.value / 1e3, 2),
+            "walks_per_s": round(evals * 256 / ms.value * 1e3, 0), "brdf": [float(v) for v in probe[:3]]}
+
+
 def volume_bench(cpu_rows=128, threads=16, cpu=True):
     """The heterogeneous SGGX volume: 256^3 binary16 grid (mean extinction
     4), 1024x1024, 128 camera samples, depth 32, delta tracking.  One timed
@@ -285,6 +308,7 @@ def main():
     ap.add_argument("--no-gonio", action="store_true", help="skip the batched gonioreflectometer run")
     ap.add_argument("--gonio-only", action="store_true", help="only the batched gonioreflectometer run")
     ap.add_argument("--no-volume", action="store_true", help="skip the volume run")
+    ap.add_argument("--no-pseudo-volume", action="store_true", help="skip the pseudo-volume BRDF microbenchmark")
     ap.add_argument("--volume-only", action="store_true", help="only the volume run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 512 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
@@ -383,6 +407,11 @@ def main():
                "speedup_off_emitter": (round(plain["off_emitter"]["seconds"] / cached["off_emitter"]["seconds"], 3)
                                        if plain["off_emitter"]["seconds"] and cached["off_emitter"]["seconds"]
                                        else None)}
+
+    pv = None
+    if not args.no_pseudo_volume:
+        pv = pseudo_volume_bench()
+        log("pseudo_volume_brdf: %s" % json.dumps(pv))
 
     vol = None
     if not args.no_volume:

@@ -221,6 +221,17 @@ typedef struct tr_scene_info {
 
 const char* tr_last_error_message(void);
 
+/* The reference's pseudo-volume benchmark kernel ComputeImage
+ * (tests/benchmark/pseudo-volume-performance-test.cu:16-41): every pixel of
+ * a width x height image evaluates SGGXMicroFlakePseudoVolumeBRDF of `m`
+ * (model 25, constant StandardDeviation and basis) at wi = (0, sin 45,
+ * cos 45), wo = (0, -sin 45, cos 45) after SGGXSurfaceCache and stores
+ * ToColor of it.  One warm-up and `repeats` timed launches on the current
+ * device; rgba receives the image (host memory), probe pixel (0, 0)'s BRDF
+ * and directional density, ms_per_image the mean launch time. */
+tr_status tr_bench_pseudo_volume_image(const tr_material* m, uint32_t width, uint32_t height, uint32_t repeats,
+                                       uint32_t* rgba, float probe[4], float* ms_per_image);
+
 /* RTMaterial::setup (ray-tracing.cc:80-87): applies MaterialSetupLookup,
  * i.e. DefaultMicrofacetMaterialSetup (illumination-models.hh:599-614). */
 tr_status tr_material_init(tr_material* material, uint32_t model);

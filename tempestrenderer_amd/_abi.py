@@ -185,6 +185,8 @@ _st = ctypes.c_int
 # name -> (restype, argtypes)
 SIGNATURES = {
     "tr_last_error_message": (ctypes.c_char_p, []),
+    "tr_bench_pseudo_volume_image": (_st, [P(tr_material), _u32, _u32, _u32, P(_u32), P(ctypes.c_float),
+                                           P(ctypes.c_float)]),
     "tr_material_init": (_st, [P(tr_material), _u32]),
     "tr_material_setup": (_st, [P(tr_material)]),
     "tr_scene_create": (_st, [_u32, _u32, P(ctypes.c_float), P(tr_settings), P(_vp)]),

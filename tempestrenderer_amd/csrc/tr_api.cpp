@@ -1872,4 +1872,48 @@ tr_status tr_debug_bvh_check(const tr_scene* s, uint64_t* violations)
     return TR_OK;
 }
 
+/* The pseudo-volume BRDF microbenchmark (ComputeImage,
+ * tests/benchmark/pseudo-volume-performance-test.cu:16-41): `repeats` timed
+ * width x height launches after one warm-up. */
+tr_status tr_bench_pseudo_volume_image(const tr_material* m, uint32_t width, uint32_t height, uint32_t repeats,
+                                       uint32_t* rgba, float probe[4], float* ms_per_image)
+{
+    if (!m || !rgba || !probe || !ms_per_image) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (m->model != TR_MODEL_SGGX_PSEUDO_VOLUME) return fail(TR_ERROR_INVALID_ARG, "the material is not a pseudo-volume");
+    if (m->stddev_map || m->basis_map) return fail(TR_ERROR_INVALID_ARG, "the benchmark takes constant parameters");
+    if (m->sggx_sample_count < 1 || m->sggx_sample_count > 4096 || m->sggx_depth > 64)
+        return fail(TR_ERROR_INVALID_ARG, "pseudo-volume SampleCount must be 1..4096 and Depth at most 64");
+    if (width < 1 || height < 1 || width > 16384 || height > 16384 || repeats < 1 || repeats > 1000)
+        return fail(TR_ERROR_INVALID_ARG, "image size or repeat count out of range");
+    const size_t n = (size_t)width * height;
+    DevBuf<tr_material> dm;
+    DevBuf<uint32_t> dimg;
+    DevBuf<float> dprobe;
+    TR_HIP(dm.reserve(1));
+    TR_HIP(dimg.reserve(n));
+    TR_HIP(dprobe.reserve(4));
+    TR_HIP(hipMemcpy(dm.ptr, m, sizeof(tr_material), hipMemcpyHostToDevice));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TR_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(TR_ERROR_DEVICE, "hipEventCreate failed");
+    }
+    launch_pseudo_volume_image(nullptr, dm.ptr, width, height, dimg.ptr, dprobe.ptr);
+    (void)hipEventRecord(e0, nullptr);
+    for (uint32_t r = 0; r < repeats; ++r) launch_pseudo_volume_image(nullptr, dm.ptr, width, height, dimg.ptr, dprobe.ptr);
+    (void)hipEventRecord(e1, nullptr);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TR_HIP(e);
+    TR_HIP(hipGetLastError());
+    TR_HIP(hipMemcpy(rgba, dimg.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    TR_HIP(hipMemcpy(probe, dprobe.ptr, 4 * sizeof(float), hipMemcpyDeviceToHost));
+    *ms_per_image = ms / (float)repeats;
+    return TR_OK;
+}
+
 }  // extern "C"

@@ -182,6 +182,8 @@ void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, t
 /* The backbuffer of the frame's mean radiance through the picture
  * post-process; the auto-exposure modes use `partial` (3 doubles per 256
  * pixels) and write their exposure to `exposure`. */
+void launch_pseudo_volume_image(hipStream_t s, const tr_material* m, uint32_t width, uint32_t height, uint32_t* rgba,
+                                float* probe);
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,
                     uint32_t n_pixels, uint32_t post, double* partial, float* exposure);
 

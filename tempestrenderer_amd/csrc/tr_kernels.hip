@@ -1123,6 +1123,39 @@ __device__ inline uint32_t to_color(float x, float y, float z)
     return (uint32_t)(255.0f * x) | ((uint32_t)(255.0f * y) << 8u) | ((uint32_t)(255.0f * z) << 16u) | (255u << 24u);
 }
 
+/* ComputeImage (tests/benchmark/pseudo-volume-performance-test.cu:16-41):
+ * every pixel evaluates the pseudo-volume BRDF at wi = (0, sin 45, cos 45),
+ * wo = (0, -sin 45, cos 45) in the unit frame, after SGGXSurfaceCache, and
+ * stores ToColor of it.  TexCoord is (0, 0), so every pixel runs the same
+ * walks (the benchmark's "most optimistic case"). */
+__global__ void k_pseudo_volume_image(const tr_material* m, uint32_t width, uint32_t height, uint32_t* rgba,
+                                      float* probe)
+{
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    float sn, cs;
+    tr_sincos_turns(0.125f, &sn, &cs);
+    Sample sd{};
+    sd.mat = m;
+    sd.wi = mk(0.0f, sn, cs);
+    sd.wo = mk(0.0f, -sn, cs);
+    sd.tangent = mk(1.0f, 0.0f, 0.0f);
+    sd.binormal = mk(0.0f, 1.0f, 0.0f);
+    sd.normal = mk(0.0f, 0.0f, 1.0f);
+    sd.tcx = sd.tcy = 0.0f;
+    const NoTex tv{};
+    sd.dir_density = sggx_surface_density(sd, tv);
+    V3 f = sggx_pseudo_volume_brdf(sd, tv);
+    rgba[(size_t)y * width + x] = to_color(f.x, f.y, f.z);
+    if (x == 0 && y == 0) {
+        probe[0] = f.x;
+        probe[1] = f.y;
+        probe[2] = f.z;
+        probe[3] = sd.dir_density;
+    }
+}
+
 /* ACESFilm (math/hdr.hh:76-84): x (a x + b) / (x (c x + d) + e), clamped
  * to [0, 1]. */
 __device__ inline float aces_film(float x)
@@ -1216,6 +1249,14 @@ void launch_k(K kernel, uint32_t grid, uint32_t block, hipStream_t s, const Laun
 {
     if (ev) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, ev->start, ev->stop, 0u, args...);
     else kernel<<<dim3(grid), dim3(block), 0, s>>>(args...);
+}
+
+void launch_pseudo_volume_image(hipStream_t s, const tr_material* m, uint32_t width, uint32_t height, uint32_t* rgba,
+                                float* probe)
+{
+    dim3 block(8, 8, 1);
+    dim3 grid((width + 7) / 8, (height + 7) / 8, 1);
+    hipLaunchKernelGGL(k_pseudo_volume_image, grid, block, 0, s, m, width, height, rgba, probe);
 }
 
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,

@@ -198,6 +198,37 @@ def test_pseudo_volume_arguments_are_checked():
             g.commitScene()
 
 
+def test_pseudo_volume_benchmark_image():
+    """The ComputeImage microbenchmark: every pixel holds ToColor of the
+    same BRDF value, which matches the oracle's at the returned directional
+    density; that density is the closed-form projected area
+    sqrt(0.25 sin^2 + cos^2) of stddev (0.5, 0.5, 1) at 45 degrees."""
+    print("case: pseudo-volume benchmark image 64x32", flush=True)
+    import ctypes
+    import _oracle_units as U
+    from tempestrenderer_amd import _abi
+    fields = dict(diffuse=(0.0, 0.0, 0.0), specular=(0.75, 0.75, 0.75), specular_power=(0.5, 0.5),
+                  sggx_basis=(0.0, 0.0, 0.0, 1.0), sggx_depth=1, sggx_sample_count=256)
+    m = gpu_material(workloads.SGGX_PSEUDO_VOLUME, fields)
+    img = np.zeros(64 * 32, dtype=np.uint32)
+    probe = np.zeros(4, dtype=np.float32)
+    ms = ctypes.c_float()
+    _abi.check(_abi.load().tr_bench_pseudo_volume_image(
+        ctypes.byref(m.rec), 64, 32, 2, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+        probe.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(ms)))
+    assert ms.value > 0.0
+    assert np.all(img == img[0])
+    sn = float(U.fmath(0, np.array([0.125], np.float32))[0])
+    cs = float(U.fmath(1, np.array([0.125], np.float32))[0])
+    assert probe[3] == pytest.approx(np.sqrt(0.25 * 0.5 + 0.5), rel=1e-6)
+    om = oracle.material(workloads.SGGX_PSEUDO_VOLUME, fields)
+    f, _ = U.bsdf_eval(om, (0, 0, 1), (0, -sn, cs), (0, sn, cs), t=(1, 0, 0), b=(0, 1, 0),
+                       dir_density=float(probe[3]))
+    np.testing.assert_allclose(probe[:3], f, rtol=1e-5)
+    q = [int(np.float32(255.0) * np.float32(min(max(v, 0.0), 1.0))) for v in probe[:3]]
+    assert img[0] == (q[0] | (q[1] << 8) | (q[2] << 16) | (255 << 24))
+
+
 def test_texture_arguments_are_checked():
     print("case: texture arguments", flush=True)
     from tempestrenderer_amd._abi import TempestError

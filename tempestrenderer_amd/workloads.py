@@ -493,8 +493,9 @@ def pseudo_volume(width=32, height=32, samples_camera=2, max_depth=3, seed=9):
     coordinates 0..1, so its BRDF's walks are seeded per point) with
     StandardDeviation (RG8) and basis (RGBA32F) maps and 256 walks of depth
     2, two spheres with constant parameters (64 walks of depth 3; 32 walks
-    of depth 1 and a specular channel under the walks' 1e-3 cut), a GGX floor
-    and a directional light."""
+    of depth 1 and a specular channel under the walks' 1e-3 cut), an
+    anisotropic Beckmann-conductor sphere, a GGX floor and a directional
+    light."""
     rng = np.random.Generator(np.random.PCG64(seed))
     vpi = perspective_view_proj_inv((0.0, 0.5, 2.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, width / height)
     d = SceneDesc("pseudo_volume", width, height, vpi)
@@ -518,6 +519,12 @@ def pseudo_volume(width=32, height=32, samples_camera=2, max_depth=3, seed=9):
     d.spheres.append(((0.45, -0.25, 0.2), 0.25, dict(model=SGGX_PSEUDO_VOLUME, diffuse=(0.2, 0.1, 0.1),
                                                        specular=(0.7, 0.0005, 0.7), specular_power=(0.5, 0.2),
                                                        sggx_depth=1, sggx_sample_count=32)))
+    # an anisotropic sphere, so the texture-aware kernel instances run those models too
+    d.spheres.append(((0.05, 0.35, -0.3), 0.18, dict(model=BECKMANN_CONDUCTOR_ANISOTROPIC, diffuse=(0.1, 0.1, 0.1),
+                                                       specular=(0.8, 0.8, 0.8),
+                                                       specular_power=(alpha_to_specular_power(0.08),
+                                                                       alpha_to_specular_power(0.4)),
+                                                       fresnel=(0.3, 2.5))))
     ld = np.array([0.3, 0.8, 0.5]) / np.linalg.norm([0.3, 0.8, 0.5])
     d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(2.0, 2.0, 2.0)))
     d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=max_depth, rr=1.0,

@@ -116,7 +116,7 @@ def gonio_bench(cpu_paths=2.0e8, threads=16, cpu=True):
     return out
 
 
-def pseudo_volume_bench(width=1920, height=1080, repeats=5):
+def pseudo_volume_bench(width=1920, height=1080, repeats=5, cpu=True):
     """The reference's pseudo-volume microbenchmark (ComputeImage,
     tests/benchmark/pseudo-volume-performance-test.cu): one SGGX pseudo-volume
     BRDF evaluation per pixel of a 1080p image, SampleCount 256, Depth 1,
@@ -134,9 +134,32 @@ def pseudo_volume_bench(width=1920, height=1080, repeats=5):
         ctypes.byref(m.rec), width, height, repeats, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
         probe.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(ms)))
     evals = width * height
-    return {"workload": "SGGX pseudo-volume BRDF, %dx%d evaluations, SampleCount 256, Depth 1" % (width, height),
-            "ms_per_image": round(ms.value, 3), "mevals_per_s": round(evals / ms.value / 1e3, 2),
-            "walks_per_s": round(evals * 256 / ms.value * 1e3, 0), "brdf": [float(v) for v in probe[:3]]}
+    out = {"workload": "SGGX pseudo-volume BRDF, %dx%d evaluations, SampleCount 256, Depth 1" % (width, height),
+           "ms_per_image": round(ms.value, 3), "mevals_per_s": round(evals / ms.value / 1e3, 2),
+           "walks_per_s": round(evals * 256 / ms.value * 1e3, 0), "brdf": [float(v) for v in probe[:3]]}
+    if cpu:
+        # the oracle's FP32 evaluation of the same BRDF, one thread
+        import time
+        import oracle
+        lib = oracle.load()
+        pf = ctypes.POINTER(ctypes.c_float)
+        lib.orc_bsdf_eval.argtypes = [ctypes.POINTER(_abi.tr_material), pf, pf, pf, pf, pf, ctypes.c_float, pf]
+        om = oracle.material(25, dict(diffuse=(0.0, 0.0, 0.0), specular=(0.75, 0.75, 0.75), specular_power=(0.5, 0.5),
+                                      sggx_basis=(0.0, 0.0, 0.0, 1.0), sggx_depth=1, sggx_sample_count=256))
+        c = float(np.sqrt(0.5))
+        vec = [np.array(v, dtype=np.float32) for v in ((0, 0, 1), (1, 0, 0), (0, 1, 0), (0, -c, c), (0, c, c))]
+        res = np.zeros(4, dtype=np.float32)
+        ptrs = [v.ctypes.data_as(pf) for v in vec]
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 2.0:
+            for _ in range(200):
+                lib.orc_bsdf_eval(ctypes.byref(om.rec), *ptrs, ctypes.c_float(float(probe[3])), res.ctypes.data_as(pf))
+            n += 200
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dt / 1e6, 5), "unit": "Mevals/s", "cores": 1, "kind": "port",
+                               "sample": "%d evaluations of the same BRDF, one after another, %.1f s" % (n, dt)}
+    return out
 
 
 def volume_bench(cpu_rows=128, threads=16, cpu=True):
@@ -410,7 +433,7 @@ def main():
 
     pv = None
     if not args.no_pseudo_volume:
-        pv = pseudo_volume_bench()
+        pv = pseudo_volume_bench(cpu=not args.no_cpu)
         log("pseudo_volume_brdf: %s" % json.dumps(pv))
 
     vol = None

@@ -1748,5 +1748,19 @@ void orc_sggx(const float* stddev, const float* quat, const float* dir, float* o
     out[1] = sggx_projected_area(sd, basis, d);
 }
 
+/* Microfacet terms at FP32: kind 0 G1_GGX, 1 G1_Beckmann (the rational
+ * fit), 2 the GGX distribution, 3 the Beckmann distribution
+ * (ggx-models-impl.hh:126-151, beckmann-models-impl.hh:34-70).  For G1,
+ * a = cos(v, m) and b = cos(v, n); for the distributions a = cos(n, h). */
+float orc_microfacet_term(int kind, float g2, float a, float b)
+{
+    switch (kind) {
+    case 0: return ggx_g1(g2, a, b);
+    case 1: return beckmann_g1(g2, a, b);
+    case 2: return ggx_d(g2, a);
+    default: return beckmann_d(g2, a);
+    }
+}
+
 }  // extern "C"
 

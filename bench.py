@@ -35,6 +35,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Host cores of this job for the CPU baselines: the cores the process
+    may run on (sched_getaffinity), limited to the job's CPU share where the
+    environment states one (OMP_NUM_THREADS; 16 per GPU on the MI355X
+    boxes), one oracle thread per core."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
 def gpu_material(model, fields):
     from tempestrenderer_amd.scene import Material
     return Material(model, **fields).setup()
@@ -66,12 +78,13 @@ def cpu_baseline(desc, rows, threads):
     return samples / secs / 1e6, secs, samples, r0
 
 
-def gonio_bench(cpu_paths=2.0e8, threads=16, cpu=True):
+def gonio_bench(cpu_paths=2.0e8, threads=None, cpu=True):
     """The batched virtual gonioreflectometer: all 6561 requests in one
     tr_scene_render_requests call (graph replay, autotuned), timed after a
     small warm-up call; beside it the CPU oracle rendering the first
     requests one after another, as the reference's request loop does, until
     about `cpu_paths` paths."""
+    threads = threads or host_threads()
     import copy
     from tempestrenderer_amd import workloads
     from tempestrenderer_amd.scene import RayTracerScene
@@ -162,11 +175,12 @@ def pseudo_volume_bench(width=1920, height=1080, repeats=5, cpu=True):
     return out
 
 
-def volume_bench(cpu_rows=128, threads=16, cpu=True):
+def volume_bench(cpu_rows=128, threads=None, cpu=True):
     """The heterogeneous SGGX volume: 256^3 binary16 grid (mean extinction
     4), 1024x1024, 128 camera samples, depth 32, delta tracking.  One timed
     frame; a counting frame (8 camera samples) gives tentative collisions
     per path for the gather-byte numerator (8 corners x 8 B each)."""
+    threads = threads or host_threads()
     from tempestrenderer_amd import workloads
     from tempestrenderer_amd.scene import RayTracerScene
     d = workloads.volume()
@@ -438,17 +452,17 @@ def main():
 
     vol = None
     if not args.no_volume:
-        vol = volume_bench(cpu=not args.no_cpu, threads=min(16, os.cpu_count() or 1))
+        vol = volume_bench(cpu=not args.no_cpu, threads=host_threads())
         log("volume: %s" % json.dumps(vol))
 
     gonio = None
     if not args.no_gonio:
-        gonio = gonio_bench(cpu=not args.no_cpu, threads=min(16, os.cpu_count() or 1))
+        gonio = gonio_bench(cpu=not args.no_cpu, threads=host_threads())
         log("gonioreflectometer: %s" % json.dumps(gonio))
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         args.cpu_rows = min(args.cpu_rows, desc.height)
         cpu_value, cpu_secs, cpu_samples, r0 = cpu_baseline(desc, args.cpu_rows, threads)
         cpu = {"value": round(cpu_value, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",

@@ -84,6 +84,8 @@ def _declare(lib):
         "orc_resolve": (f, [sz, P(ctypes.c_double), ctypes.c_double, u32, f, P(u32)]),
         "orc_render_rows": (ctypes.c_double, [vp, u32, u32, u32, ctypes.c_int, P(ctypes.c_double),
                                               P(ctypes.c_double), ctypes.c_int]),
+        "orc_render_pixels": (ctypes.c_double, [vp, sz, P(u32), u32, ctypes.c_int, P(ctypes.c_double),
+                                                P(ctypes.c_double), ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -339,6 +341,18 @@ class OracleScene:
     def tracePath(self, pixel, camera_sample):
         """Prints every vertex of one camera sample to stderr."""
         self.lib.orc_trace_path(self.h, int(pixel), int(camera_sample))
+
+    def renderPixels(self, pixels, frame=0, fp64=False, threads=0):
+        """Per-pixel sums and sums of squares over camera samples for a pixel
+        list; returns (sum [N, 3], sumsq [N, 3], seconds)."""
+        px = np.ascontiguousarray(np.asarray(pixels, dtype=np.uint32))
+        s = np.zeros(px.size * 3, dtype=np.float64)
+        q = np.zeros(px.size * 3, dtype=np.float64)
+        secs = self.lib.orc_render_pixels(self.h, px.size, px.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                          int(frame), int(bool(fp64)),
+                                          s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), threads)
+        return s.reshape(-1, 3), q.reshape(-1, 3), secs
 
     def renderRows(self, row_begin, row_end, frame=0, fp64=False, threads=0):
         """Per-pixel sums and sums of squares over camera samples for a row

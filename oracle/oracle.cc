@@ -1404,6 +1404,35 @@ void orc_render_paths(const orc_scene* s, size_t pixel_count, const uint32_t* pi
     });
 }
 
+/* Per-pixel sums and sums of squares over the frame's camera samples of the
+ * listed pixels (the pixel-list twin of orc_render_rows). */
+double orc_render_pixels(const orc_scene* s, size_t pixel_count, const uint32_t* pixels, uint32_t frame,
+                         int use_fp64, double* out_sum, double* out_sumsq, int nthreads)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    parallel_for((uint32_t)pixel_count, nthreads, 4, [&](uint32_t i) {
+        double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < s->spp; ++k) {
+            uint32_t hash;
+            double c[3];
+            if (use_fp64) {
+                V3<double> v = render_camera_sample<double>(s, pixels[i], frame * s->spp + k, &hash, i);
+                c[0] = v.x; c[1] = v.y; c[2] = v.z;
+            } else {
+                V3<float> v = render_camera_sample<float>(s, pixels[i], frame * s->spp + k, &hash, i);
+                c[0] = v.x; c[1] = v.y; c[2] = v.z;
+            }
+            for (int a = 0; a < 3; ++a) { sum[a] += c[a]; sq[a] += c[a] * c[a]; }
+        }
+        for (int a = 0; a < 3; ++a) {
+            out_sum[(size_t)i * 3 + a] = sum[a];
+            if (out_sumsq) out_sumsq[(size_t)i * 3 + a] = sq[a];
+        }
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
 double orc_render_rows(const orc_scene* s, uint32_t row_begin, uint32_t row_end, uint32_t frame, int use_fp64,
                        double* out_sum, double* out_sumsq, int nthreads)
 {

@@ -250,11 +250,18 @@ SceneView scene_view(const tr_scene* s)
     return v;
 }
 
-uint32_t persistent_grid(const tr_scene* s)
+/* Persistent traversal grids: trace_waves_per_cu if set, else 16 waves per
+ * CU for the volume-tracking, shadow and ray-set kernels (4 resident per
+ * SIMD at their register counts) and 4 * kRefillWavesPerSimd for the
+ * refill kernel.  The spill area is sized for the larger. */
+uint32_t persistent_grid(const tr_scene* s, bool refill = false)
 {
-    const uint32_t waves = s->settings.trace_waves_per_cu ? s->settings.trace_waves_per_cu : 16u;
+    const uint32_t dflt = refill ? 4u * kRefillWavesPerSimd : 16u;
+    const uint32_t waves = s->settings.trace_waves_per_cu ? s->settings.trace_waves_per_cu : dflt;
     return (uint32_t)std::max(1, s->cu_count) * waves;
 }
+
+uint32_t spill_grid(const tr_scene* s) { return std::max(persistent_grid(s), persistent_grid(s, true)); }
 
 tr_status sync_lights(tr_scene* s)
 {
@@ -307,7 +314,7 @@ tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
     TR_HIP(s->sh_c.reserve((size_t)cap * msh));
     TR_HIP(s->counters.reserve(8));
     TR_HIP(s->gsum.reserve(3 * kCompactGroups));
-    TR_HIP(s->spill.reserve((size_t)persistent_grid(s) * kTraceBlock * kStackSpill));
+    TR_HIP(s->spill.reserve((size_t)spill_grid(s) * kTraceBlock * kStackSpill));
     TR_HIP(s->counts.reserve(8));
     s->capacity = cap;
     s->max_shadow = msh;
@@ -428,6 +435,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
     }
     uint32_t ppb = (uint32_t)std::max<uint64_t>(1, s->capacity / per_pixel);
     uint32_t grid = persistent_grid(s);
+    uint32_t grid_refill = persistent_grid(s, true);
     uint64_t* counts = nullptr;
     if (s->counting) {
         counts = s->counts.ptr;
@@ -482,7 +490,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         int cur = 0, nxt = 1;
         for (uint32_t v = 0; v <= s->max_depth; ++v) {
             if ((st2 = stage(EV_TRACE, [&](const LaunchEvents* ev) {
-                     launch_trace(s->stream, p, pool, out, slot_base, v, cur, nxt, grid, counts, ev);
+                     launch_trace(s->stream, p, pool, out, slot_base, v, cur, nxt, grid, grid_refill, counts, ev);
                  })) != TR_OK)
                 return st2;
             if ((st2 = stage(EV_SHADE, [&](const LaunchEvents* ev) {
@@ -1599,7 +1607,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
     std::vector<std::pair<uint32_t, uint32_t>> cands;
     if (o.autotune) {
         for (uint32_t pool : {1u << 20, 1u << 21, 1u << 22})
-            for (uint32_t w : {8u, 16u, 32u}) cands.push_back({pool, w});
+            for (uint32_t w : {8u, 16u, 24u}) cands.push_back({pool, w});
     } else {
         cands.push_back({o.pool_slots, o.trace_waves_per_cu});
     }
@@ -1680,7 +1688,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
         launch_raygen(stream, p, pool, out, 0, slots, rec);
         int cur = 0, nxt = 1;
         for (uint32_t v = 0; v <= s->max_depth; ++v) {
-            launch_trace(stream, p, pool, out, 0, v, cur, nxt, grid, nullptr);
+            launch_trace(stream, p, pool, out, 0, v, cur, nxt, grid, grid, nullptr);
             launch_shade(stream, p, pool, out, 0, v, cur, nxt, slots, rec);
             launch_compact(stream, pool, slots, nxt, false);
             if (p.sc.n_lights > 0) launch_shadow(stream, p, pool, out, 0, v, grid, nullptr);

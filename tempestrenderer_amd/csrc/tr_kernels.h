@@ -140,13 +140,20 @@ struct LaunchEvents {
 constexpr int kTraceBlock = 64;       /* one wave per block */
 constexpr uint32_t kSegment = 256;    /* shading block = queue segment */
 constexpr uint32_t kCompactGroups = 256; /* k_compact blocks per queue, at most */
-constexpr int kStackLds = 16;         /* stack entries per lane kept in LDS */
+constexpr int kStackLds = 10;         /* stack entries per lane kept in LDS (5 KB per wave) */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
+/* Waves per SIMD the refill traversal is compiled for (its register budget,
+ * 80 VGPRs); its default persistent grid is 4 * kRefillWavesPerSimd waves
+ * per CU, the most that stay resident. */
+constexpr uint32_t kRefillWavesPerSimd = 6;
 
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                    uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev = nullptr);
+/* grid: persistent grid of the volume-tracking kernel; grid_refill: of the
+ * refill kernel (scenes without volumes). */
 void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                  uint32_t vertex, int cur, int nxt, uint32_t grid, uint64_t* counts, const LaunchEvents* ev = nullptr);
+                  uint32_t vertex, int cur, int nxt, uint32_t grid, uint32_t grid_refill, uint64_t* counts,
+                  const LaunchEvents* ev = nullptr);
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                   uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec,
                   const LaunchEvents* ev = nullptr);

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
 constexpr uint32_t kRefill = 16;
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_refill(FrameParams p, PathPool pool, int cur,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRefillWavesPerSimd))) void k_trace_refill(FrameParams p, PathPool pool, int cur,
                                                               unsigned long long* counts)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
@@ -1266,16 +1266,17 @@ void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, co
 }
 
 void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
-                  uint32_t vertex, int cur, int nxt, uint32_t grid, uint64_t* counts, const LaunchEvents* ev)
+                  uint32_t vertex, int cur, int nxt, uint32_t grid, uint32_t grid_refill, uint64_t* counts,
+                  const LaunchEvents* ev)
 {
     unsigned long long* c = (unsigned long long*)counts;
     bool vol = p.sc.n_volumes > 0;
     if (counts) {
         if (vol) launch_k(k_trace<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else launch_k(k_trace_refill<true>, grid, kTraceBlock, s, ev, p, pool, cur, c);
+        else launch_k(k_trace_refill<true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     } else {
         if (vol) launch_k(k_trace<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else launch_k(k_trace_refill<false>, grid, kTraceBlock, s, ev, p, pool, cur, c);
+        else launch_k(k_trace_refill<false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     }
 }
 

@@ -187,6 +187,22 @@ def box(width=256, height=256, samples_camera=16, max_depth=4, seed=1):
     return d
 
 
+def sphere_light_box(width=48, height=48, samples_camera=4, max_depth=4, enclosing=False):
+    """The box scene with a sphere-area light (addSphereLightSource): a small
+    sphere under the ceiling beside the mesh emitter, or (enclosing) a sphere
+    of radius 5 around the whole box and the camera, so every shading point
+    lies inside it and samples it uniformly over its area."""
+    d = box(width, height, samples_camera=samples_camera, max_depth=max_depth)
+    d.name = "sphere_light_box"
+    if enclosing:
+        d.sphere_lights.append(dict(kind=LIGHT_SPHERE_AREA, radiance=(0.6, 0.5, 0.4), center=(0.0, 0.0, 0.0),
+                                    radius=5.0))
+    else:
+        d.sphere_lights.append(dict(kind=LIGHT_SPHERE_AREA, radiance=(6.0, 5.0, 4.0), center=(0.35, 0.55, 0.3),
+                                    radius=0.15))
+    return d
+
+
 def value_noise_fbm(n, octaves=6, amplitude=0.1, seed=1):
     """fBm of bilinear value noise on an n x n grid over [-1, 1]^2
     (lacunarity 2, gain 0.5), heights in roughly [-amplitude, amplitude]."""
@@ -542,6 +558,30 @@ def random_rays(desc, n, seed=7, kind="sphere"):
     rays = np.zeros((n, 8), dtype=np.float32)
     rays[:, 0:3] = org
     rays[:, 3] = 0.0
+    rays[:, 4:7] = v
+    rays[:, 7] = np.inf
+    return rays
+
+
+def surface_rays(desc, n, seed=11, offset=1e-3):
+    """Secondary rays of a fixed ray set: origins at uniformly chosen points
+    of uniformly chosen triangles of the description's first mesh, lifted by
+    `offset` along the triangle's front normal, directions uniform on the
+    sphere (float64, stored as float32)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    verts, tris = desc.meshes[0][0], desc.meshes[0][1].reshape(-1, 3)
+    pos = verts[:, :3].astype(np.float64)
+    t = rng.integers(0, tris.shape[0], size=n)
+    r1, r2 = rng.random(n), rng.random(n)
+    s = np.sqrt(r1)
+    p0, p1, p2 = pos[tris[t, 0]], pos[tris[t, 1]], pos[tris[t, 2]]
+    pt = p0 * (1.0 - s)[:, None] + p1 * (s * (1.0 - r2))[:, None] + p2 * (s * r2)[:, None]
+    nrm = np.cross(p1 - p0, p2 - p0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), dtype=np.float32)
+    rays[:, 0:3] = pt + offset * nrm
     rays[:, 4:7] = v
     rays[:, 7] = np.inf
     return rays

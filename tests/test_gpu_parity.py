@@ -72,6 +72,29 @@ def test_paths_heightfield():
     _check_paths(g_rgb, o_rgb, g_hash, o_hash)
 
 
+@pytest.mark.parametrize("enclosing", [False, True])
+@pytest.mark.parametrize("estimator,samples_light", [(0, 1), (0, 2), (1, 1), (1, 2)])
+def test_paths_sphere_light(enclosing, estimator, samples_light):
+    """A sphere-area light (addSphereLightSource): the cone sampler
+    SampleSphereArea and its PDF from outside, the uniform area sampler from
+    inside (enclosing sphere), the reference estimator's balance-weighted
+    area-light NEE (ray-tracing.cc:1852-1860) and the unbiased estimator's
+    multi-sample weights, with 1 and 2 samples per area light.  Inside the
+    enclosing sphere, shadow rays that leave through the back of a one-sided
+    wall reach the light with a negative cosine; the reference's BRDFs have
+    no cosine guard (ggx-models-impl.hh:168-179), so such samples subtract,
+    on both sides alike."""
+    print("case: paths sphere light enclosing=%d estimator=%d samples_light=%d" % (enclosing, estimator,
+                                                                                   samples_light), flush=True)
+    desc = workloads.sphere_light_box(40, 40, samples_camera=4, max_depth=4, enclosing=enclosing)
+    desc.settings.update(estimator=estimator, samples_light=samples_light)
+    g, o = _pair(desc)
+    px = np.arange(desc.pixels, dtype=np.uint32)
+    g_rgb, g_hash = g.renderPaths(px, 4)
+    o_rgb, o_hash = o.renderPaths(px)
+    _check_paths(g_rgb, o_rgb, g_hash, o_hash)
+
+
 def test_draw_once_accumulates_paths():
     print("case: drawOnce box", flush=True)
     desc = workloads.box(48, 48, samples_camera=4, max_depth=3)

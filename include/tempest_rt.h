@@ -475,6 +475,18 @@ tr_status tr_scene_set_frame_index(tr_scene* scene, uint32_t frame);
  * (ray-tracing.cc:2056; ray-tracing.cu:821-846) answer it. */
 tr_status tr_scene_trace_rays(tr_scene* scene, size_t ray_count, const tr_ray* rays,
                               tr_hit* hits);
+/* tr_scene_trace_rays without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Rays and hits are DEVICE
+ * pointers the caller owns; the traversal's spill stack lives in a 16 MiB
+ * file-scope device array that each GPU holds until the process ends, so
+ * the BVH may be at most 24 levels deep (TR_ERROR_UNSUPPORTED otherwise)
+ * and the persistent grid at most 2048 waves.  `stream` (a hipStream_t)
+ * orders the work and the call returns without waiting; NULL uses the
+ * scene's stream and waits.  Calls that overlap on one GPU share that
+ * array and race. */
+tr_status tr_scene_trace_rays_noalloc(tr_scene* scene, size_t ray_count, const tr_ray* device_rays,
+                                      tr_hit* device_hits, void* stream);
 /* Renders every camera sample of the listed pixels and returns, per
  * camera sample, its radiance (3 floats) and a hash of its hit sequence. */
 tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint32_t* pixels,

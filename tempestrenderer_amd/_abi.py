@@ -235,6 +235,7 @@ SIGNATURES = {
     "tr_scene_set_estimator": (_st, [_vp, _u32]),
     "tr_scene_set_frame_index": (_st, [_vp, _u32]),
     "tr_scene_trace_rays": (_st, [_vp, _sz, P(tr_ray), P(tr_hit)]),
+    "tr_scene_trace_rays_noalloc": (_st, [_vp, _sz, _vp, _vp, _vp]),
     "tr_scene_render_paths": (_st, [_vp, _sz, P(_u32), P(ctypes.c_float), P(_u32)]),
     "tr_scene_get_stats": (_st, [_vp, P(tr_stats)]),
     "tr_scene_render_requests": (_st, [_vp, _sz, P(tr_request), P(tr_request_options), P(ctypes.c_float),

@@ -1468,6 +1468,25 @@ tr_status tr_scene_trace_rays(tr_scene* s, size_t n, const tr_ray* rays, tr_hit*
     return st;
 }
 
+tr_status tr_scene_trace_rays_noalloc(tr_scene* s, size_t n, const tr_ray* device_rays, tr_hit* device_hits,
+                                      void* stream)
+{
+    if (!valid(s) || (n && (!device_rays || !device_hits))) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed || !s->device_ready) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    if (n == 0) return TR_OK;
+    if (n > 0xffffffffull - 64) return fail(TR_ERROR_INVALID_ARG, "too many rays");
+    if (s->bvh_stats.depth + 2 > (uint32_t)kStackLds + kNoAllocSpill)
+        return fail(TR_ERROR_UNSUPPORTED, "the allocation-free trace holds BVHs of at most " +
+                                              std::to_string(kStackLds + kNoAllocSpill - 2) + " levels; this one has " +
+                                              std::to_string(s->bvh_stats.depth));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    if (launch_trace_rays_noalloc(st, scene_view(s), device_rays, device_hits, (uint32_t)n, persistent_grid(s)) !=
+        hipSuccess)
+        return fail(TR_ERROR_DEVICE, "trace launch failed");
+    if (!stream && hipStreamSynchronize(s->stream) != hipSuccess) return fail(TR_ERROR_DEVICE, "trace failed");
+    return TR_OK;
+}
+
 tr_status tr_scene_render_paths(tr_scene* s, size_t pixel_count, const uint32_t* pixels, float* out_rgb,
                                 uint32_t* out_hash)
 {

@@ -184,6 +184,14 @@ void launch_accumulate_requests(hipStream_t s, const FrameParams& p, const PathP
  * in order). */
 void launch_request_outputs(hipStream_t s, const ReqView& rv, uint32_t n_gi, const double* accum, float* mean,
                             float* image, uint64_t n_pix_total);
+/* The allocation-free ray-set trace: at most kNoAllocWaves persistent
+ * waves, kNoAllocSpill spill entries per lane in a file-scope device array
+ * (its per-device address is looked up once per device); BVHs up to
+ * kStackLds + kNoAllocSpill - 2 levels deep. */
+constexpr uint32_t kNoAllocWaves = 2048;
+constexpr uint32_t kNoAllocSpill = 16;
+hipError_t launch_trace_rays_noalloc(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
+                                     uint32_t grid);
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
                        uint2* spill, uint32_t grid);
 /* The backbuffer of the frame's mean radiance through the picture

@@ -30,6 +30,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <mutex>
+
 namespace trk {
 
 namespace {
@@ -1080,15 +1082,17 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
 
 /* ---------------- fixed ray sets ---------------- */
 
+/* spill_stride: spill entries per lane (the host checks that the BVH depth
+ * fits kStackLds + spill_stride) */
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
-                                                            uint2* spill)
+                                                            uint2* spill, uint32_t spill_stride)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
     uint32_t lane = threadIdx.x;
     uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     DevStack st;
     st.lds = &stk[0][lane];
-    st.spill = spill + (size_t)gtid * kStackSpill;
+    st.spill = spill + (size_t)gtid * spill_stride;
     const uint32_t stride = gridDim.x * kTraceBlock;
     for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += stride) {
         uint32_t i = base + lane;
@@ -1379,7 +1383,51 @@ void launch_request_outputs(hipStream_t s, const ReqView& rv, uint32_t n_gi, con
 void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n, uint2* spill,
                        uint32_t grid)
 {
-    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill);
+    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill,
+                       (uint32_t)kStackSpill);
+}
+
+/* Scratch of the allocation-free ray-set trace: the traversal spill stack
+ * of a persistent grid of at most kNoAllocWaves one-wave blocks,
+ * kNoAllocSpill entries per lane (16 MiB).  Each device holds its own copy
+ * until the process ends; host code uses the per-device address from
+ * hipGetSymbolAddress, never the array's name. */
+__device__ __align__(256) uint2 scratch[(size_t)kNoAllocWaves * kTraceBlock * kNoAllocSpill];
+
+namespace {
+constexpr int kMaxDevices = 64;
+uint2* g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
+}  // namespace
+
+hipError_t noalloc_scratch(uint2** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<uint2*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_trace_rays_noalloc(hipStream_t s, const SceneView& sc, const tr_ray* rays, tr_hit* hits, uint32_t n,
+                                     uint32_t grid)
+{
+    uint2* spill = nullptr;
+    hipError_t e = noalloc_scratch(&spill);
+    if (e != hipSuccess) return e;
+    if (grid > kNoAllocWaves) grid = kNoAllocWaves;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, s, sc, rays, hits, n, spill,
+                       (uint32_t)kNoAllocSpill);
+    return hipGetLastError();
 }
 
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba, uint32_t n, uint32_t post,

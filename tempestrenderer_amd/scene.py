@@ -402,6 +402,14 @@ class RayTracerScene:
                                            hits.ctypes.data_as(ctypes.POINTER(_abi.tr_hit))))
         return hits
 
+    def traceRaysNoAlloc(self, device_rays, device_hits, count, stream=None):
+        """tr_scene_trace_rays_noalloc: rays and hits are device addresses
+        (tr_ray / tr_hit records) the caller allocated; `stream` a
+        hipStream_t handle (None: the scene's stream, and the call waits)."""
+        check(self.lib.tr_scene_trace_rays_noalloc(self.h, int(count), ctypes.c_void_p(int(device_rays)),
+                                                   ctypes.c_void_p(int(device_hits)),
+                                                   ctypes.c_void_p(int(stream) if stream else 0)))
+
     def traceRaysHost(self, rays):
         """The same closest-hit query through the host copy of the BVH8."""
         r = np.ascontiguousarray(np.asarray(rays, dtype=np.float32).reshape(-1, 8))

@@ -285,7 +285,16 @@ tr_status tr_scene_bind_texture(tr_scene* scene, const tr_texture_desc* desc, co
  * removes it. */
 tr_status tr_scene_set_global_cube_map(tr_scene* scene, uint32_t side, const uint32_t* const faces[6]);
 tr_status tr_scene_set_picture_post_process(tr_scene* scene, uint32_t mode);
+/* setTransparentBackground (ray-tracing.hh:319; ray-tracing.cc:2364-2365):
+ * a pixel none of whose camera rays of a frame hit geometry keeps its
+ * backbuffer value (0 before any write) and stays out of the
+ * auto-exposure log sum; accumulation is unchanged. */
 tr_status tr_scene_set_transparent_background(tr_scene* scene, int32_t enabled);
+/* RenderMode (ray-tracing-common.hh:132-139; ray-tracing.cc:2078-2089):
+ * 0 Normal; 1 DebugTangents, 2 DebugBinormals, 3 DebugNormals (the frame
+ * vector * 0.5 + 0.5 of the first hit), 4 DebugLighting (max(0, N . -dir))
+ * at the camera vertex, which ends the path.  Debug modes reject volumes,
+ * the radiance cache and render requests. */
 tr_status tr_scene_set_render_mode(tr_scene* scene, uint32_t mode);
 /* repaint (ray-tracing.hh:323): restart progressive accumulation. */
 tr_status tr_scene_repaint(tr_scene* scene);
@@ -309,6 +318,41 @@ tr_status tr_scene_add_triangle_mesh(tr_scene* scene, const float world[16],
 /* addSphere (ray-tracing.cc:1341-1355). */
 tr_status tr_scene_add_sphere(tr_scene* scene, const float center[3], float radius,
                               const tr_material* material, uint64_t* out_geom_id);
+/* User shapes.  Each returns its geometry slot.  Hits are limited to the
+ * ray's (tnear, tfar), which the reference's Embree callbacks do not check.
+ *
+ * addRect (ray-tracing.cc:1402-1428; IntersectRect3, intersect.hh:105-131):
+ * a one-sided rectangle at pos with half sizes size[0] along tan and
+ * size[1] along norm x tan, facing norm.  tc_box = {min u, min v, max u,
+ * max v} maps the hit's (u, v) to TexCoord (NULL = the unit square);
+ * tangent_map is a handle from tr_scene_bind_texture (0 = none) whose
+ * sampleRGB is the tangent-space direction of the shading frame. */
+tr_status tr_scene_add_rect(tr_scene* scene, const float pos[3], const float tan[3], const float norm[3],
+                            const float size[2], const tr_material* material, const float tc_box[4],
+                            uint32_t tangent_map, uint64_t* out_geom_id);
+/* addDisk (ray-tracing.cc:1430-1446; IntersectDisk3, intersect.hh:135-161):
+ * a one-sided annulus facing norm. */
+tr_status tr_scene_add_disk(tr_scene* scene, const float pos[3], const float norm[3], float inner_radius,
+                            float outer_radius, const tr_material* material, uint64_t* out_geom_id);
+/* addCylinder (ray-tracing.cc:1356-1369; intersection :892-960): a capped
+ * cylinder along z. */
+tr_status tr_scene_add_cylinder(tr_scene* scene, const float center[3], float radius, float half_height,
+                                const tr_material* material, uint64_t* out_geom_id);
+/* addObliqueCylinder (ray-tracing.cc:1371-1384; intersection :992-1046): a
+ * capped cylinder whose axis leans by tilt * tilt_direction over its half
+ * height.  Shaded with the frame of its hit normal (the reference leaves
+ * its frame zero, ray-tracing.cc:1210-1237). */
+tr_status tr_scene_add_oblique_cylinder(tr_scene* scene, const float center[3], float radius, float half_height,
+                                        const float tilt_direction[2], float tilt, const tr_material* material,
+                                        uint64_t* out_geom_id);
+/* addEllipsoid (ray-tracing.cc:1315-1339; IntersectEllipsoid,
+ * intersect.hh:192-221): orientation is a quaternion (x, y, z, w). */
+tr_status tr_scene_add_ellipsoid(tr_scene* scene, const float center[3], const float scale[3],
+                                 const float orientation[4], const tr_material* material, uint64_t* out_geom_id);
+/* addBlocker (ray-tracing.cc:1386-1400): a rect without material; paths
+ * and shadow rays that hit it end there. */
+tr_status tr_scene_add_blocker(tr_scene* scene, const float pos[3], const float tan[3], const float norm[3],
+                               const float size[2], uint64_t* out_geom_id);
 
 /* A heterogeneous participating medium sampled on a regular grid.  It
  * replaces addHierarchicalVolume (ray-tracing.hh:381, ray-tracing.cc:
@@ -487,6 +531,25 @@ tr_status tr_scene_trace_rays(tr_scene* scene, size_t ray_count, const tr_ray* r
  * array and race. */
 tr_status tr_scene_trace_rays_noalloc(tr_scene* scene, size_t ray_count, const tr_ray* device_rays,
                                       tr_hit* device_hits, void* stream);
+/* SampleData as rayQuery fills it (ray-tracing-common.hh:184-198, the
+ * fields sample() writes): the material pointer the geometry was added
+ * with (NULL for blockers and sphere lights, whose material is internal),
+ * the shading frame, TexCoord, and the hit distance along the query ray. */
+typedef struct tr_sample_data {
+    const tr_material* material;
+    float normal[3];
+    float tangent[3];
+    float binormal[3];
+    float texcoord[2];
+    float t;
+} tr_sample_data;
+
+/* rayQuery (ray-tracing.hh:372, ray-tracing.cc:1932-1979): the closest hit
+ * of the camera ray through texture coordinate tc of the frame (tc.y
+ * downwards).  *out_geom_id is the id the geometry's add call returned, or
+ * TR_INVALID_GEOMETRY on a miss (sample_data is then left untouched). */
+tr_status tr_scene_ray_query(tr_scene* scene, const float tc[2], tr_sample_data* sample_data,
+                             uint64_t* out_geom_id);
 /* Renders every camera sample of the listed pixels and returns, per
  * camera sample, its radiance (3 floats) and a hash of its hit sequence. */
 tr_status tr_scene_render_paths(tr_scene* scene, size_t pixel_count, const uint32_t* pixels,

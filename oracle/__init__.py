@@ -64,6 +64,12 @@ def _declare(lib):
         "orc_add_triangle_mesh": (ctypes.c_int, [vp, P(f), sz, P(_abi.tr_submesh), sz, P(ctypes.c_int32), sz, vp,
                                                  P(_abi.tr_mesh_options), P(u64)]),
         "orc_add_sphere": (ctypes.c_int, [vp, P(f), f, P(_abi.tr_material), P(u64)]),
+        "orc_add_rect": (ctypes.c_int, [vp, P(f), P(f), P(f), P(f), P(_abi.tr_material), P(f), u32, P(u64)]),
+        "orc_add_blocker": (ctypes.c_int, [vp, P(f), P(f), P(f), P(f), P(u64)]),
+        "orc_add_disk": (ctypes.c_int, [vp, P(f), P(f), f, f, P(_abi.tr_material), P(u64)]),
+        "orc_add_cylinder": (ctypes.c_int, [vp, P(f), f, f, P(_abi.tr_material), P(u64)]),
+        "orc_add_oblique_cylinder": (ctypes.c_int, [vp, P(f), f, f, P(f), f, P(_abi.tr_material), P(u64)]),
+        "orc_add_ellipsoid": (ctypes.c_int, [vp, P(f), P(f), P(f), P(_abi.tr_material), P(u64)]),
         "orc_add_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
         "orc_add_sphere_light_source": (ctypes.c_int, [vp, P(_abi.tr_light), P(u32)]),
         "orc_add_grid_volume": (ctypes.c_int, [vp, P(_abi.tr_grid_volume), P(_abi.tr_material), P(u64)]),
@@ -82,6 +88,10 @@ def _declare(lib):
         "orc_trace_path": (None, [vp, u32, u32]),
         "orc_render_paths": (None, [vp, sz, P(u32), u32, P(f), P(u32), ctypes.c_int]),
         "orc_resolve": (f, [sz, P(ctypes.c_double), ctypes.c_double, u32, f, P(u32)]),
+        "orc_resolve_masked": (f, [sz, P(ctypes.c_double), ctypes.c_double, u32, f, P(ctypes.c_uint8), P(u32)]),
+        "orc_set_render_mode": (None, [vp, u32]),
+        "orc_camera_hits": (None, [vp, u32, P(ctypes.c_uint8)]),
+        "orc_ray_query": (u32, [vp, P(f), P(f)]),
         "orc_render_rows": (ctypes.c_double, [vp, u32, u32, u32, ctypes.c_int, P(ctypes.c_double),
                                               P(ctypes.c_double), ctypes.c_int]),
         "orc_render_pixels": (ctypes.c_double, [vp, sz, P(u32), u32, ctypes.c_int, P(ctypes.c_double),
@@ -138,15 +148,21 @@ def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
-def resolve(accum, rcp_count, post, exposure=0.0):
+def resolve(accum, rcp_count, post, exposure=0.0, mask=None, initial=None):
     """Backbuffer of an FP64 radiance sum [n, 3] through a picture
     post-process (ray-tracing.cc:2366-2375, 1638-1696): (uint32 RGBA [n],
-    the computed auto exposure).  exposure > 0 replaces the computed one."""
+    the computed auto exposure).  exposure > 0 replaces the computed one.
+    With `mask` (TransparentBackground), pixels whose mask is 0 keep their
+    `initial` value (default 0) and stay out of the exposure."""
     lib = load()
     acc = np.ascontiguousarray(accum, dtype=np.float64).reshape(-1, 3)
-    out = np.zeros(acc.shape[0], dtype=np.uint32)
-    e = lib.orc_resolve(acc.shape[0], acc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(rcp_count),
-                        int(post), float(exposure), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    out = np.zeros(acc.shape[0], dtype=np.uint32) if initial is None else np.array(initial, dtype=np.uint32)
+    mp = None
+    if mask is not None:
+        mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        mp = mk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    e = lib.orc_resolve_masked(acc.shape[0], acc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(rcp_count),
+                               int(post), float(exposure), mp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
     return out, float(e)
 
 
@@ -188,6 +204,20 @@ class OracleScene:
     def setSeed(self, s): self.lib.orc_set_seed(self.h, int(s))
     def setPixelKeyBase(self, b): self.lib.orc_set_pixel_key_base(self.h, int(b))
     def setEstimator(self, m): self.lib.orc_set_estimator(self.h, int(m))
+    def setRenderMode(self, m): self.lib.orc_set_render_mode(self.h, int(m))
+
+    def cameraHits(self, frame=0):
+        """Per pixel: whether a camera ray of the frame hit geometry."""
+        out = np.zeros(self.width * self.height, dtype=np.uint8)
+        self.lib.orc_camera_hits(self.h, int(frame), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return out
+
+    def rayQuery(self, tc):
+        """(geometry slot or 0xFFFFFFFF, [t, normal 3, tangent 3, binormal 3, TexCoord 2])."""
+        t = np.asarray(tc, dtype=np.float32)
+        out = np.zeros(12, dtype=np.float32)
+        g = self.lib.orc_ray_query(self.h, _fp(t), _fp(out))
+        return int(g), out
 
     def setBackgroundSpectrum(self, rgb):
         a = np.asarray(rgb, dtype=np.float32)
@@ -221,6 +251,54 @@ class OracleScene:
         idx = ctypes.c_uint32()
         self.lib.orc_add_light_source(self.h, ctypes.byref(light), ctypes.byref(idx))
         return idx.value
+
+    # user shapes (ray-tracing.cc:1315-1446)
+    def _shape_call(self, fn, material, *args):
+        gid = ctypes.c_uint64()
+        if material is not None:
+            self._keep.append(material)
+        self._shape_status(fn(self.h, *args, ctypes.byref(gid)), fn.__name__)
+        return gid.value
+
+    def addRect(self, pos, tan, norm, size, material, tc=None, tangent_map=0):
+        f = [np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (pos, tan, norm, size)]
+        self._keep.extend(f)
+        tcp = None
+        if tc is not None:
+            tcv = np.ascontiguousarray(np.asarray(tc, dtype=np.float32))
+            self._keep.append(tcv)
+            tcp = _fp(tcv)
+        return self._shape_call(self.lib.orc_add_rect, material, *[_fp(v) for v in f], ctypes.byref(material.rec), tcp,
+                                int(tangent_map))
+
+    def addBlocker(self, pos, tan, norm, size):
+        f = [np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (pos, tan, norm, size)]
+        return self._shape_call(self.lib.orc_add_blocker, None, *[_fp(v) for v in f])
+
+    def addDisk(self, pos, norm, inner_radius, outer_radius, material):
+        p, n = (np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (pos, norm))
+        return self._shape_call(self.lib.orc_add_disk, material, _fp(p), _fp(n), float(inner_radius),
+                                float(outer_radius), ctypes.byref(material.rec))
+
+    def addCylinder(self, center, radius, half_height, material):
+        c = np.ascontiguousarray(np.asarray(center, dtype=np.float32))
+        return self._shape_call(self.lib.orc_add_cylinder, material, _fp(c), float(radius), float(half_height),
+                                ctypes.byref(material.rec))
+
+    def addObliqueCylinder(self, center, radius, half_height, tilt_direction, tilt, material):
+        c, td = (np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (center, tilt_direction))
+        return self._shape_call(self.lib.orc_add_oblique_cylinder, material, _fp(c), float(radius), float(half_height),
+                                _fp(td), float(tilt), ctypes.byref(material.rec))
+
+    def addEllipsoid(self, center, scale, orientation, material):
+        c, sc, q = (np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (center, scale, orientation))
+        return self._shape_call(self.lib.orc_add_ellipsoid, material, _fp(c), _fp(sc), _fp(q),
+                                ctypes.byref(material.rec))
+
+    @staticmethod
+    def _shape_status(st, name):
+        if st:
+            raise RuntimeError("oracle %s failed" % name)
 
     def addSphereLightSource(self, light):
         gid = ctypes.c_uint32()

@@ -50,7 +50,10 @@ namespace {
 thread_local FILE* g_trace = nullptr;
 #define ORC_TRACE(...) do { if (g_trace) fprintf(g_trace, __VA_ARGS__); } while (0)
 
-enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2, GEOM_VOLUME = 3 };
+enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2, GEOM_VOLUME = 3, GEOM_SHAPE = 4 };
+/* the material of a blocker (addBlocker, ray-tracing.cc:1386-1400 and
+ * BlockerGeometry::sample :1239-1244): none */
+constexpr uint32_t ORC_NO_MATERIAL = 0xffffffffu;
 enum { STREAM_CAMERA = 0, STREAM_BSDF = 1, STREAM_LIGHT = 2, STREAM_RR = 3 };
 
 struct Geom {
@@ -65,12 +68,14 @@ struct Geom {
     float sphere[4];
     uint32_t light;         /* sphere light: light index */
     uint32_t volume;        /* grid volume: index into orc_scene::volumes */
+    uint32_t shape;         /* user shape: index into orc_scene::shapes */
 };
 
 struct Prim {
-    float v0[3], v1[3], v2[3]; /* triangle; spheres: v0 = centre, v1[0] = radius; volumes: box v0..v1 */
+    float v0[3], v1[3], v2[3]; /* triangle; spheres: v0 = centre, v1[0] = radius; volumes: box v0..v1;
+                                  shapes: box v1..v2 */
     uint32_t geom, prim;
-    uint32_t is_sphere, two_sided, is_volume;
+    uint32_t is_sphere, two_sided, is_volume, is_shape;
 };
 
 struct Light {
@@ -95,6 +100,7 @@ struct orc_scene {
     float background[3] = {0, 0, 0};
     uint32_t key[2] = {1, 0};
     uint32_t estimator = TR_ESTIMATOR_REFERENCE;
+    uint32_t render_mode = 0;  /* RenderMode (ray-tracing-common.hh:132-139) */
     /* added to the pixel index in the random-number key: a request of a
      * batched render draws the numbers of its pixels' position over all
      * requests */
@@ -121,6 +127,7 @@ struct orc_scene {
     std::vector<BNode> nodes;
     std::vector<Light> lights;
     std::vector<GridVolume> volumes;
+    std::vector<OrcShape> shapes;
     OrcTexSet texset;          /* bound textures and the global cube map */
     uint32_t instances = 0;
     bool committed = false;
@@ -148,6 +155,8 @@ void prim_bounds(const Prim& p, float lo[3], float hi[3])
 {
     if (p.is_sphere) {
         for (int a = 0; a < 3; ++a) { lo[a] = p.v0[a] - p.v1[0]; hi[a] = p.v0[a] + p.v1[0]; }
+    } else if (p.is_shape) {
+        for (int a = 0; a < 3; ++a) { lo[a] = p.v1[a]; hi[a] = p.v2[a]; }
     } else if (p.is_volume) {
         for (int a = 0; a < 3; ++a) { lo[a] = p.v0[a]; hi[a] = p.v1[a]; }
     } else {
@@ -253,7 +262,10 @@ inline void test_prim(const orc_scene* s, uint32_t pi, const WatertightRay& wr, 
     float t, u = 0.0f, v = 0.0f;
     int hit;
     if (p.is_sphere) hit = sphere_hit(org, dir, p.v0, p.v1[0], tnear, tfar, &t);
-    else if (p.is_volume) hit = volume_box_hit(org, dir, p.v0, p.v1, tnear, tfar, &t, &u);
+    else if (p.is_shape) {
+        float n[3];
+        hit = orc_shape_hit(s->shapes[s->geoms[p.geom].shape], org, dir, &t, &u, &v, n) && t > tnear && t < tfar;
+    } else if (p.is_volume) hit = volume_box_hit(org, dir, p.v0, p.v1, tnear, tfar, &t, &u);
     else hit = watertight_triangle(wr, org, p.v0, p.v1, p.v2, (int)p.two_sided, tnear, tfar, &t, &u, &v);
     if (!hit) return;
     if (best->geom != TR_INVALID_GEOMETRY && !hit_better(t, p.geom, p.prim, best->t, best->geom, best->prim)) return;
@@ -360,6 +372,75 @@ tr_hit trace_segment(const orc_scene* s, const float org[3], const float dir[3],
 
 /* ---------------- geometry sampling ---------------- */
 
+/* The sample() of the user shapes (computed in FP32, as the hit):
+ * RectGeometry::sample (ray-tracing.cc:1246-1272), DiskGeometry::sample
+ * (:1274-1283), CylinderGeometry::sample (:1142-1164, also for the oblique
+ * cylinder, whose own sample() is commented out), EllipsoidGeometry::sample
+ * (:1182-1208).  The normal is the intersection's, recomputed from the same
+ * ray. */
+template <class T>
+void shape_sample(const orc_scene* s, const OrcShape& sh, const tr_hit& h, V3<T> org, V3<T> dir, Sample<T>* sd)
+{
+    float of[3] = {(float)org.x, (float)org.y, (float)org.z};
+    float df[3] = {(float)dir.x, (float)dir.y, (float)dir.z};
+    float t, u, v, n[3] = {0.0f, 0.0f, 1.0f};
+    orc_shape_hit(sh, of, df, &t, &u, &v, n);
+    const float* f = sh.f;
+    float N[3] = {n[0], n[1], n[2]}, Tn[3], B[3];
+    float tcx = 0.0f, tcy = 0.0f;
+    if (sh.kind == ORC_RECT) {
+        tcx = f[14] + h.u * f[16];
+        tcy = f[15] + h.v * f[17];
+        M3<float> o;
+        o.c0 = v3(f[3], f[4], f[5]);
+        o.c1 = v3(f[6], f[7], f[8]);
+        o.c2 = v3(f[9], f[10], f[11]);
+        if (sh.tangent_map) {
+            Rgba rgb = orc_tex_sample(s->texset, sh.tangent_map, tcx, tcy, ORC_TEX_RGB);
+            M3<float> rel = make_basis_tangent(normalize(v3(rgb.v[0], rgb.v[1], rgb.v[2])));
+            o = mul(o, rel);
+        }
+        Tn[0] = o.c0.x; Tn[1] = o.c0.y; Tn[2] = o.c0.z;
+        B[0] = o.c1.x; B[1] = o.c1.y; B[2] = o.c1.z;
+        N[0] = o.c2.x; N[1] = o.c2.y; N[2] = o.c2.z;
+    } else if (sh.kind == ORC_DISK) {
+        V3<float> hp = v3(h.t * df[0] + of[0], h.t * df[1] + of[1], h.t * df[2] + of[2]);
+        V3<float> bn = normalize(v3(f[0], f[1], f[2]) - hp);
+        V3<float> tn = cross(bn, v3(n[0], n[1], n[2]));
+        Tn[0] = tn.x; Tn[1] = tn.y; Tn[2] = tn.z;
+        B[0] = bn.x; B[1] = bn.y; B[2] = bn.z;
+    } else if (sh.kind == ORC_ELLIPSOID) {
+        V3<float> qn = v3(f[14], f[15], f[16]), nn = v3(n[0], n[1], n[2]);
+        V3<float> bn, tn;
+        if (fabsf(dot(qn, nn)) < 1e-6f) {
+            M3<float> bs = make_basis(nn);
+            bn = bs.c1;
+            tn = bs.c0;
+        } else {
+            bn = normalize(cross(qn, nn));
+            tn = cross(bn, nn);
+        }
+        Tn[0] = tn.x; Tn[1] = tn.y; Tn[2] = tn.z;
+        B[0] = bn.x; B[1] = bn.y; B[2] = bn.z;
+    } else {
+        if (fabsf(n[2]) == 1.0f) {
+            float p[2] = {of[0] + h.t * df[0], of[1] + h.t * df[1]};
+            float tv[2] = {p[0] - f[0], p[1] - f[1]};
+            float l = sqrtf(tv[0] * tv[0] + tv[1] * tv[1]);
+            if (l != 0.0f) { tv[0] /= l; tv[1] /= l; }
+            Tn[0] = tv[0]; Tn[1] = tv[1]; Tn[2] = 0.0f;
+        } else {
+            Tn[0] = 0.0f; Tn[1] = 0.0f; Tn[2] = 1.0f;
+        }
+        cross3f(n, Tn, B);
+    }
+    sd->normal = v3((T)N[0], (T)N[1], (T)N[2]);
+    sd->tangent = v3((T)Tn[0], (T)Tn[1], (T)Tn[2]);
+    sd->binormal = v3((T)B[0], (T)B[1], (T)B[2]);
+    sd->tcx = (T)tcx;
+    sd->tcy = (T)tcy;
+}
+
 template <class T>
 void sample_geometry(const orc_scene* s, const tr_hit& h, V3<T> org, V3<T> dir, Sample<T>* sd)
 {
@@ -406,6 +487,8 @@ void sample_geometry(const orc_scene* s, const tr_hit& h, V3<T> org, V3<T> dir, 
             sd->binormal = b.c1;
         }
         sd->normal = norm;
+    } else if (g.type == GEOM_SHAPE) {
+        shape_sample(s, s->shapes[g.shape], h, org, dir, sd);
     } else {
         float of[3] = {(float)org.x, (float)org.y, (float)org.z};
         float df[3] = {(float)dir.x, (float)dir.y, (float)dir.z};
@@ -775,6 +858,59 @@ template <class T> void cache_inputs(const orc_scene* s, V3<T> pos, V3<T> dir, c
     in[9] = cache_roughness(sd.mat);
 }
 
+/* The ray from the near to the far plane through screen point (sx, sy):
+ * Matrix4 * Vector4 (matrix4.hh:858-880), ToVector3 (vector4.hh:125-130)
+ * and the normalized difference. */
+void screen_ray(const orc_scene* s, float sx, float sy, float start_f[3], float dir_f[3])
+{
+    const float* m = s->vpi;
+    float in0[4] = {sx, sy, -1.0f, 1.0f}, in1[4] = {sx, sy, 1.0f, 1.0f};
+    float p0[4], p1[4];
+    for (int r = 0; r < 4; ++r) {
+        p0[r] = m[r] * in0[0]; p1[r] = m[r] * in1[0];
+        p0[r] += m[4 + r] * in0[1]; p1[r] += m[4 + r] * in1[1];
+        p0[r] += m[8 + r] * in0[2]; p1[r] += m[8 + r] * in1[2];
+        p0[r] += m[12 + r] * in0[3]; p1[r] += m[12 + r] * in1[3];
+    }
+    float w0 = 1.0f / p0[3], w1 = 1.0f / p1[3];
+    float end_f[3] = {p1[0] * w1, p1[1] * w1, p1[2] * w1};
+    start_f[0] = p0[0] * w0; start_f[1] = p0[1] * w0; start_f[2] = p0[2] * w0;
+    for (int a = 0; a < 3; ++a) dir_f[a] = end_f[a] - start_f[a];
+    float l = sqrtf(dir_f[0] * dir_f[0] + dir_f[1] * dir_f[1] + dir_f[2] * dir_f[2]);
+    if (l != 0.0f) { float r = 1.0f / l; dir_f[0] *= r; dir_f[1] *= r; dir_f[2] *= r; }
+}
+
+/* The camera ray of pixel (x, y) for the sample whose Rng is `rng`
+ * (subdraw, ray-tracing.cc:2011-2037): jittered when SamplesCamera > 1,
+ * else through the pixel's corner grid point. */
+void camera_ray(const orc_scene* s, const Rng& rng, uint32_t x, uint32_t y, float start_f[3], float dir_f[3])
+{
+    float sx, sy;
+    if (s->spp > 1) {
+        uint32_t xi[4];
+        rng.draw(0, STREAM_CAMERA, 0, xi);
+        sx = 2.0f * ((float)x + orc_u01(xi[0])) / (float)s->width - 1.0f;
+        sy = 2.0f * ((float)y + orc_u01(xi[1])) / (float)s->height - 1.0f;
+    } else {
+        sx = 2.0f * (float)x / (float)(s->width - 1) - 1.0f;
+        sy = 2.0f * (float)y / (float)(s->height - 1) - 1.0f;
+    }
+    screen_ray(s, sx, sy, start_f, dir_f);
+}
+
+/* RenderMode: the tangent, binormal or normal * 0.5 + 0.5, or
+ * max(0, N . -dir) (ray-tracing.cc:2078-2085). */
+template <class T> V3<T> debug_color(uint32_t mode, V3<T> t, V3<T> b, V3<T> n, V3<T> dir)
+{
+    if (mode == 4) {
+        T c = dot(n, -dir);
+        c = (T)0 > c ? (T)0 : c;
+        return v3(c, c, c);
+    }
+    V3<T> v = mode == 1 ? t : (mode == 2 ? b : n);
+    return v * (T)0.5f + v3((T)0.5f, (T)0.5f, (T)0.5f);
+}
+
 /* One camera sample of one pixel: the body of the camera loop of subdraw
  * (ray-tracing.cc:2011-2345) for surfaces. */
 template <class T>
@@ -795,34 +931,8 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
     rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
     rng.pixel = pixel + s->pixel_key_base; rng.sample = cs; rng.split = 0;
     uint32_t hash = 2166136261u;
-    float sx, sy;
-    if (s->spp > 1) {
-        uint32_t xi[4];
-        rng.draw(0, STREAM_CAMERA, 0, xi);
-        sx = 2.0f * ((float)x + orc_u01(xi[0])) / (float)s->width - 1.0f;
-        sy = 2.0f * ((float)y + orc_u01(xi[1])) / (float)s->height - 1.0f;
-    } else {
-        sx = 2.0f * (float)x / (float)(s->width - 1) - 1.0f;
-        sy = 2.0f * (float)y / (float)(s->height - 1) - 1.0f;
-    }
-    /* Matrix4 * Vector4 (matrix4.hh:858-880) and ToVector3 (vector4.hh:125-130) */
-    const float* m = s->vpi;
-    float in0[4] = {sx, sy, -1.0f, 1.0f}, in1[4] = {sx, sy, 1.0f, 1.0f};
-    float p0[4], p1[4];
-    for (int r = 0; r < 4; ++r) {
-        p0[r] = m[r] * in0[0]; p1[r] = m[r] * in1[0];
-        p0[r] += m[4 + r] * in0[1]; p1[r] += m[4 + r] * in1[1];
-        p0[r] += m[8 + r] * in0[2]; p1[r] += m[8 + r] * in1[2];
-        p0[r] += m[12 + r] * in0[3]; p1[r] += m[12 + r] * in1[3];
-    }
-    float w0 = 1.0f / p0[3], w1 = 1.0f / p1[3];
-    float start_f[3] = {p0[0] * w0, p0[1] * w0, p0[2] * w0};
-    float end_f[3] = {p1[0] * w1, p1[1] * w1, p1[2] * w1};
-    float dir_f[3] = {end_f[0] - start_f[0], end_f[1] - start_f[1], end_f[2] - start_f[2]};
-    {
-        float l = sqrtf(dir_f[0] * dir_f[0] + dir_f[1] * dir_f[1] + dir_f[2] * dir_f[2]);
-        if (l != 0.0f) { float r = 1.0f / l; dir_f[0] *= r; dir_f[1] *= r; dir_f[2] *= r; }
-    }
+    float start_f[3], dir_f[3];
+    camera_ray(s, rng, x, y, start_f, dir_f);
     V3<T> pos = to_v3<T>(start_f);
     Sample<T> sd;
     sd.wi = to_v3<T>(dir_f);
@@ -837,7 +947,19 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
         return env_radiance<T>(s, df);
     }
     hash_hit(&hash, h);
+    if (s->geoms[h.geom].material == ORC_NO_MATERIAL) {
+        /* a blocker ends the camera sample (ray-tracing.cc:2072-2077); its
+         * sample() frame is zero (:1239-1244) */
+        *hash_out = repeat_hash(hash, s->n_gi);
+        V3<T> z = v3((T)0, (T)0, (T)0);
+        return s->render_mode ? debug_color<T>(s->render_mode, z, z, z, sd.wi) : z;
+    }
     bool volume = surface_or_volume(s, h, &pos, &sd);
+    if (s->render_mode && !volume) {
+        /* RenderMode debug output at the camera vertex (ray-tracing.cc:2078-2089) */
+        *hash_out = repeat_hash(hash, s->n_gi);
+        return debug_color<T>(s->render_mode, sd.tangent, sd.binormal, sd.normal, sd.wi);
+    }
     V3<T> start_T = v3((T)1, (T)1, (T)1);
     if (volume) {
         /* albedo * density / density (ray-tracing.cc:2134-2142) */
@@ -913,6 +1035,7 @@ V3<T> render_camera_sample(const orc_scene* s, uint32_t pixel, uint32_t cs, uint
             }
             hash_hit(&hash, hh);
             ORC_TRACE("   hit geom=%u prim=%u t=%g\n", hh.geom, hh.prim, (double)hh.t);
+            if (s->geoms[hh.geom].material == ORC_NO_MATERIAL) break;  /* blocker (ray-tracing.cc:2236-2240) */
             V3<T> prev_pos = cpos;
             cur_volume = surface_or_volume(s, hh, &cpos, &cur);
             if (cur_volume) {
@@ -1035,6 +1158,49 @@ void orc_set_background_spectrum(orc_scene* s, const float rgb[3]) { memcpy(s->b
 void orc_set_seed(orc_scene* s, uint64_t seed) { s->key[0] = (uint32_t)seed; s->key[1] = (uint32_t)(seed >> 32); }
 void orc_set_pixel_key_base(orc_scene* s, uint32_t base) { s->pixel_key_base = base; }
 void orc_set_estimator(orc_scene* s, uint32_t mode) { s->estimator = mode; }
+void orc_set_render_mode(orc_scene* s, uint32_t mode) { s->render_mode = mode; }
+
+/* Per pixel of frame `frame`: 1 if one of its camera rays hit geometry
+ * (after tracking through volumes), the reference's `intersections`
+ * behind TransparentBackground (ray-tracing.cc:2062-2066, 2364). */
+void orc_camera_hits(const orc_scene* s, uint32_t frame, uint8_t* out)
+{
+    const uint32_t n = s->width * s->height;
+    parallel_for(n, 0, 64, [&](uint32_t p) {
+        uint8_t any = 0;
+        for (uint32_t k = 0; k < s->spp && !any; ++k) {
+            Rng rng;
+            rng.key[0] = s->key[0]; rng.key[1] = s->key[1];
+            rng.pixel = p + s->pixel_key_base; rng.sample = frame * s->spp + k; rng.split = 0;
+            float o[3], d[3];
+            camera_ray(s, rng, p % s->width, p / s->width, o, d);
+            any = trace_segment(s, o, d, rng, 0).geom != TR_INVALID_GEOMETRY;
+        }
+        out[p] = any;
+    });
+}
+
+/* rayQuery (ray-tracing.cc:1932-1979): out = t, normal, tangent, binormal,
+ * TexCoord; returns the geometry slot or TR_INVALID_GEOMETRY. */
+uint32_t orc_ray_query(const orc_scene* s, const float tc[2], float out[12])
+{
+    float o[3], d[3];
+    screen_ray(s, 2.0f * tc[0] - 1.0f, 1.0f - 2.0f * tc[1], o, d);
+    tr_hit h = trace_closest(s, o, d, 0.0f, INFINITY, false);
+    for (int k = 0; k < 12; ++k) out[k] = 0.0f;
+    if (h.geom == TR_INVALID_GEOMETRY) return h.geom;
+    out[0] = h.t;
+    const Geom& g = s->geoms[h.geom];
+    if (g.material != ORC_NO_MATERIAL && g.type != GEOM_VOLUME) {
+        Sample<float> sd{};
+        sample_geometry<float>(s, h, v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), &sd);
+        V3<float> f[3] = {sd.normal, sd.tangent, sd.binormal};
+        for (int k = 0; k < 3; ++k) { out[1 + 3 * k] = f[k].x; out[2 + 3 * k] = f[k].y; out[3 + 3 * k] = f[k].z; }
+        out[10] = sd.tcx;
+        out[11] = sd.tcy;
+    }
+    return h.geom;
+}
 
 /* Cache mode with a constant network output `value`; the scene box of the
  * position input is the box of all primitives (set at commit).  Training
@@ -1140,6 +1306,208 @@ int orc_add_sphere(orc_scene* s, const float center[3], float radius, const tr_m
 {
     if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
     uint32_t slot = add_sphere_geom(s, center, radius, GEOM_SPHERE, intern_material(s, m), 0);
+    if (out) *out = slot;
+    return 0;
+}
+
+/* ---- user shapes (ray-tracing.cc:1315-1446), parameters in FP32 ---- */
+
+static uint32_t add_shape_geom(orc_scene* s, const OrcShape& sh, uint32_t material, const double lo[3],
+                               const double hi[3])
+{
+    Geom g;
+    memset(&g, 0, sizeof(g));
+    g.type = GEOM_SHAPE;
+    g.material = material;
+    g.shape = (uint32_t)s->shapes.size();
+    s->shapes.push_back(sh);
+    uint32_t slot = (uint32_t)s->geoms.size();
+    s->geoms.push_back(g);
+    Prim p;
+    memset(&p, 0, sizeof(p));
+    for (int a = 0; a < 3; ++a) {
+        double pad = 1e-6 * (fabs(lo[a]) + fabs(hi[a]) + (hi[a] - lo[a])) + 1e-30;
+        p.v1[a] = std::nextafter((float)(lo[a] - pad), -INFINITY);
+        p.v2[a] = std::nextafter((float)(hi[a] + pad), INFINITY);
+    }
+    p.geom = slot;
+    p.prim = 0;
+    p.is_shape = 1;
+    s->prims.push_back(p);
+    return slot;
+}
+
+/* Orientation = ToQuaternion(Matrix3(tan, norm x tan, norm))
+ * (MatrixToQuaternion, quaternion.hh:59-70), then ToMatrix3 of it. */
+static int add_rect_geom(orc_scene* s, const float pos[3], const float tan[3], const float norm[3], const float size[2],
+                         uint32_t material, const float tc[4], uint32_t tangent_map, uint64_t* out)
+{
+    float b[3];
+    cross3f(norm, tan, b);
+    const float* c0 = tan;
+    const float* c1 = b;
+    const float* c2 = norm;
+    float q[4];
+    q[3] = sqrtf(std::max(0.0f, 1.0f + c0[0] + c1[1] + c2[2])) / 2.0f;
+    q[0] = sqrtf(std::max(0.0f, 1.0f + c0[0] - c1[1] - c2[2])) / 2.0f;
+    q[1] = sqrtf(std::max(0.0f, 1.0f - c0[0] + c1[1] - c2[2])) / 2.0f;
+    q[2] = sqrtf(std::max(0.0f, 1.0f - c0[0] - c1[1] + c2[2])) / 2.0f;
+    q[0] = std::copysign(q[0], c1[2] - c2[1]);
+    q[1] = std::copysign(q[1], c2[0] - c0[2]);
+    q[2] = std::copysign(q[2], c0[1] - c1[0]);
+    M3<float> o = quat_to_m3(q);
+    OrcShape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.kind = ORC_RECT;
+    sh.tangent_map = tangent_map;
+    for (int a = 0; a < 3; ++a) sh.f[a] = pos[a];
+    V3<float> cols[3] = {o.c0, o.c1, o.c2};
+    for (int k = 0; k < 3; ++k) {
+        sh.f[3 + 3 * k] = cols[k].x;
+        sh.f[4 + 3 * k] = cols[k].y;
+        sh.f[5 + 3 * k] = cols[k].z;
+    }
+    sh.f[12] = size[0];
+    sh.f[13] = size[1];
+    if (tc) {
+        sh.f[14] = tc[0]; sh.f[15] = tc[1];
+        sh.f[16] = tc[2] - tc[0]; sh.f[17] = tc[3] - tc[1];
+    } else {
+        sh.f[16] = 1.0f; sh.f[17] = 1.0f;
+    }
+    /* Rect3Bounds (shapes.hh:260-266) */
+    double lo[3], hi[3];
+    double t[3] = {o.c0.x, o.c0.y, o.c0.z}, bn[3] = {o.c1.x, o.c1.y, o.c1.z};
+    for (int a = 0; a < 3; ++a) {
+        double e = fabs(size[0] * t[a]) + fabs(size[1] * bn[a]);
+        lo[a] = pos[a] - e;
+        hi[a] = pos[a] + e;
+    }
+    uint32_t slot = add_shape_geom(s, sh, material, lo, hi);
+    if (out) *out = slot;
+    return 0;
+}
+
+int orc_add_rect(orc_scene* s, const float pos[3], const float tan[3], const float norm[3], const float size[2],
+                 const tr_material* m, const float tc[4], uint32_t tangent_map, uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    return add_rect_geom(s, pos, tan, norm, size, intern_material(s, m), tc, tangent_map, out);
+}
+
+int orc_add_blocker(orc_scene* s, const float pos[3], const float tan[3], const float norm[3], const float size[2],
+                    uint64_t* out)
+{
+    if (s->committed) return -1;
+    return add_rect_geom(s, pos, tan, norm, size, ORC_NO_MATERIAL, nullptr, 0, out);
+}
+
+int orc_add_disk(orc_scene* s, const float pos[3], const float norm[3], float inner, float outer, const tr_material* m,
+                 uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    OrcShape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.kind = ORC_DISK;
+    for (int a = 0; a < 3; ++a) { sh.f[a] = pos[a]; sh.f[3 + a] = norm[a]; }
+    sh.f[6] = inner;
+    sh.f[7] = outer;
+    double nl = sqrt((double)norm[0] * norm[0] + (double)norm[1] * norm[1] + (double)norm[2] * norm[2]);
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        double na = norm[a] / nl, e = outer * sqrt(std::max(0.0, 1.0 - na * na));
+        lo[a] = pos[a] - e;
+        hi[a] = pos[a] + e;
+    }
+    uint32_t slot = add_shape_geom(s, sh, intern_material(s, m), lo, hi);
+    if (out) *out = slot;
+    return 0;
+}
+
+static int add_cylinder_geom(orc_scene* s, const float c[3], float r, float hh, const float* tilt_dir, float tilt,
+                             const tr_material* m, uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    OrcShape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.kind = tilt_dir ? ORC_OBLIQUE : ORC_CYLINDER;
+    for (int a = 0; a < 3; ++a) sh.f[a] = c[a];
+    sh.f[3] = r;
+    sh.f[4] = hh;
+    double ex = 0.0, ey = 0.0;
+    if (tilt_dir) {
+        /* bending = Tilt * TiltDirection / HalfHeight (ray-tracing.cc:999) */
+        sh.f[5] = (tilt * tilt_dir[0]) / hh;
+        sh.f[6] = (tilt * tilt_dir[1]) / hh;
+        ex = fabs((double)tilt * tilt_dir[0]);
+        ey = fabs((double)tilt * tilt_dir[1]);
+    }
+    double lo[3] = {c[0] - r - ex, c[1] - r - ey, (double)c[2] - hh};
+    double hi[3] = {c[0] + r + ex, c[1] + r + ey, (double)c[2] + hh};
+    uint32_t slot = add_shape_geom(s, sh, intern_material(s, m), lo, hi);
+    if (out) *out = slot;
+    return 0;
+}
+
+int orc_add_cylinder(orc_scene* s, const float c[3], float r, float hh, const tr_material* m, uint64_t* out)
+{
+    return add_cylinder_geom(s, c, r, hh, nullptr, 0.0f, m, out);
+}
+
+int orc_add_oblique_cylinder(orc_scene* s, const float c[3], float r, float hh, const float tilt_dir[2], float tilt,
+                             const tr_material* m, uint64_t* out)
+{
+    return add_cylinder_geom(s, c, r, hh, tilt_dir, tilt, m, out);
+}
+
+int orc_add_ellipsoid(orc_scene* s, const float c[3], const float scale[3], const float q[4], const tr_material* m,
+                      uint64_t* out)
+{
+    if (s->committed || s->materials.size() + 1 > s->materials.capacity()) return -1;
+    OrcShape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.kind = ORC_ELLIPSOID;
+    for (int a = 0; a < 3; ++a) { sh.f[a] = c[a]; sh.f[3 + a] = scale[a]; }
+    for (int k = 0; k < 4; ++k) sh.f[6 + k] = q[k];
+    sh.f[10] = -q[0]; sh.f[11] = -q[1]; sh.f[12] = -q[2]; sh.f[13] = q[3];   /* Inverse = Conjugate */
+    {   /* ToNormal (quaternion.hh:170-181) */
+        float x = q[0], y = q[1], z = q[2], w = q[3];
+        float xx = x * x, xz = x * z, xw = x * w, yy = y * y, yz = y * z, yw = y * w;
+        sh.f[14] = 2.0f * (xz + yw);
+        sh.f[15] = 2.0f * (yz - xw);
+        sh.f[16] = 1.0f - 2.0f * (xx + yy);
+    }
+    /* box: the surface is c + M^-1 S u, |u| = 1, with M v = Transform(conj q,
+     * v); the half extent along axis i is the norm of row i of M^-1 S */
+    double Mm[3][3];
+    for (int j = 0; j < 3; ++j) {
+        double e[3] = {0, 0, 0};
+        e[j] = 1.0;
+        double qv[3] = {-(double)q[0], -(double)q[1], -(double)q[2]}, w = q[3];
+        double c1[3] = {qv[1] * e[2] - qv[2] * e[1] + w * e[0], qv[2] * e[0] - qv[0] * e[2] + w * e[1],
+                        qv[0] * e[1] - qv[1] * e[0] + w * e[2]};
+        double c2[3] = {qv[1] * c1[2] - qv[2] * c1[1], qv[2] * c1[0] - qv[0] * c1[2], qv[0] * c1[1] - qv[1] * c1[0]};
+        for (int i = 0; i < 3; ++i) Mm[i][j] = e[i] + 2.0 * c2[i];
+    }
+    double det = Mm[0][0] * (Mm[1][1] * Mm[2][2] - Mm[1][2] * Mm[2][1]) -
+                 Mm[0][1] * (Mm[1][0] * Mm[2][2] - Mm[1][2] * Mm[2][0]) +
+                 Mm[0][2] * (Mm[1][0] * Mm[2][1] - Mm[1][1] * Mm[2][0]);
+    if (!(fabs(det) > 1e-12)) return -1;
+    double inv[3][3] = {{(Mm[1][1] * Mm[2][2] - Mm[1][2] * Mm[2][1]) / det, (Mm[0][2] * Mm[2][1] - Mm[0][1] * Mm[2][2]) / det,
+                         (Mm[0][1] * Mm[1][2] - Mm[0][2] * Mm[1][1]) / det},
+                        {(Mm[1][2] * Mm[2][0] - Mm[1][0] * Mm[2][2]) / det, (Mm[0][0] * Mm[2][2] - Mm[0][2] * Mm[2][0]) / det,
+                         (Mm[0][2] * Mm[1][0] - Mm[0][0] * Mm[1][2]) / det},
+                        {(Mm[1][0] * Mm[2][1] - Mm[1][1] * Mm[2][0]) / det, (Mm[0][1] * Mm[2][0] - Mm[0][0] * Mm[2][1]) / det,
+                         (Mm[0][0] * Mm[1][1] - Mm[0][1] * Mm[1][0]) / det}};
+    double lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        double e2 = 0.0;
+        for (int j = 0; j < 3; ++j) e2 += (inv[i][j] * scale[j]) * (inv[i][j] * scale[j]);
+        double e = sqrt(e2) * (1.0 + 1e-5);
+        lo[i] = c[i] - e;
+        hi[i] = c[i] + e;
+    }
+    uint32_t slot = add_shape_geom(s, sh, intern_material(s, m), lo, hi);
     if (out) *out = slot;
     return 0;
 }
@@ -1524,7 +1892,19 @@ static float orc_aces(float x)
     return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
 }
 
+float orc_resolve_masked(size_t n, const double* accum, double rcp_count, uint32_t post, float exposure_in,
+                         const uint8_t* mask, uint32_t* rgba);
+
 float orc_resolve(size_t n, const double* accum, double rcp_count, uint32_t post, float exposure_in, uint32_t* rgba)
+{
+    return orc_resolve_masked(n, accum, rcp_count, post, exposure_in, nullptr, rgba);
+}
+
+/* With TransparentBackground (ray-tracing.cc:2364-2365): pixels whose mask
+ * is 0 are neither written nor part of the log sum (whose mean still
+ * divides by all n pixels). */
+float orc_resolve_masked(size_t n, const double* accum, double rcp_count, uint32_t post, float exposure_in,
+                         const uint8_t* mask, uint32_t* rgba)
 {
     auto mean = [&](size_t i, float c[3]) {
         for (int k = 0; k < 3; ++k) c[k] = (float)(accum[i * 3 + k] * rcp_count);
@@ -1534,6 +1914,7 @@ float orc_resolve(size_t n, const double* accum, double rcp_count, uint32_t post
     if (autoexp) {
         double log_sum[3] = {0.0, 0.0, 0.0};
         for (size_t i = 0; i < n; ++i) {
+            if (mask && !mask[i]) continue;
             float c[3];
             mean(i, c);
             if (!(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2]))) continue;
@@ -1545,6 +1926,7 @@ float orc_resolve(size_t n, const double* accum, double rcp_count, uint32_t post
     }
     const float e = exposure_in > 0.0f ? exposure_in : exposure;
     for (size_t i = 0; i < n; ++i) {
+        if (mask && !mask[i]) continue;
         float c[3];
         mean(i, c);
         if (post == TR_POST_LINEAR) {
@@ -1764,6 +2146,21 @@ void orc_bsdf_eval_batch(const tr_material* m, const float* n, const float* t, c
         out[4 * i] = f.x; out[4 * i + 1] = f.y; out[4 * i + 2] = f.z;
         out[4 * i + 3] = bsdf_pdf(s);
     }
+}
+
+/* A user shape's intersection without range: kind 1-5 (rect, disk,
+ * cylinder, oblique cylinder, ellipsoid) and its 20 parameters; out = t, u,
+ * v, normal.  Returns 1 on a hit. */
+int orc_shape_intersect(uint32_t kind, const float* params, const float* org, const float* dir, float* out)
+{
+    OrcShape sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.kind = kind;
+    memcpy(sh.f, params, sizeof(sh.f));
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    int hit = orc_shape_hit(sh, org, dir, &out[0], &out[1], &out[2], n);
+    out[3] = n[0]; out[4] = n[1]; out[5] = n[2];
+    return hit;
 }
 
 /* SGGX helpers: out[0] = SGGXMicroFlakeNDF(stddev, basis, dir),

@@ -44,7 +44,9 @@ struct OrcTexSet {
     }
 };
 
-enum { ORC_TEX_SPECTRUM = 0, ORC_TEX_RG = 1, ORC_TEX_RGBA = 2 };
+/* SampleSpectrum, SampleRG, SampleRGBA and sampleRGB (8-bit channels as
+ * byte / 255, texture.hh:203-240) */
+enum { ORC_TEX_SPECTRUM = 0, ORC_TEX_RG = 1, ORC_TEX_RGBA = 2, ORC_TEX_RGB = 3 };
 
 inline uint32_t orc_texel_size(uint32_t fmt)
 {

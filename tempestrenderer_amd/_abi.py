@@ -56,6 +56,17 @@ class tr_submesh(ctypes.Structure):
     ]
 
 
+class tr_sample_data(ctypes.Structure):
+    _fields_ = [
+        ("material", ctypes.c_void_p),
+        ("normal", ctypes.c_float * 3),
+        ("tangent", ctypes.c_float * 3),
+        ("binormal", ctypes.c_float * 3),
+        ("texcoord", ctypes.c_float * 2),
+        ("t", ctypes.c_float),
+    ]
+
+
 class tr_mesh_options(ctypes.Structure):
     _fields_ = [("two_sided", ctypes.c_int32)]
 
@@ -209,6 +220,17 @@ SIGNATURES = {
     "tr_scene_add_triangle_mesh": (_st, [_vp, P(ctypes.c_float), _sz, P(tr_submesh), _sz, P(ctypes.c_int32),
                                          _sz, _vp, P(tr_mesh_options), P(_u64)]),
     "tr_scene_add_sphere": (_st, [_vp, P(ctypes.c_float), ctypes.c_float, P(tr_material), P(_u64)]),
+    "tr_scene_add_rect": (_st, [_vp, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                P(tr_material), P(ctypes.c_float), _u32, P(_u64)]),
+    "tr_scene_add_blocker": (_st, [_vp, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                   P(_u64)]),
+    "tr_scene_add_disk": (_st, [_vp, P(ctypes.c_float), P(ctypes.c_float), ctypes.c_float, ctypes.c_float,
+                                P(tr_material), P(_u64)]),
+    "tr_scene_add_cylinder": (_st, [_vp, P(ctypes.c_float), ctypes.c_float, ctypes.c_float, P(tr_material), P(_u64)]),
+    "tr_scene_add_oblique_cylinder": (_st, [_vp, P(ctypes.c_float), ctypes.c_float, ctypes.c_float, P(ctypes.c_float),
+                                            ctypes.c_float, P(tr_material), P(_u64)]),
+    "tr_scene_add_ellipsoid": (_st, [_vp, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float), P(tr_material),
+                                     P(_u64)]),
     "tr_scene_add_grid_volume": (_st, [_vp, P(tr_grid_volume), P(tr_material), P(_u64)]),
     "tr_scene_commit": (_st, [_vp]),
     "tr_scene_get_info": (_st, [_vp, P(tr_scene_info)]),
@@ -235,6 +257,7 @@ SIGNATURES = {
     "tr_scene_set_estimator": (_st, [_vp, _u32]),
     "tr_scene_set_frame_index": (_st, [_vp, _u32]),
     "tr_scene_trace_rays": (_st, [_vp, _sz, P(tr_ray), P(tr_hit)]),
+    "tr_scene_ray_query": (_st, [_vp, P(ctypes.c_float), P(tr_sample_data), P(_u64)]),
     "tr_scene_trace_rays_noalloc": (_st, [_vp, _sz, _vp, _vp, _vp]),
     "tr_scene_render_paths": (_st, [_vp, _sz, P(_u32), P(ctypes.c_float), P(_u32)]),
     "tr_scene_get_stats": (_st, [_vp, P(tr_stats)]),

@@ -78,6 +78,9 @@ struct tr_scene {
     std::vector<int32_t> ipool;
     std::vector<GeomRec> geoms;
     std::vector<PrimRec> prims_in;
+    std::vector<ShapeRec> shapes;   /* user shapes (addRect, addDisk, ...) */
+    std::vector<uint64_t> ext_ids;  /* geometry slot -> the id its add call returned (meshes: instance << 32 | submesh) */
+    DevBuf<RayQueryOut> query_out;
     std::vector<LightRec> lights;
     std::vector<VolumeRec> vols;
     std::vector<uint64_t> voxels;   /* voxel words of all volumes (kVoxelHalf / kVoxelQuantized) */
@@ -104,6 +107,7 @@ struct tr_scene {
     DevBuf<uint8_t> d_vpool;
     DevBuf<int32_t> d_ipool;
     DevBuf<VolumeRec> d_vols;
+    DevBuf<ShapeRec> d_shapes;
     DevBuf<uint64_t> d_voxels;
     std::vector<TexRec> tex_recs;    /* bound textures */
     std::vector<uint8_t> texels;     /* their texels, then the cube map's faces */
@@ -127,6 +131,7 @@ struct tr_scene {
     /* frame accumulation */
     DevBuf<double> accum;
     DevBuf<uint32_t> rgba;
+    DevBuf<uint8_t> hitmask;      /* TransparentBackground: camera rays of the frame hit geometry, per pixel */
     DevBuf<double> expo_partial;  /* auto exposure: 3 log sums per 256 pixels */
     DevBuf<float> exposure;       /* auto exposure: the frame's factor */
     std::vector<uint32_t> rgba_host;
@@ -172,10 +177,11 @@ tr_scene::~tr_scene()
     ev_used.clear();
     release_events(this);
     d_nodes.release(); d_prims.release(); d_geoms.release(); d_mats.release(); d_lights.release();
-    d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); sh_tmax.release();
+    d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); sh_tmax.release(); d_shapes.release();
+    query_out.release();
     ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
     sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
-    shq.release(); qq.release(); seg.release(); seg_n.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release(); expo_partial.release(); exposure.release();
+    shq.release(); qq.release(); seg.release(); seg_n.release(); sh_n.release(); counters.release(); spill.release(); counts.release(); accum.release(); rgba.release(); hitmask.release(); expo_partial.release(); exposure.release();
     if (net) tr_mlp_destroy(net);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -236,6 +242,7 @@ SceneView scene_view(const tr_scene* s)
     v.ipool = s->d_ipool.ptr;
     v.volumes = s->d_vols.ptr;
     v.voxels = s->d_voxels.ptr;
+    v.shapes = s->d_shapes.ptr;
     v.n_lights = (uint32_t)s->lights.size();
     v.n_prims = (uint32_t)s->prims.size();
     v.n_volumes = (uint32_t)s->vols.size();
@@ -366,6 +373,7 @@ FrameParams frame_params(const tr_scene* s)
         p.box_rcp[a] = ext > 0.0f ? 1.0f / ext : 0.0f;
     }
     memcpy(p.vpi, s->vpi, sizeof(p.vpi));
+    p.render_mode = s->render_mode;
     return p;
 }
 
@@ -386,6 +394,10 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
                                           "sampler has no closed-form density");
     if (!s->vols.empty() && (s->lights.size() > 256 || s->n_light > 256))
         return fail(TR_ERROR_UNSUPPORTED, "scenes with volumes take at most 256 lights and 256 samples per area light");
+    if (s->render_mode != 0 && !s->vols.empty())
+        return fail(TR_ERROR_UNSUPPORTED, "debug render modes do not take volumes");
+    if (s->render_mode != 0 && s->cache.enabled)
+        return fail(TR_ERROR_UNSUPPORTED, "debug render modes do not take the radiance cache");
     if (s->cache.enabled) {
         if (s->estimator != TR_ESTIMATOR_UNBIASED)
             return fail(TR_ERROR_UNSUPPORTED, "the radiance cache needs the unbiased estimator");
@@ -616,7 +628,12 @@ tr_status start_frame(tr_scene* s)
     if (!s->committed) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
     uint64_t npx = (uint64_t)s->width * s->height;
     TR_HIP(s->accum.reserve(npx * 3));
-    TR_HIP(s->rgba.reserve(npx));
+    if (s->rgba.count < npx) {
+        /* a pixel that TransparentBackground never writes stays 0 */
+        TR_HIP(s->rgba.reserve(npx));
+        TR_HIP(hipMemsetAsync(s->rgba.ptr, 0, npx * sizeof(uint32_t), s->stream));
+    }
+    if (s->transparent) TR_HIP(s->hitmask.reserve(npx));
     s->expo_used = s->post == TR_POST_AUTO_EXPOSURE_SRGB || s->post == TR_POST_AUTO_EXPOSURE_HDR;
     if (s->expo_used) {
         TR_HIP(s->expo_partial.reserve((npx + 255) / 256 * 3));
@@ -626,12 +643,13 @@ tr_status start_frame(tr_scene* s)
     BatchOut out;
     memset(&out, 0, sizeof(out));
     out.accum = s->accum.ptr;
+    out.hitmask = s->transparent ? s->hitmask.ptr : nullptr;
     tr_status st = enqueue_frame(s, npx, out);
     if (st != TR_OK) return st;
     s->accum_samples += s->spp;
     s->frame_index += 1;
     launch_resolve(s->stream, s->accum.ptr, 1.0 / (double)s->accum_samples, s->rgba.ptr, (uint32_t)npx, s->post,
-                   s->expo_partial.ptr, s->exposure.ptr);
+                   s->expo_partial.ptr, s->exposure.ptr, out.hitmask);
     TR_HIP(hipGetLastError());
     s->frame_pending = true;
     return TR_OK;
@@ -644,18 +662,14 @@ bool valid(const tr_scene* s) { return s != nullptr; }
 void scene_box(tr_scene* s)
 {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (const PrimRec& p : s->prims_in)
+    for (const PrimRec& p : s->prims_in) {
+        float pl[3], ph[3];
+        prim_box(p, pl, ph);
         for (int a = 0; a < 3; ++a) {
-            float pl, ph;
-            if (p.flags & PRIM_SPHERE) { pl = p.v0[a] - p.v1[0]; ph = p.v0[a] + p.v1[0]; }
-            else if (p.flags & PRIM_VOLUME) { pl = p.v0[a]; ph = p.v1[a]; }
-            else {
-                pl = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
-                ph = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));
-            }
-            lo[a] = std::min(lo[a], pl);
-            hi[a] = std::max(hi[a], ph);
+            lo[a] = std::min(lo[a], pl[a]);
+            hi[a] = std::max(hi[a], ph[a]);
         }
+    }
     for (int a = 0; a < 3; ++a) {
         if (!(lo[a] <= hi[a])) { lo[a] = 0.0f; hi[a] = 1.0f; }
         s->box_lo[a] = lo[a];
@@ -748,7 +762,13 @@ tr_status tr_scene_set_picture_post_process(tr_scene* s, uint32_t mode)
     return TR_OK;
 }
 SETTER(tr_scene_set_transparent_background, transparent, int32_t)
-SETTER(tr_scene_set_render_mode, render_mode, uint32_t)
+tr_status tr_scene_set_render_mode(tr_scene* s, uint32_t mode)
+{
+    if (!valid(s)) return fail(TR_ERROR_INVALID_ARG, "null scene");
+    if (mode > 4) return fail(TR_ERROR_INVALID_ARG, "RenderMode is 0 (Normal) to 4 (DebugLighting)");
+    s->render_mode = mode;
+    return TR_OK;
+}
 SETTER(tr_scene_set_seed, seed, uint64_t)
 SETTER(tr_scene_set_frame_index, frame_index, uint32_t)
 #undef SETTER
@@ -995,6 +1015,8 @@ tr_status tr_scene_add_triangle_mesh(tr_scene* s, const float world[16], size_t 
             s->lights.push_back(L);
             s->lights_dirty = true;
         }
+        if (s->ext_ids.size() < s->geoms.size()) s->ext_ids.resize(s->geoms.size(), UINT64_MAX);
+        s->ext_ids[slot] = ((uint64_t)inst << 32) | (uint64_t)si;
         if (geom_ids) geom_ids[si] = ((uint64_t)inst << 32) | (uint64_t)si;
     }
     return TR_OK;
@@ -1008,6 +1030,279 @@ tr_status tr_scene_add_sphere(tr_scene* s, const float center[3], float radius, 
     if (!model_supported(m->model)) return fail(TR_ERROR_UNSUPPORTED, "illumination model not supported by the GPU tracer");
     uint32_t slot = add_sphere_prim(s, center, radius, GEOM_SPHERE, intern_material(s, m));
     if (out) *out = slot;
+    return TR_OK;
+}
+
+/* ---------------- user shapes (ray-tracing.cc:1315-1446) ---------------- */
+
+namespace {
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+/* MatrixToQuaternion (quaternion.hh:59-70) of the matrix with columns c0,
+ * c1, c2; matrix(i, j) is component j of column i. */
+void matrix_to_quat(V3 c0, V3 c1, V3 c2, float q[4])
+{
+    q[3] = sqrtf(std::max(0.0f, 1.0f + c0.x + c1.y + c2.z)) / 2.0f;
+    q[0] = sqrtf(std::max(0.0f, 1.0f + c0.x - c1.y - c2.z)) / 2.0f;
+    q[1] = sqrtf(std::max(0.0f, 1.0f - c0.x + c1.y - c2.z)) / 2.0f;
+    q[2] = sqrtf(std::max(0.0f, 1.0f - c0.x - c1.y + c2.z)) / 2.0f;
+    q[0] = std::copysign(q[0], c1.z - c2.y);
+    q[1] = std::copysign(q[1], c2.x - c0.z);
+    q[2] = std::copysign(q[2], c0.y - c1.x);
+}
+
+/* A shape's geometry slot, record and primitive; its box [lo, hi] is
+ * widened by one float ulp-scale step outward. */
+uint32_t add_shape_prim(tr_scene* s, ShapeRec rec, uint32_t material, const double lo[3], const double hi[3])
+{
+    GeomRec g;
+    memset(&g, 0, sizeof(g));
+    g.type = GEOM_SHAPE;
+    g.material = material;
+    g.index_base = (uint32_t)s->shapes.size();
+    const uint32_t slot = (uint32_t)s->geoms.size();
+    s->geoms.push_back(g);
+    rec.geom = slot;
+    s->shapes.push_back(rec);
+    PrimRec p;
+    memset(&p, 0, sizeof(p));
+    p.v0[0] = tr_u2f(g.index_base);
+    for (int a = 0; a < 3; ++a) {
+        const double pad = 1e-6 * (std::fabs(lo[a]) + std::fabs(hi[a]) + (hi[a] - lo[a])) + 1e-30;
+        p.v1[a] = std::nextafter((float)(lo[a] - pad), -INFINITY);
+        p.v2[a] = std::nextafter((float)(hi[a] + pad), INFINITY);
+    }
+    p.prim = 0;
+    p.geom = slot;
+    p.flags = PRIM_SHAPE | (rec.kind << 8);
+    s->prims_in.push_back(p);
+    return slot;
+}
+
+/* Rect and blocker: Orientation = ToQuaternion(Matrix3(tan, norm x tan,
+ * norm)) and its ToMatrix3 columns; Rect3Bounds (shapes.hh:260-266). */
+tr_status add_rect_shape(tr_scene* s, const float pos[3], const float tan[3], const float norm[3], const float size[2],
+                         uint32_t material, const float tc[4], uint32_t tangent_map, uint64_t* out)
+{
+    if (!finite3(pos) || !finite3(tan) || !finite3(norm) || !std::isfinite(size[0]) || !std::isfinite(size[1]) ||
+        !(size[0] > 0.0f) || !(size[1] > 0.0f))
+        return fail(TR_ERROR_INVALID_ARG, "rect: finite position, tangent and normal and positive half sizes needed");
+    const V3 T = mk(tan[0], tan[1], tan[2]), N = mk(norm[0], norm[1], norm[2]);
+    float q[4];
+    matrix_to_quat(T, cross(N, T), N, q);
+    const M3 o = quat_to_m3(q[0], q[1], q[2], q[3]);
+    ShapeRec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = SHAPE_RECT;
+    r.tangent_map = tangent_map;
+    const V3 cols[3] = {o.c0, o.c1, o.c2};
+    for (int a = 0; a < 3; ++a) r.f[a] = pos[a];
+    for (int k = 0; k < 3; ++k) {
+        r.f[3 + 3 * k] = cols[k].x;
+        r.f[4 + 3 * k] = cols[k].y;
+        r.f[5 + 3 * k] = cols[k].z;
+    }
+    r.f[12] = size[0];
+    r.f[13] = size[1];
+    if (tc) {
+        r.f[14] = tc[0];
+        r.f[15] = tc[1];
+        r.f[16] = tc[2] - tc[0];
+        r.f[17] = tc[3] - tc[1];
+    } else {
+        r.f[16] = 1.0f;
+        r.f[17] = 1.0f;
+    }
+    double lo[3], hi[3];
+    const double ext[3] = {std::fabs((double)size[0] * o.c0.x) + std::fabs((double)size[1] * o.c1.x),
+                           std::fabs((double)size[0] * o.c0.y) + std::fabs((double)size[1] * o.c1.y),
+                           std::fabs((double)size[0] * o.c0.z) + std::fabs((double)size[1] * o.c1.z)};
+    for (int a = 0; a < 3; ++a) { lo[a] = pos[a] - ext[a]; hi[a] = pos[a] + ext[a]; }
+    const uint32_t slot = add_shape_prim(s, r, material, lo, hi);
+    if (out) *out = slot;
+    return TR_OK;
+}
+
+tr_status shape_common(tr_scene* s, const tr_material* m)
+{
+    if (s->committed) return fail(TR_ERROR_INVALID_ARG, "scene already committed");
+    if (m && !model_supported(m->model)) return fail(TR_ERROR_UNSUPPORTED, "illumination model not supported by the GPU tracer");
+    return TR_OK;
+}
+
+/* The cylinder's box: the axis moves by bending * dz over the height
+ * (ObliqueCylinderBoundsFunction, ray-tracing.cc:981-990, widened by
+ * |Tilt * TiltDirection| per axis instead of Tilt). */
+void cylinder_box(const ShapeRec& r, double tx, double ty, double lo[3], double hi[3])
+{
+    const double ext[3] = {(double)r.f[3] + std::fabs(tx), (double)r.f[3] + std::fabs(ty), (double)r.f[4]};
+    for (int a = 0; a < 3; ++a) { lo[a] = r.f[a] - ext[a]; hi[a] = r.f[a] + ext[a]; }
+}
+
+}  // namespace
+
+tr_status tr_scene_add_rect(tr_scene* s, const float pos[3], const float tan[3], const float norm[3],
+                            const float size[2], const tr_material* m, const float tc_box[4], uint32_t tangent_map,
+                            uint64_t* out_geom_id)
+{
+    if (!valid(s) || !pos || !tan || !norm || !size || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    tr_status st = shape_common(s, m);
+    if (st != TR_OK) return st;
+    if (tangent_map > s->tex_recs.size()) return fail(TR_ERROR_INVALID_ARG, "rect: unknown tangent-map texture");
+    return add_rect_shape(s, pos, tan, norm, size, intern_material(s, m), tc_box, tangent_map, out_geom_id);
+}
+
+tr_status tr_scene_add_blocker(tr_scene* s, const float pos[3], const float tan[3], const float norm[3],
+                               const float size[2], uint64_t* out_geom_id)
+{
+    if (!valid(s) || !pos || !tan || !norm || !size) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    tr_status st = shape_common(s, nullptr);
+    if (st != TR_OK) return st;
+    return add_rect_shape(s, pos, tan, norm, size, kNoMaterial, nullptr, 0, out_geom_id);
+}
+
+tr_status tr_scene_add_disk(tr_scene* s, const float pos[3], const float norm[3], float inner_radius,
+                            float outer_radius, const tr_material* m, uint64_t* out_geom_id)
+{
+    if (!valid(s) || !pos || !norm || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    tr_status st = shape_common(s, m);
+    if (st != TR_OK) return st;
+    const double nl = std::sqrt((double)norm[0] * norm[0] + (double)norm[1] * norm[1] + (double)norm[2] * norm[2]);
+    if (!finite3(pos) || !finite3(norm) || !(nl > 0.0) || !(inner_radius >= 0.0f) || !(outer_radius > inner_radius) ||
+        !std::isfinite(outer_radius))
+        return fail(TR_ERROR_INVALID_ARG, "disk: finite centre, non-zero normal and 0 <= inner < outer radius needed");
+    ShapeRec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = SHAPE_DISK;
+    for (int a = 0; a < 3; ++a) { r.f[a] = pos[a]; r.f[3 + a] = norm[a]; }
+    r.f[6] = inner_radius;
+    r.f[7] = outer_radius;
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const double na = norm[a] / nl;
+        const double e = outer_radius * std::sqrt(std::max(0.0, 1.0 - na * na));
+        lo[a] = pos[a] - e;
+        hi[a] = pos[a] + e;
+    }
+    const uint32_t slot = add_shape_prim(s, r, intern_material(s, m), lo, hi);
+    if (out_geom_id) *out_geom_id = slot;
+    return TR_OK;
+}
+
+tr_status tr_scene_add_cylinder(tr_scene* s, const float center[3], float radius, float half_height,
+                                const tr_material* m, uint64_t* out_geom_id)
+{
+    if (!valid(s) || !center || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    tr_status st = shape_common(s, m);
+    if (st != TR_OK) return st;
+    if (!finite3(center) || !(radius > 0.0f) || !(half_height > 0.0f) || !std::isfinite(radius) ||
+        !std::isfinite(half_height))
+        return fail(TR_ERROR_INVALID_ARG, "cylinder: finite centre, positive radius and half height needed");
+    ShapeRec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = SHAPE_CYLINDER;
+    for (int a = 0; a < 3; ++a) r.f[a] = center[a];
+    r.f[3] = radius;
+    r.f[4] = half_height;
+    double lo[3], hi[3];
+    cylinder_box(r, 0.0, 0.0, lo, hi);
+    const uint32_t slot = add_shape_prim(s, r, intern_material(s, m), lo, hi);
+    if (out_geom_id) *out_geom_id = slot;
+    return TR_OK;
+}
+
+tr_status tr_scene_add_oblique_cylinder(tr_scene* s, const float center[3], float radius, float half_height,
+                                        const float tilt_direction[2], float tilt, const tr_material* m,
+                                        uint64_t* out_geom_id)
+{
+    if (!valid(s) || !center || !tilt_direction || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    tr_status st = shape_common(s, m);
+    if (st != TR_OK) return st;
+    if (!finite3(center) || !(radius > 0.0f) || !(half_height > 0.0f) || !std::isfinite(radius) ||
+        !std::isfinite(half_height) || !std::isfinite(tilt) || !std::isfinite(tilt_direction[0]) ||
+        !std::isfinite(tilt_direction[1]))
+        return fail(TR_ERROR_INVALID_ARG, "oblique cylinder: finite parameters, positive radius and half height needed");
+    ShapeRec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = SHAPE_OBLIQUE_CYLINDER;
+    for (int a = 0; a < 3; ++a) r.f[a] = center[a];
+    r.f[3] = radius;
+    r.f[4] = half_height;
+    /* bending = Tilt * TiltDirection / HalfHeight (ray-tracing.cc:999) */
+    r.f[5] = (tilt * tilt_direction[0]) / half_height;
+    r.f[6] = (tilt * tilt_direction[1]) / half_height;
+    double lo[3], hi[3];
+    cylinder_box(r, (double)tilt * tilt_direction[0], (double)tilt * tilt_direction[1], lo, hi);
+    const uint32_t slot = add_shape_prim(s, r, intern_material(s, m), lo, hi);
+    if (out_geom_id) *out_geom_id = slot;
+    return TR_OK;
+}
+
+tr_status tr_scene_add_ellipsoid(tr_scene* s, const float center[3], const float scale[3],
+                                 const float orientation[4], const tr_material* m, uint64_t* out_geom_id)
+{
+    if (!valid(s) || !center || !scale || !orientation || !m) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    tr_status st = shape_common(s, m);
+    if (st != TR_OK) return st;
+    const double ql = std::sqrt((double)orientation[0] * orientation[0] + (double)orientation[1] * orientation[1] +
+                                (double)orientation[2] * orientation[2] + (double)orientation[3] * orientation[3]);
+    if (!finite3(center) || !finite3(scale) || !(scale[0] > 0.0f) || !(scale[1] > 0.0f) || !(scale[2] > 0.0f) ||
+        !(ql > 0.0) || !std::isfinite(ql))
+        return fail(TR_ERROR_INVALID_ARG, "ellipsoid: finite centre, positive scale and a non-zero quaternion needed");
+    ShapeRec r;
+    memset(&r, 0, sizeof(r));
+    r.kind = SHAPE_ELLIPSOID;
+    for (int a = 0; a < 3; ++a) { r.f[a] = center[a]; r.f[3 + a] = scale[a]; }
+    for (int k = 0; k < 4; ++k) r.f[6 + k] = orientation[k];
+    /* Inverse = Conjugate (quaternion.hh:215-241) */
+    r.f[10] = -orientation[0];
+    r.f[11] = -orientation[1];
+    r.f[12] = -orientation[2];
+    r.f[13] = orientation[3];
+    /* ToNormal (quaternion.hh:170-181) */
+    {
+        const float x = orientation[0], y = orientation[1], z = orientation[2], w = orientation[3];
+        const float xx = x * x, xz = x * z, xw = x * w, yy = y * y, yz = y * z, yw = y * w;
+        r.f[14] = 2.0f * (xz + yw);
+        r.f[15] = 2.0f * (yz - xw);
+        r.f[16] = 1.0f - 2.0f * (xx + yy);
+    }
+    /* box: the surface is c + L^-1 S u, |u| = 1, with L the linear map
+     * v -> Transform(conj q, v); half extent per axis = row norm of L^-1 S */
+    double L[3][3];
+    for (int j = 0; j < 3; ++j) {
+        const double e[3] = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0};
+        const double qv[3] = {-(double)orientation[0], -(double)orientation[1], -(double)orientation[2]};
+        const double w = orientation[3];
+        const double c1[3] = {qv[1] * e[2] - qv[2] * e[1] + w * e[0], qv[2] * e[0] - qv[0] * e[2] + w * e[1],
+                              qv[0] * e[1] - qv[1] * e[0] + w * e[2]};
+        const double c2[3] = {qv[1] * c1[2] - qv[2] * c1[1], qv[2] * c1[0] - qv[0] * c1[2], qv[0] * c1[1] - qv[1] * c1[0]};
+        for (int i = 0; i < 3; ++i) L[i][j] = e[i] + 2.0 * c2[i];
+    }
+    const double det = L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) - L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0]) +
+                       L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]);
+    if (!(std::fabs(det) > 1e-12)) return fail(TR_ERROR_INVALID_ARG, "ellipsoid: degenerate orientation");
+    double inv[3][3];
+    inv[0][0] = (L[1][1] * L[2][2] - L[1][2] * L[2][1]) / det;
+    inv[0][1] = (L[0][2] * L[2][1] - L[0][1] * L[2][2]) / det;
+    inv[0][2] = (L[0][1] * L[1][2] - L[0][2] * L[1][1]) / det;
+    inv[1][0] = (L[1][2] * L[2][0] - L[1][0] * L[2][2]) / det;
+    inv[1][1] = (L[0][0] * L[2][2] - L[0][2] * L[2][0]) / det;
+    inv[1][2] = (L[0][2] * L[1][0] - L[0][0] * L[1][2]) / det;
+    inv[2][0] = (L[1][0] * L[2][1] - L[1][1] * L[2][0]) / det;
+    inv[2][1] = (L[0][1] * L[2][0] - L[0][0] * L[2][1]) / det;
+    inv[2][2] = (L[0][0] * L[1][1] - L[0][1] * L[1][0]) / det;
+    double lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        double e2 = 0.0;
+        for (int j = 0; j < 3; ++j) e2 += (inv[i][j] * scale[j]) * (inv[i][j] * scale[j]);
+        const double e = std::sqrt(e2) * (1.0 + 1e-5);
+        lo[i] = center[i] - e;
+        hi[i] = center[i] + e;
+    }
+    const uint32_t slot = add_shape_prim(s, r, intern_material(s, m), lo, hi);
+    if (out_geom_id) *out_geom_id = slot;
     return TR_OK;
 }
 
@@ -1311,6 +1606,9 @@ tr_status tr_scene_commit(tr_scene* s)
             tr_status ms = check_maps(s, m);
             if (ms != TR_OK) return ms;
         }
+        for (const ShapeRec& r : s->shapes)
+            if (r.tangent_map && !texel_kind_ok(s->tex_recs[r.tangent_map - 1].format, TEX_RGB))
+                return fail(TR_ERROR_INVALID_ARG, "rect tangent map: a format sampleRGB does not read");
         if (s->cube_side) {
             s->cube_offset = (s->texels.size() + 15) & ~(size_t)15;
             s->texels.resize(s->cube_offset + s->cube.size());
@@ -1335,6 +1633,7 @@ tr_status tr_scene_commit(tr_scene* s)
     if ((st = upload(s, s->d_ipool, s->ipool)) != TR_OK) return st;
     if ((st = upload(s, s->d_vols, s->vols)) != TR_OK) return st;
     if ((st = upload(s, s->d_voxels, s->voxels)) != TR_OK) return st;
+    if ((st = upload(s, s->d_shapes, s->shapes)) != TR_OK) return st;
     if ((st = upload(s, s->d_tex, s->tex_recs)) != TR_OK) return st;
     if ((st = upload(s, s->d_texels, s->texels)) != TR_OK) return st;
     srgb_table(&s->srgb);
@@ -1487,6 +1786,37 @@ tr_status tr_scene_trace_rays_noalloc(tr_scene* s, size_t n, const tr_ray* devic
     return TR_OK;
 }
 
+tr_status tr_scene_ray_query(tr_scene* s, const float tc[2], tr_sample_data* sd, uint64_t* out_geom_id)
+{
+    if (!valid(s) || !tc || !sd || !out_geom_id) return fail(TR_ERROR_INVALID_ARG, "null argument");
+    if (!s->committed || !s->device_ready) return fail(TR_ERROR_NOT_COMMITTED, "commitScene has not been called");
+    if (!std::isfinite(tc[0]) || !std::isfinite(tc[1])) return fail(TR_ERROR_INVALID_ARG, "non-finite screen point");
+    if (s->frame_pending) {
+        tr_status st = tr_scene_frame_wait(s);
+        if (st != TR_OK) return st;
+    }
+    TR_HIP(s->query_out.reserve(1));
+    launch_ray_query(s->stream, scene_view(s), s->vpi, tc[0], tc[1], s->query_out.ptr);
+    TR_HIP(hipGetLastError());
+    RayQueryOut o;
+    TR_HIP(hipMemcpyAsync(&o, s->query_out.ptr, sizeof(o), hipMemcpyDeviceToHost, s->stream));
+    TR_HIP(hipStreamSynchronize(s->stream));
+    if (o.geom == TR_INVALID_GEOMETRY) {
+        *out_geom_id = TR_INVALID_GEOMETRY;
+        return TR_OK;
+    }
+    memset(sd, 0, sizeof(*sd));
+    const GeomRec& g = s->geoms[o.geom];
+    sd->material = g.material == kNoMaterial ? nullptr : s->mat_keys[g.material];
+    memcpy(sd->normal, o.normal, sizeof(sd->normal));
+    memcpy(sd->tangent, o.tangent, sizeof(sd->tangent));
+    memcpy(sd->binormal, o.binormal, sizeof(sd->binormal));
+    memcpy(sd->texcoord, o.tc, sizeof(sd->texcoord));
+    sd->t = o.t;
+    *out_geom_id = o.geom < s->ext_ids.size() && s->ext_ids[o.geom] != UINT64_MAX ? s->ext_ids[o.geom] : o.geom;
+    return TR_OK;
+}
+
 tr_status tr_scene_render_paths(tr_scene* s, size_t pixel_count, const uint32_t* pixels, float* out_rgb,
                                 uint32_t* out_hash)
 {
@@ -1583,6 +1913,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
     if (count >= (1u << 24)) return fail(TR_ERROR_INVALID_ARG, "at most 2^24 - 1 requests");
     if (s->lights.empty() || s->lights[0].src.type != TR_LIGHT_DIRECTIONAL)
         return fail(TR_ERROR_INVALID_ARG, "render requests turn light 0, which must be a directional light");
+    if (s->render_mode != 0) return fail(TR_ERROR_UNSUPPORTED, "render requests take the Normal render mode only");
     if (!s->vols.empty()) return fail(TR_ERROR_UNSUPPORTED, "render requests do not take grid volumes");
     if (s->cache.enabled) return fail(TR_ERROR_UNSUPPORTED, "render requests do not take the radiance cache");
     tr_request_options o;
@@ -1841,7 +2172,7 @@ tr_status tr_debug_trace_rays_host(const tr_scene* s, size_t n, const tr_ray* ra
         RayPre r = ray_setup(rays[i].org, rays[i].dir);
         stack->sp = 0;
         uint32_t v = 0, p = 0;
-        Hit h = traverse_closest(s->nodes.data(), s->prims.data(), r, rays[i].tnear, rays[i].tfar, *stack, &v, &p);
+        Hit h = traverse_closest(s->nodes.data(), s->prims.data(), s->shapes.data(), r, rays[i].tnear, rays[i].tfar, *stack, &v, &p);
         nv += v;
         np += p;
         hits[i].t = h.t; hits[i].u = h.u; hits[i].v = h.v; hits[i].geom = h.geom; hits[i].prim = h.prim;
@@ -1879,16 +2210,10 @@ tr_status tr_debug_bvh_check(const tr_scene* s, uint64_t* violations)
                     if (first + k >= s->prims.size()) { ++bad; continue; }
                     seen[first + k]++;
                     const PrimRec& p = s->prims[first + k];
-                    for (int a = 0; a < 3; ++a) {
-                        float pl, ph;
-                        if (p.flags & PRIM_SPHERE) { pl = p.v0[a] - p.v1[0]; ph = p.v0[a] + p.v1[0]; }
-                        else if (p.flags & PRIM_VOLUME) { pl = p.v0[a]; ph = p.v1[a]; }
-                        else {
-                            pl = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
-                            ph = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));
-                        }
-                        if (pl < lo[a] || ph > hi[a]) ++bad;
-                    }
+                    float pl[3], ph[3];
+                    prim_box(p, pl, ph);
+                    for (int a = 0; a < 3; ++a)
+                        if (pl[a] < lo[a] || ph[a] > hi[a]) ++bad;
                 }
             }
         }

@@ -260,16 +260,7 @@ bool build_bvh8(const std::vector<PrimRec>& in, std::vector<Node8>& nodes_out, s
     for (uint32_t i = 0; i < n; ++i) {
         const PrimRec& p = in[i];
         Box bx;
-        if (p.flags & PRIM_SPHERE) {
-            for (int a = 0; a < 3; ++a) { bx.lo[a] = p.v0[a] - p.v1[0]; bx.hi[a] = p.v0[a] + p.v1[0]; }
-        } else if (p.flags & PRIM_VOLUME) {
-            for (int a = 0; a < 3; ++a) { bx.lo[a] = p.v0[a]; bx.hi[a] = p.v1[a]; }
-        } else {
-            for (int a = 0; a < 3; ++a) {
-                bx.lo[a] = std::min(p.v0[a], std::min(p.v1[a], p.v2[a]));
-                bx.hi[a] = std::max(p.v0[a], std::max(p.v1[a], p.v2[a]));
-            }
-        }
+        prim_box(p, bx.lo, bx.hi);
         /* relative outward padding: slack for rounding in the slab test */
         for (int a = 0; a < 3; ++a) {
             float pad = (std::fabs(bx.lo[a]) + std::fabs(bx.hi[a]) + (bx.hi[a] - bx.lo[a])) * 7.62939453e-06f + 1e-30f;

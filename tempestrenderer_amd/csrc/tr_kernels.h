@@ -67,6 +67,8 @@ struct FrameParams {
     float box_lo[3];         /* scene box for the position input */
     float box_rcp[3];
     ReqView req;             /* request mode (req.n > 0) */
+    uint32_t render_mode;    /* RenderMode (ray-tracing-common.hh:132-139): 0 Normal, 1-3 debug tangents /
+                                binormals / normals, 4 debug lighting */
 };
 
 /* Path flags of cache mode: training path, next vertex is the query
@@ -128,6 +130,7 @@ struct BatchOut {
     uint32_t* path_hash;  /* per camera sample hit hash, may be null */
     const uint32_t* pixel_list; /* explicit pixel list, or null for pixel-major order */
     uint64_t* count_out;  /* node/prim counters when counting, may be null */
+    uint8_t* hitmask;     /* per pixel: 1 if a camera ray of the frame hit geometry (TransparentBackground), may be null */
 };
 
 /* Timing mode: the launch records `start` and `stop` as part of the kernel
@@ -199,8 +202,23 @@ void launch_trace_rays(hipStream_t s, const SceneView& sc, const tr_ray* rays, t
  * pixels) and write their exposure to `exposure`. */
 void launch_pseudo_volume_image(hipStream_t s, const tr_material* m, uint32_t width, uint32_t height, uint32_t* rgba,
                                 float* probe);
+/* rayQuery: the closest hit of the ray through a screen point and its
+ * geometry's frame (zero for blockers and volumes). */
+struct Mat16 {
+    float m[16];
+};
+struct RayQueryOut {
+    float t, u, v;
+    uint32_t geom, prim;
+    float normal[3], tangent[3], binormal[3], tc[2];
+};
+void launch_ray_query(hipStream_t s, const SceneView& sc, const float vpi[16], float tx, float ty, RayQueryOut* out);
+/* hitmask (TransparentBackground, may be null): pixels whose camera rays
+ * all missed keep their backbuffer value and stay out of the exposure's
+ * log sum (ray-tracing.cc:2364-2365). */
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba,
-                    uint32_t n_pixels, uint32_t post, double* partial, float* exposure);
+                    uint32_t n_pixels, uint32_t post, double* partial, float* exposure,
+                    const uint8_t* hitmask = nullptr);
 
 }  // namespace trk
 

@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
             float org[3] = {o.x, o.y, o.z}, dir[3] = {d.x, d.y, d.z};
             RayPre r = ray_setup(org, dir);
             st.sp = 0;
-            Hit h = traverse_closest(sc.nodes, sc.prims, r, 0.0f, INFINITY, st, COUNT ? &nv : nullptr,
+            Hit h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, 0.0f, INFINITY, st, COUNT ? &nv : nullptr,
                                      COUNT ? &np : nullptr);
             if (VOL) {
                 /* the tracking loops of subdraw (ray-tracing.cc:2040-2128, 2212-2272) */
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
                         break;
                     }
                     st.sp = 0;
-                    h = traverse_closest(sc.nodes, sc.prims, r, tc, INFINITY, st, COUNT ? &nv : nullptr,
+                    h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, tc, INFINITY, st, COUNT ? &nv : nullptr,
                                          COUNT ? &np : nullptr);
                 }
             }
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                 const uint32_t count = meta >> 5;
                 for (uint32_t k = 0; k < count; ++k) {
                     if (COUNT) ++np;
-                    test_prim(load_prim(sc.prims, first + k), r, 0.0f, INFINITY, &best);
+                    test_prim(load_prim(sc.prims, first + k), sc.shapes, r, 0.0f, INFINITY, &best);
                 }
             }
             if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
@@ -462,6 +462,18 @@ template <> struct TexSel<false> {
     typedef NoTex type;
     __device__ static NoTex get(const SceneView&) { return NoTex(); }
 };
+
+/* The debug outputs of RenderMode at the camera vertex (ray-tracing.cc:
+ * 2078-2089): tangent, binormal or normal * 0.5 + 0.5, or max(0, N . -dir). */
+__device__ inline V3 debug_color(uint32_t mode, V3 tangent, V3 binormal, V3 normal, V3 dir)
+{
+    if (mode == 4u) {
+        const float c = fmax_(0.0f, dot(normal, neg(dir)));
+        return mk(c, c, c);
+    }
+    const V3 v = mode == 1u ? tangent : (mode == 2u ? binormal : normal);
+    return add(muls(v, 0.5f), mk(0.5f, 0.5f, 0.5f));
+}
 
 template <bool VOL, bool CACHE, bool TEX>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
@@ -527,6 +539,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
             V3 env = (TEX && sc.tex.cube_side) ? cube_sample(sc.tex, dir) : ld3(p.background);
             if (vertex == 0) { local = env; write_local = true; }
             else global = add(global, mulv(env, T));
+        } else if (sc.geoms[geom].material == kNoMaterial) {
+            /* a blocker: the path ends with nothing (ray-tracing.cc:2072-2077,
+             * 2236-2240); its sample() frame is zero (:1239-1244) */
+            hash = fnv_hit(hash, geom, pool.hit_prim[slot]);
+            if (vertex == 0 && p.render_mode) {
+                const V3 z = mk(0.0f, 0.0f, 0.0f);
+                local = debug_color(p.render_mode, z, z, z, dir);
+            }
         } else {
             uint32_t prim = pool.hit_prim[slot];
             hash = fnv_hit(hash, geom, prim);
@@ -557,7 +577,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                 sd.wi = dir;
                 material_cache(&sd, tv);
             }
-            if (volume && T.x < 1e-3f && T.y < 1e-3f && T.z < 1e-3f) {
+            if (!VOL && vertex == 0 && p.render_mode) {
+                /* RenderMode debug output instead of shading (the camera ray ends) */
+                local = debug_color(p.render_mode, sd.tangent, sd.binormal, sd.normal, dir);
+            } else if (volume && T.x < 1e-3f && T.y < 1e-3f && T.z < 1e-3f) {
                 /* absorbed: no contribution from this vertex on */
             } else if (is_emissive(sd.mat->model)) {
                 V3 le = ld3(sd.mat->diffuse);
@@ -701,7 +724,8 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         }
         pool.hash[slot] = hash;
         pool.global[slot] = st4(global, 0.0f);
-        if (write_local) pool.local[slot] = st4(local, 0.0f);
+        /* .w: the camera ray hit geometry (TransparentBackground) */
+        if (write_local) pool.local[slot] = st4(local, geom != TR_INVALID_GEOMETRY ? 1.0f : 0.0f);
     }
     const uint32_t qpos = segment_push(pool, extend, shadow, cache_query, slot);
     if (CACHE && cache_query) {
@@ -964,7 +988,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                     RayPre r = ray_setup(org, dir);
                     uint32_t light_geom = __float_as_uint(d4.w);
                     st.sp = 0;
-                    Hit h = traverse_closest(sc.nodes, sc.prims, r, 0.0f, o4.w, st, COUNT ? &nv : nullptr,
+                    Hit h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, 0.0f, o4.w, st, COUNT ? &nv : nullptr,
                                              COUNT ? &np : nullptr);
                     if (!VOL) {
                         if (h.geom == TR_INVALID_GEOMETRY || h.geom == light_geom) color = add(color, ld4(c4));
@@ -993,7 +1017,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                             }
                             float tn = dist - 1e-3f < step_through ? dist - 1e-3f : step_through;
                             st.sp = 0;
-                            h = traverse_closest(sc.nodes, sc.prims, r, tn, dist, st, COUNT ? &nv : nullptr,
+                            h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, tn, dist, st, COUNT ? &nv : nullptr,
                                                  COUNT ? &np : nullptr);
                         }
                         if (reached) color = add(color, muls(ld4(c4), tf));
@@ -1004,7 +1028,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                 radiance = add(radiance, divs(color, (float)samples));
             }
             if (vertex == 0) {
-                pool.local[slot] = st4(radiance, 0.0f);
+                /* .w = 1: the camera ray hit geometry (it has shadow rays) */
+                pool.local[slot] = st4(radiance, 1.0f);
             } else {
                 float4 g = pool.global[slot];
                 pool.global[slot] = st4(add(ld4(g), radiance), 0.0f);
@@ -1033,6 +1058,7 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
     double sum[3] = {0.0, 0.0, 0.0};
     float rcp_gi = 1.0f / (float)p.n_gi;
     uint32_t c = 0;
+    bool any_hit = false;
     if (p.n_gi == 1 && !out.path_rgb) {
         /* runs of 8 consecutive slots: each lane consumes whole cache lines
          * of its pixel back to back (same sums, same order) */
@@ -1050,6 +1076,7 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
                 sum[0] += (double)v.x;
                 sum[1] += (double)v.y;
                 sum[2] += (double)v.z;
+                any_hit |= l[u].w != 0.0f;
             }
         }
     }
@@ -1061,7 +1088,9 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
             g = add(g, ld4(pool.global[s0 + j]));
             h = (h * 0x9E3779B1u) ^ pool.hash[s0 + j];
         }
-        V3 v = add(ld4(pool.local[s0]), muls(g, rcp_gi));
+        const float4 l0 = pool.local[s0];
+        any_hit |= l0.w != 0.0f;
+        V3 v = add(ld4(l0), muls(g, rcp_gi));
         sum[0] += (double)v.x;
         sum[1] += (double)v.y;
         sum[2] += (double)v.z;
@@ -1073,6 +1102,7 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
             if (out.path_hash) out.path_hash[k] = h;
         }
     }
+    if (out.hitmask) out.hitmask[pixel] = any_hit ? 1u : 0u;
     if (out.accum) {
         out.accum[(size_t)pixel * 3 + 0] += sum[0];
         out.accum[(size_t)pixel * 3 + 1] += sum[1];
@@ -1102,12 +1132,58 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView sc, const 
             float org[3] = {a.x, a.y, a.z}, dir[3] = {b.x, b.y, b.z};
             RayPre r = ray_setup(org, dir);
             st.sp = 0;
-            Hit h = traverse_closest(sc.nodes, sc.prims, r, a.w, b.w, st, (uint32_t*)nullptr, (uint32_t*)nullptr);
+            Hit h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, a.w, b.w, st, (uint32_t*)nullptr, (uint32_t*)nullptr);
             tr_hit o;
             o.t = h.t; o.u = h.u; o.v = h.v; o.geom = h.geom; o.prim = h.prim;
             hits[i] = o;
         }
     }
+}
+
+/* ---------------- rayQuery ---------------- */
+
+/* RayTracerScene::rayQuery (ray-tracing.cc:1932-1979): the ray through
+ * screen point (2 tc.x - 1, 1 - 2 tc.y) from the near to the far plane,
+ * its closest hit and the hit geometry's sample() frame.  One thread. */
+__global__ void k_ray_query(SceneView sc, Mat16 vpi, float tx, float ty, RayQueryOut* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float sx = 2.0f * tx - 1.0f, sy = 1.0f - 2.0f * ty;
+    const float* m = vpi.m;
+    float p0[4], p1[4];
+    for (int r = 0; r < 4; ++r) {
+        p0[r] = m[r] * sx;
+        p1[r] = m[r] * sx;
+        p0[r] = p0[r] + m[4 + r] * sy;
+        p1[r] = p1[r] + m[4 + r] * sy;
+        p0[r] = p0[r] + m[8 + r] * -1.0f;
+        p1[r] = p1[r] + m[8 + r] * 1.0f;
+        p0[r] = p0[r] + m[12 + r] * 1.0f;
+        p1[r] = p1[r] + m[12 + r] * 1.0f;
+    }
+    const float w0 = 1.0f / p0[3], w1 = 1.0f / p1[3];
+    const V3 start = mk(p0[0] * w0, p0[1] * w0, p0[2] * w0);
+    const V3 end = mk(p1[0] * w1, p1[1] * w1, p1[2] * w1);
+    const V3 dir = normalize(sub(end, start));
+    const float org[3] = {start.x, start.y, start.z}, d[3] = {dir.x, dir.y, dir.z};
+    RayPre r = ray_setup(org, d);
+    HostStack st;
+    Hit h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, 0.0f, INFINITY, st, (uint32_t*)nullptr,
+                             (uint32_t*)nullptr);
+    RayQueryOut o;
+    o.t = h.t; o.u = h.u; o.v = h.v; o.geom = h.geom; o.prim = h.prim;
+    for (int k = 0; k < 3; ++k) o.normal[k] = o.tangent[k] = o.binormal[k] = 0.0f;
+    o.tc[0] = o.tc[1] = 0.0f;
+    if (h.geom != TR_INVALID_GEOMETRY && sc.geoms[h.geom].material != kNoMaterial &&
+        sc.geoms[h.geom].type != GEOM_VOLUME) {
+        Sample sd;
+        sample_geometry<true>(sc, h.geom, h.prim, h.t, h.u, h.v, start, dir, &sd);
+        o.normal[0] = sd.normal.x; o.normal[1] = sd.normal.y; o.normal[2] = sd.normal.z;
+        o.tangent[0] = sd.tangent.x; o.tangent[1] = sd.tangent.y; o.tangent[2] = sd.tangent.z;
+        o.binormal[0] = sd.binormal.x; o.binormal[1] = sd.binormal.y; o.binormal[2] = sd.binormal.z;
+        o.tc[0] = sd.tcx; o.tc[1] = sd.tcy;
+    }
+    *out = o;
 }
 
 /* ---------------- backbuffer ---------------- */
@@ -1173,12 +1249,13 @@ __device__ inline float aces_film(float x)
  * finite pixels; second pass (one block): the blocks' sums in order, then
  * the exposure 0.18 / RGBToLuminance(exp(sum / pixels)) (ray-tracing.cc:
  * 1649-1659, spectrum.hh:328-331). */
-__global__ __launch_bounds__(256) void k_log_sum(const double* accum, double rcp_count, uint32_t n, double* partial)
+__global__ __launch_bounds__(256) void k_log_sum(const double* accum, double rcp_count, uint32_t n, double* partial,
+                                                 const uint8_t* hitmask)
 {
     __shared__ double red[3][256];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double s[3] = {0.0, 0.0, 0.0};
-    if (i < n) {
+    if (i < n && (!hitmask || hitmask[i])) {
         const float v[3] = {(float)(accum[(size_t)i * 3 + 0] * rcp_count), (float)(accum[(size_t)i * 3 + 1] * rcp_count),
                             (float)(accum[(size_t)i * 3 + 2] * rcp_count)};
         if (isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))
@@ -1223,10 +1300,11 @@ __global__ __launch_bounds__(256) void k_exposure(const double* partial, uint32_
  * 32-35) then sRGB; ToColor truncates 255 * clamp(v, 0, 1)
  * (spectrum.hh:792-798). */
 __global__ __launch_bounds__(256) void k_resolve(const double* accum, double rcp_count, uint32_t* rgba, uint32_t n,
-                                                 uint32_t post, const float* exposure)
+                                                 uint32_t post, const float* exposure, const uint8_t* hitmask)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (hitmask && !hitmask[i]) return;
     float c[3] = {(float)(accum[(size_t)i * 3 + 0] * rcp_count), (float)(accum[(size_t)i * 3 + 1] * rcp_count),
                   (float)(accum[(size_t)i * 3 + 2] * rcp_count)};
     if (post == TR_POST_LINEAR) {
@@ -1430,15 +1508,23 @@ hipError_t launch_trace_rays_noalloc(hipStream_t s, const SceneView& sc, const t
     return hipGetLastError();
 }
 
+void launch_ray_query(hipStream_t s, const SceneView& sc, const float vpi[16], float tx, float ty, RayQueryOut* out)
+{
+    Mat16 m;
+    for (int k = 0; k < 16; ++k) m.m[k] = vpi[k];
+    hipLaunchKernelGGL(k_ray_query, dim3(1), dim3(64), 0, s, sc, m, tx, ty, out);
+}
+
 void launch_resolve(hipStream_t s, const double* accum, double rcp_count, uint32_t* rgba, uint32_t n, uint32_t post,
-                    double* partial, float* exposure)
+                    double* partial, float* exposure, const uint8_t* hitmask)
 {
     uint32_t grid = (n + 255) / 256;
     if (post == TR_POST_AUTO_EXPOSURE_SRGB || post == TR_POST_AUTO_EXPOSURE_HDR) {
-        hipLaunchKernelGGL(k_log_sum, dim3(grid), dim3(256), 0, s, accum, rcp_count, n, partial);
+        hipLaunchKernelGGL(k_log_sum, dim3(grid), dim3(256), 0, s, accum, rcp_count, n, partial, hitmask);
         hipLaunchKernelGGL(k_exposure, dim3(1), dim3(256), 0, s, (const double*)partial, grid, n, exposure);
     }
-    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, s, accum, rcp_count, rgba, n, post, (const float*)exposure);
+    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, s, accum, rcp_count, rgba, n, post, (const float*)exposure,
+                       hitmask);
 }
 
 }  // namespace trk

@@ -69,6 +69,14 @@ TR_HD M3 make_basis(V3 n)
     m.c2 = n;
     return m;
 }
+/* Matrix3::makeBasisTangent (matrix3.hh:110-120). */
+TR_HD M3 make_basis_tangent(V3 tan)
+{
+    V3 d0 = cross(mk(0.0f, 1.0f, 0.0f), tan), d1 = cross(mk(0.0f, 0.0f, 1.0f), tan);
+    float d0_len = dot(d0, d0), d1_len = dot(d1, d1);
+    V3 b = d0_len > d1_len ? divs(d0, sqrtf(d0_len)) : divs(d1, sqrtf(d1_len));
+    return m3(tan, b, cross(tan, b));
+}
 TR_HD M3 make_basis_orthogonalize(V3 tan, V3 n)
 {
     V3 binorm = normalize(cross(n, tan));

@@ -45,8 +45,33 @@ struct alignas(128) Node8 {
 static_assert(sizeof(Node8) == 128, "Node8 must be one 128-byte line");
 
 /* Primitive kinds.  A grid volume is one PRIM_VOLUME primitive: its box
- * lo = v0, hi = v1. */
-enum { PRIM_SPHERE = 1u, PRIM_TWO_SIDED = 2u, PRIM_VOLUME = 4u };
+ * lo = v0, hi = v1.  A user shape (PRIM_SHAPE, the shape kind in bits 8-15
+ * of flags) keeps its ShapeRec index in the bits of v0[0] and its box in
+ * v1 (lo) and v2 (hi). */
+enum { PRIM_SPHERE = 1u, PRIM_TWO_SIDED = 2u, PRIM_VOLUME = 4u, PRIM_SHAPE = 8u };
+
+/* User shapes of the scene API (ray-tracing.cc:1315-1446).  Rects and
+ * blockers share the rect test. */
+enum { SHAPE_RECT = 1u, SHAPE_DISK = 2u, SHAPE_CYLINDER = 3u, SHAPE_OBLIQUE_CYLINDER = 4u, SHAPE_ELLIPSOID = 5u };
+
+/* A user shape's parameters, precomputed once on the host in FP32:
+ *   rect:      f[0-2] centre, f[3-11] columns T, B, N of ToMatrix3 of its
+ *              quaternion, f[12-13] half sizes, f[14-15] texture-coordinate
+ *              start, f[16-17] multiplier
+ *   disk:      f[0-2] centre, f[3-5] normal, f[6] inner, f[7] outer radius
+ *   cylinder:  f[0-2] centre, f[3] radius, f[4] half height
+ *   oblique:   as cylinder, f[5-6] bending = Tilt * TiltDirection / HalfHeight
+ *   ellipsoid: f[0-2] centre, f[3-5] scale, f[6-9] orientation (x, y, z, w),
+ *              f[10-13] its inverse (the conjugate), f[14-16] ToNormal of it
+ * tangent_map: the rect's tangent-space texture handle (0 = none). */
+struct alignas(16) ShapeRec {
+    uint32_t kind;
+    uint32_t geom;
+    uint32_t tangent_map;
+    uint32_t pad;
+    float f[20];
+};
+static_assert(sizeof(ShapeRec) == 96, "ShapeRec layout");
 
 struct alignas(16) PrimRec {
     float v0[3];
@@ -58,7 +83,24 @@ struct alignas(16) PrimRec {
 };
 static_assert(sizeof(PrimRec) == 48, "PrimRec must be 48 bytes");
 
-enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2, GEOM_VOLUME = 3 };
+/* A primitive's bounding box (host side: BVH build, scene box, checks). */
+inline void prim_box(const PrimRec& p, float lo[3], float hi[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        if (p.flags & PRIM_SPHERE) { lo[a] = p.v0[a] - p.v1[0]; hi[a] = p.v0[a] + p.v1[0]; }
+        else if (p.flags & PRIM_VOLUME) { lo[a] = p.v0[a]; hi[a] = p.v1[a]; }
+        else if (p.flags & PRIM_SHAPE) { lo[a] = p.v1[a]; hi[a] = p.v2[a]; }
+        else {
+            lo[a] = p.v0[a] < p.v1[a] ? (p.v0[a] < p.v2[a] ? p.v0[a] : p.v2[a]) : (p.v1[a] < p.v2[a] ? p.v1[a] : p.v2[a]);
+            hi[a] = p.v0[a] > p.v1[a] ? (p.v0[a] > p.v2[a] ? p.v0[a] : p.v2[a]) : (p.v1[a] > p.v2[a] ? p.v1[a] : p.v2[a]);
+        }
+    }
+}
+
+/* GEOM_SHAPE: a user shape, its ShapeRec at index_base; a blocker
+ * (addBlocker) has material kNoMaterial and ends every path that hits it. */
+enum { GEOM_MESH = 0, GEOM_SPHERE = 1, GEOM_SPHERE_LIGHT = 2, GEOM_VOLUME = 3, GEOM_SHAPE = 4 };
+constexpr uint32_t kNoMaterial = 0xffffffffu;
 
 struct GeomRec {
     uint32_t type;
@@ -136,6 +178,7 @@ struct SceneView {
     const int32_t* ipool;
     const VolumeRec* volumes;
     const uint64_t* voxels; /* packed binary16 x4 samples of every grid volume */
+    const ShapeRec* shapes;
     uint32_t n_lights;
     uint32_t n_prims;
     uint32_t n_volumes;

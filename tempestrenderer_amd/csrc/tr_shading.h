@@ -19,6 +19,7 @@
 #include "tr_math.h"
 #include "tr_scene.h"
 #include "tr_texture.h"
+#include "tr_traverse.h"
 
 namespace trk {
 
@@ -808,6 +809,69 @@ TR_HD V3 sphere_normal(V3 org, V3 dir, const float* c, float t)
     return normalize(sub(smul(t, dir), dc));
 }
 
+/* The sample() of the user shapes: the geometric normal is the one the
+ * shape's intersection wrote (recomputed from the same ray, so the same
+ * bits).
+ *  - RectGeometry::sample (ray-tracing.cc:1246-1272): TexCoord = start +
+ *    (u, v) * multiplier; the frame of ToMatrix3(Orientation), or with a
+ *    tangent map Orientation * makeBasisTangent(normalized sampleRGB).
+ *  - DiskGeometry::sample (:1274-1283): Binormal towards the centre.
+ *  - CylinderGeometry::sample (:1142-1164): on a cap the tangent points
+ *    away from the axis, on the side it is +z; Binormal = Normal x Tangent.
+ *    Oblique cylinders take the same frame around their own normal (their
+ *    reference sample() leaves it zero, :1210-1237).
+ *  - EllipsoidGeometry::sample (:1182-1208): Binormal = ToNormal(q) x N,
+ *    or makeBasis(N) where the two are parallel. */
+template <bool TC>
+TR_HD void shape_sample(const SceneView& sc, const ShapeRec& sh, float t, float u, float v, V3 org, V3 dir, Sample* sd)
+{
+    float st, su, sv;
+    V3 n = mk(0.0f, 0.0f, 1.0f);
+    shape_intersect(sh, org, dir, &st, &su, &sv, &n);
+    sd->tcx = 0.0f;
+    sd->tcy = 0.0f;
+    const float* f = sh.f;
+    if (sh.kind == SHAPE_RECT) {
+        sd->tcx = f[14] + u * f[16];
+        sd->tcy = f[15] + v * f[17];
+        M3 o = m3(mk(f[3], f[4], f[5]), mk(f[6], f[7], f[8]), mk(f[9], f[10], f[11]));
+        if (TC && sh.tangent_map) {
+            V4f c = tex_sample(sc.tex, sh.tangent_map, sd->tcx, sd->tcy, TEX_RGB);
+            o = mul(o, make_basis_tangent(normalize(mk(c.x, c.y, c.z))));
+        }
+        sd->tangent = o.c0;
+        sd->binormal = o.c1;
+        sd->normal = o.c2;
+    } else if (sh.kind == SHAPE_DISK) {
+        sd->binormal = normalize(sub(mk(f[0], f[1], f[2]), add(smul(t, dir), org)));
+        sd->tangent = cross(sd->binormal, n);
+        sd->normal = n;
+    } else if (sh.kind == SHAPE_ELLIPSOID) {
+        const V3 qn = mk(f[14], f[15], f[16]);
+        sd->normal = n;
+        if (fabsf(dot(qn, n)) < 1e-6f) {
+            const M3 b = make_basis(n);
+            sd->binormal = b.c1;
+            sd->tangent = b.c0;
+        } else {
+            sd->binormal = normalize(cross(qn, n));
+            sd->tangent = cross(sd->binormal, sd->normal);
+        }
+    } else {
+        sd->normal = n;
+        if (fabsf(n.z) == 1.0f) {
+            float px = org.x + t * dir.x, py = org.y + t * dir.y;
+            float tx = px - f[0], ty = py - f[1];
+            const float l = sqrtf(tx * tx + ty * ty);
+            if (l != 0.0f) { tx = tx / l; ty = ty / l; }
+            sd->tangent = mk(tx, ty, 0.0f);
+        } else {
+            sd->tangent = mk(0.0f, 0.0f, 1.0f);
+        }
+        sd->binormal = cross(n, sd->tangent);
+    }
+}
+
 /* RTMesh::sample + DefaultMeshSample (ray-tracing.cc:375-461),
  * SphereGeometry::sample (:1166-1180), SphereAreaLightGeometry::sample
  * (:1775-1783).  Tangent frames of vertex formats without tangents are
@@ -852,6 +916,8 @@ TR_HD void sample_geometry(const SceneView& sc, uint32_t geom, uint32_t prim, fl
             sd->binormal = b.c1;
         }
         sd->normal = norm;
+    } else if (g.type == GEOM_SHAPE) {
+        shape_sample<TC>(sc, sc.shapes[g.index_base], t, u, v, org, dir, sd);
     } else if (g.type == GEOM_SPHERE) {
         V3 n = sphere_normal(org, dir, g.sphere, t);
         sd->normal = n;

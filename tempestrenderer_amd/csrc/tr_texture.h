@@ -24,7 +24,10 @@
 
 namespace trk {
 
-enum { TEX_SPECTRUM = 0, TEX_RG = 1, TEX_RGBA = 2 };
+/* Extractors: SampleSpectrum (8-bit channels through the sRGB decode),
+ * SampleRG, SampleRGBA and sampleRGB (texture.hh:557-560; 8-bit channels as
+ * byte / 255, texture.hh:203-240). */
+enum { TEX_SPECTRUM = 0, TEX_RG = 1, TEX_RGBA = 2, TEX_RGB = 3 };
 
 struct V4f {
     float x, y, z, w;
@@ -50,6 +53,7 @@ TR_HD bool texel_kind_ok(uint32_t fmt, int kind)
     if (kind == TEX_SPECTRUM) return fmt == TR_TEX_RGBA8_UNORM || fmt == TR_TEX_RGB32F || fmt == TR_TEX_RGBA32F;
     if (kind == TEX_RG)
         return fmt == TR_TEX_RG8_UNORM || fmt == TR_TEX_RG32F || fmt == TR_TEX_RGB32F || fmt == TR_TEX_RGBA32F;
+    if (kind == TEX_RGB) return fmt == TR_TEX_RGBA8_UNORM || fmt == TR_TEX_RGB32F || fmt == TR_TEX_RGBA32F;
     return fmt == TR_TEX_RGBA8_UNORM || fmt == TR_TEX_RGBA32F;
 }
 

@@ -13,7 +13,8 @@
  * BackfaceCulling keeps a hit iff N_ccw . d < 0 (ray-tracing.cc:1303-1307);
  * TwoSidedBias drops hits closer than 1e-3 (ray-tracing.cc:1309-1313).
  * Sphere test: IntersectSphere (include/tempest/math/intersect.hh:163-190)
- * plus the (tnear, tfar) range check.
+ * plus the (tnear, tfar) range check.  User shapes (rect, disk, cylinder,
+ * oblique cylinder, ellipsoid) likewise.
  */
 #ifndef TR_TRAVERSE_H
 #define TR_TRAVERSE_H
@@ -159,13 +160,137 @@ TR_HD bool volume_box_hit(const RayPre& r, const float* lo, const float* hi, flo
     return true;
 }
 
-TR_HD void test_prim(const PrimRec& p, const RayPre& r, float tnear, float tfar, Hit* best)
+/* ---------------- user shapes (ray-tracing.cc:858-1140) ----------------
+ * Each test follows its reference function operation by operation, in FP32,
+ * and returns the distance, (u, v) and the geometric normal without a range
+ * check; the caller applies (tnear, tfar), which the reference's Embree
+ * callbacks omit. */
+
+/* IntersectRect3 (include/tempest/math/intersect.hh:105-131) over the
+ * precomputed columns of ToMatrix3(Orientation). */
+TR_HD bool rect_hit(const float* f, V3 o, V3 d, float* t, float* u, float* v, V3* n)
+{
+    const V3 c = mk(f[0], f[1], f[2]), T = mk(f[3], f[4], f[5]), B = mk(f[6], f[7], f[8]), N = mk(f[9], f[10], f[11]);
+    const float denom = dot(d, N);
+    if (denom > -1e-6f) return false;
+    const float it = dot(sub(c, o), N) / denom;
+    if (it < 0.0f) return false;
+    const V3 pp = sub(add(smul(it, d), o), c);
+    const float x = dot(pp, T), y = dot(pp, B);
+    if (fabsf(x) >= f[12] || fabsf(y) >= f[13]) return false;
+    *t = it;
+    *u = (x / f[12]) * 0.5f + 0.5f;
+    *v = (y / f[13]) * 0.5f + 0.5f;
+    *n = N;
+    return true;
+}
+
+/* IntersectDisk3 (intersect.hh:135-161). */
+TR_HD bool disk_hit(const float* f, V3 o, V3 d, float* t, V3* n)
+{
+    const V3 c = mk(f[0], f[1], f[2]), N = mk(f[3], f[4], f[5]);
+    const float denom = dot(d, N);
+    if (denom > -1e-6f) return false;
+    const float it = dot(sub(c, o), N) / denom;
+    if (it < 0.0f) return false;
+    const float dc = length(sub(c, add(smul(it, d), o)));
+    if (f[6] > dc || dc > f[7]) return false;
+    *t = it;
+    *n = N;
+    return true;
+}
+
+/* CylinderShapeIntersectFunction / ObliqueCylinderIntersectFunction
+ * (ray-tracing.cc:892-1046): the side as a 2-D quadratic in the plane
+ * perpendicular to z (the oblique form with bending (bx, by); bending 0
+ * gives the upright cylinder bit for bit), then the two caps. */
+TR_HD bool cylinder_hit(const float* f, float bx, float by, V3 o, V3 d, float* t_out, V3* n_out)
+{
+    float dcx = f[0] - o.x, dcy = f[1] - o.y;
+    dcx = dcx + bx * (o.z - f[2]);
+    dcy = dcy + by * (o.z - f[2]);
+    const float ax = d.x - bx * d.z, ay = d.y - by * d.z;
+    const float b = dcx * ax + dcy * ay;
+    const float r = f[3], h = f[4];
+    const float cq = (dcx * dcx + dcy * dcy) - r * r;
+    const float a = ax * ax + ay * ay;
+    const float D = b * b - cq * a;
+    if (D < 0.0f || b < 0.0f) return false;
+    const float dz = f[2] - o.z;
+    const float sq = sqrtf(D);
+    float t = (b + (2.0f * (float)(b < sq) - 1.0f) * sq) / a;
+    if (b > sq && fabsf(t * d.z - dz) > h) t = (b + sq) / a;
+    float nx = t * ax - dcx, ny = t * ay - dcy;
+    const float l = sqrtf(nx * nx + ny * ny);
+    if (l != 0.0f) { nx = nx / l; ny = ny / l; }
+    V3 n = mk(nx, ny, 0.0f);
+    const float r2 = r * r;
+    const float t_top = (dz + h) / d.z;
+    const float ux = t_top * ax - dcx, uy = t_top * ay - dcy;
+    if (t_top >= 0.0f && t_top < t && ux * ux + uy * uy < r2) { t = t_top; n = mk(0.0f, 0.0f, 1.0f); }
+    const float t_bot = (dz - h) / d.z;
+    const float wx = t_bot * ax - dcx, wy = t_bot * ay - dcy;
+    if (t_bot >= 0.0f && t_bot < t && wx * wx + wy * wy < r2) { t = t_bot; n = mk(0.0f, 0.0f, -1.0f); }
+    if (fabsf(t * d.z - dz) > h) return false;
+    if (dot(n, d) > 0.0f) return false;
+    *t_out = t;
+    *n_out = n;
+    return true;
+}
+
+/* Transform(Quaternion, Vector3) (quaternion.hh:347-350): v + 2 q x (q x v + w v). */
+TR_HD V3 quat_rotate(const float* q, V3 v)
+{
+    const V3 qv = mk(q[0], q[1], q[2]);
+    return add(v, smul(2.0f, cross(qv, add(cross(qv, v), smul(q[3], v)))));
+}
+
+/* IntersectEllipsoid (intersect.hh:192-221). */
+TR_HD bool ellipsoid_hit(const float* f, V3 o, V3 d, float* t_out, V3* n_out)
+{
+    const V3 sc = mk(f[3], f[4], f[5]);
+    const V3 qd = quat_rotate(f + 10, d), qc = quat_rotate(f + 10, sub(mk(f[0], f[1], f[2]), o));
+    const V3 de = mk(qd.x / sc.x, qd.y / sc.y, qd.z / sc.z);
+    const V3 ce = mk(qc.x / sc.x, qc.y / sc.y, qc.z / sc.z);
+    const float a = dot(de, de), b = dot(ce, de), c = dot(ce, ce) - 1.0f;
+    const float D = b * b - c * a;
+    if (D < 0.0f || b < 0.0f) return false;
+    const float sq = sqrtf(D);
+    const float t = (b + (2.0f * (float)(b < sq) - 1.0f) * sq) / a;
+    const V3 n = normalize(quat_rotate(f + 6, mulv(sub(smul(t, de), ce), sc)));
+    if (dot(n, d) > 0.0f) return false;
+    *t_out = t;
+    *n_out = n;
+    return true;
+}
+
+/* A user shape's hit: distance, (u, v) (the rect's, else 0) and normal. */
+TR_HD bool shape_intersect(const ShapeRec& sh, V3 o, V3 d, float* t, float* u, float* v, V3* n)
+{
+    *u = 0.0f;
+    *v = 0.0f;
+    switch (sh.kind) {
+    case SHAPE_RECT: return rect_hit(sh.f, o, d, t, u, v, n);
+    case SHAPE_DISK: return disk_hit(sh.f, o, d, t, n);
+    case SHAPE_CYLINDER: return cylinder_hit(sh.f, 0.0f, 0.0f, o, d, t, n);
+    case SHAPE_OBLIQUE_CYLINDER: return cylinder_hit(sh.f, sh.f[5], sh.f[6], o, d, t, n);
+    case SHAPE_ELLIPSOID: return ellipsoid_hit(sh.f, o, d, t, n);
+    default: return false;
+    }
+}
+
+TR_HD void test_prim(const PrimRec& p, const ShapeRec* shapes, const RayPre& r, float tnear, float tfar, Hit* best)
 {
     float t, u = 0.0f, v = 0.0f;
     bool hit;
     if (p.flags & PRIM_SPHERE) hit = sphere_hit(r, p.v0, p.v1[0], tnear, tfar, &t);
     else if (p.flags & PRIM_VOLUME) hit = volume_box_hit(r, p.v0, p.v1, tnear, tfar, &t, &u);
-    else hit = watertight_triangle(r, p.v0, p.v1, p.v2, (p.flags & PRIM_TWO_SIDED) != 0, tnear, tfar, &t, &u, &v);
+    else if (p.flags & PRIM_SHAPE) {
+        V3 n;
+        hit = shape_intersect(shapes[tr_f2u(p.v0[0])], mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]),
+                              &t, &u, &v, &n) &&
+              t > tnear && t < tfar;
+    } else hit = watertight_triangle(r, p.v0, p.v1, p.v2, (p.flags & PRIM_TWO_SIDED) != 0, tnear, tfar, &t, &u, &v);
     if (hit && hit_better(t, p.geom, p.prim, *best)) {
         best->t = t; best->u = u; best->v = v; best->geom = p.geom; best->prim = p.prim;
     }
@@ -302,8 +427,8 @@ TR_HD PrimRec load_prim(const PrimRec* prims, uint32_t idx)
  * kernels keep the first entries in LDS and spill deeper ones to global
  * memory; this host/device template takes any stack with push/pop/empty. */
 template <class Stack>
-TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const RayPre& r, float tnear, float tfar,
-                           Stack& stack, uint32_t* node_visits, uint32_t* prim_tests)
+TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const ShapeRec* shapes, const RayPre& r, float tnear,
+                           float tfar, Stack& stack, uint32_t* node_visits, uint32_t* prim_tests)
 {
     Hit best;
     best.t = tfar; best.u = 0.0f; best.v = 0.0f; best.geom = TR_INVALID_GEOMETRY; best.prim = TR_INVALID_GEOMETRY;
@@ -336,7 +461,7 @@ TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const RayPr
             uint32_t count = meta >> 5;
             for (uint32_t k = 0; k < count; ++k) {
                 if (prim_tests) ++*prim_tests;
-                test_prim(load_prim(prims, first + k), r, tnear, tfar, &best);
+                test_prim(load_prim(prims, first + k), shapes, r, tnear, tfar, &best);
             }
         }
         if (inner) stack.push(n.d[4], to_visit_order(inner, oct) | (nimask << 8));

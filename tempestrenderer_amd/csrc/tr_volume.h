@@ -144,15 +144,6 @@ TR_HD V3 cell_orientation(const VoxelCell& c, const uint64_t* voxels, const Volu
     return v;
 }
 
-/* Matrix3::makeBasisTangent (matrix3.hh:110-120). */
-TR_HD M3 make_basis_tangent(V3 tan)
-{
-    V3 d0 = cross(mk(0.0f, 1.0f, 0.0f), tan), d1 = cross(mk(0.0f, 0.0f, 1.0f), tan);
-    float d0_len = dot(d0, d0), d1_len = dot(d1, d1);
-    V3 b = d0_len > d1_len ? divs(d0, sqrtf(d0_len)) : divs(d1, sqrtf(d1_len));
-    return m3(tan, b, cross(tan, b));
-}
-
 /* Position along a ray, the same expression everywhere. */
 TR_HD V3 ray_point(V3 org, V3 dir, float t) { return add(org, smul(t, dir)); }
 

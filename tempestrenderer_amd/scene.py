@@ -309,6 +309,53 @@ class RayTracerScene:
         self._keep.append(material)
         return gid.value
 
+    # user shapes (ray-tracing.cc:1315-1446)
+    def _shape_call(self, fn, material, *args):
+        gid = ctypes.c_uint64()
+        if material is not None:
+            self._keep.append(material)
+        self._shape_status(fn(self.h, *args, ctypes.byref(gid)), fn.__name__)
+        return gid.value
+
+    def addRect(self, pos, tan, norm, size, material, tc=None, tangent_map=0):
+        f = [np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (pos, tan, norm, size)]
+        self._keep.extend(f)
+        tcp = None
+        if tc is not None:
+            tcv = np.ascontiguousarray(np.asarray(tc, dtype=np.float32))
+            self._keep.append(tcv)
+            tcp = _fptr(tcv)
+        return self._shape_call(self.lib.tr_scene_add_rect, material, *[_fptr(v) for v in f], ctypes.byref(material.rec), tcp,
+                                int(tangent_map))
+
+    def addBlocker(self, pos, tan, norm, size):
+        f = [np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (pos, tan, norm, size)]
+        return self._shape_call(self.lib.tr_scene_add_blocker, None, *[_fptr(v) for v in f])
+
+    def addDisk(self, pos, norm, inner_radius, outer_radius, material):
+        p, n = (np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (pos, norm))
+        return self._shape_call(self.lib.tr_scene_add_disk, material, _fptr(p), _fptr(n), float(inner_radius),
+                                float(outer_radius), ctypes.byref(material.rec))
+
+    def addCylinder(self, center, radius, half_height, material):
+        c = np.ascontiguousarray(np.asarray(center, dtype=np.float32))
+        return self._shape_call(self.lib.tr_scene_add_cylinder, material, _fptr(c), float(radius), float(half_height),
+                                ctypes.byref(material.rec))
+
+    def addObliqueCylinder(self, center, radius, half_height, tilt_direction, tilt, material):
+        c, td = (np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (center, tilt_direction))
+        return self._shape_call(self.lib.tr_scene_add_oblique_cylinder, material, _fptr(c), float(radius), float(half_height),
+                                _fptr(td), float(tilt), ctypes.byref(material.rec))
+
+    def addEllipsoid(self, center, scale, orientation, material):
+        c, sc, q = (np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for v in (center, scale, orientation))
+        return self._shape_call(self.lib.tr_scene_add_ellipsoid, material, _fptr(c), _fptr(sc), _fptr(q),
+                                ctypes.byref(material.rec))
+
+    @staticmethod
+    def _shape_status(st, name):
+        check(st)
+
     def bindTexture(self, texels, fmt, sampling=0):
         """bindTexture (ray-tracing.hh:340-354): texels [height, width(, c)]
         in the layout of `fmt` (TR_TEX_*); returns the handle for the
@@ -401,6 +448,15 @@ class RayTracerScene:
         check(self.lib.tr_scene_trace_rays(self.h, n, r.ctypes.data_as(ctypes.POINTER(_abi.tr_ray)),
                                            hits.ctypes.data_as(ctypes.POINTER(_abi.tr_hit))))
         return hits
+
+    def rayQuery(self, tc):
+        """rayQuery (ray-tracing.cc:1932-1979): (geometry id, tr_sample_data
+        or None on a miss)."""
+        t = np.asarray(tc, dtype=np.float32)
+        sd = _abi.tr_sample_data()
+        gid = ctypes.c_uint64()
+        check(self.lib.tr_scene_ray_query(self.h, _fptr(t), ctypes.byref(sd), ctypes.byref(gid)))
+        return gid.value, (sd if gid.value != 0xFFFFFFFF else None)
 
     def traceRaysNoAlloc(self, device_rays, device_hits, count, stream=None):
         """tr_scene_trace_rays_noalloc: rays and hits are device addresses

@@ -87,6 +87,7 @@ class SceneDesc:
         self.sphere_lights = []
         self.volumes = []      # (dims, min, max, density f16 [nz, ny, nx], direction f16 [nz, ny, nx, 3], material)
         self.brick_volumes = []  # (VolumeRoot, material)
+        self.shapes = []       # (scene-API method, keyword arguments, material or None)
         self.textures = []     # (texels, TR_TEX_* format, sampling): handles 1, 2, ... in this order
         self.cube_faces = None  # six uint32 RGBA8 faces
         self.settings = dict(samples_camera=1, samples_gi=1, samples_light=1, max_depth=1, rr=1.0,
@@ -132,6 +133,11 @@ class SceneDesc:
             target.addGridVolume(dims, lo, hi, dens, dirs, mat(m))
         for root, m in self.brick_volumes:
             target.addHierarchicalVolume(root, mat(m))
+        for method, kw, m in self.shapes:
+            if m is None:
+                getattr(target, method)(**kw)
+            else:
+                getattr(target, method)(material=mat(m), **kw)
         for L in self.sphere_lights:
             target.addSphereLightSource(target.make_light(**L))
         for L in self.lights:
@@ -200,6 +206,106 @@ def sphere_light_box(width=48, height=48, samples_camera=4, max_depth=4, enclosi
     else:
         d.sphere_lights.append(dict(kind=LIGHT_SPHERE_AREA, radiance=(6.0, 5.0, 4.0), center=(0.35, 0.55, 0.3),
                                     radius=0.15))
+    return d
+
+
+def shapes(width=48, height=48, samples_camera=4, max_depth=4, seed=3, tangent_map=True):
+    """The user shapes of the scene API in the box (walls and emitter of
+    box(), without its spheres): a rect on the back wall with a texture-
+    coordinate box and (tangent_map) an RGBA8 map of tangent directions
+    near +x (sampleRGB reads them as byte / 255), a disk (annulus)
+    on the floor, an upright cylinder lying along z, an oblique cylinder, a
+    rotated ellipsoid and a blocker under the ceiling; a directional light
+    besides the mesh emitter, so shadow rays cross the shapes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = box(width, height, samples_camera=samples_camera, max_depth=max_depth)
+    d.name = "shapes"
+    d.spheres = []
+    if tangent_map:
+        t = np.abs(rng.normal(size=(4, 4, 3)) * 0.35 + np.array([1.0, 0.0, 0.0]))
+        t /= np.linalg.norm(t, axis=-1, keepdims=True)
+        b = np.clip(np.round(t * 255.0), 0, 255).astype(np.uint32)
+        d.textures.append((b[..., 0] | (b[..., 1] << 8) | (b[..., 2] << 16) | np.uint32(255 << 24), 2, 0))
+    rough = dict(model=GGX, diffuse=(0.5, 0.45, 0.4), specular=(0.4, 0.4, 0.4),
+                 specular_power=(alpha_to_specular_power(0.35),) * 2)
+    metal = dict(model=GGX_CONDUCTOR, diffuse=(0.0, 0.0, 0.0), specular=(1.0, 1.0, 1.0),
+                 specular_power=(alpha_to_specular_power(0.25),) * 2, fresnel=(0.3, 2.8))
+    beck = dict(model=BECKMANN, diffuse=(0.3, 0.5, 0.3), specular=(0.3, 0.3, 0.3),
+                specular_power=(alpha_to_specular_power(0.4),) * 2)
+    matte = dict(model=GGX, diffuse=(0.7, 0.6, 0.3), specular=(0.0, 0.0, 0.0), specular_power=(1.0, 1.0))
+    d.shapes.append(("addRect", dict(pos=(0.0, 0.15, -0.97), tan=(1.0, 0.0, 0.0), norm=(0.0, 0.0, 1.0),
+                                     size=(0.45, 0.3), tc=(0.0, 0.0, 2.0, 1.0),
+                                     tangent_map=1 if tangent_map else 0), rough))
+    d.shapes.append(("addDisk", dict(pos=(0.45, -0.99, 0.35), norm=(0.0, 1.0, 0.0), inner_radius=0.08,
+                                     outer_radius=0.28), beck))
+    d.shapes.append(("addCylinder", dict(center=(-0.5, -0.75, 0.1), radius=0.18, half_height=0.4), metal))
+    d.shapes.append(("addObliqueCylinder", dict(center=(0.05, -0.6, -0.45), radius=0.1, half_height=0.3,
+                                                tilt_direction=(1.0, 0.0), tilt=0.15), matte))
+    q = np.array([0.2, 0.3, 0.1, 0.9])
+    q /= np.linalg.norm(q)
+    d.shapes.append(("addEllipsoid", dict(center=(0.45, -0.55, 0.0), scale=(0.22, 0.12, 0.16),
+                                          orientation=tuple(float(v) for v in q)), rough))
+    d.shapes.append(("addBlocker", dict(pos=(-0.35, 0.6, 0.2), tan=(1.0, 0.0, 0.0), norm=(0.0, -1.0, 0.0),
+                                        size=(0.2, 0.15)), None))
+    ld = np.array([0.3, 0.9, 0.4]) / np.linalg.norm([0.3, 0.9, 0.4])
+    d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(1.5, 1.5, 1.5)))
+    return d
+
+
+# samples/virtual-gonioreflectometer.cc:33-41
+GONIO_FIBER_RADIUS, GONIO_FIBER_LENGTH, GONIO_YARN_RADIUS = 0.1, 20.0, 1.0
+
+
+def gonio_cylinder_light(samp, samples=9):
+    """The light direction of request `samp` of the reference's loop
+    (virtual-gonioreflectometer.cc:159-182): tc on a samples x samples grid
+    over [-1, 1]^2, Direction = (-cos theta, -sin theta tc.x, -sin theta tc.y)
+    with sin^2 theta = clamp(tc.tc, 0, 1)."""
+    nxt = samp + 1
+    tcx = 2.0 * (nxt % samples) / (samples - 1) - 1.0
+    tcy = 2.0 * (nxt // samples) / (samples - 1) - 1.0
+    s2 = min(max(tcx * tcx + tcy * tcy, 0.0), 1.0)
+    ct, st = math.sqrt(1.0 - s2), math.sqrt(s2)
+    return (-ct, -st * tcx, -st * tcy)
+
+
+def gonio_cylinders(width=90, height=90, samples_camera=64):
+    """The reference's own virtual gonioreflectometer scene
+    (samples/virtual-gonioreflectometer.cc:43-143): three cylinders of
+    radius (YarnRadius + FiberRadius) / 3 and half height FiberLength along
+    z, at -2r y, +2r x and +2r y; GGX Diffuse 0.05, Specular 0.95,
+    SpecularPower 10, Fresnel ComputeReflectionCoefficient(1.56); a
+    directional light of radiance 10 from normalize(0, 1, 1); 64 camera
+    samples, one light and one GI sample, depth 1, Russian roulette 0.9,
+    black background; an orthographic camera of half size YarnRadius +
+    FiberRadius looking along +x with z up (its view * rot)."""
+    r = (GONIO_YARN_RADIUS + GONIO_FIBER_RADIUS) / 3.0
+    span = GONIO_YARN_RADIUS + GONIO_FIBER_RADIUS
+    e = np.array([-1.0, 0.0, 0.0])
+    eye = 15.0 * e
+    f = -e
+    up = np.array([0.0, 0.0, 1.0])
+    sv = np.cross(f, up)
+    sv /= np.linalg.norm(sv)
+    u = np.cross(sv, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = sv, u, -f
+    view[:3, 3] = -view[:3, :3] @ eye
+    proj = np.eye(4)
+    proj[0, 0] = proj[1, 1] = 1.0 / span
+    proj[2, 2] = -2.0 / (1000.0 - 0.1)
+    proj[2, 3] = -(1000.0 + 0.1) / (1000.0 - 0.1)
+    vpi = np.ascontiguousarray(np.linalg.inv(proj @ view).T.reshape(16).astype(np.float32))
+    d = SceneDesc("gonio_cylinders", width, height, vpi)
+    f0 = ((1.56 - 1.0) / (1.56 + 1.0)) ** 2          # ComputeReflectionCoefficient(1.56)
+    m = dict(model=GGX, diffuse=(0.05, 0.05, 0.05), specular=(0.95, 0.95, 0.95), specular_power=(10.0, 10.0),
+             fresnel=(f0, 0.0))
+    for c in ((0.0, -2.0 * r, 0.0), (2.0 * r, 0.0, 0.0), (0.0, 2.0 * r, 0.0)):
+        d.shapes.append(("addCylinder", dict(center=c, radius=r, half_height=GONIO_FIBER_LENGTH), m))
+    ld = np.array([0.0, 1.0, 1.0]) / math.sqrt(2.0)
+    d.lights.append(dict(kind=LIGHT_DIRECTIONAL, direction=tuple(float(x) for x in ld), radiance=(10.0, 10.0, 10.0)))
+    d.settings.update(samples_camera=samples_camera, samples_gi=1, samples_light=1, max_depth=1, rr=0.9,
+                      background=(0.0, 0.0, 0.0), seed=1)
     return d
 
 

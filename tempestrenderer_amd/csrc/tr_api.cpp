@@ -242,7 +242,7 @@ SceneView scene_view(const tr_scene* s)
     v.ipool = s->d_ipool.ptr;
     v.volumes = s->d_vols.ptr;
     v.voxels = s->d_voxels.ptr;
-    v.shapes = s->d_shapes.ptr;
+    v.shapes = s->shapes.empty() ? nullptr : s->d_shapes.ptr;   /* null selects the shape-free traversal */
     v.n_lights = (uint32_t)s->lights.size();
     v.n_prims = (uint32_t)s->prims.size();
     v.n_volumes = (uint32_t)s->vols.size();

@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
  * active lane's ray and tests that node's leaves. */
 constexpr uint32_t kRefill = 16;
 
-template <bool COUNT>
+template <bool COUNT, bool SHAPES>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRefillWavesPerSimd))) void k_trace_refill(FrameParams p, PathPool pool, int cur,
                                                               unsigned long long* counts)
 {
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                 const uint32_t count = meta >> 5;
                 for (uint32_t k = 0; k < count; ++k) {
                     if (COUNT) ++np;
-                    test_prim(load_prim(sc.prims, first + k), sc.shapes, r, 0.0f, INFINITY, &best);
+                    test_prim<SHAPES>(load_prim(sc.prims, first + k), sc.shapes, r, 0.0f, INFINITY, &best);
                 }
             }
             if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
@@ -1353,12 +1353,15 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 {
     unsigned long long* c = (unsigned long long*)counts;
     bool vol = p.sc.n_volumes > 0;
+    bool shapes = p.sc.shapes != nullptr;
     if (counts) {
         if (vol) launch_k(k_trace<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else launch_k(k_trace_refill<true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else if (shapes) launch_k(k_trace_refill<true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else launch_k(k_trace_refill<true, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     } else {
         if (vol) launch_k(k_trace<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else launch_k(k_trace_refill<false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else if (shapes) launch_k(k_trace_refill<false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else launch_k(k_trace_refill<false, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     }
 }
 

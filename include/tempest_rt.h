@@ -163,7 +163,11 @@ typedef struct tr_mesh_options {
 typedef struct tr_settings {
     uint32_t data_pool_size;   /* accepted for compatibility, unused */
     uint32_t path_pool_size;   /* paths resident per batch; 0 = automatic */
-    uint32_t trace_waves_per_cu; /* persistent traversal waves per CU, 1..32; 0 = 16 */
+    uint32_t trace_waves_per_cu; /* persistent traversal waves per CU, 1..32; 0 = automatic (24 for
+                                    surface scenes, 16 with volumes) */
+    uint32_t shade_order;      /* 0: shading runs paths in queue order (slot order, coalesced);
+                                  1: each shading block first sorts its paths by material class.
+                                  Results are the same. */
 } tr_settings;
 
 /* A ray of a fixed ray set and its closest hit. */

@@ -73,7 +73,7 @@ class tr_mesh_options(ctypes.Structure):
 
 class tr_settings(ctypes.Structure):
     _fields_ = [("data_pool_size", ctypes.c_uint32), ("path_pool_size", ctypes.c_uint32),
-                ("trace_waves_per_cu", ctypes.c_uint32)]
+                ("trace_waves_per_cu", ctypes.c_uint32), ("shade_order", ctypes.c_uint32)]
 
 
 class tr_ray(ctypes.Structure):

@@ -374,6 +374,7 @@ FrameParams frame_params(const tr_scene* s)
     }
     memcpy(p.vpi, s->vpi, sizeof(p.vpi));
     p.render_mode = s->render_mode;
+    p.shade_sort = s->settings.shade_order;
     return p;
 }
 
@@ -735,7 +736,12 @@ tr_status tr_scene_create(uint32_t width, uint32_t height, const float vpi[16], 
         return fail(TR_ERROR_INVALID_ARG, "trace_waves_per_cu above 32");
     }
     if (settings) s->settings = *settings;
-    else { s->settings.data_pool_size = 32u << 20; s->settings.path_pool_size = 0; s->settings.trace_waves_per_cu = 0; }
+    else { s->settings.data_pool_size = 32u << 20; s->settings.path_pool_size = 0; s->settings.trace_waves_per_cu = 0;
+           s->settings.shade_order = 0; }
+    if (s->settings.shade_order > 1) {
+        delete s;
+        return fail(TR_ERROR_INVALID_ARG, "shade_order above 1");
+    }
     memset(&s->stats, 0, sizeof(s->stats));
     *out = s;
     return TR_OK;
@@ -1627,6 +1633,7 @@ tr_status tr_scene_commit(tr_scene* s)
     if (st != TR_OK) return st;
     if ((st = upload(s, s->d_nodes, s->nodes)) != TR_OK) return st;
     if ((st = upload(s, s->d_prims, s->prims)) != TR_OK) return st;
+    for (GeomRec& g : s->geoms) g.shade_class = shade_class_of(g, g.material == kNoMaterial ? nullptr : &s->mats[g.material]);
     if ((st = upload(s, s->d_geoms, s->geoms)) != TR_OK) return st;
     if ((st = upload(s, s->d_mats, s->mats)) != TR_OK) return st;
     if ((st = upload(s, s->d_vpool, s->vpool)) != TR_OK) return st;

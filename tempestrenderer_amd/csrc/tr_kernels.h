@@ -69,6 +69,7 @@ struct FrameParams {
     ReqView req;             /* request mode (req.n > 0) */
     uint32_t render_mode;    /* RenderMode (ray-tracing-common.hh:132-139): 0 Normal, 1-3 debug tangents /
                                 binormals / normals, 4 debug lighting */
+    uint32_t shade_sort;     /* 1: shading blocks sort their paths by material class */
 };
 
 /* Path flags of cache mode: training path, next vertex is the query

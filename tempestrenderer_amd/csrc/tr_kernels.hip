@@ -475,6 +475,59 @@ __device__ inline V3 debug_color(uint32_t mode, V3 tangent, V3 binormal, V3 norm
     return add(muls(v, 0.5f), mk(0.5f, 0.5f, 0.5f));
 }
 
+/* Material-coherent shading: a stable counting sort of a 256-thread
+ * block's queue entries by the shading class of their hit (misses first,
+ * then paths without a BSDF, then one BSDF family after another), in LDS,
+ * so that the lanes of a wave run one BSDF's code.  Entries past the end of
+ * the queue take the last class, so thread i of the block still holds an
+ * entry iff i is below the block's entry count.  The order changes neither
+ * a path's result nor the queue contents, only the queue order.  It costs
+ * coalescing: a wave's slots then come from the block's whole range of
+ * 256 instead of 64 consecutive queue entries. */
+__device__ inline uint32_t sort_by_class(const SceneView& sc, const PathPool& pool, uint32_t slot, bool active,
+                                         float4* hit)
+{
+    constexpr uint32_t kWaves = 256 / 64;
+    __shared__ uint32_t s_off[kShadeClasses * kWaves];   /* [class][wave]: counts, then offsets */
+    __shared__ uint32_t s_slot[256];
+    __shared__ float4 s_hit[256];                        /* the hit record moves with its slot */
+    uint32_t key = SC_NONE;
+    float4 h = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (active) {
+        h = pool.hit[slot];
+        const uint32_t geom = __float_as_uint(h.w);
+        key = geom == TR_INVALID_GEOMETRY ? (uint32_t)SC_MISS : sc.geoms[geom].shade_class;
+    }
+    const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t rank = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kShadeClasses; ++k) {
+        const uint64_t m = __ballot(key == k);
+        if (key == k) rank = (uint32_t)__popcll(m & lt);
+        if (lane == 0) s_off[k * kWaves + w] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (w == 0) {
+        /* exclusive scan of the 36 counts in class-major, wave-minor order */
+        const uint32_t c = lane < kShadeClasses * kWaves ? s_off[lane] : 0u;
+        uint32_t v = c;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(v, d, 64);
+            if (lane >= d) v += u;
+        }
+        if (lane < kShadeClasses * kWaves) s_off[lane] = v - c;
+    }
+    __syncthreads();
+    const uint32_t pos = s_off[key * kWaves + w] + rank;
+    s_slot[pos] = slot;
+    s_hit[pos] = h;
+    __syncthreads();
+    *hit = s_hit[threadIdx.x];
+    return s_slot[threadIdx.x];
+}
+
 template <bool VOL, bool CACHE, bool TEX>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
                                                uint32_t vertex, int cur, int nxt, CacheRecords rec)
@@ -488,6 +541,8 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
     float qin[kMlpIn];
     V3 qthr = mk(0.0f, 0.0f, 0.0f);
     uint32_t slot = active ? pool.queue[cur][qi] : 0u;
+    float4 sorted_hit;
+    if (p.shade_sort) slot = sort_by_class(p.sc, pool, slot, active, &sorted_hit);
     if (active) {
         const SceneView& sc = p.sc;
         const typename TexSel<TEX>::type tv = TexSel<TEX>::get(sc);
@@ -505,7 +560,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
         Rng rng;
         rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
         const bool unbiased = p.estimator == TR_ESTIMATOR_UNBIASED;
-        float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = pool.hit[slot];
+        float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = p.shade_sort ? sorted_hit : pool.hit[slot];
         const float4 t4 = vertex == 0 ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : pool.thr[slot];
         V3 org = ld4(o4), dir = ld4(d4);
         V3 T = ld4(t4);

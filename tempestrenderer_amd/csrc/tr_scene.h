@@ -111,7 +111,36 @@ struct GeomRec {
     uint32_t index_base;  /* offset of the submesh's first index in the index pool */
     uint32_t tri_count;   /* volumes: index into SceneView::volumes */
     float sphere[4];
+    uint32_t shade_class; /* shading class of its hits (shade_class_of), set at commit */
+    uint32_t pad;
 };
+
+/* Shading classes: the order in which a shading block runs its paths, so
+ * that the lanes of a wave run the same BSDF code.  0 is a miss; an entry
+ * past the end of the queue takes kShadeClasses - 1. */
+constexpr uint32_t kShadeClasses = 9;
+enum { SC_MISS = 0, SC_NO_BSDF = 1, SC_PHONG = 2, SC_GGX = 3, SC_BECKMANN = 4, SC_ANISO = 5, SC_SGGX = 6,
+       SC_WALK = 7, SC_NONE = 8 };
+
+/* The class of a hit on geometry g with material m (nullptr: none). */
+inline uint32_t shade_class_of(const GeomRec& g, const tr_material* m)
+{
+    if (!m) return SC_NO_BSDF;
+    if (g.type == GEOM_VOLUME) return SC_WALK;
+    switch (m->model) {
+    case TR_MODEL_EMISSIVE: return SC_NO_BSDF;
+    case TR_MODEL_BLINN_PHONG: return SC_PHONG;
+    case TR_MODEL_GGX_MICROFACET:
+    case TR_MODEL_GGX_MICROFACET_CONDUCTOR:
+    case TR_MODEL_GGX_MICROFACET_DIELECTRIC: return SC_GGX;
+    case TR_MODEL_BECKMANN_MICROFACET:
+    case TR_MODEL_BECKMANN_MICROFACET_CONDUCTOR:
+    case TR_MODEL_BECKMANN_MICROFACET_DIELECTRIC: return SC_BECKMANN;
+    case TR_MODEL_SGGX_SURFACE: return SC_SGGX;
+    case TR_MODEL_SGGX_PSEUDO_VOLUME: return SC_WALK;
+    default: return SC_ANISO;
+    }
+}
 
 /* A grid volume as the tracking code reads it.  Samples are packed per
  * grid node as four binary16 values (direction x, y, z, density): one

@@ -223,10 +223,10 @@ def ParseVolume(density, orientation):
 class RayTracerScene:
     """The MI355X tracer behind the reference scene API."""
 
-    def __init__(self, width, height, view_proj_inv, path_pool_size=0, trace_waves_per_cu=0):
+    def __init__(self, width, height, view_proj_inv, path_pool_size=0, trace_waves_per_cu=0, shade_order=0):
         self.lib = _abi.load()
         vpi = np.ascontiguousarray(np.asarray(view_proj_inv, dtype=np.float32).reshape(16))
-        settings = _abi.tr_settings(32 << 20, int(path_pool_size), int(trace_waves_per_cu))
+        settings = _abi.tr_settings(32 << 20, int(path_pool_size), int(trace_waves_per_cu), int(shade_order))
         h = ctypes.c_void_p()
         check(self.lib.tr_scene_create(int(width), int(height), _fptr(vpi), ctypes.byref(settings), ctypes.byref(h)))
         self.h = h

@@ -63,11 +63,13 @@ tr_status MlpState::ensure_train(uint32_t n)
     TR_HIP(ht.reserve((size_t)4 * 64 * cap));
     TR_HIP(dzt.reserve((size_t)5 * 64 * cap));
     TR_HIP(hipMemsetAsync(dzt.ptr, 0, (size_t)5 * 64 * cap * sizeof(uint16_t), stream));
-    TR_HIP(partial.reserve((size_t)(cap / kSlice) * kMlpParams));
+    TR_HIP(partial.reserve((size_t)mlp_wgrad_slices(cap) * kGradSlots));
     TR_HIP(loss_part.reserve((size_t)kTrainGrid * kMaxLossSlots));
     TR_HIP(s_in.reserve((size_t)kMlpIn * cap));
     TR_HIP(s_target.reserve((size_t)3 * cap));
     TR_HIP(s_valid.reserve(cap));
+    /* the zeroed rows are read by steps on other streams (the scene's) */
+    TR_HIP(hipStreamSynchronize(stream));
     train_cap = cap;
     return TR_OK;
 }
@@ -119,8 +121,8 @@ tr_status MlpState::enqueue_step(uint32_t n, uint32_t n_valid, hipStream_t s, bo
     smp.target += offset;
     smp.valid += offset;
     launch_mlp_train(s, weights(), smp, padded, loss_scale, inv_valid, tb, kTrainGrid);
-    launch_mlp_wgrad(s, smp, padded, tb, padded / kSlice);
-    last_slices = padded / kSlice;
+    launch_mlp_wgrad(s, smp, padded, tb);
+    last_slices = mlp_wgrad_slices(padded);
     if (update) {
         ++step;
         float bc1 = 1.0f - powf(0.9f, (float)step), bc2 = 1.0f - powf(0.99f, (float)step);
@@ -251,11 +253,13 @@ static tr_status mlp_pass(tr_mlp* m, size_t n, const float* inputs, const float*
     if (loss) *loss = m->st.read_loss();
     if (grad) {
         uint32_t slices = m->st.last_slices;
-        std::vector<float> part((size_t)slices * kMlpParams);
+        std::vector<float> part((size_t)slices * kGradSlots);
         TR_HIP(hipMemcpy(part.data(), m->st.partial.ptr, part.size() * sizeof(float), hipMemcpyDeviceToHost));
-        for (int p = 0; p < kMlpParams; ++p) {
+        for (int q = 0; q < kGradSlots; ++q) {
+            const int p = mlp_grad_param(q);
+            if (p < 0) continue;
             float g = 0.0f;
-            for (uint32_t s2 = 0; s2 < slices; ++s2) g += part[(size_t)s2 * kMlpParams + p];
+            for (uint32_t s2 = 0; s2 < slices; ++s2) g += part[(size_t)s2 * kGradSlots + q];
             grad[p] = g * (1.0f / m->st.loss_scale);
         }
     }

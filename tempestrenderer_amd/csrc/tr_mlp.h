@@ -94,6 +94,13 @@ __host__ __device__ inline void mlp_band_quad(float x, int k0, float s[4], float
     }
 }
 
+/* A training input as the network reads it: samples of weight 0 (records
+ * of paths that never reached a query vertex, padding up to a whole
+ * slice) may hold any bits, so a value outside [-1e30, 1e30] or NaN reads
+ * as 0; their features then stay finite and their zero pre-activation
+ * gradients give exactly zero weight gradients. */
+__host__ __device__ inline float mlp_clean_input(float x) { return fabsf(x) <= 1e30f ? x : 0.0f; }
+
 /* Feature of an input scalar: band k, sin (c = 0) or cos (c = 1), by the
  * same operations as mlp_band_quad (bit-identical to it). */
 __host__ __device__ inline float mlp_feature(float x, int k, int c)
@@ -109,11 +116,50 @@ __host__ __device__ inline float mlp_feature(float x, int k, int c)
     return c ? co : s;
 }
 
+/* Layout of one weight-gradient partial ("gradient slots"), the order in
+ * which the gradient kernel's matrix-core tiles leave the registers: 16x16
+ * tiles of 256 floats, lane l's four rows 4 (l / 16) + i of column l % 16
+ * at 4 l + i, so a wave stores a tile as 1 KB of consecutive bytes.
+ *   tiles 0..63     layer 0: wave block w = 0..3, tile 4 nt + mt; column
+ *                   l % 16 of block w is feature group q = 16 w + l % 16
+ *                   (input q / 6, bands 2 (q % 6) + nt / 2, sin for even
+ *                   nt), feature 4 q + nt; groups 60..63 are padding
+ *   tiles 64..111   layers 1..3, 16 tiles each, tile 4 nt + mt
+ *   tiles 112..115  layer 4, tile nt (rows 0..2 of 16 used)
+ *   then the biases b0..b3 (64 each) and b4 (3, padded to 16).
+ * mlp_grad_param maps a slot to its parameter index, or -1 for padding. */
+constexpr int kGradTiles = 116;
+constexpr int kGradBiasOff = kGradTiles * 256;
+constexpr int kGradSlots = kGradBiasOff + 4 * 64 + 16;
+
+__host__ __device__ inline int mlp_grad_param(int q)
+{
+    if (q >= kGradBiasOff) {
+        const int b = q - kGradBiasOff, L = b / 64, m = b % 64;
+        if (L < 4) return kMlpOffB[L] + m;
+        return m < kMlpOut ? kMlpOffB[4] + m : -1;
+    }
+    const int tile = q >> 8, l = (q >> 2) & 63, i = q & 3;
+    const int row = 4 * (l >> 4) + i, col = l & 15;
+    if (tile < 64) {
+        const int w = tile >> 4, nt = (tile >> 2) & 3, mt = tile & 3;
+        const int grp = 16 * w + col;
+        if (grp >= 60) return -1;
+        return kMlpOffW[0] + (16 * mt + row) * kMlpFeat + 4 * grp + nt;
+    }
+    if (tile < 112) {
+        const int L = 1 + (tile - 64) / 16, t = (tile - 64) % 16, nt = t >> 2, mt = t & 3;
+        return kMlpOffW[L] + (16 * mt + row) * kMlpHidden + 16 * nt + col;
+    }
+    const int nt = tile - 112;
+    return row < kMlpOut ? kMlpOffW[4] + row * kMlpHidden + 16 * nt + col : -1;
+}
+
 /* Device buffers of one training step (sized for `capacity` samples). */
 struct MlpTrainBufs {
     uint16_t* ht;       /* [4][64][capacity]: H1..H4, [neuron][sample] */
     uint16_t* dzt;      /* [5][64][capacity]: dZ of layers 0..4 (layer 4: 16 rows used) */
-    float* partial;     /* [slices][kMlpParams] */
+    float* partial;     /* [slices][kGradSlots] */
     float* loss_part;   /* [grid] */
     uint32_t capacity;
 };
@@ -127,7 +173,10 @@ void launch_mlp_infer_counted(hipStream_t s, const MlpWeights& w, const MlpQueri
                               const uint32_t* count, float4* radiance, uint32_t grid);
 void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& smp, uint32_t n, float loss_scale,
                       float inv_valid, const MlpTrainBufs& tb, uint32_t grid);
-void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb, uint32_t slices);
+/* Weight-gradient partials of n samples (a multiple of 128): one partial
+ * per slice, mlp_wgrad_slices(n) of them. */
+uint32_t mlp_wgrad_slices(uint32_t n);
+void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb);
 void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* partial, uint32_t slices,
                      float grad_scale, float lr, float bc1, float bc2, uint16_t* w, uint16_t* wt, float* bias,
                      int update);

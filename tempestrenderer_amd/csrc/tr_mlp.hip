@@ -382,7 +382,7 @@ __global__ __launch_bounds__(kTrainWaves * 64) void k_mlp_train(MlpWeights W, Ml
         const uint32_t b = base + wave * kTile + col;   /* n is a multiple of 512: always < n */
         InferIn x;
 #pragma unroll
-        for (int i = 0; i < kMlpIn; ++i) x.in[i] = smp.in[(size_t)i * smp.stride + b];
+        for (int i = 0; i < kMlpIn; ++i) x.in[i] = mlp_clean_input(smp.in[(size_t)i * smp.stride + b]);
         /* forward, layer 0 */
         f32x4 acc[4];
 #pragma unroll
@@ -438,8 +438,9 @@ __global__ __launch_bounds__(kTrainWaves * 64) void k_mlp_train(MlpWeights W, Ml
             for (int c = 0; c < 3; ++c) {
                 float y = o[c] + sb[256 + c];
                 float t = smp.target[(size_t)c * smp.stride + b];
-                float w = vw * inv_valid / (y * y + 0.01f);
-                float e = y - t;
+                /* weight-0 samples contribute nothing, whatever their target holds */
+                float w = vw != 0.0f ? vw * inv_valid / (y * y + 0.01f) : 0.0f;
+                float e = vw != 0.0f ? y - t : 0.0f;
                 loss_acc += w * e * e;
                 d[c] = 2.0f * w * e * loss_scale;
             }
@@ -512,129 +513,261 @@ __global__ __launch_bounds__(kTrainWaves * 64) void k_mlp_train(MlpWeights W, Ml
 
 /* ---------------- weight gradients ---------------- */
 
-/* Work items of one sample slice: layer 0 feature tiles 0..14 and its bias
- * (item 15), then for layers 1..4 four input tiles and the bias. */
-constexpr int kWgradItems = 16 + 4 * 5;
-
-__global__ __launch_bounds__(64) void k_mlp_wgrad(MlpSampleSoA smp, uint32_t n, MlpTrainBufs tb, uint32_t slices)
+/* Sine and cosine of bands b0 and b0 + 1 (b0 even) of input scalar x by
+ * the forward pass's own operations (mlp_band_quad: the shared sin/cos of
+ * band b0 & ~3, then angle doublings), so bit-identical to its features. */
+__device__ inline void band_pair(float x, int b0, float* s0, float* c0, float* s1, float* c1)
 {
-    const int item = blockIdx.x, slice = blockIdx.y;
-    const int lane = threadIdx.x, g = lane >> 4, col = lane & 15;
-    int L, nt;
-    bool bias;
-    if (item < 16) {
-        L = 0;
-        nt = item;
-        bias = item == 15;
+    float s, c;
+    tr_sincos_turns(x * __builtin_ldexpf(1.0f, (b0 & ~3) - 1), &s, &c);
+    if (b0 & 2) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const float t = 2.0f * s;
+            const float s2 = t * c;
+            c = 1.0f - t * s;
+            s = s2;
+        }
+    }
+    *s0 = s;
+    *c0 = c;
+    const float t = 2.0f * s;
+    *s1 = t * c;
+    *c1 = 1.0f - t * s;
+}
+
+/* Sum of a fragment's eight BF16 values in FP32, in order. */
+__device__ inline float sum8(bf16x8 v)
+{
+    uint16_t h[8];
+    __builtin_memcpy(h, &v, 16);
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += bf2f(h[j]);
+    return s;
+}
+
+/* Row sums of a wave's A fragments: each lane holds 8 samples of row
+ * col of each tile; the four lane groups are added in a fixed order. */
+__device__ inline float rows_total(float v)
+{
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+/* dW_L = dZ_L H_L^T and db_L = dZ_L 1 over one slice of kWgradK samples
+ * per block; every wave owns one 64-row output block and loads its operand
+ * fragments (16 bytes per lane) for the whole slice before the matrix-core
+ * work:
+ *   waves 0-3  layer 0.  Lane col of wave w owns feature group
+ *              q = 16 w + col: input q / 6, bands 2 (q % 6) and +1; column
+ *              tile nt holds band 2 (q % 6) + nt / 2, sin (nt even) or cos.
+ *              Groups 60..63 are padding.  The features are made from the
+ *              inputs here, one sin/cos per sample and group.  Wave 0 also
+ *              sums b0's gradient;
+ *   waves 4-6  layers 1..3 (64 x 64 each) and their biases;
+ *   wave 7     layer 4 (one 16-row tile, rows 0..2 used) and b4.
+ * Bias gradients are row sums of the dZ fragments (FP32, fixed order).
+ * Tiles are stored whole, 16 bytes per lane, in the gradient-slot layout
+ * (mlp_grad_param); each block writes its own partial and k_mlp_adam sums
+ * the slices in a fixed order, so training is deterministic. */
+constexpr int kWgradK = 128;                 /* samples per slice */
+constexpr int kWgradSteps = kWgradK / 32;    /* MFMA k-steps per slice */
+
+__device__ inline void store_tile(float* part, int tile, int lane, f32x4 v)
+{
+    *reinterpret_cast<f32x4*>(part + tile * 256 + 4 * lane) = v;
+}
+
+__global__ __launch_bounds__(512) void k_mlp_wgrad(MlpSampleSoA smp, MlpTrainBufs tb)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    const size_t cap = tb.capacity;
+    const uint32_t k0 = blockIdx.x * kWgradK + 8 * g;
+    float* part = tb.partial + (size_t)blockIdx.x * kGradSlots;
+    if (wave < 4) {
+        const int q = 16 * wave + col;
+        const bool real = q < 60;
+        const int inp = real ? q / 6 : 0, b0 = real ? 2 * (q % 6) : 0;
+        bf16x8 a[kWgradSteps][4];
+        float4 x[kWgradSteps][2];
+#pragma unroll
+        for (int u = 0; u < kWgradSteps; ++u) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) a[u][mt] = ld8(tb.dzt + (16 * mt + col) * cap + k0 + 32 * u);
+            const float4* src = reinterpret_cast<const float4*>(smp.in + (size_t)inp * smp.stride + k0 + 32 * u);
+            x[u][0] = src[0];
+            x[u][1] = src[1];
+        }
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < kWgradSteps; ++u) {
+            float v[8] = {x[u][0].x, x[u][0].y, x[u][0].z, x[u][0].w, x[u][1].x, x[u][1].y, x[u][1].z, x[u][1].w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = mlp_clean_input(v[j]);
+            float f[4][8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                band_pair(v[j], b0, &f[0][j], &f[1][j], &f[2][j], &f[3][j]);
+                if (!real) f[0][j] = f[1][j] = f[2][j] = f[3][j] = 0.0f;
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const bf16x8 b = pack8(f[nt]);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) acc[mt][nt] = mfma(a[u][mt], b, acc[mt][nt]);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) store_tile(part, 16 * wave + 4 * nt + mt, lane, acc[mt][nt]);
+        if (wave == 0) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                float r = 0.0f;
+#pragma unroll
+                for (int u = 0; u < kWgradSteps; ++u) r += sum8(a[u][mt]);
+                r = rows_total(r);
+                if (g == 0) part[kGradBiasOff + 16 * mt + col] = r;
+            }
+        }
+    } else if (wave < 7) {
+        const int L = wave - 3;
+        const uint16_t* dz = tb.dzt + (size_t)L * 64 * cap;
+        const uint16_t* h = tb.ht + (size_t)(L - 1) * 64 * cap;
+        bf16x8 a[kWgradSteps][4], b[kWgradSteps][4];
+#pragma unroll
+        for (int u = 0; u < kWgradSteps; ++u)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                a[u][t] = ld8(dz + (16 * t + col) * cap + k0 + 32 * u);
+                b[u][t] = ld8(h + (16 * t + col) * cap + k0 + 32 * u);
+            }
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < kWgradSteps; ++u)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = mfma(a[u][mt], b[u][nt], acc[mt][nt]);
+        const int base = 64 + 16 * (L - 1);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) store_tile(part, base + 4 * nt + mt, lane, acc[mt][nt]);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            float r = 0.0f;
+#pragma unroll
+            for (int u = 0; u < kWgradSteps; ++u) r += sum8(a[u][mt]);
+            r = rows_total(r);
+            if (g == 0) part[kGradBiasOff + 64 * L + 16 * mt + col] = r;
+        }
     } else {
-        L = 1 + (item - 16) / 5;
-        nt = (item - 16) % 5;
-        bias = nt == 4;
-    }
-    const int mtiles = L == 4 ? 1 : 4;
-    const uint32_t cap = tb.capacity;
-    const uint32_t per = n / slices;              /* multiple of 128 */
-    const uint32_t k_begin = slice * per, k_end = k_begin + per;
-    f32x4 acc[4];
+        const uint16_t* dz4 = tb.dzt + (size_t)4 * 64 * cap;
+        const uint16_t* h4 = tb.ht + (size_t)3 * 64 * cap;
+        bf16x8 a[kWgradSteps], b[kWgradSteps][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const uint16_t* dz = tb.dzt + (size_t)L * 64 * cap;
-    /* four 32-sample k-steps per iteration: all their loads are issued
-     * before the matrix-core work (the slice length is a multiple of 128) */
-    constexpr int kU = 4;
-    for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32 * kU) {
-        bf16x8 bv[kU], av[kU][4];
+        for (int u = 0; u < kWgradSteps; ++u) {
+            a[u] = ld8(dz4 + col * cap + k0 + 32 * u);
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const uint32_t k = k0 + 32 * u;
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-                if (mt < mtiles) av[u][mt] = ld8(dz + (size_t)(16 * mt + col) * cap + k + 8 * g);
-            if (bias) {
-                uint32_t one = col == 0 ? 0x3f80u : 0u;
-                uint32_t w4[4] = {one | (one << 16), one | (one << 16), one | (one << 16), one | (one << 16)};
-                __builtin_memcpy(&bv[u], w4, 16);
-            } else if (L == 0) {
-                int f = 16 * nt + col;                 /* < 240 */
-                int in = f / 24, band = (f % 24) / 2, c = f & 1;
-                const float4* src = reinterpret_cast<const float4*>(smp.in + (size_t)in * smp.stride + k + 8 * g);
-                float4 x0 = src[0], x1 = src[1];
-                float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-                for (int j = 0; j < 8; ++j) v[j] = mlp_feature(v[j], band, c);
-                bv[u] = pack8(v);
-            } else {
-                bv[u] = ld8(tb.ht + ((size_t)(L - 1) * 64 + 16 * nt + col) * cap + k + 8 * g);
-            }
+            for (int t = 0; t < 4; ++t) b[u][t] = ld8(h4 + (16 * t + col) * cap + k0 + 32 * u);
         }
+        f32x4 acc[4];
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
+        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-                if (mt < mtiles) acc[mt] = mfma(av[u][mt], bv[u], acc[mt]);
-    }
-    float* part = tb.partial + (size_t)slice * kMlpParams;
-    const int fan_out = kMlpFanOut[L], fan_in = kMlpFanIn[L];
+        for (int u = 0; u < kWgradSteps; ++u)
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-        if (mt >= mtiles) break;
+            for (int t = 0; t < 4; ++t) acc[t] = mfma(a[u], b[u][t], acc[t]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            int m = 16 * mt + 4 * g + i;
-            if (m >= fan_out) continue;
-            if (bias) {
-                if (col == 0) part[kMlpOffB[L] + m] = acc[mt][i];
-            } else {
-                int nn = 16 * nt + col;
-                if (nn < fan_in) part[kMlpOffW[L] + m * fan_in + nn] = acc[mt][i];
-            }
-        }
+        for (int t = 0; t < 4; ++t) store_tile(part, 112 + t, lane, acc[t]);
+        float r = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kWgradSteps; ++u) r += sum8(a[u]);
+        r = rows_total(r);
+        if (g == 0) part[kGradBiasOff + 256 + col] = r;
     }
 }
 
 /* ---------------- Adam and repacking ---------------- */
 
+/* 32 gradient slots per 256-thread block; the eight 32-thread groups sum
+ * slices g, g + 8, g + 16, ... (loads coalesced over consecutive slots,
+ * eight in flight per thread), then the eight group sums are added in a
+ * fixed order: the result does not depend on timing.  The slot's
+ * parameter (mlp_grad_param) then takes the Adam step and its BF16 copies
+ * are repacked.  With update 0 every parameter is only repacked. */
+constexpr int kAdamSlots = 32;
+constexpr int kAdamGroups = 8;
+
+__device__ inline void repack_param(int p, float x, uint16_t* w, uint16_t* wt, float* bias)
+{
+    int L = 4;
+    while (L > 0 && p < kMlpOffW[L]) --L;
+    const int local = p - kMlpOffW[L];
+    const int fan_in = kMlpFanIn[L], fan_out = kMlpFanOut[L];
+    const uint16_t h = (uint16_t)f2bf(x);
+    if (local < fan_out * fan_in) {
+        const int o = local / fan_in, i = local % fan_in;
+        w[kPackOff[L] + o * kPackInPad[L] + i] = h;
+        if (L > 0) wt[kPackTOff[L] + i * kPackTOutPad[L] + o] = h;
+    } else {
+        bias[64 * L + (local - fan_out * fan_in)] = x;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_mlp_adam(float* master, float* mom, float* vel, const float* partial,
                                                   uint32_t slices, float grad_scale, float lr, float bc1, float bc2,
                                                   uint16_t* w, uint16_t* wt, float* bias, int update)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpParams) return;
-    float x = master[p];
-    if (update) {
-        /* slices summed in order; the loads of eight are in flight at once */
-        float gsum = 0.0f;
-        uint32_t s = 0;
-        for (; s + 8 <= slices; s += 8) {
+    if (!update) {
+        const int p = blockIdx.x * blockDim.x + threadIdx.x;
+        if (p < kMlpParams) repack_param(p, master[p], w, wt, bias);
+        return;
+    }
+    __shared__ float red[kAdamGroups][kAdamSlots];
+    const int q = blockIdx.x * kAdamSlots + (threadIdx.x & (kAdamSlots - 1));
+    const int grp = threadIdx.x / kAdamSlots;
+    float gsum = 0.0f;
+    if (q < kGradSlots) {
+        uint32_t s = grp;
+        for (; s + 7 * kAdamGroups < slices; s += 8 * kAdamGroups) {
             float v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(s + u) * kMlpParams + p];
+            for (int u = 0; u < 8; ++u) v[u] = partial[(size_t)(s + u * kAdamGroups) * kGradSlots + q];
 #pragma unroll
             for (int u = 0; u < 8; ++u) gsum += v[u];
         }
-        for (; s < slices; ++s) gsum += partial[(size_t)s * kMlpParams + p];
-        float gr = gsum * grad_scale;
-        float m = 0.9f * mom[p] + 0.1f * gr;
-        float v = 0.99f * vel[p] + 0.01f * gr * gr;
-        mom[p] = m;
-        vel[p] = v;
-        float mh = m / bc1, vh = v / bc2;
-        x = x - lr * mh / (sqrtf(vh) + 1e-8f);
-        master[p] = x;
+        for (; s < slices; s += kAdamGroups) gsum += partial[(size_t)s * kGradSlots + q];
     }
-    int L = 4;
-    while (L > 0 && p < kMlpOffW[L]) --L;
-    int local = p - kMlpOffW[L];
-    int fan_in = kMlpFanIn[L], fan_out = kMlpFanOut[L];
-    uint16_t h = (uint16_t)f2bf(x);
-    if (local < fan_out * fan_in) {
-        int o = local / fan_in, i = local % fan_in;
-        w[kPackOff[L] + o * kPackInPad[L] + i] = h;
-        if (L > 0) wt[kPackTOff[L] + i * kPackTOutPad[L] + o] = h;
-    } else {
-        int o = local - fan_out * fan_in;
-        bias[64 * L + o] = x;
-    }
+    red[grp][threadIdx.x & (kAdamSlots - 1)] = gsum;
+    __syncthreads();
+    if (grp != 0 || q >= kGradSlots) return;
+#pragma unroll
+    for (int k = 1; k < kAdamGroups; ++k) gsum += red[k][threadIdx.x];
+    const int p = mlp_grad_param(q);
+    if (p < 0) return;
+    float x = master[p];
+    const float gr = gsum * grad_scale;
+    const float m = 0.9f * mom[p] + 0.1f * gr;
+    const float v = 0.99f * vel[p] + 0.01f * gr * gr;
+    mom[p] = m;
+    vel[p] = v;
+    const float mh = m / bc1, vh = v / bc2;
+    x = x - lr * mh / (sqrtf(vh) + 1e-8f);
+    master[p] = x;
+    repack_param(p, x, w, wt, bias);
 }
 
 }  // namespace
@@ -661,16 +794,19 @@ void launch_mlp_train(hipStream_t s, const MlpWeights& w, const MlpSampleSoA& sm
     hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(kTrainWaves * 64), 0, s, w, smp, n, loss_scale, inv_valid, tb);
 }
 
-void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb, uint32_t slices)
+uint32_t mlp_wgrad_slices(uint32_t n) { return n / kWgradK; }
+
+void launch_mlp_wgrad(hipStream_t s, const MlpSampleSoA& smp, uint32_t n, const MlpTrainBufs& tb)
 {
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(kWgradItems, slices), dim3(64), 0, s, smp, n, tb, slices);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(n / kWgradK), dim3(512), 0, s, smp, tb);
 }
 
 void launch_mlp_adam(hipStream_t s, float* master, float* m, float* v, const float* partial, uint32_t slices,
                      float grad_scale, float lr, float bc1, float bc2, uint16_t* w, uint16_t* wt, float* bias, int update)
 {
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, master, m, v, partial, slices,
-                       grad_scale, lr, bc1, bc2, w, wt, bias, update);
+    const uint32_t grid = update ? (kGradSlots + kAdamSlots - 1) / kAdamSlots : (kMlpParams + 255) / 256;
+    hipLaunchKernelGGL(k_mlp_adam, dim3(grid), dim3(256), 0, s, master, m, v, partial, slices, grad_scale, lr, bc1,
+                       bc2, w, wt, bias, update);
 }
 
 }  // namespace trk

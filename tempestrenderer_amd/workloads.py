@@ -193,6 +193,23 @@ def box(width=256, height=256, samples_camera=16, max_depth=4, seed=1):
     return d
 
 
+def off_emitter_mask(ref_mean):
+    """Pixels of a converged image [H, W, 3] whose radiance is below 1 in
+    every channel, away from any pixel that is not (two 4-neighbour
+    dilations): the box image without the directly seen emitter and its
+    anti-aliased silhouette, whose pixel-sampling noise no radiance cache
+    can reduce."""
+    bright = ref_mean.max(axis=2) >= 1.0
+    for _ in range(2):
+        d = bright.copy()
+        d[1:] |= bright[:-1]
+        d[:-1] |= bright[1:]
+        d[:, 1:] |= bright[:, :-1]
+        d[:, :-1] |= bright[:, 1:]
+        bright = d
+    return ~bright
+
+
 def sphere_light_box(width=48, height=48, samples_camera=4, max_depth=4, enclosing=False):
     """The box scene with a sphere-area light (addSphereLightSource): a small
     sphere under the ceiling beside the mesh emitter, or (enclosing) a sphere

@@ -2205,8 +2205,8 @@ tr_status tr_debug_bvh_check(const tr_scene* s, uint64_t* violations)
         for (int slot = 0; slot < 8; ++slot) {
             float lo[3], hi[3];
             for (int a = 0; a < 3; ++a) {
-                lo[a] = n.origin[a] + (float)n.qlo[a][slot] * sc[a];
-                hi[a] = n.origin[a] + (float)n.qhi[a][slot] * sc[a];
+                lo[a] = n.origin[a] + half_value(n.qlo[a][slot]) * sc[a];
+                hi[a] = n.origin[a] + half_value(n.qhi[a][slot]) * sc[a];
             }
             if ((n.imask >> slot) & 1u) {
                 if (n.child_base + rank >= s->nodes.size()) ++bad;
@@ -2222,6 +2222,8 @@ tr_status tr_debug_bvh_check(const tr_scene* s, uint64_t* violations)
                     for (int a = 0; a < 3; ++a)
                         if (pl[a] < lo[a] || ph[a] > hi[a]) ++bad;
                 }
+            } else if (!(lo[0] > hi[0] && lo[1] > hi[1] && lo[2] > hi[2])) {
+                ++bad; /* an empty slot's box must be inverted */
             }
         }
     }

@@ -8,8 +8,8 @@
  * axis, leaves of at most 4 primitives) and collapses it into 8-wide nodes
  * by repeatedly opening the child with the largest surface area.  Children
  * are placed in octant slots so that traversal can order them by the
- * ray's direction signs, their boxes are quantised to 16 bits per bound
- * against a per-node origin and power-of-two scale, and the primitives of a
+ * ray's direction signs, their boxes are stored as binary16 offsets from
+ * the node's centre at a power-of-two scale, rounded outward, and the primitives of a
  * node's leaf children are stored contiguously in leaf order.
  */
 #include "tr_bvh8.h"
@@ -198,40 +198,68 @@ uint32_t build_par(Builder& b, std::vector<BNode>& out, uint32_t begin, uint32_t
     return pidx;
 }
 
-inline float decode(float origin, uint32_t q, int e)
+/* Binary16 codes are handled through an ordered key: key k >= 0 is the
+ * pattern k (value >= 0), k < 0 the pattern 0x8000 | -k, so that keys
+ * -0x7BFF .. 0x7BFF enumerate the finite values in increasing order. */
+constexpr int kKeyMax = 0x7BFF;
+
+inline uint16_t key_bits(int k) { return k >= 0 ? (uint16_t)k : (uint16_t)(0x8000 | -k); }
+
+/* origin + value(k) * 2^e in FP32, as the node is decoded. */
+inline float decode(float origin, int k, int e)
 {
     float s;
     uint32_t bits = (uint32_t)(e + 127) << 23;
     memcpy(&s, &bits, 4);
-    return origin + (float)q * s;
+    return origin + half_value(key_bits(k)) * s;
 }
 
-/* Quantises child bounds [lo, hi] (already padded) on one axis. */
-bool quantise_axis(float node_lo, float node_hi, const float* clo, const float* chi, int nchild, int8_t* exp_out,
-                   uint16_t* qlo, uint16_t* qhi)
+/* Key of a binary16 value near x (|x| <= 65504 after clamping); the caller
+ * steps it to the exact outward bound. */
+int near_key(double x)
 {
-    float ext = node_hi - node_lo;
+    const double a = std::fabs(x);
+    int k;
+    if (a >= 65504.0) k = kKeyMax;
+    else if (a < 6.103515625e-05) k = (int)std::lround(a * 16777216.0); /* subnormal: m * 2^-24 */
+    else {
+        int E;
+        const double f = std::frexp(a, &E); /* a = f 2^E, f in [0.5, 1) */
+        long m = std::lround((f * 2.0 - 1.0) * 1024.0);
+        int ef = E + 14;
+        if (m == 1024) { m = 0; ++ef; }
+        k = std::min((ef << 10) | (int)m, kKeyMax);
+    }
+    return x < 0.0 ? -k : k;
+}
+
+/* Codes for child bounds [lo, hi] (already padded) on one axis, as
+ * offsets from `origin` (the node's centre) at a power-of-two scale. */
+bool quantise_axis(float origin, const float* clo, const float* chi, int nchild, int8_t* exp_out, uint16_t* qlo,
+                   uint16_t* qhi)
+{
+    double maxoff = 0.0;
+    for (int c = 0; c < nchild; ++c)
+        maxoff = std::max(maxoff, std::max(std::fabs((double)clo[c] - origin), std::fabs((double)chi[c] - origin)));
     int e = -126;
-    if (ext > 0.0f) {
+    if (maxoff > 0.0) {
         int ee;
-        frexpf(ext / 65535.0f, &ee); /* ext/65535 in [2^(ee-1), 2^ee) */
+        std::frexp(maxoff / 32768.0, &ee); /* maxoff / 2^ee <= 32768 */
         e = std::max(ee, -126);
     }
     for (; e <= 127; ++e) {
         bool ok = true;
+        const double sc = std::ldexp(1.0, e);
         for (int c = 0; c < nchild && ok; ++c) {
-            float sc = ldexpf(1.0f, e);
-            double fl = std::floor(((double)clo[c] - (double)node_lo) / (double)sc);
-            double fh = std::ceil(((double)chi[c] - (double)node_lo) / (double)sc);
-            int64_t l = (int64_t)std::max(0.0, fl) - 1;
-            int64_t h = (int64_t)fh + 1;
-            if (l < 0) l = 0;
-            while (l > 0 && decode(node_lo, (uint32_t)l, e) > clo[c]) --l;
-            while (h <= 65535 && decode(node_lo, (uint32_t)h, e) < chi[c]) ++h;
-            if (h > 65535) { h = 65535; if (decode(node_lo, 65535u, e) < chi[c]) { ok = false; break; } }
-            if (decode(node_lo, (uint32_t)l, e) > clo[c]) { ok = false; break; }
-            qlo[c] = (uint16_t)l;
-            qhi[c] = (uint16_t)h;
+            int l = near_key(((double)clo[c] - origin) / sc);
+            while (l > -kKeyMax && decode(origin, l, e) > clo[c]) --l;
+            while (l < kKeyMax && decode(origin, l + 1, e) <= clo[c]) ++l;
+            int h = near_key(((double)chi[c] - origin) / sc);
+            while (h < kKeyMax && decode(origin, h, e) < chi[c]) ++h;
+            while (h > -kKeyMax && decode(origin, h - 1, e) >= chi[c]) --h;
+            if (decode(origin, l, e) > clo[c] || decode(origin, h, e) < chi[c]) { ok = false; break; }
+            qlo[c] = key_bits(l);
+            qhi[c] = key_bits(h);
         }
         if (ok) { *exp_out = (int8_t)e; return true; }
     }
@@ -249,6 +277,8 @@ bool build_bvh8(const std::vector<PrimRec>& in, std::vector<Node8>& nodes_out, s
     if (n == 0) {
         Node8 root;
         memset(&root, 0, sizeof(root));
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 8; ++k) { root.qlo[a][k] = kHalfMax; root.qhi[a][k] = kHalfMinusMax; }
         nodes_out.push_back(root);
         if (stats) { stats->depth = 1; stats->nodes = 1; }
         return true;
@@ -341,6 +371,8 @@ bool build_bvh8(const std::vector<PrimRec>& in, std::vector<Node8>& nodes_out, s
 
         Node8 node;
         memset(&node, 0, sizeof(node));
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 8; ++k) { node.qlo[a][k] = kHalfMax; node.qhi[a][k] = kHalfMinusMax; }
         float clo[3][8], chi[3][8];
         int nslot = 0;
         int slot_list[8];
@@ -351,9 +383,9 @@ bool build_bvh8(const std::vector<PrimRec>& in, std::vector<Node8>& nodes_out, s
             slot_list[nslot++] = s;
         }
         for (int a = 0; a < 3; ++a) {
-            node.origin[a] = nb.lo[a];
+            node.origin[a] = 0.5f * (nb.lo[a] + nb.hi[a]);
             uint16_t ql[8], qh[8];
-            if (!quantise_axis(nb.lo[a], nb.hi[a], clo[a], chi[a], nslot, &node.exp[a], ql, qh)) {
+            if (!quantise_axis(node.origin[a], clo[a], chi[a], nslot, &node.exp[a], ql, qh)) {
                 if (err) *err = "BVH8 quantisation failed";
                 return false;
             }

@@ -16,22 +16,27 @@
 #define TR_SCENE_H
 
 #include <stdint.h>
+#include <string.h>
 #include "../../include/tempest_rt.h"
 
 namespace trk {
 
 /* 8-wide BVH node, one 128-byte cache line.
  * Child slot s (0..7) has the box
- *   lo = origin + q_lo[s] * 2^exp,  hi = origin + q_hi[s] * 2^exp  (per axis)
- * computed in FP32 exactly as written; the builder rounds outward and
- * checks that the decoded box contains the child.  imask bit s marks an
- * internal child: its node index is child_base + popcount(imask & ((1<<s)-1)).
- * A leaf child has meta[s] = count << 5 | offset (count 1..7): primitives
- * prim_base + offset .. + count - 1.  meta[s] = 0 and imask bit clear marks
- * an empty slot.  Slots are assigned so that slot s lies towards the
- * (+x if bit0, +y if bit1, +z if bit2) side of the node; a ray visits
- * slots in the order i ^ octant(ray direction), i = 0..7, i.e. roughly
- * front to back. */
+ *   lo = origin + h_lo[s] * 2^exp,  hi = origin + h_hi[s] * 2^exp  (per axis)
+ * where h_lo and h_hi are binary16 values (signed offsets from the node's
+ * centre, so the coarsest code spacing is 2^-12 of the node's extent) and
+ * the sum is computed in FP32 exactly as written; the builder rounds
+ * outward and checks that the decoded box contains the child.  The
+ * traversal converts and scales a code in one mixed-precision fused
+ * multiply-add.  An empty slot has an inverted box (lo = +65504,
+ * hi = -65504), which no ray enters.  imask bit s marks an internal child:
+ * its node index is child_base + popcount(imask & ((1<<s)-1)).  A leaf
+ * child has meta[s] = count << 5 | offset (count 1..7): primitives
+ * prim_base + offset .. + count - 1.  Slots are assigned so that slot s
+ * lies towards the (+x if bit0, +y if bit1, +z if bit2) side of the node;
+ * a ray visits slots in the order i ^ octant(ray direction), i = 0..7,
+ * i.e. roughly front to back. */
 struct alignas(128) Node8 {
     float origin[3];
     int8_t exp[3];
@@ -39,10 +44,26 @@ struct alignas(128) Node8 {
     uint32_t child_base;
     uint32_t prim_base;
     uint8_t meta[8];
-    uint16_t qlo[3][8];
+    uint16_t qlo[3][8];   /* binary16 bit patterns */
     uint16_t qhi[3][8];
 };
 static_assert(sizeof(Node8) == 128, "Node8 must be one 128-byte line");
+
+/* Empty-slot codes: +65504 and -65504. */
+constexpr uint16_t kHalfMax = 0x7BFFu, kHalfMinusMax = 0xFBFFu;
+
+/* The value of a finite binary16 bit pattern (exact in FP32). */
+inline float half_value(uint16_t h)
+{
+    const uint32_t e = (h >> 10) & 31u, m = h & 1023u;
+    float v;
+    if (e == 0) v = (float)m * 5.9604644775390625e-08f; /* m * 2^-24 */
+    else {
+        uint32_t bits = ((e + 112u) << 23) | (m << 13);
+        memcpy(&v, &bits, 4);
+    }
+    return (h & 0x8000u) ? -v : v;
+}
 
 /* Primitive kinds.  A grid volume is one PRIM_VOLUME primitive: its box
  * lo = v0, hi = v1.  A user shape (PRIM_SHAPE, the shape kind in bits 8-15

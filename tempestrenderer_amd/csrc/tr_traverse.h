@@ -303,7 +303,7 @@ TR_HD float exp2i(int e) { return tr_u2f((uint32_t)(e + 127) << 23); }
 
 /* A node as 32 dwords, fetched with eight 16-byte loads.  Field offsets
  * (little-endian): 0-2 origin, 3 exp[0..2] | imask << 24, 4 child_base,
- * 5 prim_base, 6-7 meta[8], 8-19 qlo[3][8], 20-31 qhi[3][8]. */
+ * 5 prim_base, 6-7 meta[8], 8-19 qlo[3][8], 20-31 qhi[3][8] (binary16). */
 struct NodeRegs {
     uint32_t d[32];
 };
@@ -336,12 +336,36 @@ TR_HD uint32_t node_meta(const NodeRegs& n, uint32_t s)
     return (uint32_t)(m >> (8u * s)) & 0xffu;
 }
 
+/* Binary16 code in the low or high half of a node word, as FP32.  On the
+ * GPU the conversion folds into the fused multiply-add that uses it
+ * (v_fma_mix_f32 with an op_sel for the half). */
+TR_HD float half_lo(uint32_t w)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+#else
+    return half_value((uint16_t)(w & 0xffffu));
+#endif
+}
+TR_HD float half_hi(uint32_t w)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+#else
+    return half_value((uint16_t)(w >> 16));
+#endif
+}
+
 /* Slab test of the 8 children of a node; returns the hit mask in slot
  * order.  tmax is the current far bound (closest hit so far or tfar).  A
- * child plane at o + q s (q the 16-bit code, s a power of two) is crossed
- * at t = q (s inv) + (o - org) inv: s inv is exact, so one fused
+ * child plane at o + h s (h the binary16 code, s a power of two) is crossed
+ * at t = h (s inv) + (o - org) inv: s inv is exact, so one fused
  * multiply-add per plane replaces the decode, subtraction and product,
- * with no more roundings than those three. */
+ * with no more roundings than those three.  On an axis where the ray runs
+ * backwards the hi plane is crossed first, so the words of the two plane
+ * sets are exchanged once per node and each child takes the maximum of
+ * its three near planes and the minimum of its three far planes.  Empty
+ * slots carry inverted boxes and fail the test. */
 TR_HD uint32_t intersect_children(const NodeRegs& n, const RayPre& r, float tnear, float tmax)
 {
     const float ox = tr_u2f(n.d[0]), oy = tr_u2f(n.d[1]), oz = tr_u2f(n.d[2]);
@@ -351,35 +375,38 @@ TR_HD uint32_t intersect_children(const NodeRegs& n, const RayPre& r, float tnea
     const float bx = (ox - r.org[0]) * r.inv[0];
     const float by = (oy - r.org[1]) * r.inv[1];
     const float bz = (oz - r.org[2]) * r.inv[2];
-    /* slots with an internal child or a non-empty leaf */
-    uint32_t valid = node_imask(n);
-    uint32_t mlo = n.d[6], mhi = n.d[7];
+    const bool nx = r.inv[0] < 0.0f, ny = r.inv[1] < 0.0f, nz = r.inv[2] < 0.0f;
+    uint32_t near_w[12], far_w[12];
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-    for (uint32_t s = 0; s < 4; ++s) {
-        if ((mlo >> (8u * s)) & 0xffu) valid |= 1u << s;
-        if ((mhi >> (8u * s)) & 0xffu) valid |= 1u << (s + 4);
+    for (int w = 0; w < 4; ++w) {
+        near_w[w] = nx ? n.d[20 + w] : n.d[8 + w];
+        far_w[w] = nx ? n.d[8 + w] : n.d[20 + w];
+        near_w[4 + w] = ny ? n.d[24 + w] : n.d[12 + w];
+        far_w[4 + w] = ny ? n.d[12 + w] : n.d[24 + w];
+        near_w[8 + w] = nz ? n.d[28 + w] : n.d[16 + w];
+        far_w[8 + w] = nz ? n.d[16 + w] : n.d[28 + w];
     }
     uint32_t mask = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
     for (uint32_t s = 0; s < 8; ++s) {
-        uint32_t sh = 16u * (s & 1u);
-        uint32_t w = s >> 1;
-        float tx0 = fmaf((float)((n.d[8 + w] >> sh) & 0xffffu), ax, bx);
-        float ty0 = fmaf((float)((n.d[12 + w] >> sh) & 0xffffu), ay, by);
-        float tz0 = fmaf((float)((n.d[16 + w] >> sh) & 0xffffu), az, bz);
-        float tx1 = fmaf((float)((n.d[20 + w] >> sh) & 0xffffu), ax, bx);
-        float ty1 = fmaf((float)((n.d[24 + w] >> sh) & 0xffffu), ay, by);
-        float tz1 = fmaf((float)((n.d[28 + w] >> sh) & 0xffffu), az, bz);
-        float t0 = fmax_(fmax_(fmin_(tx0, tx1), fmin_(ty0, ty1)), fmax_(fmin_(tz0, tz1), tnear));
-        float t1 = fmin_(fmin_(fmax_(tx0, tx1), fmax_(ty0, ty1)), fmin_(fmax_(tz0, tz1), tmax));
+        const uint32_t w = s >> 1;
+        const bool up = (s & 1u) != 0;
+        const float tx0 = fmaf(up ? half_hi(near_w[w]) : half_lo(near_w[w]), ax, bx);
+        const float ty0 = fmaf(up ? half_hi(near_w[4 + w]) : half_lo(near_w[4 + w]), ay, by);
+        const float tz0 = fmaf(up ? half_hi(near_w[8 + w]) : half_lo(near_w[8 + w]), az, bz);
+        const float tx1 = fmaf(up ? half_hi(far_w[w]) : half_lo(far_w[w]), ax, bx);
+        const float ty1 = fmaf(up ? half_hi(far_w[4 + w]) : half_lo(far_w[4 + w]), ay, by);
+        const float tz1 = fmaf(up ? half_hi(far_w[8 + w]) : half_lo(far_w[8 + w]), az, bz);
+        const float t0 = fmaxf(fmaxf(fmaxf(tx0, ty0), tz0), tnear);
+        float t1 = fminf(fminf(fminf(tx1, ty1), tz1), tmax);
         t1 = t1 * 1.00000095f; /* 1 + 2^-20: keeps boxes touching a hit at t */
         if (t0 <= t1) mask |= 1u << s;
     }
-    return mask & valid;
+    return mask;
 }
 
 /* Permutes a slot-order mask into visiting order: bit i <-> slot i ^ octant

@@ -347,11 +347,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
 /* Closest hit for scenes without volumes, with ray refill: a lane whose
  * ray has finished takes the wave's next ray once at least kRefill lanes
  * are idle, so the traversal loop runs with mostly full waves instead of
- * waiting for the slowest of a fixed group of 64 rays.  Wave w traces the
- * same rays as in k_trace (the queue chunks w, w + grid, ... of 64), in
- * the same per-ray arithmetic; each iteration visits one node of every
- * active lane's ray and tests that node's leaves. */
+ * waiting for the slowest of a fixed group of 64 rays.  The queue is dealt
+ * to the waves in chunks of kRayChunk consecutive rays (wave w takes
+ * chunks w, w + grid, ...).  Each iteration makes one memory round trip
+ * per lane: a lane either tests the next primitive of the hit leaves of
+ * its last node (a leaf cursor) or pops and tests the next node, and the
+ * fetches of both kinds are in flight together.  The visiting order is
+ * that of traverse_closest: a node's hit leaves, in visiting order, before
+ * the next node is popped. */
 constexpr uint32_t kRefill = 16;
+constexpr uint32_t kRayChunk = 16;
 
 template <bool COUNT, bool SHAPES>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRefillWavesPerSimd))) void k_trace_refill(FrameParams p, PathPool pool, int cur,
@@ -370,13 +375,17 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     st.sp = 0;
     uint32_t nv = 0, np = 0, nr = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
-    /* queue index of the wave's k-th ray */
-    auto ray_index = [&](uint32_t k) -> uint32_t { return (blockIdx.x + (k >> 6) * grid) * kTraceBlock + (k & 63u); };
+    auto ray_index = [&](uint32_t k) -> uint32_t {
+        return (blockIdx.x + (k / kRayChunk) * grid) * kRayChunk + (k % kRayChunk);
+    };
     uint32_t next = 0;                /* the wave's next unassigned ray (uniform) */
     bool active = false;
     uint32_t slot = 0;
     RayPre r;
     Hit best;
+    /* leaf cursor: hit leaves still to test (visiting order), the meta words
+     * and primitive base of their node, the position in the current leaf */
+    uint32_t lv = 0, lk = 0, lm0 = 0, lm1 = 0, lpb = 0;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         const bool more = ray_index(next) < n;
@@ -394,6 +403,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                     best.geom = best.prim = TR_INVALID_GEOMETRY;
                     st.sp = 0;
                     st.push(0u, (1u << r.octant) | (1u << 8));
+                    lv = 0;
                     active = true;
                     ++nr;
                 }
@@ -405,37 +415,73 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
             break;
         }
         if (active) {
-            /* one node of the ray's traversal (traverse_closest's loop body) */
             const uint32_t oct = r.octant;
-            uint32_t base, masks;
-            st.pop(&base, &masks);
-            uint32_t vis = masks & 0xffu;
-            const uint32_t imask = masks >> 8;
-            const uint32_t i = ctz32(vis);
-            vis &= vis - 1u;
-            if (vis) st.push(base, vis | (imask << 8));
-            const uint32_t cslot = i ^ oct;
-            const uint32_t node_index = base + popc8(imask & ((1u << cslot) - 1u));
-            NodeRegs nd = load_node(sc.nodes, node_index);
-            if (COUNT) ++nv;
-            const float tmax = best.geom == TR_INVALID_GEOMETRY ? INFINITY : best.t;
-            const uint32_t hit = intersect_children(nd, r, 0.0f, tmax);
-            const uint32_t nimask = node_imask(nd);
-            const uint32_t inner = hit & nimask;
-            uint32_t lv = to_visit_order(hit & ~nimask, oct);
-            while (lv) {
-                const uint32_t li = ctz32(lv);
-                lv &= lv - 1u;
-                const uint32_t meta = node_meta(nd, li ^ oct);
-                const uint32_t first = nd.d[5] + (meta & 31u);
-                const uint32_t count = meta >> 5;
-                for (uint32_t k = 0; k < count; ++k) {
-                    if (COUNT) ++np;
-                    test_prim<SHAPES>(load_prim(sc.prims, first + k), sc.shapes, r, 0.0f, INFINITY, &best);
-                }
+            const bool prim_step = lv != 0;
+            uint32_t count = 0, index;
+            if (prim_step) {
+                const uint64_t m = (uint64_t)lm0 | ((uint64_t)lm1 << 32);
+                const uint32_t meta = (uint32_t)(m >> (8u * (ctz32(lv) ^ oct))) & 0xffu;
+                count = meta >> 5;
+                index = lpb + (meta & 31u) + lk;
+            } else {
+                uint32_t base, masks;
+                st.pop(&base, &masks);
+                uint32_t vis = masks & 0xffu;
+                const uint32_t imask = masks >> 8;
+                const uint32_t i = ctz32(vis);
+                vis &= vis - 1u;
+                if (vis) st.push(base, vis | (imask << 8));
+                index = base + popc8(imask & ((1u << (i ^ oct)) - 1u));
             }
-            if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
-            if (st.empty()) {
+            /* the fetch: the first three 16-byte loads are common to a
+             * primitive and a node, a node takes five more (a skipped empty
+             * leaf reads node 0, which always exists) */
+            const uint4* src = prim_step && count ? reinterpret_cast<const uint4*>(sc.prims + index)
+                                                  : reinterpret_cast<const uint4*>(sc.nodes + (prim_step ? 0u : index));
+            uint4 q[8];
+            q[0] = src[0];
+            q[1] = src[1];
+            q[2] = src[2];
+            if (!prim_step) {
+#pragma unroll
+                for (int k = 3; k < 8; ++k) q[k] = src[k];
+            }
+            if (prim_step) {
+                if (count) {
+                    if (COUNT) ++np;
+                    PrimRec pr;
+                    pr.v0[0] = __uint_as_float(q[0].x); pr.v0[1] = __uint_as_float(q[0].y);
+                    pr.v0[2] = __uint_as_float(q[0].z); pr.prim = q[0].w;
+                    pr.v1[0] = __uint_as_float(q[1].x); pr.v1[1] = __uint_as_float(q[1].y);
+                    pr.v1[2] = __uint_as_float(q[1].z); pr.geom = q[1].w;
+                    pr.v2[0] = __uint_as_float(q[2].x); pr.v2[1] = __uint_as_float(q[2].y);
+                    pr.v2[2] = __uint_as_float(q[2].z); pr.flags = q[2].w;
+                    test_prim<SHAPES, false>(pr, sc.shapes, r, 0.0f, INFINITY, &best);
+                }
+                if (++lk >= count) {
+                    lk = 0;
+                    lv &= lv - 1u;
+                }
+            } else {
+                NodeRegs nd;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    nd.d[4 * k + 0] = q[k].x; nd.d[4 * k + 1] = q[k].y;
+                    nd.d[4 * k + 2] = q[k].z; nd.d[4 * k + 3] = q[k].w;
+                }
+                if (COUNT) ++nv;
+                const float tmax = best.geom == TR_INVALID_GEOMETRY ? INFINITY : best.t;
+                const uint32_t hit = intersect_children(nd, r, 0.0f, tmax);
+                const uint32_t nimask = node_imask(nd);
+                const uint32_t inner = hit & nimask;
+                lv = to_visit_order(hit & ~nimask, oct);
+                lk = 0;
+                lm0 = nd.d[6];
+                lm1 = nd.d[7];
+                lpb = nd.d[5];
+                if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
+            }
+            if (lv == 0 && st.empty()) {
                 if (best.geom == TR_INVALID_GEOMETRY) best.t = INFINITY;
                 pool.hit[slot] = make_float4(best.t, best.u, best.v, __uint_as_float(best.geom));
                 pool.hit_prim[slot] = best.prim;

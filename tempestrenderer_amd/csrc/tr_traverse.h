@@ -280,14 +280,14 @@ TR_HD bool shape_intersect(const ShapeRec& sh, V3 o, V3 d, float* t, float* u, f
 }
 
 /* SHAPES false leaves the user-shape intersections out of the code, for
- * scenes that have none. */
-template <bool SHAPES = true>
+ * scenes that have none; VOLUMES false the volume boxes. */
+template <bool SHAPES = true, bool VOLUMES = true>
 TR_HD void test_prim(const PrimRec& p, const ShapeRec* shapes, const RayPre& r, float tnear, float tfar, Hit* best)
 {
     float t, u = 0.0f, v = 0.0f;
     bool hit;
     if (p.flags & PRIM_SPHERE) hit = sphere_hit(r, p.v0, p.v1[0], tnear, tfar, &t);
-    else if (p.flags & PRIM_VOLUME) hit = volume_box_hit(r, p.v0, p.v1, tnear, tfar, &t, &u);
+    else if (VOLUMES && (p.flags & PRIM_VOLUME)) hit = volume_box_hit(r, p.v0, p.v1, tnear, tfar, &t, &u);
     else if (SHAPES && (p.flags & PRIM_SHAPE)) {
         V3 n;
         hit = shape_intersect(shapes[tr_f2u(p.v0[0])], mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]),

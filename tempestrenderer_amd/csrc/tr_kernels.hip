@@ -999,9 +999,22 @@ __global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_
         sc[threadIdx.x] = c;
         so[threadIdx.x] = o;
         __syncthreads();
+        /* thread t moves entry t of each segment; the loads of 16
+         * segments are issued before their stores */
         const uint32_t nb = min(s1 - c0, kSegment);
-        for (uint32_t j = 0; j < nb; ++j)
-            if (threadIdx.x < sc[j]) dst[base + so[j] + threadIdx.x] = src[(size_t)(c0 + j) * kSegment + threadIdx.x];
+        for (uint32_t j0 = 0; j0 < nb; j0 += 16) {
+            uint32_t v[16];
+#pragma unroll
+            for (uint32_t u = 0; u < 16; ++u) {
+                const uint32_t j = j0 + u;
+                if (j < nb && threadIdx.x < sc[j]) v[u] = src[(size_t)(c0 + j) * kSegment + threadIdx.x];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 16; ++u) {
+                const uint32_t j = j0 + u;
+                if (j < nb && threadIdx.x < sc[j]) dst[base + so[j] + threadIdx.x] = v[u];
+            }
+        }
         __syncthreads();
         base += sum;
     }

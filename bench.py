@@ -188,6 +188,7 @@ def volume_bench(cpu_rows=128, threads=None, cpu=True):
     g = RayTracerScene(d.width, d.height, d.vpi)
     d.populate(g, gpu_material)
     g.commitScene()
+    g_info = g.info()
     g.setSamplesCamera(8)
     g.setCounting(True)
     g.drawOnce()
@@ -209,12 +210,29 @@ def volume_bench(cpu_rows=128, threads=None, cpu=True):
     paths = d.width * d.height * st["samples_camera"]
     steps_per_path = cnt["volume_steps"] / max(1, cnt["paths"])
     gather = steps_per_path * paths * 64.0
+    # roofline of the closest-hit + delta-tracking kernel: algorithmic bytes
+    # of the counted frame (node records, primitives, 64 B of ray state and
+    # hit per segment, 8 corners x 8 B per tentative collision), scaled from
+    # its 8 camera samples to the timed frame's, over the timing frame's
+    # summed launch time of that kernel
+    scale = st["samples_camera"] / 8.0
+    info = g_info
+    node_bytes = info["bvh_bytes"] / max(1, info["bvh_nodes"])
+    alg = scale * (node_bytes * cnt["node_visits"] + 48.0 * cnt["prim_tests"] + 64.0 * cnt["segments"] +
+                   64.0 * cnt["volume_steps"])
+    achieved = alg / (tim["trace_ms"] / 1e3) / 1e9
     out = {"workload": "SGGX fibre volume 256^3 binary16 (mean extinction 4), %dx%d, %d spp, depth %d, albedo 0.9, "
                        "delta tracking, env light" % (d.width, d.height, st["samples_camera"], st["max_depth"]),
            "msamples_per_s": round(paths / secs / 1e6, 1), "ms_per_frame": round(secs * 1e3, 2),
            "tentative_collisions_per_path": round(steps_per_path, 2),
            "trace_ms": round(tim["trace_ms"], 2), "shade_ms": round(tim["shade_ms"], 2),
-           "gather_gbs": round(gather / (tim["trace_ms"] / 1e3) / 1e9, 1)}
+           "gather_gbs": round(gather / (tim["trace_ms"] / 1e3) / 1e9, 1),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": load_traffic(os.path.join(ROOT, "profiles", "volume_kernels_pmc.txt")),
+                        "kernel": "k_trace<false, true> (closest hit + delta tracking)",
+                        "alg_bytes_per_frame": alg, "launches_per_frame": tim["trace_launches"],
+                        "avg_launch_us": round(tim["trace_ms"] * 1e3 / max(1, tim["trace_launches"]), 2)}}
     if cpu:
         import oracle
         o = oracle.OracleScene(d.width, d.height, d.vpi)

@@ -128,7 +128,10 @@ __device__ inline void request_coords(const FrameParams& p, uint32_t r, uint32_t
     const ReqBatch& b = p.req.batches[*p.req.cursor];
     const uint64_t local = b.path_base + slot - p.req.path_off[r];
     const uint64_t per_pixel = (uint64_t)p.req.reqs[r].spp * p.n_gi;
-    const uint64_t lp = local / per_pixel;
+    /* a 32-bit quotient where it fits (a 64-bit division is a long
+     * instruction sequence); the same value */
+    const uint64_t lp = ((local | per_pixel) >> 32) == 0 ? (uint64_t)((uint32_t)local / (uint32_t)per_pixel)
+                                                          : local / per_pixel;
     const uint32_t s = (uint32_t)(local - lp * per_pixel);
     *lpix = (uint32_t)lp;
     *gpix = (uint32_t)(p.req.pix_off[r] + lp);
@@ -148,11 +151,25 @@ __device__ inline void slot_coords(const FrameParams& p, const BatchOut& out, ui
         return;
     }
     uint64_t per_pixel = (uint64_t)p.spp * p.n_gi;
-    uint64_t ps = slot / per_pixel;
+    uint64_t ps;
+    uint32_t sample, split_;
+    if ((slot >> 32) == 0 && (per_pixel >> 32) == 0) {
+        /* 32-bit quotients (a 64-bit division is a long instruction
+         * sequence); the same values */
+        const uint32_t s32 = (uint32_t)slot;
+        const uint32_t q = s32 / p.n_gi;
+        ps = q / p.spp;
+        sample = q - (uint32_t)ps * p.spp;
+        split_ = s32 - q * p.n_gi;
+    } else {
+        ps = slot / per_pixel;
+        sample = (uint32_t)((slot / p.n_gi) % p.spp);
+        split_ = (uint32_t)(slot % p.n_gi);
+    }
     *pseq = ps;
     *pixel = out.pixel_list ? out.pixel_list[ps] : (uint32_t)ps;
-    *cs = p.frame * p.spp + (uint32_t)((slot / p.n_gi) % p.spp);
-    *split = (uint32_t)(slot % p.n_gi);
+    *cs = p.frame * p.spp + sample;
+    *split = split_;
 }
 
 __device__ inline V3 ld4(float4 v) { return mk(v.x, v.y, v.z); }
@@ -315,7 +332,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
                     }
                     const VolumeRec& V = sc.volumes[sc.geoms[h.geom].tri_count];
                     float tc;
-                    if (delta_track_any(sc.voxels, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
+                    if (delta_track_any(sc.voxels, sc.dens, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
                                     COUNT ? &ns : nullptr)) {
                         h.t = tc;
                         h.u = h.v = 0.0f;
@@ -1121,7 +1138,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                             const GeomRec& G = sc.geoms[h.geom];
                             if (G.type != GEOM_VOLUME || ++crossings > kMaxVolumeCrossings) break;
                             const VolumeRec& V = sc.volumes[G.tri_count];
-                            uint32_t succ = woodcock_any(sc.voxels, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
+                            uint32_t succ = woodcock_any(sc.voxels, sc.dens, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
                                                      &step, &step_through, COUNT ? &ns : nullptr);
                             tf = tf * ((float)succ / 2.0f);
                             if (tmax_T * tf < 1e-3f) break;

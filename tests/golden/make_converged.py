@@ -1,12 +1,15 @@
-"""Writes tests/golden/box_unbiased_fp64.npz: the converged image of the
-box workload (256 x 256, depth 4) rendered by the CPU oracle in FP64 with
-the unbiased estimator, and its per-pixel standard error.
+"""Writes tests/golden/box_unbiased_fp64.npz (or, with --estimator 0,
+box_reference_fp64.npz): the converged image of the box workload (256 x
+256, depth 4) rendered by the CPU oracle in FP64 with the unbiased (or the
+reference) estimator, and its per-pixel standard error.
 
-The time-to-RMSE measurement compares GPU renders against it.  It uses its
-own Philox seed, so its noise is independent of every GPU render (which use
-seed 1).  Usage (about 20 minutes on 6 threads for 32768 samples/pixel):
+The time-to-RMSE measurement compares GPU renders against the unbiased one,
+the image tests compare GPU renders of each estimator against its own.  It
+uses its own Philox seed, so its noise is independent of every GPU render
+(which use seed 1).  Usage (about 20 minutes on 6 threads for 32768
+samples/pixel):
 
-    python tests/golden/make_converged.py [--spp 32768] [--threads 6]
+    python tests/golden/make_converged.py [--spp 32768] [--threads 6] [--estimator 1]
 
 The fixture holds data only: the per-pixel mean (float32) and the standard
 error of that mean (float32), RGB.
@@ -32,10 +35,12 @@ def main():
     ap.add_argument("--spp", type=int, default=32768)
     ap.add_argument("--threads", type=int, default=6)
     ap.add_argument("--frame-spp", type=int, default=256)
-    ap.add_argument("--out", default=os.path.join(HERE, "box_unbiased_fp64.npz"))
+    ap.add_argument("--estimator", type=int, default=1, help="1 unbiased, 0 the reference estimator")
+    ap.add_argument("--out", default=None)
     args = ap.parse_args()
+    out = args.out or os.path.join(HERE, "box_unbiased_fp64.npz" if args.estimator else "box_reference_fp64.npz")
     d = workloads.box(256, 256, samples_camera=args.frame_spp, max_depth=4)
-    d.settings.update(estimator=1, seed=SEED)
+    d.settings.update(estimator=args.estimator, seed=SEED)
     o = oracle.OracleScene(d.width, d.height, d.vpi)
     d.populate(o, oracle.material)
     o.commitScene()
@@ -54,8 +59,8 @@ def main():
         print("frame %d/%d: %d spp, %.0f s, RMSE of the mean %.2e" % (
             f + 1, frames, n, time.time() - t0, float(np.sqrt(np.mean(se * se)))), flush=True)
         if (f + 1) % 16 == 0 or f + 1 == frames:
-            np.savez_compressed(args.out, mean=mean.astype(np.float32), stderr=se.astype(np.float32),
-                                spp=np.int64(n), seed=np.int64(SEED))
+            np.savez_compressed(out, mean=mean.astype(np.float32), stderr=se.astype(np.float32),
+                                spp=np.int64(n), seed=np.int64(SEED), estimator=np.int64(args.estimator))
 
 
 if __name__ == "__main__":

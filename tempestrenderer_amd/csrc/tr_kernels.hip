@@ -371,11 +371,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
  * its last node (a leaf cursor) or pops and tests the next node, and the
  * fetches of both kinds are in flight together.  The visiting order is
  * that of traverse_closest: a node's hit leaves, in visiting order, before
- * the next node is popped. */
+ * the next node is popped.  LEAVES instead tests a node and then all of
+ * its hit leaves' primitives in one iteration: for scenes of a few hundred
+ * primitives (at most kLeafLoopPrims), which stay in L1, the primitive
+ * fetches are short and the per-iteration work of the loop dominates
+ * (box scene, 12 primitives: 28.5 us per launch against 38.4 us; the
+ * 7,938-triangle gonioreflectometer patch is 5 % faster with one round
+ * trip per iteration).  The queue is dealt in chunks of 16 rays, or 64
+ * with LEAVES (box: 34.8 us with chunks of 16). */
 constexpr uint32_t kRefill = 16;
-constexpr uint32_t kRayChunk = 16;
+constexpr uint32_t kLeafLoopPrims = 1024;
 
-template <bool COUNT, bool SHAPES>
+template <bool COUNT, bool SHAPES, bool LEAVES>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRefillWavesPerSimd))) void k_trace_refill(FrameParams p, PathPool pool, int cur,
                                                               unsigned long long* counts)
 {
@@ -392,6 +399,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     st.sp = 0;
     uint32_t nv = 0, np = 0, nr = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
+    constexpr uint32_t kRayChunk = LEAVES ? 64u : 16u;
     auto ray_index = [&](uint32_t k) -> uint32_t {
         return (blockIdx.x + (k / kRayChunk) * grid) * kRayChunk + (k % kRayChunk);
     };
@@ -431,7 +439,44 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
             if (ray_index(next) < n) continue;
             break;
         }
-        if (active) {
+        if (LEAVES && active) {
+            /* one node of the ray's traversal and its hit leaves */
+            const uint32_t oct = r.octant;
+            uint32_t base, masks;
+            st.pop(&base, &masks);
+            uint32_t vis = masks & 0xffu;
+            const uint32_t imask = masks >> 8;
+            const uint32_t i = ctz32(vis);
+            vis &= vis - 1u;
+            if (vis) st.push(base, vis | (imask << 8));
+            const uint32_t cslot = i ^ oct;
+            const uint32_t node_index = base + popc8(imask & ((1u << cslot) - 1u));
+            NodeRegs nd = load_node(sc.nodes, node_index);
+            if (COUNT) ++nv;
+            const float tmax = best.geom == TR_INVALID_GEOMETRY ? INFINITY : best.t;
+            const uint32_t hit = intersect_children(nd, r, 0.0f, tmax);
+            const uint32_t nimask = node_imask(nd);
+            const uint32_t inner = hit & nimask;
+            uint32_t leaves = to_visit_order(hit & ~nimask, oct);
+            while (leaves) {
+                const uint32_t li = ctz32(leaves);
+                leaves &= leaves - 1u;
+                const uint32_t meta = node_meta(nd, li ^ oct);
+                const uint32_t first = nd.d[5] + (meta & 31u);
+                const uint32_t count = meta >> 5;
+                for (uint32_t k = 0; k < count; ++k) {
+                    if (COUNT) ++np;
+                    test_prim<SHAPES, false>(load_prim(sc.prims, first + k), sc.shapes, r, 0.0f, INFINITY, &best);
+                }
+            }
+            if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
+            if (st.empty()) {
+                if (best.geom == TR_INVALID_GEOMETRY) best.t = INFINITY;
+                pool.hit[slot] = make_float4(best.t, best.u, best.v, __uint_as_float(best.geom));
+                pool.hit_prim[slot] = best.prim;
+                active = false;
+            }
+        } else if (!LEAVES && active) {
             const uint32_t oct = r.octant;
             const bool prim_step = lv != 0;
             uint32_t count = 0, index;
@@ -1485,14 +1530,21 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
     unsigned long long* c = (unsigned long long*)counts;
     bool vol = p.sc.n_volumes > 0;
     bool shapes = p.sc.shapes != nullptr;
+    bool small = p.sc.n_prims <= kLeafLoopPrims;
     if (counts) {
         if (vol) launch_k(k_trace<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else if (shapes) launch_k(k_trace_refill<true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
-        else launch_k(k_trace_refill<true, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else if (small) {
+            if (shapes) launch_k(k_trace_refill<true, true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+            else launch_k(k_trace_refill<true, false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        } else if (shapes) launch_k(k_trace_refill<true, true, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else launch_k(k_trace_refill<true, false, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     } else {
         if (vol) launch_k(k_trace<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
-        else if (shapes) launch_k(k_trace_refill<false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
-        else launch_k(k_trace_refill<false, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else if (small) {
+            if (shapes) launch_k(k_trace_refill<false, true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+            else launch_k(k_trace_refill<false, false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        } else if (shapes) launch_k(k_trace_refill<false, true, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
+        else launch_k(k_trace_refill<false, false, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     }
 }
 

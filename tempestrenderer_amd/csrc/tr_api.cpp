@@ -84,7 +84,6 @@ struct tr_scene {
     std::vector<LightRec> lights;
     std::vector<VolumeRec> vols;
     std::vector<uint64_t> voxels;   /* voxel words of all volumes (kVoxelHalf / kVoxelQuantized) */
-    std::vector<uint16_t> dens;     /* their densities in 4x4x4 bricks (VolumeRec::dens_brick) */
     uint64_t dir_table = UINT64_MAX; /* brick volumes' direction tables in voxels */
     uint32_t instances = 0;
     bool committed = false;
@@ -110,7 +109,6 @@ struct tr_scene {
     DevBuf<VolumeRec> d_vols;
     DevBuf<ShapeRec> d_shapes;
     DevBuf<uint64_t> d_voxels;
-    DevBuf<uint16_t> d_dens;
     std::vector<TexRec> tex_recs;    /* bound textures */
     std::vector<uint8_t> texels;     /* their texels, then the cube map's faces */
     std::vector<uint8_t> cube;       /* six flipped RGBA8 faces */
@@ -179,7 +177,7 @@ tr_scene::~tr_scene()
     ev_used.clear();
     release_events(this);
     d_nodes.release(); d_prims.release(); d_geoms.release(); d_mats.release(); d_lights.release();
-    d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); d_dens.release(); sh_tmax.release(); d_shapes.release();
+    d_vpool.release(); d_ipool.release(); d_vols.release(); d_voxels.release(); sh_tmax.release(); d_shapes.release();
     query_out.release();
     ray_o.release(); ray_d.release(); hit.release(); thr.release(); local.release(); global.release();
     sh_o.release(); sh_d.release(); sh_c.release(); hit_prim.release(); hash.release(); q0.release(); q1.release();
@@ -244,7 +242,6 @@ SceneView scene_view(const tr_scene* s)
     v.ipool = s->d_ipool.ptr;
     v.volumes = s->d_vols.ptr;
     v.voxels = s->d_voxels.ptr;
-    v.dens = s->d_dens.ptr;
     v.shapes = s->shapes.empty() ? nullptr : s->d_shapes.ptr;   /* null selects the shape-free traversal */
     v.n_lights = (uint32_t)s->lights.size();
     v.n_prims = (uint32_t)s->prims.size();
@@ -679,33 +676,6 @@ void scene_box(tr_scene* s)
         s->box_lo[a] = lo[a];
         s->box_hi[a] = hi[a];
     }
-}
-
-}  // namespace
-
-namespace {
-
-/* The densities (bits 48-63) of the volume's voxel words, just appended in
- * x-fastest order, copied into 4 x 4 x 4 bricks of 128 B (x fastest inside
- * a brick and across bricks; nodes beyond the grid are 0 and never read). */
-bool append_density_bricks(tr_scene* s, VolumeRec* r)
-{
-    const uint32_t nb[3] = {(r->dims[0] + 3u) >> 2, (r->dims[1] + 3u) >> 2, (r->dims[2] + 3u) >> 2};
-    const uint64_t bricks = (uint64_t)nb[0] * nb[1] * nb[2];
-    const uint64_t first = s->dens.size() / 64u;
-    if (first + bricks > 0xffffffffull) return false;
-    r->dens_brick = (uint32_t)first;
-    s->dens.resize(s->dens.size() + bricks * 64u, 0);
-    uint16_t* out = s->dens.data() + first * 64u;
-    const uint64_t* w = s->voxels.data() + r->voxel_base;
-    for (uint32_t z = 0; z < r->dims[2]; ++z)
-        for (uint32_t y = 0; y < r->dims[1]; ++y)
-            for (uint32_t x = 0; x < r->dims[0]; ++x) {
-                const uint64_t b = ((uint64_t)(z >> 2) * nb[1] + (y >> 2)) * nb[0] + (x >> 2);
-                const uint32_t in = ((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u);
-                out[b * 64u + in] = (uint16_t)(w[((uint64_t)z * r->dims[1] + y) * r->dims[0] + x] >> 48);
-            }
-    return true;
 }
 
 }  // namespace
@@ -1392,7 +1362,6 @@ tr_status tr_scene_add_grid_volume(tr_scene* s, const tr_grid_volume* v, const t
     for (uint64_t i = 0; i < n; ++i)
         s->voxels.push_back((uint64_t)v->direction[i * 3] | ((uint64_t)v->direction[i * 3 + 1] << 16) |
                             ((uint64_t)v->direction[i * 3 + 2] << 32) | ((uint64_t)v->density[i] << 48));
-    if (!append_density_bricks(s, &r)) return fail(TR_ERROR_INVALID_ARG, "volume density bricks exceed 2^32");
     GeomRec g;
     memset(&g, 0, sizeof(g));
     g.type = GEOM_VOLUME;
@@ -1506,7 +1475,6 @@ tr_status tr_scene_add_hierarchical_volume(tr_scene* s, const tr_volume_root* ro
             const uint8_t* q = br.data + n + 2 * i;
             s->voxels.push_back((uint64_t)q[0] | ((uint64_t)q[1] << 8) | (u8_half_bits(br.data[i]) << 48));
         }
-        if (!append_density_bricks(s, &r)) return fail(TR_ERROR_INVALID_ARG, "volume density bricks exceed 2^32");
         GeomRec g;
         memset(&g, 0, sizeof(g));
         g.type = GEOM_VOLUME;
@@ -1672,7 +1640,6 @@ tr_status tr_scene_commit(tr_scene* s)
     if ((st = upload(s, s->d_ipool, s->ipool)) != TR_OK) return st;
     if ((st = upload(s, s->d_vols, s->vols)) != TR_OK) return st;
     if ((st = upload(s, s->d_voxels, s->voxels)) != TR_OK) return st;
-    if ((st = upload(s, s->d_dens, s->dens)) != TR_OK) return st;
     if ((st = upload(s, s->d_shapes, s->shapes)) != TR_OK) return st;
     if ((st = upload(s, s->d_tex, s->tex_recs)) != TR_OK) return st;
     if ((st = upload(s, s->d_texels, s->texels)) != TR_OK) return st;

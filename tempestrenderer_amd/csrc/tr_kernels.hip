@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(FrameParams p, PathPool p
                     }
                     const VolumeRec& V = sc.volumes[sc.geoms[h.geom].tri_count];
                     float tc;
-                    if (delta_track_any(sc.voxels, sc.dens, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
+                    if (delta_track_any(sc.voxels, V, ld4(o), ld4(d), h.t, h.u, rng, vertex, &step, &tc,
                                     COUNT ? &ns : nullptr)) {
                         h.t = tc;
                         h.u = h.v = 0.0f;
@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
                             const GeomRec& G = sc.geoms[h.geom];
                             if (G.type != GEOM_VOLUME || ++crossings > kMaxVolumeCrossings) break;
                             const VolumeRec& V = sc.volumes[G.tri_count];
-                            uint32_t succ = woodcock_any(sc.voxels, sc.dens, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
+                            uint32_t succ = woodcock_any(sc.voxels, V, ld4(o4), ld4(d4), h.t, h.u, rng, vertex, sub_hi,
                                                      &step, &step_through, COUNT ? &ns : nullptr);
                             tf = tf * ((float)succ / 2.0f);
                             if (tmax_T * tf < 1e-3f) break;

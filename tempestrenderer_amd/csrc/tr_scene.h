@@ -176,7 +176,7 @@ struct VolumeRec {
     float stddev[3];      /* SGGX standard deviation of the fibre frame (tangent, binormal, normal) */
     uint32_t format;      /* kVoxelHalf or kVoxelQuantized */
     uint32_t dims[3];
-    uint32_t dens_brick;  /* first 4x4x4 density brick in SceneView::dens */
+    uint32_t pad1;
     uint64_t voxel_base;  /* first sample in SceneView::voxels */
     uint64_t table_base;  /* kVoxelQuantized: the direction tables in SceneView::voxels */
 };
@@ -228,7 +228,6 @@ struct SceneView {
     const int32_t* ipool;
     const VolumeRec* volumes;
     const uint64_t* voxels; /* packed binary16 x4 samples of every grid volume */
-    const uint16_t* dens;   /* the same densities (binary16) in 4x4x4 bricks of 128 B */
     const ShapeRec* shapes;
     uint32_t n_lights;
     uint32_t n_prims;

@@ -44,90 +44,48 @@ struct VoxelCell {
     float f[3];      /* fractional position in the cell */
 };
 
-/* The grid cell around a point: lower and upper node index per axis
- * (clamped to the grid) and the fractional position. */
-struct CellCoords {
-    uint32_t i0[3], i1[3];
-    float f[3];
-};
-
-TR_HD CellCoords cell_coords(const VolumeRec& V, V3 pos)
+TR_HD VoxelCell fetch_cell(const uint64_t* voxels, const VolumeRec& V, V3 pos)
 {
-    CellCoords c;
+    VoxelCell c;
     float p[3] = {(pos.x - V.lo[0]) * V.scale[0], (pos.y - V.lo[1]) * V.scale[1], (pos.z - V.lo[2]) * V.scale[2]};
+    uint32_t i0[3], i1[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float dm = (float)(V.dims[a] - 1u);
         float v = p[a] > 0.0f ? p[a] : 0.0f;
         v = v < dm ? v : dm;
         float fl = floorf(v);
-        c.i0[a] = (uint32_t)fl;
-        c.i1[a] = c.i0[a] + 1u < V.dims[a] ? c.i0[a] + 1u : c.i0[a];
+        i0[a] = (uint32_t)fl;
+        i1[a] = i0[a] + 1u < V.dims[a] ? i0[a] + 1u : i0[a];
         c.f[a] = v - fl;
     }
-    return c;
-}
-
-TR_HD VoxelCell fetch_cell_at(const uint64_t* voxels, const VolumeRec& V, const CellCoords& cc)
-{
-    VoxelCell c;
-    c.f[0] = cc.f[0]; c.f[1] = cc.f[1]; c.f[2] = cc.f[2];
     const uint64_t* base = voxels + V.voxel_base;
     uint64_t sx = V.dims[0], sxy = (uint64_t)V.dims[0] * V.dims[1];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        uint64_t idx = (uint64_t)((k & 4) ? cc.i1[2] : cc.i0[2]) * sxy + (uint64_t)((k & 2) ? cc.i1[1] : cc.i0[1]) * sx +
-                       (uint64_t)((k & 1) ? cc.i1[0] : cc.i0[0]);
+        uint64_t idx = (uint64_t)((k & 4) ? i1[2] : i0[2]) * sxy + (uint64_t)((k & 2) ? i1[1] : i0[1]) * sx +
+                       (uint64_t)((k & 1) ? i1[0] : i0[0]);
         c.s[k] = base[idx];
     }
     return c;
-}
-
-TR_HD VoxelCell fetch_cell(const uint64_t* voxels, const VolumeRec& V, V3 pos)
-{
-    return fetch_cell_at(voxels, V, cell_coords(V, pos));
-}
-
-/* The eight binary16 densities of a cell from the volume's density bricks:
- * 4 x 4 x 4 nodes per brick, 128 B (one cache line), x fastest inside a
- * brick and across bricks, so that the corners of most cells share one or
- * two lines (a lookup in the x-fastest grid touches four). */
-TR_HD void fetch_cell_densities(const uint16_t* dens, const VolumeRec& V, const CellCoords& cc, uint32_t h[8])
-{
-    const uint32_t nbx = (V.dims[0] + 3u) >> 2, nby = (V.dims[1] + 3u) >> 2;
-    const uint16_t* base = dens + (uint64_t)V.dens_brick * 64u;
-    const uint32_t ox[2] = {((cc.i0[0] >> 2) << 6) | (cc.i0[0] & 3u), ((cc.i1[0] >> 2) << 6) | (cc.i1[0] & 3u)};
-    const uint32_t oy[2] = {(cc.i0[1] >> 2) * nbx * 64u + ((cc.i0[1] & 3u) << 2),
-                            (cc.i1[1] >> 2) * nbx * 64u + ((cc.i1[1] & 3u) << 2)};
-    const uint64_t oz[2] = {(uint64_t)(cc.i0[2] >> 2) * nbx * nby * 64u + ((cc.i0[2] & 3u) << 4),
-                            (uint64_t)(cc.i1[2] >> 2) * nbx * nby * 64u + ((cc.i1[2] & 3u) << 4)};
-#pragma unroll
-    for (int k = 0; k < 8; ++k) h[k] = base[oz[(k >> 2) & 1] + oy[(k >> 1) & 1] + ox[k & 1]];
 }
 
 /* Trilinear density in the reference's evaluation order
  * (TrilinearInterpolation, ray-tracing.cc:89-108): corner weights
  * (wx * wy) * wz, the two z layers summed per (x, y) corner, then the four
  * corner sums pairwise. */
-TR_HD float trilinear_density(const uint32_t h[8], const float f[3])
+TR_HD float cell_density(const VoxelCell& c)
 {
-    float fx = f[0], fy = f[1], fz = f[2];
+    float fx = c.f[0], fy = c.f[1], fz = c.f[2];
     float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
     float wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
     float lane[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float r0 = h2f(h[k]), r1 = h2f(h[k + 4]);
+        float r0 = h2f((uint32_t)(c.s[k] >> 48)), r1 = h2f((uint32_t)(c.s[k + 4] >> 48));
         lane[k] = r0 * (wxy[k] * gz) + r1 * (wxy[k] * fz);
     }
     return (lane[0] + lane[1]) + (lane[2] + lane[3]);
-}
-TR_HD float cell_density(const VoxelCell& c)
-{
-    uint32_t h[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) h[k] = (uint32_t)(c.s[k] >> 48);
-    return trilinear_density(h, c.f);
 }
 
 /* Fibre direction at the point: dominant eigenvector of the structure
@@ -208,24 +166,22 @@ TR_HD M3 volume_frame_any(const VoxelCell& c, const uint64_t* voxels, const Volu
  * against the threshold, then (DensityScale / 255) * density * sigma *
  * RcpMaxExtinction. */
 template <bool QUANT>
-TR_HD int volume_accept(const uint64_t* voxels, const uint16_t* dens_bricks, const VolumeRec& V, V3 pos, V3 dir,
-                        float threshold)
+TR_HD int accept_cell(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V, V3 dir, float threshold)
 {
-    /* the densities first, from the density bricks; the fibre directions
-     * only for the collisions the density does not reject */
-    const CellCoords cc = cell_coords(V, pos);
-    uint32_t h[8];
-    fetch_cell_densities(dens_bricks, V, cc, h);
-    float rho = trilinear_density(h, cc.f);
+    float rho = cell_density(c);
     if (rho == 0.0f) return 2;
     const bool quant = QUANT;
     float dens = quant ? rho / 255.0f : rho * V.rcp_rho_max;
     if (dens < threshold) return 0;
-    const VoxelCell c = fetch_cell_at(voxels, V, cc);
     M3 basis = volume_frame<QUANT>(c, voxels, V);
     float sigma = length(mulv(transform_rotation_inverse(basis, dir), ld3(V.stddev)));
     if (quant) return (kDensityScale / 255.0f) * rho * sigma * V.rcp_majorant >= threshold ? 1 : 0;
     return dens * sigma * V.rcp_smax >= threshold ? 1 : 0;
+}
+template <bool QUANT>
+TR_HD int volume_accept(const uint64_t* voxels, const VolumeRec& V, V3 pos, V3 dir, float threshold)
+{
+    return accept_cell<QUANT>(fetch_cell(voxels, V, pos), voxels, V, dir, threshold);
 }
 
 /* Tracking budget: a walk that makes this many tentative collisions in one
@@ -246,44 +202,66 @@ enum { STREAM_VOLUME = 4, STREAM_SHADOW_VOLUME = 5 };
 /* Delta tracking from the volume entry t0 to its exit t1.  Returns true and
  * the collision distance in *t when a collision is accepted, false and the
  * distance beyond the exit where the walk left the volume otherwise.
- * `step` counts tentative collisions of this path vertex across volumes. */
+ * `step` counts tentative collisions of this path vertex across volumes.
+ * Collision k draws its free flight and acceptance numbers from words
+ * (a, b) (k even) or (c, d) (k odd) of Philox draw k >> 1.  A free flight
+ * does not depend on the previous collision's outcome, so the walk takes
+ * collisions in pairs: both cells are fetched together (one memory round
+ * trip for two collisions; the second fetch is wasted when the first
+ * collision is accepted, and clamped into the grid when it lies beyond the
+ * exit), then tested in order, with the same results as one at a time. */
 template <bool QUANT>
-TR_HD bool delta_track(const uint64_t* voxels, const uint16_t* dens, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+TR_HD bool delta_track(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
                        const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
 {
     float tc = t0;
-    U4 r;
-    bool have = false;
     for (;;) {
-        uint32_t k = *step;
-        if (!have || (k & 1u) == 0u) r = rng.draw(vertex, STREAM_VOLUME, k >> 1);
-        have = true;
-        uint32_t xa = (k & 1u) ? r.c : r.a, xb = (k & 1u) ? r.d : r.b;
+        const uint32_t k = *step;
+        const U4 r0 = rng.draw(vertex, STREAM_VOLUME, k >> 1);
+        const uint32_t xa1 = (k & 1u) ? r0.c : r0.a, xb1 = (k & 1u) ? r0.d : r0.b;
         *step = k + 1u;
         if (nsteps) ++*nsteps;
         /* d = logf(xi) * RcpMaxExtinction; pos -= dir * d */
-        tc = tc - tr_logf(u01(xa)) * V.rcp_majorant;
-        if (tc >= t1) {
-            *t = tc;
+        const float tc1 = tc - tr_logf(u01(xa1)) * V.rcp_majorant;
+        if (tc1 >= t1) {
+            *t = tc1;
             return false;
         }
         if (k + 1u >= kMaxVolumeSteps) {
-            *t = tc;
+            *t = tc1;
             return true;
         }
-        int acc = volume_accept<QUANT>(voxels, dens, V, ray_point(org, dir, tc), dir, u01(xb));
-        if (acc == 1) {
-            *t = tc;
+        const U4 r1 = (k & 1u) ? rng.draw(vertex, STREAM_VOLUME, (k + 1u) >> 1) : r0;
+        const uint32_t xa2 = (k & 1u) ? r1.a : r1.c, xb2 = (k & 1u) ? r1.b : r1.d;
+        const float tc2 = tc1 - tr_logf(u01(xa2)) * V.rcp_majorant;
+        const VoxelCell c1 = fetch_cell(voxels, V, ray_point(org, dir, tc1));
+        const VoxelCell c2 = fetch_cell(voxels, V, ray_point(org, dir, tc2));
+        if (accept_cell<QUANT>(c1, voxels, V, dir, u01(xb1)) == 1) {
+            *t = tc1;
             return true;
         }
+        *step = k + 2u;
+        if (nsteps) ++*nsteps;
+        if (tc2 >= t1) {
+            *t = tc2;
+            return false;
+        }
+        if (k + 2u >= kMaxVolumeSteps) {
+            *t = tc2;
+            return true;
+        }
+        if (accept_cell<QUANT>(c2, voxels, V, dir, u01(xb2)) == 1) {
+            *t = tc2;
+            return true;
+        }
+        tc = tc2;
     }
 }
-TR_HD bool delta_track_any(const uint64_t* voxels, const uint16_t* dens, const VolumeRec& V, V3 org, V3 dir, float t0,
-                           float t1, const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
+TR_HD bool delta_track_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+                           const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
 {
-    if (V.format == kVoxelQuantized)
-        return delta_track<true>(voxels, dens, V, org, dir, t0, t1, rng, vertex, step, t, nsteps);
-    return delta_track<false>(voxels, dens, V, org, dir, t0, t1, rng, vertex, step, t, nsteps);
+    if (V.format == kVoxelQuantized) return delta_track<true>(voxels, V, org, dir, t0, t1, rng, vertex, step, t, nsteps);
+    return delta_track<false>(voxels, V, org, dir, t0, t1, rng, vertex, step, t, nsteps);
 }
 
 /* The two Woodcock repetitions of a shadow ray through one volume crossing
@@ -294,7 +272,7 @@ TR_HD bool delta_track_any(const uint64_t* voxels, const uint16_t* dens, const V
  * from the shadow-ray origin, which skips a volume lying further than one
  * free path from the shading point.  Returns the number of successes. */
 template <bool QUANT>
-TR_HD uint32_t woodcock(const uint64_t* voxels, const uint16_t* dens, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+TR_HD uint32_t woodcock(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
                         const Rng& rng, uint32_t vertex, uint32_t sub_hi, uint32_t* step, float* step_through,
                         uint32_t* nsteps)
 {
@@ -317,7 +295,7 @@ TR_HD uint32_t woodcock(const uint64_t* voxels, const uint16_t* dens, const Volu
                 ++success;
                 break;
             }
-            int acc = volume_accept<QUANT>(voxels, dens, V, ray_point(org, dir, sd), dir, u01(xb));
+            int acc = volume_accept<QUANT>(voxels, V, ray_point(org, dir, sd), dir, u01(xb));
             if (acc == 2) {
                 *step_through = sd;
                 ++success;
@@ -328,13 +306,13 @@ TR_HD uint32_t woodcock(const uint64_t* voxels, const uint16_t* dens, const Volu
     }
     return success;
 }
-TR_HD uint32_t woodcock_any(const uint64_t* voxels, const uint16_t* dens, const VolumeRec& V, V3 org, V3 dir, float t0,
-                            float t1, const Rng& rng, uint32_t vertex, uint32_t sub_hi, uint32_t* step,
-                            float* step_through, uint32_t* nsteps)
+TR_HD uint32_t woodcock_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
+                            const Rng& rng, uint32_t vertex, uint32_t sub_hi, uint32_t* step, float* step_through,
+                            uint32_t* nsteps)
 {
     if (V.format == kVoxelQuantized)
-        return woodcock<true>(voxels, dens, V, org, dir, t0, t1, rng, vertex, sub_hi, step, step_through, nsteps);
-    return woodcock<false>(voxels, dens, V, org, dir, t0, t1, rng, vertex, sub_hi, step, step_through, nsteps);
+        return woodcock<true>(voxels, V, org, dir, t0, t1, rng, vertex, sub_hi, step, step_through, nsteps);
+    return woodcock<false>(voxels, V, org, dir, t0, t1, rng, vertex, sub_hi, step, step_through, nsteps);
 }
 
 }  // namespace trk

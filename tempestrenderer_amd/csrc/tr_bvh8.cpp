@@ -54,6 +54,7 @@ struct Builder {
 
 constexpr int kBins = 16;
 constexpr uint32_t kMaxLeaf = 4;
+constexpr float kNodeCost = 0.25f;
 
 /* Builds the subtree over order[begin, end) into `out` (a local node list),
  * returning the index of its root in `out`. */
@@ -117,8 +118,14 @@ uint32_t build_rec(Builder& b, std::vector<BNode>& out, uint32_t begin, uint32_t
             if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = k; }
         }
     }
+    /* SAH with a node-test cost of a quarter of a primitive test: after the
+     * collapse into 8-wide nodes most binary nodes disappear, and a
+     * primitive in a leaf costs a memory round trip of its own in the
+     * traversal (heightfield: 1.81 -> 1.67 primitives per segment at 5.57
+     * nodes, 0.9 % faster; a cost of 1 / 2 / 3 primitive tests gave
+     * 1.81 / 2.71 / 3.19 primitives per segment and 0 / -11 / -16 %) */
     float leaf_cost = node.box.area() * (float)n;
-    if (n <= kMaxLeaf && (best_axis < 0 || leaf_cost <= best_cost + node.box.area())) {
+    if (n <= kMaxLeaf && (best_axis < 0 || leaf_cost <= best_cost + kNodeCost * node.box.area())) {
         out[idx].first = begin;
         out[idx].count = n;
         return idx;

@@ -558,6 +558,126 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     }
 }
 
+/* Closest hit with delta tracking for scenes with volumes, with ray refill
+ * (k_trace_refill's scheme: a lane whose ray has finished takes the wave's
+ * next ray once kRefill lanes are idle; the queue is dealt in chunks of
+ * kVolChunk rays).  A lane either searches its ray's closest hit from its
+ * current near bound (a whole traversal; volume scenes hold few
+ * primitives) or, inside a volume box, takes one pair of tentative
+ * collisions per iteration, so lanes whose walk ends early take new rays
+ * instead of waiting for the longest walk of a fixed group of 64.  Each
+ * ray follows k_trace's sequence exactly: a walk that leaves the box
+ * re-traces from beyond its exit, at most kMaxVolumeCrossings times. */
+constexpr uint32_t kVolChunk = 16;
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_vol(FrameParams p, PathPool pool, BatchOut out,
+                                                           uint64_t slot_base, uint32_t vertex, int cur,
+                                                           unsigned long long* counts)
+{
+    __shared__ uint2 stk[kStackLds][kTraceBlock];
+    const SceneView& sc = p.sc;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
+    const uint32_t n = pool.counters[cur];
+    const uint32_t* queue = pool.queue[cur];
+    const uint32_t grid = gridDim.x;
+    DevStack st;
+    st.lds = &stk[0][lane];
+    st.spill = pool.spill + (size_t)gtid * kStackSpill;
+    st.sp = 0;
+    uint32_t nv = 0, np = 0, nr = 0, ns = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    auto ray_index = [&](uint32_t k) -> uint32_t {
+        return (blockIdx.x + (k / kVolChunk) * grid) * kVolChunk + (k % kVolChunk);
+    };
+    uint32_t next = 0;
+    bool active = false, tracking = false;
+    uint32_t slot = 0, step = 0, crossings = 0, vol = 0;
+    float tnear = 0.0f, tc = 0.0f, t_exit = 0.0f;
+    V3 org = mk(0.0f, 0.0f, 0.0f), dir = mk(0.0f, 0.0f, 0.0f);
+    RayPre r;
+    Rng rng;
+    Hit h;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const bool more = ray_index(next) < n;
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (more && (n_idle >= kRefill || idle == ~0ull)) {
+            if (!active) {
+                const uint32_t qi = ray_index(next + (uint32_t)__popcll(idle & lt));
+                if (qi < n) {
+                    slot = queue[qi];
+                    const float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
+                    org = ld4(o);
+                    dir = ld4(d);
+                    const float ao[3] = {o.x, o.y, o.z}, ad[3] = {d.x, d.y, d.z};
+                    r = ray_setup(ao, ad);
+                    uint32_t pixel, cs, split;
+                    uint64_t pseq;
+                    slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+                    rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
+                    tnear = 0.0f;
+                    step = 0;
+                    crossings = 0;
+                    tracking = false;
+                    active = true;
+                    ++nr;
+                }
+            }
+            next += n_idle;
+        }
+        if (__ballot(active) == 0ull) {
+            if (ray_index(next) < n) continue;
+            break;
+        }
+        bool done = false;
+        if (active && !tracking) {
+            st.sp = 0;
+            h = traverse_closest(sc.nodes, sc.prims, sc.shapes, r, tnear, INFINITY, st, COUNT ? &nv : nullptr,
+                                 COUNT ? &np : nullptr);
+            if (h.geom != TR_INVALID_GEOMETRY && sc.geoms[h.geom].type == GEOM_VOLUME) {
+                /* the tracking loops of subdraw (ray-tracing.cc:2040-2128, 2212-2272) */
+                tracking = true;
+                vol = sc.geoms[h.geom].tri_count;
+                tc = h.t;
+                t_exit = h.u;
+            } else {
+                done = true;
+            }
+        }
+        if (active && tracking) {
+            const int res = delta_track_pair_any(sc.voxels, sc.volumes[vol], org, dir, &tc, t_exit, rng, vertex,
+                                                 &step, COUNT ? &ns : nullptr);
+            if (res > 0) {
+                h.t = tc;
+                h.u = h.v = 0.0f;
+                done = true;
+            } else if (res < 0) {
+                tracking = false;
+                if (++crossings >= kMaxVolumeCrossings) {
+                    h.geom = h.prim = TR_INVALID_GEOMETRY;
+                    h.t = INFINITY;
+                    done = true;
+                } else {
+                    tnear = tc;
+                }
+            }
+        }
+        if (done) {
+            pool.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.geom));
+            pool.hit_prim[slot] = h.prim;
+            active = false;
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counts[0], (unsigned long long)nv);
+        atomicAdd(&counts[1], (unsigned long long)np);
+        atomicAdd(&counts[2], (unsigned long long)nr);
+        atomicAdd(&counts[6], (unsigned long long)ns);
+    }
+}
+
 /* ---------------- shading ---------------- */
 
 /* The texture view of k_shade: the scene's textures, or none. */
@@ -1532,14 +1652,14 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
     bool shapes = p.sc.shapes != nullptr;
     bool small = p.sc.n_prims <= kLeafLoopPrims;
     if (counts) {
-        if (vol) launch_k(k_trace<true, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+        if (vol) launch_k(k_trace_vol<true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, c);
         else if (small) {
             if (shapes) launch_k(k_trace_refill<true, true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
             else launch_k(k_trace_refill<true, false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
         } else if (shapes) launch_k(k_trace_refill<true, true, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
         else launch_k(k_trace_refill<true, false, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     } else {
-        if (vol) launch_k(k_trace<false, true>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, nxt, c);
+        if (vol) launch_k(k_trace_vol<false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, c);
         else if (small) {
             if (shapes) launch_k(k_trace_refill<false, true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
             else launch_k(k_trace_refill<false, false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);

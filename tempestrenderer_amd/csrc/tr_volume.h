@@ -199,62 +199,68 @@ constexpr uint32_t kMaxVolumeCrossings = 256u;
  * (a, b) then (c, d) = (free-flight, acceptance). */
 enum { STREAM_VOLUME = 4, STREAM_SHADOW_VOLUME = 5 };
 
-/* Delta tracking from the volume entry t0 to its exit t1.  Returns true and
- * the collision distance in *t when a collision is accepted, false and the
- * distance beyond the exit where the walk left the volume otherwise.
- * `step` counts tentative collisions of this path vertex across volumes.
- * Collision k draws its free flight and acceptance numbers from words
- * (a, b) (k even) or (c, d) (k odd) of Philox draw k >> 1.  A free flight
- * does not depend on the previous collision's outcome, so the walk takes
- * collisions in pairs: both cells are fetched together (one memory round
+/* Delta tracking from the volume entry t0 to its exit t1, one pair of
+ * tentative collisions per call.  Collision k draws its free flight and
+ * acceptance numbers from words (a, b) (k even) or (c, d) (k odd) of Philox
+ * draw k >> 1.  A free flight does not depend on the previous collision's
+ * outcome, so both cells of a pair are fetched together (one memory round
  * trip for two collisions; the second fetch is wasted when the first
  * collision is accepted, and clamped into the grid when it lies beyond the
- * exit), then tested in order, with the same results as one at a time. */
+ * exit), then tested in order, with the results of one at a time.  *tc is
+ * the walk's position; returns 1 when a collision is accepted at *tc (or
+ * the step budget is spent), -1 when the walk left the volume (*tc beyond
+ * the exit), 0 when it goes on.  `step` counts tentative collisions of
+ * this path vertex across volumes. */
+template <bool QUANT>
+TR_HD int delta_track_pair(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float* tc, float t1,
+                           const Rng& rng, uint32_t vertex, uint32_t* step, uint32_t* nsteps)
+{
+    const uint32_t k = *step;
+    const U4 r0 = rng.draw(vertex, STREAM_VOLUME, k >> 1);
+    const uint32_t xa1 = (k & 1u) ? r0.c : r0.a, xb1 = (k & 1u) ? r0.d : r0.b;
+    *step = k + 1u;
+    if (nsteps) ++*nsteps;
+    /* d = logf(xi) * RcpMaxExtinction; pos -= dir * d */
+    const float tc1 = *tc - tr_logf(u01(xa1)) * V.rcp_majorant;
+    *tc = tc1;
+    if (tc1 >= t1) return -1;
+    if (k + 1u >= kMaxVolumeSteps) return 1;
+    const U4 r1 = (k & 1u) ? rng.draw(vertex, STREAM_VOLUME, (k + 1u) >> 1) : r0;
+    const uint32_t xa2 = (k & 1u) ? r1.a : r1.c, xb2 = (k & 1u) ? r1.b : r1.d;
+    const float tc2 = tc1 - tr_logf(u01(xa2)) * V.rcp_majorant;
+    const VoxelCell c1 = fetch_cell(voxels, V, ray_point(org, dir, tc1));
+    const VoxelCell c2 = fetch_cell(voxels, V, ray_point(org, dir, tc2));
+    if (accept_cell<QUANT>(c1, voxels, V, dir, u01(xb1)) == 1) return 1;
+    *step = k + 2u;
+    if (nsteps) ++*nsteps;
+    *tc = tc2;
+    if (tc2 >= t1) return -1;
+    if (k + 2u >= kMaxVolumeSteps) return 1;
+    if (accept_cell<QUANT>(c2, voxels, V, dir, u01(xb2)) == 1) return 1;
+    return 0;
+}
+TR_HD int delta_track_pair_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float* tc, float t1,
+                               const Rng& rng, uint32_t vertex, uint32_t* step, uint32_t* nsteps)
+{
+    if (V.format == kVoxelQuantized)
+        return delta_track_pair<true>(voxels, V, org, dir, tc, t1, rng, vertex, step, nsteps);
+    return delta_track_pair<false>(voxels, V, org, dir, tc, t1, rng, vertex, step, nsteps);
+}
+
+/* The whole walk: true and the collision distance in *t when a collision
+ * is accepted, false and the distance beyond the exit where the walk left
+ * the volume otherwise. */
 template <bool QUANT>
 TR_HD bool delta_track(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,
                        const Rng& rng, uint32_t vertex, uint32_t* step, float* t, uint32_t* nsteps)
 {
     float tc = t0;
     for (;;) {
-        const uint32_t k = *step;
-        const U4 r0 = rng.draw(vertex, STREAM_VOLUME, k >> 1);
-        const uint32_t xa1 = (k & 1u) ? r0.c : r0.a, xb1 = (k & 1u) ? r0.d : r0.b;
-        *step = k + 1u;
-        if (nsteps) ++*nsteps;
-        /* d = logf(xi) * RcpMaxExtinction; pos -= dir * d */
-        const float tc1 = tc - tr_logf(u01(xa1)) * V.rcp_majorant;
-        if (tc1 >= t1) {
-            *t = tc1;
-            return false;
+        const int res = delta_track_pair<QUANT>(voxels, V, org, dir, &tc, t1, rng, vertex, step, nsteps);
+        if (res != 0) {
+            *t = tc;
+            return res > 0;
         }
-        if (k + 1u >= kMaxVolumeSteps) {
-            *t = tc1;
-            return true;
-        }
-        const U4 r1 = (k & 1u) ? rng.draw(vertex, STREAM_VOLUME, (k + 1u) >> 1) : r0;
-        const uint32_t xa2 = (k & 1u) ? r1.a : r1.c, xb2 = (k & 1u) ? r1.b : r1.d;
-        const float tc2 = tc1 - tr_logf(u01(xa2)) * V.rcp_majorant;
-        const VoxelCell c1 = fetch_cell(voxels, V, ray_point(org, dir, tc1));
-        const VoxelCell c2 = fetch_cell(voxels, V, ray_point(org, dir, tc2));
-        if (accept_cell<QUANT>(c1, voxels, V, dir, u01(xb1)) == 1) {
-            *t = tc1;
-            return true;
-        }
-        *step = k + 2u;
-        if (nsteps) ++*nsteps;
-        if (tc2 >= t1) {
-            *t = tc2;
-            return false;
-        }
-        if (k + 2u >= kMaxVolumeSteps) {
-            *t = tc2;
-            return true;
-        }
-        if (accept_cell<QUANT>(c2, voxels, V, dir, u01(xb2)) == 1) {
-            *t = tc2;
-            return true;
-        }
-        tc = tc2;
     }
 }
 TR_HD bool delta_track_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float t0, float t1,

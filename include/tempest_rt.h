@@ -162,7 +162,8 @@ typedef struct tr_mesh_options {
 /* RTSettings (ray-tracing-common.hh:141-144) plus the device path pool. */
 typedef struct tr_settings {
     uint32_t data_pool_size;   /* accepted for compatibility, unused */
-    uint32_t path_pool_size;   /* paths resident per batch; 0 = automatic */
+    uint32_t path_pool_size;   /* paths resident per batch; 0 = automatic: the frame's paths, at most
+                                  2^27 and a quarter of the device memory free when the pool is made */
     uint32_t trace_waves_per_cu; /* persistent traversal waves per CU, 1..32; 0 = automatic (24 for
                                     surface scenes, 16 with volumes) */
     uint32_t shade_order;      /* 0: shading runs paths in queue order (slot order, coalesced);
@@ -582,7 +583,7 @@ typedef struct tr_request_options {
     int32_t use_graph;           /* 1: replay one captured graph per batch; 0: plain launches */
     int32_t autotune;            /* 1: time candidate pool sizes and traversal grids on the
                                   * first batches and keep the fastest for the rest */
-    uint32_t pool_slots;         /* paths per batch without autotuning (0: 2^22) */
+    uint32_t pool_slots;         /* paths per batch without autotuning (0: 2^24) */
     uint32_t trace_waves_per_cu; /* persistent traversal waves per CU without autotuning (0: 16) */
 } tr_request_options;
 

@@ -287,9 +287,31 @@ uint32_t shadow_entries(const tr_scene* s)
     return n;
 }
 
+/* Device bytes per path slot: six float4 and six uint32 arrays, three
+ * queue segments, three more words, and the shadow entries. */
+uint64_t slot_bytes(uint32_t max_shadow) { return 96u + 24u + 12u + 12u + 48ull * max_shadow; }
+
+/* The default path pool: the whole frame's paths resident at once, up to
+ * 2^27 slots (25 GB at one shadow entry per slot) and a quarter of the
+ * device memory free now.  Fewer, larger batches leave the persistent
+ * traversal less tail per launch and fewer nearly empty late-bounce
+ * launches: heightfield frame 2^22 / 2^23 / 2^24 / 2^25 / 2^26 / 2^27
+ * slots: 2,112 / 2,614 / 3,029 / 3,253 / 3,340 / 3,450 Msamples/s. */
+uint32_t default_pool_slots(uint32_t max_shadow)
+{
+    uint32_t slots = 1u << 27;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
+        const uint64_t fit = (uint64_t)free_b / 4u / slot_bytes(max_shadow);
+        while (slots > (1u << 20) && slots > fit) slots >>= 1;
+    }
+    return slots;
+}
+
 tr_status ensure_pool(tr_scene* s, uint64_t slots_needed)
 {
-    uint32_t want = s->settings.path_pool_size ? s->settings.path_pool_size : (1u << 22);
+    uint32_t want = s->settings.path_pool_size ? s->settings.path_pool_size
+                                               : default_pool_slots(std::max(1u, shadow_entries(s)));
     uint64_t per_pixel = (uint64_t)s->spp * s->n_gi;
     if (per_pixel > want) want = (uint32_t)per_pixel;
     if (slots_needed < want) want = (uint32_t)std::max<uint64_t>(slots_needed, per_pixel);
@@ -1927,7 +1949,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
     memset(&o, 0, sizeof(o));
     o.use_graph = 1;
     if (opt) o = *opt;
-    if (o.pool_slots == 0) o.pool_slots = 1u << 22;
+    if (o.pool_slots == 0) o.pool_slots = 1u << 24;
     if (o.trace_waves_per_cu == 0) o.trace_waves_per_cu = 16;
     if (o.trace_waves_per_cu > 32) return fail(TR_ERROR_INVALID_ARG, "trace_waves_per_cu above 32");
     if (s->frame_pending) {
@@ -1963,8 +1985,8 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
     /* candidate configurations: (paths per batch, traversal waves per CU) */
     std::vector<std::pair<uint32_t, uint32_t>> cands;
     if (o.autotune) {
-        for (uint32_t pool : {1u << 20, 1u << 21, 1u << 22})
-            for (uint32_t w : {8u, 16u, 24u}) cands.push_back({pool, w});
+        for (uint32_t pool : {1u << 22, 1u << 23, 1u << 24, 1u << 25})
+            for (uint32_t w : {16u, 24u}) cands.push_back({pool, w});
     } else {
         cands.push_back({o.pool_slots, o.trace_waves_per_cu});
     }

@@ -143,7 +143,7 @@ struct LaunchEvents {
 
 constexpr int kTraceBlock = 64;       /* one wave per block */
 constexpr uint32_t kSegment = 256;    /* shading block = queue segment */
-constexpr uint32_t kCompactGroups = 256; /* k_compact blocks per queue, at most */
+constexpr uint32_t kCompactGroups = 4096; /* k_compact blocks per queue, at most (a multiple of kSegment) */
 constexpr int kStackLds = 10;         /* stack entries per lane kept in LDS (5 KB per wave) */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
 /* Waves per SIMD the refill traversal is compiled for (its register budget,

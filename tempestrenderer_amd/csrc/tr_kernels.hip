@@ -1156,11 +1156,41 @@ __global__ __launch_bounds__(kSegment) void k_compact_sums(PathPool pool, uint32
     if (threadIdx.x == 0) group_sum[k * gridDim.x + blockIdx.x] = total;
 }
 
+/* Exclusive scan of queue k's group sums (block k, up to kCompactGroups
+ * of them), in place: group x's sum becomes the number of entries in the
+ * groups before it; the queue size goes to its counter. */
+__global__ __launch_bounds__(kSegment) void k_compact_scan(PathPool pool, uint32_t n_groups, int nxt,
+                                                           uint32_t* group_sum)
+{
+    const uint32_t k = blockIdx.x;
+    uint32_t* gs = group_sum + k * n_groups;
+    constexpr uint32_t kPer = kCompactGroups / kSegment;
+    uint32_t v[kPer];
+    uint32_t part = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t g = threadIdx.x * kPer + j;
+        v[j] = g < n_groups ? gs[g] : 0u;
+        part += v[j];
+    }
+    uint32_t total;
+    uint32_t before = block_exclusive_scan(part, &total);
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t g = threadIdx.x * kPer + j;
+        if (g < n_groups) gs[g] = before;
+        before += v[j];
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t slot = k == 0 ? (uint32_t)nxt : k + 1;
+        pool.counters[slot] = total;
+    }
+}
+
 /* Block (x, k) copies queue k's segments of group x to the dense queue at
- * the offset given by the group sums before it (at most a few hundred);
- * the last block writes the queue size. */
+ * the offset k_compact_scan left in the group's sum. */
 __global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_seg, uint32_t group, int nxt,
-                                                      const uint32_t* group_sum)
+                                                      const uint32_t* group_off)
 {
     const uint32_t k = blockIdx.y;
     const uint32_t* cnt = pool.seg_n + (size_t)k * pool.seg_cap;
@@ -1168,11 +1198,7 @@ __global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_
     uint32_t* dst = k == 0 ? pool.queue[nxt] : (k == 1 ? pool.shadow_queue : pool.query_queue);
     const uint32_t s0 = blockIdx.x * group;
     const uint32_t s1 = min(s0 + group, n_seg);
-    const uint32_t* gs = group_sum + k * gridDim.x;
-    uint32_t part = 0;
-    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kSegment) part += gs[j];
-    uint32_t base;
-    (void)block_exclusive_scan(part, &base);
+    uint32_t base = group_off[k * gridDim.x + blockIdx.x];
     for (uint32_t c0 = s0; c0 < s1; c0 += kSegment) {
         __shared__ uint32_t sc[kSegment], so[kSegment];
         uint32_t c = c0 + threadIdx.x < s1 ? cnt[c0 + threadIdx.x] : 0u;
@@ -1199,10 +1225,6 @@ __global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_
         }
         __syncthreads();
         base += sum;
-    }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        const uint32_t slot = k == 0 ? (uint32_t)nxt : k + 1;
-        pool.counters[slot] = base;
     }
 }
 
@@ -1698,19 +1720,23 @@ void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int n
                     const LaunchEvents* ev)
 {
     uint32_t n_seg = (n_slots + kSegment - 1) / kSegment;
-    /* about 256 blocks per queue: the per-block prefix reads stay a small
-     * multiple of the count array */
-    uint32_t group = (n_seg + 255) / 256;
+    /* groups of at least 16 segments, at most kCompactGroups of them per
+     * queue: sums, one scan of the sums, then the copies */
+    uint32_t group = (n_seg + kCompactGroups - 1) / kCompactGroups;
     if (group < 16) group = 16;
     uint32_t blocks = (n_seg + group - 1) / group;   /* at most kCompactGroups */
-    dim3 grid(blocks, queries ? 3 : 2);
+    const uint32_t nq = queries ? 3 : 2;
+    dim3 grid(blocks, nq);
     if (ev) {
         hipExtLaunchKernelGGL(k_compact_sums, grid, dim3(kSegment), 0, s, ev->start, nullptr, 0u, pool, n_seg, group,
+                              pool.group_sum);
+        hipExtLaunchKernelGGL(k_compact_scan, dim3(nq), dim3(kSegment), 0, s, nullptr, nullptr, 0u, pool, blocks, nxt,
                               pool.group_sum);
         hipExtLaunchKernelGGL(k_compact, grid, dim3(kSegment), 0, s, nullptr, ev->stop, 0u, pool, n_seg, group, nxt,
                               (const uint32_t*)pool.group_sum);
     } else {
         k_compact_sums<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, pool.group_sum);
+        k_compact_scan<<<dim3(nq), dim3(kSegment), 0, s>>>(pool, blocks, nxt, pool.group_sum);
         k_compact<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, nxt, (const uint32_t*)pool.group_sum);
     }
 }

@@ -756,9 +756,14 @@ __device__ inline uint32_t sort_by_class(const SceneView& sc, const PathPool& po
     return s_slot[threadIdx.x];
 }
 
-template <bool VOL, bool CACHE, bool TEX>
-__global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
-                                               uint32_t vertex, int cur, int nxt, CacheRecords rec)
+/* LIGHTS false is the instance for scenes without light sources (no
+ * next-event estimation, no light terms in the balance heuristic): without
+ * that code the kernel holds 80 VGPRs, 6 waves per SIMD, which the
+ * latency-bound shading of whole-frame launches needs. */
+template <bool VOL, bool CACHE, bool TEX, bool LIGHTS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 : 6))) void k_shade(
+    FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base, uint32_t vertex, int cur, int nxt,
+    CacheRecords rec)
 {
     uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = pool.counters[cur];
@@ -926,7 +931,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                  * a query vertex has one only with cache_nee */
                 if ((vertex != 0 || split == 0) && (!cache_query || p.cache_nee)) {
                     uint32_t ne = 0;
-                    for (uint32_t li = 0; li < sc.n_lights; ++li) {
+                    for (uint32_t li = 0; li < (LIGHTS ? sc.n_lights : 0u); ++li) {
                         const LightRec& L = sc.lights[li];
                         /* request mode: the request's direction of light 0 */
                         const float* dir_override = p.req.n && li == 0 ? p.req.reqs[req].light_dir : nullptr;
@@ -945,7 +950,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                                     float a = (float)samples * smp.pdf;
                                     balance = a / (a + (float)ss * bsdf_pdf(lsd, tv));
                                 } else {
-                                    float total = total_balance(sc, (float)ls, (float)ss, lsd, pos, tv);
+                                    float total = total_balance(sc, (float)ls, (float)ss, lsd, pos, tv, LIGHTS);
                                     balance = (light_pdf(sc, L, pos, lsd.wi) * (float)samples) / total;
                                 }
                                 curT = muls(curT, balance);
@@ -990,7 +995,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, PathPool pool, Bat
                             if (unbiased) {
                                 nb = (float)ss * sd.pdf;
                             } else {
-                                float total = total_balance(sc, (float)ls, (float)ss, sd, pos, tv);
+                                float total = total_balance(sc, (float)ls, (float)ss, sd, pos, tv, LIGHTS);
                                 nb = total > 0.0f ? ((float)ss * sd.pdf) / total : 0.0f;
                             }
                             T = mulv(T, divs(bsdf_eval(sd, tv), sd.pdf));
@@ -1697,21 +1702,26 @@ void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, con
     /* material maps are bound only to scenes that have textures */
     /* the texture instances also carry the pseudo-volume BRDF's walks */
     const bool tex = p.sc.tex.count > 0 || p.sc.tex.cube_side > 0 || p.sc.tex.pseudo_volume;
+    /* scenes without light sources take the instance without next-event
+     * estimation (texture-free, cache-free) */
+    const bool lights = p.sc.n_lights > 0;
     if (p.sc.n_volumes > 0) {
         if (tex) {
-            if (p.cache) launch_k(k_shade<true, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-            else launch_k(k_shade<true, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            if (p.cache) launch_k(k_shade<true, true, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<true, false, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
         } else {
-            if (p.cache) launch_k(k_shade<true, true, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-            else launch_k(k_shade<true, false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            if (p.cache) launch_k(k_shade<true, true, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else if (lights) launch_k(k_shade<true, false, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<true, false, false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
         }
     } else {
         if (tex) {
-            if (p.cache) launch_k(k_shade<false, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-            else launch_k(k_shade<false, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            if (p.cache) launch_k(k_shade<false, true, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<false, false, true, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
         } else {
-            if (p.cache) launch_k(k_shade<false, true, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
-            else launch_k(k_shade<false, false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            if (p.cache) launch_k(k_shade<false, true, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else if (lights) launch_k(k_shade<false, false, false, true>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
+            else launch_k(k_shade<false, false, false, false>, grid, 256, s, ev, p, pool, out, slot_base, vertex, cur, nxt, rec);
         }
     }
 }

@@ -1107,10 +1107,11 @@ TR_HD float hit_light_pdf(const SceneView& sc, uint32_t geom, uint32_t prim, V3 
 /* EvaluateTotalBalanceHeuristic (ray-tracing.cc:1791-1806) */
 template <class TV>
 TR_HD float total_balance(const SceneView& sc, float light_samples, float surface_samples, const Sample& s, V3 pos,
-                          const TV& tv)
+                          const TV& tv, bool lights = true)
 {
     float total = 0.0f;
-    for (uint32_t i = 0; i < sc.n_lights; ++i) {
+    const uint32_t n_lights = lights ? sc.n_lights : 0u;
+    for (uint32_t i = 0; i < n_lights; ++i) {
         const LightRec& L = sc.lights[i];
         float n = L.area ? light_samples : 1.0f;
         total = total + light_pdf(sc, L, pos, s.wi) * n;

@@ -580,7 +580,8 @@ typedef struct tr_request {
 } tr_request;
 
 typedef struct tr_request_options {
-    int32_t use_graph;           /* 1: replay one captured graph per batch; 0: plain launches */
+    int32_t use_graph;           /* 1: the bounce loops of all batches of a configuration in one
+                                  * captured graph, launched once; 0: plain launches */
     int32_t autotune;            /* 1: time candidate pool sizes and traversal grids on the
                                   * first batches and keep the fastest for the rest */
     uint32_t pool_slots;         /* paths per batch without autotuning (0: 2^24) */

@@ -2090,10 +2090,13 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
         uint32_t max_pix = 1;
         for (uint32_t k = b0; k < b1; ++k) max_pix = std::max(max_pix, batches[k].n_pix);
         if (o.use_graph) {
+            /* one graph holding the bounce loops of all these batches, one
+             * after another, launched once (each batch's parameters come
+             * from the device batch table through the cursor) */
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             TR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            enqueue_batch(slots, waves, max_pix);
+            for (uint32_t k = b0; k < b1; ++k) enqueue_batch(slots, waves, max_pix);
             hipError_t e = hipStreamEndCapture(stream, &graph);
             if (e != hipSuccess) return fail(TR_ERROR_DEVICE, "graph capture failed");
             e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -2101,7 +2104,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
                 (void)hipGraphDestroy(graph);
                 return fail(TR_ERROR_DEVICE, "graph instantiation failed");
             }
-            for (uint32_t k = b0; k < b1 && e == hipSuccess; ++k) e = hipGraphLaunch(exec, stream);
+            e = hipGraphLaunch(exec, stream);
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(TR_ERROR_DEVICE, "graph launch failed");

@@ -98,3 +98,38 @@ def test_requests_reject_bad_arguments():
     d2.lights = []
     with pytest.raises(TempestError):
         _gpu_scene(d2).renderRequests(reqs[:1])
+
+
+def test_full_gonio_workload_sampled_requests_match_oracle():
+    """The whole batched gonioreflectometer workload of the benchmark (6,561
+    requests, 32^2-256^2 pixels, 4-64 camera samples, 3.56 G paths) in one
+    call with autotuning and graph launches; the per-request means of five
+    requests spread over the call (first, last, the largest images among
+    them) against the oracle rendering each request on its own with its
+    pixel key base, to the per-path contract (1e-4 relative)."""
+    print("case: full gonioreflectometer workload, 6561 requests, autotune + graph, 5 sampled requests", flush=True)
+    d, reqs = workloads.gonio()
+    g = _gpu_scene(d)
+    mean, _, st = g.renderRequests(reqs, use_graph=True, autotune=True, images=False)
+    g.close()
+    sizes = [q["width"] * q["height"] for q in reqs]
+    assert st["paths"] == sum(n * q["samples_camera"] for n, q in zip(sizes, reqs))
+    base = np.concatenate([[0], np.cumsum(sizes)])
+    big = [i for i in range(len(reqs)) if reqs[i]["width"] == 256]
+    picks = sorted({0, len(reqs) - 1, len(reqs) // 2, big[0], big[-1]})
+    for r in picks:
+        q = reqs[r]
+        d2 = copy.copy(d)
+        d2.lights = [dict(d.lights[0], direction=tuple(q["light_direction"]))]
+        o = oracle.OracleScene(q["width"], q["height"], q["view_proj_inv"])
+        d2.populate(o, oracle.material)
+        o.setSamplesCamera(q["samples_camera"])
+        o.setPixelKeyBase(int(base[r]))
+        o.commitScene()
+        n = sizes[r]
+        rgb, _ = o.renderPaths(np.arange(n, dtype=np.uint32), frame=0)
+        ref = rgb.reshape(n, q["samples_camera"], 3).astype(np.float64).mean(axis=1).mean(axis=0)
+        print("request %d (%dx%d, %d spp): GPU %s oracle %s" % (r, q["width"], q["height"], q["samples_camera"],
+                                                                mean[r], ref), flush=True)
+        assert ref.max() > 0.0
+        np.testing.assert_allclose(mean[r], ref, rtol=1e-4, atol=1e-9)

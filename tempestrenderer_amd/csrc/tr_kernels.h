@@ -147,9 +147,10 @@ constexpr uint32_t kCompactGroups = 4096; /* k_compact blocks per queue, at most
 constexpr int kStackLds = 10;         /* stack entries per lane kept in LDS (5 KB per wave) */
 constexpr int kStackSpill = 48;       /* further entries per lane in global memory */
 /* Waves per SIMD the refill traversal is compiled for (its register budget,
- * 80 VGPRs); its default persistent grid is 4 * kRefillWavesPerSimd waves
- * per CU, the most that stay resident. */
-constexpr uint32_t kRefillWavesPerSimd = 6;
+ * 72 VGPRs; 6 waves with 80 VGPRs were 1.2 % slower on the heightfield, 8
+ * with 64 spill); its default persistent grid is 4 * kRefillWavesPerSimd
+ * waves per CU, the most that stay resident (LDS: 28 x 5 KB per CU). */
+constexpr uint32_t kRefillWavesPerSimd = 7;
 
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                    uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev = nullptr);

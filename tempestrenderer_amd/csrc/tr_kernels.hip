@@ -66,6 +66,36 @@ struct DevStack {
     __device__ bool empty() const { return sp == 0; }
 };
 
+/* The refill kernel's stack: the same entries as DevStack, addressed from
+ * the lane and the wave's spill base on each access, so that no per-lane
+ * pointer is held in registers. */
+struct LaneStack {
+    uint2 (*lds)[kTraceBlock];   /* the block's LDS stack array */
+    uint2* spill;                /* the wave's spill area: lane l at l * kStackSpill */
+    int sp;
+    __device__ void push(uint32_t b, uint32_t m)
+    {
+        const uint2 e = make_uint2(b, m);
+        if (sp < kStackLds) lds[sp][threadIdx.x] = e;
+        else spill[threadIdx.x * kStackSpill + (sp - kStackLds)] = e;
+        ++sp;
+    }
+    __device__ void pop(uint32_t* b, uint32_t* m)
+    {
+        --sp;
+        const uint2 e = sp < kStackLds ? lds[sp][threadIdx.x] : spill[threadIdx.x * kStackSpill + (sp - kStackLds)];
+        *b = e.x;
+        *m = e.y;
+    }
+    __device__ bool empty() const { return sp == 0; }
+};
+
+/* The number of set bits of a wave mask below this lane. */
+__device__ inline uint32_t bits_below_lane(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 /* Segment-local append of one entry to each of the three shading queues
  * (next path, shadow, cache query) for the threads whose predicate holds:
  * the block's entries go to its own segment in thread order and the
@@ -389,16 +419,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     __shared__ uint2 stk[kStackLds][kTraceBlock];
     const SceneView& sc = p.sc;
     const uint32_t lane = threadIdx.x;
-    const uint32_t gtid = blockIdx.x * kTraceBlock + lane;
     const uint32_t n = pool.counters[cur];
     const uint32_t* queue = pool.queue[cur];
     const uint32_t grid = gridDim.x;
-    DevStack st;
-    st.lds = &stk[0][lane];
-    st.spill = pool.spill + (size_t)gtid * kStackSpill;
+    LaneStack st;
+    st.lds = stk;
+    st.spill = pool.spill + (size_t)blockIdx.x * kTraceBlock * kStackSpill;
     st.sp = 0;
     uint32_t nv = 0, np = 0, nr = 0;
-    const uint64_t lt = (1ull << lane) - 1ull;
     constexpr uint32_t kRayChunk = LEAVES ? 64u : 16u;
     auto ray_index = [&](uint32_t k) -> uint32_t {
         return (blockIdx.x + (k / kRayChunk) * grid) * kRayChunk + (k % kRayChunk);
@@ -410,14 +438,15 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     Hit best;
     /* leaf cursor: hit leaves still to test (visiting order), the meta words
      * and primitive base of their node, the position in the current leaf */
-    uint32_t lv = 0, lk = 0, lm0 = 0, lm1 = 0, lpb = 0;
+    /* lv: bits 0-7 the leaf mask, bits 8-10 the position in the leaf */
+    uint32_t lv = 0, lm0 = 0, lm1 = 0, lpb = 0;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         const bool more = ray_index(next) < n;
         const uint32_t n_idle = (uint32_t)__popcll(idle);
         if (more && (n_idle >= kRefill || idle == ~0ull)) {
             if (!active) {
-                const uint32_t qi = ray_index(next + (uint32_t)__popcll(idle & lt));
+                const uint32_t qi = ray_index(next + bits_below_lane(idle));
                 if (qi < n) {
                     slot = queue[qi];
                     float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
@@ -427,7 +456,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                     best.u = best.v = 0.0f;
                     best.geom = best.prim = TR_INVALID_GEOMETRY;
                     st.sp = 0;
-                    st.push(0u, (1u << r.octant) | (1u << 8));
+                    st.push(0u, (1u << r.octant()) | (1u << 8));
                     lv = 0;
                     active = true;
                     ++nr;
@@ -441,7 +470,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
         }
         if (LEAVES && active) {
             /* one node of the ray's traversal and its hit leaves */
-            const uint32_t oct = r.octant;
+            const uint32_t oct = r.octant();
             uint32_t base, masks;
             st.pop(&base, &masks);
             uint32_t vis = masks & 0xffu;
@@ -477,14 +506,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                 active = false;
             }
         } else if (!LEAVES && active) {
-            const uint32_t oct = r.octant;
-            const bool prim_step = lv != 0;
+            const uint32_t oct = r.octant();
+            const bool prim_step = (lv & 0xffu) != 0;
             uint32_t count = 0, index;
             if (prim_step) {
                 const uint64_t m = (uint64_t)lm0 | ((uint64_t)lm1 << 32);
-                const uint32_t meta = (uint32_t)(m >> (8u * (ctz32(lv) ^ oct))) & 0xffu;
+                const uint32_t meta = (uint32_t)(m >> (8u * (ctz32(lv & 0xffu) ^ oct))) & 0xffu;
                 count = meta >> 5;
-                index = lpb + (meta & 31u) + lk;
+                index = lpb + (meta & 31u) + (lv >> 8);
             } else {
                 uint32_t base, masks;
                 st.pop(&base, &masks);
@@ -520,9 +549,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                     pr.v2[2] = __uint_as_float(q[2].z); pr.flags = q[2].w;
                     test_prim<SHAPES, false>(pr, sc.shapes, r, 0.0f, INFINITY, &best);
                 }
-                if (++lk >= count) {
-                    lk = 0;
-                    lv &= lv - 1u;
+                if ((lv >> 8) + 1u >= count) {
+                    lv = (lv & 0xffu) & ((lv & 0xffu) - 1u);
+                } else {
+                    lv += 1u << 8;
                 }
             } else {
                 NodeRegs nd;
@@ -537,13 +567,12 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
                 const uint32_t nimask = node_imask(nd);
                 const uint32_t inner = hit & nimask;
                 lv = to_visit_order(hit & ~nimask, oct);
-                lk = 0;
                 lm0 = nd.d[6];
                 lm1 = nd.d[7];
                 lpb = nd.d[5];
                 if (inner) st.push(nd.d[4], to_visit_order(inner, oct) | (nimask << 8));
             }
-            if (lv == 0 && st.empty()) {
+            if ((lv & 0xffu) == 0 && st.empty()) {
                 if (best.geom == TR_INVALID_GEOMETRY) best.t = INFINITY;
                 pool.hit[slot] = make_float4(best.t, best.u, best.v, __uint_as_float(best.geom));
                 pool.hit_prim[slot] = best.prim;

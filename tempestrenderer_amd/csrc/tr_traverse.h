@@ -29,12 +29,19 @@ struct Hit {
     uint32_t geom, prim;
 };
 
+/* A ray prepared for traversal.  The watertight test's axis permutation
+ * (kx, ky, kz) and the direction octant share one word: kx in bits 0-1,
+ * ky in 2-3, kz in 4-5, octant in 8-10, so that the traversal loop holds
+ * fewer registers. */
 struct RayPre {
     float org[3], dir[3];
     float inv[3];
-    int kx, ky, kz;
     float sx, sy, sz;
-    uint32_t octant;
+    uint32_t packed;
+    TR_HD int kx() const { return (int)(packed & 3u); }
+    TR_HD int ky() const { return (int)((packed >> 2) & 3u); }
+    TR_HD int kz() const { return (int)((packed >> 4) & 3u); }
+    TR_HD uint32_t octant() const { return (packed >> 8) & 7u; }
 };
 
 TR_HD RayPre ray_setup(const float org[3], const float dir[3])
@@ -48,14 +55,15 @@ TR_HD RayPre ray_setup(const float org[3], const float dir[3])
         r.inv[a] = 1.0f / d;
     }
     float ax = fabsf(dir[0]), ay = fabsf(dir[1]), az = fabsf(dir[2]);
-    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-    r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
-    r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
-    if (dir[r.kz] < 0.0f) { int t = r.kx; r.kx = r.ky; r.ky = t; }
-    r.sx = dir[r.kx] / dir[r.kz];
-    r.sy = dir[r.ky] / dir[r.kz];
-    r.sz = 1.0f / dir[r.kz];
-    r.octant = (dir[0] < 0.0f ? 1u : 0u) | (dir[1] < 0.0f ? 2u : 0u) | (dir[2] < 0.0f ? 4u : 0u);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    if (dir[kz] < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.sx = dir[kx] / dir[kz];
+    r.sy = dir[ky] / dir[kz];
+    r.sz = 1.0f / dir[kz];
+    const uint32_t octant = (dir[0] < 0.0f ? 1u : 0u) | (dir[1] < 0.0f ? 2u : 0u) | (dir[2] < 0.0f ? 4u : 0u);
+    r.packed = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4) | (octant << 8);
     return r;
 }
 
@@ -67,13 +75,14 @@ TR_HD bool watertight_triangle(const RayPre& r, const float* v0, const float* v1
     float A[3] = {v0[0] - r.org[0], v0[1] - r.org[1], v0[2] - r.org[2]};
     float B[3] = {v1[0] - r.org[0], v1[1] - r.org[1], v1[2] - r.org[2]};
     float Cc[3] = {v2[0] - r.org[0], v2[1] - r.org[1], v2[2] - r.org[2]};
-    float Akz = pick(A, r.kz), Bkz = pick(B, r.kz), Ckz = pick(Cc, r.kz);
-    float Ax = pick(A, r.kx) - r.sx * Akz;
-    float Ay = pick(A, r.ky) - r.sy * Akz;
-    float Bx = pick(B, r.kx) - r.sx * Bkz;
-    float By = pick(B, r.ky) - r.sy * Bkz;
-    float Cx = pick(Cc, r.kx) - r.sx * Ckz;
-    float Cy = pick(Cc, r.ky) - r.sy * Ckz;
+    const int kx = r.kx(), ky = r.ky(), kz = r.kz();
+    float Akz = pick(A, kz), Bkz = pick(B, kz), Ckz = pick(Cc, kz);
+    float Ax = pick(A, kx) - r.sx * Akz;
+    float Ay = pick(A, ky) - r.sy * Akz;
+    float Bx = pick(B, kx) - r.sx * Bkz;
+    float By = pick(B, ky) - r.sy * Bkz;
+    float Cx = pick(Cc, kx) - r.sx * Ckz;
+    float Cy = pick(Cc, ky) - r.sy * Ckz;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -462,7 +471,7 @@ TR_HD Hit traverse_closest(const Node8* nodes, const PrimRec* prims, const Shape
 {
     Hit best;
     best.t = tfar; best.u = 0.0f; best.v = 0.0f; best.geom = TR_INVALID_GEOMETRY; best.prim = TR_INVALID_GEOMETRY;
-    uint32_t oct = r.octant;
+    uint32_t oct = r.octant();
     /* virtual root group: node 0 sits in slot 0 of an imask=1 group */
     stack.push(0u, (1u << oct) | (1u << 8));
     while (!stack.empty()) {

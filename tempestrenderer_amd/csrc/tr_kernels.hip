@@ -418,7 +418,6 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
     const SceneView& sc = p.sc;
-    const uint32_t lane = threadIdx.x;
     const uint32_t n = pool.counters[cur];
     const uint32_t* queue = pool.queue[cur];
     const uint32_t grid = gridDim.x;

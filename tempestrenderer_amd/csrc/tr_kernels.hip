@@ -73,19 +73,32 @@ struct LaneStack {
     uint2 (*lds)[kTraceBlock];   /* the block's LDS stack array */
     uint2* spill;                /* the wave's spill area: lane l at l * kStackSpill */
     int sp;
+    /* The spill side moves an entry as one 64-bit word and the LDS side as
+     * two 32-bit words, so that the compiler keeps an LDS access (ds_*)
+     * and a global one instead of merging both into one flat access
+     * through a selected address, which waits on both memory counters. */
     __device__ void push(uint32_t b, uint32_t m)
     {
-        const uint2 e = make_uint2(b, m);
-        if (sp < kStackLds) lds[sp][threadIdx.x] = e;
-        else spill[threadIdx.x * kStackSpill + (sp - kStackLds)] = e;
+        if (sp < kStackLds) {
+            lds[sp][threadIdx.x] = make_uint2(b, m);
+        } else {
+            uint64_t* w = reinterpret_cast<uint64_t*>(spill + threadIdx.x * kStackSpill + (sp - kStackLds));
+            *w = (uint64_t)b | ((uint64_t)m << 32);
+        }
         ++sp;
     }
     __device__ void pop(uint32_t* b, uint32_t* m)
     {
         --sp;
-        const uint2 e = sp < kStackLds ? lds[sp][threadIdx.x] : spill[threadIdx.x * kStackSpill + (sp - kStackLds)];
-        *b = e.x;
-        *m = e.y;
+        if (sp < kStackLds) {
+            const uint2 e = lds[sp][threadIdx.x];
+            *b = e.x;
+            *m = e.y;
+        } else {
+            const uint64_t w = *reinterpret_cast<const uint64_t*>(spill + threadIdx.x * kStackSpill + (sp - kStackLds));
+            *b = (uint32_t)w;
+            *m = (uint32_t)(w >> 32);
+        }
     }
     __device__ bool empty() const { return sp == 0; }
 };

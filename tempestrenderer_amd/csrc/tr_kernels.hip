@@ -45,23 +45,33 @@ __device__ inline uint32_t fnv_hit(uint32_t h, uint32_t geom, uint32_t prim)
     return (h ^ prim) * kFnvPrime;
 }
 
+/* The LDS side moves an entry as two 32-bit words and the spill side as
+ * one 64-bit word (in DevStack and LaneStack), so that the compiler keeps
+ * an LDS access (ds_*) and a global one instead of merging both into one
+ * flat access through a selected address, which waits on both memory
+ * counters. */
 struct DevStack {
     uint2* lds;     /* &stk[0][lane], stride kTraceBlock */
     uint2* spill;   /* kStackSpill entries of this lane */
     int sp;
     __device__ void push(uint32_t b, uint32_t m)
     {
-        uint2 e = make_uint2(b, m);
-        if (sp < kStackLds) lds[sp * kTraceBlock] = e;
-        else spill[sp - kStackLds] = e;
+        if (sp < kStackLds) lds[sp * kTraceBlock] = make_uint2(b, m);
+        else *reinterpret_cast<uint64_t*>(spill + (sp - kStackLds)) = (uint64_t)b | ((uint64_t)m << 32);
         ++sp;
     }
     __device__ void pop(uint32_t* b, uint32_t* m)
     {
         --sp;
-        uint2 e = sp < kStackLds ? lds[sp * kTraceBlock] : spill[sp - kStackLds];
-        *b = e.x;
-        *m = e.y;
+        if (sp < kStackLds) {
+            const uint2 e = lds[sp * kTraceBlock];
+            *b = e.x;
+            *m = e.y;
+        } else {
+            const uint64_t w = *reinterpret_cast<const uint64_t*>(spill + (sp - kStackLds));
+            *b = (uint32_t)w;
+            *m = (uint32_t)(w >> 32);
+        }
     }
     __device__ bool empty() const { return sp == 0; }
 };
@@ -73,10 +83,6 @@ struct LaneStack {
     uint2 (*lds)[kTraceBlock];   /* the block's LDS stack array */
     uint2* spill;                /* the wave's spill area: lane l at l * kStackSpill */
     int sp;
-    /* The spill side moves an entry as one 64-bit word and the LDS side as
-     * two 32-bit words, so that the compiler keeps an LDS access (ds_*)
-     * and a global one instead of merging both into one flat access
-     * through a selected address, which waits on both memory counters. */
     __device__ void push(uint32_t b, uint32_t m)
     {
         if (sp < kStackLds) {

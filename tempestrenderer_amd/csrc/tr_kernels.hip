@@ -846,9 +846,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
         V3 T = ld4(t4);
         float bal = t4.w;
         uint32_t geom = __float_as_uint(h4.w);
-        uint32_t hash = vertex == 0 ? kFnvBasis : pool.hash[slot];
+        /* The hit-sequence hash is kept only for a batch that outputs it. */
+        const bool want_hash = out.path_hash != nullptr;
+        uint32_t hash = vertex == 0 || !want_hash ? kFnvBasis : pool.hash[slot];
         V3 local = mk(0.0f, 0.0f, 0.0f);
-        V3 global = vertex == 0 ? mk(0.0f, 0.0f, 0.0f) : ld4(pool.global[slot]);
+        /* The radiance of later vertices is read and written only by the
+         * vertices that add to it (a miss, an emitter) and by the camera
+         * vertex, which clears it; the cache mode reads it at every vertex. */
+        V3 global = CACHE && vertex != 0 ? ld4(pool.global[slot]) : mk(0.0f, 0.0f, 0.0f);
+        bool write_global = CACHE || vertex == 0;
         /* the camera vertex writes the slot's camera-vertex radiance, zero or not */
         bool write_local = vertex == 0;
         uint32_t pf = 0;
@@ -872,12 +878,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
             /* m_GlobalCubeMap->sampleSpectrum(ray direction) or the
              * background spectrum (ray-tracing.cc:2060, 2224) */
             V3 env = (TEX && sc.tex.cube_side) ? cube_sample(sc.tex, dir) : ld3(p.background);
-            if (vertex == 0) { local = env; write_local = true; }
-            else global = add(global, mulv(env, T));
+            if (vertex == 0) {
+                local = env;
+                write_local = true;
+            } else {
+                if (!CACHE) global = ld4(pool.global[slot]);
+                global = add(global, mulv(env, T));
+                write_global = true;
+            }
         } else if (sc.geoms[geom].material == kNoMaterial) {
             /* a blocker: the path ends with nothing (ray-tracing.cc:2072-2077,
              * 2236-2240); its sample() frame is zero (:1239-1244) */
-            hash = fnv_hit(hash, geom, pool.hit_prim[slot]);
+            if (want_hash) hash = fnv_hit(hash, geom, pool.hit_prim[slot]);
             if (vertex == 0 && p.render_mode) {
                 const V3 z = mk(0.0f, 0.0f, 0.0f);
                 local = debug_color(p.render_mode, z, z, z, dir);
@@ -931,7 +943,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
                         float ls_prev = vertex == 1 ? (float)p.n_light : 1.0f;
                         w = bal / (bal + ls_prev * lp);
                     }
+                    if (!CACHE) global = ld4(pool.global[slot]);
                     global = add(global, mulv(muls(T, w), le));
+                    write_global = true;
                 }
             } else {
                 sd.wo = neg(dir);
@@ -1057,8 +1071,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
                 }
             }
         }
-        pool.hash[slot] = hash;
-        pool.global[slot] = st4(global, 0.0f);
+        if (want_hash) pool.hash[slot] = hash;
+        if (write_global) pool.global[slot] = st4(global, 0.0f);
         /* .w: the camera ray hit geometry (TransparentBackground) */
         if (write_local) pool.local[slot] = st4(local, geom != TR_INVALID_GEOMETRY ? 1.0f : 0.0f);
     }
@@ -1453,10 +1467,10 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool
     for (; c < p.spp; ++c) {
         uint32_t s0 = (uint32_t)(first_slot - slot_base) + c * p.n_gi;
         V3 g = ld4(pool.global[s0]);
-        uint32_t h = pool.hash[s0];
+        uint32_t h = out.path_hash ? pool.hash[s0] : 0u;
         for (uint32_t j = 1; j < p.n_gi; ++j) {
             g = add(g, ld4(pool.global[s0 + j]));
-            h = (h * 0x9E3779B1u) ^ pool.hash[s0 + j];
+            if (out.path_hash) h = (h * 0x9E3779B1u) ^ pool.hash[s0 + j];
         }
         const float4 l0 = pool.local[s0];
         any_hit |= l0.w != 0.0f;

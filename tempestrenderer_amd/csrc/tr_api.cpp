@@ -533,7 +533,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
                  })) != TR_OK)
                 return st2;
             if ((st2 = stage(EV_SHADE, [&](const LaunchEvents* ev) {
-                     launch_compact(s->stream, pool, nslots, nxt, s->cache.enabled, ev);
+                     launch_compact(s->stream, pool, nslots, cur, nxt, s->cache.enabled, ev);
                  })) != TR_OK)
                 return st2;
             if (s->cache.enabled && v > 0) {
@@ -2069,7 +2069,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
         for (uint32_t v = 0; v <= s->max_depth; ++v) {
             launch_trace(stream, p, pool, out, 0, v, cur, nxt, grid, grid, nullptr);
             launch_shade(stream, p, pool, out, 0, v, cur, nxt, slots, rec);
-            launch_compact(stream, pool, slots, nxt, false);
+            launch_compact(stream, pool, slots, cur, nxt, false);
             if (p.sc.n_lights > 0) launch_shadow(stream, p, pool, out, 0, v, grid, nullptr);
             std::swap(cur, nxt);
         }

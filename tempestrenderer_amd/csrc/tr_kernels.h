@@ -165,7 +165,7 @@ void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 /* Concatenates the segment-local queues of the last shade into the next
  * path queue, the shadow queue and (cache mode) the query queue, in
  * segment order, and writes their sizes to the counters. */
-void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int nxt, bool queries,
+void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int cur, int nxt, bool queries,
                     const LaunchEvents* ev = nullptr);
 /* Cache mode: network predictions for the queries of the last shade, added
  * to their paths' radiance. */

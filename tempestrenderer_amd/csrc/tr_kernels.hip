@@ -125,7 +125,7 @@ __device__ inline uint32_t bits_below_lane(uint64_t m)
  * call it.  No atomics: one counter bumped by every wave of a launch
  * serialises (tens of thousands of same-address atomics per launch),
  * while k_compact concatenates the segments in a few microseconds. */
-__device__ inline uint32_t segment_push(const PathPool& pool, bool e0, bool e1, bool e2, uint32_t value)
+__device__ inline uint32_t segment_push(const PathPool& pool, uint32_t seg, bool e0, bool e1, bool e2, uint32_t value)
 {
     __shared__ uint32_t wc[3][kSegment / 64];
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
@@ -143,14 +143,14 @@ __device__ inline uint32_t segment_push(const PathPool& pool, bool e0, bool e1, 
     for (int k = 0; k < 3; ++k) {
         uint32_t o = 0;
         for (uint32_t j = 0; j < w; ++j) o += wc[k][j];
-        const uint32_t pos = blockIdx.x * kSegment + o + (uint32_t)__popcll(m[k] & lt);
+        const uint32_t pos = seg * kSegment + o + (uint32_t)__popcll(m[k] & lt);
         if (k == 2) pos2 = pos;
         if (e[k]) pool.seg[k * seg_stride + pos] = k == 2 ? pos : value;
     }
     if (threadIdx.x < 3) {
         uint32_t t = 0;
         for (uint32_t j = 0; j < kSegment / 64; ++j) t += wc[threadIdx.x][j];
-        pool.seg_n[threadIdx.x * pool.seg_cap + blockIdx.x] = t;
+        pool.seg_n[threadIdx.x * pool.seg_cap + seg] = t;
     }
     return pos2;
 }
@@ -803,17 +803,14 @@ __device__ inline uint32_t sort_by_class(const SceneView& sc, const PathPool& po
     return s_slot[threadIdx.x];
 }
 
-/* LIGHTS false is the instance for scenes without light sources (no
- * next-event estimation, no light terms in the balance heuristic): without
- * that code the kernel holds 80 VGPRs, 6 waves per SIMD, which the
- * latency-bound shading of whole-frame launches needs. */
+/* Shades queue entries seg * kSegment .. + kSegment - 1 (those below n),
+ * one per thread of the block, and writes segment seg of the next queues. */
 template <bool VOL, bool CACHE, bool TEX, bool LIGHTS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 : 6))) void k_shade(
-    FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base, uint32_t vertex, int cur, int nxt,
-    CacheRecords rec)
+__device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPool& pool, const BatchOut& out,
+                                              uint64_t slot_base, uint32_t vertex, int cur, const CacheRecords& rec,
+                                              uint32_t seg, uint32_t n)
 {
-    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = pool.counters[cur];
+    uint32_t qi = seg * kSegment + threadIdx.x;
     bool active = qi < n;
     bool extend = false;
     bool shadow = false;
@@ -1076,7 +1073,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
         /* .w: the camera ray hit geometry (TransparentBackground) */
         if (write_local) pool.local[slot] = st4(local, geom != TR_INVALID_GEOMETRY ? 1.0f : 0.0f);
     }
-    const uint32_t qpos = segment_push(pool, extend, shadow, cache_query, slot);
+    const uint32_t qpos = segment_push(pool, seg, extend, shadow, cache_query, slot);
     if (CACHE && cache_query) {
         const MlpQueries& q = rec.queries;
 #pragma unroll
@@ -1086,6 +1083,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
         q.thr[2 * (size_t)q.stride + qpos] = qthr.z;
         q.slot[qpos] = slot;
     }
+}
+
+/* One block per segment of the largest possible queue (the queue size is
+ * known only on the device); blocks past the queue's end return at once
+ * and write no segment.
+ * LIGHTS false is the instance for scenes without light sources (no
+ * next-event estimation, no light terms in the balance heuristic): without
+ * that code the kernel holds under 80 VGPRs, 6 waves per SIMD, which the
+ * latency-bound shading of whole-frame launches needs. */
+template <bool VOL, bool CACHE, bool TEX, bool LIGHTS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 : 6))) void k_shade(
+    FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base, uint32_t vertex, int cur, int nxt,
+    CacheRecords rec)
+{
+    (void)nxt;
+    const uint32_t n = pool.counters[cur];
+    if (blockIdx.x * kSegment >= n) return;
+    shade_segment<VOL, CACHE, TEX, LIGHTS>(p, pool, out, slot_base, vertex, cur, rec, blockIdx.x, n);
 }
 
 /* ---------------- request mode: accumulation and outputs ---------------- */
@@ -1206,13 +1221,25 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
     return before + x - v;
 }
 
+/* The shading launch wrote segments 0 .. n_seg - 1, one per kSegment
+ * entries of path queue cur; they fall into groups of `group` segments,
+ * one per block of the compaction grid (whose x size is the same in the
+ * three kernels), at least 16 segments to a group. */
+__device__ inline void compact_groups(const PathPool& pool, int cur, uint32_t* n_seg, uint32_t* group)
+{
+    const uint32_t n = pool.counters[cur];
+    *n_seg = (n + kSegment - 1) / kSegment;
+    *group = max(16u, (*n_seg + gridDim.x - 1) / gridDim.x);
+}
+
 /* Block (x, k) sums queue k's segment counts of group x (segments
  * [x * group, (x + 1) * group)), in a fixed order. */
-__global__ __launch_bounds__(kSegment) void k_compact_sums(PathPool pool, uint32_t n_seg, uint32_t group,
-                                                            uint32_t* group_sum)
+__global__ __launch_bounds__(kSegment) void k_compact_sums(PathPool pool, int cur, uint32_t* group_sum)
 {
     const uint32_t k = blockIdx.y;
     const uint32_t* cnt = pool.seg_n + (size_t)k * pool.seg_cap;
+    uint32_t n_seg, group;
+    compact_groups(pool, cur, &n_seg, &group);
     const uint32_t s0 = blockIdx.x * group;
     const uint32_t s1 = min(s0 + group, n_seg);
     uint32_t part = 0;
@@ -1255,10 +1282,11 @@ __global__ __launch_bounds__(kSegment) void k_compact_scan(PathPool pool, uint32
 
 /* Block (x, k) copies queue k's segments of group x to the dense queue at
  * the offset k_compact_scan left in the group's sum. */
-__global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, uint32_t n_seg, uint32_t group, int nxt,
-                                                      const uint32_t* group_off)
+__global__ __launch_bounds__(kSegment) void k_compact(PathPool pool, int cur, int nxt, const uint32_t* group_off)
 {
     const uint32_t k = blockIdx.y;
+    uint32_t n_seg, group;
+    compact_groups(pool, cur, &n_seg, &group);
     const uint32_t* cnt = pool.seg_n + (size_t)k * pool.seg_cap;
     const uint32_t* src = pool.seg + (size_t)k * pool.seg_cap * kSegment;
     uint32_t* dst = k == 0 ? pool.queue[nxt] : (k == 1 ? pool.shadow_queue : pool.query_queue);
@@ -1759,7 +1787,7 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
 void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                   uint32_t vertex, int cur, int nxt, uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev)
 {
-    const uint32_t grid = (n_slots + 255) / 256;
+    const uint32_t grid = (n_slots + kSegment - 1) / kSegment;
     /* material maps are bound only to scenes that have textures */
     /* the texture instances also carry the pseudo-volume BRDF's walks */
     const bool tex = p.sc.tex.count > 0 || p.sc.tex.cube_side > 0 || p.sc.tex.pseudo_volume;
@@ -1787,28 +1815,27 @@ void launch_shade(hipStream_t s, const FrameParams& p, const PathPool& pool, con
     }
 }
 
-void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int nxt, bool queries,
+void launch_compact(hipStream_t s, const PathPool& pool, uint32_t n_slots, int cur, int nxt, bool queries,
                     const LaunchEvents* ev)
 {
-    uint32_t n_seg = (n_slots + kSegment - 1) / kSegment;
-    /* groups of at least 16 segments, at most kCompactGroups of them per
-     * queue: sums, one scan of the sums, then the copies */
-    uint32_t group = (n_seg + kCompactGroups - 1) / kCompactGroups;
-    if (group < 16) group = 16;
-    uint32_t blocks = (n_seg + group - 1) / group;   /* at most kCompactGroups */
+    /* one block per group of at least 16 segments of the largest possible
+     * queue, at most kCompactGroups per queue: sums, one scan of the sums,
+     * then the copies; the kernels size the groups to the shaded queue */
+    const uint32_t n_seg = (n_slots + kSegment - 1) / kSegment;
+    const uint32_t blocks = std::max(1u, std::min(kCompactGroups, (n_seg + 15) / 16));
     const uint32_t nq = queries ? 3 : 2;
     dim3 grid(blocks, nq);
     if (ev) {
-        hipExtLaunchKernelGGL(k_compact_sums, grid, dim3(kSegment), 0, s, ev->start, nullptr, 0u, pool, n_seg, group,
+        hipExtLaunchKernelGGL(k_compact_sums, grid, dim3(kSegment), 0, s, ev->start, nullptr, 0u, pool, cur,
                               pool.group_sum);
         hipExtLaunchKernelGGL(k_compact_scan, dim3(nq), dim3(kSegment), 0, s, nullptr, nullptr, 0u, pool, blocks, nxt,
                               pool.group_sum);
-        hipExtLaunchKernelGGL(k_compact, grid, dim3(kSegment), 0, s, nullptr, ev->stop, 0u, pool, n_seg, group, nxt,
+        hipExtLaunchKernelGGL(k_compact, grid, dim3(kSegment), 0, s, nullptr, ev->stop, 0u, pool, cur, nxt,
                               (const uint32_t*)pool.group_sum);
     } else {
-        k_compact_sums<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, pool.group_sum);
+        k_compact_sums<<<grid, dim3(kSegment), 0, s>>>(pool, cur, pool.group_sum);
         k_compact_scan<<<dim3(nq), dim3(kSegment), 0, s>>>(pool, blocks, nxt, pool.group_sum);
-        k_compact<<<grid, dim3(kSegment), 0, s>>>(pool, n_seg, group, nxt, (const uint32_t*)pool.group_sum);
+        k_compact<<<grid, dim3(kSegment), 0, s>>>(pool, cur, nxt, (const uint32_t*)pool.group_sum);
     }
 }
 

@@ -522,7 +522,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
-            "kernel": "k_trace_refill (closest hit, BVH8)",
+            "kernel": "closest-hit BVH8 traversal: k_trace_packet (camera rays) + k_trace_refill (later vertices)",
             "alg_bytes_per_step": alg_bytes,
             "segments_per_step": segs,
             "launches_per_frame": tim["trace_launches"],

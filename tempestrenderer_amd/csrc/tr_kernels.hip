@@ -605,6 +605,39 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     }
 }
 
+/* Closest hit of camera rays: a wave traces 64 consecutive queue entries
+ * together, without refill.  The camera vertex's queue is the batch in
+ * slot order, pixel-major, so the 64 rays of a wave are samples of one
+ * or two pixels and visit nearly the same nodes: each node load serves
+ * the whole wave from one or two cache lines.  Refill would mix the rays
+ * of several pixels in a wave as they finish at different times. */
+template <bool SHAPES>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRefillWavesPerSimd))) void k_trace_packet(
+    FrameParams p, PathPool pool, int cur)
+{
+    __shared__ uint2 stk[kStackLds][kTraceBlock];
+    const SceneView& sc = p.sc;
+    const uint32_t n = pool.counters[cur];
+    const uint32_t* queue = pool.queue[cur];
+    DevStack st;
+    st.lds = &stk[0][threadIdx.x];
+    st.spill = pool.spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * kStackSpill;
+    for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += gridDim.x * kTraceBlock) {
+        const uint32_t qi = base + threadIdx.x;
+        if (qi < n) {
+            const uint32_t slot = queue[qi];
+            const float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
+            float org[3] = {o.x, o.y, o.z}, dir[3] = {d.x, d.y, d.z};
+            const RayPre r = ray_setup(org, dir);
+            st.sp = 0;
+            const Hit h = traverse_closest(sc.nodes, sc.prims, SHAPES ? sc.shapes : nullptr, r, 0.0f, INFINITY, st,
+                                           (uint32_t*)nullptr, (uint32_t*)nullptr);
+            pool.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.geom));
+            pool.hit_prim[slot] = h.prim;
+        }
+    }
+}
+
 /* Closest hit with delta tracking for scenes with volumes, with ray refill
  * (k_trace_refill's scheme: a lane whose ray has finished takes the wave's
  * next ray once kRefill lanes are idle; the queue is dealt in chunks of
@@ -1776,7 +1809,10 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
         else launch_k(k_trace_refill<true, false, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
     } else {
         if (vol) launch_k(k_trace_vol<false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, c);
-        else if (small) {
+        else if (vertex == 0 && !small) {
+            if (shapes) launch_k(k_trace_packet<true>, grid_refill, kTraceBlock, s, ev, p, pool, cur);
+            else launch_k(k_trace_packet<false>, grid_refill, kTraceBlock, s, ev, p, pool, cur);
+        } else if (small) {
             if (shapes) launch_k(k_trace_refill<false, true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
             else launch_k(k_trace_refill<false, false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
         } else if (shapes) launch_k(k_trace_refill<false, true, false>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);

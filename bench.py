@@ -527,6 +527,11 @@ def main():
             "segments_per_step": segs,
             "launches_per_frame": tim["trace_launches"],
             "avg_launch_us": round(tim["trace_ms"] * 1e3 / max(1, tim["trace_launches"]), 2),
+            # HBM bytes the counters measured per launch over the launch time: the
+            # algorithmic bytes count every node fetch per ray, while the camera
+            # rays of one wave share theirs through the caches
+            "traffic_frac": (round(traffic / (tim["trace_ms"] * 1e-3 / max(1, tim["trace_launches"])) / 1e9
+                                   / HBM_PEAK_GBS, 5) if traffic and tim["trace_ms"] > 0 else None),
         },
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,

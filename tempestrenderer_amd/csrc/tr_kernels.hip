@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
  * instead of waiting for the longest walk of a fixed group of 64.  Each
  * ray follows k_trace's sequence exactly: a walk that leaves the box
  * re-traces from beyond its exit, at most kMaxVolumeCrossings times. */
-constexpr uint32_t kVolChunk = 16;
+constexpr uint32_t kVolChunk = 32;   /* 16: 0.9 % slower on the volume frame, 64: 2.7 % */
 
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_vol(FrameParams p, PathPool pool, BatchOut out,

@@ -476,6 +476,7 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
         counts = s->counts.ptr;
         TR_HIP(hipMemsetAsync(counts, 0, 8 * sizeof(uint64_t), s->stream));
     }
+    p.camera_fused = camera_rays_in_kernels(p, counts != nullptr) ? 1u : 0u;
     /* In timing mode every launch carries its own start / stop events in
      * the dispatch (hipExtLaunchKernelGGL) and is waited for, so the pair
      * measures the kernel alone, without launch gaps. */

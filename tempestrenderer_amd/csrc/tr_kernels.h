@@ -70,6 +70,9 @@ struct FrameParams {
     uint32_t render_mode;    /* RenderMode (ray-tracing-common.hh:132-139): 0 Normal, 1-3 debug tangents /
                                 binormals / normals, 4 debug lighting */
     uint32_t shade_sort;     /* 1: shading blocks sort their paths by material class */
+    uint32_t camera_fused;   /* 1: the camera vertex's kernels (k_trace_packet, k_shade) compute the camera
+                                rays from the slot and take queue position = slot; k_raygen only sets the
+                                queue size (camera_rays_in_kernels) */
 };
 
 /* Path flags of cache mode: training path, next vertex is the query
@@ -152,6 +155,10 @@ constexpr int kStackSpill = 48;       /* further entries per lane in global memo
  * waves per CU, the most that stay resident (LDS: 28 x 5 KB per CU). */
 constexpr uint32_t kRefillWavesPerSimd = 7;
 
+/* Whether a frame's camera vertex can compute its rays in the kernels that
+ * use them: plain frames (no requests, no cache, not counting) of scenes
+ * whose camera rays k_trace_packet traces. */
+bool camera_rays_in_kernels(const FrameParams& p, bool counting);
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                    uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev = nullptr);
 /* grid: persistent grid of the volume-tracking kernel; grid_refill: of the

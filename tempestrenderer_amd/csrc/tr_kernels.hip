@@ -268,33 +268,13 @@ __device__ inline uint32_t train_record(const FrameParams& p, uint64_t pseq, uin
 
 /* ---------------- ray generation ---------------- */
 
-__global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
-                                                uint32_t n_slots, CacheRecords rec)
+/* The camera ray of camera sample cs of image pixel lpix (pixel: the
+ * random-number key): through the jittered pixel (the pixel's corner grid
+ * with one sample), from the near to the far plane of the inverse
+ * view-projection m. */
+__device__ inline void camera_ray(const FrameParams& p, uint32_t pixel, uint32_t lpix, uint32_t cs, uint32_t width,
+                                  uint32_t height, uint32_t spp, const float* m, V3* start_out, V3* dir_out)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p.req.n) n_slots = p.req.batches[*p.req.cursor].n_slots;
-    if (i == 0) pool.counters[0] = n_slots;
-    if (i >= n_slots) return;
-    uint32_t pixel, cs, split, lpix;
-    uint64_t pseq;
-    uint32_t width = p.width, height = p.height, spp = p.spp;
-    const float* m = p.vpi;
-    if (p.req.n) {
-        /* request mode: the slot's request, its camera and image size */
-        const ReqBatch& b = p.req.batches[*p.req.cursor];
-        const uint32_t r = find_request(p.req.path_off, p.req.n, b.path_base + i);
-        pool.slot_req[i] = r;
-        const ReqRec& R = p.req.reqs[r];
-        request_coords(p, r, i, &pixel, &lpix, &cs, &split);
-        pseq = lpix;
-        width = R.width;
-        height = R.height;
-        spp = R.spp;
-        m = R.vpi;
-    } else {
-        slot_coords(p, out, slot_base + i, &pixel, &cs, &split, &pseq);
-        lpix = pixel;
-    }
     uint32_t x = lpix % width, y = lpix / width;
     float sx, sy;
     if (spp > 1) {
@@ -321,7 +301,39 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, Ba
     float w0 = 1.0f / p0[3], w1 = 1.0f / p1[3];
     V3 start = mk(p0[0] * w0, p0[1] * w0, p0[2] * w0);
     V3 end = mk(p1[0] * w1, p1[1] * w1, p1[2] * w1);
-    V3 dir = normalize(sub(end, start));
+    *start_out = start;
+    *dir_out = normalize(sub(end, start));
+}
+
+__global__ __launch_bounds__(256) void k_raygen(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                                uint32_t n_slots, CacheRecords rec)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p.req.n) n_slots = p.req.batches[*p.req.cursor].n_slots;
+    if (i == 0) pool.counters[0] = n_slots;
+    if (p.camera_fused || i >= n_slots) return;
+    uint32_t pixel, cs, split, lpix;
+    uint64_t pseq;
+    uint32_t width = p.width, height = p.height, spp = p.spp;
+    const float* m = p.vpi;
+    if (p.req.n) {
+        /* request mode: the slot's request, its camera and image size */
+        const ReqBatch& b = p.req.batches[*p.req.cursor];
+        const uint32_t r = find_request(p.req.path_off, p.req.n, b.path_base + i);
+        pool.slot_req[i] = r;
+        const ReqRec& R = p.req.reqs[r];
+        request_coords(p, r, i, &pixel, &lpix, &cs, &split);
+        pseq = lpix;
+        width = R.width;
+        height = R.height;
+        spp = R.spp;
+        m = R.vpi;
+    } else {
+        slot_coords(p, out, slot_base + i, &pixel, &cs, &split, &pseq);
+        lpix = pixel;
+    }
+    V3 start, dir;
+    camera_ray(p, pixel, lpix, cs, width, height, spp, m, &start, &dir);
     pool.ray_o[i] = st4(start, 0.0f);
     pool.ray_d[i] = st4(dir, 0.0f);
     /* throughput 1, balance 1, zero radiance and the hash basis are the
@@ -613,7 +625,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
  * of several pixels in a wave as they finish at different times. */
 template <bool SHAPES>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRefillWavesPerSimd))) void k_trace_packet(
-    FrameParams p, PathPool pool, int cur)
+    FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base, int cur)
 {
     __shared__ uint2 stk[kStackLds][kTraceBlock];
     const SceneView& sc = p.sc;
@@ -625,9 +637,24 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += gridDim.x * kTraceBlock) {
         const uint32_t qi = base + threadIdx.x;
         if (qi < n) {
-            const uint32_t slot = queue[qi];
-            const float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
-            float org[3] = {o.x, o.y, o.z}, dir[3] = {d.x, d.y, d.z};
+            uint32_t slot;
+            float org[3], dir[3];
+            if (p.camera_fused) {
+                /* the camera vertex's queue is the batch in slot order */
+                slot = qi;
+                uint32_t pixel, cs, split;
+                uint64_t pseq;
+                slot_coords(p, out, slot_base + slot, &pixel, &cs, &split, &pseq);
+                V3 o, d;
+                camera_ray(p, pixel, pixel, cs, p.width, p.height, p.spp, p.vpi, &o, &d);
+                org[0] = o.x; org[1] = o.y; org[2] = o.z;
+                dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+            } else {
+                slot = queue[qi];
+                const float4 o = pool.ray_o[slot], d = pool.ray_d[slot];
+                org[0] = o.x; org[1] = o.y; org[2] = o.z;
+                dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+            }
             const RayPre r = ray_setup(org, dir);
             st.sp = 0;
             const Hit h = traverse_closest(sc.nodes, sc.prims, SHAPES ? sc.shapes : nullptr, r, 0.0f, INFINITY, st,
@@ -850,7 +877,9 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
     bool cache_query = false;
     float qin[kMlpIn];
     V3 qthr = mk(0.0f, 0.0f, 0.0f);
-    uint32_t slot = active ? pool.queue[cur][qi] : 0u;
+    /* the camera vertex's queue is the batch in slot order */
+    const bool fused = vertex == 0 && p.camera_fused;
+    uint32_t slot = active ? (fused ? qi : pool.queue[cur][qi]) : 0u;
     float4 sorted_hit;
     if (p.shade_sort) slot = sort_by_class(p.sc, pool, slot, active, &sorted_hit);
     if (active) {
@@ -870,9 +899,15 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
         Rng rng;
         rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
         const bool unbiased = p.estimator == TR_ESTIMATOR_UNBIASED;
-        float4 o4 = pool.ray_o[slot], d4 = pool.ray_d[slot], h4 = p.shade_sort ? sorted_hit : pool.hit[slot];
+        V3 org, dir;
+        if (fused) {
+            camera_ray(p, pixel, pixel, cs, p.width, p.height, p.spp, p.vpi, &org, &dir);
+        } else {
+            org = ld4(pool.ray_o[slot]);
+            dir = ld4(pool.ray_d[slot]);
+        }
+        float4 h4 = p.shade_sort ? sorted_hit : pool.hit[slot];
         const float4 t4 = vertex == 0 ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : pool.thr[slot];
-        V3 org = ld4(o4), dir = ld4(d4);
         V3 T = ld4(t4);
         float bal = t4.w;
         uint32_t geom = __float_as_uint(h4.w);
@@ -1789,7 +1824,14 @@ void launch_pseudo_volume_image(hipStream_t s, const tr_material* m, uint32_t wi
 void launch_raygen(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
                    uint32_t n_slots, const CacheRecords& rec, const LaunchEvents* ev)
 {
-    launch_k(k_raygen, (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_slots, rec);
+    /* with camera_fused only the queue size is set */
+    launch_k(k_raygen, p.camera_fused ? 1u : (n_slots + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_slots, rec);
+}
+
+bool camera_rays_in_kernels(const FrameParams& p, bool counting)
+{
+    /* launch_trace's choice of k_trace_packet at the camera vertex */
+    return !counting && !p.cache && p.req.n == 0 && p.sc.n_volumes == 0 && p.sc.n_prims > kLeafLoopPrims;
 }
 
 void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out, uint64_t slot_base,
@@ -1810,8 +1852,8 @@ void launch_trace(hipStream_t s, const FrameParams& p, const PathPool& pool, con
     } else {
         if (vol) launch_k(k_trace_vol<false>, grid, kTraceBlock, s, ev, p, pool, out, slot_base, vertex, cur, c);
         else if (vertex == 0 && !small) {
-            if (shapes) launch_k(k_trace_packet<true>, grid_refill, kTraceBlock, s, ev, p, pool, cur);
-            else launch_k(k_trace_packet<false>, grid_refill, kTraceBlock, s, ev, p, pool, cur);
+            if (shapes) launch_k(k_trace_packet<true>, grid_refill, kTraceBlock, s, ev, p, pool, out, slot_base, cur);
+            else launch_k(k_trace_packet<false>, grid_refill, kTraceBlock, s, ev, p, pool, out, slot_base, cur);
         } else if (small) {
             if (shapes) launch_k(k_trace_refill<false, true, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);
             else launch_k(k_trace_refill<false, false, true>, grid_refill, kTraceBlock, s, ev, p, pool, cur, c);

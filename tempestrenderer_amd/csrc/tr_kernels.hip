@@ -1526,40 +1526,58 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
 
 /* ---------------- accumulation ---------------- */
 
-__global__ __launch_bounds__(256) void k_accumulate(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
-                                                    uint32_t n_pixels)
+/* One wave per 64 pixels (pixel i = blockIdx.x * 64 + lane).  With one GI
+ * split and no per-path output, the wave stages 8 camera samples of its 64
+ * pixels at a time through LDS: 8 lanes read one pixel's 8 consecutive
+ * slots (one 128-byte line of each array), so every load instruction
+ * covers whole lines, instead of each lane reading 16 bytes of 64 lines
+ * per instruction.  Each lane then adds its pixel's samples in sample
+ * order, the order of the per-lane loop below. */
+constexpr uint32_t kAccBlock = 64;
+__global__ __launch_bounds__(kAccBlock) void k_accumulate(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
+                                                          uint32_t n_pixels)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_pixels) return;
+    __shared__ float4 s_l[8][kAccBlock + 1], s_g[8][kAccBlock + 1];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i0 = blockIdx.x * kAccBlock;
+    const uint32_t i = i0 + lane;
+    const bool valid = i < n_pixels;
     uint64_t per_pixel = (uint64_t)p.spp * p.n_gi;
     uint64_t first_slot = slot_base + (uint64_t)i * per_pixel;
     uint64_t pseq = first_slot / per_pixel;
-    uint32_t pixel = out.pixel_list ? out.pixel_list[pseq] : (uint32_t)pseq;
     double sum[3] = {0.0, 0.0, 0.0};
     float rcp_gi = 1.0f / (float)p.n_gi;
     uint32_t c = 0;
     bool any_hit = false;
     if (p.n_gi == 1 && !out.path_rgb) {
-        /* runs of 8 consecutive slots: each lane consumes whole cache lines
-         * of its pixel back to back (same sums, same order) */
-        const uint32_t s_first = (uint32_t)(first_slot - slot_base);
+        const uint32_t sub = lane & 7u, row = lane >> 3;
         for (; c + 8 <= p.spp; c += 8) {
-            float4 g[8], l[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                g[u] = pool.global[s_first + c + u];
-                l[u] = pool.local[s_first + c + u];
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t q = row + 8 * k;
+                if (i0 + q < n_pixels) {
+                    const uint32_t sl = (i0 + q) * p.spp + c + sub;
+                    s_g[sub][q] = pool.global[sl];
+                    s_l[sub][q] = pool.local[sl];
+                }
             }
+            __syncthreads();
+            if (valid) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                V3 v = add(ld4(l[u]), muls(ld4(g[u]), rcp_gi));
-                sum[0] += (double)v.x;
-                sum[1] += (double)v.y;
-                sum[2] += (double)v.z;
-                any_hit |= l[u].w != 0.0f;
+                for (int u = 0; u < 8; ++u) {
+                    const float4 l = s_l[u][lane], g = s_g[u][lane];
+                    V3 v = add(ld4(l), muls(ld4(g), rcp_gi));
+                    sum[0] += (double)v.x;
+                    sum[1] += (double)v.y;
+                    sum[2] += (double)v.z;
+                    any_hit |= l.w != 0.0f;
+                }
             }
+            __syncthreads();
         }
     }
+    if (!valid) return;
+    uint32_t pixel = out.pixel_list ? out.pixel_list[pseq] : (uint32_t)pseq;
     for (; c < p.spp; ++c) {
         uint32_t s0 = (uint32_t)(first_slot - slot_base) + c * p.n_gi;
         V3 g = ld4(pool.global[s0]);
@@ -1948,7 +1966,7 @@ void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, co
 void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
                        uint64_t slot_base, uint32_t n_pixels, const LaunchEvents* ev)
 {
-    launch_k(k_accumulate, (n_pixels + 255) / 256, 256, s, ev, p, pool, out, slot_base, n_pixels);
+    launch_k(k_accumulate, (n_pixels + kAccBlock - 1) / kAccBlock, kAccBlock, s, ev, p, pool, out, slot_base, n_pixels);
 }
 
 void launch_accumulate_requests(hipStream_t s, const FrameParams& p, const PathPool& pool, double* accum,

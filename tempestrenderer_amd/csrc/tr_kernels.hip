@@ -1173,50 +1173,76 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 
 
 /* ---------------- request mode: accumulation and outputs ---------------- */
 
+/* threads of an accumulation block: one wave */
+constexpr uint32_t kAccBlock = 64;
+
 /* Per-pixel sums of the current batch (pixels b.pix_base ..), in sample
  * order, each added once to the pixel's FP64 sum: batches end at pixel
  * boundaries, so the result does not depend on the batch size. */
-__global__ __launch_bounds__(256) void k_accumulate_req(FrameParams p, PathPool pool, double* accum,
-                                                        const uint32_t* cursor)
+__global__ __launch_bounds__(kAccBlock) void k_accumulate_req(FrameParams p, PathPool pool, double* accum,
+                                                              const uint32_t* cursor)
 {
+    __shared__ float4 s_l[8][kAccBlock + 1], s_g[8][kAccBlock + 1];
+    __shared__ uint32_t s_first_of[kAccBlock], s_spp[kAccBlock];
     const ReqBatch& b = p.req.batches[*cursor];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n_pix) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * kAccBlock + lane;
+    if (blockIdx.x * kAccBlock >= b.n_pix) return;
+    const bool valid = i < b.n_pix;
     const uint64_t gp = b.pix_base + i;
-    const uint32_t r = find_request(p.req.pix_off, p.req.n, gp);
-    const uint32_t spp = p.req.reqs[r].spp;
-    const uint64_t per_pixel = (uint64_t)spp * p.n_gi;
-    const uint32_t s_first = (uint32_t)(p.req.path_off[r] + (gp - p.req.pix_off[r]) * per_pixel - b.path_base);
+    uint32_t spp = 0, s_first = 0;
+    if (valid) {
+        const uint32_t r = find_request(p.req.pix_off, p.req.n, gp);
+        spp = p.req.reqs[r].spp;
+        const uint64_t per_pixel = (uint64_t)spp * p.n_gi;
+        s_first = (uint32_t)(p.req.path_off[r] + (gp - p.req.pix_off[r]) * per_pixel - b.path_base);
+    }
     const float rcp_gi = 1.0f / (float)p.n_gi;
     double sum[3] = {0.0, 0.0, 0.0};
-    uint32_t c = 0;
     if (p.n_gi == 1) {
-        /* runs of 8 consecutive slots, as in k_accumulate */
-        for (; c + 8 <= spp; c += 8) {
-            float4 g[8], l[8];
+        /* k_accumulate's staging through LDS, with each pixel's own first
+         * slot and sample count (requests differ in both) */
+        s_first_of[lane] = s_first;
+        s_spp[lane] = spp;
+        uint32_t max_spp = spp;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                g[u] = pool.global[s_first + c + u];
-                l[u] = pool.local[s_first + c + u];
-            }
+        for (int d = 32; d >= 1; d >>= 1) max_spp = max(max_spp, (uint32_t)__shfl_xor((int)max_spp, d));
+        __syncthreads();
+        const uint32_t sub = lane & 7u, row = lane >> 3;
+        for (uint32_t c = 0; c < max_spp; c += 8) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                V3 v = add(ld4(l[u]), muls(ld4(g[u]), rcp_gi));
-                sum[0] += (double)v.x;
-                sum[1] += (double)v.y;
-                sum[2] += (double)v.z;
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t q = row + 8 * k;
+                if (c + sub < s_spp[q]) {
+                    const uint32_t sl = s_first_of[q] + c + sub;
+                    s_g[sub][q] = pool.global[sl];
+                    s_l[sub][q] = pool.local[sl];
+                }
             }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                if (c + u < spp) {
+                    V3 v = add(ld4(s_l[u][lane]), muls(ld4(s_g[u][lane]), rcp_gi));
+                    sum[0] += (double)v.x;
+                    sum[1] += (double)v.y;
+                    sum[2] += (double)v.z;
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        for (uint32_t c = 0; c < spp; ++c) {
+            const uint32_t s0 = s_first + c * p.n_gi;
+            V3 g = ld4(pool.global[s0]);
+            for (uint32_t j = 1; j < p.n_gi; ++j) g = add(g, ld4(pool.global[s0 + j]));
+            V3 v = add(ld4(pool.local[s0]), muls(g, rcp_gi));
+            sum[0] += (double)v.x;
+            sum[1] += (double)v.y;
+            sum[2] += (double)v.z;
         }
     }
-    for (; c < spp; ++c) {
-        const uint32_t s0 = s_first + c * p.n_gi;
-        V3 g = ld4(pool.global[s0]);
-        for (uint32_t j = 1; j < p.n_gi; ++j) g = add(g, ld4(pool.global[s0 + j]));
-        V3 v = add(ld4(pool.local[s0]), muls(g, rcp_gi));
-        sum[0] += (double)v.x;
-        sum[1] += (double)v.y;
-        sum[2] += (double)v.z;
-    }
+    if (!valid) return;
     accum[gp * 3 + 0] += sum[0];
     accum[gp * 3 + 1] += sum[1];
     accum[gp * 3 + 2] += sum[2];
@@ -1533,7 +1559,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(FrameParams p, PathPool 
  * covers whole lines, instead of each lane reading 16 bytes of 64 lines
  * per instruction.  Each lane then adds its pixel's samples in sample
  * order, the order of the per-lane loop below. */
-constexpr uint32_t kAccBlock = 64;
 __global__ __launch_bounds__(kAccBlock) void k_accumulate(FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base,
                                                           uint32_t n_pixels)
 {
@@ -1972,7 +1997,8 @@ void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool
 void launch_accumulate_requests(hipStream_t s, const FrameParams& p, const PathPool& pool, double* accum,
                                 uint32_t max_pix, uint32_t* cursor, const LaunchEvents* ev)
 {
-    launch_k(k_accumulate_req, (max_pix + 255) / 256, 256, s, ev, p, pool, accum, (const uint32_t*)cursor);
+    launch_k(k_accumulate_req, (max_pix + kAccBlock - 1) / kAccBlock, kAccBlock, s, ev, p, pool, accum,
+             (const uint32_t*)cursor);
     k_req_next<<<1, 64, 0, s>>>(cursor);
 }
 

@@ -899,12 +899,20 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
         Rng rng;
         rng.k0 = p.key0; rng.k1 = p.key1; rng.pixel = pixel; rng.sample = cs; rng.split = split;
         const bool unbiased = p.estimator == TR_ESTIMATOR_UNBIASED;
+        /* Scenes without light sources and without the cache write a path's
+         * radiance of later vertices only where a vertex adds to it, and
+         * zero when the path ends with none; the ray direction's .w carries
+         * "written" from vertex to vertex (the camera vertex writes none). */
+        constexpr bool kLazyGlobal = !CACHE && !LIGHTS;
+        bool has_global = false;
         V3 org, dir;
         if (fused) {
             camera_ray(p, pixel, pixel, cs, p.width, p.height, p.spp, p.vpi, &org, &dir);
         } else {
             org = ld4(pool.ray_o[slot]);
-            dir = ld4(pool.ray_d[slot]);
+            const float4 d4 = pool.ray_d[slot];
+            dir = ld4(d4);
+            has_global = kLazyGlobal && d4.w != 0.0f;
         }
         float4 h4 = p.shade_sort ? sorted_hit : pool.hit[slot];
         const float4 t4 = vertex == 0 ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : pool.thr[slot];
@@ -919,7 +927,7 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
          * vertices that add to it (a miss, an emitter) and by the camera
          * vertex, which clears it; the cache mode reads it at every vertex. */
         V3 global = CACHE && vertex != 0 ? ld4(pool.global[slot]) : mk(0.0f, 0.0f, 0.0f);
-        bool write_global = CACHE || vertex == 0;
+        bool write_global = CACHE || (vertex == 0 && !kLazyGlobal);
         /* the camera vertex writes the slot's camera-vertex radiance, zero or not */
         bool write_local = vertex == 0;
         uint32_t pf = 0;
@@ -947,7 +955,7 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
                 local = env;
                 write_local = true;
             } else {
-                if (!CACHE) global = ld4(pool.global[slot]);
+                if (!CACHE && (!kLazyGlobal || has_global)) global = ld4(pool.global[slot]);
                 global = add(global, mulv(env, T));
                 write_global = true;
             }
@@ -1008,7 +1016,7 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
                         float ls_prev = vertex == 1 ? (float)p.n_light : 1.0f;
                         w = bal / (bal + ls_prev * lp);
                     }
-                    if (!CACHE) global = ld4(pool.global[slot]);
+                    if (!CACHE && (!kLazyGlobal || has_global)) global = ld4(pool.global[slot]);
                     global = add(global, mulv(muls(T, w), le));
                     write_global = true;
                 }
@@ -1127,7 +1135,7 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
                             T = mulv(T, divs(bsdf_eval(sd, tv), sd.pdf));
                         }
                         pool.ray_o[slot] = st4(pos, 0.0f);
-                        pool.ray_d[slot] = st4(sd.wi, 0.0f);
+                        pool.ray_d[slot] = st4(sd.wi, kLazyGlobal && (has_global || write_global) ? 1.0f : 0.0f);
                         pool.thr[slot] = st4(T, nb);
                         extend = true;
                         if (CACHE && !(pf & PF_RECORDED) && cache_roughness(sd.mat) >= p.rough_alpha)
@@ -1137,6 +1145,8 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
             }
         }
         if (want_hash) pool.hash[slot] = hash;
+        /* a path that ends with nothing written beyond its camera vertex: zero */
+        if (kLazyGlobal && !extend && !has_global) write_global = true;
         if (write_global) pool.global[slot] = st4(global, 0.0f);
         /* .w: the camera ray hit geometry (TransparentBackground) */
         if (write_local) pool.local[slot] = st4(local, geom != TR_INVALID_GEOMETRY ? 1.0f : 0.0f);

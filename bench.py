@@ -230,7 +230,7 @@ def volume_bench(cpu_rows=128, threads=None, cpu=True):
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 5),
                         "traffic": load_traffic(os.path.join(ROOT, "profiles", "volume_kernels_pmc.txt")),
-                        "kernel": "k_trace<false, true> (closest hit + delta tracking)",
+                        "kernel": "k_trace_vol<false> (closest hit + delta tracking, ray refill)",
                         "alg_bytes_per_frame": alg, "launches_per_frame": tim["trace_launches"],
                         "avg_launch_us": round(tim["trace_ms"] * 1e3 / max(1, tim["trace_launches"]), 2)}}
     if cpu:

@@ -1380,11 +1380,17 @@ tr_status tr_scene_add_grid_volume(tr_scene* s, const tr_grid_volume* v, const t
     r.rcp_rho_max = 1.0f / rho_max;
     r.rcp_majorant = 1.0f / (rho_max * smax);
     r.rcp_smax = 1.0f / smax;
+    /* tiles of 4 x 2 x 2 samples from a 16-word boundary (voxel_tile_index) */
+    s->voxels.resize((s->voxels.size() + 15u) & ~(size_t)15u, 0u);
     r.voxel_base = s->voxels.size();
-    s->voxels.reserve(s->voxels.size() + n);
-    for (uint64_t i = 0; i < n; ++i)
-        s->voxels.push_back((uint64_t)v->direction[i * 3] | ((uint64_t)v->direction[i * 3 + 1] << 16) |
-                            ((uint64_t)v->direction[i * 3 + 2] << 32) | ((uint64_t)v->density[i] << 48));
+    s->voxels.resize(r.voxel_base + voxel_tile_words(r.dims), 0u);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t x = (uint32_t)(i % r.dims[0]), y = (uint32_t)((i / r.dims[0]) % r.dims[1]);
+        const uint32_t z = (uint32_t)(i / ((uint64_t)r.dims[0] * r.dims[1]));
+        s->voxels[r.voxel_base + voxel_tile_index(x, y, z, r.dims)] =
+            (uint64_t)v->direction[i * 3] | ((uint64_t)v->direction[i * 3 + 1] << 16) |
+            ((uint64_t)v->direction[i * 3 + 2] << 32) | ((uint64_t)v->density[i] << 48);
+    }
     GeomRec g;
     memset(&g, 0, sizeof(g));
     g.type = GEOM_VOLUME;
@@ -1491,12 +1497,17 @@ tr_status tr_scene_add_hierarchical_volume(tr_scene* s, const tr_volume_root* ro
         r.rcp_majorant = 1.0f / kDensityScale;  /* RcpMaxExtinction (:1463) */
         r.rcp_smax = 1.0f;
         r.table_base = s->dir_table;
+        /* tiles of 4 x 2 x 2 samples from a 16-word boundary (voxel_tile_index) */
+        s->voxels.resize((s->voxels.size() + 15u) & ~(size_t)15u, 0u);
         r.voxel_base = s->voxels.size();
         const uint64_t n = (uint64_t)br.dims[0] * br.dims[1] * br.dims[2];
-        s->voxels.reserve(s->voxels.size() + n);
+        s->voxels.resize(r.voxel_base + voxel_tile_words(r.dims), 0u);
         for (uint64_t i = 0; i < n; ++i) {
             const uint8_t* q = br.data + n + 2 * i;
-            s->voxels.push_back((uint64_t)q[0] | ((uint64_t)q[1] << 8) | (u8_half_bits(br.data[i]) << 48));
+            const uint32_t x = (uint32_t)(i % r.dims[0]), y = (uint32_t)((i / r.dims[0]) % r.dims[1]);
+            const uint32_t z = (uint32_t)(i / ((uint64_t)r.dims[0] * r.dims[1]));
+            s->voxels[r.voxel_base + voxel_tile_index(x, y, z, r.dims)] =
+                (uint64_t)q[0] | ((uint64_t)q[1] << 8) | (u8_half_bits(br.data[i]) << 48);
         }
         GeomRec g;
         memset(&g, 0, sizeof(g));

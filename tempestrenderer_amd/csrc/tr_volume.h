@@ -38,6 +38,25 @@ TR_HD float h2f(uint32_t h)
 #endif
 }
 
+/* A volume's voxel words lie in tiles of 4 x 2 x 2 samples (16 words, one
+ * 128-byte cache line; x fastest inside a tile), the tiles in x-, y-, then
+ * z-order, from a voxel_base that is a multiple of 16 words.  A trilinear
+ * lookup's eight samples then span 1.25 x 1.5 x 1.5 = 2.8 lines on
+ * average, against 4.25 with the grid in plain x-fastest order, where the
+ * y and z neighbours always lie in other lines. */
+TR_HD uint64_t voxel_tile_index(uint32_t x, uint32_t y, uint32_t z, const uint32_t dims[3])
+{
+    const uint64_t ntx = (dims[0] + 3u) >> 2, nty = (dims[1] + 1u) >> 1;
+    return ((((uint64_t)(z >> 1) * nty + (y >> 1)) * ntx + (x >> 2)) << 4) + ((z & 1u) << 3) + ((y & 1u) << 2) +
+           (x & 3u);
+}
+
+/* Words of a volume's tiles (the grid padded to whole tiles). */
+TR_HD uint64_t voxel_tile_words(const uint32_t dims[3])
+{
+    return (uint64_t)((dims[0] + 3u) >> 2) * ((dims[1] + 1u) >> 1) * ((dims[2] + 1u) >> 1) * 16u;
+}
+
 /* The eight grid samples around a point and its position in the cell. */
 struct VoxelCell {
     uint64_t s[8];   /* corner k: x from bit 0, y from bit 1, z from bit 2 */
@@ -60,13 +79,15 @@ TR_HD VoxelCell fetch_cell(const uint64_t* voxels, const VolumeRec& V, V3 pos)
         c.f[a] = v - fl;
     }
     const uint64_t* base = voxels + V.voxel_base;
-    uint64_t sx = V.dims[0], sxy = (uint64_t)V.dims[0] * V.dims[1];
+    /* voxel_tile_index, split into its x, y and z terms */
+    const uint32_t ntx = (V.dims[0] + 3u) >> 2, nty = (V.dims[1] + 1u) >> 1;
+    const uint64_t ox[2] = {((uint64_t)(i0[0] >> 2) << 4) + (i0[0] & 3u), ((uint64_t)(i1[0] >> 2) << 4) + (i1[0] & 3u)};
+    const uint64_t oy[2] = {((uint64_t)(i0[1] >> 1) * ntx << 4) + ((i0[1] & 1u) << 2),
+                            ((uint64_t)(i1[1] >> 1) * ntx << 4) + ((i1[1] & 1u) << 2)};
+    const uint64_t oz[2] = {((uint64_t)(i0[2] >> 1) * nty * ntx << 4) + ((i0[2] & 1u) << 3),
+                            ((uint64_t)(i1[2] >> 1) * nty * ntx << 4) + ((i1[2] & 1u) << 3)};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        uint64_t idx = (uint64_t)((k & 4) ? i1[2] : i0[2]) * sxy + (uint64_t)((k & 2) ? i1[1] : i0[1]) * sx +
-                       (uint64_t)((k & 1) ? i1[0] : i0[0]);
-        c.s[k] = base[idx];
-    }
+    for (int k = 0; k < 8; ++k) c.s[k] = base[oz[(k >> 2) & 1] + oy[(k >> 1) & 1] + ox[k & 1]];
     return c;
 }
 

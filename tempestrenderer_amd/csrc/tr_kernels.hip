@@ -1169,9 +1169,13 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
  * LIGHTS false is the instance for scenes without light sources (no
  * next-event estimation, no light terms in the balance heuristic): without
  * that code the kernel holds under 80 VGPRs, 6 waves per SIMD, which the
- * latency-bound shading of whole-frame launches needs. */
+ * latency-bound shading of whole-frame launches needs.  The instance with
+ * light sources (no cache, no textures) is held to 5 waves per SIMD (96
+ * VGPRs, a 28-byte spill; 2.5 % faster than 4 waves on the
+ * gonioreflectometer requests and the box frame); the cache and texture
+ * instances spill over 100 bytes at 5 waves and keep their registers. */
 template <bool VOL, bool CACHE, bool TEX, bool LIGHTS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? 1 : 6))) void k_shade(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? (CACHE || TEX ? 1 : 5) : 6))) void k_shade(
     FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base, uint32_t vertex, int cur, int nxt,
     CacheRecords rec)
 {

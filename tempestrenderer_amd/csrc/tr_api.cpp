@@ -546,7 +546,8 @@ tr_status enqueue_frame(tr_scene* s, uint64_t n_pixels, const BatchOut& out)
             }
             if (p.sc.n_lights > 0) {
                 if ((st2 = stage(EV_SHADOW, [&](const LaunchEvents* ev) {
-                         launch_shadow(s->stream, p, pool, out, slot_base, v, grid, counts, ev);
+                         launch_shadow(s->stream, p, pool, out, slot_base, v, grid, counts,
+                                       grid > 16u * (uint32_t)std::max(1, s->cu_count), ev);
                      })) != TR_OK)
                     return st2;
             }
@@ -2082,7 +2083,7 @@ tr_status tr_scene_render_requests(tr_scene* s, size_t count, const tr_request* 
             launch_trace(stream, p, pool, out, 0, v, cur, nxt, grid, grid, nullptr);
             launch_shade(stream, p, pool, out, 0, v, cur, nxt, slots, rec);
             launch_compact(stream, pool, slots, cur, nxt, false);
-            if (p.sc.n_lights > 0) launch_shadow(stream, p, pool, out, 0, v, grid, nullptr);
+            if (p.sc.n_lights > 0) launch_shadow(stream, p, pool, out, 0, v, grid, nullptr, waves > 16);
             std::swap(cur, nxt);
         }
         launch_accumulate_requests(stream, p, pool, d_accum.ptr, max_pix, d_cursor.ptr);

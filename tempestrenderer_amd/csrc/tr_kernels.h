@@ -181,8 +181,9 @@ void launch_cache_infer(hipStream_t s, const MlpWeights& w, const PathPool& pool
 /* Cache mode: targets of the training records [r0, r1) of a batch. */
 void launch_cache_targets(hipStream_t s, const PathPool& pool, const CacheRecords& rec, uint32_t r0, uint32_t r1,
                           const LaunchEvents* ev = nullptr);
+/* wide: the persistent grid holds more than 16 waves per CU */
 void launch_shadow(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
-                   uint64_t slot_base, uint32_t vertex, uint32_t grid, uint64_t* counts,
+                   uint64_t slot_base, uint32_t vertex, uint32_t grid, uint64_t* counts, bool wide,
                    const LaunchEvents* ev = nullptr);
 void launch_accumulate(hipStream_t s, const FrameParams& p, const PathPool& pool, const BatchOut& out,
                        uint64_t slot_base, uint32_t n_pixels, const LaunchEvents* ev = nullptr);

@@ -1172,10 +1172,12 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
  * latency-bound shading of whole-frame launches needs.  The instance with
  * light sources (no cache, no textures) is held to 5 waves per SIMD (96
  * VGPRs, a 28-byte spill; 2.5 % faster than 4 waves on the
- * gonioreflectometer requests and the box frame); the cache and texture
- * instances spill over 100 bytes at 5 waves and keep their registers. */
+ * gonioreflectometer requests and the box frame); the cache instance to 4
+ * waves (128 VGPRs, no spill; cached box frames 3 % faster than at 3
+ * waves; it spills 104 bytes at 5); the texture instances keep their
+ * registers (192 bytes of spills at 5 waves). */
 template <bool VOL, bool CACHE, bool TEX, bool LIGHTS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? (CACHE || TEX ? 1 : 5) : 6))) void k_shade(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIGHTS ? (TEX ? 1 : (CACHE ? 4 : 5)) : 6))) void k_shade(
     FrameParams p, PathPool pool, BatchOut out, uint64_t slot_base, uint32_t vertex, int cur, int nxt,
     CacheRecords rec)
 {

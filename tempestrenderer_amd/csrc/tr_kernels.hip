@@ -754,11 +754,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_vol(FrameParams p, PathPo
             }
         }
         if (active && tracking) {
+            /* an accepted collision hands its cell's fibre orientation to the
+             * shading kernel in the hit's u, v and primitive words (a volume
+             * box is primitive 0 of its geometry); NaN when the step budget
+             * ended the walk without an accepted collision */
+            V3 orient = mk(__uint_as_float(0x7fc00000u), 0.0f, 0.0f);
             const int res = delta_track_pair_any(sc.voxels, sc.volumes[vol], org, dir, &tc, t_exit, rng, vertex,
-                                                 &step, COUNT ? &ns : nullptr);
+                                                 &step, COUNT ? &ns : nullptr, &orient);
             if (res > 0) {
                 h.t = tc;
-                h.u = h.v = 0.0f;
+                h.u = orient.x;
+                h.v = orient.y;
+                h.prim = __float_as_uint(orient.z);
                 done = true;
             } else if (res < 0) {
                 tracking = false;
@@ -962,25 +969,33 @@ __device__ __forceinline__ void shade_segment(const FrameParams& p, const PathPo
         } else if (sc.geoms[geom].material == kNoMaterial) {
             /* a blocker: the path ends with nothing (ray-tracing.cc:2072-2077,
              * 2236-2240); its sample() frame is zero (:1239-1244) */
-            if (want_hash) hash = fnv_hit(hash, geom, pool.hit_prim[slot]);
+            /* a volume hit's primitive word carries its fibre orientation;
+             * a volume box is primitive 0 */
+            if (want_hash)
+                hash = fnv_hit(hash, geom, VOL && sc.geoms[geom].type == GEOM_VOLUME ? 0u : pool.hit_prim[slot]);
             if (vertex == 0 && p.render_mode) {
                 const V3 z = mk(0.0f, 0.0f, 0.0f);
                 local = debug_color(p.render_mode, z, z, z, dir);
             }
         } else {
             uint32_t prim = pool.hit_prim[slot];
-            hash = fnv_hit(hash, geom, prim);
             Sample sd;
             sd.dir_density = 0.0f;
             V3 pos;
             const GeomRec& G = sc.geoms[geom];
             bool volume = VOL && G.type == GEOM_VOLUME;
+            hash = fnv_hit(hash, geom, volume ? 0u : prim);
             if (volume) {
                 /* RTVolumeSet::sampleScattering's frame and DirectionalDensity
-                 * at the accepted collision (ray-tracing.cc:327-337) */
+                 * at the accepted collision (ray-tracing.cc:327-337): the
+                 * closest-hit kernel's acceptance test computed the cell's
+                 * fibre orientation and passed it in (u, v, primitive); NaN
+                 * when the walk's step budget ended it */
                 const VolumeRec& V = sc.volumes[G.tri_count];
                 pos = ray_point(org, dir, h4.x);
-                M3 b = volume_frame_any(fetch_cell(sc.voxels, V, pos), sc.voxels, V);
+                M3 b;
+                if (h4.y == h4.y) b = make_basis_tangent(mk(h4.y, h4.z, __uint_as_float(prim)));
+                else b = volume_frame_any(fetch_cell(sc.voxels, V, pos), sc.voxels, V);
                 sd.mat = &sc.mats[G.material];
                 sd.tangent = b.c0;
                 sd.binormal = b.c1;

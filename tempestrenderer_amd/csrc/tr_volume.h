@@ -186,18 +186,25 @@ TR_HD M3 volume_frame_any(const VoxelCell& c, const uint64_t* voxels, const Volu
  * 316-337 and the loops' TotalDensity *= RcpMaxExtinction): u8 density / 255
  * against the threshold, then (DensityScale / 255) * density * sigma *
  * RcpMaxExtinction. */
+/* orient (if given) receives the cell's fibre orientation when the
+ * collision is accepted (return 1): the frame of volume_frame is
+ * make_basis_tangent of it. */
 template <bool QUANT>
-TR_HD int accept_cell(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V, V3 dir, float threshold)
+TR_HD int accept_cell(const VoxelCell& c, const uint64_t* voxels, const VolumeRec& V, V3 dir, float threshold,
+                      V3* orient = nullptr)
 {
     float rho = cell_density(c);
     if (rho == 0.0f) return 2;
     const bool quant = QUANT;
     float dens = quant ? rho / 255.0f : rho * V.rcp_rho_max;
     if (dens < threshold) return 0;
-    M3 basis = volume_frame<QUANT>(c, voxels, V);
+    const V3 o = cell_orientation<QUANT>(c, voxels, V);
+    M3 basis = make_basis_tangent(o);
     float sigma = length(mulv(transform_rotation_inverse(basis, dir), ld3(V.stddev)));
-    if (quant) return (kDensityScale / 255.0f) * rho * sigma * V.rcp_majorant >= threshold ? 1 : 0;
-    return dens * sigma * V.rcp_smax >= threshold ? 1 : 0;
+    const bool acc = quant ? (kDensityScale / 255.0f) * rho * sigma * V.rcp_majorant >= threshold
+                           : dens * sigma * V.rcp_smax >= threshold;
+    if (acc && orient) *orient = o;
+    return acc ? 1 : 0;
 }
 template <bool QUANT>
 TR_HD int volume_accept(const uint64_t* voxels, const VolumeRec& V, V3 pos, V3 dir, float threshold)
@@ -234,7 +241,7 @@ enum { STREAM_VOLUME = 4, STREAM_SHADOW_VOLUME = 5 };
  * this path vertex across volumes. */
 template <bool QUANT>
 TR_HD int delta_track_pair(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float* tc, float t1,
-                           const Rng& rng, uint32_t vertex, uint32_t* step, uint32_t* nsteps)
+                           const Rng& rng, uint32_t vertex, uint32_t* step, uint32_t* nsteps, V3* orient = nullptr)
 {
     const uint32_t k = *step;
     const U4 r0 = rng.draw(vertex, STREAM_VOLUME, k >> 1);
@@ -251,21 +258,21 @@ TR_HD int delta_track_pair(const uint64_t* voxels, const VolumeRec& V, V3 org, V
     const float tc2 = tc1 - tr_logf(u01(xa2)) * V.rcp_majorant;
     const VoxelCell c1 = fetch_cell(voxels, V, ray_point(org, dir, tc1));
     const VoxelCell c2 = fetch_cell(voxels, V, ray_point(org, dir, tc2));
-    if (accept_cell<QUANT>(c1, voxels, V, dir, u01(xb1)) == 1) return 1;
+    if (accept_cell<QUANT>(c1, voxels, V, dir, u01(xb1), orient) == 1) return 1;
     *step = k + 2u;
     if (nsteps) ++*nsteps;
     *tc = tc2;
     if (tc2 >= t1) return -1;
     if (k + 2u >= kMaxVolumeSteps) return 1;
-    if (accept_cell<QUANT>(c2, voxels, V, dir, u01(xb2)) == 1) return 1;
+    if (accept_cell<QUANT>(c2, voxels, V, dir, u01(xb2), orient) == 1) return 1;
     return 0;
 }
 TR_HD int delta_track_pair_any(const uint64_t* voxels, const VolumeRec& V, V3 org, V3 dir, float* tc, float t1,
-                               const Rng& rng, uint32_t vertex, uint32_t* step, uint32_t* nsteps)
+                               const Rng& rng, uint32_t vertex, uint32_t* step, uint32_t* nsteps, V3* orient = nullptr)
 {
     if (V.format == kVoxelQuantized)
-        return delta_track_pair<true>(voxels, V, org, dir, tc, t1, rng, vertex, step, nsteps);
-    return delta_track_pair<false>(voxels, V, org, dir, tc, t1, rng, vertex, step, nsteps);
+        return delta_track_pair<true>(voxels, V, org, dir, tc, t1, rng, vertex, step, nsteps, orient);
+    return delta_track_pair<false>(voxels, V, org, dir, tc, t1, rng, vertex, step, nsteps, orient);
 }
 
 /* The whole walk: true and the collision distance in *t when a collision

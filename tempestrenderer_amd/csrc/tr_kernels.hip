@@ -631,9 +631,12 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kRe
     const SceneView& sc = p.sc;
     const uint32_t n = pool.counters[cur];
     const uint32_t* queue = pool.queue[cur];
-    DevStack st;
-    st.lds = &stk[0][threadIdx.x];
-    st.spill = pool.spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * kStackSpill;
+    /* entries addressed from the lane and a wave-uniform spill base (fewer
+     * registers than per-lane pointers) */
+    LaneStack st;
+    st.lds = stk;
+    st.spill = pool.spill + (size_t)blockIdx.x * kTraceBlock * kStackSpill;
+    st.sp = 0;
     for (uint32_t base = blockIdx.x * kTraceBlock; base < n; base += gridDim.x * kTraceBlock) {
         const uint32_t qi = base + threadIdx.x;
         if (qi < n) {
